@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Headline benchmark: BFS GTEPS on a symmetrized RMAT scale-24 graph (edge
+factor 16, a/b/c = 0.57/0.19/0.19) on one MI355X, through the C ABI.
+
+One step = one direction-optimizing BFS from the next of 64 sources picked
+GAPBS-style (degree-0 vertices skipped), graph resident in HBM, depth array
+preallocated. `value` = undirected edges of the reached components summed
+over the K timed BFS runs / wall time of those runs (Graph500 convention).
+
+Also reported on the same JSON line:
+  roofline      the dominant traversal kernel's algorithmic bytes per launch
+                over its average duration (hipEvents around that launch),
+                against 8 TB/s HBM;
+  cpu_baseline  the oracle's serial BFS on the host, same graph and sources
+                (a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SEED = 27491095
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def schedule(gi, threshold: float, push_lb: int, pull_lb: int):
+    s1 = gi.Schedule(direction=gi.PUSH, load_balancing=push_lb, frontier_creation=gi.FC_SPARSE,
+                     cta_size=256)
+    s2 = gi.Schedule(direction=gi.PULL, load_balancing=pull_lb, pull_frontier_rep=gi.PULL_BITMAP,
+                     frontier_creation=gi.FC_BITMAP, cta_size=256)
+    return gi.HybridSchedule(s1, s2, threshold, gi.SELECT_HEURISTIC)
+
+
+def roofline(gi, g, sched, sources, depth):
+    """Per-kernel (direction, load balancing) totals over profiled runs."""
+    kern = {}
+    for src in sources:
+        for r in gi.bfs_profile(g, int(src), sched, depth):
+            key = ("pull" if r["direction"] == gi.PULL else "push", gi.LB_NAMES[r["load_balancing"]])
+            k = kern.setdefault(key, {"ms": 0.0, "bytes": 0.0, "launches": 0})
+            k["ms"] += r["kernel_ms"]
+            k["bytes"] += r["alg_bytes"]
+            k["launches"] += 1
+    dom_key = max(kern, key=lambda k: kern[k]["ms"])
+    d = kern[dom_key]
+    avg_ms = d["ms"] / d["launches"]
+    bytes_per_launch = d["bytes"] / d["launches"]
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    per_kernel = {f"{a}_{b}": {"launches": v["launches"], "avg_us": 1e3 * v["ms"] / v["launches"],
+                              "GBps": v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0}
+                  for (a, b), v in kern.items()}
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": f"{dom_key[0]}_{dom_key[1]}", "avg_launch_us": round(1e3 * avg_ms, 2),
+            "alg_bytes_per_launch": int(bytes_per_launch), "per_kernel": per_kernel}
+
+
+def cpu_baseline(gi, g, sources, edges_of, budget_s: float):
+    import oracle
+    off, nb, _ = g.read_csr()
+    host = oracle.CSR(off, nb)
+    done, edges, t = 0, 0, 0.0
+    for src in sources:
+        t0 = time.perf_counter()
+        oracle.bfs_depth(host, int(src))
+        t += time.perf_counter() - t0
+        edges += edges_of[int(src)]
+        done += 1
+        if t >= budget_s:
+            break
+    return {"value": round(edges / 2 / t / 1e9, 4), "unit": "GTEPS", "cores": 1, "kind": "port",
+            "sample": f"oracle serial queue BFS from {done} of the same sources on the same graph "
+                      f"({t:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=32)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--threshold", type=float, default=0.02)
+    ap.add_argument("--push-lb", type=int, default=2)
+    ap.add_argument("--pull-lb", type=int, default=0)
+    ap.add_argument("--profile-sources", type=int, default=4)
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU "
+              f"sharding is a design, not a build)", file=sys.stderr)
+        sys.exit(2)
+
+    import easy_graphit_amd as gi
+    if gi.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    t0 = time.perf_counter()
+    g = gi.Graph.rmat(args.scale, args.edge_factor, SEED, True)
+    V, E = g.num_vertices, g.num_edges
+    log(f"graph RMAT-{args.scale} sym: V={V} E={E} built in {time.perf_counter() - t0:.2f}s")
+    sources = g.pick_sources(SEED, 64)
+    sched = schedule(gi, args.threshold, args.push_lb, args.pull_lb)
+    depth = gi.DeviceArray(V, np.int32)
+
+    # untimed: reached edges per source
+    edges_of = {}
+    for src in sources:
+        gi.bfs_device(g, int(src), sched, depth)
+        edges_of[int(src)] = g.reached_edges(depth)
+    for k in range(args.warmup):
+        gi.bfs_device(g, int(sources[k % 64]), sched, depth)
+    gi.synchronize()
+
+    t0 = time.perf_counter()
+    total_edges = 0
+    for k in range(args.steps):
+        src = int(sources[k % 64])
+        gi.bfs_device(g, src, sched, depth)
+        total_edges += edges_of[src]
+    gi.synchronize()
+    dt = time.perf_counter() - t0
+    gteps = total_edges / 2 / dt / 1e9
+    log(f"{args.steps} BFS in {dt * 1e3:.1f} ms -> {gteps:.2f} GTEPS")
+
+    roof = roofline(gi, g, sched, sources[: args.profile_sources], depth)
+    log(f"roofline: {json.dumps(roof)}")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(gi, g, sources, edges_of, args.cpu_budget)
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "GTEPS per app on RMAT-24 (1 GPU); learned vs heuristic schedule speedup",
+        "value": round(gteps, 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (seeded RMAT generated on the device)",
+        "config": {"workload": f"direction-optimizing BFS, symmetrized RMAT scale {args.scale} "
+                               f"edge factor {args.edge_factor}, 64 GAPBS-style sources",
+                   "vertices": V, "edge_entries": E, "schedule": {
+                       "s1": repr(sched.s1), "s2": repr(sched.s2), "threshold": args.threshold,
+                       "selector": "heuristic"}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

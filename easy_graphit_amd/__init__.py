@@ -1,0 +1,594 @@
+"""MI355X-native GraphIt GPU runtime — Python binding of the C ABI.
+
+``libgraphit_amd.so`` (built from ``csrc/`` by ``build.py``) exports the
+``gi_*`` entry points declared in ``include/graphit_amd.h``. This module binds
+them with ctypes and mirrors the reference's runtime vocabulary
+(``GraphT``/``load_graph``, ``VertexFrontier``, ``SimpleGPUSchedule``,
+``HybridGPUSchedule``, ``edgeset_apply``, ``vertexset_apply``,
+``VertexFrontierList``, ``GPUPriorityQueue``) plus the fused app drivers.
+
+There is no CPU fallback: every call goes to the HIP library, and importing a
+compute entry point without the built library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgraphit_amd.so")
+HEADER = os.path.join(os.path.dirname(_HERE), "include", "graphit_amd.h")
+
+# --------------------------------------------------------------- constants
+GI_OK = 0
+ERRORS = {-1: "INVALID", -2: "HIP", -3: "OOM", -4: "UNSUPPORTED", -5: "RANGE", -6: "IO",
+          -7: "UNDERFLOW"}
+
+PUSH, PULL = 0, 1
+PULL_BITMAP, PULL_BOOLMAP = 0, 1
+FC_SPARSE, FC_BITMAP, FC_BOOLMAP = 0, 1, 2
+FMT_SPARSE, FMT_BITMAP, FMT_BOOLMAP = 0, 1, 2
+LB_VERTEX_BASED, LB_TWC, LB_TWCE, LB_WM, LB_CM, LB_STRICT, LB_EDGE_ONLY = range(7)
+LB_NAMES = ["VERTEX_BASED", "TWC", "TWCE", "WM", "CM", "STRICT", "EDGE_ONLY"]
+BLOCKED, UNBLOCKED = 0, 1
+SELECT_HEURISTIC, SELECT_LEARNED, SELECT_S1, SELECT_S2 = 0, 1, 2, 3
+(OP_BFS_PARENT, OP_BFS_DEPTH, OP_SSSP_RELAX, OP_CC_HOOK, OP_PR_PULL, OP_BC_FORWARD,
+ OP_BC_BACKWARD) = range(7)
+(VOP_FILL_I32, VOP_IOTA_I32, VOP_CC_PJUMP, VOP_PR_CONTRIB, VOP_PR_UPDATE, VOP_PR_RESET,
+ VOP_BC_RESET, VOP_BC_MARK, VOP_BC_BACKWARD_V, VOP_BC_FINAL) = range(10)
+
+PR_DAMP = float(np.float32(0.85))  # apps/pr.cpp:23 (float constant)
+
+
+class GIError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"gi error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+# --------------------------------------------------------------- structs
+class Schedule(C.Structure):
+    """SimpleGPUSchedule (include/graphit/schedule.h:26-118)."""
+    _fields_ = [(n, C.c_int32) for n in (
+        "direction", "pull_frontier_rep", "frontier_creation", "deduplication",
+        "load_balancing", "edge_blocking", "block_size", "kernel_fusion", "max_cta",
+        "cta_size")]
+
+    def __init__(self, **kw):
+        super().__init__()
+        lib().gi_schedule_init(C.byref(self))
+        for k, v in kw.items():
+            setattr(self, k, v)
+
+    def __repr__(self):
+        d = "PULL" if self.direction == PULL else "PUSH"
+        return f"Schedule({d},{LB_NAMES[self.load_balancing]},fc={self.frontier_creation},cta={self.cta_size})"
+
+
+class HybridSchedule(C.Structure):
+    """HybridGPUSchedule (include/graphit/schedule.h:122-139) + selector."""
+    _fields_ = [("s1", Schedule), ("s2", Schedule), ("threshold", C.c_float),
+                ("selector", C.c_int32)]
+
+    def __init__(self, s1: Schedule, s2: Schedule, threshold: float = 0.0,
+                 selector: int = SELECT_HEURISTIC):
+        super().__init__()
+        self.s1 = s1
+        self.s2 = s2
+        self.threshold = threshold
+        self.selector = selector
+
+
+class OpArgs(C.Structure):
+    _fields_ = [("data0", C.c_void_p), ("data1", C.c_void_p), ("data2", C.c_void_p),
+                ("i0", C.c_int32), ("i1", C.c_int32), ("f0", C.c_float), ("f1", C.c_float)]
+
+
+class LevelRecord(C.Structure):
+    _fields_ = [("level", C.c_int32), ("direction", C.c_int32), ("load_balancing", C.c_int32),
+                ("frontier_size", C.c_int32), ("next_size", C.c_int32), ("reserved", C.c_int32),
+                ("kernel_ms", C.c_double), ("alg_bytes", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class RunStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("push_iterations", C.c_int32),
+                ("pull_iterations", C.c_int32), ("reached", C.c_int32),
+                ("edges_traversed", C.c_int64), ("ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# --------------------------------------------------------------- loading
+_LIB = None
+
+
+def header_symbols() -> list[str]:
+    """Every function the C ABI header declares."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gi_[a-z0-9_]+)\s*\(", text)))
+
+
+def lib():
+    """Load libgraphit_amd.so. Raises when it has not been built: there is no
+    fallback path."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing; run python -m easy_graphit_amd.build")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, u64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64
+    pp = C.POINTER(C.c_void_p)
+    sig = {
+        "gi_last_error": (C.c_char_p, []),
+        "gi_version": (i32, []),
+        "gi_device_count": (i32, []),
+        "gi_malloc": (C.c_int, [pp, u64]),
+        "gi_free": (C.c_int, [vp]),
+        "gi_memcpy_h2d": (C.c_int, [vp, vp, u64, vp]),
+        "gi_memcpy_d2h": (C.c_int, [vp, vp, u64, vp]),
+        "gi_memcpy_d2d": (C.c_int, [vp, vp, u64, vp]),
+        "gi_fill_i32": (C.c_int, [vp, i64, i32, vp]),
+        "gi_fill_f32": (C.c_int, [vp, i64, C.c_float, vp]),
+        "gi_fill_u8": (C.c_int, [vp, i64, C.c_uint8, vp]),
+        "gi_stream_create": (C.c_int, [pp]),
+        "gi_stream_destroy": (C.c_int, [vp]),
+        "gi_stream_synchronize": (C.c_int, [vp]),
+        "gi_timer_start": (C.c_int, [vp]),
+        "gi_timer_stop": (C.c_int, [vp, C.POINTER(C.c_float)]),
+        "gi_schedule_init": (None, [C.POINTER(Schedule)]),
+        "gi_graph_from_csr": (C.c_int, [i32, i64, vp, vp, vp, i32, pp]),
+        "gi_graph_from_edges": (C.c_int, [i32, i64, vp, vp, i32, pp]),
+        "gi_graph_rmat": (C.c_int, [i32, i32, u64, i32, pp]),
+        "gi_graph_uniform": (C.c_int, [i32, i64, u64, i32, pp]),
+        "gi_graph_grid": (C.c_int, [i32, i64, u64, pp]),
+        "gi_graph_set_weights": (C.c_int, [vp, u64, i32, i32]),
+        "gi_graph_num_vertices": (C.c_int, [vp, C.POINTER(i32)]),
+        "gi_graph_num_edges": (C.c_int, [vp, C.POINTER(i64)]),
+        "gi_graph_read_csr": (C.c_int, [vp, vp, vp, vp]),
+        "gi_graph_device_csr": (C.c_int, [vp, pp, pp, pp]),
+        "gi_graph_transpose": (C.c_int, [vp, pp]),
+        "gi_graph_blocked": (C.c_int, [vp, i32, pp]),
+        "gi_graph_destroy": (C.c_int, [vp]),
+        "gi_frontier_create": (C.c_int, [i32, i32, pp]),
+        "gi_frontier_destroy": (C.c_int, [vp]),
+        "gi_frontier_add_vertex": (C.c_int, [vp, i32, vp]),
+        "gi_frontier_clear": (C.c_int, [vp, vp]),
+        "gi_frontier_size": (C.c_int, [vp, C.POINTER(i32), vp]),
+        "gi_frontier_convert": (C.c_int, [vp, i32, vp]),
+        "gi_frontier_format": (C.c_int, [vp, C.POINTER(i32)]),
+        "gi_frontier_read": (C.c_int, [vp, vp, i32, C.POINTER(i32), vp]),
+        "gi_frontier_swap": (C.c_int, [vp, i32, vp]),
+        "gi_frontier_dedup": (C.c_int, [vp, vp]),
+        "gi_edgeset_apply": (C.c_int, [vp, C.POINTER(Schedule), vp, vp, i32, C.POINTER(OpArgs), vp]),
+        "gi_edgeset_apply_hybrid": (C.c_int, [vp, C.POINTER(HybridSchedule), vp, vp, i32,
+                                              C.POINTER(OpArgs), vp]),
+        "gi_vertexset_apply": (C.c_int, [vp, vp, i32, C.POINTER(OpArgs), vp]),
+        "gi_frontier_list_create": (C.c_int, [i32, pp]),
+        "gi_frontier_list_insert": (C.c_int, [vp, vp, vp]),
+        "gi_frontier_list_retrieve": (C.c_int, [vp, vp, vp]),
+        "gi_frontier_list_levels": (C.c_int, [vp, C.POINTER(i32)]),
+        "gi_frontier_list_destroy": (C.c_int, [vp]),
+        "gi_bfs": (C.c_int, [vp, i32, C.POINTER(HybridSchedule), vp, vp, C.POINTER(RunStats), vp]),
+        "gi_bfs_profile": (C.c_int, [vp, i32, C.POINTER(HybridSchedule), vp,
+                                     C.POINTER(LevelRecord), i32, C.POINTER(i32), vp]),
+        "gi_graph_pick_sources": (C.c_int, [vp, u64, i32, vp]),
+        "gi_reached_edges": (C.c_int, [vp, vp, C.POINTER(i64), vp]),
+        "gi_pq_create": (C.c_int, [vp, vp, i32, i32, i32, pp]),
+        "gi_pq_finished": (C.c_int, [vp, C.POINTER(i32), vp]),
+        "gi_pq_dequeue": (C.c_int, [vp, pp, vp]),
+        "gi_pq_cutoff": (C.c_int, [vp, C.POINTER(i32)]),
+        "gi_pq_destroy": (C.c_int, [vp]),
+        "gi_sssp": (C.c_int, [vp, i32, i32, C.POINTER(Schedule), vp, C.POINTER(RunStats), vp]),
+        "gi_cc": (C.c_int, [vp, C.POINTER(Schedule), vp, C.POINTER(RunStats), vp]),
+        "gi_pagerank": (C.c_int, [vp, i32, C.c_float, C.POINTER(Schedule), vp,
+                                  C.POINTER(RunStats), vp]),
+        "gi_bc": (C.c_int, [vp, vp, i32, i32, vp, C.POINTER(RunStats), vp]),
+    }
+    for name, (res, args) in sig.items():
+        if hasattr(L, name):
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != GI_OK:
+        raise GIError(rc, lib().gi_last_error().decode(errors="replace"))
+
+
+def device_count() -> int:
+    return int(lib().gi_device_count())
+
+
+# --------------------------------------------------------------- device memory
+class DeviceArray:
+    """A device buffer allocated through gi_malloc (typed, 1-D)."""
+
+    def __init__(self, n: int, dtype):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        p = C.c_void_p()
+        check(lib().gi_malloc(C.byref(p), max(1, self.n * self.dtype.itemsize)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        check(lib().gi_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes, stream))
+        return d
+
+    def numpy(self, stream=None):
+        out = np.empty(self.n, self.dtype)
+        check(lib().gi_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes, stream))
+        return out
+
+    def fill(self, value, stream=None):
+        if self.dtype == np.int32:
+            check(lib().gi_fill_i32(self.ptr, self.n, int(value), stream))
+        elif self.dtype == np.float32:
+            check(lib().gi_fill_f32(self.ptr, self.n, float(value), stream))
+        elif self.dtype == np.uint8:
+            check(lib().gi_fill_u8(self.ptr, self.n, int(value), stream))
+        else:
+            self.copy_from(np.full(self.n, value, self.dtype), stream)
+        return self
+
+    def copy_from(self, a, stream=None):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.size == self.n
+        check(lib().gi_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes, stream))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().gi_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize(stream=None):
+    check(lib().gi_stream_synchronize(stream))
+
+
+# --------------------------------------------------------------- graph
+class Graph:
+    """Device CSR graph (GraphT<int> + load_graph, runtime/infra_gpu/graph.h)."""
+
+    def __init__(self, handle, owner=True):
+        self.h = handle
+        self.owner = owner
+
+    @property
+    def num_vertices(self) -> int:
+        v = C.c_int32()
+        check(lib().gi_graph_num_vertices(self.h, C.byref(v)))
+        return v.value
+
+    @property
+    def num_edges(self) -> int:
+        e = C.c_int64()
+        check(lib().gi_graph_num_edges(self.h, C.byref(e)))
+        return e.value
+
+    @classmethod
+    def rmat(cls, scale: int, edge_factor: int = 16, seed: int = 27491095,
+             symmetrize: bool = True):
+        h = C.c_void_p()
+        check(lib().gi_graph_rmat(scale, edge_factor, seed, int(symmetrize), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def uniform(cls, num_vertices: int, num_pairs: int, seed: int, symmetrize: bool = True):
+        h = C.c_void_p()
+        check(lib().gi_graph_uniform(num_vertices, num_pairs, seed, int(symmetrize), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def grid(cls, width: int, num_random: int, seed: int):
+        h = C.c_void_p()
+        check(lib().gi_graph_grid(width, num_random, seed, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_csr(cls, offsets, nbrs, weights=None, directed: bool = False):
+        off = np.ascontiguousarray(offsets, np.int32)
+        nb = np.ascontiguousarray(nbrs, np.int32)
+        w = None if weights is None else np.ascontiguousarray(weights, np.int32)
+        h = C.c_void_p()
+        check(lib().gi_graph_from_csr(len(off) - 1, len(nb), off.ctypes.data, nb.ctypes.data,
+                                      None if w is None else w.ctypes.data, int(directed),
+                                      C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_edges(cls, num_vertices: int, src, dst, symmetrize: bool):
+        s = np.ascontiguousarray(src, np.int32)
+        d = np.ascontiguousarray(dst, np.int32)
+        h = C.c_void_p()
+        check(lib().gi_graph_from_edges(num_vertices, len(s), s.ctypes.data, d.ctypes.data,
+                                        int(symmetrize), C.byref(h)))
+        return cls(h.value)
+
+    def pick_sources(self, seed: int, count: int) -> np.ndarray:
+        """SourcePicker semantics (benchmark.h:48-71): skip degree-0 vertices."""
+        out = np.empty(count, np.int32)
+        check(lib().gi_graph_pick_sources(self.h, seed, count, out.ctypes.data))
+        return out
+
+    def reached_edges(self, depth: "DeviceArray", stream=None) -> int:
+        e = C.c_int64()
+        check(lib().gi_reached_edges(self.h, depth.ptr, C.byref(e), stream))
+        return e.value
+
+    def set_weights(self, seed: int, wmin: int, wmax: int):
+        check(lib().gi_graph_set_weights(self.h, seed, wmin, wmax))
+        return self
+
+    def read_csr(self, weights: bool = False):
+        n, m = self.num_vertices, self.num_edges
+        off = np.empty(n + 1, np.int32)
+        nb = np.empty(max(m, 1), np.int32)
+        w = np.empty(max(m, 1), np.int32) if weights else None
+        check(lib().gi_graph_read_csr(self.h, off.ctypes.data, nb.ctypes.data,
+                                      None if w is None else w.ctypes.data))
+        return off, nb[:m], (None if w is None else w[:m])
+
+    def transpose(self) -> "Graph":
+        h = C.c_void_p()
+        check(lib().gi_graph_transpose(self.h, C.byref(h)))
+        return Graph(h.value, owner=False)
+
+    def blocked(self, block_size: int) -> "Graph":
+        h = C.c_void_p()
+        check(lib().gi_graph_blocked(self.h, block_size, C.byref(h)))
+        return Graph(h.value, owner=False)
+
+    def destroy(self):
+        if self.owner and self.h:
+            lib().gi_graph_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+load_graph = Graph  # name of the reference entry point (graph.h:236)
+
+
+# --------------------------------------------------------------- frontier
+class Frontier:
+    """VertexFrontier (runtime/infra_gpu/vertex_frontier.h:26-166)."""
+
+    def __init__(self, num_vertices: int, init_all: bool = False):
+        h = C.c_void_p()
+        check(lib().gi_frontier_create(num_vertices, int(init_all), C.byref(h)))
+        self.h = h.value
+        self.num_vertices = num_vertices
+
+    def add_vertex(self, v: int, stream=None):
+        check(lib().gi_frontier_add_vertex(self.h, v, stream))
+
+    addVertex = add_vertex
+
+    def clear(self, stream=None):
+        check(lib().gi_frontier_clear(self.h, stream))
+
+    def size(self, stream=None) -> int:
+        n = C.c_int32()
+        check(lib().gi_frontier_size(self.h, C.byref(n), stream))
+        return n.value
+
+    def convert(self, fmt: int, stream=None):
+        check(lib().gi_frontier_convert(self.h, fmt, stream))
+
+    @property
+    def format(self) -> int:
+        f = C.c_int32()
+        check(lib().gi_frontier_format(self.h, C.byref(f)))
+        return f.value
+
+    def read(self, stream=None) -> np.ndarray:
+        cap = self.num_vertices
+        out = np.empty(cap, np.int32)
+        n = C.c_int32()
+        check(lib().gi_frontier_read(self.h, out.ctypes.data, cap, C.byref(n), stream))
+        return out[: n.value]
+
+    def swap(self, fmt: int, stream=None):
+        check(lib().gi_frontier_swap(self.h, fmt, stream))
+
+    def dedup(self, stream=None):
+        check(lib().gi_frontier_dedup(self.h, stream))
+
+    def destroy(self):
+        if self.h and not getattr(self, "borrowed", False):
+            lib().gi_frontier_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+new_vertex_subset = Frontier  # runtime.cpp name for create_new_vertex_set
+
+
+class FrontierList:
+    """VertexFrontierList (runtime/infra_gpu/list.h:18-116)."""
+
+    def __init__(self, max_elems: int):
+        h = C.c_void_p()
+        check(lib().gi_frontier_list_create(max_elems, C.byref(h)))
+        self.h = h.value
+
+    def insert(self, f: Frontier, stream=None):
+        check(lib().gi_frontier_list_insert(self.h, f.h, stream))
+
+    def retrieve(self, f: Frontier, stream=None):
+        check(lib().gi_frontier_list_retrieve(self.h, f.h, stream))
+
+    @property
+    def levels(self) -> int:
+        n = C.c_int32()
+        check(lib().gi_frontier_list_levels(self.h, C.byref(n)))
+        return n.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().gi_frontier_list_destroy(self.h)
+        except Exception:
+            pass
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    return x
+
+
+def op_args(data0=None, data1=None, data2=None, i0=0, i1=0, f0=0.0, f1=0.0) -> OpArgs:
+    a = OpArgs()
+    a.data0, a.data1, a.data2 = _ptr(data0), _ptr(data1), _ptr(data2)
+    a.i0, a.i1, a.f0, a.f1 = i0, i1, f0, f1
+    return a
+
+
+def edgeset_apply(g: Graph, s, frm: Frontier | None, to: Frontier | None, op: int,
+                  args: OpArgs, stream=None):
+    """edgeset_apply(s).from(frm).apply / apply_modified (operators.cpp:513-650)."""
+    if isinstance(s, HybridSchedule):
+        check(lib().gi_edgeset_apply_hybrid(g.h, C.byref(s), frm.h if frm else None,
+                                            to.h if to else None, op, C.byref(args), stream))
+    else:
+        check(lib().gi_edgeset_apply(g.h, C.byref(s), frm.h if frm else None,
+                                     to.h if to else None, op, C.byref(args), stream))
+
+
+def vertexset_apply(g: Graph, f: Frontier | None, vop: int, args: OpArgs, stream=None):
+    check(lib().gi_vertexset_apply(g.h, f.h if f else None, vop, C.byref(args), stream))
+
+
+# --------------------------------------------------------------- app drivers
+def bfs(g: Graph, source: int, schedule: HybridSchedule | None = None,
+        want_parent: bool = False, stream=None):
+    """Depths (and optionally CAS parents) from source; returns numpy arrays."""
+    n = g.num_vertices
+    depth = DeviceArray(n, np.int32)
+    parent = DeviceArray(n, np.int32) if want_parent else None
+    st = RunStats()
+    check(lib().gi_bfs(g.h, source, C.byref(schedule) if schedule is not None else None,
+                       depth.ptr, parent.ptr if parent else None, C.byref(st), stream))
+    return depth.numpy(stream), (parent.numpy(stream) if parent else None), st.as_dict()
+
+
+def bfs_device(g: Graph, source: int, schedule, depth: DeviceArray, parent=None, stream=None):
+    st = RunStats()
+    check(lib().gi_bfs(g.h, source, C.byref(schedule) if schedule is not None else None,
+                       depth.ptr, parent.ptr if parent else None, C.byref(st), stream))
+    return st.as_dict()
+
+
+def bfs_profile(g: Graph, source: int, schedule, depth: DeviceArray, stream=None):
+    recs = (LevelRecord * 4096)()
+    n = C.c_int32()
+    check(lib().gi_bfs_profile(g.h, source, C.byref(schedule) if schedule is not None else None,
+                               depth.ptr, recs, 4096, C.byref(n), stream))
+    return [recs[i].as_dict() for i in range(min(n.value, 4096))]
+
+
+class PriorityQueue:
+    """GPUPriorityQueue<int> (runtime/infra_gpu/gpu_priority_queue.h:21-247)."""
+
+    def __init__(self, g: Graph, prio: DeviceArray, initial: int, delta: int, source: int):
+        h = C.c_void_p()
+        check(lib().gi_pq_create(g.h, prio.ptr, initial, delta, source, C.byref(h)))
+        self.h = h.value
+        self.g = g
+
+    def finished(self, stream=None) -> bool:
+        f = C.c_int32()
+        check(lib().gi_pq_finished(self.h, C.byref(f), stream))
+        return bool(f.value)
+
+    def dequeue(self, stream=None) -> "Frontier":
+        h = C.c_void_p()
+        check(lib().gi_pq_dequeue(self.h, C.byref(h), stream))
+        f = Frontier.__new__(Frontier)
+        f.h = h.value
+        f.num_vertices = self.g.num_vertices
+        f.borrowed = True
+        return f
+
+    def prio_cutoff(self) -> int:
+        c = C.c_int32()
+        check(lib().gi_pq_cutoff(self.h, C.byref(c)))
+        return c.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().gi_pq_destroy(self.h)
+        except Exception:
+            pass
+
+
+def sssp(g: Graph, source: int, delta: int, schedule: Schedule | None = None, stream=None):
+    n = g.num_vertices
+    dist = DeviceArray(n, np.int32)
+    st = RunStats()
+    check(lib().gi_sssp(g.h, source, delta, C.byref(schedule) if schedule is not None else None,
+                        dist.ptr, C.byref(st), stream))
+    return dist.numpy(stream), st.as_dict()
+
+
+def cc(g: Graph, schedule: Schedule | None = None, stream=None):
+    n = g.num_vertices
+    lab = DeviceArray(n, np.int32)
+    st = RunStats()
+    check(lib().gi_cc(g.h, C.byref(schedule) if schedule is not None else None, lab.ptr,
+                      C.byref(st), stream))
+    return lab.numpy(stream), st.as_dict()
+
+
+def pagerank(g: Graph, iters: int = 20, damping: float = PR_DAMP,
+             schedule: Schedule | None = None, stream=None):
+    n = g.num_vertices
+    rank = DeviceArray(n, np.float32)
+    st = RunStats()
+    check(lib().gi_pagerank(g.h, iters, damping,
+                            C.byref(schedule) if schedule is not None else None, rank.ptr,
+                            C.byref(st), stream))
+    return rank.numpy(stream), st.as_dict()
+
+
+def bc(g: Graph, sources, batch: int = 0, stream=None):
+    n = g.num_vertices
+    src = np.ascontiguousarray(sources, np.int32)
+    scores = DeviceArray(n, np.float32)
+    st = RunStats()
+    check(lib().gi_bc(g.h, src.ctypes.data, len(src), batch, scores.ptr, C.byref(st), stream))
+    return scores.numpy(stream), st.as_dict()

@@ -1,0 +1,267 @@
+// Internal definitions shared by the HIP translation units of libgraphit_amd.
+// gfx950 (CDNA4) only: wave64, 64-bit ballots, 64-bit bitmap words.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+
+#include "../../include/graphit_amd.h"
+
+namespace gi {
+
+// ----------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+int fail(int code, const char *fmt, ...);
+
+#define GI_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t _e = (call);                                                   \
+        if (_e != hipSuccess)                                                     \
+            return ::gi::fail(GI_ERR_HIP, "%s failed: %s (%s:%d)", #call,         \
+                              hipGetErrorString(_e), __FILE__, __LINE__);         \
+    } while (0)
+
+#define GI_TRY(call)                   \
+    do {                               \
+        int _rc = (call);              \
+        if (_rc != GI_OK) return _rc;  \
+    } while (0)
+
+#define GI_LAUNCH_CHECK() GI_HIP(hipGetLastError())
+
+// ----------------------------------------------------------------- device
+struct DeviceInfo {
+    int num_cus = 256;
+    int max_threads_per_cu = 2048;
+};
+const DeviceInfo &device_info();
+// Grid size for persistent grid-stride kernels: enough workgroups to fill
+// every CU several times over.
+int persistent_grid(int block, int per_cu = 8);
+
+__host__ __device__ inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+__host__ __device__ inline int64_t bitmap_words(int64_t n) { return (n + 63) / 64; }
+
+// Device allocation helpers (return GI codes).
+int dmalloc(void **p, size_t bytes);
+template <class T>
+int dalloc(T **p, size_t count) {
+    return dmalloc(reinterpret_cast<void **>(p), count * sizeof(T) + 1);
+}
+void dfree(void *p);
+
+// Small fill kernels (graph-capture safe, unlike hipMemsetAsync nodes).
+int fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t s);
+int fill_u64(uint64_t *p, int64_t n, uint64_t v, hipStream_t s);
+int fill_u8(uint8_t *p, int64_t n, uint8_t v, hipStream_t s);
+int fill_f32(float *p, int64_t n, float v, hipStream_t s);
+int fill_f64(double *p, int64_t n, double v, hipStream_t s);
+
+// Device-wide exclusive scan of int32 counts into int64-safe int32 prefix
+// (n + 1 outputs, out[n] = total); rocPRIM under the hood, workspace cached.
+int exclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n, hipStream_t s);
+
+// ----------------------------------------------------------------- views
+struct DGraph {
+    int32_t V;
+    int32_t E;
+    const int32_t *off;   // [V+1]
+    const int32_t *nbr;   // [E]
+    const int32_t *w;     // [E] or null (weight 1)
+    const int32_t *esrc;  // [E] COO source or null
+};
+
+enum SinkMode : int32_t { SINK_NONE = 0, SINK_SPARSE = 1, SINK_BITMAP = 2, SINK_BOOLMAP = 3 };
+
+// Where tracked writes go (the output side of a frontier).
+struct Sink {
+    int32_t mode;
+    int32_t cap;         // sparse capacity
+    int32_t *q;          // sparse queue
+    uint64_t *bm;        // bitmap words
+    uint8_t *bmap;       // boolmap bytes
+    int32_t *count;      // element count
+    int32_t *dedup;      // per-vertex epoch (sparse dedup) or null
+    int32_t epoch;
+    int32_t *overflow;   // set when a sparse append exceeds cap
+};
+
+// Enumerator: all vertices (q == null), or the members of a sparse queue
+// whose length lives on the device.
+struct VEnum {
+    const int32_t *q;
+    const int32_t *count;
+    int32_t V;  // vertex count, and the capacity of q
+    __device__ int32_t size() const {
+        if (!q) return V;
+        int32_t n = *count;
+        return n < V ? n : V;
+    }
+    __device__ int32_t at(int32_t i) const { return q ? q[i] : i; }
+};
+
+// Membership test of the input side (pull's dst filter, edge-only's src filter).
+struct Member {
+    int32_t mode;        // GI_FMT_BITMAP or GI_FMT_BOOLMAP (or -1 = everyone)
+    const uint64_t *bm;
+    const uint8_t *bmap;
+};
+
+// ------------------------------------------------------- wave64 primitives
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Wave-aggregated atomic add: every active lane passes its amount; returns
+// the lane's exclusive offset from the single atomic the wave issues.
+__device__ __forceinline__ int32_t wave_atomic_add(int32_t *ctr, bool pred) {
+    uint64_t mask = __ballot(pred);
+    if (mask == 0) return 0;
+    int leader = __ffsll((unsigned long long)mask) - 1;
+    int32_t base = 0;
+    if ((int)lane_id() == leader) base = atomicAdd(ctr, (int32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (int32_t)lanes_below(mask);
+}
+
+__device__ __forceinline__ bool bit_test(const uint64_t *bm, int32_t v) {
+    return (bm[v >> 6] >> (v & 63)) & 1ull;
+}
+
+// Returns true when this call set the bit.
+__device__ __forceinline__ bool bit_claim(uint64_t *bm, int32_t v) {
+    unsigned long long bit = 1ull << (v & 63);
+    unsigned long long old = atomicOr((unsigned long long *)&bm[v >> 6], bit);
+    return (old & bit) == 0;
+}
+
+// Byte flag set through a 32-bit atomic on the containing word; true when
+// this call set it (exact counts, where the reference's bytemap enqueue
+// over-counts, vertex_frontier.h:195-201).
+__device__ __forceinline__ bool byte_claim(uint8_t *bmap, int32_t v) {
+    uint32_t *word = reinterpret_cast<uint32_t *>(bmap + (v & ~3));
+    uint32_t bit = 1u << (8 * (v & 3));
+    uint32_t old = atomicOr(word, bit);
+    return (old & (0xffu << (8 * (v & 3)))) == 0;
+}
+
+__device__ __forceinline__ bool member(const Member &m, int32_t v) {
+    if (m.mode == GI_FMT_BITMAP) return bit_test(m.bm, v);
+    if (m.mode == GI_FMT_BOOLMAP) return m.bmap[v] != 0;
+    return true;
+}
+
+// Called by every active lane of the wave (pred may be false).
+__device__ __forceinline__ void sink_emit(const Sink &s, bool pred, int32_t v) {
+    if (s.mode == SINK_NONE) return;
+    if (s.mode == SINK_SPARSE) {
+        if (pred && s.dedup) pred = atomicMax(&s.dedup[v], s.epoch) < s.epoch;
+        int32_t pos = wave_atomic_add(s.count, pred);
+        if (pred) {
+            if (pos < s.cap) s.q[pos] = v;
+            else atomicOr(s.overflow, 1);
+        }
+    } else if (s.mode == SINK_BITMAP) {
+        bool fresh = pred && bit_claim(s.bm, v);
+        wave_atomic_add(s.count, fresh);
+    } else {
+        bool fresh = pred && byte_claim(s.bmap, v);
+        wave_atomic_add(s.count, fresh);
+    }
+}
+
+// Upper bound in a sorted int32 array: first index i in [lo, hi) with a[i] > x.
+__device__ __forceinline__ int32_t upper_bound_i32(const int32_t *a, int32_t lo, int32_t hi,
+                                                   int32_t x) {
+    while (lo < hi) {
+        int32_t mid = lo + ((hi - lo) >> 1);
+        if (a[mid] <= x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+}  // namespace gi
+
+// ----------------------------------------------------------- host handles
+struct gi_graph {
+    int32_t V = 0;
+    int64_t E = 0;
+    int32_t directed = 0;
+    int32_t *d_off = nullptr;
+    int32_t *d_nbr = nullptr;
+    int32_t *d_w = nullptr;
+    int32_t *d_esrc = nullptr;       // COO source ids, built on demand
+    int32_t *d_bucket_end = nullptr;  // blocked graphs: bucket end offsets
+    int32_t num_buckets = 0;
+    int32_t block_size = -1;
+    int32_t max_degree = -1;
+    int32_t *d_scratch = nullptr;    // load-balance scratch (TWC bins, STRICT prefix)
+    int64_t scratch_len = 0;
+    gi_frontier *work = nullptr;     // frontier reused by the fused drivers
+    uint64_t *d_visited = nullptr;   // visited bitmap of the fused BFS
+    gi_graph *transposed = nullptr;  // owned unless is_transpose_of is set
+    gi_graph *transpose_owner = nullptr;
+    std::map<int32_t, gi_graph *> blocked;  // owned
+    bool borrowed = false;  // owned by another graph
+
+    gi::DGraph view() const {
+        gi::DGraph g;
+        g.V = V;
+        g.E = (int32_t)E;
+        g.off = d_off;
+        g.nbr = d_nbr;
+        g.w = d_w;
+        g.esrc = d_esrc;
+        return g;
+    }
+};
+
+struct gi_frontier {
+    int32_t V = 0;
+    int32_t cap = 0;
+    int32_t fmt = GI_FMT_SPARSE;  // format of the input side
+    int32_t side = 0;             // index of the input side
+    int32_t *q[2] = {nullptr, nullptr};
+    uint64_t *bm[2] = {nullptr, nullptr};
+    uint8_t *bmap[2] = {nullptr, nullptr};
+    int32_t *counts = nullptr;    // [2] element counts + [2] overflow flags
+    int32_t *dedup = nullptr;     // per-vertex epochs
+    int32_t epoch = 0;
+    int64_t words = 0;
+
+    int32_t *count_in() const { return counts + side; }
+    int32_t *count_out() const { return counts + (side ^ 1); }
+    int32_t *overflow() const { return counts + 2; }
+};
+
+namespace gi {
+int ensure_coo(gi_graph *g, hipStream_t s);
+int ensure_max_degree(gi_graph *g, hipStream_t s);
+int ensure_scratch(gi_graph *g, int64_t len);
+// Scratch layout per graph: load-balance region first (TWC bins 3V+3, or the
+// STRICT prefix V + chunk sums), then the pull enumerator (count, queue).
+inline int64_t scratch_lb_len(int64_t V) {
+    int64_t twc = 3 * V + 64, strict = V + 64 + 1024 + 64;
+    return twc > strict ? twc : strict;
+}
+inline int64_t scratch_total_len(int64_t V) { return scratch_lb_len(V) + 64 + V + 64; }
+int validate_schedule(const gi_schedule *s);
+int plan_grid(const gi_schedule *s);
+struct OpBfsVisit;
+int edgeset_apply(gi_graph *g, const gi_schedule *s, gi_frontier *from, gi_frontier *to,
+                  int32_t op, const gi_op_args &a, hipStream_t st, const OpBfsVisit *visit,
+                  hipEvent_t *events = nullptr);
+int frontier_convert(gi_frontier *f, int32_t fmt, hipStream_t s);
+int frontier_clear_out(gi_frontier *f, int32_t fmt, hipStream_t s);
+Sink make_sink(gi_frontier *f, int32_t fmt, bool dedup);
+Member make_member(const gi_frontier *f);
+int read_count(const int32_t *d_count, int32_t *out, hipStream_t s);
+}  // namespace gi

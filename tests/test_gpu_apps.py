@@ -1,0 +1,72 @@
+"""SSSP, CC, PageRank and BC on the device vs the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+SEED = 27491095
+
+
+def host_csr(g, weights=False):
+    off, nb, w = g.read_csr(weights=weights)
+    return O.CSR(off, nb, w)
+
+
+@pytest.mark.parametrize("lb", [0, 1, 2, 3, 4, 5])
+def test_sssp_grid_bit_exact(gi, lb):
+    g = gi.Graph.grid(96, 92, 3).set_weights(5, 1, 1000)
+    host = host_csr(g, True)
+    want = O.sssp(host, 0)
+    s = gi.Schedule(direction=gi.PUSH, load_balancing=lb)
+    dist, st = gi.sssp(g, 0, 64, s)
+    assert np.array_equal(dist, want)
+
+
+def test_sssp_rmat_default_schedule(gi):
+    g = gi.Graph.rmat(14, 16, SEED, True).set_weights(SEED, 1, 255)
+    host = host_csr(g, True)
+    for src in O.pick_sources(host, 1, 3):
+        dist, _ = gi.sssp(g, int(src), 64)
+        assert np.array_equal(dist, O.sssp(host, int(src)))
+
+
+@pytest.mark.parametrize("lb", [6, 0, 2, 4])
+def test_cc_bit_exact(gi, lb):
+    g = gi.Graph.uniform(20000, 15000, 4, True)
+    host = host_csr(g)
+    lab, st = gi.cc(g, gi.Schedule(direction=gi.PUSH, load_balancing=lb))
+    assert np.array_equal(lab, O.cc(host))
+
+
+def test_cc_rmat(gi):
+    g = gi.Graph.rmat(15, 16, SEED, True)
+    lab, _ = gi.cc(g)
+    assert np.array_equal(lab, O.cc(host_csr(g)))
+
+
+def rel_l1(a, b):
+    return float(np.abs(a.astype(np.float64) - b).sum() / np.abs(b).sum())
+
+
+def test_pagerank_default_schedule(gi):
+    g = gi.Graph.rmat(14, 16, SEED, False)
+    r, _ = gi.pagerank(g, 20)
+    want = O.pagerank(host_csr(g), 20)
+    assert rel_l1(r, want) <= 1e-5  # FP32 vs FP64 oracle, relative L1
+
+
+@pytest.mark.parametrize("direction,lb", [(1, 0), (1, 2), (0, 6), (1, 6), (0, 0)])
+def test_pagerank_schedules(gi, direction, lb):
+    g = gi.Graph.rmat(12, 16, 5, False)
+    r, _ = gi.pagerank(g, 20, schedule=gi.Schedule(direction=direction, load_balancing=lb))
+    assert rel_l1(r, O.pagerank(host_csr(g), 20)) <= 1e-5
+
+
+def test_bc_small(gi):
+    g = gi.Graph.rmat(12, 8, SEED, True)
+    host = host_csr(g)
+    srcs = O.pick_sources(host, 7, 4)
+    sc, _ = gi.bc(g, srcs)
+    want = O.bc(host, srcs)
+    assert rel_l1(sc, want) <= 1e-4
