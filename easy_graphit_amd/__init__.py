@@ -182,6 +182,13 @@ def lib():
                                      C.POINTER(LevelRecord), i32, C.POINTER(i32), vp]),
         "gi_graph_pick_sources": (C.c_int, [vp, u64, i32, vp]),
         "gi_reached_edges": (C.c_int, [vp, vp, C.POINTER(i64), vp]),
+        "gi_selector_param_count": (i32, [i32]),
+        "gi_selector_load": (C.c_int, [vp, i32, i32, vp, vp]),
+        "gi_selector_clear": (C.c_int, []),
+        "gi_selector_loaded": (i32, []),
+        "gi_selector_loss_grad": (C.c_int, [vp, vp, vp, i32, i32, vp, vp, vp]),
+        "gi_selector_train": (C.c_int, [vp, vp, vp, i32, i32, i32, C.c_float, C.c_float, vp, vp]),
+        "gi_bfs_sweep": (C.c_int, [vp, i32, i32, i32, i32, vp, vp, i32, C.POINTER(i32), vp]),
         "gi_pq_create": (C.c_int, [vp, vp, i32, i32, i32, pp]),
         "gi_pq_finished": (C.c_int, [vp, C.POINTER(i32), vp]),
         "gi_pq_dequeue": (C.c_int, [vp, pp, vp]),
@@ -518,6 +525,64 @@ def bfs_profile(g: Graph, source: int, schedule, depth: DeviceArray, stream=None
     check(lib().gi_bfs_profile(g.h, source, C.byref(schedule) if schedule is not None else None,
                                depth.ptr, recs, 4096, C.byref(n), stream))
     return [recs[i].as_dict() for i in range(min(n.value, 4096))]
+
+
+BFS_MODES = 7
+BFS_MODE_NAMES = ["push_thread", "push_wave", "push_twce", "push_cm", "push_twc",
+                  "pull_thread", "pull_group"]
+SEL_INPUTS, SEL_HIDDEN = 16, 64
+
+
+def bfs_sweep(g: Graph, source: int, mode_mask: int = (1 << BFS_MODES) - 1, follow: int = -1,
+              reps: int = 2, max_levels: int = 512, stream=None):
+    """Time every fused-BFS step mode at every level (dataset recorder)."""
+    feats = np.zeros((max_levels, SEL_INPUTS), np.float32)
+    times = np.zeros((max_levels, BFS_MODES), np.float32)
+    n = C.c_int32()
+    check(lib().gi_bfs_sweep(g.h, source, mode_mask, follow, reps, feats.ctypes.data,
+                             times.ctypes.data, max_levels, C.byref(n), stream))
+    return feats[: n.value].copy(), times[: n.value].copy()
+
+
+def selector_param_count(n_out: int = BFS_MODES) -> int:
+    return int(lib().gi_selector_param_count(n_out))
+
+
+def selector_load(params, n_out: int, allowed_mask: int, feat_mean, feat_scale):
+    p = np.ascontiguousarray(params, np.float32)
+    assert p.size == selector_param_count(n_out)
+    mu = np.ascontiguousarray(feat_mean, np.float32)
+    sc = np.ascontiguousarray(feat_scale, np.float32)
+    check(lib().gi_selector_load(p.ctypes.data, n_out, allowed_mask, mu.ctypes.data,
+                                 sc.ctypes.data))
+
+
+def selector_clear():
+    check(lib().gi_selector_clear())
+
+
+def selector_loss_grad(X, T, M, params, n_out: int):
+    X = np.ascontiguousarray(X, np.float32)
+    T = np.ascontiguousarray(T, np.float32)
+    M = np.ascontiguousarray(M, np.float32)
+    p = np.ascontiguousarray(params, np.float32)
+    g = np.zeros_like(p)
+    loss = np.zeros(1, np.float32)
+    check(lib().gi_selector_loss_grad(X.ctypes.data, T.ctypes.data, M.ctypes.data, X.shape[0],
+                                      n_out, p.ctypes.data, g.ctypes.data, loss.ctypes.data))
+    return float(loss[0]), g
+
+
+def selector_train(X, T, M, params, n_out: int, epochs: int, lr: float = 3e-3,
+                   weight_decay: float = 0.0):
+    X = np.ascontiguousarray(X, np.float32)
+    T = np.ascontiguousarray(T, np.float32)
+    M = np.ascontiguousarray(M, np.float32)
+    p = np.ascontiguousarray(params, np.float32).copy()
+    hist = np.zeros(max(epochs, 1), np.float32)
+    check(lib().gi_selector_train(X.ctypes.data, T.ctypes.data, M.ctypes.data, X.shape[0], n_out,
+                                  epochs, lr, weight_decay, p.ctypes.data, hist.ctypes.data))
+    return p, hist[:epochs]
 
 
 class PriorityQueue:

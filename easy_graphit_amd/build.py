@@ -53,6 +53,15 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # C++ apps over the mirror API (include/graphit_amd.hpp), linked to the library
+    include = os.path.join(os.path.dirname(HERE), "include")
+    for app in sorted(glob.glob(os.path.join(HERE, "apps", "*.cpp"))):
+        exe = app[:-4] + "_app"
+        cmd = ["g++", "-std=c++17", "-O2", f"-I{include}", app, f"-L{HERE}", "-lgraphit_amd",
+               "-Wl,-rpath,$ORIGIN/..", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"app build failed for {app}:\n{r.stderr}")
     if verbose:
         print(f"built {LIB} from {len(sources)} sources")
     return LIB

@@ -5,6 +5,7 @@
 
 #include "gi_hash.h"
 
+#include "gi_bfs_fused.h"
 #include "gi_lb.h"
 
 namespace gi {
@@ -136,6 +137,46 @@ static int bfs_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int
     if (!h) { default_bfs_schedule(&def); h = &def; }
     GI_TRY(validate_schedule(&h->s1));
     GI_TRY(validate_schedule(&h->s2));
+    if (h->s1.kernel_fusion == GI_FUSION_ENABLED) {
+        // fused engine: schedule decisions on the device
+        SelectorView sel{};
+        if (h->selector == GI_SELECT_LEARNED) sel = selector_view();  // empty: heuristic fallback
+        if (!sel.w1) sel.mode = h->selector == GI_SELECT_LEARNED ? GI_SELECT_HEURISTIC : h->selector;
+        FusedTrace ft;
+        EventTimer timer(s);
+        int32_t levels = 0;
+        int rc = bfs_fused_run(g, source, h, sel, d_depth, d_parent, s, &ft, trace != nullptr, &levels);
+        double ms = timer.stop();
+        if (rc == GI_OK && trace) {
+            for (int32_t i = 0; i < levels && trace->n < trace->max; ++i) {
+                gi_level_record &r = trace->recs[trace->n++];
+                r.level = i;
+                r.direction = ft.modes[i] >= BFS_PULL_THREAD ? GI_PULL : GI_PUSH;
+                r.load_balancing = 100 + ft.modes[i];  // fused mode id
+                r.frontier_size = ft.sizes[i];
+                r.next_size = i + 1 < levels ? ft.sizes[i + 1] : 0;
+            }
+            trace->ev.swap(ft.ev);
+        }
+        for (hipEvent_t e : ft.ev) (void)hipEventDestroy(e);
+        GI_TRY(rc);
+        if (stats) {
+            stats->iterations = levels;
+            stats->push_iterations = 0;
+            stats->pull_iterations = 0;
+            for (int32_t m : ft.modes) (m >= BFS_PULL_THREAD ? stats->pull_iterations : stats->push_iterations)++;
+            stats->ms = ms;
+            stats->edges_traversed = 0;
+            int32_t *cnt = nullptr;
+            GI_TRY(dalloc(&cnt, 1));
+            fill_i32(cnt, 1, 0, s);
+            count_reached_kernel<<<grid_for(g->V), 256, 0, s>>>(d_depth, g->V, cnt);
+            int rc2 = read_count(cnt, &stats->reached, s);
+            dfree(cnt);
+            GI_TRY(rc2);
+        }
+        return GI_OK;
+    }
     GI_TRY(ensure_work(g));
     gi_frontier *f = g->work;
     EventTimer timer(s);

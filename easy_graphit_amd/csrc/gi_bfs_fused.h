@@ -1,0 +1,106 @@
+// Shared definitions of the fused BFS engine and the schedule selector.
+#pragma once
+
+#include <vector>
+
+#include "gi_internal.h"
+
+namespace gi {
+
+// Step modes of the fused engine (the selector's output classes).
+enum BfsMode : int32_t {
+    BFS_PUSH_THREAD = 0,  // VERTEX_BASED push
+    BFS_PUSH_WAVE = 1,    // WM push
+    BFS_PUSH_TWCE = 2,    // TWCE push (lane / wave granules inside a workgroup)
+    BFS_PUSH_CM = 3,      // CM push (workgroup merge-path)
+    BFS_PUSH_TWC = 4,     // TWC push with a grid-wide bin for hubs
+    BFS_PULL_THREAD = 5,  // pull, lane per vertex, bitmap frontier
+    BFS_PULL_GROUP = 6,   // pull, 8 lanes per vertex, bitmap frontier
+    BFS_NUM_MODES = 7
+};
+
+constexpr int kBfsMaxLevels = 4096;
+constexpr int kSelInputs = 16;
+constexpr int kSelHidden = 64;
+constexpr int kSelMaxOut = 8;
+constexpr int32_t kHubDegree = 4096;  // push vertices at least this wide go to the grid bin
+constexpr int32_t kMaxHubs = 8192;
+
+struct BfsState {
+    int32_t count[2];               // sparse frontier sizes by level parity
+    unsigned long long fdeg[2];     // sum of frontier degrees by parity
+    int32_t done;
+    int32_t level;                  // level of the frontier the next step expands
+    int32_t levels;                 // BFS depth + 1 when done
+    int32_t mode;                   // mode of the next step
+    long long unvisited;            // vertices not reached yet
+    long long unvisited_edges;      // sum of their degrees
+    int32_t hub_count;              // deferred push vertices of the current step
+    int32_t pad;
+    int32_t hubs[kMaxHubs];
+    int32_t hist_mode[kBfsMaxLevels];
+    int32_t hist_next[kBfsMaxLevels];
+};
+
+// Device view of the selector network (BF16 weights, FP32 accumulation).
+struct SelectorView {
+    const uint16_t *w1, *b1, *w2, *b2, *w3, *b3;  // [in][hid], [hid], [hid][hid], [hid], [hid][out], [out]
+    int32_t n_out;
+    int32_t allowed;     // bit mask of modes the selector may pick
+    int32_t mode;        // gi_selector_mode
+    int32_t mode_s1;     // heuristic's s1 / s2 modes
+    int32_t mode_s2;
+    float threshold;
+    const float *feat_mean;  // feature normalisation (kSelInputs)
+    const float *feat_scale;
+};
+
+// The selector's 16 input features, computed from device counters (no host
+// sync). Shared by the decide kernel and the dataset recorder so training and
+// inference see identical inputs.
+__host__ __device__ inline void bfs_raw_features(float *f, int32_t n, unsigned long long fdeg,
+                                                 long long unvisited, long long unvisited_edges,
+                                                 int32_t level, int32_t prev_mode,
+                                                 int32_t prev_n, int32_t V, int64_t E) {
+    double dV = V > 0 ? (double)V : 1.0, dE = E > 0 ? (double)E : 1.0;
+    double nn = n, fd = (double)fdeg;
+    f[0] = (float)(nn / dV);                                   // frontier share of V
+    f[1] = (float)(fd / dE);                                   // frontier share of E
+    f[2] = (float)((double)unvisited / dV);                    // unvisited share of V
+    f[3] = (float)((double)unvisited_edges / dE);              // unvisited edges share
+    f[4] = (float)(log2(1.0 + nn) / 32.0);
+    f[5] = (float)(log2(1.0 + fd) / 40.0);
+    f[6] = (float)(log2(1.0 + fd / (nn > 0 ? nn : 1.0)) / 20.0);  // mean frontier degree
+    f[7] = (float)(log2(1.0 + dE / dV) / 20.0);               // mean degree
+    f[8] = (float)(log2(dV) / 32.0);
+    f[9] = (float)(level < 64 ? level / 64.0 : 1.0);
+    f[10] = prev_mode >= BFS_PULL_THREAD ? 1.f : 0.f;          // previous step pulled
+    f[11] = (float)(log2(1.0 + nn) - log2(1.0 + (double)prev_n)) / 16.f;  // growth
+    f[12] = (float)(fd / (1.0 + (double)unvisited_edges));     // Beamer's m_f / m_u
+    f[13] = (float)(nn / (1.0 + (double)unvisited));
+    f[14] = (float)(log2(1.0 + (double)unvisited) / 32.0);
+    f[15] = 1.f;                                               // bias feature
+}
+
+__device__ inline void bfs_features(float *f, int32_t n, unsigned long long fdeg,
+                                    long long unvisited, long long unvisited_edges,
+                                    int32_t level, int32_t prev_mode, int32_t prev_n,
+                                    int32_t V, int64_t E) {
+    bfs_raw_features(f, n, fdeg, unvisited, unvisited_edges, level, prev_mode, prev_n, V, E);
+}
+
+struct FusedTrace {
+    std::vector<hipEvent_t> ev;   // two per level: around the step + hub kernels
+    std::vector<int32_t> modes;   // mode of each level
+    std::vector<int32_t> sizes;   // frontier size of each level
+};
+
+int32_t fused_mode_of(const gi_schedule &s);
+int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, SelectorView sel,
+                  int32_t *d_depth, int32_t *d_parent, hipStream_t s, FusedTrace *trace,
+                  bool timed, int32_t *levels_out);
+void release_fused(const gi_graph *g);
+// The process-wide selector model (null weights when none is loaded).
+SelectorView selector_view();
+
+}  // namespace gi

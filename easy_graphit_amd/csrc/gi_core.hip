@@ -12,6 +12,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include "gi_bfs_fused.h"
 #include "gi_hash.h"
 #include "gi_internal.h"
 
@@ -743,6 +744,7 @@ int gi_graph_destroy(gi_graph *g) {
     dfree(g->d_scratch);
     dfree(g->d_visited);
     if (g->work) gi_frontier_destroy(g->work);
+    release_fused(g);
     delete g;
     return GI_OK;
 }

@@ -332,6 +332,44 @@ int gi_graph_pick_sources(const gi_graph *g, uint64_t seed, int32_t count, int32
  * undirected edges of the reached component on a symmetric graph). */
 int gi_reached_edges(const gi_graph *g, const int32_t *d_depth, int64_t *out, void *stream);
 
+/* ------------------------------------------------------- learned selector
+ * No counterpart in the reference (its only schedule choice is the fixed
+ * hybrid rule, operators.cpp:526). With kernel_fusion ENABLED in s1, gi_bfs
+ * runs the fused engine: per level a decide kernel on the device picks one of
+ * GI_BFS_MODES step modes, either by GraphIt's rule over (s1, s2) or, with
+ * selector GI_SELECT_LEARNED and a loaded model, by a 16-64-64-K MLP that
+ * predicts each mode's log time (BF16 weights, FP32 accumulation); outputs
+ * are identical whichever mode runs. Modes: 0 push lane-per-vertex, 1 push
+ * wave-per-vertex, 2 push TWCE, 3 push CM, 4 push TWC with a grid-wide hub
+ * bin, 5 pull lane-per-vertex, 6 pull 8-lanes-per-vertex. */
+#define GI_BFS_MODES 7
+/* Parameters: w1[16][64], b1[64], w2[64][64], b2[64], w3[64][n_out], b3[n_out]
+ * (row-major [in][out]), FP32; stored on the device as BF16. Features are
+ * normalised as (x - feat_mean) * feat_scale before the first layer. */
+int32_t gi_selector_param_count(int32_t n_out);
+int gi_selector_load(const float *params, int32_t n_out, int32_t allowed_mask,
+                     const float *feat_mean, const float *feat_scale);
+int gi_selector_clear(void);
+int32_t gi_selector_loaded(void);
+/* One forward + backward pass of the FP32 training network on the device:
+ * masked mean squared error between outputs and T over entries where M = 1.
+ * X[batch][16] (already normalised), T and M [batch][n_out]. */
+int gi_selector_loss_grad(const float *X, const float *T, const float *M, int32_t batch,
+                          int32_t n_out, const float *params, float *grads, float *loss);
+/* Full-batch Adam for `epochs` steps on the device; params updated in
+ * place; loss_history[epochs] optional. */
+int gi_selector_train(const float *X, const float *T, const float *M, int32_t batch,
+                      int32_t n_out, int32_t epochs, float lr, float weight_decay,
+                      float *params, float *loss_history);
+/* Dataset recorder: a BFS from source where every level is run once per
+ * mode of mode_mask from the same snapshot and timed (min of reps); the run
+ * then continues with `follow` (or the fastest mode when follow < 0).
+ * features[max_levels][16] (raw, unnormalised), times_ms[max_levels][7]
+ * (-1 for modes not measured). Synchronises. */
+int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow, int32_t reps,
+                 float *features, float *times_ms, int32_t max_levels, int32_t *num_levels,
+                 void *stream);
+
 /* Delta-stepping SSSP (apps/sssp.cpp:34-57); d_dist int32, INT_MAX unreached. */
 int gi_sssp(gi_graph *g, int32_t source, int32_t delta, const gi_schedule *s,
             int32_t *d_dist, gi_run_stats *stats, void *stream);

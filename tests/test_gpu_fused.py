@@ -1,0 +1,144 @@
+"""Fused BFS engine (device-side schedule decisions) and the learned
+selector: every step mode, the hybrid rule and the MLP give depths
+bit-exact with the oracle; the selector's device gradients match the FP64
+oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+SEED = 27491095
+
+# fused mode -> (direction, load balancing) that selects it
+MODE_SCHED = {0: (0, 0), 1: (0, 3), 2: (0, 2), 3: (0, 4), 4: (0, 1), 5: (1, 0), 6: (1, 2)}
+
+
+def fused(gi, mode):
+    d, lb = MODE_SCHED[mode]
+    return gi.Schedule(direction=d, load_balancing=lb, kernel_fusion=1,
+                       pull_frontier_rep=gi.PULL_BITMAP, frontier_creation=gi.FC_BITMAP)
+
+
+@pytest.fixture(scope="module")
+def c1(gi):
+    g = gi.Graph.rmat(16, 16, SEED, True)
+    off, nb, _ = g.read_csr()
+    host = O.CSR(off, nb)
+    return g, host, O.bfs_depth(host, 0)
+
+
+@pytest.mark.parametrize("mode", range(7))
+def test_every_fused_mode_bit_exact(gi, c1, mode):
+    g, host, want = c1
+    s = fused(gi, mode)
+    depth, parent, st = gi.bfs(g, 0, gi.HybridSchedule(s, s, selector=gi.SELECT_S1),
+                               want_parent=True)
+    assert np.array_equal(depth, want)
+    assert O.bfs_parent_violations(host, 0, depth, parent) == 0
+    assert st["iterations"] == want.max() + 1
+
+
+@pytest.mark.parametrize("t", [0.0, 0.01, 0.1, 2.0])
+def test_fused_hybrid_rule(gi, c1, t):
+    g, host, want = c1
+    for src in [0] + list(O.pick_sources(host, 9, 3)):
+        h = gi.HybridSchedule(fused(gi, 4), fused(gi, 5), t, gi.SELECT_HEURISTIC)
+        depth, _, _ = gi.bfs(g, int(src), h)
+        assert np.array_equal(depth, O.bfs_depth(host, int(src)))
+
+
+def test_fused_hub_bin_on_wide_star(gi):
+    # centre 0 with 20000 leaves and a tail: the centre exceeds the hub degree
+    n = 20003
+    e = [(0, i) for i in range(1, 20001)] + [(20000, 20001), (20001, 20002)]
+    e = np.array(e, np.int32)
+    g = gi.Graph.from_edges(n, e[:, 0], e[:, 1], True)
+    host = O.build_csr(n, e[:, 0], e[:, 1], True)
+    for mode in (0, 1, 2, 3, 4):
+        s = fused(gi, mode)
+        depth, _, _ = gi.bfs(g, 0, gi.HybridSchedule(s, s, selector=gi.SELECT_S1))
+        assert np.array_equal(depth, O.bfs_depth(host, 0)), mode
+
+
+def test_fused_grid_graph_many_levels(gi):
+    g = gi.Graph.grid(200, 0, 1)
+    off, nb, _ = g.read_csr()
+    host = O.CSR(off, nb)
+    h = gi.HybridSchedule(fused(gi, 4), fused(gi, 5), 0.05, gi.SELECT_HEURISTIC)
+    depth, _, st = gi.bfs(g, 0, h)
+    assert np.array_equal(depth, O.bfs_depth(host, 0)) and st["iterations"] == 399
+
+
+def test_sweep_records_every_mode(gi, c1):
+    g, host, want = c1
+    feats, times = gi.bfs_sweep(g, 0, reps=1)
+    assert feats.shape[0] == want.max() + 1
+    assert (times > 0).all()
+    assert np.allclose(feats[:, 15], 1.0)  # bias feature
+
+
+def _init_params(rng, n_out):
+    p = [rng.normal(size=(16, 64)) * 0.25, np.zeros(64), rng.normal(size=(64, 64)) * 0.125,
+         np.zeros(64), rng.normal(size=(64, n_out)) * 0.125, np.zeros(n_out)]
+    return p
+
+
+def _flat(p):
+    return np.concatenate([a.reshape(-1) for a in p]).astype(np.float32)
+
+
+def _unflat(v, n_out):
+    shapes = [(16, 64), (64,), (64, 64), (64,), (64, n_out), (n_out,)]
+    out, i = [], 0
+    for s in shapes:
+        k = int(np.prod(s))
+        out.append(v[i:i + k].reshape(s).astype(np.float64))
+        i += k
+    return out
+
+
+def test_selector_gradients_match_fp64_oracle(gi):
+    rng = np.random.default_rng(3)
+    n_out = 7
+    p = _init_params(rng, n_out)
+    p[1] = rng.normal(size=64) * 0.1
+    p[3] = rng.normal(size=64) * 0.1
+    X = rng.normal(size=(300, 16))
+    T = rng.normal(size=(300, n_out))
+    M = (rng.random((300, n_out)) > 0.2).astype(np.float64)
+    loss, g = gi.selector_loss_grad(X, T, M, _flat(p), n_out)
+    want_loss, want = O.mlp_loss_backward(X, T, M, *p)
+    assert abs(loss - want_loss) <= 1e-4 * abs(want_loss)
+    got = _unflat(g, n_out)
+    for a, b in zip(got, want):
+        assert np.allclose(a, b, rtol=2e-3, atol=2e-5 * np.abs(b).max())
+
+
+def test_selector_training_reduces_loss(gi):
+    rng = np.random.default_rng(4)
+    n_out = 3
+    X = rng.normal(size=(256, 16))
+    W = rng.normal(size=(16, n_out))
+    T = np.tanh(X @ W)
+    M = np.ones_like(T)
+    p0 = _flat(_init_params(rng, n_out))
+    p, hist = gi.selector_train(X, T, M, p0, n_out, 400, 3e-3)
+    assert hist[-1] < 0.2 * hist[0]
+    loss, _ = gi.selector_loss_grad(X, T, M, p, n_out)
+    want, _ = O.mlp_loss_backward(X, T, M, *_unflat(p, n_out))
+    assert abs(loss - want) <= 1e-3 * max(want, 1e-6) + 1e-6
+
+
+def test_learned_selector_outputs_stay_exact(gi, c1):
+    g, host, want = c1
+    rng = np.random.default_rng(5)
+    p = _flat(_init_params(rng, 7))
+    gi.selector_load(p, 7, 0x7f, np.zeros(16), np.ones(16))
+    try:
+        h = gi.HybridSchedule(fused(gi, 4), fused(gi, 5), 0.05, gi.SELECT_LEARNED)
+        for src in [0, 17, 999]:
+            depth, _, _ = gi.bfs(g, src, h)
+            assert np.array_equal(depth, O.bfs_depth(host, src))
+    finally:
+        gi.selector_clear()
