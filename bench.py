@@ -35,20 +35,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def schedule(gi, threshold: float, push_lb: int, pull_lb: int):
+def schedule(gi, threshold: float, push_lb: int, pull_lb: int, fused: bool):
+    f = 1 if fused else 0
     s1 = gi.Schedule(direction=gi.PUSH, load_balancing=push_lb, frontier_creation=gi.FC_SPARSE,
-                     cta_size=256)
+                     cta_size=256, kernel_fusion=f)
     s2 = gi.Schedule(direction=gi.PULL, load_balancing=pull_lb, pull_frontier_rep=gi.PULL_BITMAP,
-                     frontier_creation=gi.FC_BITMAP, cta_size=256)
+                     frontier_creation=gi.FC_BITMAP, cta_size=256, kernel_fusion=f)
     return gi.HybridSchedule(s1, s2, threshold, gi.SELECT_HEURISTIC)
 
 
 def roofline(gi, g, sched, sources, depth):
     """Per-kernel (direction, load balancing) totals over profiled runs."""
     kern = {}
-    for src in sources:
-        for r in gi.bfs_profile(g, int(src), sched, depth):
-            key = ("pull" if r["direction"] == gi.PULL else "push", gi.LB_NAMES[r["load_balancing"]])
+    for i, src in enumerate(sources):
+        recs = gi.bfs_profile(g, int(src), sched, depth)
+        if i == 0:
+            for r in recs:
+                log(f"  level {r['level']}: dir={r['direction']} lb={r['load_balancing']} "
+                    f"|F|={r['frontier_size']} |F'|={r['next_size']} {r['kernel_ms']:.3f} ms "
+                    f"{r['alg_bytes'] / max(r['kernel_ms'], 1e-9) / 1e6:.1f} GB/s")
+        for r in recs:
+            lb = r["load_balancing"]
+            name = gi.BFS_MODE_NAMES[lb - 100] if lb >= 100 else gi.LB_NAMES[lb]
+            key = ("pull" if r["direction"] == gi.PULL else "push", name)
             k = kern.setdefault(key, {"ms": 0.0, "bytes": 0.0, "launches": 0})
             k["ms"] += r["kernel_ms"]
             k["bytes"] += r["alg_bytes"]
@@ -93,7 +102,8 @@ def main():
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--threshold", type=float, default=0.02)
-    ap.add_argument("--push-lb", type=int, default=2)
+    ap.add_argument("--engine", choices=["fused", "unfused"], default="fused")
+    ap.add_argument("--push-lb", type=int, default=1)
     ap.add_argument("--pull-lb", type=int, default=0)
     ap.add_argument("--profile-sources", type=int, default=4)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
@@ -114,14 +124,17 @@ def main():
     V, E = g.num_vertices, g.num_edges
     log(f"graph RMAT-{args.scale} sym: V={V} E={E} built in {time.perf_counter() - t0:.2f}s")
     sources = g.pick_sources(SEED, 64)
-    sched = schedule(gi, args.threshold, args.push_lb, args.pull_lb)
+    sched = schedule(gi, args.threshold, args.push_lb, args.pull_lb, args.engine == "fused")
     depth = gi.DeviceArray(V, np.int32)
 
     # untimed: reached edges per source
     edges_of = {}
-    for src in sources:
+    t0 = time.perf_counter()
+    for i, src in enumerate(sources):
         gi.bfs_device(g, int(src), sched, depth)
         edges_of[int(src)] = g.reached_edges(depth)
+        if i % 16 == 15:
+            log(f"reference runs {i + 1}/64 ({time.perf_counter() - t0:.1f}s)")
     for k in range(args.warmup):
         gi.bfs_device(g, int(sources[k % 64]), sched, depth)
     gi.synchronize()
@@ -161,7 +174,7 @@ def main():
                                f"edge factor {args.edge_factor}, 64 GAPBS-style sources",
                    "vertices": V, "edge_entries": E, "schedule": {
                        "s1": repr(sched.s1), "s2": repr(sched.s2), "threshold": args.threshold,
-                       "selector": "heuristic"}},
+                       "selector": "heuristic", "engine": args.engine}},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

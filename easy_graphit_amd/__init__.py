@@ -512,11 +512,15 @@ def bfs(g: Graph, source: int, schedule: HybridSchedule | None = None,
     return depth.numpy(stream), (parent.numpy(stream) if parent else None), st.as_dict()
 
 
-def bfs_device(g: Graph, source: int, schedule, depth: DeviceArray, parent=None, stream=None):
-    st = RunStats()
+def bfs_device(g: Graph, source: int, schedule, depth: DeviceArray, parent=None, stream=None,
+               stats: bool = False):
+    """BFS into caller-owned device arrays; with stats=False nothing but the
+    traversal runs (the form timed by the benchmark)."""
+    st = RunStats() if stats else None
     check(lib().gi_bfs(g.h, source, C.byref(schedule) if schedule is not None else None,
-                       depth.ptr, parent.ptr if parent else None, C.byref(st), stream))
-    return st.as_dict()
+                       depth.ptr, parent.ptr if parent else None,
+                       C.byref(st) if st is not None else None, stream))
+    return st.as_dict() if st is not None else None
 
 
 def bfs_profile(g: Graph, source: int, schedule, depth: DeviceArray, stream=None):

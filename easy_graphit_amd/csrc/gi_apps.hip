@@ -143,10 +143,24 @@ static int bfs_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int
         if (h->selector == GI_SELECT_LEARNED) sel = selector_view();  // empty: heuristic fallback
         if (!sel.w1) sel.mode = h->selector == GI_SELECT_LEARNED ? GI_SELECT_HEURISTIC : h->selector;
         FusedTrace ft;
-        EventTimer timer(s);
+        bool want_trace = trace != nullptr || stats != nullptr;
+        hipEvent_t t0 = nullptr, t1 = nullptr;
+        if (stats) {
+            GI_HIP(hipEventCreate(&t0));
+            GI_HIP(hipEventCreate(&t1));
+            GI_HIP(hipEventRecord(t0, s));
+        }
         int32_t levels = 0;
-        int rc = bfs_fused_run(g, source, h, sel, d_depth, d_parent, s, &ft, trace != nullptr, &levels);
-        double ms = timer.stop();
+        int rc = bfs_fused_run(g, source, h, sel, d_depth, d_parent, s, want_trace ? &ft : nullptr,
+                               trace != nullptr, &levels);
+        float ms = 0.f;
+        if (stats && rc == GI_OK) {
+            GI_HIP(hipEventRecord(t1, s));
+            GI_HIP(hipEventSynchronize(t1));
+            GI_HIP(hipEventElapsedTime(&ms, t0, t1));
+        }
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
         if (rc == GI_OK && trace) {
             for (int32_t i = 0; i < levels && trace->n < trace->max; ++i) {
                 gi_level_record &r = trace->recs[trace->n++];
@@ -167,13 +181,7 @@ static int bfs_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int
             for (int32_t m : ft.modes) (m >= BFS_PULL_THREAD ? stats->pull_iterations : stats->push_iterations)++;
             stats->ms = ms;
             stats->edges_traversed = 0;
-            int32_t *cnt = nullptr;
-            GI_TRY(dalloc(&cnt, 1));
-            fill_i32(cnt, 1, 0, s);
-            count_reached_kernel<<<grid_for(g->V), 256, 0, s>>>(d_depth, g->V, cnt);
-            int rc2 = read_count(cnt, &stats->reached, s);
-            dfree(cnt);
-            GI_TRY(rc2);
+            GI_TRY(fused_reached(g, d_depth, &stats->reached, s));
         }
         return GI_OK;
     }
@@ -616,7 +624,30 @@ int gi_bc(gi_graph *g, const int32_t *h_sources, int32_t num_sources, int32_t ba
     if (!g || !d_scores || (num_sources > 0 && !h_sources)) return fail(GI_ERR_INVALID, "null argument");
     for (int32_t i = 0; i < num_sources; ++i)
         if (h_sources[i] < 0 || h_sources[i] >= g->V) return fail(GI_ERR_RANGE, "source %d out of range", h_sources[i]);
-    (void)batch;
+    if (batch > 1) {
+        // batched: groups of up to 256 sources advance level-synchronously
+        hipStream_t bs = (hipStream_t)stream;
+        if (batch > 256) batch = 256;
+        EventTimer timer(bs);
+        GI_TRY(fill_f32(d_scores, g->V, 0.f, bs));
+        int32_t total_levels = 0;
+        for (int32_t i = 0; i < num_sources; i += batch) {
+            int32_t nb = num_sources - i < batch ? num_sources - i : batch;
+            int32_t lv = 0;
+            GI_TRY(bc_batch(g, h_sources + i, nb, d_scores, bs, &lv));
+            total_levels += lv;
+        }
+        double ms = timer.stop();
+        if (stats) {
+            stats->iterations = num_sources;
+            stats->push_iterations = total_levels;
+            stats->pull_iterations = 0;
+            stats->ms = ms;
+            stats->reached = 0;
+            stats->edges_traversed = 0;
+        }
+        return GI_OK;
+    }
     hipStream_t s = (hipStream_t)stream;
     gi_schedule sched;  // apps/bc.cpp:235-237
     gi_schedule_init(&sched);

@@ -24,10 +24,12 @@ constexpr int kSelInputs = 16;
 constexpr int kSelHidden = 64;
 constexpr int kSelMaxOut = 8;
 constexpr int32_t kHubDegree = 4096;  // push vertices at least this wide go to the grid bin
-constexpr int32_t kMaxHubs = 8192;
+constexpr int32_t kMaxHubs = 1024;
+constexpr int32_t kMaxGrid = 8192;          // partial-sum slots per kernel
+constexpr int32_t kMaxSumWords = 1024;      // summary words (V up to 2^28)
 
 struct BfsState {
-    int32_t count[2];               // sparse frontier sizes by level parity
+    int32_t count[2];               // frontier sizes by level parity
     unsigned long long fdeg[2];     // sum of frontier degrees by parity
     int32_t done;
     int32_t level;                  // level of the frontier the next step expands
@@ -36,7 +38,7 @@ struct BfsState {
     long long unvisited;            // vertices not reached yet
     long long unvisited_edges;      // sum of their degrees
     int32_t hub_count;              // deferred push vertices of the current step
-    int32_t pad;
+    int32_t cursor;                 // dynamic chunk assignment of push steps
     int32_t hubs[kMaxHubs];
     int32_t hist_mode[kBfsMaxLevels];
     int32_t hist_next[kBfsMaxLevels];
@@ -100,6 +102,7 @@ int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, Sele
                   int32_t *d_depth, int32_t *d_parent, hipStream_t s, FusedTrace *trace,
                   bool timed, int32_t *levels_out);
 void release_fused(const gi_graph *g);
+int fused_reached(gi_graph *g, const int32_t *depth, int32_t *out, hipStream_t s);
 // The process-wide selector model (null weights when none is loaded).
 SelectorView selector_view();
 

@@ -8,11 +8,20 @@
 // stream: a persistent step kernel that switches on the chosen mode, a hub
 // kernel that spreads the edges of very wide push vertices over the whole
 // grid (a no-op on most levels), and a one-wave decide kernel (GraphIt's
-// hybrid rule or the MLP selector, from device counters). Every step writes
-// the next frontier in both representations (sparse queue and 64-bit-word
-// bitmap), so a direction change needs no conversion pass, and keeps the
-// counts (|F|, sum of frontier degrees, unvisited vertices and their
-// degrees) that feed the decision.
+// hybrid rule or the MLP selector, from device counters).
+//
+// Frontiers are bitmaps only (one 64-bit word per wavefront of vertices),
+// each with a summary bitmap of its non-empty 64-word chunks:
+//   * a step claims discoveries in the OUTPUT bitmap (one atomicOr per new
+//     vertex) and filters against visited | input; the input bitmap is folded
+//     into `visited` by plain stores during the same step (the thread that
+//     folds word w is its only writer), so no vertex can be claimed twice;
+//   * push steps visit only the non-empty chunks of the input (from its
+//     summary), handed to workgroups round-robin, and list their vertices in
+//     LDS — no global queue, no shared cursor;
+//   * |F'| and the sum of its degrees are reduced per workgroup into a slot
+//     of a partial-sum array (plain stores) that the decide kernel adds up,
+//     so no counter is shared by the whole grid.
 #include <vector>
 
 #include "gi_bfs_fused.h"
@@ -24,204 +33,297 @@ __device__ __forceinline__ int32_t deg_of(const int32_t *off, int32_t v) {
     return off[v + 1] - off[v];
 }
 
+constexpr int kStepBlock = 256;
+constexpr int kChunkWords = 64;                  // chunk: 64 words = 4096 vertices
+constexpr int kChunkVerts = kChunkWords * 64;
+
 struct StepArgs {
     BfsState *st;
     const int32_t *off;
     const int32_t *nbr;
     int32_t V;
     int32_t words;
+    int32_t chunks;
+    int32_t sum_words;
     int32_t *depth;
     int32_t *parent;
     uint64_t *visited;
     uint64_t *bm[3];
-    int32_t *q[2];
+    uint64_t *sum[3];
+    int32_t *pcount;            // [2][kMaxGrid]: step slots, then hub slots
+    unsigned long long *pfdeg;  // same layout
 };
 
 struct StepCtx {
     int32_t level;
-    int32_t *qout;
-    int32_t *count_out;
+    const uint64_t *bm_in;
     uint64_t *bm_out;
-    unsigned long long *fdeg_out;
+    uint64_t *sum_out;
+    const uint64_t *visited;
+    int32_t cnt;               // per-lane |F'| contribution
+    unsigned long long fdeg;   // per-lane sum of degrees of F'
 };
 
-// Push discovery: claim the visited bit, record depth (and parent), append
-// to the sparse queue, set the bitmap bit, accumulate |F'| and the degree
-// sum. Every active lane of the wave calls it once per step.
-__device__ __forceinline__ void push_edge(const StepArgs &a, const StepCtx &c, int32_t s,
-                                          int32_t e, bool active) {
-    int32_t d = 0;
-    bool fresh = false;
-    if (active) {
-        d = a.nbr[e];
-        if (!bit_test(a.visited, d) && bit_claim(a.visited, d)) {
-            fresh = true;
-            a.depth[d] = c.level + 1;
-            if (a.parent) a.parent[d] = s;
-            atomicOr((unsigned long long *)&c.bm_out[d >> 6], 1ull << (d & 63));
-        }
-    }
-    uint64_t m = __ballot(fresh);
-    if (m == 0) return;
-    int32_t pos = wave_atomic_add(c.count_out, fresh);
-    if (fresh) c.qout[pos] = d;
-    unsigned long long dg = fresh ? (unsigned long long)deg_of(a.off, d) : 0ull;
-    for (int o = 32; o > 0; o >>= 1) dg += __shfl_xor(dg, o);
-    if (lane_id() == 0) atomicAdd(c.fdeg_out, dg);
+__device__ __forceinline__ bool seen(const StepCtx &c, int32_t v) {
+    uint64_t bit = 1ull << (v & 63);
+    return ((c.visited[v >> 6] | c.bm_in[v >> 6]) & bit) != 0;
 }
 
-// Lane-per-vertex edge walk; the wave loops while any lane has edges left.
-__device__ __forceinline__ void lane_walk(const StepArgs &a, const StepCtx &c, int32_t s,
-                                          int32_t e, int32_t e1) {
-    while (__ballot(e < e1)) {
-        bool live = e < e1;
-        push_edge(a, c, s, e, live);
-        e += live ? 1 : 0;
-    }
+__device__ __forceinline__ void mark_chunk(uint64_t *sum, int32_t word) {
+    int32_t ch = word / kChunkWords;
+    uint64_t bit = 1ull << (ch & 63);
+    if (!(sum[ch >> 6] & bit)) atomicOr((unsigned long long *)&sum[ch >> 6], bit);
 }
 
-__device__ __forceinline__ void wave_walk(const StepArgs &a, const StepCtx &c, int32_t s,
-                                          int32_t e0, int32_t e1) {
+// Push discovery: filter, claim in the output bitmap, record.
+__device__ __forceinline__ void push_edge(const StepArgs &a, StepCtx &c, int32_t s, int32_t e,
+                                          bool active) {
+    if (!active) return;
+    int32_t d = a.nbr[e];
+    if (seen(c, d)) return;
+    uint64_t bit = 1ull << (d & 63);
+    unsigned long long old = atomicOr((unsigned long long *)&c.bm_out[d >> 6], bit);
+    if (old & bit) return;
+    if (old == 0) mark_chunk(c.sum_out, d >> 6);
+    a.depth[d] = c.level + 1;
+    if (a.parent) a.parent[d] = s;
+    c.cnt += 1;
+    c.fdeg += (unsigned long long)deg_of(a.off, d);
+}
+
+__device__ __forceinline__ void lane_walk(const StepArgs &a, StepCtx &c, int32_t s, int32_t e,
+                                          int32_t e1) {
+    for (; e < e1; ++e) push_edge(a, c, s, e, true);
+}
+
+__device__ __forceinline__ void wave_walk(const StepArgs &a, StepCtx &c, int32_t s, int32_t e0,
+                                          int32_t e1) {
     int32_t lane = lane_id();
     for (int32_t b = e0; b < e1; b += 64) push_edge(a, c, s, b + lane, b + lane < e1);
 }
 
-// ---------------------------------------------------------------- step kernel
-constexpr int kStepBlock = 256;
+// Workgroup reduction of the lane counters into this block's slot.
+__device__ __forceinline__ void flush_counts(int32_t *pcount, unsigned long long *pfdeg,
+                                             StepCtx &c) {
+    __shared__ int32_t wc[kStepBlock / 64];
+    __shared__ unsigned long long wf[kStepBlock / 64];
+    int32_t cnt = c.cnt;
+    unsigned long long fd = c.fdeg;
+    for (int o = 32; o > 0; o >>= 1) {
+        cnt += __shfl_xor(cnt, o);
+        fd += __shfl_xor(fd, o);
+    }
+    if (lane_id() == 0) {
+        wc[threadIdx.x >> 6] = cnt;
+        wf[threadIdx.x >> 6] = fd;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t tc = 0;
+        unsigned long long tf = 0;
+        for (int i = 0; i < kStepBlock / 64; ++i) {
+            tc += wc[i];
+            tf += wf[i];
+        }
+        pcount[blockIdx.x] = tc;
+        pfdeg[blockIdx.x] = tf;
+    }
+}
 
+// Prefix of set bits over the summary words (LDS), for k-th-chunk lookup.
+// Returns the number of non-empty chunks. Every thread of the block calls it.
+__device__ int32_t summary_prefix(const uint64_t *sum, int32_t sum_words, int32_t *pre) {
+    __shared__ int32_t total_s;
+    // one wave scans 64 words at a time
+    if (threadIdx.x < 64) {
+        int32_t carry = 0;
+        int32_t lane = lane_id();
+        for (int32_t base = 0; base < sum_words; base += 64) {
+            int32_t j = base + lane;
+            int32_t pc = j < sum_words ? __popcll(sum[j]) : 0;
+            int32_t incl = pc;
+            for (int o = 1; o < 64; o <<= 1) {
+                int32_t y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            if (j < sum_words) pre[j] = carry + incl - pc;
+            carry += __shfl(incl, 63);
+        }
+        if (lane == 0) total_s = carry;
+    }
+    __syncthreads();
+    return total_s;
+}
+
+// Chunk id of the k-th non-empty chunk.
+__device__ __forceinline__ int32_t kth_chunk(const uint64_t *sum, const int32_t *pre,
+                                             int32_t sum_words, int32_t k) {
+    int32_t j = upper_bound_i32(pre, 0, sum_words, k) - 1;
+    uint64_t w = sum[j];
+    for (int32_t r = k - pre[j]; r > 0; --r) w &= w - 1;
+    return j * 64 + (__ffsll((unsigned long long)w) - 1);
+}
+
+// One batch (<= kStepBlock vertices of the block's LDS list) in push mode.
+__device__ void push_batch(const StepArgs &a, StepCtx &c, int32_t mode, const int32_t *list,
+                           int32_t nb) {
+    __shared__ int32_t soff[kStepBlock], sscan[kStepBlock + 1];
+    const int32_t tid = threadIdx.x, wid = tid >> 6;
+    const bool has = tid < nb;
+    const int32_t s = has ? list[tid] : 0;
+    const int32_t e0 = has ? a.off[s] : 0, e1 = has ? a.off[s + 1] : 0;
+    const int32_t dg = e1 - e0;
+    if (mode == BFS_PUSH_THREAD) {
+        lane_walk(a, c, s, e0, e1);
+    } else if (mode == BFS_PUSH_WAVE) {
+        for (int32_t j = wid; j < nb; j += kStepBlock / 64) {
+            int32_t v = list[j];
+            wave_walk(a, c, v, a.off[v], a.off[v + 1]);
+        }
+    } else if (mode == BFS_PUSH_TWC) {
+        // lane bin (< 64), wave bin (< kHubDegree), grid bin (deferred to the hub kernel)
+        bool hub = has && dg >= kHubDegree;
+        int32_t hpos = wave_atomic_add(&a.st->hub_count, hub);
+        if (hub && hpos < kMaxHubs) a.st->hubs[hpos] = s;
+        bool wide = has && dg >= 64 && (!hub || hpos >= kMaxHubs);
+        uint64_t wm = __ballot(wide);
+        while (wm) {
+            int32_t l = __ffsll((unsigned long long)wm) - 1;
+            wm &= wm - 1;
+            int32_t ws = __shfl(s, l), w0 = __shfl(e0, l), w1 = __shfl(e1, l);
+            wave_walk(a, c, ws, w0, w1);
+        }
+        if (has && dg < 64) lane_walk(a, c, s, e0, e1);
+    } else {
+        // TWCE and CM need the workgroup's degree scan
+        int32_t total = 0;
+        int32_t ex = block_exclusive_scan(dg, &total);
+        sscan[tid] = ex;
+        soff[tid] = e0;
+        __syncthreads();
+        if (mode == BFS_PUSH_CM) {
+            for (int32_t k = tid; k - tid < total; k += kStepBlock) {
+                bool act = k < total;
+                int32_t v = 0, e = 0;
+                if (act) {
+                    int32_t j = upper_bound_i32(sscan, 0, nb, k) - 1;
+                    v = list[j];
+                    e = soff[j] + (k - sscan[j]);
+                }
+                push_edge(a, c, v, e, act);
+            }
+        } else {
+            // TWCE: workgroup granule (degree >= block), wave granule (>= 64), lanes
+            for (int32_t j = 0; j < nb; ++j) {
+                int32_t dj = (j + 1 < nb ? sscan[j + 1] : total) - sscan[j];
+                if (dj < kStepBlock) continue;  // uniform
+                int32_t v = list[j], o = soff[j];
+                for (int32_t b = 0; b < dj; b += kStepBlock) push_edge(a, c, v, o + b + tid, b + tid < dj);
+            }
+            for (int32_t j = wid; j < nb; j += kStepBlock / 64) {
+                int32_t dj = (j + 1 < nb ? sscan[j + 1] : total) - sscan[j];
+                if (dj < 64 || dj >= kStepBlock) continue;  // wave-uniform
+                wave_walk(a, c, list[j], soff[j], soff[j] + dj);
+            }
+            if (has && dg < 64) lane_walk(a, c, s, e0, e1);
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------- step kernel
 __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a) {
     BfsState *st = a.st;
     if (st->done) return;
+    __shared__ int32_t pre[kMaxSumWords + 1];
     StepCtx c;
     c.level = st->level;
     const int32_t mode = st->mode;
-    const int32_t p = c.level & 1;
-    const int32_t *qin = a.q[p];
-    c.qout = a.q[p ^ 1];
-    const int32_t n = st->count[p];
-    c.count_out = &st->count[p ^ 1];
-    const uint64_t *bm_in = a.bm[c.level % 3];
-    c.bm_out = a.bm[(c.level + 1) % 3];
-    uint64_t *bm_clear = a.bm[(c.level + 2) % 3];
-    c.fdeg_out = &st->fdeg[p ^ 1];
+    const int32_t iin = c.level % 3, iout = (c.level + 1) % 3, iclr = (c.level + 2) % 3;
+    c.bm_in = a.bm[iin];
+    c.bm_out = a.bm[iout];
+    c.sum_out = a.sum[iout];
+    c.visited = a.visited;
+    c.cnt = 0;
+    c.fdeg = 0;
     const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t threads = gridDim.x * blockDim.x;
     const int32_t lane = lane_id();
 
-    // the bitmap two levels ahead is free: clear it for the next step
-    for (int32_t w = tid; w < a.words; w += threads) bm_clear[w] = 0;
+    // the bitmap two levels ahead is free: clear its non-empty chunks (its
+    // summary is cleared by the decide kernel after this step)
+    {
+        const uint64_t *sc = a.sum[iclr];
+        uint64_t *bc = a.bm[iclr];
+        int32_t nz = summary_prefix(sc, a.sum_words, pre);
+        for (int32_t k = blockIdx.x; k < nz; k += gridDim.x) {
+            int32_t ch = kth_chunk(sc, pre, a.sum_words, k);
+            if (threadIdx.x < kChunkWords) {
+                int32_t w = ch * kChunkWords + (int32_t)threadIdx.x;
+                if (w < a.words) bc[w] = 0;
+            }
+        }
+        __syncthreads();
+    }
 
-    if (mode == BFS_PUSH_THREAD) {
-        for (int32_t i = tid; i - lane < n; i += threads) {
-            bool act = i < n;
-            int32_t s = act ? qin[i] : 0;
-            lane_walk(a, c, s, act ? a.off[s] : 0, act ? a.off[s + 1] : 0);
-        }
-    } else if (mode == BFS_PUSH_WAVE) {
-        for (int32_t i = tid >> 6; i < n; i += threads >> 6) {
-            int32_t s = qin[i];
-            wave_walk(a, c, s, a.off[s], a.off[s + 1]);
-        }
-    } else if (mode == BFS_PUSH_TWC) {
-        // lane bin (< 64), wave bin (< kHubDegree), grid bin (deferred)
-        for (int32_t i = tid; i - lane < n; i += threads) {
-            bool act = i < n;
-            int32_t s = act ? qin[i] : 0;
-            int32_t e0 = act ? a.off[s] : 0, e1 = act ? a.off[s + 1] : 0;
-            int32_t dg = e1 - e0;
-            bool hub = act && dg >= kHubDegree;
-            int32_t hpos = wave_atomic_add(&st->hub_count, hub);
-            if (hub && hpos < kMaxHubs) st->hubs[hpos] = s;
-            bool wide = act && dg >= 64 && (!hub || hpos >= kMaxHubs);
-            uint64_t wm = __ballot(wide);
-            while (wm) {  // the wave walks each wide vertex together
-                int32_t l = __ffsll((unsigned long long)wm) - 1;
-                wm &= wm - 1;
-                int32_t ws = __shfl(s, l), w0 = __shfl(e0, l), w1 = __shfl(e1, l);
-                wave_walk(a, c, ws, w0, w1);
+    if (mode < BFS_PULL_THREAD) {
+        __shared__ int32_t list[kChunkVerts];
+        __shared__ int32_t n_s;
+        const uint64_t *si = a.sum[iin];
+        int32_t nz = summary_prefix(si, a.sum_words, pre);
+        for (int32_t k = blockIdx.x; k < nz; k += gridDim.x) {
+            int32_t chunk = kth_chunk(si, pre, a.sum_words, k);
+            if (threadIdx.x < 64) {
+                // wave 0 folds the chunk into `visited` and lists its members
+                int32_t w = chunk * kChunkWords + (int32_t)threadIdx.x;
+                uint64_t word = w < a.words ? c.bm_in[w] : 0ull;
+                if (word) a.visited[w] |= word;  // only this thread writes word w
+                int32_t pc = __popcll(word);
+                int32_t incl = pc;
+                for (int o = 1; o < 64; o <<= 1) {
+                    int32_t y = __shfl_up(incl, o);
+                    if (lane >= o) incl += y;
+                }
+                int32_t pos = incl - pc;
+                while (word) {
+                    int32_t b = __ffsll((unsigned long long)word) - 1;
+                    word &= word - 1;
+                    list[pos++] = w * 64 + b;
+                }
+                if (threadIdx.x == 63) n_s = incl;
             }
-            bool small = act && dg < 64;
-            lane_walk(a, c, s, small ? e0 : 0, small ? e1 : 0);
-        }
-    } else if (mode == BFS_PUSH_TWCE || mode == BFS_PUSH_CM) {
-        __shared__ int32_t sv[kStepBlock], soff[kStepBlock], sscan[kStepBlock + 1];
-        for (int32_t base = blockIdx.x * kStepBlock; base < n; base += gridDim.x * kStepBlock) {
-            int32_t i = base + (int32_t)threadIdx.x;
-            int32_t dg = 0;
-            if (i < n) {
-                int32_t s = qin[i];
-                int32_t o0 = a.off[s];
-                dg = a.off[s + 1] - o0;
-                sv[threadIdx.x] = s;
-                soff[threadIdx.x] = o0;
-            }
-            int32_t total = 0;
-            int32_t ex = block_exclusive_scan(dg, &total);
-            sscan[threadIdx.x] = ex;
-            int32_t width = n - base < kStepBlock ? n - base : kStepBlock;
             __syncthreads();
-            if (mode == BFS_PUSH_CM) {
-                // merge-path over the workgroup's concatenated edge lists
-                for (int32_t k = threadIdx.x; k - (int32_t)threadIdx.x < total; k += kStepBlock) {
-                    bool act = k < total;
-                    int32_t s = 0, e = 0;
-                    if (act) {
-                        int32_t j = upper_bound_i32(sscan, 0, width, k) - 1;
-                        s = sv[j];
-                        e = soff[j] + (k - sscan[j]);
-                    }
-                    push_edge(a, c, s, e, act);
-                }
-            } else {
-                // TWCE: workgroup granule for degree >= block, wave granule
-                // for >= 64, lanes for the rest
-                int32_t wid = threadIdx.x >> 6;
-                for (int32_t j = 0; j < width; ++j) {  // workgroup granule (uniform)
-                    int32_t dj = (j + 1 < width ? sscan[j + 1] : total) - sscan[j];
-                    if (dj < kStepBlock) continue;
-                    int32_t s = sv[j], o = soff[j];
-                    for (int32_t b = 0; b < dj; b += kStepBlock) {
-                        int32_t k = b + (int32_t)threadIdx.x;
-                        push_edge(a, c, s, o + k, k < dj);
-                    }
-                }
-                for (int32_t j = wid; j < width; j += kStepBlock / 64) {  // wave granule
-                    int32_t dj = (j + 1 < width ? sscan[j + 1] : total) - sscan[j];
-                    if (dj < 64 || dj >= kStepBlock) continue;
-                    wave_walk(a, c, sv[j], soff[j], soff[j] + dj);
-                }
-                bool mine = (int32_t)threadIdx.x < width && dg > 0 && dg < 64;
-                int32_t o = mine ? soff[threadIdx.x] : 0;
-                lane_walk(a, c, mine ? sv[threadIdx.x] : 0, o, mine ? o + dg : 0);
-            }
+            const int32_t n = n_s;
+            for (int32_t b0 = 0; b0 < n; b0 += kStepBlock)
+                push_batch(a, c, mode, list + b0, n - b0 < kStepBlock ? n - b0 : kStepBlock);
             __syncthreads();
         }
     } else {
-        // pull over all vertices; waves own 64-vertex words of the bitmaps
-        unsigned long long fdeg_acc = 0;
+        // pull over all vertices; waves own 64-vertex words of every bitmap
         for (int32_t w = tid >> 6; w < a.words; w += threads >> 6) {
-            uint64_t vis = a.visited[w];
+            uint64_t in = c.bm_in[w];
+            uint64_t vis = a.visited[w] | in;
+            if (in && lane == 0) a.visited[w] = vis;  // fold the input frontier
             if (vis == ~0ull) continue;  // wave-uniform
             uint64_t found = 0;
             if (mode == BFS_PULL_THREAD) {
                 int32_t v = w * 64 + lane;
                 bool want = v < a.V && !((vis >> lane) & 1ull);
                 int32_t e = want ? a.off[v] : 0, e1 = want ? a.off[v + 1] : 0;
-                bool hit = false;
                 int32_t par = -1;
                 for (; e < e1; ++e) {
                     int32_t u = a.nbr[e];
-                    if ((bm_in[u >> 6] >> (u & 63)) & 1ull) {
-                        hit = true;
+                    if ((c.bm_in[u >> 6] >> (u & 63)) & 1ull) {
                         par = u;
                         break;
                     }
                 }
+                bool hit = par >= 0;
                 found = __ballot(hit);
                 if (hit) {
                     a.depth[v] = c.level + 1;
                     if (a.parent) a.parent[v] = par;
-                    fdeg_acc += (unsigned long long)(e1 - a.off[v]);
+                    c.fdeg += (unsigned long long)(e1 - a.off[v]);
                 }
             } else {
                 const int32_t sub = lane >> 3, g8 = lane & 7;
@@ -238,7 +340,7 @@ __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a) {
                         int32_t u = 0;
                         if (e < e1) {
                             u = a.nbr[e];
-                            h = (bm_in[u >> 6] >> (u & 63)) & 1ull;
+                            h = (c.bm_in[u >> 6] >> (u & 63)) & 1ull;
                         }
                         uint32_t gm = (uint32_t)(__ballot(h) >> (sub * 8)) & 0xffu;
                         if (gm) {
@@ -255,23 +357,18 @@ __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a) {
                     if (hit) {
                         a.depth[v] = c.level + 1;
                         if (a.parent) a.parent[v] = par;
-                        fdeg_acc += (unsigned long long)(e1 - e0);
+                        c.fdeg += (unsigned long long)(e1 - e0);
                     }
                 }
             }
-            if (found) {
-                if (lane == 0) {
-                    a.visited[w] = vis | found;  // this wave owns word w in pull steps
-                    c.bm_out[w] = found;
-                }
-                bool mine = (found >> lane) & 1ull;
-                int32_t pos = wave_atomic_add(c.count_out, mine);
-                if (mine) c.qout[pos] = w * 64 + lane;
+            if (found && lane == 0) {
+                c.bm_out[w] = found;  // this wave owns word w of the output
+                mark_chunk(c.sum_out, w);
+                c.cnt += __popcll(found);
             }
         }
-        for (int o = 32; o > 0; o >>= 1) fdeg_acc += __shfl_xor(fdeg_acc, o);
-        if (lane == 0 && fdeg_acc) atomicAdd(c.fdeg_out, fdeg_acc);
     }
+    flush_counts(a.pcount, a.pfdeg, c);
 }
 
 // Grid bin of PUSH_TWC: the edges of the deferred hubs, spread evenly over
@@ -281,18 +378,18 @@ __global__ void __launch_bounds__(kStepBlock) bfs_hub_kernel(StepArgs a) {
     BfsState *st = a.st;
     if (st->done) return;
     int32_t nh = st->hub_count;
-    if (nh <= 0) return;
+    if (nh <= 0) return;  // slots stay zero (the decide kernel clears them)
     if (nh > kMaxHubs) nh = kMaxHubs;
     StepCtx c;
     c.level = st->level;
-    const int32_t p = c.level & 1;
-    c.qout = a.q[p ^ 1];
-    c.count_out = &st->count[p ^ 1];
+    c.bm_in = a.bm[c.level % 3];
     c.bm_out = a.bm[(c.level + 1) % 3];
-    c.fdeg_out = &st->fdeg[p ^ 1];
+    c.sum_out = a.sum[(c.level + 1) % 3];
+    c.visited = a.visited;
+    c.cnt = 0;
+    c.fdeg = 0;
     __shared__ int32_t hscan[kMaxHubs + 1];
     __shared__ int32_t hv[kMaxHubs];
-    // serial-chunk scan: each thread sums a contiguous slice, then one scan
     int32_t per = (nh + kStepBlock - 1) / kStepBlock;
     int32_t lo = threadIdx.x * per, hi = lo + per < nh ? lo + per : nh;
     int32_t part = 0;
@@ -310,16 +407,12 @@ __global__ void __launch_bounds__(kStepBlock) bfs_hub_kernel(StepArgs a) {
     __syncthreads();
     int32_t chunk = (total + gridDim.x - 1) / gridDim.x;
     int32_t k0 = blockIdx.x * chunk, k1 = k0 + chunk < total ? k0 + chunk : total;
-    for (int32_t k = k0 + (int32_t)threadIdx.x; k - (int32_t)threadIdx.x < k1; k += kStepBlock) {
-        bool act = k < k1;
-        int32_t s = 0, e = 0;
-        if (act) {
-            int32_t j = upper_bound_i32(hscan, 0, nh, k) - 1;
-            s = hv[j];
-            e = a.off[s] + (k - hscan[j]);
-        }
-        push_edge(a, c, s, e, act);
+    for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += kStepBlock) {
+        int32_t j = upper_bound_i32(hscan, 0, nh, k) - 1;
+        int32_t s = hv[j];
+        push_edge(a, c, s, a.off[s] + (k - hscan[j]), true);
     }
+    flush_counts(a.pcount + kMaxGrid, a.pfdeg + kMaxGrid, c);
 }
 
 // ---------------------------------------------------------------- decide kernel
@@ -327,16 +420,37 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
     return __uint_as_float(((uint32_t)h) << 16);
 }
 
-// One wave, after step `level` (or after seeding, level = -1): closes the
-// step's bookkeeping, ends the run on an empty frontier, else picks the next
-// step's mode and resets its output counters.
-__global__ void bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int64_t E) {
+// One wave, after step `level` (or after seeding, level = -1): adds up the
+// step's partial sums, closes its bookkeeping, ends the run on an empty
+// frontier, else picks the next step's mode and clears what the next step
+// writes.
+__global__ void bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int64_t E,
+                                  int32_t *pcount, unsigned long long *pfdeg, int32_t slots,
+                                  uint64_t *sum_clear, int32_t sum_words) {
     if (st->done) return;
     const int32_t lane = lane_id();
     const int32_t next_level = st->level + 1;
     const int32_t p = next_level & 1;
-    const int32_t n = st->count[p];
-    const unsigned long long fdeg = st->fdeg[p];
+    int32_t n = 0;
+    unsigned long long fdeg = 0;
+    if (next_level == 0) {  // seeded frontier
+        n = st->count[0];
+        fdeg = st->fdeg[0];
+    } else {
+        for (int32_t i = lane; i < slots; i += 64) {
+            n += pcount[i] + pcount[kMaxGrid + i];
+            fdeg += pfdeg[i] + pfdeg[kMaxGrid + i];
+            pcount[i] = 0;
+            pcount[kMaxGrid + i] = 0;
+            pfdeg[i] = 0;
+            pfdeg[kMaxGrid + i] = 0;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            n += __shfl_xor(n, o);
+            fdeg += __shfl_xor(fdeg, o);
+        }
+        for (int32_t i = lane; i < sum_words; i += 64) sum_clear[i] = 0;
+    }
     const int32_t prev_n = next_level > 0 ? st->count[p ^ 1] : 0;
     const long long unv = st->unvisited - n;
     const long long unv_e = st->unvisited_edges - (long long)fdeg;
@@ -387,6 +501,8 @@ __global__ void bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int
             }
     }
     if (lane == 0) {
+        st->count[p] = n;
+        st->fdeg[p] = fdeg;
         st->unvisited = unv;
         st->unvisited_edges = unv_e;
         st->mode = mode;
@@ -396,19 +512,17 @@ __global__ void bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int
             st->hist_mode[next_level] = mode;
             st->hist_next[next_level] = n;
         }
-        st->count[p ^ 1] = 0;
-        st->fdeg[p ^ 1] = 0;
     }
 }
 
 __global__ void bfs_seed_fused_kernel(BfsState *st, int32_t *depth, int32_t *parent,
-                                      uint64_t *visited, uint64_t *bm0, int32_t *q0,
-                                      const int32_t *off, int32_t src, int32_t V, int64_t E) {
+                                      uint64_t *bm0, uint64_t *sum0, const int32_t *off,
+                                      int32_t src, int32_t V, int64_t E) {
     depth[src] = 0;
     if (parent) parent[src] = src;
-    visited[src >> 6] |= 1ull << (src & 63);
     bm0[src >> 6] |= 1ull << (src & 63);
-    q0[0] = src;
+    int32_t ch = (src >> 6) / kChunkWords;
+    sum0[ch >> 6] |= 1ull << (ch & 63);
     st->count[0] = 1;
     st->count[1] = 0;
     st->fdeg[0] = (unsigned long long)(off[src + 1] - off[src]);
@@ -422,32 +536,65 @@ __global__ void bfs_seed_fused_kernel(BfsState *st, int32_t *depth, int32_t *par
     st->hub_count = 0;
 }
 
+__global__ void count_reached_fused_kernel(const int32_t *depth, int32_t V, int32_t *out) {
+    int32_t cnt = 0;
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
+        cnt += depth[v] >= 0;
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if (lane_id() == 0 && cnt) atomicAdd(out, cnt);
+}
+
 // ---------------------------------------------------------------- host side
 struct FusedWork {
     BfsState *st = nullptr;
     uint64_t *bm[3] = {nullptr, nullptr, nullptr};
-    int32_t *q[2] = {nullptr, nullptr};
+    uint64_t *sum[3] = {nullptr, nullptr, nullptr};
+    int32_t *pcount = nullptr;
+    unsigned long long *pfdeg = nullptr;
+    int32_t *reached = nullptr;
     int32_t *host_flag = nullptr;  // pinned copy of `done`
+    int32_t sum_words = 0;
 };
 
 static std::map<const gi_graph *, FusedWork *> g_fused;
 
+static void free_work(FusedWork *w) {
+    dfree(w->st);
+    for (auto p : w->bm) dfree(p);
+    for (auto p : w->sum) dfree(p);
+    dfree(w->pcount);
+    dfree(w->pfdeg);
+    dfree(w->reached);
+    if (w->host_flag) (void)hipHostFree(w->host_flag);
+    delete w;
+}
+
 static int ensure_fused(gi_graph *g, FusedWork **out) {
     auto it = g_fused.find(g);
-    if (it != g_fused.end()) { *out = it->second; return GI_OK; }
-    FusedWork *w = new FusedWork();
+    if (it != g_fused.end()) {
+        *out = it->second;
+        return GI_OK;
+    }
     int64_t words = bitmap_words(g->V);
+    int64_t chunks = ceil_div(words, kChunkWords);
+    int64_t sum_words = ceil_div(chunks, 64);
+    if (sum_words > kMaxSumWords)
+        return fail(GI_ERR_RANGE, "fused BFS supports up to 2^28 vertices");
+    FusedWork *w = new FusedWork();
+    w->sum_words = (int32_t)sum_words;
     int rc = dalloc(&w->st, 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->bm[i], (size_t)words + 1);
-    for (int i = 0; i < 2 && rc == GI_OK; ++i) rc = dalloc(&w->q[i], (size_t)g->V + 64);
+    for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->sum[i], (size_t)sum_words + 1);
+    if (rc == GI_OK) rc = dalloc(&w->pcount, 2 * (size_t)kMaxGrid);
+    if (rc == GI_OK) rc = dalloc(&w->pfdeg, 2 * (size_t)kMaxGrid);
+    if (rc == GI_OK) rc = dalloc(&w->reached, 1);
     if (rc == GI_OK && !g->d_visited) rc = dalloc(&g->d_visited, (size_t)words + 1);
     if (rc == GI_OK && hipHostMalloc((void **)&w->host_flag, sizeof(int32_t) * 4) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
+    if (rc == GI_OK) rc = fill_i32(w->pcount, 2 * kMaxGrid, 0, 0);
+    if (rc == GI_OK) rc = fill_u64((uint64_t *)w->pfdeg, 2 * kMaxGrid, 0, 0);
     if (rc != GI_OK) {
-        dfree(w->st);
-        for (auto p : w->bm) dfree(p);
-        for (auto p : w->q) dfree(p);
-        delete w;
+        free_work(w);
         return rc;
     }
     g_fused[g] = w;
@@ -458,12 +605,7 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
 void release_fused(const gi_graph *g) {
     auto it = g_fused.find(g);
     if (it == g_fused.end()) return;
-    FusedWork *w = it->second;
-    dfree(w->st);
-    for (auto p : w->bm) dfree(p);
-    for (auto p : w->q) dfree(p);
-    if (w->host_flag) (void)hipHostFree(w->host_flag);
-    delete w;
+    free_work(it->second);
     g_fused.erase(it);
 }
 
@@ -478,38 +620,72 @@ int32_t fused_mode_of(const gi_schedule &s) {
     }
 }
 
-int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, SelectorView sel,
-                  int32_t *d_depth, int32_t *d_parent, hipStream_t s, FusedTrace *trace,
-                  bool timed, int32_t *levels_out) {
-    FusedWork *w = nullptr;
-    GI_TRY(ensure_fused(g, &w));
-    int64_t words = bitmap_words(g->V);
-    sel.mode_s1 = fused_mode_of(h->s1);
-    sel.mode_s2 = fused_mode_of(h->s2);
-    sel.threshold = h->threshold;
-    if (sel.mode != GI_SELECT_LEARNED) sel.mode = h->selector;
-    GI_TRY(fill_i32(d_depth, g->V, -1, s));
-    if (d_parent) GI_TRY(fill_i32(d_parent, g->V, -1, s));
-    GI_TRY(fill_u64(g->d_visited, words, 0, s));
-    for (int i = 0; i < 3; ++i) GI_TRY(fill_u64(w->bm[i], words, 0, s));
-    bfs_seed_fused_kernel<<<1, 1, 0, s>>>(w->st, d_depth, d_parent, g->d_visited, w->bm[0], w->q[0],
-                                          g->d_off, source, g->V, g->E);
-    bfs_decide_kernel<<<1, 64, 0, s>>>(w->st, sel, g->V, g->E);
-    GI_LAUNCH_CHECK();
+static int step_grid() {
+    int g = persistent_grid(kStepBlock, 8);
+    return g < kMaxGrid ? g : kMaxGrid;
+}
+
+static StepArgs step_args(gi_graph *g, FusedWork *w, int32_t *depth, int32_t *parent) {
     StepArgs a;
     a.st = w->st;
     a.off = g->d_off;
     a.nbr = g->d_nbr;
     a.V = g->V;
-    a.words = (int32_t)words;
-    a.depth = d_depth;
-    a.parent = d_parent;
+    a.words = (int32_t)bitmap_words(g->V);
+    a.chunks = (int32_t)ceil_div(a.words, kChunkWords);
+    a.sum_words = w->sum_words;
+    a.depth = depth;
+    a.parent = parent;
     a.visited = g->d_visited;
-    for (int i = 0; i < 3; ++i) a.bm[i] = w->bm[i];
-    for (int i = 0; i < 2; ++i) a.q[i] = w->q[i];
-    int grid = persistent_grid(kStepBlock, 8);
+    for (int i = 0; i < 3; ++i) {
+        a.bm[i] = w->bm[i];
+        a.sum[i] = w->sum[i];
+    }
+    a.pcount = w->pcount;
+    a.pfdeg = w->pfdeg;
+    return a;
+}
+
+static void launch_decide(gi_graph *g, FusedWork *w, const SelectorView &sel, int32_t level,
+                          hipStream_t s) {
+    // after step `level`, the bitmap two levels ahead of it is cleared
+    uint64_t *sum_clear = w->sum[(level + 2 + 3) % 3];
+    bfs_decide_kernel<<<1, 64, 0, s>>>(w->st, sel, g->V, g->E, w->pcount, w->pfdeg, step_grid(),
+                                       sum_clear, w->sum_words);
+}
+
+static int seed_run(gi_graph *g, FusedWork *w, int32_t source, int32_t *depth, int32_t *parent,
+                    const SelectorView &sel, hipStream_t s) {
+    int64_t words = bitmap_words(g->V);
+    GI_TRY(fill_i32(depth, g->V, -1, s));
+    if (parent) GI_TRY(fill_i32(parent, g->V, -1, s));
+    GI_TRY(fill_u64(g->d_visited, words, 0, s));
+    for (int i = 0; i < 3; ++i) {
+        GI_TRY(fill_u64(w->bm[i], words, 0, s));
+        GI_TRY(fill_u64(w->sum[i], w->sum_words, 0, s));
+    }
+    bfs_seed_fused_kernel<<<1, 1, 0, s>>>(w->st, depth, parent, w->bm[0], w->sum[0], g->d_off,
+                                          source, g->V, g->E);
+    launch_decide(g, w, sel, -1, s);
+    GI_LAUNCH_CHECK();
+    return GI_OK;
+}
+
+int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, SelectorView sel,
+                  int32_t *d_depth, int32_t *d_parent, hipStream_t s, FusedTrace *trace,
+                  bool timed, int32_t *levels_out) {
+    FusedWork *w = nullptr;
+    GI_TRY(ensure_fused(g, &w));
+    sel.mode_s1 = fused_mode_of(h->s1);
+    sel.mode_s2 = fused_mode_of(h->s2);
+    sel.threshold = h->threshold;
+    if (sel.mode != GI_SELECT_LEARNED) sel.mode = h->selector;
+    GI_TRY(seed_run(g, w, source, d_depth, d_parent, sel, s));
+    StepArgs a = step_args(g, w, d_depth, d_parent);
+    int grid = step_grid();
     int32_t *done_flag = &w->st->done;
-    for (int32_t level = 0; level < kBfsMaxLevels; ++level) {
+    int32_t level = 0;
+    for (; level < kBfsMaxLevels; ++level) {
         hipEvent_t *ev = nullptr;
         if (trace && timed) {
             hipEvent_t e0, e1;
@@ -523,34 +699,50 @@ int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, Sele
         bfs_step_kernel<<<grid, kStepBlock, 0, s>>>(a);
         bfs_hub_kernel<<<grid, kStepBlock, 0, s>>>(a);
         if (ev) GI_HIP(hipEventRecord(ev[1], s));
-        bfs_decide_kernel<<<1, 64, 0, s>>>(w->st, sel, g->V, g->E);
+        launch_decide(g, w, sel, level, s);
         GI_LAUNCH_CHECK();
         // one 4-byte read per level tells the host whether to go on
         GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         GI_HIP(hipStreamSynchronize(s));
         if (*w->host_flag) break;
     }
-    int32_t levels = 0;
-    GI_HIP(hipMemcpyAsync(&levels, &w->st->levels, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    GI_HIP(hipStreamSynchronize(s));
-    if (levels_out) *levels_out = levels;
-    if (trace) {
-        int32_t n = levels < kBfsMaxLevels ? levels : kBfsMaxLevels;
-        trace->modes.resize(n);
-        trace->sizes.resize(n);
-        GI_HIP(hipMemcpy(trace->modes.data(), w->st->hist_mode, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-        GI_HIP(hipMemcpy(trace->sizes.data(), w->st->hist_next, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (levels_out || trace) {
+        int32_t levels = 0;
+        GI_HIP(hipMemcpyAsync(&levels, &w->st->levels, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GI_HIP(hipStreamSynchronize(s));
+        if (levels_out) *levels_out = levels;
+        if (trace) {
+            int32_t n = levels < kBfsMaxLevels ? levels : kBfsMaxLevels;
+            trace->modes.resize(n);
+            trace->sizes.resize(n);
+            GI_HIP(hipMemcpy(trace->modes.data(), w->st->hist_mode, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+            GI_HIP(hipMemcpy(trace->sizes.data(), w->st->hist_next, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        }
     }
     return GI_OK;
 }
 
+int fused_reached(gi_graph *g, const int32_t *depth, int32_t *out, hipStream_t s) {
+    FusedWork *w = nullptr;
+    GI_TRY(ensure_fused(g, &w));
+    GI_TRY(fill_i32(w->reached, 1, 0, s));
+    int grid = step_grid();
+    count_reached_fused_kernel<<<grid, 256, 0, s>>>(depth, g->V, w->reached);
+    GI_LAUNCH_CHECK();
+    return read_count(w->reached, out, s);
+}
+
 // ---------------------------------------------------------------- sweep
 __global__ void bfs_force_mode_kernel(BfsState *st, int32_t mode) {
-    int32_t p = (st->level & 1) ^ 1;
     st->mode = mode;
-    st->count[p] = 0;
-    st->fdeg[p] = 0;
     st->hub_count = 0;
+}
+
+__global__ void zero_slots_kernel(int32_t *pcount, unsigned long long *pfdeg) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * kMaxGrid; i += gridDim.x * blockDim.x) {
+        pcount[i] = 0;
+        pfdeg[i] = 0;
+    }
 }
 
 struct SweepHost {
@@ -595,50 +787,43 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
     sel.mode = GI_SELECT_S1;
     sel.mode_s1 = BFS_PUSH_THREAD;
     sel.mode_s2 = BFS_PUSH_THREAD;
-    StepArgs a;
-    a.st = w->st;
-    a.off = g->d_off;
-    a.nbr = g->d_nbr;
-    a.V = g->V;
-    a.words = (int32_t)words;
-    a.depth = depth;
-    a.parent = nullptr;
-    a.visited = g->d_visited;
-    for (int i = 0; i < 3; ++i) a.bm[i] = w->bm[i];
-    for (int i = 0; i < 2; ++i) a.q[i] = w->q[i];
-    int grid = persistent_grid(kStepBlock, 8);
-    if (rc == GI_OK) rc = fill_i32(depth, g->V, -1, s);
-    if (rc == GI_OK) rc = fill_u64(g->d_visited, words, 0, s);
-    for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = fill_u64(w->bm[i], words, 0, s);
-    if (rc == GI_OK) {
-        bfs_seed_fused_kernel<<<1, 1, 0, s>>>(w->st, depth, nullptr, g->d_visited, w->bm[0], w->q[0],
-                                              g->d_off, source, g->V, g->E);
-        bfs_decide_kernel<<<1, 64, 0, s>>>(w->st, sel, g->V, g->E);
-    }
+    StepArgs a = step_args(g, w, depth, nullptr);
+    int grid = step_grid();
+    if (rc == GI_OK) rc = seed_run(g, w, source, depth, nullptr, sel, s);
     int32_t nl = 0, prev_mode = BFS_PUSH_THREAD, prev_n = 0;
     while (rc == GI_OK && nl < max_levels) {
         SweepHost h;
         if (hipMemcpyAsync(&h, w->st, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) { rc = fail(GI_ERR_HIP, "state read"); break; }
+            hipStreamSynchronize(s) != hipSuccess) {
+            rc = fail(GI_ERR_HIP, "state read");
+            break;
+        }
         if (h.done) break;
         int32_t p = h.level & 1;
         bfs_raw_features(features + (int64_t)nl * kSelInputs, h.count[p], h.fdeg[p], h.unvisited,
                          h.unvisited_edges, h.level, prev_mode, prev_n, g->V, g->E);
-        // snapshot, then time every mode of the mask from the same state
+        const int32_t iout = (h.level + 1) % 3, iclr = (h.level + 2) % 3;
+        // snapshot, then time every mode of the mask from the same state; the
+        // step clears bitmap iclr, which is already empty after the first run
         GI_HIP(hipMemcpyAsync(depth_bak, depth, sizeof(int32_t) * g->V, hipMemcpyDeviceToDevice, s));
         GI_HIP(hipMemcpyAsync(vis_bak, g->d_visited, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, s));
         float best = 3.4e38f;
         int32_t best_mode = -1;
-        for (int32_t m = 0; m < BFS_NUM_MODES; ++m) {
-            float *t = times_ms + (int64_t)nl * BFS_NUM_MODES + m;
-            *t = -1.f;
-            if (!((mode_mask >> m) & 1)) continue;
+        for (int32_t m = 0; m <= BFS_NUM_MODES && rc == GI_OK; ++m) {
+            bool final_run = m == BFS_NUM_MODES;
+            if (!final_run && !((mode_mask >> m) & 1)) {
+                times_ms[(int64_t)nl * BFS_NUM_MODES + m] = -1.f;
+                continue;
+            }
+            int32_t run = final_run ? (follow >= 0 ? follow : best_mode) : m;
             float tmin = 3.4e38f;
-            for (int32_t r = 0; r < reps; ++r) {
+            for (int32_t r = 0; r < (final_run ? 1 : reps); ++r) {
                 GI_HIP(hipMemcpyAsync(depth, depth_bak, sizeof(int32_t) * g->V, hipMemcpyDeviceToDevice, s));
                 GI_HIP(hipMemcpyAsync(g->d_visited, vis_bak, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, s));
-                GI_TRY(fill_u64(w->bm[(h.level + 1) % 3], words, 0, s));
-                bfs_force_mode_kernel<<<1, 1, 0, s>>>(w->st, m);
+                GI_TRY(fill_u64(w->bm[iout], words, 0, s));
+                GI_TRY(fill_u64(w->sum[iout], w->sum_words, 0, s));
+                zero_slots_kernel<<<8, 256, 0, s>>>(w->pcount, w->pfdeg);
+                bfs_force_mode_kernel<<<1, 1, 0, s>>>(w->st, run);
                 GI_HIP(hipEventRecord(e0, s));
                 bfs_step_kernel<<<grid, kStepBlock, 0, s>>>(a);
                 bfs_hub_kernel<<<grid, kStepBlock, 0, s>>>(a);
@@ -648,19 +833,19 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
                 GI_HIP(hipEventElapsedTime(&ms, e0, e1));
                 tmin = ms < tmin ? ms : tmin;
             }
-            *t = tmin;
-            if (tmin < best) { best = tmin; best_mode = m; }
+            if (!final_run) {
+                times_ms[(int64_t)nl * BFS_NUM_MODES + m] = tmin;
+                if (tmin < best) {
+                    best = tmin;
+                    best_mode = m;
+                }
+            } else {
+                launch_decide(g, w, sel, h.level, s);
+                GI_LAUNCH_CHECK();
+                prev_mode = run;
+            }
         }
-        int32_t run = follow >= 0 ? follow : best_mode;
-        GI_HIP(hipMemcpyAsync(depth, depth_bak, sizeof(int32_t) * g->V, hipMemcpyDeviceToDevice, s));
-        GI_HIP(hipMemcpyAsync(g->d_visited, vis_bak, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, s));
-        GI_TRY(fill_u64(w->bm[(h.level + 1) % 3], words, 0, s));
-        bfs_force_mode_kernel<<<1, 1, 0, s>>>(w->st, run);
-        bfs_step_kernel<<<grid, kStepBlock, 0, s>>>(a);
-        bfs_hub_kernel<<<grid, kStepBlock, 0, s>>>(a);
-        bfs_decide_kernel<<<1, 64, 0, s>>>(w->st, sel, g->V, g->E);
-        GI_LAUNCH_CHECK();
-        prev_mode = run;
+        (void)iclr;
         prev_n = h.count[p];
         ++nl;
     }

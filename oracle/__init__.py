@@ -55,6 +55,7 @@ def lib():
             "or_pick_sources": (i32, [i32, i32p, u64, i32, i32p]),
             "or_bfs_depth": (i32, [i32, i32p, i32p, i32, i32p]),
             "or_bfs_parent_valid": (i32, [i32, i32p, i32p, i32, i32p, i32p]),
+            "or_bfs_multi_depth": (i32, [i32, i32p, i32p, i32p, i32, i32p]),
             "or_bfs_pull_examined": (i64, [i32, i32p, i32p, i32p, i32]),
             "or_sssp_dijkstra": (None, [i32, i32p, i32p, i32p, i32, i32p]),
             "or_sssp_delta_stepping": (i64, [i32, i32p, i32p, i32p, i32, i32, i32p]),
@@ -191,6 +192,13 @@ def pick_sources(g: CSR, seed: int, count: int):
 def bfs_depth(g: CSR, source: int):
     d = np.empty(g.num_vertices, np.int32)
     lib().or_bfs_depth(g.num_vertices, g.offsets, g.nbrs, source, d)
+    return d
+
+
+def bfs_multi_depth(g: CSR, sources):
+    d = np.empty(g.num_vertices, np.int32)
+    src = np.ascontiguousarray(sources, np.int32)
+    lib().or_bfs_multi_depth(g.num_vertices, g.offsets, g.nbrs, src, len(src), d)
     return d
 
 

@@ -273,6 +273,37 @@ int32_t or_bfs_depth(int32_t num_vertices, const int32_t *offsets, const int32_t
     return levels;
 }
 
+/* Multi-source BFS: distance to the nearest source (queue seeded with every
+ * source at depth 0). On a disjoint union of graphs with one source per
+ * graph it is every graph's own BFS. Returns the number of levels. */
+int32_t or_bfs_multi_depth(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,
+                           const int32_t *sources, int32_t num_sources, int32_t *depth) {
+    for (int32_t v = 0; v < num_vertices; ++v) depth[v] = -1;
+    int32_t *queue = (int32_t *)malloc((size_t)num_vertices * sizeof(int32_t) + 4);
+    int64_t head = 0, tail = 0;
+    int32_t levels = 0;
+    for (int32_t i = 0; i < num_sources; ++i) {
+        int32_t s = sources[i];
+        if (s < 0 || s >= num_vertices || depth[s] == 0) continue;
+        depth[s] = 0;
+        queue[tail++] = s;
+        levels = 1;
+    }
+    while (head < tail) {
+        int32_t u = queue[head++];
+        for (int32_t e = offsets[u]; e < offsets[u + 1]; ++e) {
+            int32_t v = nbrs[e];
+            if (depth[v] == -1) {
+                depth[v] = depth[u] + 1;
+                if (depth[v] + 1 > levels) levels = depth[v] + 1;
+                queue[tail++] = v;
+            }
+        }
+    }
+    free(queue);
+    return levels;
+}
+
 static int has_edge(const int32_t *offsets, const int32_t *nbrs, int32_t u, int32_t v) {
     int32_t lo = offsets[u], hi = offsets[u + 1];
     while (lo < hi) {

@@ -70,3 +70,24 @@ def test_bc_small(gi):
     sc, _ = gi.bc(g, srcs)
     want = O.bc(host, srcs)
     assert rel_l1(sc, want) <= 1e-4
+
+
+@pytest.mark.parametrize("batch", [2, 64, 256])
+def test_bc_batched_matches_oracle(gi, batch):
+    g = gi.Graph.rmat(12, 8, SEED, True)
+    host = host_csr(g)
+    srcs = O.pick_sources(host, 11, 40)
+    srcs[5] = srcs[3]  # a repeated source counts twice, as two runs would
+    sc, st = gi.bc(g, srcs, batch=batch)
+    assert rel_l1(sc, O.bc(host, srcs)) <= 1e-4
+
+
+def test_bc_batched_equals_sequential_on_fixture(gi):
+    e = np.array([[0, 1], [1, 2], [2, 3], [0, 4], [4, 3], [3, 5]], np.int32)
+    g = gi.Graph.from_edges(7, e[:, 0], e[:, 1], True)  # vertex 6 isolated
+    srcs = np.array([0, 3, 6, 5], np.int32)
+    host = host_csr(g)
+    want = O.bc(host, srcs)
+    seq, _ = gi.bc(g, srcs, batch=1)
+    bat, _ = gi.bc(g, srcs, batch=4)
+    assert np.allclose(seq, want, rtol=1e-5) and np.allclose(bat, want, rtol=1e-5)
