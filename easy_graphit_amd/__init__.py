@@ -188,6 +188,10 @@ def lib():
         "gi_selector_loaded": (i32, []),
         "gi_selector_loss_grad": (C.c_int, [vp, vp, vp, i32, i32, vp, vp, vp]),
         "gi_selector_train": (C.c_int, [vp, vp, vp, i32, i32, i32, C.c_float, C.c_float, vp, vp]),
+        "gi_bfs_set_bins": (C.c_int, [vp, i32, i32, i32]),
+        "gi_bfs_multi": (C.c_int, [vp, vp, i32, C.POINTER(HybridSchedule), vp, vp,
+                                   C.POINTER(i32), vp]),
+        "gi_graph_uniform_batch": (C.c_int, [i32, i32, i32, i32, u64, vp, pp]),
         "gi_bfs_sweep": (C.c_int, [vp, i32, i32, i32, i32, vp, vp, i32, C.POINTER(i32), vp]),
         "gi_pq_create": (C.c_int, [vp, vp, i32, i32, i32, pp]),
         "gi_pq_finished": (C.c_int, [vp, C.POINTER(i32), vp]),
@@ -305,6 +309,19 @@ class Graph:
         h = C.c_void_p()
         check(lib().gi_graph_uniform(num_vertices, num_pairs, seed, int(symmetrize), C.byref(h)))
         return cls(h.value)
+
+    @classmethod
+    def uniform_batch(cls, num_graphs: int, vmin: int, vmax: int, avg_degree: int, seed: int):
+        """Many small uniform graphs packed in one symmetric CSR; returns
+        (graph, vertex_offsets[num_graphs+1])."""
+        offs = np.empty(num_graphs + 1, np.int32)
+        h = C.c_void_p()
+        check(lib().gi_graph_uniform_batch(num_graphs, vmin, vmax, avg_degree, seed,
+                                           offs.ctypes.data, C.byref(h)))
+        return cls(h.value), offs
+
+    def set_bins(self, lane_max: int, hub_min: int, blocks_per_cu: int = 8):
+        check(lib().gi_bfs_set_bins(self.h, lane_max, hub_min, blocks_per_cu))
 
     @classmethod
     def grid(cls, width: int, num_random: int, seed: int):
@@ -521,6 +538,19 @@ def bfs_device(g: Graph, source: int, schedule, depth: DeviceArray, parent=None,
                        depth.ptr, parent.ptr if parent else None,
                        C.byref(st) if st is not None else None, stream))
     return st.as_dict() if st is not None else None
+
+
+def bfs_multi(g: Graph, sources, schedule: HybridSchedule, depth: DeviceArray | None = None,
+              parent: DeviceArray | None = None, stream=None):
+    """Multi-source fused BFS (distance to the nearest source)."""
+    src = np.ascontiguousarray(sources, np.int32)
+    own = depth is None
+    if own:
+        depth = DeviceArray(g.num_vertices, np.int32)
+    levels = C.c_int32()
+    check(lib().gi_bfs_multi(g.h, src.ctypes.data, len(src), C.byref(schedule), depth.ptr,
+                             parent.ptr if parent else None, C.byref(levels), stream))
+    return (depth.numpy(stream) if own else None), levels.value
 
 
 def bfs_profile(g: Graph, source: int, schedule, depth: DeviceArray, stream=None):

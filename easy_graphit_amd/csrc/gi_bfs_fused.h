@@ -26,7 +26,7 @@ constexpr int kSelMaxOut = 8;
 constexpr int32_t kHubDegree = 4096;  // push vertices at least this wide go to the grid bin
 constexpr int32_t kMaxHubs = 1024;
 constexpr int32_t kMaxGrid = 8192;          // partial-sum slots per kernel
-constexpr int32_t kMaxSumWords = 1024;      // summary words (V up to 2^28)
+constexpr int32_t kMaxSumWords = 256;       // summary words (V up to 2^26)
 
 struct BfsState {
     int32_t count[2];               // frontier sizes by level parity
@@ -38,7 +38,11 @@ struct BfsState {
     long long unvisited;            // vertices not reached yet
     long long unvisited_edges;      // sum of their degrees
     int32_t hub_count;              // deferred push vertices of the current step
-    int32_t cursor;                 // dynamic chunk assignment of push steps
+    int32_t lane_max;               // push: degree < lane_max walked by one lane
+    int32_t hub_min;                // push TWC: degree >= hub_min goes to the grid bin
+    int32_t pad0;
+    int32_t *depth;                 // output arrays of the current run
+    int32_t *parent;
     int32_t hubs[kMaxHubs];
     int32_t hist_mode[kBfsMaxLevels];
     int32_t hist_next[kBfsMaxLevels];
@@ -103,6 +107,11 @@ int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, Sele
                   bool timed, int32_t *levels_out);
 void release_fused(const gi_graph *g);
 int fused_reached(gi_graph *g, const int32_t *depth, int32_t *out, hipStream_t s);
+int fused_set_bins(gi_graph *g, int32_t lane_max, int32_t hub_min, int32_t blocks_per_cu);
+int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources, int32_t single,
+                        const gi_hybrid_schedule *h, SelectorView sel, int32_t *d_depth,
+                        int32_t *d_parent, hipStream_t s, FusedTrace *trace, bool timed,
+                        int32_t *levels_out);
 // The process-wide selector model (null weights when none is loaded).
 SelectorView selector_view();
 

@@ -56,6 +56,8 @@ struct StepArgs {
 
 struct StepCtx {
     int32_t level;
+    int32_t lane_max;          // degree < lane_max: one lane walks the vertex
+    int32_t hub_min;           // TWC: degree >= hub_min goes to the grid bin
     const uint64_t *bm_in;
     uint64_t *bm_out;
     uint64_t *sum_out;
@@ -63,6 +65,13 @@ struct StepCtx {
     int32_t cnt;               // per-lane |F'| contribution
     unsigned long long fdeg;   // per-lane sum of degrees of F'
 };
+
+// Index one of three pointers without dynamic indexing into a private array
+// (which would live in scratch memory).
+template <class T>
+__device__ __forceinline__ T pick3(T p0, T p1, T p2, int32_t i) {
+    return i == 0 ? p0 : (i == 1 ? p1 : p2);
+}
 
 __device__ __forceinline__ bool seen(const StepCtx &c, int32_t v) {
     uint64_t bit = 1ull << (v & 63);
@@ -181,11 +190,11 @@ __device__ void push_batch(const StepArgs &a, StepCtx &c, int32_t mode, const in
             wave_walk(a, c, v, a.off[v], a.off[v + 1]);
         }
     } else if (mode == BFS_PUSH_TWC) {
-        // lane bin (< 64), wave bin (< kHubDegree), grid bin (deferred to the hub kernel)
-        bool hub = has && dg >= kHubDegree;
+        // lane bin, wave bin, grid bin (deferred to the hub kernel)
+        bool hub = has && dg >= c.hub_min;
         int32_t hpos = wave_atomic_add(&a.st->hub_count, hub);
         if (hub && hpos < kMaxHubs) a.st->hubs[hpos] = s;
-        bool wide = has && dg >= 64 && (!hub || hpos >= kMaxHubs);
+        bool wide = has && dg >= c.lane_max && (!hub || hpos >= kMaxHubs);
         uint64_t wm = __ballot(wide);
         while (wm) {
             int32_t l = __ffsll((unsigned long long)wm) - 1;
@@ -193,7 +202,7 @@ __device__ void push_batch(const StepArgs &a, StepCtx &c, int32_t mode, const in
             int32_t ws = __shfl(s, l), w0 = __shfl(e0, l), w1 = __shfl(e1, l);
             wave_walk(a, c, ws, w0, w1);
         }
-        if (has && dg < 64) lane_walk(a, c, s, e0, e1);
+        if (has && dg < c.lane_max) lane_walk(a, c, s, e0, e1);
     } else {
         // TWCE and CM need the workgroup's degree scan
         int32_t total = 0;
@@ -232,17 +241,22 @@ __device__ void push_batch(const StepArgs &a, StepCtx &c, int32_t mode, const in
 }
 
 // ---------------------------------------------------------------- step kernel
-__global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a) {
-    BfsState *st = a.st;
+__global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a0) {
+    BfsState *st = a0.st;
     if (st->done) return;
+    StepArgs a = a0;
+    a.depth = st->depth;
+    a.parent = st->parent;
     __shared__ int32_t pre[kMaxSumWords + 1];
     StepCtx c;
     c.level = st->level;
+    c.lane_max = st->lane_max;
+    c.hub_min = st->hub_min;
     const int32_t mode = st->mode;
     const int32_t iin = c.level % 3, iout = (c.level + 1) % 3, iclr = (c.level + 2) % 3;
-    c.bm_in = a.bm[iin];
-    c.bm_out = a.bm[iout];
-    c.sum_out = a.sum[iout];
+    c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iin);
+    c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iout);
+    c.sum_out = pick3(a0.sum[0], a0.sum[1], a0.sum[2], iout);
     c.visited = a.visited;
     c.cnt = 0;
     c.fdeg = 0;
@@ -253,8 +267,8 @@ __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a) {
     // the bitmap two levels ahead is free: clear its non-empty chunks (its
     // summary is cleared by the decide kernel after this step)
     {
-        const uint64_t *sc = a.sum[iclr];
-        uint64_t *bc = a.bm[iclr];
+        const uint64_t *sc = pick3(a0.sum[0], a0.sum[1], a0.sum[2], iclr);
+        uint64_t *bc = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iclr);
         int32_t nz = summary_prefix(sc, a.sum_words, pre);
         for (int32_t k = blockIdx.x; k < nz; k += gridDim.x) {
             int32_t ch = kth_chunk(sc, pre, a.sum_words, k);
@@ -269,7 +283,7 @@ __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a) {
     if (mode < BFS_PULL_THREAD) {
         __shared__ int32_t list[kChunkVerts];
         __shared__ int32_t n_s;
-        const uint64_t *si = a.sum[iin];
+        const uint64_t *si = pick3(a0.sum[0], a0.sum[1], a0.sum[2], iin);
         int32_t nz = summary_prefix(si, a.sum_words, pre);
         for (int32_t k = blockIdx.x; k < nz; k += gridDim.x) {
             int32_t chunk = kth_chunk(si, pre, a.sum_words, k);
@@ -374,17 +388,22 @@ __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a) {
 // Grid bin of PUSH_TWC: the edges of the deferred hubs, spread evenly over
 // the grid. Every workgroup scans the hub degrees in LDS and walks its share
 // of the concatenated edge range.
-__global__ void __launch_bounds__(kStepBlock) bfs_hub_kernel(StepArgs a) {
-    BfsState *st = a.st;
+__global__ void __launch_bounds__(kStepBlock) bfs_hub_kernel(StepArgs a0) {
+    BfsState *st = a0.st;
     if (st->done) return;
     int32_t nh = st->hub_count;
     if (nh <= 0) return;  // slots stay zero (the decide kernel clears them)
     if (nh > kMaxHubs) nh = kMaxHubs;
+    StepArgs a = a0;
+    a.depth = st->depth;
+    a.parent = st->parent;
     StepCtx c;
     c.level = st->level;
-    c.bm_in = a.bm[c.level % 3];
-    c.bm_out = a.bm[(c.level + 1) % 3];
-    c.sum_out = a.sum[(c.level + 1) % 3];
+    c.lane_max = st->lane_max;
+    c.hub_min = st->hub_min;
+    c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], c.level % 3);
+    c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], (c.level + 1) % 3);
+    c.sum_out = pick3(a0.sum[0], a0.sum[1], a0.sum[2], (c.level + 1) % 3);
     c.visited = a.visited;
     c.cnt = 0;
     c.fdeg = 0;
@@ -424,10 +443,18 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
 // step's partial sums, closes its bookkeeping, ends the run on an empty
 // frontier, else picks the next step's mode and clears what the next step
 // writes.
-__global__ void bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int64_t E,
+struct SumSet {
+    uint64_t *s[3];
+};
+
+__global__ void bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E,
                                   int32_t *pcount, unsigned long long *pfdeg, int32_t slots,
-                                  uint64_t *sum_clear, int32_t sum_words) {
+                                  SumSet sums, int32_t sum_words) {
     if (st->done) return;
+    const SelectorView sel = *selp;
+    // after step `level`, the bitmap two levels ahead of it was cleared by the
+    // step; its summary is cleared here
+    uint64_t *sum_clear = pick3(sums.s[0], sums.s[1], sums.s[2], (st->level + 2 + 3) % 3);
     const int32_t lane = lane_id();
     const int32_t next_level = st->level + 1;
     const int32_t p = next_level & 1;
@@ -515,17 +542,17 @@ __global__ void bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int
     }
 }
 
-__global__ void bfs_seed_fused_kernel(BfsState *st, int32_t *depth, int32_t *parent,
-                                      uint64_t *bm0, uint64_t *sum0, const int32_t *off,
-                                      int32_t src, int32_t V, int64_t E) {
-    depth[src] = 0;
-    if (parent) parent[src] = src;
-    bm0[src >> 6] |= 1ull << (src & 63);
-    int32_t ch = (src >> 6) / kChunkWords;
-    sum0[ch >> 6] |= 1ull << (ch & 63);
-    st->count[0] = 1;
+struct SeedArgs {
+    int32_t *depth;
+    int32_t *parent;
+    int32_t lane_max;
+    int32_t hub_min;
+};
+
+__global__ void bfs_state_init_kernel(BfsState *st, SeedArgs sa, int32_t V, int64_t E) {
+    st->count[0] = 0;
     st->count[1] = 0;
-    st->fdeg[0] = (unsigned long long)(off[src + 1] - off[src]);
+    st->fdeg[0] = 0;
     st->fdeg[1] = 0;
     st->done = 0;
     st->level = -1;
@@ -534,6 +561,40 @@ __global__ void bfs_seed_fused_kernel(BfsState *st, int32_t *depth, int32_t *par
     st->unvisited = V;
     st->unvisited_edges = E;
     st->hub_count = 0;
+    st->lane_max = sa.lane_max;
+    st->hub_min = sa.hub_min;
+    st->depth = sa.depth;
+    st->parent = sa.parent;
+}
+
+// Seeds one or many sources (a multi-source BFS gives every vertex its
+// distance to the nearest source; over a disjoint union of graphs with one
+// source each it is every graph's own BFS). Repeated sources count once.
+__global__ void bfs_seed_sources_kernel(BfsState *st, const int32_t *src, int32_t n,
+                                        int32_t single, uint64_t *bm0, uint64_t *sum0,
+                                        const int32_t *off) {
+    int32_t cnt = 0;
+    unsigned long long fd = 0;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int32_t v = src ? src[i] : single;
+        uint64_t bit = 1ull << (v & 63);
+        unsigned long long old = atomicOr((unsigned long long *)&bm0[v >> 6], bit);
+        if (old & bit) continue;
+        st->depth[v] = 0;
+        if (st->parent) st->parent[v] = v;
+        int32_t ch = (v >> 6) / kChunkWords;
+        atomicOr((unsigned long long *)&sum0[ch >> 6], 1ull << (ch & 63));
+        cnt += 1;
+        fd += (unsigned long long)(off[v + 1] - off[v]);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        cnt += __shfl_xor(cnt, o);
+        fd += __shfl_xor(fd, o);
+    }
+    if (lane_id() == 0 && cnt) {
+        atomicAdd(&st->count[0], cnt);
+        atomicAdd(&st->fdeg[0], fd);
+    }
 }
 
 __global__ void count_reached_fused_kernel(const int32_t *depth, int32_t V, int32_t *out) {
@@ -552,19 +613,38 @@ struct FusedWork {
     int32_t *pcount = nullptr;
     unsigned long long *pfdeg = nullptr;
     int32_t *reached = nullptr;
-    int32_t *host_flag = nullptr;  // pinned copy of `done`
+    SelectorView *d_sel = nullptr;   // selector of the current run (device copy)
+    int32_t *d_src = nullptr;        // sources of a multi-source run
+    int32_t src_cap = 0;
+    int32_t *host_flag = nullptr;    // pinned copy of `done`
     int32_t sum_words = 0;
+    int32_t lane_max = 64;           // bin thresholds (gi_bfs_set_bins)
+    int32_t hub_min = kHubDegree;
+    int32_t blocks_per_cu = 8;
+    // level loop captured as a hipGraph, one exec per level count
+    hipStream_t cap_stream = nullptr;
+    std::map<int32_t, hipGraphExec_t> execs;
+    int32_t last_levels = 0;
 };
 
 static std::map<const gi_graph *, FusedWork *> g_fused;
 
+static void drop_execs(FusedWork *w) {
+    for (auto &kv : w->execs) (void)hipGraphExecDestroy(kv.second);
+    w->execs.clear();
+}
+
 static void free_work(FusedWork *w) {
+    drop_execs(w);
+    if (w->cap_stream) (void)hipStreamDestroy(w->cap_stream);
     dfree(w->st);
     for (auto p : w->bm) dfree(p);
     for (auto p : w->sum) dfree(p);
     dfree(w->pcount);
     dfree(w->pfdeg);
     dfree(w->reached);
+    dfree(w->d_sel);
+    dfree(w->d_src);
     if (w->host_flag) (void)hipHostFree(w->host_flag);
     delete w;
 }
@@ -579,7 +659,7 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
     int64_t chunks = ceil_div(words, kChunkWords);
     int64_t sum_words = ceil_div(chunks, 64);
     if (sum_words > kMaxSumWords)
-        return fail(GI_ERR_RANGE, "fused BFS supports up to 2^28 vertices");
+        return fail(GI_ERR_RANGE, "fused BFS supports up to 2^26 vertices");
     FusedWork *w = new FusedWork();
     w->sum_words = (int32_t)sum_words;
     int rc = dalloc(&w->st, 1);
@@ -588,11 +668,15 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
     if (rc == GI_OK) rc = dalloc(&w->pcount, 2 * (size_t)kMaxGrid);
     if (rc == GI_OK) rc = dalloc(&w->pfdeg, 2 * (size_t)kMaxGrid);
     if (rc == GI_OK) rc = dalloc(&w->reached, 1);
+    if (rc == GI_OK) rc = dalloc(&w->d_sel, 1);
     if (rc == GI_OK && !g->d_visited) rc = dalloc(&g->d_visited, (size_t)words + 1);
     if (rc == GI_OK && hipHostMalloc((void **)&w->host_flag, sizeof(int32_t) * 4) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
+    if (rc == GI_OK && hipStreamCreateWithFlags(&w->cap_stream, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(GI_ERR_HIP, "stream create failed");
     if (rc == GI_OK) rc = fill_i32(w->pcount, 2 * kMaxGrid, 0, 0);
     if (rc == GI_OK) rc = fill_u64((uint64_t *)w->pfdeg, 2 * kMaxGrid, 0, 0);
+    if (rc == GI_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(GI_ERR_HIP, "init sync");
     if (rc != GI_OK) {
         free_work(w);
         return rc;
@@ -620,12 +704,24 @@ int32_t fused_mode_of(const gi_schedule &s) {
     }
 }
 
-static int step_grid() {
-    int g = persistent_grid(kStepBlock, 8);
+// Persistent grid: as many workgroups as can be resident at once (the
+// step kernel assigns work statically, so a second round of workgroups
+// would serialise), capped by the tuned blocks per CU.
+static int step_grid(const FusedWork *w) {
+    static int resident = -1;
+    if (resident < 0) {
+        int per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bfs_step_kernel, kStepBlock, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 4;
+        resident = per_cu;
+    }
+    int per = w->blocks_per_cu < resident ? w->blocks_per_cu : resident;
+    int g = device_info().num_cus * per;
     return g < kMaxGrid ? g : kMaxGrid;
 }
 
-static StepArgs step_args(gi_graph *g, FusedWork *w, int32_t *depth, int32_t *parent) {
+static StepArgs step_args(gi_graph *g, FusedWork *w) {
     StepArgs a;
     a.st = w->st;
     a.off = g->d_off;
@@ -634,8 +730,8 @@ static StepArgs step_args(gi_graph *g, FusedWork *w, int32_t *depth, int32_t *pa
     a.words = (int32_t)bitmap_words(g->V);
     a.chunks = (int32_t)ceil_div(a.words, kChunkWords);
     a.sum_words = w->sum_words;
-    a.depth = depth;
-    a.parent = parent;
+    a.depth = nullptr;   // read from the state inside the kernels
+    a.parent = nullptr;
     a.visited = g->d_visited;
     for (int i = 0; i < 3; ++i) {
         a.bm[i] = w->bm[i];
@@ -646,17 +742,30 @@ static StepArgs step_args(gi_graph *g, FusedWork *w, int32_t *depth, int32_t *pa
     return a;
 }
 
-static void launch_decide(gi_graph *g, FusedWork *w, const SelectorView &sel, int32_t level,
-                          hipStream_t s) {
-    // after step `level`, the bitmap two levels ahead of it is cleared
-    uint64_t *sum_clear = w->sum[(level + 2 + 3) % 3];
-    bfs_decide_kernel<<<1, 64, 0, s>>>(w->st, sel, g->V, g->E, w->pcount, w->pfdeg, step_grid(),
-                                       sum_clear, w->sum_words);
+static void launch_decide(gi_graph *g, FusedWork *w, int32_t level, hipStream_t s) {
+    (void)level;  // the kernel reads the level from the device state
+    SumSet sums{{w->sum[0], w->sum[1], w->sum[2]}};
+    bfs_decide_kernel<<<1, 64, 0, s>>>(w->st, w->d_sel, g->V, g->E, w->pcount, w->pfdeg,
+                                       step_grid(w), sums, w->sum_words);
 }
 
-static int seed_run(gi_graph *g, FusedWork *w, int32_t source, int32_t *depth, int32_t *parent,
-                    const SelectorView &sel, hipStream_t s) {
+static void launch_level(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t level,
+                         hipStream_t s, hipEvent_t *ev) {
+    int grid = step_grid(w);
+    if (ev) (void)hipEventRecord(ev[0], s);
+    bfs_step_kernel<<<grid, kStepBlock, 0, s>>>(a);
+    if (ev) (void)hipEventRecord(ev[1], s);  // traces time the step kernel alone
+    bfs_hub_kernel<<<grid, kStepBlock, 0, s>>>(a);
+    launch_decide(g, w, level, s);
+}
+
+// Seed: clear the arrays, set the state, mark the sources, decide level 0.
+// sources == null seeds the single vertex `single`.
+static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t n_sources,
+                    int32_t single, int32_t *depth, int32_t *parent, const SelectorView &sel,
+                    hipStream_t s) {
     int64_t words = bitmap_words(g->V);
+    GI_HIP(hipMemcpyAsync(w->d_sel, &sel, sizeof(SelectorView), hipMemcpyHostToDevice, s));
     GI_TRY(fill_i32(depth, g->V, -1, s));
     if (parent) GI_TRY(fill_i32(parent, g->V, -1, s));
     GI_TRY(fill_u64(g->d_visited, words, 0, s));
@@ -664,72 +773,144 @@ static int seed_run(gi_graph *g, FusedWork *w, int32_t source, int32_t *depth, i
         GI_TRY(fill_u64(w->bm[i], words, 0, s));
         GI_TRY(fill_u64(w->sum[i], w->sum_words, 0, s));
     }
-    bfs_seed_fused_kernel<<<1, 1, 0, s>>>(w->st, depth, parent, w->bm[0], w->sum[0], g->d_off,
-                                          source, g->V, g->E);
-    launch_decide(g, w, sel, -1, s);
+    SeedArgs sa{depth, parent, w->lane_max, w->hub_min};
+    bfs_state_init_kernel<<<1, 1, 0, s>>>(w->st, sa, g->V, g->E);
+    if (h_sources) {
+        if (n_sources > w->src_cap) {
+            dfree(w->d_src);
+            w->d_src = nullptr;
+            w->src_cap = 0;
+            GI_TRY(dalloc(&w->d_src, (size_t)n_sources));
+            w->src_cap = n_sources;
+        }
+        GI_HIP(hipMemcpyAsync(w->d_src, h_sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice, s));
+        int grid = (int)ceil_div(n_sources, 256);
+        bfs_seed_sources_kernel<<<grid < 1024 ? grid : 1024, 256, 0, s>>>(w->st, w->d_src, n_sources, 0,
+                                                                          w->bm[0], w->sum[0], g->d_off);
+    } else {
+        bfs_seed_sources_kernel<<<1, 64, 0, s>>>(w->st, nullptr, 1, single, w->bm[0], w->sum[0], g->d_off);
+    }
+    launch_decide(g, w, -1, s);
     GI_LAUNCH_CHECK();
+    if (h_sources) GI_HIP(hipStreamSynchronize(s));  // h_sources may go away on return
     return GI_OK;
 }
 
-int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, SelectorView sel,
-                  int32_t *d_depth, int32_t *d_parent, hipStream_t s, FusedTrace *trace,
-                  bool timed, int32_t *levels_out) {
+// A hipGraph of `levels` consecutive levels (step, hub, decide each); every
+// kernel reads its level from the device state, so one capture serves every
+// run on this graph.
+static int level_graph(gi_graph *g, FusedWork *w, int32_t levels, hipGraphExec_t *out) {
+    auto it = w->execs.find(levels);
+    if (it != w->execs.end()) {
+        *out = it->second;
+        return GI_OK;
+    }
+    StepArgs a = step_args(g, w);
+    hipGraph_t graph = nullptr;
+    GI_HIP(hipStreamBeginCapture(w->cap_stream, hipStreamCaptureModeThreadLocal));
+    for (int32_t l = 0; l < levels; ++l) launch_level(g, w, a, l, w->cap_stream, nullptr);
+    hipError_t e = hipStreamEndCapture(w->cap_stream, &graph);
+    if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    w->execs[levels] = exec;
+    *out = exec;
+    return GI_OK;
+}
+
+static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trace, bool timed) {
+    StepArgs a = step_args(g, w);
+    int32_t *done_flag = &w->st->done;
+    if (trace && timed) {
+        // per-level launches with events around each step
+        for (int32_t level = 0; level < kBfsMaxLevels; ++level) {
+            hipEvent_t e0, e1;
+            GI_HIP(hipEventCreate(&e0));
+            GI_HIP(hipEventCreate(&e1));
+            trace->ev.push_back(e0);
+            trace->ev.push_back(e1);
+            launch_level(g, w, a, level, s, &trace->ev[trace->ev.size() - 2]);
+            GI_LAUNCH_CHECK();
+            GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            GI_HIP(hipStreamSynchronize(s));
+            if (*w->host_flag) return GI_OK;
+        }
+        return fail(GI_ERR_RANGE, "BFS deeper than %d levels", kBfsMaxLevels);
+    }
+    // Graph mode: replay a captured block of levels sized by the previous run
+    // on this graph, check `done` once per replay, continue with longer
+    // blocks for deep graphs (levels past the end are no-op launches).
+    int32_t block = w->last_levels > 0 ? w->last_levels : 8;
+    if (block > 48) block = 48;
+    int32_t level = 0;
+    while (level < kBfsMaxLevels) {
+        hipGraphExec_t exec = nullptr;
+        GI_TRY(level_graph(g, w, block, &exec));
+        GI_HIP(hipGraphLaunch(exec, s));
+        level += block;
+        GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GI_HIP(hipStreamSynchronize(s));
+        if (*w->host_flag) return GI_OK;
+        block = 48;
+    }
+    return fail(GI_ERR_RANGE, "BFS deeper than %d levels", kBfsMaxLevels);
+}
+
+int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources, int32_t single,
+                        const gi_hybrid_schedule *h, SelectorView sel, int32_t *d_depth,
+                        int32_t *d_parent, hipStream_t s, FusedTrace *trace, bool timed,
+                        int32_t *levels_out) {
     FusedWork *w = nullptr;
     GI_TRY(ensure_fused(g, &w));
     sel.mode_s1 = fused_mode_of(h->s1);
     sel.mode_s2 = fused_mode_of(h->s2);
     sel.threshold = h->threshold;
     if (sel.mode != GI_SELECT_LEARNED) sel.mode = h->selector;
-    GI_TRY(seed_run(g, w, source, d_depth, d_parent, sel, s));
-    StepArgs a = step_args(g, w, d_depth, d_parent);
-    int grid = step_grid();
-    int32_t *done_flag = &w->st->done;
-    int32_t level = 0;
-    for (; level < kBfsMaxLevels; ++level) {
-        hipEvent_t *ev = nullptr;
-        if (trace && timed) {
-            hipEvent_t e0, e1;
-            GI_HIP(hipEventCreate(&e0));
-            GI_HIP(hipEventCreate(&e1));
-            trace->ev.push_back(e0);
-            trace->ev.push_back(e1);
-            ev = &trace->ev[trace->ev.size() - 2];
-            GI_HIP(hipEventRecord(ev[0], s));
-        }
-        bfs_step_kernel<<<grid, kStepBlock, 0, s>>>(a);
-        bfs_hub_kernel<<<grid, kStepBlock, 0, s>>>(a);
-        if (ev) GI_HIP(hipEventRecord(ev[1], s));
-        launch_decide(g, w, sel, level, s);
-        GI_LAUNCH_CHECK();
-        // one 4-byte read per level tells the host whether to go on
-        GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        GI_HIP(hipStreamSynchronize(s));
-        if (*w->host_flag) break;
-    }
-    if (levels_out || trace) {
-        int32_t levels = 0;
-        GI_HIP(hipMemcpyAsync(&levels, &w->st->levels, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        GI_HIP(hipStreamSynchronize(s));
-        if (levels_out) *levels_out = levels;
-        if (trace) {
-            int32_t n = levels < kBfsMaxLevels ? levels : kBfsMaxLevels;
-            trace->modes.resize(n);
-            trace->sizes.resize(n);
-            GI_HIP(hipMemcpy(trace->modes.data(), w->st->hist_mode, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-            GI_HIP(hipMemcpy(trace->sizes.data(), w->st->hist_next, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
-        }
+    GI_TRY(seed_run(g, w, h_sources, n_sources, single, d_depth, d_parent, sel, s));
+    GI_TRY(run_levels(g, w, s, trace, timed));
+    int32_t levels = 0;
+    GI_HIP(hipMemcpyAsync(&levels, &w->st->levels, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GI_HIP(hipStreamSynchronize(s));
+    w->last_levels = levels;
+    if (levels_out) *levels_out = levels;
+    if (trace) {
+        int32_t n = levels < kBfsMaxLevels ? levels : kBfsMaxLevels;
+        trace->modes.resize(n);
+        trace->sizes.resize(n);
+        GI_HIP(hipMemcpy(trace->modes.data(), w->st->hist_mode, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        GI_HIP(hipMemcpy(trace->sizes.data(), w->st->hist_next, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
     }
     return GI_OK;
+}
+
+int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, SelectorView sel,
+                  int32_t *d_depth, int32_t *d_parent, hipStream_t s, FusedTrace *trace,
+                  bool timed, int32_t *levels_out) {
+    return bfs_fused_run_multi(g, nullptr, 0, source, h, sel, d_depth, d_parent, s, trace, timed,
+                               levels_out);
 }
 
 int fused_reached(gi_graph *g, const int32_t *depth, int32_t *out, hipStream_t s) {
     FusedWork *w = nullptr;
     GI_TRY(ensure_fused(g, &w));
     GI_TRY(fill_i32(w->reached, 1, 0, s));
-    int grid = step_grid();
-    count_reached_fused_kernel<<<grid, 256, 0, s>>>(depth, g->V, w->reached);
+    count_reached_fused_kernel<<<step_grid(w), 256, 0, s>>>(depth, g->V, w->reached);
     GI_LAUNCH_CHECK();
     return read_count(w->reached, out, s);
+}
+
+int fused_set_bins(gi_graph *g, int32_t lane_max, int32_t hub_min, int32_t blocks_per_cu) {
+    FusedWork *w = nullptr;
+    GI_TRY(ensure_fused(g, &w));
+    if (lane_max < 1 || lane_max > 4096 || hub_min < 64 || blocks_per_cu < 1 || blocks_per_cu > 8)
+        return fail(GI_ERR_RANGE, "bins out of range");
+    w->lane_max = lane_max;
+    w->hub_min = hub_min;
+    if (w->blocks_per_cu != blocks_per_cu) drop_execs(w);  // grid size is baked into the graphs
+    w->blocks_per_cu = blocks_per_cu;
+    return GI_OK;
 }
 
 // ---------------------------------------------------------------- sweep
@@ -762,6 +943,28 @@ using namespace gi;
 
 extern "C" {
 
+int gi_bfs_set_bins(gi_graph *g, int32_t lane_max, int32_t hub_min, int32_t blocks_per_cu) {
+    if (!g) return fail(GI_ERR_INVALID, "null graph");
+    return fused_set_bins(g, lane_max, hub_min, blocks_per_cu);
+}
+
+int gi_bfs_multi(gi_graph *g, const int32_t *sources, int32_t num_sources,
+                 const gi_hybrid_schedule *h, int32_t *d_depth, int32_t *d_parent,
+                 int32_t *levels, void *stream) {
+    if (!g || !h || !d_depth || (num_sources > 0 && !sources)) return fail(GI_ERR_INVALID, "null argument");
+    if (num_sources < 1) return fail(GI_ERR_RANGE, "no sources");
+    if (!g->d_off) return fail(GI_ERR_UNSUPPORTED, "pass the base graph, not a blocked view");
+    for (int32_t i = 0; i < num_sources; ++i)
+        if (sources[i] < 0 || sources[i] >= g->V) return fail(GI_ERR_RANGE, "source %d out of range", sources[i]);
+    GI_TRY(validate_schedule(&h->s1));
+    GI_TRY(validate_schedule(&h->s2));
+    SelectorView sel{};
+    if (h->selector == GI_SELECT_LEARNED) sel = selector_view();
+    if (!sel.w1) sel.mode = h->selector == GI_SELECT_LEARNED ? GI_SELECT_HEURISTIC : h->selector;
+    return bfs_fused_run_multi(g, sources, num_sources, 0, h, sel, d_depth, d_parent,
+                               (hipStream_t)stream, nullptr, false, levels);
+}
+
 int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow, int32_t reps,
                  float *features, float *times_ms, int32_t max_levels, int32_t *num_levels,
                  void *stream) {
@@ -787,9 +990,9 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
     sel.mode = GI_SELECT_S1;
     sel.mode_s1 = BFS_PUSH_THREAD;
     sel.mode_s2 = BFS_PUSH_THREAD;
-    StepArgs a = step_args(g, w, depth, nullptr);
-    int grid = step_grid();
-    if (rc == GI_OK) rc = seed_run(g, w, source, depth, nullptr, sel, s);
+    int grid = step_grid(w);
+    if (rc == GI_OK) rc = seed_run(g, w, nullptr, 0, source, depth, nullptr, sel, s);
+    StepArgs a = step_args(g, w);
     int32_t nl = 0, prev_mode = BFS_PUSH_THREAD, prev_n = 0;
     while (rc == GI_OK && nl < max_levels) {
         SweepHost h;
@@ -840,7 +1043,7 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
                     best_mode = m;
                 }
             } else {
-                launch_decide(g, w, sel, h.level, s);
+                launch_decide(g, w, h.level, s);
                 GI_LAUNCH_CHECK();
                 prev_mode = run;
             }

@@ -9,6 +9,7 @@
 // scan).
 #include <cstdarg>
 #include <mutex>
+#include <vector>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
@@ -850,3 +851,84 @@ int gi_timer_stop(void *stream, float *ms) {
 }
 
 }  // extern "C"
+
+// ====================================================================== batch of small graphs
+namespace gi {
+
+// Vertex count of graph i: piecewise-linear log-uniform over the octaves of
+// [vmin, vmax] from one 32-bit draw, integer-only so host and oracle agree.
+static int32_t batch_graph_size(uint64_t seed, int32_t i, int32_t vmin, int32_t vmax) {
+    int32_t octaves = 0;
+    while (((int64_t)vmin << (octaves + 1)) <= vmax) ++octaves;
+    if (octaves == 0) return vmin;
+    uint32_t a = uniform_below(hash64(seed, 7, (uint64_t)i), (uint32_t)octaves * 65536u);
+    int64_t lo = (int64_t)vmin << (a / 65536u);
+    int64_t v = lo + (lo * (int64_t)(a % 65536u)) / 65536;
+    return (int32_t)(v > vmax ? vmax : v);
+}
+
+__global__ void batch_keys_kernel(const int64_t *pair_start, const int32_t *vbase,
+                                  const int32_t *vcount, int32_t G, int64_t M, uint64_t seed,
+                                  int b, uint64_t *keys) {
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < M; e += stride) {
+        int32_t lo = 0, hi = G;  // last graph whose first pair <= e
+        while (hi - lo > 1) {
+            int32_t mid = (lo + hi) / 2;
+            if (pair_start[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        uint32_t n = (uint32_t)vcount[lo];
+        uint64_t u = (uint64_t)vbase[lo] + uniform_below(hash64(seed, 8, 2 * (uint64_t)e), n);
+        uint64_t v = (uint64_t)vbase[lo] + uniform_below(hash64(seed, 8, 2 * (uint64_t)e + 1), n);
+        keys[e] = (u << b) | v;
+        keys[M + e] = (v << b) | u;
+    }
+}
+
+}  // namespace gi
+
+extern "C" int gi_graph_uniform_batch(int32_t num_graphs, int32_t vmin, int32_t vmax,
+                                      int32_t avg_degree, uint64_t seed, int32_t *vertex_offsets,
+                                      gi_graph **out) {
+    if (!out || !vertex_offsets) return fail(GI_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (num_graphs < 1 || vmin < 1 || vmax < vmin || avg_degree < 0)
+        return fail(GI_ERR_RANGE, "bad batch parameters");
+    std::vector<int32_t> vbase(num_graphs), vcount(num_graphs);
+    std::vector<int64_t> pstart(num_graphs);
+    int64_t V = 0, M = 0;
+    for (int32_t i = 0; i < num_graphs; ++i) {
+        int32_t n = batch_graph_size(seed, i, vmin, vmax);
+        vbase[i] = (int32_t)V;
+        vcount[i] = n;
+        vertex_offsets[i] = (int32_t)V;
+        pstart[i] = M;
+        V += n;
+        M += (int64_t)n * avg_degree / 2;
+        if (V >= (1ll << 31) - 1) return fail(GI_ERR_RANGE, "batch exceeds int32 vertex ids");
+    }
+    vertex_offsets[num_graphs] = (int32_t)V;
+    GI_TRY(check_sizes(V, 2 * M));
+    int b = bits_for(V);
+    uint64_t *keys = nullptr;
+    int64_t *d_ps = nullptr;
+    int32_t *d_vb = nullptr, *d_vc = nullptr;
+    GI_TRY(dalloc(&keys, (size_t)(2 * M) + 1));
+    int rc = dalloc(&d_ps, (size_t)num_graphs);
+    if (rc == GI_OK) rc = dalloc(&d_vb, (size_t)num_graphs);
+    if (rc == GI_OK) rc = dalloc(&d_vc, (size_t)num_graphs);
+    if (rc == GI_OK && (hipMemcpy(d_ps, pstart.data(), sizeof(int64_t) * num_graphs, hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(d_vb, vbase.data(), sizeof(int32_t) * num_graphs, hipMemcpyHostToDevice) != hipSuccess ||
+                        hipMemcpy(d_vc, vcount.data(), sizeof(int32_t) * num_graphs, hipMemcpyHostToDevice) != hipSuccess))
+        rc = fail(GI_ERR_HIP, "copy batch layout");
+    if (rc == GI_OK && M) {
+        batch_keys_kernel<<<grid_for(M), 256>>>(d_ps, d_vb, d_vc, num_graphs, M, seed, b, keys);
+        if (hipGetLastError() != hipSuccess) rc = fail(GI_ERR_HIP, "batch keys kernel");
+    }
+    dfree(d_ps);
+    dfree(d_vb);
+    dfree(d_vc);
+    if (rc != GI_OK) { dfree(keys); return rc; }
+    return finish_build(new gi_graph(), (int32_t)V, keys, 2 * M, b, 0, out);
+}

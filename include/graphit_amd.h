@@ -361,6 +361,29 @@ int gi_selector_loss_grad(const float *X, const float *T, const float *M, int32_
 int gi_selector_train(const float *X, const float *T, const float *M, int32_t batch,
                       int32_t n_out, int32_t epochs, float lr, float weight_decay,
                       float *params, float *loss_history);
+/* Bin thresholds of the fused push steps (chosen by the project's autotuner,
+ * easy_graphit_amd/autotune.py): vertices with degree < lane_max are walked
+ * by one lane, wider ones by a wave, and in TWC mode those with degree >=
+ * hub_min by the whole grid; blocks_per_cu sets the persistent grid
+ * (1..8 workgroups of 256 per CU). Per graph; no reference counterpart. */
+int gi_bfs_set_bins(gi_graph *g, int32_t lane_max, int32_t hub_min, int32_t blocks_per_cu);
+/* Multi-source BFS with the fused engine (kernel_fusion must be ENABLED in
+ * h->s1): d_depth[v] = distance to the nearest source. Over a graph that is
+ * a disjoint union with one source per component (many small graphs packed
+ * in one CSR) this runs every component's BFS in the same launches. The level
+ * loop is a captured hipGraph replayed until the frontier is empty. The
+ * reference only seeds one source (apps/bfs.cpp:49-52). */
+int gi_bfs_multi(gi_graph *g, const int32_t *sources, int32_t num_sources,
+                 const gi_hybrid_schedule *h, int32_t *d_depth, int32_t *d_parent,
+                 int32_t *levels, void *stream);
+/* Many small uniform random graphs packed as one symmetric CSR:
+ * num_graphs graphs, vertex counts log-uniform in [vmin, vmax], avg_degree
+ * edge entries per vertex (avg_degree/2 undirected pairs, squished). Vertex
+ * id ranges are contiguous; vertex_offsets[num_graphs+1] (host) receives
+ * them. */
+int gi_graph_uniform_batch(int32_t num_graphs, int32_t vmin, int32_t vmax, int32_t avg_degree,
+                           uint64_t seed, int32_t *vertex_offsets, gi_graph **out);
+
 /* Dataset recorder: a BFS from source where every level is run once per
  * mode of mode_mask from the same snapshot and timed (min of reps); the run
  * then continues with `follow` (or the fastest mode when follow < 0).

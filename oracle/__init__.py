@@ -49,6 +49,8 @@ def lib():
             "or_grid_edges": (i64, [C.c_int, i64, u64, i32p, i32p]),
             "or_uniform_edges": (None, [i32, i64, u64, i32p, i32p]),
             "or_edge_weight": (i32, [u64, i32, i32, i32, i32]),
+            "or_uniform_batch_edges": (i64, [i32, i32, i32, i32, u64, i32p, C.c_void_p,
+                                             C.c_void_p]),
             "or_build_csr": (i64, [i32, i64, i32p, i32p, C.c_int, i32p, i32p]),
             "or_fill_weights": (None, [i32, i32p, i32p, u64, i32, i32, i32p]),
             "or_transpose": (None, [i32, i32p, i32p, C.c_void_p, i32p, i32p, C.c_void_p]),
@@ -117,6 +119,17 @@ def uniform_edges(num_vertices: int, num_edges: int, seed: int):
     d = np.empty(num_edges, np.int32)
     lib().or_uniform_edges(num_vertices, num_edges, seed, s, d)
     return s, d
+
+
+def uniform_batch_edges(num_graphs: int, vmin: int, vmax: int, avg_degree: int, seed: int):
+    """Packed small uniform graphs: (vertex_offsets[G+1], src, dst)."""
+    offs = np.empty(num_graphs + 1, np.int32)
+    m = lib().or_uniform_batch_edges(num_graphs, vmin, vmax, avg_degree, seed, offs, None, None)
+    s = np.empty(m, np.int32)
+    d = np.empty(m, np.int32)
+    lib().or_uniform_batch_edges(num_graphs, vmin, vmax, avg_degree, seed, offs,
+                                 s.ctypes.data, d.ctypes.data)
+    return offs, s, d
 
 
 class CSR:

@@ -141,6 +141,39 @@ void or_uniform_edges(int32_t num_vertices, int64_t num_edges, uint64_t seed,
     }
 }
 
+/* Many small uniform graphs packed with contiguous id ranges: graph i has
+ * n_i vertices (piecewise-linear log-uniform over the octaves of
+ * [vmin, vmax] from one 32-bit draw of stream 7) and n_i*avg_degree/2
+ * uniform pairs inside its range (stream 8). Writes vertex_offsets[G+1];
+ * returns the number of pairs written (src/dst must hold sum n_i*avg/2). */
+int64_t or_uniform_batch_edges(int32_t num_graphs, int32_t vmin, int32_t vmax,
+                               int32_t avg_degree, uint64_t seed, int32_t *vertex_offsets,
+                               int32_t *src, int32_t *dst) {
+    int32_t octaves = 0;
+    while (((int64_t)vmin << (octaves + 1)) <= vmax) ++octaves;
+    int64_t V = 0, e = 0;
+    for (int32_t i = 0; i < num_graphs; ++i) {
+        int64_t n = vmin;
+        if (octaves > 0) {
+            uint32_t a = uniform_below(or_hash64(seed, 7, (uint64_t)i), (uint32_t)octaves * 65536u);
+            int64_t lo = (int64_t)vmin << (a / 65536u);
+            n = lo + (lo * (int64_t)(a % 65536u)) / 65536;
+            if (n > vmax) n = vmax;
+        }
+        vertex_offsets[i] = (int32_t)V;
+        int64_t m = n * avg_degree / 2;
+        for (int64_t k = 0; k < m; ++k, ++e) {
+            if (src) {
+                src[e] = (int32_t)(V + uniform_below(or_hash64(seed, 8, 2 * (uint64_t)e), (uint32_t)n));
+                dst[e] = (int32_t)(V + uniform_below(or_hash64(seed, 8, 2 * (uint64_t)e + 1), (uint32_t)n));
+            }
+        }
+        V += n;
+    }
+    vertex_offsets[num_graphs] = (int32_t)V;
+    return e;
+}
+
 /* Weight of the undirected edge {u,v}, uniform in [wmin, wmax]; a function of
  * the endpoints only, so both directions and every duplicate agree
  * (the role of InsertWeights, generator.h:124-139). */

@@ -29,6 +29,9 @@ int64_t or_grid_edges(int width, int64_t num_random, uint64_t seed,
                       int32_t *src, int32_t *dst);
 void or_uniform_edges(int32_t num_vertices, int64_t num_edges, uint64_t seed,
                       int32_t *src, int32_t *dst);
+int64_t or_uniform_batch_edges(int32_t num_graphs, int32_t vmin, int32_t vmax,
+                               int32_t avg_degree, uint64_t seed, int32_t *vertex_offsets,
+                               int32_t *src, int32_t *dst);
 int32_t or_edge_weight(uint64_t seed, int32_t u, int32_t v, int32_t wmin, int32_t wmax);
 
 /* ---- CSR construction (GAPBS builder semantics) ----------------------- */

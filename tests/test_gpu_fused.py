@@ -142,3 +142,41 @@ def test_learned_selector_outputs_stay_exact(gi, c1):
             assert np.array_equal(depth, O.bfs_depth(host, src))
     finally:
         gi.selector_clear()
+
+
+def test_multi_source_bfs_matches_oracle(gi, c1):
+    g, host, _ = c1
+    srcs = O.pick_sources(host, 21, 9)
+    h = gi.HybridSchedule(fused(gi, 4), fused(gi, 5), 0.01, gi.SELECT_HEURISTIC)
+    depth, levels = gi.bfs_multi(g, srcs, h)
+    want = O.bfs_multi_depth(host, srcs)
+    assert np.array_equal(depth, want) and levels == want.max() + 1
+
+
+def test_packed_small_graphs_bfs(gi):
+    g, offs = gi.Graph.uniform_batch(64, 1024, 65536, 8, 5)
+    want_offs, s, d = O.uniform_batch_edges(64, 1024, 65536, 8, 5)
+    assert np.array_equal(offs, want_offs)
+    host = O.build_csr(int(offs[-1]), s, d, True)
+    off, nb, _ = g.read_csr()
+    assert np.array_equal(off, host.offsets) and np.array_equal(nb, host.nbrs)
+    srcs = offs[:-1] + (np.arange(64) * 7919) % np.diff(offs)
+    h = gi.HybridSchedule(fused(gi, 4), fused(gi, 5), 0.02, gi.SELECT_HEURISTIC)
+    for bins in [(64, 4096, 8), (16, 1024, 4), (128, 16384, 8)]:
+        g.set_bins(*bins)
+        depth, _ = gi.bfs_multi(g, srcs, h)
+        assert np.array_equal(depth, O.bfs_multi_depth(host, srcs)), bins
+
+
+def test_graph_replay_many_runs_stay_exact(gi, c1):
+    """The level loop is a captured hipGraph replayed per run; every replay,
+    with eager work between them, must give the oracle's depths."""
+    g, host, _ = c1
+    h = gi.HybridSchedule(fused(gi, 4), fused(gi, 5), 0.005, gi.SELECT_HEURISTIC)
+    srcs = O.pick_sources(host, 33, 6)
+    depth = gi.DeviceArray(host.num_vertices, np.int32)
+    for s in srcs:
+        gi.bfs_device(g, int(s), h, depth)
+        got = depth.numpy()
+        _ = gi.bfs(g, int(s), None)  # unfused run in between
+        assert np.array_equal(got, O.bfs_depth(host, int(s)))
