@@ -108,6 +108,9 @@ def main():
     ap.add_argument("--profile-sources", type=int, default=4)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sources", type=int, default=64, help="distinct BFS sources cycled over")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run three device-to-device copies of MIB MiB (HBM counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (the multi-GPU "
@@ -119,11 +122,21 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy << 18  # int32 elements
+        a, b = gi.DeviceArray(n, np.int32), gi.DeviceArray(n, np.int32)
+        for _ in range(3):
+            gi.copy_device(b, a)
+        gi.synchronize()
+        print(json.dumps({"copy_bytes": n * 4, "copies": 3}), flush=True)
+        return
+
     t0 = time.perf_counter()
     g = gi.Graph.rmat(args.scale, args.edge_factor, SEED, True)
     V, E = g.num_vertices, g.num_edges
     log(f"graph RMAT-{args.scale} sym: V={V} E={E} built in {time.perf_counter() - t0:.2f}s")
-    sources = g.pick_sources(SEED, 64)
+    sources = g.pick_sources(SEED, 64)[: max(1, min(64, args.sources))]
+    ns = len(sources)
     sched = schedule(gi, args.threshold, args.push_lb, args.pull_lb, args.engine == "fused")
     depth = gi.DeviceArray(V, np.int32)
 
@@ -134,15 +147,15 @@ def main():
         gi.bfs_device(g, int(src), sched, depth)
         edges_of[int(src)] = g.reached_edges(depth)
         if i % 16 == 15:
-            log(f"reference runs {i + 1}/64 ({time.perf_counter() - t0:.1f}s)")
+            log(f"reference runs {i + 1}/{ns} ({time.perf_counter() - t0:.1f}s)")
     for k in range(args.warmup):
-        gi.bfs_device(g, int(sources[k % 64]), sched, depth)
+        gi.bfs_device(g, int(sources[k % ns]), sched, depth)
     gi.synchronize()
 
     t0 = time.perf_counter()
     total_edges = 0
     for k in range(args.steps):
-        src = int(sources[k % 64])
+        src = int(sources[k % ns])
         gi.bfs_device(g, src, sched, depth)
         total_edges += edges_of[src]
     gi.synchronize()
@@ -150,7 +163,7 @@ def main():
     gteps = total_edges / 2 / dt / 1e9
     log(f"{args.steps} BFS in {dt * 1e3:.1f} ms -> {gteps:.2f} GTEPS")
 
-    roof = roofline(gi, g, sched, sources[: args.profile_sources], depth)
+    roof = roofline(gi, g, sched, sources[: args.profile_sources], depth) if args.profile_sources > 0 else None
     log(f"roofline: {json.dumps(roof)}")
     cpu = None
     if not args.no_cpu:

@@ -277,6 +277,14 @@ def synchronize(stream=None):
     check(lib().gi_stream_synchronize(stream))
 
 
+def copy_device(dst: DeviceArray, src: DeviceArray, stream=None):
+    """Device-to-device copy of src into dst (equal byte sizes)."""
+    nbytes = src.n * src.dtype.itemsize
+    if dst.n * dst.dtype.itemsize != nbytes:
+        raise ValueError("copy_device: sizes differ")
+    check(lib().gi_memcpy_d2d(dst.ptr, src.ptr, nbytes, stream))
+
+
 # --------------------------------------------------------------- graph
 class Graph:
     """Device CSR graph (GraphT<int> + load_graph, runtime/infra_gpu/graph.h)."""

@@ -15,7 +15,7 @@ enum BfsMode : int32_t {
     BFS_PUSH_CM = 3,      // CM push (workgroup merge-path)
     BFS_PUSH_TWC = 4,     // TWC push with a grid-wide bin for hubs
     BFS_PULL_THREAD = 5,  // pull, lane per vertex, bitmap frontier
-    BFS_PULL_GROUP = 6,   // pull, 8 lanes per vertex, bitmap frontier
+    BFS_PULL_GROUP = 6,   // pull, wave-cooperative over the unresolved vertices of a word
     BFS_NUM_MODES = 7
 };
 
@@ -40,7 +40,7 @@ struct BfsState {
     int32_t hub_count;              // deferred push vertices of the current step
     int32_t lane_max;               // push: degree < lane_max walked by one lane
     int32_t hub_min;                // push TWC: degree >= hub_min goes to the grid bin
-    int32_t pad0;
+    int32_t want_fdeg;              // accumulate frontier degrees (learned selector, sweeps)
     int32_t *depth;                 // output arrays of the current run
     int32_t *parent;
     int32_t hubs[kMaxHubs];

@@ -7,18 +7,24 @@
 // frontier size (operators.cpp:526). Here each level is three launches on one
 // stream: a persistent step kernel that switches on the chosen mode, a hub
 // kernel that spreads the edges of very wide push vertices over the whole
-// grid (a no-op on most levels), and a one-wave decide kernel (GraphIt's
+// grid (a no-op on most levels), and a one-workgroup decide kernel (GraphIt's
 // hybrid rule or the MLP selector, from device counters).
 //
 // Frontiers are bitmaps only (one 64-bit word per wavefront of vertices),
 // each with a summary bitmap of its non-empty 64-word chunks:
-//   * a step claims discoveries in the OUTPUT bitmap (one atomicOr per new
-//     vertex) and filters against visited | input; the input bitmap is folded
-//     into `visited` by plain stores during the same step (the thread that
-//     folds word w is its only writer), so no vertex can be claimed twice;
+//   * `visited` holds every vertex claimed so far (and, from the start, the
+//     vertices no edge can reach, plus the padding bits past V). A push step
+//     tests it with one plain load per edge and claims with one atomicOr, so
+//     only edges to still-unclaimed vertices cost an atomic; the winner then
+//     sets the vertex in the OUTPUT bitmap;
 //   * push steps visit only the non-empty chunks of the input (from its
 //     summary), handed to workgroups round-robin, and list their vertices in
 //     LDS — no global queue, no shared cursor;
+//   * pull steps give each wave 64-vertex words; a vertex first probes one
+//     precomputed candidate parent (its widest in-neighbour, stored
+//     vertex-major so the probe is a coalesced read), then scans its
+//     in-neighbours lane-serially (PULL_THREAD) or cooperatively with the
+//     other unresolved vertices of its word (PULL_GROUP);
 //   * |F'| and the sum of its degrees are reduced per workgroup into a slot
 //     of a partial-sum array (plain stores) that the decide kernel adds up,
 //     so no counter is shared by the whole grid.
@@ -34,13 +40,18 @@ __device__ __forceinline__ int32_t deg_of(const int32_t *off, int32_t v) {
 }
 
 constexpr int kStepBlock = 256;
+constexpr int kDecideBlock = 256;
 constexpr int kChunkWords = 64;                  // chunk: 64 words = 4096 vertices
 constexpr int kChunkVerts = kChunkWords * 64;
+constexpr int kCandScan = 128;                   // in-neighbours examined per candidate
 
 struct StepArgs {
     BfsState *st;
-    const int32_t *off;
+    const int32_t *off;         // out-edges (push)
     const int32_t *nbr;
+    const int32_t *poff;        // in-edges (pull); the same arrays on a symmetric graph
+    const int32_t *pnbr;
+    const int32_t *cand;        // candidate parent per vertex, -1 when none
     int32_t V;
     int32_t words;
     int32_t chunks;
@@ -58,10 +69,10 @@ struct StepCtx {
     int32_t level;
     int32_t lane_max;          // degree < lane_max: one lane walks the vertex
     int32_t hub_min;           // TWC: degree >= hub_min goes to the grid bin
+    int32_t want_fdeg;
     const uint64_t *bm_in;
     uint64_t *bm_out;
     uint64_t *sum_out;
-    const uint64_t *visited;
     int32_t cnt;               // per-lane |F'| contribution
     unsigned long long fdeg;   // per-lane sum of degrees of F'
 };
@@ -73,9 +84,8 @@ __device__ __forceinline__ T pick3(T p0, T p1, T p2, int32_t i) {
     return i == 0 ? p0 : (i == 1 ? p1 : p2);
 }
 
-__device__ __forceinline__ bool seen(const StepCtx &c, int32_t v) {
-    uint64_t bit = 1ull << (v & 63);
-    return ((c.visited[v >> 6] | c.bm_in[v >> 6]) & bit) != 0;
+__device__ __forceinline__ bool in_bitmap(const uint64_t *bm, int32_t v) {
+    return (bm[v >> 6] >> (v & 63)) & 1ull;
 }
 
 __device__ __forceinline__ void mark_chunk(uint64_t *sum, int32_t word) {
@@ -84,20 +94,21 @@ __device__ __forceinline__ void mark_chunk(uint64_t *sum, int32_t word) {
     if (!(sum[ch >> 6] & bit)) atomicOr((unsigned long long *)&sum[ch >> 6], bit);
 }
 
-// Push discovery: filter, claim in the output bitmap, record.
+// Push discovery: test, claim in `visited`, record in the output bitmap.
 __device__ __forceinline__ void push_edge(const StepArgs &a, StepCtx &c, int32_t s, int32_t e,
                                           bool active) {
     if (!active) return;
     int32_t d = a.nbr[e];
-    if (seen(c, d)) return;
     uint64_t bit = 1ull << (d & 63);
-    unsigned long long old = atomicOr((unsigned long long *)&c.bm_out[d >> 6], bit);
+    uint64_t *vw = a.visited + (d >> 6);
+    if (*vw & bit) return;  // a stale word only costs the atomic below
+    unsigned long long old = atomicOr((unsigned long long *)vw, bit);
     if (old & bit) return;
-    if (old == 0) mark_chunk(c.sum_out, d >> 6);
-    a.depth[d] = c.level + 1;
-    if (a.parent) a.parent[d] = s;
+    unsigned long long o2 = atomicOr((unsigned long long *)&c.bm_out[d >> 6], bit);
+    if (o2 == 0) mark_chunk(c.sum_out, d >> 6);
+    if (a.parent) a.parent[d] = s;  // the depth is written in word order by the next step
     c.cnt += 1;
-    c.fdeg += (unsigned long long)deg_of(a.off, d);
+    if (c.want_fdeg) c.fdeg += (unsigned long long)deg_of(a.off, d);
 }
 
 __device__ __forceinline__ void lane_walk(const StepArgs &a, StepCtx &c, int32_t s, int32_t e,
@@ -240,6 +251,130 @@ __device__ void push_batch(const StepArgs &a, StepCtx &c, int32_t mode, const in
     }
 }
 
+// 64-bit OR over the wave.
+__device__ __forceinline__ uint64_t wave_or(uint64_t x) {
+    for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o);
+    return x;
+}
+
+constexpr int kPullWords = 4;     // words a wave works on at once (memory-level parallelism)
+constexpr int kPullUnit = kPullWords * (kStepBlock / 64);  // words per workgroup unit
+static_assert(kChunkWords % kPullUnit == 0, "a pull unit lies inside one chunk");
+constexpr int kPullLaneSteps = 8; // PULL_GROUP: lane-serial edges before the cooperative scan
+
+// Cooperative scan of the open vertices of one word: their remaining in-edge
+// lists [e, e1) concatenated, 64 edges per pass, until every open vertex has
+// a parent in the frontier or its list is exhausted. Returns the resolved
+// lanes.
+__device__ __forceinline__ uint64_t pull_coop(const StepArgs &a, const StepCtx &c, int32_t w,
+                                              int32_t e, int32_t e1) {
+    const int32_t lane = lane_id();
+    const int32_t len = e1 > e ? e1 - e : 0;
+    int32_t incl = len;
+    for (int o = 1; o < 64; o <<= 1) {
+        int32_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    const int32_t total = __shfl(incl, 63);
+    const int32_t pre = incl - len;
+    const uint64_t open_mask = __ballot(len > 0);
+    uint64_t res = 0;
+    for (int32_t base = 0; base < total; base += 64) {
+        const int32_t g = base + lane;
+        // owner: the last lane whose exclusive prefix is <= g
+        int32_t j = 0;
+        for (int32_t step = 32; step > 0; step >>= 1) {
+            int32_t t = j + step;
+            int32_t pt = __shfl(pre, t & 63);
+            if (t < 64 && pt <= g) j = t;
+        }
+        const int32_t ej = __shfl(e, j), pj = __shfl(pre, j);
+        bool hit = false;
+        int32_t u = 0;
+        if (g < total && !((res >> j) & 1ull)) {
+            u = a.pnbr[ej + (g - pj)];
+            hit = in_bitmap(c.bm_in, u);
+        }
+        if (__ballot(hit)) {
+            if (hit && a.parent) a.parent[w * 64 + j] = u;  // any hit is a valid parent
+            res |= wave_or(hit ? (1ull << j) : 0ull);
+            if ((res | ~open_mask) == ~0ull) break;  // every open vertex resolved
+        }
+    }
+    return res;
+}
+
+// Pull over words [w0, w0 + kPullWords) by one wave; lane l owns vertex
+// 64 w + l of each. The kPullWords vertices of a lane advance together so
+// their loads are in flight at the same time.
+__device__ __forceinline__ bool pull_words(const StepArgs &a, StepCtx &c, int32_t mode, int32_t w0,
+                                           const uint64_t *vis) {
+    const int32_t lane = lane_id();
+    int32_t par[kPullWords], e[kPullWords], e1[kPullWords], cd[kPullWords];
+    // candidate and in-edge range of every lane's vertex, loaded without
+    // waiting for `visited` so the loads overlap (past V: a clamped index,
+    // never used since padding bits are set)
+#pragma unroll
+    for (int i = 0; i < kPullWords; ++i) {
+        int32_t v = (w0 + i) * 64 + lane;
+        v = v < a.V ? v : a.V - 1;
+        cd[i] = a.cand[v];
+        e[i] = a.poff[v];
+        e1[i] = a.poff[v + 1];
+    }
+#pragma unroll
+    for (int i = 0; i < kPullWords; ++i) {
+        const bool want = !((vis[i] >> lane) & 1ull);  // padding bits past V are set
+        par[i] = (want && cd[i] >= 0 && in_bitmap(c.bm_in, cd[i])) ? cd[i] : -1;
+        if (!want || par[i] >= 0) e1[i] = e[i];  // closed: nothing to scan
+    }
+    bool any = false;
+    // lane-serial scan, the lane's vertices interleaved
+    const int32_t steps = mode == BFS_PULL_THREAD ? 0x7fffffff : kPullLaneSteps;
+    for (int32_t it = 0; it < steps; ++it) {
+        bool more = false;
+#pragma unroll
+        for (int i = 0; i < kPullWords; ++i) more |= e[i] < e1[i];
+        if (!__ballot(more)) break;
+        int32_t u[kPullWords];
+#pragma unroll
+        for (int i = 0; i < kPullWords; ++i) u[i] = e[i] < e1[i] ? a.pnbr[e[i]] : -1;
+#pragma unroll
+        for (int i = 0; i < kPullWords; ++i) {
+            if (u[i] >= 0) {
+                if (in_bitmap(c.bm_in, u[i])) {
+                    par[i] = u[i];
+                    e1[i] = e[i];
+                } else {
+                    ++e[i];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kPullWords; ++i) {
+        const int32_t w = w0 + i;
+        uint64_t coop = 0;
+        if (mode != BFS_PULL_THREAD && __ballot(e[i] < e1[i])) coop = pull_coop(a, c, w, e[i], e1[i]);
+        const int32_t v = w * 64 + lane;
+        const bool found_self = par[i] >= 0 || ((coop >> lane) & 1ull);
+        if (found_self) {
+            if (a.parent && par[i] >= 0) a.parent[v] = par[i];
+            if (c.want_fdeg) c.fdeg += (unsigned long long)deg_of(a.off, v);
+        }
+        const uint64_t found = __ballot(found_self);
+        if (found) {
+            any = true;
+            if (lane == 0) {
+                a.visited[w] = vis[i] | found;  // this wave owns word w in a pull step
+                c.bm_out[w] = found;
+                c.cnt += __popcll(found);
+            }
+        }
+    }
+    return any;
+}
+
 // ---------------------------------------------------------------- step kernel
 __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a0) {
     BfsState *st = a0.st;
@@ -252,12 +387,12 @@ __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a0) {
     c.level = st->level;
     c.lane_max = st->lane_max;
     c.hub_min = st->hub_min;
+    c.want_fdeg = st->want_fdeg;
     const int32_t mode = st->mode;
     const int32_t iin = c.level % 3, iout = (c.level + 1) % 3, iclr = (c.level + 2) % 3;
     c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iin);
     c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iout);
     c.sum_out = pick3(a0.sum[0], a0.sum[1], a0.sum[2], iout);
-    c.visited = a.visited;
     c.cnt = 0;
     c.fdeg = 0;
     const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -288,10 +423,9 @@ __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a0) {
         for (int32_t k = blockIdx.x; k < nz; k += gridDim.x) {
             int32_t chunk = kth_chunk(si, pre, a.sum_words, k);
             if (threadIdx.x < 64) {
-                // wave 0 folds the chunk into `visited` and lists its members
+                // wave 0 lists the chunk's members
                 int32_t w = chunk * kChunkWords + (int32_t)threadIdx.x;
                 uint64_t word = w < a.words ? c.bm_in[w] : 0ull;
-                if (word) a.visited[w] |= word;  // only this thread writes word w
                 int32_t pc = __popcll(word);
                 int32_t incl = pc;
                 for (int o = 1; o < 64; o <<= 1) {
@@ -303,6 +437,7 @@ __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a0) {
                     int32_t b = __ffsll((unsigned long long)word) - 1;
                     word &= word - 1;
                     list[pos++] = w * 64 + b;
+                    a.depth[w * 64 + b] = c.level;  // members of F_level get their depth here
                 }
                 if (threadIdx.x == 63) n_s = incl;
             }
@@ -313,73 +448,27 @@ __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a0) {
             __syncthreads();
         }
     } else {
-        // pull over all vertices; waves own 64-vertex words of every bitmap
-        for (int32_t w = tid >> 6; w < a.words; w += threads >> 6) {
-            uint64_t in = c.bm_in[w];
-            uint64_t vis = a.visited[w] | in;
-            if (in && lane == 0) a.visited[w] = vis;  // fold the input frontier
-            if (vis == ~0ull) continue;  // wave-uniform
-            uint64_t found = 0;
-            if (mode == BFS_PULL_THREAD) {
-                int32_t v = w * 64 + lane;
-                bool want = v < a.V && !((vis >> lane) & 1ull);
-                int32_t e = want ? a.off[v] : 0, e1 = want ? a.off[v + 1] : 0;
-                int32_t par = -1;
-                for (; e < e1; ++e) {
-                    int32_t u = a.nbr[e];
-                    if ((c.bm_in[u >> 6] >> (u & 63)) & 1ull) {
-                        par = u;
-                        break;
-                    }
-                }
-                bool hit = par >= 0;
-                found = __ballot(hit);
-                if (hit) {
-                    a.depth[v] = c.level + 1;
-                    if (a.parent) a.parent[v] = par;
-                    c.fdeg += (unsigned long long)(e1 - a.off[v]);
-                }
-            } else {
-                const int32_t sub = lane >> 3, g8 = lane & 7;
-                for (int32_t r = 0; r < 8; ++r) {
-                    int32_t slot = r * 8 + sub;
-                    int32_t v = w * 64 + slot;
-                    bool want = v < a.V && !((vis >> slot) & 1ull);
-                    int32_t e0 = want ? a.off[v] : 0, e1 = want ? a.off[v + 1] : 0;
-                    bool hit = false;
-                    int32_t par = -1;
-                    for (int32_t b = e0; b < e1; b += 8) {  // group-uniform trip count
-                        int32_t e = b + g8;
-                        bool h = false;
-                        int32_t u = 0;
-                        if (e < e1) {
-                            u = a.nbr[e];
-                            h = (c.bm_in[u >> 6] >> (u & 63)) & 1ull;
-                        }
-                        uint32_t gm = (uint32_t)(__ballot(h) >> (sub * 8)) & 0xffu;
-                        if (gm) {
-                            if (g8 == __ffs(gm) - 1) {
-                                hit = true;
-                                par = u;
-                            }
-                            break;
-                        }
-                    }
-                    uint64_t hm = __ballot(hit);
-                    for (int32_t sg = 0; sg < 8; ++sg)
-                        if ((hm >> (sg * 8)) & 0xffull) found |= 1ull << (r * 8 + sg);
-                    if (hit) {
-                        a.depth[v] = c.level + 1;
-                        if (a.parent) a.parent[v] = par;
-                        c.fdeg += (unsigned long long)(e1 - e0);
-                    }
-                }
+        // pull over all vertices in units of kPullUnit words, one unit per
+        // workgroup at a time (each wave takes kPullWords of them), so a
+        // unit's summary bit costs at most one atomic
+        const int32_t units = (a.words + kPullUnit - 1) / kPullUnit;
+        const int32_t wid = threadIdx.x >> 6;
+        for (int32_t un = blockIdx.x; un < units; un += gridDim.x) {
+            const int32_t w0 = un * kPullUnit + wid * kPullWords;
+            uint64_t vis[kPullWords];
+            uint64_t all = ~0ull;
+#pragma unroll
+            for (int i = 0; i < kPullWords; ++i) {
+                const bool in_range = w0 + i < a.words;
+                vis[i] = in_range ? a.visited[w0 + i] : ~0ull;
+                all &= vis[i];
+                // members of F_level get their depth here
+                const uint64_t fin = in_range ? c.bm_in[w0 + i] : 0ull;
+                if ((fin >> lane) & 1ull) a.depth[(w0 + i) * 64 + lane] = c.level;
             }
-            if (found && lane == 0) {
-                c.bm_out[w] = found;  // this wave owns word w of the output
-                mark_chunk(c.sum_out, w);
-                c.cnt += __popcll(found);
-            }
+            bool any = false;
+            if (all != ~0ull) any = pull_words(a, c, mode, w0, vis);  // wave-uniform
+            if (__syncthreads_or(any) && threadIdx.x == 0) mark_chunk(c.sum_out, un * kPullUnit);
         }
     }
     flush_counts(a.pcount, a.pfdeg, c);
@@ -404,7 +493,7 @@ __global__ void __launch_bounds__(kStepBlock) bfs_hub_kernel(StepArgs a0) {
     c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], c.level % 3);
     c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], (c.level + 1) % 3);
     c.sum_out = pick3(a0.sum[0], a0.sum[1], a0.sum[2], (c.level + 1) % 3);
-    c.visited = a.visited;
+    c.want_fdeg = st->want_fdeg;
     c.cnt = 0;
     c.fdeg = 0;
     __shared__ int32_t hscan[kMaxHubs + 1];
@@ -439,32 +528,34 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
     return __uint_as_float(((uint32_t)h) << 16);
 }
 
-// One wave, after step `level` (or after seeding, level = -1): adds up the
-// step's partial sums, closes its bookkeeping, ends the run on an empty
+// One workgroup, after step `level` (or after seeding, level = -1): adds up
+// the step's partial sums, closes its bookkeeping, ends the run on an empty
 // frontier, else picks the next step's mode and clears what the next step
-// writes.
+// writes. The MLP runs on wave 0; every thread passes the same barriers.
 struct SumSet {
     uint64_t *s[3];
 };
 
-__global__ void bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E,
-                                  int32_t *pcount, unsigned long long *pfdeg, int32_t slots,
-                                  SumSet sums, int32_t sum_words) {
-    if (st->done) return;
+__global__ void __launch_bounds__(kDecideBlock)
+bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E, int32_t *pcount,
+                  unsigned long long *pfdeg, int32_t slots, SumSet sums, int32_t sum_words) {
+    if (st->done) return;  // uniform
     const SelectorView sel = *selp;
     // after step `level`, the bitmap two levels ahead of it was cleared by the
     // step; its summary is cleared here
     uint64_t *sum_clear = pick3(sums.s[0], sums.s[1], sums.s[2], (st->level + 2 + 3) % 3);
-    const int32_t lane = lane_id();
+    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
     const int32_t next_level = st->level + 1;
     const int32_t p = next_level & 1;
+    __shared__ int32_t rn[kDecideBlock / 64];
+    __shared__ unsigned long long rf[kDecideBlock / 64];
     int32_t n = 0;
     unsigned long long fdeg = 0;
     if (next_level == 0) {  // seeded frontier
         n = st->count[0];
         fdeg = st->fdeg[0];
     } else {
-        for (int32_t i = lane; i < slots; i += 64) {
+        for (int32_t i = threadIdx.x; i < slots; i += kDecideBlock) {
             n += pcount[i] + pcount[kMaxGrid + i];
             fdeg += pfdeg[i] + pfdeg[kMaxGrid + i];
             pcount[i] = 0;
@@ -476,7 +567,18 @@ __global__ void bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_
             n += __shfl_xor(n, o);
             fdeg += __shfl_xor(fdeg, o);
         }
-        for (int32_t i = lane; i < sum_words; i += 64) sum_clear[i] = 0;
+        if (lane == 0) {
+            rn[wid] = n;
+            rf[wid] = fdeg;
+        }
+        for (int32_t i = threadIdx.x; i < sum_words; i += kDecideBlock) sum_clear[i] = 0;
+        __syncthreads();
+        n = 0;
+        fdeg = 0;
+        for (int i = 0; i < kDecideBlock / 64; ++i) {
+            n += rn[i];
+            fdeg += rf[i];
+        }
     }
     const int32_t prev_n = next_level > 0 ? st->count[p ^ 1] : 0;
     const long long unv = st->unvisited - n;
@@ -485,36 +587,41 @@ __global__ void bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_
     __shared__ float h1[kSelHidden];
     __shared__ float h2[kSelHidden];
     __shared__ float y[kSelMaxOut];
-    if (n == 0) {
-        if (lane == 0) {
+    if (n == 0) {  // uniform
+        if (threadIdx.x == 0) {
             st->done = 1;
             st->levels = next_level;
         }
         return;
     }
     int32_t mode;
-    bool learned = sel.mode == GI_SELECT_LEARNED && sel.w1 != nullptr;
+    const bool learned = sel.mode == GI_SELECT_LEARNED && sel.w1 != nullptr;
+    const bool w0 = threadIdx.x < 64;
     if (!learned) {
         if (sel.mode == GI_SELECT_S1) mode = sel.mode_s1;
         else if (sel.mode == GI_SELECT_S2) mode = sel.mode_s2;
         else mode = ((double)n <= (double)sel.threshold * (double)V) ? sel.mode_s1 : sel.mode_s2;
     } else {
-        if (lane == 0) {
+        if (threadIdx.x == 0) {
             float raw[kSelInputs];
             bfs_features(raw, n, fdeg, unv, unv_e, next_level, st->mode, prev_n, V, E);
             for (int k = 0; k < kSelInputs; ++k) feat[k] = (raw[k] - sel.feat_mean[k]) * sel.feat_scale[k];
         }
         __syncthreads();
-        float acc = bf16_to_f32(sel.b1[lane]);
-        for (int k = 0; k < kSelInputs; ++k) acc += feat[k] * bf16_to_f32(sel.w1[k * kSelHidden + lane]);
-        h1[lane] = acc > 0.f ? acc : 0.f;
+        if (w0) {
+            float acc = bf16_to_f32(sel.b1[lane]);
+            for (int k = 0; k < kSelInputs; ++k) acc += feat[k] * bf16_to_f32(sel.w1[k * kSelHidden + lane]);
+            h1[lane] = acc > 0.f ? acc : 0.f;
+        }
         __syncthreads();
-        acc = bf16_to_f32(sel.b2[lane]);
-        for (int k = 0; k < kSelHidden; ++k) acc += h1[k] * bf16_to_f32(sel.w2[k * kSelHidden + lane]);
-        h2[lane] = acc > 0.f ? acc : 0.f;
+        if (w0) {
+            float acc = bf16_to_f32(sel.b2[lane]);
+            for (int k = 0; k < kSelHidden; ++k) acc += h1[k] * bf16_to_f32(sel.w2[k * kSelHidden + lane]);
+            h2[lane] = acc > 0.f ? acc : 0.f;
+        }
         __syncthreads();
-        if (lane < sel.n_out) {
-            acc = bf16_to_f32(sel.b3[lane]);
+        if (w0 && lane < sel.n_out) {
+            float acc = bf16_to_f32(sel.b3[lane]);
             for (int k = 0; k < kSelHidden; ++k) acc += h2[k] * bf16_to_f32(sel.w3[k * sel.n_out + lane]);
             y[lane] = acc;
         }
@@ -527,7 +634,7 @@ __global__ void bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_
                 mode = j;
             }
     }
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
         st->count[p] = n;
         st->fdeg[p] = fdeg;
         st->unvisited = unv;
@@ -547,24 +654,92 @@ struct SeedArgs {
     int32_t *parent;
     int32_t lane_max;
     int32_t hub_min;
+    int32_t want_fdeg;
 };
 
-__global__ void bfs_state_init_kernel(BfsState *st, SeedArgs sa, int32_t V, int64_t E) {
-    st->count[0] = 0;
-    st->count[1] = 0;
-    st->fdeg[0] = 0;
-    st->fdeg[1] = 0;
-    st->done = 0;
-    st->level = -1;
-    st->levels = 0;
-    st->mode = BFS_PUSH_THREAD;
-    st->unvisited = V;
-    st->unvisited_edges = E;
-    st->hub_count = 0;
-    st->lane_max = sa.lane_max;
-    st->hub_min = sa.hub_min;
-    st->depth = sa.depth;
-    st->parent = sa.parent;
+struct ResetArgs {
+    int32_t *depth;
+    int32_t *parent;
+    uint64_t *visited;
+    const uint64_t *dead;   // vertices without in-edges, and the padding past V
+    uint64_t *bm[3];
+    uint64_t *sum[3];
+    int32_t V;
+    int32_t words;
+    int32_t sum_words;
+};
+
+// Clears every array of a run in one pass and initialises the state.
+__global__ void bfs_reset_kernel(BfsState *st, SeedArgs sa, ResetArgs r, int64_t E) {
+    const int32_t stride = gridDim.x * blockDim.x;
+    const int32_t n = r.V > r.sum_words + 1 ? r.V : r.sum_words + 1;  // words <= V
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (i < r.V) {
+            r.depth[i] = -1;
+            if (r.parent) r.parent[i] = -1;
+        }
+        if (i < r.words) {
+            r.visited[i] = r.dead[i];
+            r.bm[0][i] = 0;
+            r.bm[1][i] = 0;
+            r.bm[2][i] = 0;
+        }
+        if (i <= r.sum_words) {
+            r.sum[0][i] = 0;
+            r.sum[1][i] = 0;
+            r.sum[2][i] = 0;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->count[0] = 0;
+        st->count[1] = 0;
+        st->fdeg[0] = 0;
+        st->fdeg[1] = 0;
+        st->done = 0;
+        st->level = -1;
+        st->levels = 0;
+        st->mode = BFS_PUSH_THREAD;
+        st->unvisited = r.V;
+        st->unvisited_edges = E;
+        st->hub_count = 0;
+        st->lane_max = sa.lane_max;
+        st->hub_min = sa.hub_min;
+        st->want_fdeg = sa.want_fdeg;
+        st->depth = sa.depth;
+        st->parent = sa.parent;
+    }
+}
+
+// Pull-side tables of a graph: the dead mask (no in-edges, or past V) and
+// the candidate parent of each vertex, its in-neighbour of largest out-degree
+// among the first kCandScan (the neighbour most likely to be reached early).
+__global__ void bfs_dead_kernel(const int32_t *poff, int32_t V, int32_t words, uint64_t *dead) {
+    for (int32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += gridDim.x * blockDim.x) {
+        uint64_t m = 0;
+        for (int b = 0; b < 64; ++b) {
+            int32_t v = w * 64 + b;
+            if (v >= V || poff[v + 1] == poff[v]) m |= 1ull << b;
+        }
+        dead[w] = m;
+    }
+}
+
+__global__ void bfs_cand_kernel(const int32_t *poff, const int32_t *pnbr, const int32_t *off,
+                                int32_t V, int32_t *cand) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        int32_t e0 = poff[v], e1 = poff[v + 1];
+        if (e1 - e0 > kCandScan) e1 = e0 + kCandScan;
+        int32_t best = -1, bd = -1;
+        for (int32_t e = e0; e < e1; ++e) {
+            int32_t u = pnbr[e];
+            int32_t d = off[u + 1] - off[u];
+            if (d > bd) {
+                bd = d;
+                best = u;
+            }
+        }
+        cand[v] = best;
+    }
 }
 
 // Seeds one or many sources (a multi-source BFS gives every vertex its
@@ -572,7 +747,7 @@ __global__ void bfs_state_init_kernel(BfsState *st, SeedArgs sa, int32_t V, int6
 // source each it is every graph's own BFS). Repeated sources count once.
 __global__ void bfs_seed_sources_kernel(BfsState *st, const int32_t *src, int32_t n,
                                         int32_t single, uint64_t *bm0, uint64_t *sum0,
-                                        const int32_t *off) {
+                                        uint64_t *visited, const int32_t *off) {
     int32_t cnt = 0;
     unsigned long long fd = 0;
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -580,6 +755,7 @@ __global__ void bfs_seed_sources_kernel(BfsState *st, const int32_t *src, int32_
         uint64_t bit = 1ull << (v & 63);
         unsigned long long old = atomicOr((unsigned long long *)&bm0[v >> 6], bit);
         if (old & bit) continue;
+        atomicOr((unsigned long long *)&visited[v >> 6], bit);
         st->depth[v] = 0;
         if (st->parent) st->parent[v] = v;
         int32_t ch = (v >> 6) / kChunkWords;
@@ -613,6 +789,10 @@ struct FusedWork {
     int32_t *pcount = nullptr;
     unsigned long long *pfdeg = nullptr;
     int32_t *reached = nullptr;
+    uint64_t *dead = nullptr;        // pull-side tables (see bfs_dead_kernel)
+    int32_t *cand = nullptr;
+    const int32_t *poff = nullptr;   // in-edge CSR (the graph itself when symmetric)
+    const int32_t *pnbr = nullptr;
     SelectorView *d_sel = nullptr;   // selector of the current run (device copy)
     int32_t *d_src = nullptr;        // sources of a multi-source run
     int32_t src_cap = 0;
@@ -643,6 +823,8 @@ static void free_work(FusedWork *w) {
     dfree(w->pcount);
     dfree(w->pfdeg);
     dfree(w->reached);
+    dfree(w->dead);
+    dfree(w->cand);
     dfree(w->d_sel);
     dfree(w->d_src);
     if (w->host_flag) (void)hipHostFree(w->host_flag);
@@ -660,14 +842,24 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
     int64_t sum_words = ceil_div(chunks, 64);
     if (sum_words > kMaxSumWords)
         return fail(GI_ERR_RANGE, "fused BFS supports up to 2^26 vertices");
+    const gi_graph *pg = g;  // in-edges for pull steps
+    if (g->directed) {
+        gi_graph *t = nullptr;
+        GI_TRY(gi_graph_transpose(g, &t));
+        pg = t;
+    }
     FusedWork *w = new FusedWork();
     w->sum_words = (int32_t)sum_words;
+    w->poff = pg->d_off;
+    w->pnbr = pg->d_nbr;
     int rc = dalloc(&w->st, 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->bm[i], (size_t)words + 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->sum[i], (size_t)sum_words + 1);
     if (rc == GI_OK) rc = dalloc(&w->pcount, 2 * (size_t)kMaxGrid);
     if (rc == GI_OK) rc = dalloc(&w->pfdeg, 2 * (size_t)kMaxGrid);
     if (rc == GI_OK) rc = dalloc(&w->reached, 1);
+    if (rc == GI_OK) rc = dalloc(&w->dead, (size_t)words + 1);
+    if (rc == GI_OK) rc = dalloc(&w->cand, (size_t)g->V + 1);
     if (rc == GI_OK) rc = dalloc(&w->d_sel, 1);
     if (rc == GI_OK && !g->d_visited) rc = dalloc(&g->d_visited, (size_t)words + 1);
     if (rc == GI_OK && hipHostMalloc((void **)&w->host_flag, sizeof(int32_t) * 4) != hipSuccess)
@@ -676,6 +868,13 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
         rc = fail(GI_ERR_HIP, "stream create failed");
     if (rc == GI_OK) rc = fill_i32(w->pcount, 2 * kMaxGrid, 0, 0);
     if (rc == GI_OK) rc = fill_u64((uint64_t *)w->pfdeg, 2 * kMaxGrid, 0, 0);
+    if (rc == GI_OK) {
+        int grid = (int)ceil_div(g->V, 256);
+        grid = grid < 4096 ? (grid < 1 ? 1 : grid) : 4096;
+        bfs_dead_kernel<<<grid, 256>>>(w->poff, g->V, (int32_t)words, w->dead);
+        bfs_cand_kernel<<<grid, 256>>>(w->poff, w->pnbr, g->d_off, g->V, w->cand);
+        if (hipGetLastError() != hipSuccess) rc = fail(GI_ERR_HIP, "pull table launch failed");
+    }
     if (rc == GI_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(GI_ERR_HIP, "init sync");
     if (rc != GI_OK) {
         free_work(w);
@@ -726,6 +925,9 @@ static StepArgs step_args(gi_graph *g, FusedWork *w) {
     a.st = w->st;
     a.off = g->d_off;
     a.nbr = g->d_nbr;
+    a.poff = w->poff;
+    a.pnbr = w->pnbr;
+    a.cand = w->cand;
     a.V = g->V;
     a.words = (int32_t)bitmap_words(g->V);
     a.chunks = (int32_t)ceil_div(a.words, kChunkWords);
@@ -745,7 +947,7 @@ static StepArgs step_args(gi_graph *g, FusedWork *w) {
 static void launch_decide(gi_graph *g, FusedWork *w, int32_t level, hipStream_t s) {
     (void)level;  // the kernel reads the level from the device state
     SumSet sums{{w->sum[0], w->sum[1], w->sum[2]}};
-    bfs_decide_kernel<<<1, 64, 0, s>>>(w->st, w->d_sel, g->V, g->E, w->pcount, w->pfdeg,
+    bfs_decide_kernel<<<1, kDecideBlock, 0, s>>>(w->st, w->d_sel, g->V, g->E, w->pcount, w->pfdeg,
                                        step_grid(w), sums, w->sum_words);
 }
 
@@ -763,18 +965,24 @@ static void launch_level(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t l
 // sources == null seeds the single vertex `single`.
 static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t n_sources,
                     int32_t single, int32_t *depth, int32_t *parent, const SelectorView &sel,
-                    hipStream_t s) {
-    int64_t words = bitmap_words(g->V);
+                    bool want_fdeg, hipStream_t s) {
     GI_HIP(hipMemcpyAsync(w->d_sel, &sel, sizeof(SelectorView), hipMemcpyHostToDevice, s));
-    GI_TRY(fill_i32(depth, g->V, -1, s));
-    if (parent) GI_TRY(fill_i32(parent, g->V, -1, s));
-    GI_TRY(fill_u64(g->d_visited, words, 0, s));
+    SeedArgs sa{depth, parent, w->lane_max, w->hub_min, want_fdeg ? 1 : 0};
+    ResetArgs r;
+    r.depth = depth;
+    r.parent = parent;
+    r.visited = g->d_visited;
+    r.dead = w->dead;
     for (int i = 0; i < 3; ++i) {
-        GI_TRY(fill_u64(w->bm[i], words, 0, s));
-        GI_TRY(fill_u64(w->sum[i], w->sum_words, 0, s));
+        r.bm[i] = w->bm[i];
+        r.sum[i] = w->sum[i];
     }
-    SeedArgs sa{depth, parent, w->lane_max, w->hub_min};
-    bfs_state_init_kernel<<<1, 1, 0, s>>>(w->st, sa, g->V, g->E);
+    r.V = g->V;
+    r.words = (int32_t)bitmap_words(g->V);
+    r.sum_words = w->sum_words;
+    int grid = (int)ceil_div(g->V > w->sum_words + 1 ? g->V : w->sum_words + 1, 256);
+    grid = grid < 4096 ? grid : 4096;
+    bfs_reset_kernel<<<grid, 256, 0, s>>>(w->st, sa, r, g->E);
     if (h_sources) {
         if (n_sources > w->src_cap) {
             dfree(w->d_src);
@@ -784,11 +992,12 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
             w->src_cap = n_sources;
         }
         GI_HIP(hipMemcpyAsync(w->d_src, h_sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice, s));
-        int grid = (int)ceil_div(n_sources, 256);
-        bfs_seed_sources_kernel<<<grid < 1024 ? grid : 1024, 256, 0, s>>>(w->st, w->d_src, n_sources, 0,
-                                                                          w->bm[0], w->sum[0], g->d_off);
+        int sg = (int)ceil_div(n_sources, 256);
+        bfs_seed_sources_kernel<<<sg < 1024 ? sg : 1024, 256, 0, s>>>(w->st, w->d_src, n_sources, 0, w->bm[0],
+                                                                      w->sum[0], g->d_visited, g->d_off);
     } else {
-        bfs_seed_sources_kernel<<<1, 64, 0, s>>>(w->st, nullptr, 1, single, w->bm[0], w->sum[0], g->d_off);
+        bfs_seed_sources_kernel<<<1, 64, 0, s>>>(w->st, nullptr, 1, single, w->bm[0], w->sum[0], g->d_visited,
+                                                 g->d_off);
     }
     launch_decide(g, w, -1, s);
     GI_LAUNCH_CHECK();
@@ -868,7 +1077,8 @@ int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources
     sel.mode_s2 = fused_mode_of(h->s2);
     sel.threshold = h->threshold;
     if (sel.mode != GI_SELECT_LEARNED) sel.mode = h->selector;
-    GI_TRY(seed_run(g, w, h_sources, n_sources, single, d_depth, d_parent, sel, s));
+    GI_TRY(seed_run(g, w, h_sources, n_sources, single, d_depth, d_parent, sel,
+                    sel.mode == GI_SELECT_LEARNED, s));
     GI_TRY(run_levels(g, w, s, trace, timed));
     int32_t levels = 0;
     GI_HIP(hipMemcpyAsync(&levels, &w->st->levels, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -991,7 +1201,7 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
     sel.mode_s1 = BFS_PUSH_THREAD;
     sel.mode_s2 = BFS_PUSH_THREAD;
     int grid = step_grid(w);
-    if (rc == GI_OK) rc = seed_run(g, w, nullptr, 0, source, depth, nullptr, sel, s);
+    if (rc == GI_OK) rc = seed_run(g, w, nullptr, 0, source, depth, nullptr, sel, true, s);
     StepArgs a = step_args(g, w);
     int32_t nl = 0, prev_mode = BFS_PUSH_THREAD, prev_n = 0;
     while (rc == GI_OK && nl < max_levels) {
