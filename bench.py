@@ -35,18 +35,42 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def schedule(gi, threshold: float, push_lb: int, pull_lb: int, fused: bool):
+def schedule(gi, threshold: float, push_lb: int, pull_lb: int, fused: bool,
+             selector: int | None = None):
+    """GraphIt's hybrid schedule: s1 = push (TWCE in the reference BFS app,
+    apps/bfs.cpp:69-80), s2 = pull on a bitmap, s2 when |F| > t*V."""
     f = 1 if fused else 0
     s1 = gi.Schedule(direction=gi.PUSH, load_balancing=push_lb, frontier_creation=gi.FC_SPARSE,
                      cta_size=256, kernel_fusion=f)
     s2 = gi.Schedule(direction=gi.PULL, load_balancing=pull_lb, pull_frontier_rep=gi.PULL_BITMAP,
                      frontier_creation=gi.FC_BITMAP, cta_size=256, kernel_fusion=f)
-    return gi.HybridSchedule(s1, s2, threshold, gi.SELECT_HEURISTIC)
+    return gi.HybridSchedule(s1, s2, threshold,
+                             gi.SELECT_HEURISTIC if selector is None else selector)
+
+
+def time_bfs(gi, g, sched, sources, depth, edges_of, steps):
+    """Wall time of `steps` BFS runs cycling over `sources`; GTEPS."""
+    gi.synchronize()
+    t0 = time.perf_counter()
+    total_edges = 0
+    for k in range(steps):
+        src = int(sources[k % len(sources)])
+        gi.bfs_device(g, src, sched, depth)
+        total_edges += edges_of[src]
+    gi.synchronize()
+    dt = time.perf_counter() - t0
+    return dt, total_edges / 2 / dt / 1e9
 
 
 def roofline(gi, g, sched, sources, depth):
-    """Per-kernel (direction, load balancing) totals over profiled runs."""
-    kern = {}
+    """Dominant step kernel of the traced BFS runs from `sources`.
+
+    Traced runs launch one step kernel per level between two HIP events on
+    the BFS stream (bfs_push_step_traced or bfs_pull_step_traced, the same
+    code as the captured kernels under a symbol of their own, so a rocprof
+    summary of this command lists exactly these launches). achieved =
+    algorithmic bytes per launch / average launch duration."""
+    kern, modes = {}, {}
     for i, src in enumerate(sources):
         recs = gi.bfs_profile(g, int(src), sched, depth)
         if i == 0:
@@ -55,25 +79,31 @@ def roofline(gi, g, sched, sources, depth):
                     f"|F|={r['frontier_size']} |F'|={r['next_size']} {r['kernel_ms']:.3f} ms "
                     f"{r['alg_bytes'] / max(r['kernel_ms'], 1e-9) / 1e6:.1f} GB/s")
         for r in recs:
+            pull = r["direction"] == gi.PULL
+            name = "bfs_pull_step_traced" if pull else "bfs_push_step_traced"
             lb = r["load_balancing"]
-            name = gi.BFS_MODE_NAMES[lb - 100] if lb >= 100 else gi.LB_NAMES[lb]
-            key = ("pull" if r["direction"] == gi.PULL else "push", name)
-            k = kern.setdefault(key, {"ms": 0.0, "bytes": 0.0, "launches": 0})
-            k["ms"] += r["kernel_ms"]
-            k["bytes"] += r["alg_bytes"]
-            k["launches"] += 1
-    dom_key = max(kern, key=lambda k: kern[k]["ms"])
-    d = kern[dom_key]
+            mode = gi.BFS_MODE_NAMES[lb - 100] if lb >= 100 else gi.LB_NAMES[lb]
+            for d, key in ((kern, name), (modes, mode)):
+                k = d.setdefault(key, {"ms": 0.0, "bytes": 0.0, "launches": 0})
+                k["ms"] += r["kernel_ms"]
+                k["bytes"] += r["alg_bytes"]
+                k["launches"] += 1
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    d = kern[dom]
     avg_ms = d["ms"] / d["launches"]
     bytes_per_launch = d["bytes"] / d["launches"]
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    per_kernel = {f"{a}_{b}": {"launches": v["launches"], "avg_us": 1e3 * v["ms"] / v["launches"],
-                              "GBps": v["bytes"] / (v["ms"] * 1e-3) / 1e9 if v["ms"] > 0 else 0.0}
-                  for (a, b), v in kern.items()}
+
+    def summary(v):
+        return {"launches": v["launches"], "avg_us": round(1e3 * v["ms"] / v["launches"], 2),
+                "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else 0.0}
+
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": f"{dom_key[0]}_{dom_key[1]}", "avg_launch_us": round(1e3 * avg_ms, 2),
-            "alg_bytes_per_launch": int(bytes_per_launch), "per_kernel": per_kernel}
+            "kernel": dom, "avg_launch_us": round(1e3 * avg_ms, 2),
+            "alg_bytes_per_launch": int(bytes_per_launch),
+            "per_kernel": {k: summary(v) for k, v in kern.items()},
+            "per_mode": {k: summary(v) for k, v in modes.items()}}
 
 
 def cpu_baseline(gi, g, sources, edges_of, budget_s: float):
@@ -101,10 +131,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=int, default=24)
     ap.add_argument("--edge-factor", type=int, default=16)
-    ap.add_argument("--threshold", type=float, default=0.02)
+    ap.add_argument("--threshold", type=float, default=None,
+                    help="hybrid threshold t (default: the one tuned with the selector)")
     ap.add_argument("--engine", choices=["fused", "unfused"], default="fused")
-    ap.add_argument("--push-lb", type=int, default=1)
-    ap.add_argument("--pull-lb", type=int, default=0)
+    ap.add_argument("--push-lb", type=int, default=2, help="push schedule (default TWCE)")
+    ap.add_argument("--pull-lb", type=int, default=0, help="pull schedule (default VERTEX_BASED)")
+    ap.add_argument("--selector", choices=["learned", "heuristic"], default="learned")
     ap.add_argument("--profile-sources", type=int, default=4)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -137,31 +169,58 @@ def main():
     log(f"graph RMAT-{args.scale} sym: V={V} E={E} built in {time.perf_counter() - t0:.2f}s")
     sources = g.pick_sources(SEED, 64)[: max(1, min(64, args.sources))]
     ns = len(sources)
-    sched = schedule(gi, args.threshold, args.push_lb, args.pull_lb, args.engine == "fused")
     depth = gi.DeviceArray(V, np.int32)
 
-    # untimed: reached edges per source
+    # the learned selector (trained on calibration graphs, not this one) and
+    # GraphIt's fixed hybrid rule at the threshold tuned on the same graphs
+    from easy_graphit_amd import selector as sel_mod
+    model = sel_mod.load() if os.path.exists(sel_mod.MODEL_PATH) else None
+    threshold = args.threshold
+    if threshold is None:
+        threshold = float(model["threshold"]) if model is not None and "threshold" in model else 0.0005
+    fused = args.engine == "fused"
+    heur = schedule(gi, threshold, args.push_lb, args.pull_lb, fused)
+    learned = None
+    if model is not None and fused:
+        sel_mod.install(model)
+        learned = schedule(gi, threshold, args.push_lb, args.pull_lb, fused, gi.SELECT_LEARNED)
+    use_learned = args.selector == "learned" and learned is not None
+    sched = learned if use_learned else heur
+
+    # untimed: reached edges per source; both policies must give equal depths
     edges_of = {}
     t0 = time.perf_counter()
     for i, src in enumerate(sources):
         gi.bfs_device(g, int(src), sched, depth)
         edges_of[int(src)] = g.reached_edges(depth)
+        if i < 4 and learned is not None:
+            want = depth.numpy()
+            gi.bfs_device(g, int(src), heur if use_learned else learned, depth)
+            if not np.array_equal(want, depth.numpy()):
+                raise SystemExit(f"learned and heuristic BFS disagree from source {src}")
         if i % 16 == 15:
             log(f"reference runs {i + 1}/{ns} ({time.perf_counter() - t0:.1f}s)")
     for k in range(args.warmup):
         gi.bfs_device(g, int(sources[k % ns]), sched, depth)
     gi.synchronize()
 
-    t0 = time.perf_counter()
-    total_edges = 0
-    for k in range(args.steps):
-        src = int(sources[k % ns])
-        gi.bfs_device(g, src, sched, depth)
-        total_edges += edges_of[src]
-    gi.synchronize()
-    dt = time.perf_counter() - t0
-    gteps = total_edges / 2 / dt / 1e9
-    log(f"{args.steps} BFS in {dt * 1e3:.1f} ms -> {gteps:.2f} GTEPS")
+    dt, gteps = time_bfs(gi, g, sched, sources, depth, edges_of, args.steps)
+    log(f"{args.steps} BFS in {dt * 1e3:.1f} ms -> {gteps:.2f} GTEPS "
+        f"({'learned selector' if use_learned else 'heuristic'})")
+    selector_cmp = None
+    if learned is not None:
+        other = heur if use_learned else learned
+        for k in range(args.warmup):
+            gi.bfs_device(g, int(sources[k % ns]), other, depth)
+        dt2, gteps2 = time_bfs(gi, g, other, sources, depth, edges_of, args.steps)
+        g_l, g_h = (gteps, gteps2) if use_learned else (gteps2, gteps)
+        selector_cmp = {"learned_gteps": round(g_l, 3), "heuristic_gteps": round(g_h, 3),
+                        "speedup": round(g_l / g_h, 3),
+                        "heuristic": f"GraphIt hybrid |F| <= t*V: PUSH {gi.LB_NAMES[args.push_lb]} "
+                                     f"else PULL {gi.LB_NAMES[args.pull_lb]} bitmap, t={threshold:g}",
+                        "learned": "MLP 16-64-64-7 (BF16 weights, FP32 accumulation) in the decide "
+                                   "kernel, trained on calibration RMAT/uniform/grid graphs"}
+        log(f"selector: {selector_cmp}")
 
     roof = roofline(gi, g, sched, sources[: args.profile_sources], depth) if args.profile_sources > 0 else None
     log(f"roofline: {json.dumps(roof)}")
@@ -184,10 +243,12 @@ def main():
         "dtype": "int32",
         "data": "synthetic (seeded RMAT generated on the device)",
         "config": {"workload": f"direction-optimizing BFS, symmetrized RMAT scale {args.scale} "
-                               f"edge factor {args.edge_factor}, 64 GAPBS-style sources",
+                               f"edge factor {args.edge_factor}, {ns} GAPBS-style sources",
                    "vertices": V, "edge_entries": E, "schedule": {
-                       "s1": repr(sched.s1), "s2": repr(sched.s2), "threshold": args.threshold,
-                       "selector": "heuristic", "engine": args.engine}},
+                       "s1": repr(sched.s1), "s2": repr(sched.s2), "threshold": threshold,
+                       "selector": "learned" if use_learned else "heuristic",
+                       "engine": args.engine}},
+        "selector": selector_cmp,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -62,37 +62,40 @@ struct SelectorView {
 };
 
 // The selector's 16 input features, computed from device counters (no host
-// sync). Shared by the decide kernel and the dataset recorder so training and
-// inference see identical inputs.
+// sync). Shared by the decide kernel (one thread per feature) and the dataset
+// recorder, so training and inference see identical inputs.
+__host__ __device__ inline float bfs_raw_feature(int k, int32_t n, unsigned long long fdeg,
+                                                 long long unvisited, long long unvisited_edges,
+                                                 int32_t level, int32_t prev_mode,
+                                                 int32_t prev_n, int32_t V, int64_t E) {
+    const double dV = V > 0 ? (double)V : 1.0, dE = E > 0 ? (double)E : 1.0;
+    const double nn = n, fd = (double)fdeg;
+    switch (k) {
+        case 0: return (float)(nn / dV);                                  // frontier share of V
+        case 1: return (float)(fd / dE);                                  // frontier share of E
+        case 2: return (float)((double)unvisited / dV);                   // unvisited share of V
+        case 3: return (float)((double)unvisited_edges / dE);             // unvisited edges share
+        case 4: return (float)(log2(1.0 + nn) / 32.0);
+        case 5: return (float)(log2(1.0 + fd) / 40.0);
+        case 6: return (float)(log2(1.0 + fd / (nn > 0 ? nn : 1.0)) / 20.0);  // mean frontier degree
+        case 7: return (float)(log2(1.0 + dE / dV) / 20.0);              // mean degree
+        case 8: return (float)(log2(dV) / 32.0);
+        case 9: return (float)(level < 64 ? level / 64.0 : 1.0);
+        case 10: return prev_mode >= BFS_PULL_THREAD ? 1.f : 0.f;         // previous step pulled
+        case 11: return (float)(log2(1.0 + nn) - log2(1.0 + (double)prev_n)) / 16.f;  // growth
+        case 12: return (float)(fd / (1.0 + (double)unvisited_edges));    // Beamer's m_f / m_u
+        case 13: return (float)(nn / (1.0 + (double)unvisited));
+        case 14: return (float)(log2(1.0 + (double)unvisited) / 32.0);
+        default: return 1.f;                                              // bias feature
+    }
+}
+
 __host__ __device__ inline void bfs_raw_features(float *f, int32_t n, unsigned long long fdeg,
                                                  long long unvisited, long long unvisited_edges,
                                                  int32_t level, int32_t prev_mode,
                                                  int32_t prev_n, int32_t V, int64_t E) {
-    double dV = V > 0 ? (double)V : 1.0, dE = E > 0 ? (double)E : 1.0;
-    double nn = n, fd = (double)fdeg;
-    f[0] = (float)(nn / dV);                                   // frontier share of V
-    f[1] = (float)(fd / dE);                                   // frontier share of E
-    f[2] = (float)((double)unvisited / dV);                    // unvisited share of V
-    f[3] = (float)((double)unvisited_edges / dE);              // unvisited edges share
-    f[4] = (float)(log2(1.0 + nn) / 32.0);
-    f[5] = (float)(log2(1.0 + fd) / 40.0);
-    f[6] = (float)(log2(1.0 + fd / (nn > 0 ? nn : 1.0)) / 20.0);  // mean frontier degree
-    f[7] = (float)(log2(1.0 + dE / dV) / 20.0);               // mean degree
-    f[8] = (float)(log2(dV) / 32.0);
-    f[9] = (float)(level < 64 ? level / 64.0 : 1.0);
-    f[10] = prev_mode >= BFS_PULL_THREAD ? 1.f : 0.f;          // previous step pulled
-    f[11] = (float)(log2(1.0 + nn) - log2(1.0 + (double)prev_n)) / 16.f;  // growth
-    f[12] = (float)(fd / (1.0 + (double)unvisited_edges));     // Beamer's m_f / m_u
-    f[13] = (float)(nn / (1.0 + (double)unvisited));
-    f[14] = (float)(log2(1.0 + (double)unvisited) / 32.0);
-    f[15] = 1.f;                                               // bias feature
-}
-
-__device__ inline void bfs_features(float *f, int32_t n, unsigned long long fdeg,
-                                    long long unvisited, long long unvisited_edges,
-                                    int32_t level, int32_t prev_mode, int32_t prev_n,
-                                    int32_t V, int64_t E) {
-    bfs_raw_features(f, n, fdeg, unvisited, unvisited_edges, level, prev_mode, prev_n, V, E);
+    for (int k = 0; k < kSelInputs; ++k)
+        f[k] = bfs_raw_feature(k, n, fdeg, unvisited, unvisited_edges, level, prev_mode, prev_n, V, E);
 }
 
 struct FusedTrace {

@@ -84,6 +84,11 @@ __device__ __forceinline__ T pick3(T p0, T p1, T p2, int32_t i) {
     return i == 0 ? p0 : (i == 1 ? p1 : p2);
 }
 
+template <class T>
+__device__ __forceinline__ T pick4(const T *x, int32_t i) {
+    return i == 0 ? x[0] : (i == 1 ? x[1] : (i == 2 ? x[2] : x[3]));
+}
+
 __device__ __forceinline__ bool in_bitmap(const uint64_t *bm, int32_t v) {
     return (bm[v >> 6] >> (v & 63)) & 1ull;
 }
@@ -257,7 +262,7 @@ __device__ __forceinline__ uint64_t wave_or(uint64_t x) {
     return x;
 }
 
-constexpr int kPullWords = 4;     // words a wave works on at once (memory-level parallelism)
+constexpr int kPullWords = 4;     // words a wave works on at once (memory-level parallelism; pick4)
 constexpr int kPullUnit = kPullWords * (kStepBlock / 64);  // words per workgroup unit
 static_assert(kChunkWords % kPullUnit == 0, "a pull unit lies inside one chunk");
 constexpr int kPullLaneSteps = 8; // PULL_GROUP: lane-serial edges before the cooperative scan
@@ -351,22 +356,25 @@ __device__ __forceinline__ bool pull_words(const StepArgs &a, StepCtx &c, int32_
             }
         }
     }
-#pragma unroll
+    // one word at a time from here (not unrolled: the cooperative scan is
+    // inlined once); the arrays are read through selects, not indexing
+#pragma unroll 1
     for (int i = 0; i < kPullWords; ++i) {
         const int32_t w = w0 + i;
+        const int32_t ei = pick4(e, i), e1i = pick4(e1, i), pi = pick4(par, i);
         uint64_t coop = 0;
-        if (mode != BFS_PULL_THREAD && __ballot(e[i] < e1[i])) coop = pull_coop(a, c, w, e[i], e1[i]);
+        if (mode != BFS_PULL_THREAD && __ballot(ei < e1i)) coop = pull_coop(a, c, w, ei, e1i);
         const int32_t v = w * 64 + lane;
-        const bool found_self = par[i] >= 0 || ((coop >> lane) & 1ull);
+        const bool found_self = pi >= 0 || ((coop >> lane) & 1ull);
         if (found_self) {
-            if (a.parent && par[i] >= 0) a.parent[v] = par[i];
+            if (a.parent && pi >= 0) a.parent[v] = pi;
             if (c.want_fdeg) c.fdeg += (unsigned long long)deg_of(a.off, v);
         }
         const uint64_t found = __ballot(found_self);
         if (found) {
             any = true;
             if (lane == 0) {
-                a.visited[w] = vis[i] | found;  // this wave owns word w in a pull step
+                a.visited[w] = pick4(vis, i) | found;  // this wave owns word w in a pull step
                 c.bm_out[w] = found;
                 c.cnt += __popcll(found);
             }
@@ -375,10 +383,19 @@ __device__ __forceinline__ bool pull_words(const StepArgs &a, StepCtx &c, int32_
     return any;
 }
 
-// ---------------------------------------------------------------- step kernel
-__global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a0) {
+// ---------------------------------------------------------------- step kernels
+// One level's step. The push and pull paths are separate kernels (each with
+// its own register and LDS budget, hence occupancy); a captured level
+// launches both and the one that does not match the decided mode returns at
+// once. Traced runs (per-level timing) launch only the matching one, under
+// their own symbols so their launches can be told apart in a profile.
+enum StepKind : int { kPushStep = 0, kPullStep = 1 };
+
+template <int Kind>
+__device__ __forceinline__ void step_body(const StepArgs &a0) {
     BfsState *st = a0.st;
     if (st->done) return;
+    if ((Kind == kPullStep) != (st->mode >= BFS_PULL_THREAD)) return;  // uniform
     StepArgs a = a0;
     a.depth = st->depth;
     a.parent = st->parent;
@@ -415,7 +432,7 @@ __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a0) {
         __syncthreads();
     }
 
-    if (mode < BFS_PULL_THREAD) {
+    if constexpr (Kind == kPushStep) {
         __shared__ int32_t list[kChunkVerts];
         __shared__ int32_t n_s;
         const uint64_t *si = pick3(a0.sum[0], a0.sum[1], a0.sum[2], iin);
@@ -474,12 +491,20 @@ __global__ void __launch_bounds__(kStepBlock) bfs_step_kernel(StepArgs a0) {
     flush_counts(a.pcount, a.pfdeg, c);
 }
 
+// (the pull kernels fit 64 VGPRs without spilling: 8 waves per SIMD)
+#define GI_PULL_ATTR __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ void __launch_bounds__(kStepBlock) bfs_push_step(StepArgs a) { step_body<kPushStep>(a); }
+__global__ void GI_PULL_ATTR bfs_pull_step(StepArgs a) { step_body<kPullStep>(a); }
+__global__ void __launch_bounds__(kStepBlock) bfs_push_step_traced(StepArgs a) { step_body<kPushStep>(a); }
+__global__ void GI_PULL_ATTR bfs_pull_step_traced(StepArgs a) { step_body<kPullStep>(a); }
+#undef GI_PULL_ATTR
+
 // Grid bin of PUSH_TWC: the edges of the deferred hubs, spread evenly over
 // the grid. Every workgroup scans the hub degrees in LDS and walks its share
 // of the concatenated edge range.
 __global__ void __launch_bounds__(kStepBlock) bfs_hub_kernel(StepArgs a0) {
     BfsState *st = a0.st;
-    if (st->done) return;
+    if (st->done || st->mode >= BFS_PULL_THREAD) return;
     int32_t nh = st->hub_count;
     if (nh <= 0) return;  // slots stay zero (the decide kernel clears them)
     if (nh > kMaxHubs) nh = kMaxHubs;
@@ -547,6 +572,23 @@ bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E, 
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
     const int32_t next_level = st->level + 1;
     const int32_t p = next_level & 1;
+    const bool learned = sel.mode == GI_SELECT_LEARNED && sel.w1 != nullptr;
+    // the network's BF16 parameters (one contiguous blob: w1 b1 w2 b2 w3 b3)
+    // go to LDS with 16-byte loads while the partial sums are added
+    constexpr int kBlobMax = kSelInputs * kSelHidden + kSelHidden + kSelHidden * kSelHidden +
+                             kSelHidden + kSelHidden * kSelMaxOut + kSelMaxOut;
+    __shared__ __align__(16) uint16_t blob[(kBlobMax + 7) / 8 * 8];
+    const int32_t n_params = kSelInputs * kSelHidden + kSelHidden + kSelHidden * kSelHidden +
+                             kSelHidden + (kSelHidden + 1) * sel.n_out;
+    if (learned) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(sel.w1);  // 256-byte aligned allocation
+        uint4 *dst = reinterpret_cast<uint4 *>(blob);
+        for (int i = threadIdx.x; i < n_params / 8; i += kDecideBlock) dst[i] = src[i];
+        for (int i = n_params / 8 * 8 + (int)threadIdx.x; i < n_params; i += kDecideBlock) blob[i] = sel.w1[i];
+    }
+    const uint16_t *sw1 = blob, *sb1 = sw1 + kSelInputs * kSelHidden;
+    const uint16_t *sw2 = sb1 + kSelHidden, *sb2 = sw2 + kSelHidden * kSelHidden;
+    const uint16_t *sw3 = sb2 + kSelHidden, *sb3 = sw3 + kSelHidden * sel.n_out;
     __shared__ int32_t rn[kDecideBlock / 64];
     __shared__ unsigned long long rf[kDecideBlock / 64];
     int32_t n = 0;
@@ -587,6 +629,7 @@ bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E, 
     __shared__ float h1[kSelHidden];
     __shared__ float h2[kSelHidden];
     __shared__ float y[kSelMaxOut];
+    __shared__ float part[kDecideBlock / 64][64];
     if (n == 0) {  // uniform
         if (threadIdx.x == 0) {
             st->done = 1;
@@ -595,34 +638,57 @@ bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E, 
         return;
     }
     int32_t mode;
-    const bool learned = sel.mode == GI_SELECT_LEARNED && sel.w1 != nullptr;
     const bool w0 = threadIdx.x < 64;
     if (!learned) {
         if (sel.mode == GI_SELECT_S1) mode = sel.mode_s1;
         else if (sel.mode == GI_SELECT_S2) mode = sel.mode_s2;
         else mode = ((double)n <= (double)sel.threshold * (double)V) ? sel.mode_s1 : sel.mode_s2;
     } else {
-        if (threadIdx.x == 0) {
-            float raw[kSelInputs];
-            bfs_features(raw, n, fdeg, unv, unv_e, next_level, st->mode, prev_n, V, E);
-            for (int k = 0; k < kSelInputs; ++k) feat[k] = (raw[k] - sel.feat_mean[k]) * sel.feat_scale[k];
+        if (threadIdx.x < kSelInputs) {
+            const int k = threadIdx.x;
+            const float raw = bfs_raw_feature(k, n, fdeg, unv, unv_e, next_level, st->mode, prev_n, V, E);
+            feat[k] = (raw - sel.feat_mean[k]) * sel.feat_scale[k];
+        }
+        __syncthreads();
+        // each layer: wave `wid` sums its quarter of the inputs for unit `lane`
+        constexpr int kWaves = kDecideBlock / 64;
+        {
+            float acc = 0.f;
+            for (int k = wid; k < kSelInputs; k += kWaves)
+                acc += feat[k] * bf16_to_f32(sw1[k * kSelHidden + lane]);
+            part[wid][lane] = acc;
         }
         __syncthreads();
         if (w0) {
-            float acc = bf16_to_f32(sel.b1[lane]);
-            for (int k = 0; k < kSelInputs; ++k) acc += feat[k] * bf16_to_f32(sel.w1[k * kSelHidden + lane]);
+            float acc = bf16_to_f32(sb1[lane]);
+            for (int q = 0; q < kWaves; ++q) acc += part[q][lane];
             h1[lane] = acc > 0.f ? acc : 0.f;
         }
         __syncthreads();
+        {
+            float acc = 0.f;
+            for (int k = wid; k < kSelHidden; k += kWaves)
+                acc += h1[k] * bf16_to_f32(sw2[k * kSelHidden + lane]);
+            part[wid][lane] = acc;
+        }
+        __syncthreads();
         if (w0) {
-            float acc = bf16_to_f32(sel.b2[lane]);
-            for (int k = 0; k < kSelHidden; ++k) acc += h1[k] * bf16_to_f32(sel.w2[k * kSelHidden + lane]);
+            float acc = bf16_to_f32(sb2[lane]);
+            for (int q = 0; q < kWaves; ++q) acc += part[q][lane];
             h2[lane] = acc > 0.f ? acc : 0.f;
         }
         __syncthreads();
+        {
+            float acc = 0.f;
+            if (lane < sel.n_out)
+                for (int k = wid; k < kSelHidden; k += kWaves)
+                    acc += h2[k] * bf16_to_f32(sw3[k * sel.n_out + lane]);
+            part[wid][lane] = acc;
+        }
+        __syncthreads();
         if (w0 && lane < sel.n_out) {
-            float acc = bf16_to_f32(sel.b3[lane]);
-            for (int k = 0; k < kSelHidden; ++k) acc += h2[k] * bf16_to_f32(sel.w3[k * sel.n_out + lane]);
+            float acc = bf16_to_f32(sb3[lane]);
+            for (int q = 0; q < kWaves; ++q) acc += part[q][lane];
             y[lane] = acc;
         }
         __syncthreads();
@@ -903,21 +969,36 @@ int32_t fused_mode_of(const gi_schedule &s) {
     }
 }
 
-// Persistent grid: as many workgroups as can be resident at once (the
-// step kernel assigns work statically, so a second round of workgroups
+// Persistent grids: as many workgroups as can be resident at once (the
+// step kernels assign work statically, so a second round of workgroups
 // would serialise), capped by the tuned blocks per CU.
-static int step_grid(const FusedWork *w) {
-    static int resident = -1;
-    if (resident < 0) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bfs_step_kernel, kStepBlock, 0) != hipSuccess ||
-            per_cu < 1)
-            per_cu = 4;
-        resident = per_cu;
-    }
+static int resident_per_cu(const void *kernel) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kStepBlock, 0) != hipSuccess || per_cu < 1)
+        per_cu = 4;
+    return per_cu;
+}
+
+static int grid_of(const FusedWork *w, int resident) {
     int per = w->blocks_per_cu < resident ? w->blocks_per_cu : resident;
     int g = device_info().num_cus * per;
     return g < kMaxGrid ? g : kMaxGrid;
+}
+
+static int push_grid(const FusedWork *w) {
+    static int resident = resident_per_cu((const void *)bfs_push_step);
+    return grid_of(w, resident);
+}
+
+static int pull_grid(const FusedWork *w) {
+    static int resident = resident_per_cu((const void *)bfs_pull_step);
+    return grid_of(w, resident);
+}
+
+// Partial-sum slots the decide kernel adds up: every block of either step.
+static int step_slots(const FusedWork *w) {
+    int a = push_grid(w), b = pull_grid(w);
+    return a > b ? a : b;
 }
 
 static StepArgs step_args(gi_graph *g, FusedWork *w) {
@@ -948,16 +1029,27 @@ static void launch_decide(gi_graph *g, FusedWork *w, int32_t level, hipStream_t 
     (void)level;  // the kernel reads the level from the device state
     SumSet sums{{w->sum[0], w->sum[1], w->sum[2]}};
     bfs_decide_kernel<<<1, kDecideBlock, 0, s>>>(w->st, w->d_sel, g->V, g->E, w->pcount, w->pfdeg,
-                                       step_grid(w), sums, w->sum_words);
+                                                 step_slots(w), sums, w->sum_words);
 }
 
-static void launch_level(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t level,
-                         hipStream_t s, hipEvent_t *ev) {
-    int grid = step_grid(w);
-    if (ev) (void)hipEventRecord(ev[0], s);
-    bfs_step_kernel<<<grid, kStepBlock, 0, s>>>(a);
-    if (ev) (void)hipEventRecord(ev[1], s);  // traces time the step kernel alone
-    bfs_hub_kernel<<<grid, kStepBlock, 0, s>>>(a);
+// A captured level: both step kernels (one returns at once), the hub kernel
+// (a no-op after a pull step), the decide kernel.
+static void launch_level(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t level, hipStream_t s) {
+    bfs_push_step<<<push_grid(w), kStepBlock, 0, s>>>(a);
+    bfs_pull_step<<<pull_grid(w), kStepBlock, 0, s>>>(a);
+    bfs_hub_kernel<<<push_grid(w), kStepBlock, 0, s>>>(a);
+    launch_decide(g, w, level, s);
+}
+
+// A traced level with the mode already known on the host: the matching step
+// kernel alone between the two events.
+static void launch_level_traced(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t level,
+                                int32_t mode, hipStream_t s, hipEvent_t *ev) {
+    (void)hipEventRecord(ev[0], s);
+    if (mode >= BFS_PULL_THREAD) bfs_pull_step_traced<<<pull_grid(w), kStepBlock, 0, s>>>(a);
+    else bfs_push_step_traced<<<push_grid(w), kStepBlock, 0, s>>>(a);
+    (void)hipEventRecord(ev[1], s);
+    if (mode < BFS_PULL_THREAD) bfs_hub_kernel<<<push_grid(w), kStepBlock, 0, s>>>(a);
     launch_decide(g, w, level, s);
 }
 
@@ -1017,7 +1109,7 @@ static int level_graph(gi_graph *g, FusedWork *w, int32_t levels, hipGraphExec_t
     StepArgs a = step_args(g, w);
     hipGraph_t graph = nullptr;
     GI_HIP(hipStreamBeginCapture(w->cap_stream, hipStreamCaptureModeThreadLocal));
-    for (int32_t l = 0; l < levels; ++l) launch_level(g, w, a, l, w->cap_stream, nullptr);
+    for (int32_t l = 0; l < levels; ++l) launch_level(g, w, a, l, w->cap_stream);
     hipError_t e = hipStreamEndCapture(w->cap_stream, &graph);
     if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
     hipGraphExec_t exec = nullptr;
@@ -1033,18 +1125,21 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
     StepArgs a = step_args(g, w);
     int32_t *done_flag = &w->st->done;
     if (trace && timed) {
-        // per-level launches with events around each step
+        // per-level launches with events around each step; the host reads
+        // {done, level, levels, mode} after every decide
+        static_assert(offsetof(BfsState, mode) == offsetof(BfsState, done) + 3 * sizeof(int32_t),
+                      "done, level, levels, mode are contiguous");
         for (int32_t level = 0; level < kBfsMaxLevels; ++level) {
+            GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+            GI_HIP(hipStreamSynchronize(s));
+            if (w->host_flag[0]) return GI_OK;
             hipEvent_t e0, e1;
             GI_HIP(hipEventCreate(&e0));
             GI_HIP(hipEventCreate(&e1));
             trace->ev.push_back(e0);
             trace->ev.push_back(e1);
-            launch_level(g, w, a, level, s, &trace->ev[trace->ev.size() - 2]);
+            launch_level_traced(g, w, a, level, w->host_flag[3], s, &trace->ev[trace->ev.size() - 2]);
             GI_LAUNCH_CHECK();
-            GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-            GI_HIP(hipStreamSynchronize(s));
-            if (*w->host_flag) return GI_OK;
         }
         return fail(GI_ERR_RANGE, "BFS deeper than %d levels", kBfsMaxLevels);
     }
@@ -1106,7 +1201,7 @@ int fused_reached(gi_graph *g, const int32_t *depth, int32_t *out, hipStream_t s
     FusedWork *w = nullptr;
     GI_TRY(ensure_fused(g, &w));
     GI_TRY(fill_i32(w->reached, 1, 0, s));
-    count_reached_fused_kernel<<<step_grid(w), 256, 0, s>>>(depth, g->V, w->reached);
+    count_reached_fused_kernel<<<push_grid(w), 256, 0, s>>>(depth, g->V, w->reached);
     GI_LAUNCH_CHECK();
     return read_count(w->reached, out, s);
 }
@@ -1200,7 +1295,6 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
     sel.mode = GI_SELECT_S1;
     sel.mode_s1 = BFS_PUSH_THREAD;
     sel.mode_s2 = BFS_PUSH_THREAD;
-    int grid = step_grid(w);
     if (rc == GI_OK) rc = seed_run(g, w, nullptr, 0, source, depth, nullptr, sel, true, s);
     StepArgs a = step_args(g, w);
     int32_t nl = 0, prev_mode = BFS_PUSH_THREAD, prev_n = 0;
@@ -1238,8 +1332,12 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
                 zero_slots_kernel<<<8, 256, 0, s>>>(w->pcount, w->pfdeg);
                 bfs_force_mode_kernel<<<1, 1, 0, s>>>(w->st, run);
                 GI_HIP(hipEventRecord(e0, s));
-                bfs_step_kernel<<<grid, kStepBlock, 0, s>>>(a);
-                bfs_hub_kernel<<<grid, kStepBlock, 0, s>>>(a);
+                if (run >= BFS_PULL_THREAD) {
+                    bfs_pull_step<<<pull_grid(w), kStepBlock, 0, s>>>(a);
+                } else {
+                    bfs_push_step<<<push_grid(w), kStepBlock, 0, s>>>(a);
+                    bfs_hub_kernel<<<push_grid(w), kStepBlock, 0, s>>>(a);
+                }
                 GI_HIP(hipEventRecord(e1, s));
                 GI_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;

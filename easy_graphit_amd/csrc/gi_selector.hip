@@ -65,15 +65,16 @@ __global__ void mlp_fwd_kernel(const float *X, const float *W, const float *bias
     Y[t] = (relu && acc < 0.f) ? 0.f : acc;
 }
 
-// Masked mean squared error: G = 2 m (y - t) / sum(m); loss partials per block.
+// Weighted mean squared error L = sum m (y - t)^2 / sum m (m = 0 masks an
+// entry out): G = 2 m (y - t) / sum m; loss partials per block.
 __global__ void mlp_loss_kernel(const float *Y, const float *T, const float *M, float *G,
                                 int64_t n, float inv_msum, float *loss) {
     float part = 0.f;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * blockDim.x) {
-        float d = (Y[i] - T[i]) * M[i];
-        part += d * d;
-        G[i] = 2.f * d * inv_msum;
+        float d = Y[i] - T[i];
+        part += M[i] * d * d;
+        G[i] = 2.f * M[i] * d * inv_msum;
     }
     for (int o = 32; o > 0; o >>= 1) part += __shfl_xor(part, o);
     if (lane_id() == 0) atomicAdd(loss, part * inv_msum);

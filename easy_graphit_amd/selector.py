@@ -36,6 +36,10 @@ CALIBRATION = [
     ("rmat", dict(scale=19, edge_factor=8, seed=14)),
     ("rmat", dict(scale=20, edge_factor=32, seed=15)),
     ("rmat", dict(scale=21, edge_factor=16, seed=16)),
+    ("rmat", dict(scale=22, edge_factor=16, seed=20)),
+    ("rmat", dict(scale=22, edge_factor=8, seed=21)),
+    ("rmat", dict(scale=23, edge_factor=16, seed=24)),
+    ("rmat", dict(scale=24, edge_factor=16, seed=25)),
     ("uniform", dict(num_vertices=1 << 20, num_pairs=8 << 20, seed=17)),
     ("uniform", dict(num_vertices=1 << 18, num_pairs=4 << 18, seed=18)),
     ("grid", dict(width=512, num_random=2600, seed=19)),
@@ -70,10 +74,16 @@ def record(specs=CALIBRATION, sources_per_graph: int = 4, reps: int = 2, log=pri
 
 
 def targets(times: np.ndarray):
-    """log(ms) targets and a mask of measured entries."""
-    mask = (times > 0).astype(np.float32)
-    t = np.where(times > 0, np.log(np.maximum(times, 1e-4)), 0.0).astype(np.float32)
-    return t, mask
+    """log(ms) targets and per-entry weights: 0 for modes not measured, else
+    the level's best time relative to the mean best (clipped to
+    [0.05, 20]), so that the levels where the choice costs time dominate
+    the fit."""
+    measured = times > 0
+    t = np.where(measured, np.log(np.maximum(times, 1e-4)), 0.0).astype(np.float32)
+    best = np.where(measured, times, np.inf).min(axis=1)
+    level_w = np.clip(best / best.mean(), 0.05, 20.0)
+    w = (measured * level_w[:, None]).astype(np.float32)
+    return t, w
 
 
 def init_params(rng, n_out: int = N_OUT):
@@ -119,15 +129,19 @@ def predict(model, feats):
 
 
 def save(model, path: str = MODEL_PATH):
+    extra = {"threshold": np.float32(model["threshold"])} if "threshold" in model else {}
     np.savez(path, params=model["params"], mean=model["mean"], scale=model["scale"],
              loss=model["loss"], n_out=np.int32(model["n_out"]),
-             allowed=np.int32(model["allowed"]))
+             allowed=np.int32(model["allowed"]), **extra)
 
 
 def load(path: str = MODEL_PATH):
     with np.load(path, allow_pickle=False) as z:
-        return {"params": z["params"], "mean": z["mean"], "scale": z["scale"],
-                "loss": z["loss"], "n_out": int(z["n_out"]), "allowed": int(z["allowed"])}
+        m = {"params": z["params"], "mean": z["mean"], "scale": z["scale"],
+             "loss": z["loss"], "n_out": int(z["n_out"]), "allowed": int(z["allowed"])}
+        if "threshold" in z.files:
+            m["threshold"] = float(z["threshold"])
+        return m
 
 
 def install(model):
@@ -157,8 +171,11 @@ def time_policy(g: gi.Graph, sched, sources, depth, reps: int = 1) -> float:
     return time.perf_counter() - t0
 
 
-def tune_threshold(specs=CALIBRATION, grid=(0.001, 0.003, 0.01, 0.02, 0.05, 0.1, 0.2),
-                   sources_per_graph: int = 4, push_lb: int = gi.LB_TWCE, log=print):
+THRESHOLDS = (0.0002, 0.0005, 0.001, 0.003, 0.01, 0.02, 0.05, 0.1, 0.2)
+
+
+def tune_threshold(specs=CALIBRATION, grid=THRESHOLDS, sources_per_graph: int = 4,
+                   push_lb: int = gi.LB_TWCE, log=print):
     """GraphIt's rule |F| <= t*V: pick t by total calibration time."""
     total = {t: 0.0 for t in grid}
     for kind, args in specs:
@@ -174,24 +191,50 @@ def tune_threshold(specs=CALIBRATION, grid=(0.001, 0.003, 0.01, 0.02, 0.05, 0.1,
 
 
 def evaluate(model, threshold: float, scale: int = 24, edge_factor: int = 16,
-             seed: int = 27491095, num_sources: int = 16, push_lb: int = gi.LB_TWCE, log=print):
-    """Learned selector vs GraphIt's fixed rule on a test graph; outputs of
-    both are checked equal before timing."""
+             seed: int = 27491095, num_sources: int = 16, grid=THRESHOLDS, log=print):
+    """Learned selector vs GraphIt's fixed hybrid rule on a test graph that
+    is not a calibration graph.
+
+    The rule is the reference BFS app's "power" schedule (apps/bfs.cpp:69-80:
+    PUSH TWCE, else PULL VERTEX_BASED on a bitmap when |F| > t*V), timed at
+    the threshold tuned on the calibration graphs and at every threshold of
+    `grid` (so the best fixed t for this very graph is reported too). The
+    same rule with this runtime's grid-binned TWC push is timed as well, as
+    the strongest fixed schedule. Depths of every policy are checked equal to
+    the learned run's on every source before timing."""
     g = gi.Graph.rmat(scale, edge_factor, seed, True)
     srcs = g.pick_sources(seed, num_sources)
     depth = gi.DeviceArray(g.num_vertices, np.int32)
-    heur = fused_schedule(push_lb, threshold=threshold)
-    learned = fused_schedule(push_lb, threshold=threshold, selector=gi.SELECT_LEARNED)
+    learned = fused_schedule(threshold=threshold, selector=gi.SELECT_LEARNED)
     install(model)
-    d_h, _, st_h = gi.bfs(g, int(srcs[0]), heur)
-    d_l, _, st_l = gi.bfs(g, int(srcs[0]), learned)
-    if not np.array_equal(d_h, d_l):
-        raise AssertionError("learned and heuristic schedules disagree")
-    t_h = min(time_policy(g, heur, srcs, depth) for _ in range(3))
+    heur = fused_schedule(gi.LB_TWCE, threshold=threshold)
+    levels = {}
+    for s in srcs:
+        d_l, _, st_l = gi.bfs(g, int(s), learned)
+        d_h, _, st_h = gi.bfs(g, int(s), heur)
+        d_c, _, _ = gi.bfs(g, int(s), fused_schedule(gi.LB_TWC, threshold=threshold))
+        if not (np.array_equal(d_h, d_l) and np.array_equal(d_c, d_l)):
+            raise AssertionError(f"schedules disagree from source {s}")
+        levels[int(s)] = {"heuristic_push_pull": (st_h["push_iterations"], st_h["pull_iterations"]),
+                          "learned_push_pull": (st_l["push_iterations"], st_l["pull_iterations"])}
     t_l = min(time_policy(g, learned, srcs, depth) for _ in range(3))
-    res = {"graph": f"RMAT-{scale} EF{edge_factor}", "sources": num_sources,
-           "heuristic_s": t_h, "learned_s": t_l, "speedup": t_h / t_l, "threshold": threshold,
-           "levels_heuristic": st_h["iterations"], "levels_learned": st_l["iterations"]}
+    t_h = min(time_policy(g, heur, srcs, depth) for _ in range(3))
+    sweep = {"twce": {}, "twc": {}}
+    for name, lb in (("twce", gi.LB_TWCE), ("twc", gi.LB_TWC)):
+        for t in grid:
+            sweep[name][t] = min(time_policy(g, fused_schedule(lb, threshold=t), srcs, depth)
+                                 for _ in range(2))
+    best_twce = min(sweep["twce"], key=sweep["twce"].get)
+    best_twc = min(sweep["twc"], key=sweep["twc"].get)
+    res = {"graph": f"RMAT-{scale} EF{edge_factor} seed {seed}", "sources": num_sources,
+           "threshold": threshold, "heuristic_s": t_h, "learned_s": t_l,
+           "speedup": t_h / t_l,
+           "best_test_threshold": best_twce,
+           "speedup_vs_best_threshold": sweep["twce"][best_twce] / t_l,
+           "twc_best_test_threshold": best_twc,
+           "speedup_vs_best_twc_schedule": sweep["twc"][best_twc] / t_l,
+           "threshold_sweep_s": {k: {str(t): v for t, v in d.items()} for k, d in sweep.items()},
+           "levels": {str(k): v for k, v in levels.items()}}
     log(f"evaluate: {res}")
     return res
 
@@ -220,7 +263,6 @@ def main(argv=None):
         t_best, sweep = tune_threshold(log=log)
         model["threshold"] = t_best
         save(model, args.out)
-        np.savez(args.out, **{k: v for k, v in load(args.out).items()}, threshold=np.float32(t_best))
         # offline choice quality on the training set
         pred = predict(model, feats)
         measured = np.where(times > 0, times, np.inf)
@@ -233,9 +275,7 @@ def main(argv=None):
                           "train_regret": regret}))
     else:
         model = load(args.out)
-        with np.load(args.out, allow_pickle=False) as z:
-            t = float(z["threshold"]) if "threshold" in z.files else 0.05
-        print(json.dumps(evaluate(model, t, scale=args.scale, log=log)))
+        print(json.dumps(evaluate(model, model.get("threshold", 0.0005), scale=args.scale, log=log)))
 
 
 if __name__ == "__main__":

@@ -710,9 +710,9 @@ double or_mlp_loss_backward(int32_t batch, int32_t n_in, int32_t n_hidden, int32
     for (int64_t i = 0; i < (int64_t)batch * n_out; ++i) msum += mask[i];
     if (msum <= 0.0) msum = 1.0;
     for (int64_t i = 0; i < (int64_t)batch * n_out; ++i) {
-        double d = (y[i] - target[i]) * mask[i];
-        loss += d * d;
-        gy[i] = 2.0 * d / msum;
+        double d = y[i] - target[i];
+        loss += mask[i] * d * d;
+        gy[i] = 2.0 * mask[i] * d / msum;
     }
     loss /= msum;
     dense_backward(batch, n_hidden, n_out, h2, w3, gy, gw3, gb3, g2);

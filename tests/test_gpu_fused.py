@@ -106,7 +106,7 @@ def test_selector_gradients_match_fp64_oracle(gi):
     p[3] = rng.normal(size=64) * 0.1
     X = rng.normal(size=(300, 16))
     T = rng.normal(size=(300, n_out))
-    M = (rng.random((300, n_out)) > 0.2).astype(np.float64)
+    M = (rng.random((300, n_out)) > 0.2) * rng.uniform(0.1, 4.0, size=(300, n_out))
     loss, g = gi.selector_loss_grad(X, T, M, _flat(p), n_out)
     want_loss, want = O.mlp_loss_backward(X, T, M, *p)
     assert abs(loss - want_loss) <= 1e-4 * abs(want_loss)

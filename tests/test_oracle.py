@@ -377,8 +377,11 @@ def test_mlp_gradients_finite_differences():
     p = _mlp_params(rng)
     x = rng.normal(size=(6, 16))
     t = rng.normal(size=(6, 5))
-    m = (rng.random((6, 5)) > 0.3).astype(np.float64)
+    # weights: zeros mask entries out, the rest weigh them
+    m = (rng.random((6, 5)) > 0.3) * rng.uniform(0.2, 3.0, size=(6, 5))
     loss, grads = O.mlp_loss_backward(x, t, m, *p)
+    y = O.mlp_forward(x, *p)
+    assert np.isclose(loss, (m * (y - t) ** 2).sum() / m.sum(), rtol=1e-12)
     eps = 1e-6
     for pi, (param, grad) in enumerate(zip(p, grads)):
         flat = param.reshape(-1)
