@@ -106,6 +106,20 @@ def roofline(gi, g, sched, sources, depth):
             "per_mode": {k: summary(v) for k, v in modes.items()}}
 
 
+def pmc_traffic(kernel: str, scale: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE runs of this benchmark, scaled by a
+    plain copy of known size); counters cannot be read from inside the run."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_rmat{scale}_pmc.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1])), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def cpu_baseline(gi, g, sources, edges_of, budget_s: float):
     import oracle
     off, nb, _ = g.read_csr()
@@ -223,6 +237,8 @@ def main():
         log(f"selector: {selector_cmp}")
 
     roof = roofline(gi, g, sched, sources[: args.profile_sources], depth) if args.profile_sources > 0 else None
+    if roof is not None:
+        roof["traffic"], roof["traffic_source"] = pmc_traffic(roof["kernel"], args.scale)
     log(f"roofline: {json.dumps(roof)}")
     cpu = None
     if not args.no_cpu:
