@@ -532,6 +532,22 @@ int gi_cc(gi_graph *g, const gi_schedule *sched, int32_t *d_labels, gi_run_stats
         sched = &def;
     }
     hipStream_t s = (hipStream_t)stream;
+    if (sched->kernel_fusion == GI_FUSION_ENABLED) {
+        // fused engine: hook sweeps and shortcutting with device-side termination
+        EventTimer timer(s);
+        int32_t rounds = 0;
+        GI_TRY(cc_fused_run(g, d_labels, &rounds, s));
+        double ms = timer.stop();
+        if (stats) {
+            stats->iterations = rounds;
+            stats->push_iterations = rounds;
+            stats->pull_iterations = 0;
+            stats->ms = ms;
+            stats->reached = g->V;
+            stats->edges_traversed = (int64_t)rounds * g->E;
+        }
+        return GI_OK;
+    }
     EventTimer timer(s);
     int32_t *upd = nullptr;
     GI_TRY(dalloc(&upd, 2));
@@ -581,6 +597,22 @@ int gi_pagerank(gi_graph *g, int32_t iters, float damping, const gi_schedule *sc
         sched = &def;
     }
     hipStream_t s = (hipStream_t)stream;
+    if (sched->kernel_fusion == GI_FUSION_ENABLED) {
+        // fused engine: deterministic pull over the in-edges, one captured graph
+        if (sched->direction != GI_PULL) return fail(GI_ERR_UNSUPPORTED, "fused PageRank pulls");
+        EventTimer timer(s);
+        GI_TRY(pr_fused_run(g, iters, damping, d_rank, s));
+        double ms = timer.stop();
+        if (stats) {
+            stats->iterations = iters;
+            stats->push_iterations = 0;
+            stats->pull_iterations = iters;
+            stats->ms = ms;
+            stats->reached = g->V;
+            stats->edges_traversed = (int64_t)iters * g->E;
+        }
+        return GI_OK;
+    }
     float *nw = nullptr, *contrib = nullptr;
     GI_TRY(dalloc(&nw, (size_t)g->V));
     int rc = dalloc(&contrib, (size_t)g->V);

@@ -1,0 +1,295 @@
+// Fused connected components: the loop of apps/cc.cpp:53-78 (hook every
+// edge, ID[ID[s]] min= ID[ID[d]] both ways, then pointer jumping until
+// stable, until a sweep changes nothing) with every decision on the device
+// and the rounds captured as a hipGraph.
+//
+// Per round (three launches):
+//   * cc_hook: an edge-parallel sweep of the CSR. Edges are cut into chunks
+//     of kCcChunk; the source of each chunk's first edge is precomputed once
+//     per graph. A wave takes 256 edges at a time (4 per lane, loads in
+//     flight together) and finds each edge's source from a window of the
+//     next 64 offsets held one per lane (binary search by shuffles), so no
+//     COO source array is read. Only the larger of the two parent labels is
+//     lowered (atomicMin), and only when it is larger.
+//   * cc_compress: every vertex follows its label chain to the root and
+//     stores it (full shortcutting; the fixed point is the same as the
+//     reference's repeated single jumps).
+//   * cc_control: ends the run when a sweep lowered nothing.
+// The result is the component's minimum vertex id, as in the reference:
+// labels are vertex ids of the same component and only decrease, and the
+// minimum vertex never takes another label.
+#include <map>
+#include <vector>
+
+#include "gi_internal.h"
+
+namespace gi {
+
+constexpr int kCcBlock = 256;
+constexpr int32_t kCcChunk = 8192;   // edges per chunk (a wave's unit of work)
+constexpr int kCcEdgesPerLane = 4;
+
+struct CcState {
+    int32_t changed;   // set by a hook sweep that lowered a label
+    int32_t done;
+    int32_t rounds;
+    int32_t pad;
+};
+
+struct CcArgs {
+    const int32_t *off;
+    const int32_t *nbr;
+    const int32_t *chunk_v;  // source vertex of each chunk's first edge
+    int32_t *id;
+    CcState *st;
+    int32_t V;
+    int64_t E;
+    int32_t chunks;
+};
+
+__global__ void cc_chunk_owner_kernel(const int32_t *off, int32_t V, int32_t chunks, int32_t *chunk_v) {
+    for (int32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < chunks; c += gridDim.x * blockDim.x) {
+        const int64_t e = (int64_t)c * kCcChunk;
+        // last v with off[v] <= e
+        int32_t lo = 0, hi = V;  // answer in [lo, hi)
+        while (hi - lo > 1) {
+            int32_t mid = lo + (hi - lo) / 2;
+            if ((int64_t)off[mid] <= e) lo = mid;
+            else hi = mid;
+        }
+        chunk_v[c] = lo;
+    }
+}
+
+__global__ void cc_init_kernel(int32_t *id, int32_t V, CcState *st) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) id[v] = v;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->changed = 0;
+        st->done = 0;
+        st->rounds = 0;
+    }
+}
+
+// Source of edge e: the last v >= v0 with off[v] <= e. `wo` holds
+// off[v0 + 1 + lane] (INT_MAX past V); returns -1 when the source lies past
+// the window.
+__device__ __forceinline__ int32_t window_owner(int32_t v0, int32_t wo, int32_t e) {
+    const bool past = __shfl(wo, 63) <= e;  // all 64 window offsets <= e
+    int32_t c = 0;  // number of window offsets <= e, when below 64
+    for (int32_t step = 32; step > 0; step >>= 1) {
+        int32_t t = c + step;
+        if (__shfl(wo, t - 1) <= e) c = t;
+    }
+    return past ? -1 : v0 + c;
+}
+
+__device__ __forceinline__ int32_t global_owner(const int32_t *off, int32_t lo, int32_t V, int32_t e) {
+    int32_t hi = V;
+    while (hi - lo > 1) {
+        int32_t mid = lo + (hi - lo) / 2;
+        if (off[mid] <= e) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kCcBlock) cc_hook_kernel(CcArgs a) {
+    if (a.st->done) return;
+    const int32_t lane = lane_id();
+    const int32_t waves = gridDim.x * (kCcBlock / 64);
+    bool changed = false;
+    for (int32_t c = blockIdx.x * (kCcBlock / 64) + (threadIdx.x >> 6); c < a.chunks; c += waves) {
+        const int32_t c0 = c * kCcChunk;
+        const int32_t c1 = (int64_t)c0 + kCcChunk < a.E ? c0 + kCcChunk : (int32_t)a.E;
+        int32_t vcur = a.chunk_v[c];
+        for (int32_t b = c0; b < c1; b += 64 * kCcEdgesPerLane) {
+            const int32_t wv = vcur + 1 + lane;
+            const int32_t wo = wv <= a.V ? a.off[wv] : 0x7fffffff;
+            int32_t u[kCcEdgesPerLane], d[kCcEdgesPerLane];
+#pragma unroll
+            for (int k = 0; k < kCcEdgesPerLane; ++k) {
+                const int32_t e = b + 64 * k + lane;
+                const bool valid = e < c1;
+                d[k] = valid ? a.nbr[e] : -1;
+                int32_t o = window_owner(vcur, wo, e);
+                if (valid && o < 0) o = global_owner(a.off, vcur + 64, a.V, e);  // past the window
+                u[k] = valid ? o : -1;
+            }
+            // next window starts at the source of the batch's last edge
+            int32_t last = b + 64 * kCcEdgesPerLane - 1;
+            last = last < c1 ? last : c1 - 1;
+            const int32_t ol = window_owner(vcur, wo, last);
+            vcur = ol >= 0 ? ol : global_owner(a.off, vcur + 64, a.V, last);
+            int32_t sid[kCcEdgesPerLane], did[kCcEdgesPerLane];
+#pragma unroll
+            for (int k = 0; k < kCcEdgesPerLane; ++k) {
+                sid[k] = u[k] >= 0 ? a.id[u[k]] : 0;
+                did[k] = u[k] >= 0 ? a.id[d[k]] : 0;
+            }
+#pragma unroll
+            for (int k = 0; k < kCcEdgesPerLane; ++k) {
+                if (u[k] < 0) continue;
+                const int32_t ps = a.id[sid[k]], pd = a.id[did[k]];  // apps/cc.cpp:23-29
+                if (pd < ps) {
+                    if (atomicMin(&a.id[sid[k]], pd) > pd) changed = true;
+                } else if (ps < pd) {
+                    if (atomicMin(&a.id[did[k]], ps) > ps) changed = true;
+                }
+            }
+        }
+    }
+    if (__ballot(changed) && lane == 0 && !a.st->changed) a.st->changed = 1;
+}
+
+__global__ void cc_compress_kernel(CcArgs a) {
+    if (a.st->done) return;
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.V; v += gridDim.x * blockDim.x) {
+        const int32_t y0 = a.id[v];
+        int32_t y = y0, x = a.id[y];
+        while (x != y) {  // labels strictly decrease along a chain, so it ends at a root
+            y = x;
+            x = a.id[y];
+        }
+        if (y != y0) a.id[v] = y;
+    }
+}
+
+__global__ void cc_control_kernel(CcState *st) {
+    if (st->done) return;
+    st->rounds += 1;
+    if (!st->changed) st->done = 1;
+    st->changed = 0;
+}
+
+// ---------------------------------------------------------------- host side
+struct CcPlan {
+    int32_t chunks = 0;
+    int32_t *chunk_v = nullptr;
+    CcState *st = nullptr;
+    int32_t *host = nullptr;  // pinned copy of the state
+    hipStream_t cap = nullptr;
+    std::map<std::pair<int32_t, int32_t *>, hipGraphExec_t> execs;  // (rounds, labels)
+    int32_t last_rounds = 0;
+};
+
+static std::map<const gi_graph *, CcPlan *> g_cc;
+
+static void free_cc(CcPlan *p) {
+    for (auto &kv : p->execs) (void)hipGraphExecDestroy(kv.second);
+    if (p->cap) (void)hipStreamDestroy(p->cap);
+    dfree(p->chunk_v);
+    dfree(p->st);
+    if (p->host) (void)hipHostFree(p->host);
+    delete p;
+}
+
+void release_cc(const gi_graph *g) {
+    auto it = g_cc.find(g);
+    if (it == g_cc.end()) return;
+    free_cc(it->second);
+    g_cc.erase(it);
+}
+
+static int ensure_cc(gi_graph *g, CcPlan **out) {
+    auto it = g_cc.find(g);
+    if (it != g_cc.end()) {
+        *out = it->second;
+        return GI_OK;
+    }
+    if (g->E >= 0x7fffffffLL - kCcChunk) return fail(GI_ERR_RANGE, "fused CC needs E < 2^31");
+    CcPlan *p = new CcPlan();
+    p->chunks = (int32_t)ceil_div(g->E, kCcChunk);
+    int rc = dalloc(&p->chunk_v, (size_t)p->chunks + 1);
+    if (rc == GI_OK) rc = dalloc(&p->st, 1);
+    if (rc == GI_OK && hipHostMalloc((void **)&p->host, sizeof(CcState)) != hipSuccess)
+        rc = fail(GI_ERR_OOM, "pinned allocation failed");
+    if (rc == GI_OK && hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(GI_ERR_HIP, "stream create failed");
+    if (rc == GI_OK && p->chunks > 0) {
+        int grid = (int)ceil_div(p->chunks, 256);
+        cc_chunk_owner_kernel<<<grid, 256>>>(g->d_off, g->V, p->chunks, p->chunk_v);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+            rc = fail(GI_ERR_HIP, "chunk owners");
+    }
+    if (rc != GI_OK) {
+        free_cc(p);
+        return rc;
+    }
+    g_cc[g] = p;
+    *out = p;
+    return GI_OK;
+}
+
+static int hook_grid() {
+    static int per_cu = -1;
+    if (per_cu < 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cc_hook_kernel, kCcBlock, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 4;
+    }
+    return device_info().num_cus * per_cu;
+}
+
+static void launch_round(const CcArgs &a, hipStream_t s) {
+    cc_hook_kernel<<<hook_grid(), kCcBlock, 0, s>>>(a);
+    int cg = (int)ceil_div(a.V, 256);
+    cc_compress_kernel<<<cg < 8192 ? cg : 8192, 256, 0, s>>>(a);
+    cc_control_kernel<<<1, 1, 0, s>>>(a.st);
+}
+
+int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, hipStream_t s) {
+    if (g->V == 0) {
+        if (rounds_out) *rounds_out = 0;
+        return GI_OK;
+    }
+    if (g->directed) return fail(GI_ERR_UNSUPPORTED, "fused CC needs a symmetric graph");
+    CcPlan *p = nullptr;
+    GI_TRY(ensure_cc(g, &p));
+    CcArgs a;
+    a.off = g->d_off;
+    a.nbr = g->d_nbr;
+    a.chunk_v = p->chunk_v;
+    a.id = d_labels;
+    a.st = p->st;
+    a.V = g->V;
+    a.E = g->E;
+    a.chunks = p->chunks;
+    int ig = (int)ceil_div(g->V, 256);
+    cc_init_kernel<<<ig < 8192 ? ig : 8192, 256, 0, s>>>(d_labels, g->V, p->st);
+    GI_LAUNCH_CHECK();
+    int32_t block = p->last_rounds > 0 ? p->last_rounds : 4;
+    for (int32_t done_rounds = 0; done_rounds < (1 << 20);) {
+        if (block > 32) block = 32;
+        auto key = std::make_pair(block, d_labels);
+        auto it = p->execs.find(key);
+        if (it == p->execs.end()) {
+            if (p->execs.size() >= 8) {
+                for (auto &kv : p->execs) (void)hipGraphExecDestroy(kv.second);
+                p->execs.clear();
+            }
+            hipGraph_t graph = nullptr;
+            GI_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+            for (int32_t r = 0; r < block; ++r) launch_round(a, p->cap);
+            hipError_t e = hipStreamEndCapture(p->cap, &graph);
+            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            hipGraphExec_t exec = nullptr;
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+            it = p->execs.emplace(key, exec).first;
+        }
+        GI_HIP(hipGraphLaunch(it->second, s));
+        GI_HIP(hipMemcpyAsync(p->host, p->st, sizeof(CcState), hipMemcpyDeviceToHost, s));
+        GI_HIP(hipStreamSynchronize(s));
+        done_rounds += block;
+        if (p->host[1]) {  // done
+            p->last_rounds = p->host[2];
+            if (rounds_out) *rounds_out = p->host[2];
+            return GI_OK;
+        }
+        block *= 2;
+    }
+    return fail(GI_ERR_RANGE, "CC did not converge");
+}
+
+}  // namespace gi

@@ -746,6 +746,8 @@ int gi_graph_destroy(gi_graph *g) {
     dfree(g->d_visited);
     if (g->work) gi_frontier_destroy(g->work);
     release_fused(g);
+    release_pr(g);
+    release_cc(g);
     delete g;
     return GI_OK;
 }

@@ -266,4 +266,11 @@ int frontier_clear_out(gi_frontier *f, int32_t fmt, hipStream_t s);
 Sink make_sink(gi_frontier *f, int32_t fmt, bool dedup);
 Member make_member(const gi_frontier *f);
 int read_count(const int32_t *d_count, int32_t *out, hipStream_t s);
+
+// Fused whole-run engines and the per-graph state they cache (released by
+// gi_graph_destroy).
+int pr_fused_run(gi_graph *g, int32_t iters, float damping, float *d_rank, hipStream_t s);
+void release_pr(const gi_graph *g);
+int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds, hipStream_t s);
+void release_cc(const gi_graph *g);
 }  // namespace gi

@@ -1,0 +1,304 @@
+// Fused PageRank: the iteration loop of apps/pr.cpp:28-46 (contrib =
+// rank / out-degree; new = sum of in-neighbour contribs; rank = (1-d)/N +
+// d * new; no dangling redistribution) as a pull over the in-edge CSR with
+// no atomics, captured once as a hipGraph of all iterations.
+//
+// Work per iteration (two launches):
+//   * pr_gather: work items are groups of 64 consecutive vertices and
+//     512-edge chunks of the in-edge lists of wide vertices (in-degree >= 64).
+//     A group's wave sums the in-edge lists of its narrow vertices
+//     cooperatively (the lists are contiguous: 64 edges per pass, owner by
+//     binary search over the lane prefix, a segmented scan across the wave,
+//     accumulation per vertex in LDS) and writes their new rank and next
+//     contribution. A chunk's wave writes one partial sum.
+//   * pr_finalize: thread per wide vertex adds its chunk partials in chunk
+//     order and writes its rank and next contribution.
+// Every sum has a fixed order, so results are deterministic.
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "gi_internal.h"
+
+namespace gi {
+
+constexpr int kPrBlock = 256;
+constexpr int32_t kPrNarrow = 64;      // in-degree below this: summed inside its group
+constexpr int32_t kPrChunk = 512;      // edges per chunk of a wide vertex
+
+struct PrArgs {
+    const int32_t *poff;      // in-edge CSR
+    const int32_t *pnbr;
+    const int32_t *off;       // out-edge CSR offsets (out-degree)
+    int32_t V;
+    int32_t groups;           // ceil(V / 64)
+    int32_t chunks;
+    const int32_t *chunk_b;   // wide-vertex index of each chunk
+    const int32_t *chunk_e0;  // first edge of each chunk
+    const int32_t *wide;      // wide vertices
+    const int32_t *wide_c0;   // first chunk of each wide vertex (nwide + 1)
+    int32_t nwide;
+    float *partial;           // one per chunk
+    const float *contrib;     // this iteration's contributions
+    float *contrib_next;
+    float *rank;              // written when write_rank
+    float damping;
+    float beta;               // (1 - d) / N
+    int32_t write_rank;
+};
+
+__device__ __forceinline__ void pr_store(const PrArgs &a, int32_t v, float sum) {
+    float r = a.beta + a.damping * sum;
+    if (a.write_rank) a.rank[v] = r;
+    a.contrib_next[v] = r / (float)(a.off[v + 1] - a.off[v]);  // apps/pr.cpp:28-30
+}
+
+__global__ void __launch_bounds__(kPrBlock) pr_gather_kernel(PrArgs a) {
+    __shared__ float acc_s[kPrBlock / 64][64];
+    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    float *acc = acc_s[wid];
+    const int32_t waves = gridDim.x * (kPrBlock / 64);
+    const int32_t items = a.groups + a.chunks;
+    for (int32_t item = blockIdx.x * (kPrBlock / 64) + wid; item < items; item += waves) {
+        if (item < a.groups) {
+            const int32_t v = item * 64 + lane;
+            int32_t e0 = 0, len = 0;
+            bool narrow = false;
+            if (v < a.V) {
+                e0 = a.poff[v];
+                len = a.poff[v + 1] - e0;
+                narrow = len < kPrNarrow;
+                if (!narrow) len = 0;
+            }
+            acc[lane] = 0.f;
+            int32_t incl = len;
+            for (int o = 1; o < 64; o <<= 1) {
+                int32_t y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            const int32_t total = __shfl(incl, 63);
+            const int32_t pre = incl - len;
+            __builtin_amdgcn_wave_barrier();
+            for (int32_t base = 0; base < total; base += 64) {
+                const int32_t g = base + lane;
+                int32_t j = 0;  // owner: last lane whose exclusive prefix is <= g
+                for (int32_t step = 32; step > 0; step >>= 1) {
+                    int32_t t = j + step;
+                    int32_t pt = __shfl(pre, t & 63);
+                    if (t < 64 && pt <= g) j = t;
+                }
+                const bool valid = g < total;
+                const int32_t ej = __shfl(e0, j), pj = __shfl(pre, j);  // every lane takes part
+                float x = 0.f;
+                if (valid) x = a.contrib[a.pnbr[ej + (g - pj)]];
+                else j = 64;  // past the last owner
+                // segmented inclusive scan (segments = runs of equal owner)
+                for (int o = 1; o < 64; o <<= 1) {
+                    float y = __shfl_up(x, o);
+                    int32_t jo = __shfl_up(j, o);
+                    if (lane >= o && jo == j) x += y;
+                }
+                const int32_t jn = __shfl_down(j, 1);
+                if (valid && (lane == 63 || jn != j)) acc[j] += x;  // segment end
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (narrow) pr_store(a, v, acc[lane]);
+            __builtin_amdgcn_wave_barrier();
+        } else {
+            const int32_t k = item - a.groups;
+            const int32_t b = a.chunk_b[k];
+            const int32_t vw = a.wide[b];
+            const int32_t e0 = a.chunk_e0[k];
+            const int32_t end = a.poff[vw + 1];
+            const int32_t e1 = e0 + kPrChunk < end ? e0 + kPrChunk : end;
+            float x = 0.f;
+            for (int32_t e = e0 + lane; e < e1; e += 64) x += a.contrib[a.pnbr[e]];
+            for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+            if (lane == 0) a.partial[k] = x;
+        }
+    }
+}
+
+__global__ void pr_finalize_kernel(PrArgs a) {
+    for (int32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < a.nwide; b += gridDim.x * blockDim.x) {
+        float sum = 0.f;
+        for (int32_t k = a.wide_c0[b]; k < a.wide_c0[b + 1]; ++k) sum += a.partial[k];
+        pr_store(a, a.wide[b], sum);
+    }
+}
+
+__global__ void pr_init_kernel(const int32_t *off, int32_t V, float *contrib, float *rank) {
+    const float r0 = (float)(1.0 / (double)V);  // apps/pr.cpp:43-46
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        rank[v] = r0;
+        contrib[v] = r0 / (float)(off[v + 1] - off[v]);
+    }
+}
+
+// ---------------------------------------------------------------- host side
+struct PrPlan {
+    const int32_t *poff = nullptr, *pnbr = nullptr;
+    int32_t nwide = 0, chunks = 0;
+    int32_t *wide = nullptr, *wide_c0 = nullptr, *chunk_b = nullptr, *chunk_e0 = nullptr;
+    float *partial = nullptr;
+    float *contrib[2] = {nullptr, nullptr};
+    hipStream_t cap = nullptr;
+    // captured runs keyed by (iterations, output array, damping bits)
+    std::map<std::tuple<int32_t, const float *, uint32_t>, hipGraphExec_t> execs;
+};
+
+static std::map<const gi_graph *, PrPlan *> g_pr;
+
+static void free_plan(PrPlan *p) {
+    for (auto &kv : p->execs) (void)hipGraphExecDestroy(kv.second);
+    if (p->cap) (void)hipStreamDestroy(p->cap);
+    dfree(p->wide);
+    dfree(p->wide_c0);
+    dfree(p->chunk_b);
+    dfree(p->chunk_e0);
+    dfree(p->partial);
+    dfree(p->contrib[0]);
+    dfree(p->contrib[1]);
+    delete p;
+}
+
+void release_pr(const gi_graph *g) {
+    auto it = g_pr.find(g);
+    if (it == g_pr.end()) return;
+    free_plan(it->second);
+    g_pr.erase(it);
+}
+
+static int ensure_plan(gi_graph *g, PrPlan **out) {
+    auto it = g_pr.find(g);
+    if (it != g_pr.end()) {
+        *out = it->second;
+        return GI_OK;
+    }
+    gi_graph *pg = g;
+    if (g->directed) GI_TRY(gi_graph_transpose(g, &pg));
+    const int32_t V = g->V;
+    std::vector<int32_t> poff((size_t)V + 1);
+    GI_HIP(hipMemcpy(poff.data(), pg->d_off, sizeof(int32_t) * (V + 1), hipMemcpyDeviceToHost));
+    std::vector<int32_t> wide, wide_c0(1, 0), chunk_b, chunk_e0;
+    for (int32_t v = 0; v < V; ++v) {
+        int32_t d = poff[v + 1] - poff[v];
+        if (d < kPrNarrow) continue;
+        int32_t b = (int32_t)wide.size();
+        wide.push_back(v);
+        for (int32_t e = poff[v]; e < poff[v + 1]; e += kPrChunk) {
+            chunk_b.push_back(b);
+            chunk_e0.push_back(e);
+        }
+        wide_c0.push_back((int32_t)chunk_b.size());
+    }
+    PrPlan *p = new PrPlan();
+    p->poff = pg->d_off;
+    p->pnbr = pg->d_nbr;
+    p->nwide = (int32_t)wide.size();
+    p->chunks = (int32_t)chunk_b.size();
+    int rc = dalloc(&p->wide, wide.size() + 1);
+    if (rc == GI_OK) rc = dalloc(&p->wide_c0, wide_c0.size());
+    if (rc == GI_OK) rc = dalloc(&p->chunk_b, chunk_b.size() + 1);
+    if (rc == GI_OK) rc = dalloc(&p->chunk_e0, chunk_e0.size() + 1);
+    if (rc == GI_OK) rc = dalloc(&p->partial, chunk_b.size() + 1);
+    if (rc == GI_OK) rc = dalloc(&p->contrib[0], (size_t)V + 1);
+    if (rc == GI_OK) rc = dalloc(&p->contrib[1], (size_t)V + 1);
+    auto up = [&](int32_t *d, const std::vector<int32_t> &h) {
+        if (rc == GI_OK && !h.empty() &&
+            hipMemcpy(d, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice) != hipSuccess)
+            rc = fail(GI_ERR_HIP, "plan upload");
+    };
+    up(p->wide, wide);
+    up(p->wide_c0, wide_c0);
+    up(p->chunk_b, chunk_b);
+    up(p->chunk_e0, chunk_e0);
+    if (rc == GI_OK && hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(GI_ERR_HIP, "stream create failed");
+    if (rc != GI_OK) {
+        free_plan(p);
+        return rc;
+    }
+    g_pr[g] = p;
+    *out = p;
+    return GI_OK;
+}
+
+static int gather_grid() {
+    static int per_cu = -1;
+    if (per_cu < 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pr_gather_kernel, kPrBlock, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 4;
+    }
+    return device_info().num_cus * per_cu;
+}
+
+static void launch_iters(gi_graph *g, PrPlan *p, int32_t iters, float damping, float *rank,
+                         hipStream_t s) {
+    PrArgs a;
+    a.poff = p->poff;
+    a.pnbr = p->pnbr;
+    a.off = g->d_off;
+    a.V = g->V;
+    a.groups = (int32_t)ceil_div(g->V, 64);
+    a.chunks = p->chunks;
+    a.chunk_b = p->chunk_b;
+    a.chunk_e0 = p->chunk_e0;
+    a.wide = p->wide;
+    a.wide_c0 = p->wide_c0;
+    a.nwide = p->nwide;
+    a.partial = p->partial;
+    a.rank = rank;
+    a.damping = damping;
+    a.beta = (float)((1.0 - (double)damping) / (double)g->V);  // apps/pr.cpp:36-41
+    int ig = (int)ceil_div(g->V, 256);
+    pr_init_kernel<<<ig < 4096 ? ig : 4096, 256, 0, s>>>(g->d_off, g->V, p->contrib[0], rank);
+    const int grid = gather_grid();
+    const int fg = (int)ceil_div(p->nwide > 0 ? p->nwide : 1, 256);
+    for (int32_t it = 0; it < iters; ++it) {
+        a.contrib = p->contrib[it & 1];
+        a.contrib_next = p->contrib[(it + 1) & 1];
+        a.write_rank = it + 1 == iters;
+        pr_gather_kernel<<<grid, kPrBlock, 0, s>>>(a);
+        if (p->nwide > 0) pr_finalize_kernel<<<fg < 4096 ? fg : 4096, 256, 0, s>>>(a);
+    }
+}
+
+int pr_fused_run(gi_graph *g, int32_t iters, float damping, float *d_rank, hipStream_t s) {
+    if (g->V == 0) return GI_OK;
+    PrPlan *p = nullptr;
+    GI_TRY(ensure_plan(g, &p));
+    if (iters == 0) {
+        int ig = (int)ceil_div(g->V, 256);
+        pr_init_kernel<<<ig < 4096 ? ig : 4096, 256, 0, s>>>(g->d_off, g->V, p->contrib[0], d_rank);
+        GI_LAUNCH_CHECK();
+        return GI_OK;
+    }
+    // the whole run is one captured graph per (iteration count, output, damping)
+    uint32_t dbits = 0;
+    memcpy(&dbits, &damping, sizeof(dbits));
+    auto key = std::make_tuple(iters, (const float *)d_rank, dbits);
+    auto it = p->execs.find(key);
+    if (it == p->execs.end()) {
+        if (p->execs.size() >= 8) {  // bounded cache
+            for (auto &kv : p->execs) (void)hipGraphExecDestroy(kv.second);
+            p->execs.clear();
+        }
+        hipGraph_t graph = nullptr;
+        GI_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+        launch_iters(g, p, iters, damping, d_rank, p->cap);
+        hipError_t e = hipStreamEndCapture(p->cap, &graph);
+        if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+        hipGraphExec_t exec = nullptr;
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+        it = p->execs.emplace(key, exec).first;
+    }
+    GI_HIP(hipGraphLaunch(it->second, s));
+    return GI_OK;
+}
+
+}  // namespace gi

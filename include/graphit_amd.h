@@ -396,10 +396,18 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
 /* Delta-stepping SSSP (apps/sssp.cpp:34-57); d_dist int32, INT_MAX unreached. */
 int gi_sssp(gi_graph *g, int32_t source, int32_t delta, const gi_schedule *s,
             int32_t *d_dist, gi_run_stats *stats, void *stream);
-/* Connected components, hook + pointer jumping (apps/cc.cpp:53-78). */
+/* Connected components, hook + pointer jumping (apps/cc.cpp:53-78); labels
+ * are the minimum vertex id of each component. Default schedule (s == NULL):
+ * the app's PUSH EDGE_ONLY (apps/cc.cpp:84-86). With kernel_fusion ENABLED
+ * (symmetric graphs) the rounds run as a captured graph with device-side
+ * termination and full shortcutting per round. */
 int gi_cc(gi_graph *g, const gi_schedule *s, int32_t *d_labels, gi_run_stats *stats,
           void *stream);
-/* PageRank (apps/pr.cpp:49-69): iters iterations, FP32; d_rank = old_rank. */
+/* PageRank (apps/pr.cpp:49-69): iters iterations, FP32; d_rank = old_rank.
+ * Default schedule (s == NULL): the app's PULL EDGE_ONLY BLOCKED 0x42000
+ * (apps/pr.cpp:78-81). With kernel_fusion ENABLED (direction PULL) the whole
+ * run is one captured graph of deterministic pull sweeps over the in-edges
+ * (no atomics). */
 int gi_pagerank(gi_graph *g, int32_t iters, float damping, const gi_schedule *s,
                 float *d_rank, gi_run_stats *stats, void *stream);
 /* Brandes BC from num_sources sources (apps/bc.cpp:150-209), summed into

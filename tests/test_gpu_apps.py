@@ -63,6 +63,62 @@ def test_pagerank_schedules(gi, direction, lb):
     assert rel_l1(r, O.pagerank(host_csr(g), 20)) <= 1e-5
 
 
+@pytest.mark.parametrize("graph", ["rmat", "grid", "forest"])
+def test_cc_fused_matches_oracle(gi, graph):
+    if graph == "rmat":
+        g = gi.Graph.rmat(15, 8, SEED, True)
+    elif graph == "grid":
+        g = gi.Graph.grid(300, 40, 3)
+    else:  # many small components, isolated vertices, long paths
+        n = 50000
+        rng = np.random.default_rng(9)
+        a = rng.integers(0, n, 30000)
+        b = np.minimum(a + rng.integers(1, 3, 30000), n - 1)
+        path = np.arange(20000, 29999)
+        e = np.concatenate([np.stack([a, b]), np.stack([path, path + 1])], axis=1).astype(np.int32)
+        g = gi.Graph.from_edges(n, e[0], e[1], True)
+    host = host_csr(g)
+    want = O.cc(host)
+    sched = gi.Schedule(direction=0, load_balancing=6, kernel_fusion=1)
+    for _ in range(2):  # second run replays the captured rounds
+        lab, st = gi.cc(g, schedule=sched)
+        assert np.array_equal(lab, want)
+        assert st["iterations"] >= 1
+
+
+def _fused_pr(gi):
+    return gi.Schedule(direction=1, load_balancing=0, kernel_fusion=1)
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_pagerank_fused_matches_oracle(gi, directed):
+    g = gi.Graph.rmat(14, 16, SEED, not directed)
+    want = O.pagerank(host_csr(g), 20)
+    r1, _ = gi.pagerank(g, 20, schedule=_fused_pr(gi))
+    assert rel_l1(r1, want) <= 1e-5
+    r2, _ = gi.pagerank(g, 20, schedule=_fused_pr(gi))  # captured graph replayed
+    assert np.array_equal(r1, r2)  # fixed summation order: bit-identical
+
+
+def test_pagerank_fused_degree_boundaries(gi):
+    # in-degrees 63, 64 (first wide), 512, 513 and 1100 (chunk edges), a
+    # vertex with no out-edges and one with no in-edges
+    n = 1200
+    src, dst = [], []
+    for hub, deg in ((0, 63), (1, 64), (2, 512), (3, 513), (4, 1100)):
+        for k in range(deg):
+            src.append(10 + (k * 11 + hub) % (n - 10))  # distinct for k < n - 10
+            dst.append(hub)
+    src += list(range(10, n - 1))
+    dst += list(range(11, n))
+    e = np.array([src, dst], np.int32)
+    g = gi.Graph.from_edges(n, e[0], e[1], False)
+    host = O.build_csr(n, e[0], e[1], False)
+    for iters, d in ((0, 0.85), (1, 0.85), (20, 0.85), (7, 0.5)):
+        r, _ = gi.pagerank(g, iters, damping=d, schedule=_fused_pr(gi))
+        assert rel_l1(r, O.pagerank(host, iters, d)) <= 1e-5, (iters, d)
+
+
 def test_bc_small(gi):
     g = gi.Graph.rmat(12, 8, SEED, True)
     host = host_csr(g)
