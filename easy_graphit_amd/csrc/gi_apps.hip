@@ -488,6 +488,22 @@ int gi_sssp(gi_graph *g, int32_t source, int32_t delta, const gi_schedule *sched
     GI_TRY(validate_schedule(sched));
     if (sched->direction != GI_PUSH) return fail(GI_ERR_UNSUPPORTED, "SSSP runs push schedules");
     hipStream_t s = (hipStream_t)stream;
+    if (sched->kernel_fusion == GI_FUSION_ENABLED) {
+        // fused engine: windows and phases decided on the device
+        EventTimer timer(s);
+        int32_t steps = 0;
+        GI_TRY(sssp_fused_run(g, source, delta, d_dist, &steps, s));
+        double ms = timer.stop();
+        if (stats) {
+            stats->iterations = steps;
+            stats->push_iterations = steps;
+            stats->pull_iterations = 0;
+            stats->ms = ms;
+            stats->reached = 0;
+            stats->edges_traversed = 0;
+        }
+        return GI_OK;
+    }
     gi_pq *pq = nullptr;
     GI_TRY(gi_pq_create(g, d_dist, 0, delta, source, &pq));
     EventTimer timer(s);

@@ -748,6 +748,7 @@ int gi_graph_destroy(gi_graph *g) {
     release_fused(g);
     release_pr(g);
     release_cc(g);
+    release_sssp(g);
     delete g;
     return GI_OK;
 }

@@ -273,4 +273,7 @@ int pr_fused_run(gi_graph *g, int32_t iters, float damping, float *d_rank, hipSt
 void release_pr(const gi_graph *g);
 int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds, hipStream_t s);
 void release_cc(const gi_graph *g);
+int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *steps,
+                   hipStream_t s);
+void release_sssp(const gi_graph *g);
 }  // namespace gi

@@ -1,0 +1,718 @@
+// Fused delta-stepping SSSP: the loop of apps/sssp.cpp:34-57 (dequeue the
+// current priority window, relax its out-edges with writeMin, re-enqueue
+// improved vertices) with the window bookkeeping on the device and the
+// phases captured as hipGraphs. Shortest-path distances are unique, so the
+// order of relaxations does not change the result.
+//
+// Sets (bitmaps over the vertices):
+//   * the near frontier F: vertices improved below the window's upper bound
+//     `hi`, three rotating bitmaps with chunk summaries (as in the fused BFS);
+//   * the far set: vertices improved to a distance >= hi, waiting for a
+//     later window (the reference's priority queue, gpu_priority_queue.h).
+// A captured slot is six launches, each a no-op unless its mode is active:
+//   ss_step    push: relax the out-edges of F (lane / wave / grid-wide hub
+//              bins); an improvement below hi joins F', else the far set;
+//   ss_pull    pull, for a large F: each unsettled vertex takes the minimum
+//              over its in-neighbours in F (no atomics on distances);
+//   ss_hub     the grid-wide bin: edges of very wide frontier vertices;
+//   ss_decide  count F'; empty -> advance the window [hi, hi + delta);
+//   ss_farmove move far vertices below the new hi into F (wave per word,
+//              coalesced distance loads);
+//   ss_decide2 count the moves; none -> next window, or done when the far
+//              set is empty.
+#include <climits>
+#include <map>
+#include <vector>
+
+#include "gi_bitset.h"
+#include "gi_lb.h"
+
+namespace gi {
+
+constexpr int kSsBlock = 256;
+constexpr int32_t kSsLaneMax = 64;    // narrower vertices: one lane walks the edges
+constexpr int32_t kSsHubMin = 4096;   // at least this wide: grid-wide bin
+constexpr int32_t kSsMaxHubs = 1024;
+constexpr int32_t kSsMaxGrid = 8192;  // partial-sum slots per kernel
+constexpr int32_t kSsPullDiv = 8;     // a step pulls when |F| > V / kSsPullDiv
+
+enum SsMode : int32_t { SS_RELAX = 0, SS_ADVANCE = 1 };
+
+struct SsState {
+    int32_t done;
+    int32_t mode;
+    int32_t phase;       // F = bm[phase % 3]
+    int32_t steps;       // relax steps run
+    int32_t windows;     // windows entered after the first
+    int32_t slots;       // captured slots consumed
+    int32_t hub_count;
+    int32_t hi;          // window upper bound (exclusive)
+    int32_t delta;
+    int32_t front;       // |F|
+    long long far;       // |far set|
+    int32_t pull;        // the next relax step pulls (large frontier)
+    int32_t pull_min;    // |F| above which a step pulls
+    int32_t hubs[kSsMaxHubs];
+};
+
+struct SsArgs {
+    SsState *st;
+    const int32_t *off;
+    const int32_t *nbr;
+    const int32_t *wt;
+    const int32_t *poff;  // in-edges with their weights (the graph itself when symmetric)
+    const int32_t *pnbr;
+    const int32_t *pwt;
+    int32_t *dist;
+    int32_t V;
+    int32_t words;
+    int32_t sum_words;
+    uint64_t *bm[3];
+    uint64_t *sum[3];
+    uint64_t *far;
+    int32_t *slots;      // [3 kernels][3 counters][kSsMaxGrid]
+    int32_t wide_chunks;       // pull: chunks of the in-lists of wide vertices
+    const int32_t *chunk_v;    // vertex of each chunk
+    const int32_t *chunk_e0;   // first in-edge of each chunk
+};
+
+struct SsCtx {
+    int32_t hi;
+    uint64_t *bm_out;
+    uint64_t *sum_out;
+    int32_t nf;    // per-lane: new members of F'
+    int32_t nfar;  // per-lane: new members of the far set
+};
+
+// writeMin relaxation (apps/sssp.cpp:24-27): a strict improvement joins F'
+// when below hi, else the far set.
+__device__ __forceinline__ void ss_relax(const SsArgs &a, SsCtx &c, int32_t du, int32_t e, bool active) {
+    if (!active) return;
+    const int32_t v = a.nbr[e];
+    const int32_t nd = du + a.wt[e];
+    if (nd >= a.dist[v]) return;  // test before the atomic
+    const int32_t old = atomicMin(&a.dist[v], nd);
+    if (nd >= old) return;
+    const uint64_t bit = 1ull << (v & 63);
+    if (nd < c.hi) {
+        const unsigned long long o = atomicOr((unsigned long long *)&c.bm_out[v >> 6], bit);
+        if (!(o & bit)) {
+            c.nf += 1;
+            if (o == 0) bs_mark_chunk(c.sum_out, v >> 6);
+        }
+    } else {
+        const unsigned long long o = atomicOr((unsigned long long *)&a.far[v >> 6], bit);
+        if (!(o & bit)) c.nfar += 1;
+    }
+}
+
+__global__ void __launch_bounds__(kSsBlock) ss_step_kernel(SsArgs a) {
+    SsState *st = a.st;
+    if (st->done || st->mode != SS_RELAX || st->pull) return;  // uniform
+    __shared__ int32_t pre[kBsMaxSumWords + 1];
+    __shared__ int32_t list[kBsChunkVerts];
+    __shared__ int32_t n_s;
+    const int32_t phase = st->phase;
+    const int32_t iin = phase % 3, iout = (phase + 1) % 3, iclr = (phase + 2) % 3;
+    SsCtx c;
+    c.hi = st->hi;
+    c.bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iout);
+    c.sum_out = bs_pick3(a.sum[0], a.sum[1], a.sum[2], iout);
+    c.nf = 0;
+    c.nfar = 0;
+    // the bitmap two phases ahead is free (its summary is cleared by ss_decide)
+    bs_clear_chunks(bs_pick3(a.bm[0], a.bm[1], a.bm[2], iclr), bs_pick3(a.sum[0], a.sum[1], a.sum[2], iclr),
+                    a.words, a.sum_words, pre);
+    const uint64_t *bm_in = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iin);
+    const uint64_t *si = bs_pick3(a.sum[0], a.sum[1], a.sum[2], iin);
+    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    const int32_t nz = bs_summary_prefix(si, a.sum_words, pre);
+    for (int32_t k = blockIdx.x; k < nz; k += gridDim.x) {
+        const int32_t chunk = bs_kth_chunk(si, pre, a.sum_words, k);
+        if (threadIdx.x < 64) {  // wave 0 lists the chunk's members
+            const int32_t w = chunk * kBsChunkWords + (int32_t)threadIdx.x;
+            uint64_t word = w < a.words ? bm_in[w] : 0ull;
+            const int32_t pc = __popcll(word);
+            int32_t incl = pc;
+            for (int o = 1; o < 64; o <<= 1) {
+                int32_t y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            int32_t pos = incl - pc;
+            while (word) {
+                const int32_t b = __ffsll((unsigned long long)word) - 1;
+                word &= word - 1;
+                list[pos++] = w * 64 + b;
+            }
+            if (threadIdx.x == 63) n_s = incl;
+        }
+        __syncthreads();
+        const int32_t n = n_s;
+        for (int32_t b0 = 0; b0 < n; b0 += kSsBlock) {
+            const int32_t j = b0 + (int32_t)threadIdx.x;
+            const bool has = j < n;
+            const int32_t s = has ? list[j] : 0;
+            const int32_t e0 = has ? a.off[s] : 0, e1 = has ? a.off[s + 1] : 0;
+            const int32_t du = has ? a.dist[s] : 0;
+            const int32_t dg = e1 - e0;
+            const bool hub = has && dg >= kSsHubMin;
+            const int32_t hpos = wave_atomic_add(&st->hub_count, hub);
+            if (hub && hpos < kSsMaxHubs) st->hubs[hpos] = s;
+            const bool wide = has && dg >= kSsLaneMax && (!hub || hpos >= kSsMaxHubs);
+            uint64_t wm = __ballot(wide);
+            while (wm) {  // wave bin
+                const int32_t l = __ffsll((unsigned long long)wm) - 1;
+                wm &= wm - 1;
+                const int32_t w0 = __shfl(e0, l), w1 = __shfl(e1, l), wd = __shfl(du, l);
+                for (int32_t bb = w0; bb < w1; bb += 64) ss_relax(a, c, wd, bb + lane, bb + lane < w1);
+            }
+            if (has && dg < kSsLaneMax)  // lane bin
+                for (int32_t e = e0; e < e1; ++e) ss_relax(a, c, du, e, true);
+        }
+        __syncthreads();
+    }
+    (void)wid;
+    bs_flush3<kSsBlock>(a.slots, kSsMaxGrid, c.nf, c.nfar, 0);
+}
+
+// Pull relax step for a large frontier: every vertex not yet settled below
+// the window takes the minimum of dist[u] + w over its in-neighbours u in F
+// (no atomics on dist: a vertex's word of 64 vertices belongs to one wave).
+// Work is in units of 16 words per workgroup (4 per wave), so a unit's
+// summary bit costs at most one atomic.
+constexpr int kSsPullUnit = 4 * (kSsBlock / 64);
+constexpr int32_t kSsPullWide = 4096;            // in-degree at least this: chunked pull
+constexpr int32_t kSsPullChunk = 16 * 1024;      // in-edges per chunk (a workgroup's item)
+constexpr int kSsPasses = 4;                     // cooperative passes in flight per wave
+
+__global__ void __launch_bounds__(kSsBlock) ss_pull_kernel(SsArgs a) {
+    SsState *st = a.st;
+    if (st->done || st->mode != SS_RELAX || !st->pull) return;  // uniform
+    __shared__ int32_t pre[kBsMaxSumWords + 1];
+    __shared__ int32_t best_s[kSsBlock / 64][64];
+    const int32_t phase = st->phase, hi = st->hi, lo = st->hi - st->delta;
+    const int32_t iin = phase % 3, iout = (phase + 1) % 3, iclr = (phase + 2) % 3;
+    bs_clear_chunks(bs_pick3(a.bm[0], a.bm[1], a.bm[2], iclr), bs_pick3(a.sum[0], a.sum[1], a.sum[2], iclr),
+                    a.words, a.sum_words, pre);
+    const uint64_t *bm_in = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iin);
+    uint64_t *bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iout);
+    uint64_t *sum_out = bs_pick3(a.sum[0], a.sum[1], a.sum[2], iout);
+    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    int32_t *best = best_s[wid];
+    int32_t nf = 0, nfar = 0;
+    const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
+    for (int32_t un = blockIdx.x; un < units; un += gridDim.x) {
+        bool any = false;
+        for (int32_t i = 0; i < 4; ++i) {
+            const int32_t w = un * kSsPullUnit + wid * 4 + i;
+            if (w >= a.words) break;  // wave-uniform
+            const int32_t v = w * 64 + lane;
+            const int32_t dv = v < a.V ? a.dist[v] : INT_MAX;
+            bool open = v < a.V && dv > lo;  // settled vertices cannot improve
+            const int32_t e0 = open ? a.poff[v] : 0;
+            int32_t len = open ? a.poff[v + 1] - e0 : 0;
+            if (len >= kSsPullWide) {  // wide: the chunk items below handle it
+                open = false;
+                len = 0;
+            }
+            best[lane] = INT_MAX;
+            int32_t incl = len;
+            for (int o = 1; o < 64; o <<= 1) {
+                int32_t y = __shfl_up(incl, o);
+                if (lane >= o) incl += y;
+            }
+            const int32_t total = __shfl(incl, 63);
+            const int32_t pre_l = incl - len;
+            __builtin_amdgcn_wave_barrier();
+            // kSsPasses passes of 64 edges at a time: their loads are in flight together
+            for (int32_t base = 0; base < total; base += 64 * kSsPasses) {
+                int32_t jj[kSsPasses], ee[kSsPasses], uu[kSsPasses], ww[kSsPasses];
+#pragma unroll
+                for (int p = 0; p < kSsPasses; ++p) {
+                    const int32_t g = base + 64 * p + lane;
+                    int32_t j = 0;  // owner: the last lane whose exclusive prefix is <= g
+                    for (int32_t step = 32; step > 0; step >>= 1) {
+                        const int32_t t = j + step;
+                        const int32_t pt = __shfl(pre_l, t & 63);
+                        if (t < 64 && pt <= g) j = t;
+                    }
+                    const int32_t ej = __shfl(e0, j), pj = __shfl(pre_l, j);
+                    jj[p] = j;
+                    ee[p] = g < total ? ej + (g - pj) : -1;
+                }
+#pragma unroll
+                for (int p = 0; p < kSsPasses; ++p) {
+                    uu[p] = ee[p] >= 0 ? a.pnbr[ee[p]] : -1;
+                    ww[p] = ee[p] >= 0 ? a.pwt[ee[p]] : 0;
+                }
+#pragma unroll
+                for (int p = 0; p < kSsPasses; ++p)
+                    if (uu[p] >= 0 && bs_test(bm_in, uu[p])) atomicMin(&best[jj[p]], a.dist[uu[p]] + ww[p]);
+                __builtin_amdgcn_wave_barrier();
+            }
+            const int32_t nd = best[lane];
+            const bool imp = open && nd < dv;
+            if (imp) a.dist[v] = nd;
+            const uint64_t near = __ballot(imp && nd < hi);
+            const uint64_t farb = __ballot(imp && nd >= hi);
+            if (lane == 0) {  // atomics: a wide vertex of the same word may join too
+                if (near) {
+                    const uint64_t old = atomicOr((unsigned long long *)&bm_out[w], near);
+                    nf += __popcll(near & ~old);
+                    any = true;
+                }
+                if (farb) {
+                    const uint64_t old = atomicOr((unsigned long long *)&a.far[w], farb);
+                    nfar += __popcll(farb & ~old);
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (__syncthreads_or(any) && threadIdx.x == 0) bs_mark_chunk(sum_out, un * kSsPullUnit);
+    }
+    // wide vertices: each chunk of an in-list is a workgroup item; its minimum
+    // joins dist[v] by atomicMin, and an improvement joins F' or the far set
+    const int32_t per_wave = kSsPullChunk / (kSsBlock / 64);
+    for (int32_t k = blockIdx.x; k < a.wide_chunks; k += gridDim.x) {
+        const int32_t v = a.chunk_v[k];
+        const int32_t dv = a.dist[v];
+        if (dv <= lo) continue;  // settled (block-uniform)
+        const int32_t end = a.poff[v + 1];
+        const int32_t e0 = a.chunk_e0[k] + wid * per_wave;
+        const int32_t e1 = e0 + per_wave < end ? e0 + per_wave : end;
+        int32_t m = INT_MAX;
+#pragma unroll 4
+        for (int32_t e = e0 + lane; e < e1; e += 64) {
+            const int32_t u = a.pnbr[e];
+            if (bs_test(bm_in, u)) {
+                const int32_t c = a.dist[u] + a.pwt[e];
+                m = c < m ? c : m;
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const int32_t y = __shfl_xor(m, o);
+            m = y < m ? y : m;
+        }
+        if (lane == 0 && m < dv) {
+            const int32_t old = atomicMin(&a.dist[v], m);
+            if (m < old) {
+                const uint64_t bit = 1ull << (v & 63);
+                if (m < hi) {
+                    const unsigned long long o = atomicOr((unsigned long long *)&bm_out[v >> 6], bit);
+                    if (!(o & bit)) {
+                        nf += 1;
+                        if (o == 0) bs_mark_chunk(sum_out, v >> 6);
+                    }
+                } else {
+                    const unsigned long long o = atomicOr((unsigned long long *)&a.far[v >> 6], bit);
+                    if (!(o & bit)) nfar += 1;
+                }
+            }
+        }
+    }
+    bs_flush3<kSsBlock>(a.slots, kSsMaxGrid, nf, nfar, 0);
+}
+
+// Grid-wide bin: the concatenated edges of the deferred hubs, spread evenly.
+__global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a) {
+    SsState *st = a.st;
+    if (st->done || st->mode != SS_RELAX || st->pull) return;
+    int32_t nh = st->hub_count;
+    if (nh <= 0) return;  // slots stay zero
+    if (nh > kSsMaxHubs) nh = kSsMaxHubs;
+    const int32_t phase = st->phase;
+    SsCtx c;
+    c.hi = st->hi;
+    c.bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], (phase + 1) % 3);
+    c.sum_out = bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 1) % 3);
+    c.nf = 0;
+    c.nfar = 0;
+    __shared__ int32_t hscan[kSsMaxHubs + 1];
+    __shared__ int32_t hv[kSsMaxHubs];
+    __shared__ int32_t hd[kSsMaxHubs];
+    const int32_t per = (nh + kSsBlock - 1) / kSsBlock;
+    const int32_t lo = threadIdx.x * per, hi = lo + per < nh ? lo + per : nh;
+    int32_t part = 0;
+    for (int32_t i = lo; i < hi; ++i) {
+        const int32_t s = st->hubs[i];
+        hv[i] = s;
+        hd[i] = a.dist[s];
+        part += a.off[s + 1] - a.off[s];
+    }
+    int32_t total = 0;
+    int32_t ex = block_exclusive_scan(part, &total);
+    for (int32_t i = lo; i < hi; ++i) {
+        hscan[i] = ex;
+        ex += a.off[hv[i] + 1] - a.off[hv[i]];
+    }
+    __syncthreads();
+    const int32_t chunk = (total + gridDim.x - 1) / gridDim.x;
+    const int32_t k0 = blockIdx.x * chunk, k1 = k0 + chunk < total ? k0 + chunk : total;
+    for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += kSsBlock) {
+        const int32_t j = upper_bound_i32(hscan, 0, nh, k) - 1;
+        ss_relax(a, c, hd[j], a.off[hv[j]] + (k - hscan[j]), true);
+    }
+    bs_flush3<kSsBlock>(a.slots + 3 * kSsMaxGrid, kSsMaxGrid, c.nf, c.nfar, 0);
+}
+
+// Adds and zeroes counter `which` of slot region `region` over `slots` blocks.
+__device__ __forceinline__ long long ss_collect(int32_t *slot_base, int32_t region, int32_t which,
+                                                int32_t nslots) {
+    __shared__ long long red[kSsBlock / 64];
+    int32_t *p = slot_base + (region * 3 + which) * kSsMaxGrid;
+    long long t = 0;
+    for (int32_t i = threadIdx.x; i < nslots; i += kSsBlock) {
+        t += p[i];
+        p[i] = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
+    if (lane_id() == 0) red[threadIdx.x >> 6] = t;
+    __syncthreads();
+    long long s = 0;
+    for (int i = 0; i < kSsBlock / 64; ++i) s += red[i];
+    __syncthreads();
+    return s;
+}
+
+__global__ void __launch_bounds__(kSsBlock) ss_decide_kernel(SsArgs a, int32_t nslots) {
+    SsState *st = a.st;
+    if (st->done || st->mode != SS_RELAX) return;  // uniform
+    const long long nf = ss_collect(a.slots, 0, 0, nslots) + ss_collect(a.slots, 1, 0, nslots);
+    const long long nfar = ss_collect(a.slots, 0, 1, nslots) + ss_collect(a.slots, 1, 1, nslots);
+    const int32_t phase = st->phase;
+    uint64_t *sclr = bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 2) % 3);
+    for (int32_t i = threadIdx.x; i <= a.sum_words; i += kSsBlock) sclr[i] = 0;
+    if (threadIdx.x == 0) {
+        st->far += nfar;
+        st->phase = phase + 1;
+        st->steps += 1;
+        st->front = (int32_t)nf;
+        st->hub_count = 0;
+        st->pull = nf > st->pull_min;
+        if (nf == 0) {
+            if (st->far == 0) {
+                st->done = 1;
+            } else {  // the window is settled: the next one is [hi, hi + delta)
+                st->mode = SS_ADVANCE;
+                st->hi += st->delta;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kSsBlock) ss_farmove_kernel(SsArgs a) {
+    SsState *st = a.st;
+    if (st->done || st->mode != SS_ADVANCE) return;
+    const int32_t hi = st->hi, phase = st->phase;
+    uint64_t *f = bs_pick3(a.bm[0], a.bm[1], a.bm[2], phase % 3);  // empty: the last step found nothing
+    uint64_t *sf = bs_pick3(a.sum[0], a.sum[1], a.sum[2], phase % 3);
+    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    int32_t moved = 0;
+    // units of 16 words per workgroup (4 per wave, loads in flight together);
+    // a unit's summary bit costs at most one atomic
+    const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
+    for (int32_t un = blockIdx.x; un < units; un += gridDim.x) {
+        const int32_t w0 = un * kSsPullUnit + wid * 4;
+        uint64_t fw[4];
+        int32_t dv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fw[i] = w0 + i < a.words ? a.far[w0 + i] : 0ull;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int32_t v = (w0 + i) * 64 + lane;
+            dv[i] = ((fw[i] >> lane) & 1ull) ? a.dist[v] : INT_MAX;
+        }
+        bool any = false;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const uint64_t m = __ballot(dv[i] < hi);
+            if (m) {
+                any = true;
+                if (lane == 0) {
+                    a.far[w0 + i] = fw[i] & ~m;  // this wave owns the word
+                    f[w0 + i] = m;
+                    moved += __popcll(m);
+                }
+            }
+        }
+        if (__syncthreads_or(any) && threadIdx.x == 0) bs_mark_chunk(sf, un * kSsPullUnit);
+    }
+    bs_flush3<kSsBlock>(a.slots + 6 * kSsMaxGrid, kSsMaxGrid, moved, 0, 0);
+}
+
+__global__ void __launch_bounds__(kSsBlock) ss_decide2_kernel(SsArgs a, int32_t nslots) {
+    SsState *st = a.st;
+    if (st->done) return;
+    if (threadIdx.x == 0) st->slots += 1;
+    if (st->mode != SS_ADVANCE) return;  // uniform
+    const long long moved = ss_collect(a.slots, 2, 0, nslots);
+    if (threadIdx.x == 0) {
+        st->far -= moved;
+        if (moved > 0) {
+            st->mode = SS_RELAX;
+            st->front = (int32_t)moved;
+            st->pull = moved > st->pull_min;
+            st->windows += 1;
+        } else if (st->far == 0) {
+            st->done = 1;
+        } else {
+            st->hi += st->delta;  // empty window
+        }
+    }
+}
+
+struct SsSeed {
+    int32_t source;
+    int32_t delta;
+    int32_t pull_min;
+};
+
+__global__ void ss_reset_kernel(SsArgs a, SsSeed sd) {
+    const int32_t n = a.V > a.sum_words + 1 ? a.V : a.sum_words + 1;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (i < a.V) a.dist[i] = i == sd.source ? 0 : INT_MAX;
+        if (i < a.words) {
+            const uint64_t s = (i == (sd.source >> 6)) ? (1ull << (sd.source & 63)) : 0ull;
+            a.bm[0][i] = s;
+            a.bm[1][i] = 0;
+            a.bm[2][i] = 0;
+            a.far[i] = 0;
+        }
+        if (i <= a.sum_words) {
+            const int32_t ch = (sd.source >> 6) / kBsChunkWords;
+            a.sum[0][i] = (i == (ch >> 6)) ? (1ull << (ch & 63)) : 0ull;
+            a.sum[1][i] = 0;
+            a.sum[2][i] = 0;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        SsState *st = a.st;
+        st->done = 0;
+        st->mode = SS_RELAX;
+        st->phase = 0;
+        st->steps = 0;
+        st->windows = 0;
+        st->slots = 0;
+        st->hub_count = 0;
+        st->hi = sd.delta;
+        st->delta = sd.delta;
+        st->front = 1;
+        st->far = 0;
+        st->pull = 0;
+        st->pull_min = sd.pull_min;
+    }
+}
+
+__global__ void ss_zero_slots_kernel(int32_t *slots, int32_t n) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) slots[i] = 0;
+}
+
+// ---------------------------------------------------------------- host side
+struct SsWork {
+    SsState *st = nullptr;
+    uint64_t *bm[3] = {nullptr, nullptr, nullptr};
+    uint64_t *sum[3] = {nullptr, nullptr, nullptr};
+    uint64_t *far = nullptr;
+    int32_t *slots = nullptr;
+    int32_t sum_words = 0;
+    int32_t *host = nullptr;  // pinned: done, mode, phase, steps, windows, slots
+    const int32_t *poff = nullptr, *pnbr = nullptr, *pwt = nullptr;
+    int32_t pull_min = 0;
+    int32_t wide_chunks = 0;
+    int32_t *chunk_v = nullptr, *chunk_e0 = nullptr;
+    hipStream_t cap = nullptr;
+    std::map<std::pair<int32_t, int32_t *>, hipGraphExec_t> execs;  // (slots, dist)
+    int32_t last_slots = 0;
+};
+
+static std::map<const gi_graph *, SsWork *> g_ss;
+
+static void free_ss(SsWork *w) {
+    for (auto &kv : w->execs) (void)hipGraphExecDestroy(kv.second);
+    if (w->cap) (void)hipStreamDestroy(w->cap);
+    dfree(w->st);
+    for (auto p : w->bm) dfree(p);
+    for (auto p : w->sum) dfree(p);
+    dfree(w->far);
+    dfree(w->slots);
+    dfree(w->chunk_v);
+    dfree(w->chunk_e0);
+    if (w->host) (void)hipHostFree(w->host);
+    delete w;
+}
+
+void release_sssp(const gi_graph *g) {
+    auto it = g_ss.find(g);
+    if (it == g_ss.end()) return;
+    free_ss(it->second);
+    g_ss.erase(it);
+}
+
+static int ensure_ss(gi_graph *g, SsWork **out) {
+    auto it = g_ss.find(g);
+    if (it != g_ss.end()) {
+        *out = it->second;
+        return GI_OK;
+    }
+    const int64_t words = bitmap_words(g->V);
+    const int64_t sum_words = ceil_div(ceil_div(words, kBsChunkWords), 64);
+    if (sum_words > kBsMaxSumWords) return fail(GI_ERR_RANGE, "fused SSSP supports up to 2^26 vertices");
+    const gi_graph *pg = g;  // in-edges for pull steps
+    if (g->directed) {
+        gi_graph *t = nullptr;
+        GI_TRY(gi_graph_transpose(g, &t));
+        pg = t;
+    }
+    if (!pg->d_w) return fail(GI_ERR_INVALID, "SSSP needs edge weights");
+    SsWork *w = new SsWork();
+    w->sum_words = (int32_t)sum_words;
+    w->poff = pg->d_off;
+    w->pnbr = pg->d_nbr;
+    w->pwt = pg->d_w;
+    w->pull_min = g->V / kSsPullDiv;
+    int rc = dalloc(&w->st, 1);
+    for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->bm[i], (size_t)words + 1);
+    for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->sum[i], (size_t)sum_words + 1);
+    if (rc == GI_OK) rc = dalloc(&w->far, (size_t)words + 1);
+    if (rc == GI_OK) rc = dalloc(&w->slots, 9 * (size_t)kSsMaxGrid);
+    if (rc == GI_OK && hipHostMalloc((void **)&w->host, 8 * sizeof(int32_t)) != hipSuccess)
+        rc = fail(GI_ERR_OOM, "pinned allocation failed");
+    if (rc == GI_OK && hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking) != hipSuccess)
+        rc = fail(GI_ERR_HIP, "stream create failed");
+    if (rc == GI_OK) {  // chunks of the wide in-lists, for pull steps
+        std::vector<int32_t> poff((size_t)g->V + 1), cv, ce;
+        if (hipMemcpy(poff.data(), pg->d_off, sizeof(int32_t) * (g->V + 1), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(GI_ERR_HIP, "offsets read");
+        for (int32_t v = 0; rc == GI_OK && v < g->V; ++v) {
+            if (poff[v + 1] - poff[v] < kSsPullWide) continue;
+            for (int32_t e = poff[v]; e < poff[v + 1]; e += kSsPullChunk) {
+                cv.push_back(v);
+                ce.push_back(e);
+            }
+        }
+        w->wide_chunks = (int32_t)cv.size();
+        if (rc == GI_OK) rc = dalloc(&w->chunk_v, cv.size() + 1);
+        if (rc == GI_OK) rc = dalloc(&w->chunk_e0, ce.size() + 1);
+        if (rc == GI_OK && !cv.empty() &&
+            (hipMemcpy(w->chunk_v, cv.data(), sizeof(int32_t) * cv.size(), hipMemcpyHostToDevice) != hipSuccess ||
+             hipMemcpy(w->chunk_e0, ce.data(), sizeof(int32_t) * ce.size(), hipMemcpyHostToDevice) != hipSuccess))
+            rc = fail(GI_ERR_HIP, "plan upload");
+    }
+    if (rc == GI_OK) {
+        ss_zero_slots_kernel<<<64, 256>>>(w->slots, 9 * kSsMaxGrid);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+            rc = fail(GI_ERR_HIP, "slot init");
+    }
+    if (rc != GI_OK) {
+        free_ss(w);
+        return rc;
+    }
+    g_ss[g] = w;
+    *out = w;
+    return GI_OK;
+}
+
+// Persistent grid: resident workgroups of the step kernel.
+static int ss_grid() {
+    static int per_cu = -1;
+    if (per_cu < 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ss_step_kernel, kSsBlock, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 4;
+    }
+    int g = device_info().num_cus * per_cu;
+    return g < kSsMaxGrid ? g : kSsMaxGrid;
+}
+
+static SsArgs ss_args(gi_graph *g, SsWork *w, int32_t *dist) {
+    SsArgs a;
+    a.st = w->st;
+    a.off = g->d_off;
+    a.nbr = g->d_nbr;
+    a.wt = g->d_w;
+    a.poff = w->poff;
+    a.pnbr = w->pnbr;
+    a.pwt = w->pwt;
+    a.dist = dist;
+    a.V = g->V;
+    a.words = (int32_t)bitmap_words(g->V);
+    a.sum_words = w->sum_words;
+    for (int i = 0; i < 3; ++i) {
+        a.bm[i] = w->bm[i];
+        a.sum[i] = w->sum[i];
+    }
+    a.far = w->far;
+    a.slots = w->slots;
+    a.wide_chunks = w->wide_chunks;
+    a.chunk_v = w->chunk_v;
+    a.chunk_e0 = w->chunk_e0;
+    return a;
+}
+
+static int ss_pull_grid() {
+    static int per_cu = -1;
+    if (per_cu < 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ss_pull_kernel, kSsBlock, 0) != hipSuccess ||
+            per_cu < 1)
+            per_cu = 4;
+    }
+    int g = device_info().num_cus * per_cu;
+    return g < kSsMaxGrid ? g : kSsMaxGrid;
+}
+
+static void launch_slot(const SsArgs &a, hipStream_t s) {
+    const int grid = ss_grid();
+    const int pgrid = ss_pull_grid();
+    ss_step_kernel<<<grid, kSsBlock, 0, s>>>(a);
+    ss_pull_kernel<<<pgrid, kSsBlock, 0, s>>>(a);
+    ss_hub_kernel<<<grid, kSsBlock, 0, s>>>(a);
+    ss_decide_kernel<<<1, kSsBlock, 0, s>>>(a, grid > pgrid ? grid : pgrid);
+    ss_farmove_kernel<<<grid, kSsBlock, 0, s>>>(a);
+    ss_decide2_kernel<<<1, kSsBlock, 0, s>>>(a, grid);
+}
+
+int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *steps_out,
+                   hipStream_t s) {
+    SsWork *w = nullptr;
+    GI_TRY(ensure_ss(g, &w));
+    SsArgs a = ss_args(g, w, d_dist);
+    const int32_t n = g->V > w->sum_words + 1 ? g->V : w->sum_words + 1;
+    int rg = (int)ceil_div(n, 256);
+    ss_reset_kernel<<<rg < 8192 ? rg : 8192, 256, 0, s>>>(a, SsSeed{source, delta, w->pull_min});
+    GI_LAUNCH_CHECK();
+    int32_t block = w->last_slots > 0 ? w->last_slots : 16;
+    for (int64_t used = 0; used < (1ll << 24);) {
+        if (block > 256) block = 256;
+        auto key = std::make_pair(block, d_dist);
+        auto it = w->execs.find(key);
+        if (it == w->execs.end()) {
+            if (w->execs.size() >= 8) {
+                for (auto &kv : w->execs) (void)hipGraphExecDestroy(kv.second);
+                w->execs.clear();
+            }
+            hipGraph_t graph = nullptr;
+            GI_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
+            for (int32_t r = 0; r < block; ++r) launch_slot(a, w->cap);
+            hipError_t e = hipStreamEndCapture(w->cap, &graph);
+            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            hipGraphExec_t exec = nullptr;
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+            it = w->execs.emplace(key, exec).first;
+        }
+        GI_HIP(hipGraphLaunch(it->second, s));
+        GI_HIP(hipMemcpyAsync(w->host, w->st, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GI_HIP(hipStreamSynchronize(s));
+        used += block;
+        if (w->host[0]) {  // done
+            w->last_slots = w->host[5];
+            if (steps_out) *steps_out = w->host[3];
+            return GI_OK;
+        }
+        block *= 2;
+    }
+    return fail(GI_ERR_RANGE, "SSSP did not finish");
+}
+
+}  // namespace gi

@@ -393,7 +393,11 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
                  float *features, float *times_ms, int32_t max_levels, int32_t *num_levels,
                  void *stream);
 
-/* Delta-stepping SSSP (apps/sssp.cpp:34-57); d_dist int32, INT_MAX unreached. */
+/* Delta-stepping SSSP (apps/sssp.cpp:34-57); d_dist int32, INT_MAX unreached.
+ * Default schedule (s == NULL): the app's PUSH TWCE with the bucketed
+ * priority queue (apps/sssp.cpp:77-79). With kernel_fusion ENABLED the
+ * windows and relaxation phases run on the device (near frontier and far
+ * set as bitmaps, captured phases, no host synchronisation per phase). */
 int gi_sssp(gi_graph *g, int32_t source, int32_t delta, const gi_schedule *s,
             int32_t *d_dist, gi_run_stats *stats, void *stream);
 /* Connected components, hook + pointer jumping (apps/cc.cpp:53-78); labels

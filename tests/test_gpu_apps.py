@@ -31,6 +31,62 @@ def test_sssp_rmat_default_schedule(gi):
         assert np.array_equal(dist, O.sssp(host, int(src)))
 
 
+def _fused_sssp(gi):
+    return gi.Schedule(direction=gi.PUSH, load_balancing=1, kernel_fusion=1)
+
+
+@pytest.mark.parametrize("delta", [1, 16, 64, 5000])
+def test_sssp_fused_rmat(gi, delta):
+    g = gi.Graph.rmat(14, 16, SEED, True).set_weights(SEED, 1, 255)
+    host = host_csr(g, True)
+    for src in O.pick_sources(host, 1, 3):
+        for _ in range(2):  # the second run replays the captured phases
+            dist, st = gi.sssp(g, int(src), delta, _fused_sssp(gi))
+            assert np.array_equal(dist, O.sssp(host, int(src))), (delta, src)
+
+
+def test_sssp_fused_grid_and_unreachable(gi):
+    g = gi.Graph.grid(96, 92, 3).set_weights(5, 1, 1000)
+    host = host_csr(g, True)
+    dist, _ = gi.sssp(g, 0, 64, _fused_sssp(gi))
+    assert np.array_equal(dist, O.sssp(host, 0))
+    # two components and an isolated vertex: unreached stay INT_MAX
+    e = np.array([[0, 1], [1, 2], [3, 4]], np.int32).T
+    g2 = gi.Graph.from_edges(6, e[0], e[1], True).set_weights(2, 1, 9)
+    h2 = host_csr(g2, True)
+    d2, _ = gi.sssp(g2, 0, 4, _fused_sssp(gi))
+    assert np.array_equal(d2, O.sssp(h2, 0)) and d2[3] == np.iinfo(np.int32).max
+
+
+def test_sssp_fused_hub_bin(gi):
+    # a centre wider than the grid-bin threshold, with a tail of long edges
+    n = 9003
+    e = [(0, i) for i in range(1, 9001)] + [(9000, 9001), (9001, 9002), (5, 9002)]
+    e = np.array(e, np.int32).T
+    g = gi.Graph.from_edges(n, e[0], e[1], True).set_weights(7, 1, 200)
+    host = host_csr(g, True)
+    for src in (0, 9002):
+        dist, _ = gi.sssp(g, src, 32, _fused_sssp(gi))
+        assert np.array_equal(dist, O.sssp(host, src))
+
+
+def test_sssp_fused_wide_and_narrow_share_a_word(gi):
+    # A pull step where a wide vertex (in-degree above the chunked-pull
+    # threshold) and narrow vertices of the same 64-vertex word improve
+    # together; each has a private successor that only it reaches.
+    n = 5400
+    e = [(100, leaf) for leaf in range(200, 5200)]          # frontier of 5000 leaves
+    e += [(leaf, 1) for leaf in range(200, 5200)]            # vertex 1: in-degree 5000
+    e += [(200 + 3 * k, k) for k in range(2, 64)]            # narrow vertices 2..63
+    e += [(k, 5300 + k) for k in range(1, 64)]               # private successors
+    e = np.array(e, np.int32).T
+    g = gi.Graph.from_edges(n, e[0], e[1], True).set_weights(11, 1, 50)
+    host = host_csr(g, True)
+    for delta in (8, 64):
+        dist, _ = gi.sssp(g, 100, delta, _fused_sssp(gi))
+        assert np.array_equal(dist, O.sssp(host, 100)), delta
+
+
 @pytest.mark.parametrize("lb", [6, 0, 2, 4])
 def test_cc_bit_exact(gi, lb):
     g = gi.Graph.uniform(20000, 15000, 4, True)
