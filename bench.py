@@ -120,6 +120,129 @@ def pmc_traffic(kernel: str, scale: int):
     return None, None
 
 
+def alg_frac(nbytes: float, seconds: float) -> float:
+    return round(nbytes / seconds / (HBM_PEAK_GBS * 1e9), 4)
+
+
+def bench_apps(gi, g, sources, edges_of, log) -> dict:
+    """The other apps on their configurations, each through its fused
+    schedule, with GTEPS and the fraction of the HBM roofline of their
+    algorithmic bytes (one definition per app, stated with each)."""
+    from easy_graphit_amd import autotune
+    out = {}
+    V, E = g.num_vertices, g.num_edges
+    # SSSP: RMAT-24 with int32 weights U[1,255], delta-stepping (delta 32)
+    g.set_weights(SEED, 1, 255)
+    fs = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWC, kernel_fusion=1)
+    srcs = [int(s) for s in sources[:4]]
+    t, edges, vr = 0.0, 0, 0
+    for s in srcs:
+        gi.sssp(g, s, 32, fs)  # warm-up
+        dist, st = gi.sssp(g, s, 32, fs)
+        t += st["ms"] * 1e-3
+        edges += edges_of[s]
+        vr += int((dist < np.iinfo(np.int32).max).sum())
+    out["sssp"] = {"gteps": round(edges / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3 / len(srcs), 3),
+                   "roofline_frac": alg_frac(16 * vr + 12 * edges, t),
+                   "alg_bytes": "16 V_reached + 12 E_reached", "config": "RMAT-24 sym, w U[1,255], "
+                   f"delta 32, {len(srcs)} sources"}
+    log(f"sssp: {out['sssp']}")
+    # CC: RMAT-24, hook + shortcutting rounds
+    fc = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_EDGE_ONLY, kernel_fusion=1)
+    gi.cc(g, fc)
+    ts, rounds = [], 0
+    for _ in range(3):
+        _, st = gi.cc(g, fc)
+        ts.append(st["ms"] * 1e-3)
+        rounds = st["iterations"]
+    t = float(np.median(ts))
+    out["cc"] = {"gteps": round(E / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3), "rounds": rounds,
+                 "roofline_frac": alg_frac(rounds * (8 * E + 12 * V) + rounds * 8 * V, t),
+                 "alg_bytes": "per round 8 E + 12 V (hook) + 8 V (shortcut)", "config": "RMAT-24 sym"}
+    log(f"cc: {out['cc']}")
+    # PageRank: RMAT-22 directed, 20 iterations, FP32 pull
+    gp = gi.Graph.rmat(22, 16, SEED, False)
+    fp = gi.Schedule(direction=gi.PULL, load_balancing=gi.LB_VERTEX_BASED, kernel_fusion=1)
+    gi.pagerank(gp, 20, schedule=fp)
+    ts = []
+    for _ in range(3):
+        _, st = gi.pagerank(gp, 20, schedule=fp)
+        ts.append(st["ms"] * 1e-3)
+    t = float(np.median(ts))
+    Vp, Ep = gp.num_vertices, gp.num_edges
+    out["pr"] = {"gteps": round(20 * Ep / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3),
+                 "iterations_per_s": round(20 / t, 1),
+                 "roofline_frac": alg_frac(20 * (8 * Ep + 20 * Vp), t),
+                 "alg_bytes": "20 (8 E + 20 V)", "config": "RMAT-22 directed, 20 iterations, d 0.85"}
+    log(f"pr: {out['pr']}")
+    gp.destroy()
+    # Batched Brandes BC: RMAT-20, 256 sources, batched vs one source at a time
+    gb = gi.Graph.rmat(20, 16, SEED, True)
+    bsrc = gb.pick_sources(SEED, 256)
+    gi.bc(gb, bsrc[:8], 256)
+    _, st_b = gi.bc(gb, bsrc, 256)
+    gi.bc(gb, bsrc[:2], 1)
+    _, st_s = gi.bc(gb, bsrc, 1)
+    dtmp = gi.DeviceArray(gb.num_vertices, np.int32)
+    be = 0
+    for s in bsrc:
+        gi.bfs_device(gb, int(s), None, dtmp)
+        be += gb.reached_edges(dtmp)
+    tb, tsq = st_b["ms"] * 1e-3, st_s["ms"] * 1e-3
+    out["bc"] = {"sources": len(bsrc), "batched_ms": round(tb * 1e3, 2), "sequential_ms": round(tsq * 1e3, 2),
+                 "speedup": round(tsq / tb, 2), "gteps": round(be / 2 / tb / 1e9, 3),
+                 "config": "RMAT-20 sym, 256 sources, batch 256 vs 1"}
+    log(f"bc: {out['bc']}")
+    gb.destroy()
+    # 1,024 uniform graphs (V log-uniform in [1k, 64k], average degree 8), one
+    # fused multi-source BFS over the packed CSR with autotuned bins
+    gm, offs = gi.Graph.uniform_batch(1024, 1024, 65536, 8, SEED)
+    moff, _, _ = gm.read_csr()
+    msrc = []
+    for i in range(1024):  # first vertex with edges in each graph
+        deg = np.diff(moff[offs[i]:offs[i + 1] + 1])
+        nz = np.flatnonzero(deg)
+        msrc.append(int(offs[i] + (nz[0] if len(nz) else 0)))
+    msrc = np.array(msrc, np.int32)
+    tuned = autotune.tune_bins(gm, msrc)
+    sched = autotune.schedule(tuned["push_lb"])
+    depth = gi.DeviceArray(gm.num_vertices, np.int32)
+    t = autotune.time_multi(gm, msrc, sched, depth, 5)
+    me = gm.reached_edges(depth)
+    # one graph at a time: 32 of the graphs built on their own, scaled to 1,024
+    rng = np.random.default_rng(SEED)
+    pick = rng.choice(1024, 32, replace=False)
+    mnb = None
+    tone = 0.0
+    for i in pick:
+        v0, v1 = int(offs[i]), int(offs[i + 1])
+        if mnb is None:
+            _, mnb, _ = gm.read_csr()
+        sub_off = moff[v0:v1 + 1] - moff[v0]
+        sub_nb = mnb[moff[v0]:moff[v1]] - v0
+        gs = gi.Graph.from_csr(sub_off, sub_nb)
+        ds = gi.DeviceArray(v1 - v0, np.int32)
+        sc1 = autotune.schedule(gi.LB_TWC)
+        gi.bfs_device(gs, int(msrc[i] - v0), sc1, ds)
+        gi.synchronize()
+        t0 = time.perf_counter()
+        gi.bfs_device(gs, int(msrc[i] - v0), sc1, ds)
+        gi.synchronize()
+        tone += time.perf_counter() - t0
+        gs.destroy()
+    tone *= 1024 / len(pick)
+    out["many_graphs"] = {"graphs": 1024, "ms": round(t * 1e3, 3), "graphs_per_s": round(1024 / t, 1),
+                          "gteps": round(me / 2 / t / 1e9, 3),
+                          "one_at_a_time_ms": round(tone * 1e3, 2), "speedup": round(tone / t, 2),
+                          "bins": {k: tuned[k] for k in ("lane_max", "hub_min", "blocks_per_cu", "push_lb")},
+                          "default_bins_ms": round(tuned["default_seconds"] * 1e3, 3)
+                          if tuned["default_seconds"] else None,
+                          "config": "1,024 uniform graphs, V log-uniform [1k,64k], avg degree 8, packed CSR"}
+    log(f"many graphs: {out['many_graphs']}")
+    gm.destroy()
+    return out
+
+
 def cpu_baseline(gi, g, sources, edges_of, budget_s: float):
     import oracle
     off, nb, _ = g.read_csr()
@@ -154,6 +277,7 @@ def main():
     ap.add_argument("--profile-sources", type=int, default=4)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-apps", action="store_true", help="BFS line only (no SSSP/CC/PR/BC/many-graph runs)")
     ap.add_argument("--sources", type=int, default=64, help="distinct BFS sources cycled over")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run three device-to-device copies of MIB MiB (HBM counter scale)")
@@ -240,6 +364,7 @@ def main():
     if roof is not None:
         roof["traffic"], roof["traffic_source"] = pmc_traffic(roof["kernel"], args.scale)
     log(f"roofline: {json.dumps(roof)}")
+    apps = None if args.no_apps else bench_apps(gi, g, sources, edges_of, log)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(gi, g, sources, edges_of, args.cpu_budget)
@@ -265,6 +390,7 @@ def main():
                        "selector": "learned" if use_learned else "heuristic",
                        "engine": args.engine}},
         "selector": selector_cmp,
+        "apps": apps,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

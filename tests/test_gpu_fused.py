@@ -180,3 +180,17 @@ def test_graph_replay_many_runs_stay_exact(gi, c1):
         got = depth.numpy()
         _ = gi.bfs(g, int(s), None)  # unfused run in between
         assert np.array_equal(got, O.bfs_depth(host, int(s)))
+
+
+def test_autotune_picks_bins_that_keep_depths(gi):
+    from easy_graphit_amd import autotune
+    g, offs = gi.Graph.uniform_batch(32, 256, 4096, 8, 7)
+    want_offs, s, d = O.uniform_batch_edges(32, 256, 4096, 8, 7)
+    host = O.build_csr(int(offs[-1]), s, d, True)
+    srcs = offs[:-1] + (np.arange(32) * 131) % np.diff(offs)
+    best = autotune.tune_bins(g, srcs, lane_max=(16, 64), hub_min=(1024,), blocks_per_cu=(4, 8),
+                              push_lb=(gi.LB_TWC, gi.LB_CM), reps=1)
+    assert best["seconds"] == min(r["seconds"] for r in best["table"])
+    assert len(best["table"]) == 2 * 1 * 2 + 2
+    depth, _ = gi.bfs_multi(g, srcs, autotune.schedule(best["push_lb"]))
+    assert np.array_equal(depth, O.bfs_multi_depth(host, srcs))
