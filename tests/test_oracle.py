@@ -393,3 +393,34 @@ def test_mlp_gradients_finite_differences():
             lm, _ = O.mlp_loss_backward(x, t, m, *p)
             flat[k] = old
             assert abs((lp - lm) / (2 * eps) - grad.reshape(-1)[k]) < 1e-6, (pi, k)
+
+
+# ---------------------------------------------------------------- batches
+def test_bfs_multi_depth_is_min_over_single_sources():
+    s, d = O.rmat_edges(10, 8, 77)
+    g = O.build_csr(1 << 10, s, d, True)
+    srcs = O.pick_sources(g, 5, 6)
+    srcs = np.concatenate([srcs, srcs[:2]])  # repeated sources count once
+    singles = np.stack([O.bfs_depth(g, int(x)) for x in srcs])
+    want = np.where((singles >= 0).any(0), np.where(singles >= 0, singles, 1 << 30).min(0), -1)
+    assert np.array_equal(O.bfs_multi_depth(g, srcs), want)
+
+
+def test_uniform_batch_sizes_ranges_and_degree():
+    G, vmin, vmax, deg = 400, 1024, 65536, 8
+    offs, s, d = O.uniform_batch_edges(G, vmin, vmax, deg, 5)
+    sizes = np.diff(offs)
+    assert offs[0] == 0 and (sizes >= vmin).all() and (sizes <= vmax).all()
+    # log-uniform sizes: log2(size) spreads evenly over the 6 octaves
+    oct_ = np.floor(np.log2(sizes / vmin)).astype(int)
+    hist = np.bincount(np.minimum(oct_, 5), minlength=6)
+    assert hist.min() > 0.5 * G / 6 and hist.max() < 1.5 * G / 6
+    # every edge stays inside its graph, pairs = size * avg_degree / 2
+    gid = np.searchsorted(offs, s, side="right") - 1
+    assert np.array_equal(gid, np.searchsorted(offs, d, side="right") - 1)
+    per_graph = np.bincount(gid, minlength=G)
+    assert np.array_equal(per_graph, sizes * deg // 2)
+    # same seed -> same batch; another seed -> another batch
+    o2, s2, _ = O.uniform_batch_edges(G, vmin, vmax, deg, 5)
+    assert np.array_equal(o2, offs) and np.array_equal(s2, s)
+    assert not np.array_equal(O.uniform_batch_edges(G, vmin, vmax, deg, 6)[0], offs)
