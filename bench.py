@@ -274,6 +274,8 @@ def main():
     ap.add_argument("--push-lb", type=int, default=2, help="push schedule (default TWCE)")
     ap.add_argument("--pull-lb", type=int, default=0, help="pull schedule (default VERTEX_BASED)")
     ap.add_argument("--selector", choices=["learned", "heuristic"], default="learned")
+    ap.add_argument("--layout", choices=["degree", "native"], default="degree",
+                    help="vertex layout the fused engines run on (results are in original ids)")
     ap.add_argument("--profile-sources", type=int, default=4)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -305,6 +307,12 @@ def main():
     g = gi.Graph.rmat(args.scale, args.edge_factor, SEED, True)
     V, E = g.num_vertices, g.num_edges
     log(f"graph RMAT-{args.scale} sym: V={V} E={E} built in {time.perf_counter() - t0:.2f}s")
+    layout = gi.LAYOUT_DEGREE if args.layout == "degree" else gi.LAYOUT_NATIVE
+    t0 = time.perf_counter()
+    g.set_layout(layout)
+    gi.synchronize()
+    layout_s = time.perf_counter() - t0
+    log(f"layout {args.layout}: {layout_s:.2f}s")
     sources = g.pick_sources(SEED, 64)[: max(1, min(64, args.sources))]
     ns = len(sources)
     depth = gi.DeviceArray(V, np.int32)
@@ -347,12 +355,24 @@ def main():
         f"({'learned selector' if use_learned else 'heuristic'})")
     selector_cmp = None
     if learned is not None:
+        # the other policy on this layout, and GraphIt's rule also on the other
+        # layout: its best of the two is what the selector is judged against
         other = heur if use_learned else learned
         for k in range(args.warmup):
             gi.bfs_device(g, int(sources[k % ns]), other, depth)
         dt2, gteps2 = time_bfs(gi, g, other, sources, depth, edges_of, args.steps)
         g_l, g_h = (gteps, gteps2) if use_learned else (gteps2, gteps)
+        h_by_layout = {args.layout: round(g_h, 3)}
+        alt = "native" if args.layout == "degree" else "degree"
+        g.set_layout(gi.LAYOUT_NATIVE if alt == "native" else gi.LAYOUT_DEGREE)
+        for k in range(args.warmup):
+            gi.bfs_device(g, int(sources[k % ns]), heur, depth)
+        _, g_h_alt = time_bfs(gi, g, heur, sources, depth, edges_of, args.steps)
+        h_by_layout[alt] = round(g_h_alt, 3)
+        g.set_layout(layout)
+        g_h = max(g_h, g_h_alt)
         selector_cmp = {"learned_gteps": round(g_l, 3), "heuristic_gteps": round(g_h, 3),
+                        "heuristic_gteps_by_layout": h_by_layout,
                         "speedup": round(g_l / g_h, 3),
                         "heuristic": f"GraphIt hybrid |F| <= t*V: PUSH {gi.LB_NAMES[args.push_lb]} "
                                      f"else PULL {gi.LB_NAMES[args.pull_lb]} bitmap, t={threshold:g}",
@@ -385,7 +405,8 @@ def main():
         "data": "synthetic (seeded RMAT generated on the device)",
         "config": {"workload": f"direction-optimizing BFS, symmetrized RMAT scale {args.scale} "
                                f"edge factor {args.edge_factor}, {ns} GAPBS-style sources",
-                   "vertices": V, "edge_entries": E, "schedule": {
+                   "vertices": V, "edge_entries": E, "layout": args.layout,
+                   "layout_build_s": round(layout_s, 3), "schedule": {
                        "s1": repr(sched.s1), "s2": repr(sched.s2), "threshold": threshold,
                        "selector": "learned" if use_learned else "heuristic",
                        "engine": args.engine}},

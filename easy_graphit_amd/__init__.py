@@ -35,6 +35,7 @@ LB_VERTEX_BASED, LB_TWC, LB_TWCE, LB_WM, LB_CM, LB_STRICT, LB_EDGE_ONLY = range(
 LB_NAMES = ["VERTEX_BASED", "TWC", "TWCE", "WM", "CM", "STRICT", "EDGE_ONLY"]
 BLOCKED, UNBLOCKED = 0, 1
 SELECT_HEURISTIC, SELECT_LEARNED, SELECT_S1, SELECT_S2 = 0, 1, 2, 3
+LAYOUT_NATIVE, LAYOUT_DEGREE = 0, 1
 (OP_BFS_PARENT, OP_BFS_DEPTH, OP_SSSP_RELAX, OP_CC_HOOK, OP_PR_PULL, OP_BC_FORWARD,
  OP_BC_BACKWARD) = range(7)
 (VOP_FILL_I32, VOP_IOTA_I32, VOP_CC_PJUMP, VOP_PR_CONTRIB, VOP_PR_UPDATE, VOP_PR_RESET,
@@ -158,6 +159,8 @@ def lib():
         "gi_graph_transpose": (C.c_int, [vp, pp]),
         "gi_graph_blocked": (C.c_int, [vp, i32, pp]),
         "gi_graph_destroy": (C.c_int, [vp]),
+        "gi_graph_set_layout": (C.c_int, [vp, i32]),
+        "gi_graph_layout": (i32, [vp]),
         "gi_frontier_create": (C.c_int, [i32, i32, pp]),
         "gi_frontier_destroy": (C.c_int, [vp]),
         "gi_frontier_add_vertex": (C.c_int, [vp, i32, vp]),
@@ -390,6 +393,16 @@ class Graph:
         h = C.c_void_p()
         check(lib().gi_graph_blocked(self.h, block_size, C.byref(h)))
         return Graph(h.value, owner=False)
+
+    def set_layout(self, layout: int):
+        """LAYOUT_DEGREE: fused BFS and CC run on a degree-ordered copy
+        (results mapped back to the original ids); LAYOUT_NATIVE drops it."""
+        check(lib().gi_graph_set_layout(self.h, layout))
+        return self
+
+    @property
+    def layout(self) -> int:
+        return int(lib().gi_graph_layout(self.h))
 
     def destroy(self):
         if self.owner and self.h:

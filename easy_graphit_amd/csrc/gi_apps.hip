@@ -150,9 +150,22 @@ static int bfs_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int
             GI_HIP(hipEventCreate(&t1));
             GI_HIP(hipEventRecord(t0, s));
         }
+        // a degree-ordered layout: run on the view, map the results back
+        gi_graph *rg = g;
+        int32_t rsrc = source, *rdepth = d_depth, *rparent = d_parent;
+        if (g->layout) {
+            rg = layout_view(g);
+            rsrc = layout_map_source(g, source);
+            rdepth = layout_tmp(g, 0);
+            rparent = d_parent ? layout_tmp(g, 1) : nullptr;
+        }
         int32_t levels = 0;
-        int rc = bfs_fused_run(g, source, h, sel, d_depth, d_parent, s, want_trace ? &ft : nullptr,
+        int rc = bfs_fused_run(rg, rsrc, h, sel, rdepth, rparent, s, want_trace ? &ft : nullptr,
                                trace != nullptr, &levels);
+        if (rc == GI_OK && g->layout) {
+            rc = layout_map_back(g, rdepth, d_depth, false, s);
+            if (rc == GI_OK && d_parent) rc = layout_map_back(g, rparent, d_parent, true, s);
+        }
         float ms = 0.f;
         if (stats && rc == GI_OK) {
             GI_HIP(hipEventRecord(t1, s));
@@ -552,7 +565,12 @@ int gi_cc(gi_graph *g, const gi_schedule *sched, int32_t *d_labels, gi_run_stats
         // fused engine: hook sweeps and shortcutting with device-side termination
         EventTimer timer(s);
         int32_t rounds = 0;
-        GI_TRY(cc_fused_run(g, d_labels, &rounds, s));
+        if (g->layout) {  // degree-ordered view: labels come back as original minimum ids
+            GI_TRY(cc_fused_run(layout_view(g), layout_tmp(g, 0), &rounds, s));
+            GI_TRY(layout_labels_back(g, layout_tmp(g, 0), d_labels, s));
+        } else {
+            GI_TRY(cc_fused_run(g, d_labels, &rounds, s));
+        }
         double ms = timer.stop();
         if (stats) {
             stats->iterations = rounds;

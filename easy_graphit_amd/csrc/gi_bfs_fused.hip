@@ -1266,8 +1266,19 @@ int gi_bfs_multi(gi_graph *g, const int32_t *sources, int32_t num_sources,
     SelectorView sel{};
     if (h->selector == GI_SELECT_LEARNED) sel = selector_view();
     if (!sel.w1) sel.mode = h->selector == GI_SELECT_LEARNED ? GI_SELECT_HEURISTIC : h->selector;
-    return bfs_fused_run_multi(g, sources, num_sources, 0, h, sel, d_depth, d_parent,
-                               (hipStream_t)stream, nullptr, false, levels);
+    if (!g->layout)
+        return bfs_fused_run_multi(g, sources, num_sources, 0, h, sel, d_depth, d_parent,
+                                   (hipStream_t)stream, nullptr, false, levels);
+    // degree-ordered view: map the sources in, the results out
+    std::vector<int32_t> mapped((size_t)num_sources);
+    for (int32_t i = 0; i < num_sources; ++i) mapped[i] = layout_map_source(g, sources[i]);
+    hipStream_t s = (hipStream_t)stream;
+    int32_t *rdepth = layout_tmp(g, 0), *rparent = d_parent ? layout_tmp(g, 1) : nullptr;
+    GI_TRY(bfs_fused_run_multi(layout_view(g), mapped.data(), num_sources, 0, h, sel, rdepth, rparent, s,
+                               nullptr, false, levels));
+    GI_TRY(layout_map_back(g, rdepth, d_depth, false, s));
+    if (d_parent) GI_TRY(layout_map_back(g, rparent, d_parent, true, s));
+    return GI_OK;
 }
 
 int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow, int32_t reps,
@@ -1280,6 +1291,10 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
     if (follow >= BFS_NUM_MODES) return fail(GI_ERR_RANGE, "bad follow mode");
     if (reps < 1) reps = 1;
     hipStream_t s = (hipStream_t)stream;
+    if (g->layout) {  // time the modes where the engine runs: on the degree-ordered view
+        source = layout_map_source(g, source);
+        g = layout_view(g);
+    }
     FusedWork *w = nullptr;
     GI_TRY(ensure_fused(g, &w));
     int64_t words = bitmap_words(g->V);

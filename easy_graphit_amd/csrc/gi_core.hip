@@ -745,6 +745,7 @@ int gi_graph_destroy(gi_graph *g) {
     dfree(g->d_scratch);
     dfree(g->d_visited);
     if (g->work) gi_frontier_destroy(g->work);
+    release_layout(g);
     release_fused(g);
     release_pr(g);
     release_cc(g);

@@ -190,6 +190,10 @@ __device__ __forceinline__ int32_t upper_bound_i32(const int32_t *a, int32_t lo,
 
 }  // namespace gi
 
+namespace gi {
+struct GraphLayout;
+}
+
 // ----------------------------------------------------------- host handles
 struct gi_graph {
     int32_t V = 0;
@@ -210,6 +214,7 @@ struct gi_graph {
     gi_graph *transposed = nullptr;  // owned unless is_transpose_of is set
     gi_graph *transpose_owner = nullptr;
     std::map<int32_t, gi_graph *> blocked;  // owned
+    gi::GraphLayout *layout = nullptr;     // degree-ordered view (gi_graph_set_layout), owned
     bool borrowed = false;  // owned by another graph
 
     gi::DGraph view() const {
@@ -276,4 +281,14 @@ void release_cc(const gi_graph *g);
 int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *steps,
                    hipStream_t s);
 void release_sssp(const gi_graph *g);
+
+// Degree-ordered layout (gi_layout.hip): the view the fused engines run on,
+// source mapping, two V-sized scratch arrays in view order, and the
+// mappings of results back to the original ids.
+void release_layout(gi_graph *g);
+gi_graph *layout_view(const gi_graph *g);
+int32_t layout_map_source(const gi_graph *g, int32_t v);
+int32_t *layout_tmp(const gi_graph *g, int i);
+int layout_map_back(const gi_graph *g, const int32_t *in_new, int32_t *out_old, bool ids, hipStream_t s);
+int layout_labels_back(const gi_graph *g, const int32_t *labels_new, int32_t *out_old, hipStream_t s);
 }  // namespace gi

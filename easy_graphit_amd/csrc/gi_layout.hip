@@ -1,0 +1,247 @@
+// Degree-ordered layout: an internal copy of a graph whose vertices are
+// renumbered by decreasing degree (ties keep their original order). The
+// vertices that most edges point at then occupy the first words of every
+// per-vertex array and bitmap, so the random probes of frontier bits, labels
+// and depths mostly hit a few cache lines. Results are mapped back to the
+// original ids, so callers see no difference.
+//
+// Neighbour lists of the view keep the original order of each list (mapped
+// to new ids); no kernel here needs them sorted.
+#include <climits>
+#include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "gi_internal.h"
+
+namespace gi {
+
+__global__ void layout_keys_kernel(const int32_t *off, int32_t V, uint32_t *keys, int32_t *vals) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        keys[v] = (uint32_t)INT_MAX - (uint32_t)(off[v + 1] - off[v]);  // ascending key = descending degree
+        vals[v] = v;
+    }
+}
+
+__global__ void layout_inverse_kernel(const int32_t *old_of_new, const int32_t *off, int32_t V,
+                                      int32_t *new_of_old, int32_t *deg_new) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        const int32_t o = old_of_new[i];
+        new_of_old[o] = i;
+        deg_new[i] = off[o + 1] - off[o];
+    }
+}
+
+// Edge-parallel over the original edges: edge e of old row o lands at the
+// same position k of new row new_of_old[o].
+__global__ void layout_edges_kernel(const int32_t *esrc, const int32_t *off, const int32_t *nbr,
+                                    const int32_t *w, const int32_t *new_of_old, const int32_t *off_new,
+                                    int64_t E, int32_t *nbr_new, int32_t *w_new) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t o = esrc[e];
+        const int32_t k = (int32_t)(e - off[o]);
+        const int32_t pos = off_new[new_of_old[o]] + k;
+        nbr_new[pos] = new_of_old[nbr[e]];
+        if (w) w_new[pos] = w[e];
+    }
+}
+
+__global__ void layout_gather_kernel(const int32_t *in_new, const int32_t *new_of_old, int32_t V,
+                                     int32_t *out_old) {
+    for (int32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < V; o += gridDim.x * blockDim.x)
+        out_old[o] = in_new[new_of_old[o]];
+}
+
+// Parents are vertex ids too: map the value back as well.
+__global__ void layout_gather_ids_kernel(const int32_t *in_new, const int32_t *new_of_old,
+                                         const int32_t *old_of_new, int32_t V, int32_t *out_old) {
+    for (int32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < V; o += gridDim.x * blockDim.x) {
+        const int32_t p = in_new[new_of_old[o]];
+        out_old[o] = p < 0 ? p : old_of_new[p];
+    }
+}
+
+__global__ void layout_fill_kernel(int32_t *a, int32_t V, int32_t x) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) a[i] = x;
+}
+
+// Component minimum of the original ids: labels_new are vertex ids of the
+// view; lanes of a wave that share a label (the common case: one giant
+// component) combine before their atomic.
+__global__ void layout_comp_min_kernel(const int32_t *labels_new, const int32_t *old_of_new, int32_t V,
+                                       int32_t *comp_min) {
+    const int32_t lane = lane_id();
+    for (int32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63; base < V;
+         base += gridDim.x * blockDim.x) {
+        const int32_t i = base + lane;
+        const bool ok = i < V;
+        const int32_t r = ok ? labels_new[i] : -1;
+        const int32_t o = ok ? old_of_new[i] : INT_MAX;
+        const int32_t r0 = __shfl(r, 0);
+        const bool same = ok && r == r0;
+        int32_t m = same ? o : INT_MAX;
+        for (int k = 32; k > 0; k >>= 1) {
+            const int32_t y = __shfl_xor(m, k);
+            m = y < m ? y : m;
+        }
+        if (lane == 0 && m < INT_MAX && m < comp_min[r0]) atomicMin(&comp_min[r0], m);
+        if (ok && !same && o < comp_min[r]) atomicMin(&comp_min[r], o);
+    }
+}
+
+__global__ void layout_labels_back_kernel(const int32_t *labels_new, const int32_t *comp_min,
+                                          const int32_t *new_of_old, int32_t V, int32_t *out_old) {
+    for (int32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < V; o += gridDim.x * blockDim.x)
+        out_old[o] = comp_min[labels_new[new_of_old[o]]];
+}
+
+static int grid_of(int64_t n) {
+    int64_t g = ceil_div(n, 256);
+    return (int)(g < 1 ? 1 : (g < 8192 ? g : 8192));
+}
+
+// The per-graph layout state lives in the graph handle (gi_graph::layout).
+struct GraphLayout {
+    gi_graph *view = nullptr;
+    int32_t *new_of_old = nullptr;
+    int32_t *old_of_new = nullptr;
+    std::vector<int32_t> h_new_of_old;  // for mapping host-side sources
+    int32_t *tmp[2] = {nullptr, nullptr};  // results in view order
+};
+
+void release_layout(gi_graph *g) {
+    GraphLayout *L = g->layout;
+    if (!L) return;
+    if (L->view) {
+        L->view->borrowed = false;
+        gi_graph_destroy(L->view);
+    }
+    dfree(L->new_of_old);
+    dfree(L->old_of_new);
+    dfree(L->tmp[0]);
+    dfree(L->tmp[1]);
+    delete L;
+    g->layout = nullptr;
+}
+
+static int build_layout(gi_graph *g, GraphLayout **out) {
+    const int32_t V = g->V;
+    const int64_t E = g->E;
+    hipStream_t s = 0;
+    GI_TRY(ensure_coo(g, s));
+    GraphLayout *L = new GraphLayout();
+    gi_graph *t = new gi_graph();
+    t->V = V;
+    t->E = E;
+    t->directed = g->directed;
+    L->view = t;
+    uint32_t *k0 = nullptr, *k1 = nullptr;
+    int32_t *v0 = nullptr, *v1 = nullptr, *deg = nullptr;
+    int rc = dalloc(&k0, (size_t)V + 1);
+    if (rc == GI_OK) rc = dalloc(&k1, (size_t)V + 1);
+    if (rc == GI_OK) rc = dalloc(&v0, (size_t)V + 1);
+    if (rc == GI_OK) rc = dalloc(&v1, (size_t)V + 1);
+    if (rc == GI_OK) rc = dalloc(&deg, (size_t)V + 1);
+    if (rc == GI_OK) rc = dalloc(&L->new_of_old, (size_t)V + 1);
+    if (rc == GI_OK) rc = dalloc(&t->d_off, (size_t)V + 2);
+    if (rc == GI_OK) rc = dalloc(&t->d_nbr, (size_t)E + 1);
+    if (rc == GI_OK && g->d_w) rc = dalloc(&t->d_w, (size_t)E + 1);
+    if (rc == GI_OK) rc = dalloc(&L->tmp[0], (size_t)V + 1);
+    if (rc == GI_OK) rc = dalloc(&L->tmp[1], (size_t)V + 1);
+    if (rc == GI_OK) {
+        layout_keys_kernel<<<grid_of(V), 256, 0, s>>>(g->d_off, V, k0, v0);
+        rocprim::double_buffer<uint32_t> kb(k0, k1);
+        rocprim::double_buffer<int32_t> vb(v0, v1);
+        size_t bytes = 0;
+        (void)rocprim::radix_sort_pairs(nullptr, bytes, kb, vb, (size_t)V, 0, 32, s);
+        void *tmp = nullptr;
+        rc = dmalloc(&tmp, bytes);
+        if (rc == GI_OK && rocprim::radix_sort_pairs(tmp, bytes, kb, vb, (size_t)V, 0, 32, s) != hipSuccess)
+            rc = fail(GI_ERR_HIP, "layout sort");
+        dfree(tmp);
+        if (rc == GI_OK) {
+            // the sorted values are old_of_new; the other buffer is freed
+            int32_t *res = vb.current();
+            dfree(res == v0 ? v1 : v0);
+            v0 = v1 = nullptr;
+            L->old_of_new = res;
+            layout_inverse_kernel<<<grid_of(V), 256, 0, s>>>(L->old_of_new, g->d_off, V, L->new_of_old, deg);
+            rc = fill_i32(deg + V, 1, 0, s);  // the scan's last term: offsets[V] = E
+            if (rc == GI_OK) rc = exclusive_scan_i32(deg, t->d_off, (int64_t)V + 1, s);
+        }
+        if (rc == GI_OK && E > 0)
+            layout_edges_kernel<<<grid_of(E), 256, 0, s>>>(g->d_esrc, g->d_off, g->d_nbr, g->d_w, L->new_of_old,
+                                                           t->d_off, E, t->d_nbr, t->d_w);
+        if (rc == GI_OK && hipGetLastError() != hipSuccess) rc = fail(GI_ERR_HIP, "layout kernels");
+        if (rc == GI_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(GI_ERR_HIP, "layout sync");
+    }
+    if (rc == GI_OK) {
+        L->h_new_of_old.resize((size_t)V);
+        if (hipMemcpy(L->h_new_of_old.data(), L->new_of_old, sizeof(int32_t) * V, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            rc = fail(GI_ERR_HIP, "layout copy");
+    }
+    dfree(k0);
+    dfree(k1);
+    dfree(v0);
+    dfree(v1);
+    dfree(deg);
+    t->borrowed = true;
+    g->layout = L;
+    if (rc != GI_OK) {
+        release_layout(g);
+        return rc;
+    }
+    *out = L;
+    return GI_OK;
+}
+
+gi_graph *layout_view(const gi_graph *g) { return g->layout ? g->layout->view : nullptr; }
+
+int32_t layout_map_source(const gi_graph *g, int32_t v) { return g->layout->h_new_of_old[(size_t)v]; }
+
+int32_t *layout_tmp(const gi_graph *g, int i) { return g->layout->tmp[i]; }
+
+int layout_map_back(const gi_graph *g, const int32_t *in_new, int32_t *out_old, bool ids, hipStream_t s) {
+    const GraphLayout *L = g->layout;
+    if (ids)
+        layout_gather_ids_kernel<<<grid_of(g->V), 256, 0, s>>>(in_new, L->new_of_old, L->old_of_new, g->V, out_old);
+    else
+        layout_gather_kernel<<<grid_of(g->V), 256, 0, s>>>(in_new, L->new_of_old, g->V, out_old);
+    GI_LAUNCH_CHECK();
+    return GI_OK;
+}
+
+int layout_labels_back(const gi_graph *g, const int32_t *labels_new, int32_t *out_old, hipStream_t s) {
+    const GraphLayout *L = g->layout;
+    int32_t *comp_min = L->tmp[1];
+    layout_fill_kernel<<<grid_of(g->V), 256, 0, s>>>(comp_min, g->V, INT_MAX);
+    layout_comp_min_kernel<<<grid_of(g->V), 256, 0, s>>>(labels_new, L->old_of_new, g->V, comp_min);
+    layout_labels_back_kernel<<<grid_of(g->V), 256, 0, s>>>(labels_new, comp_min, L->new_of_old, g->V, out_old);
+    GI_LAUNCH_CHECK();
+    return GI_OK;
+}
+
+}  // namespace gi
+
+using namespace gi;
+
+extern "C" {
+
+int gi_graph_set_layout(gi_graph *g, int32_t layout) {
+    if (!g) return fail(GI_ERR_INVALID, "null graph");
+    if (g->borrowed || !g->d_off) return fail(GI_ERR_UNSUPPORTED, "set the layout of a base graph");
+    if (layout == GI_LAYOUT_NATIVE) {
+        release_layout(g);
+        return GI_OK;
+    }
+    if (layout != GI_LAYOUT_DEGREE) return fail(GI_ERR_RANGE, "unknown layout %d", layout);
+    if (g->layout) return GI_OK;
+    GraphLayout *L = nullptr;
+    return build_layout(g, &L);
+}
+
+int32_t gi_graph_layout(const gi_graph *g) { return g && g->layout ? GI_LAYOUT_DEGREE : GI_LAYOUT_NATIVE; }
+
+}  // extern "C"

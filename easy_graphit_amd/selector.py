@@ -46,14 +46,20 @@ CALIBRATION = [
 ]
 
 
-def build_graph(kind: str, args: dict) -> gi.Graph:
+# Vertex layout the engine runs on; the selector is trained and judged on it.
+LAYOUT = gi.LAYOUT_DEGREE
+
+
+def build_graph(kind: str, args: dict, layout: int = LAYOUT) -> gi.Graph:
     if kind == "rmat":
-        return gi.Graph.rmat(args["scale"], args["edge_factor"], args["seed"], True)
-    if kind == "uniform":
-        return gi.Graph.uniform(args["num_vertices"], args["num_pairs"], args["seed"], True)
-    if kind == "grid":
-        return gi.Graph.grid(args["width"], args["num_random"], args["seed"])
-    raise ValueError(kind)
+        g = gi.Graph.rmat(args["scale"], args["edge_factor"], args["seed"], True)
+    elif kind == "uniform":
+        g = gi.Graph.uniform(args["num_vertices"], args["num_pairs"], args["seed"], True)
+    elif kind == "grid":
+        g = gi.Graph.grid(args["width"], args["num_random"], args["seed"])
+    else:
+        raise ValueError(kind)
+    return g.set_layout(layout)
 
 
 def record(specs=CALIBRATION, sources_per_graph: int = 4, reps: int = 2, log=print):
@@ -202,7 +208,7 @@ def evaluate(model, threshold: float, scale: int = 24, edge_factor: int = 16,
     same rule with this runtime's grid-binned TWC push is timed as well, as
     the strongest fixed schedule. Depths of every policy are checked equal to
     the learned run's on every source before timing."""
-    g = gi.Graph.rmat(scale, edge_factor, seed, True)
+    g = gi.Graph.rmat(scale, edge_factor, seed, True).set_layout(LAYOUT)
     srcs = g.pick_sources(seed, num_sources)
     depth = gi.DeviceArray(g.num_vertices, np.int32)
     learned = fused_schedule(threshold=threshold, selector=gi.SELECT_LEARNED)

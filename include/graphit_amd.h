@@ -159,6 +159,16 @@ int gi_graph_transpose(gi_graph *g, gi_graph **out);
 int gi_graph_blocked(gi_graph *g, int32_t block_size, gi_graph **out);
 int gi_graph_destroy(gi_graph *g);
 
+/* Vertex layout used by the fused engines (no reference counterpart: the
+ * reference keeps the file's vertex order, graph.h:236-338). DEGREE builds an
+ * internal copy renumbered by decreasing degree; fused BFS (single- and
+ * multi-source) and fused CC then run on it and map their results back to
+ * the original ids, so outputs are unchanged. NATIVE drops the copy. Costs
+ * one more CSR in device memory. */
+enum gi_layout { GI_LAYOUT_NATIVE = 0, GI_LAYOUT_DEGREE = 1 };
+int gi_graph_set_layout(gi_graph *g, int32_t layout);
+int32_t gi_graph_layout(const gi_graph *g);
+
 /* ------------------------------------------------------------------ frontier
  * Replaces VertexFrontier (runtime/infra_gpu/vertex_frontier.h:26-166) and
  * the conversions of vertex_representation.h:9-158. One frontier holds a

@@ -1,0 +1,92 @@
+"""Degree-ordered layout: fused BFS (single- and multi-source, parents, the
+learned selector) and fused CC run on the renumbered copy and must give the
+oracle's results in the original vertex ids."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+SEED = 27491095
+
+
+def fused(gi, push_lb=1, t=0.005, selector=0):
+    s1 = gi.Schedule(direction=0, load_balancing=push_lb, kernel_fusion=1)
+    s2 = gi.Schedule(direction=1, load_balancing=0, kernel_fusion=1,
+                     pull_frontier_rep=gi.PULL_BITMAP, frontier_creation=gi.FC_BITMAP)
+    return gi.HybridSchedule(s1, s2, t, selector)
+
+
+@pytest.fixture(scope="module")
+def rmat(gi):
+    g = gi.Graph.rmat(15, 16, SEED, True)
+    off, nb, _ = g.read_csr()
+    host = O.CSR(off, nb)
+    g.set_layout(gi.LAYOUT_DEGREE)
+    return g, host
+
+
+def test_layout_flag(gi, rmat):
+    g, _ = rmat
+    assert g.layout == gi.LAYOUT_DEGREE
+
+
+def test_layout_bfs_depths_and_parents(gi, rmat):
+    g, host = rmat
+    for src in O.pick_sources(host, 3, 5):
+        for push_lb in (1, 2):
+            depth, parent, st = gi.bfs(g, int(src), fused(gi, push_lb), want_parent=True)
+            assert np.array_equal(depth, O.bfs_depth(host, int(src)))
+            assert O.bfs_parent_violations(host, int(src), depth, parent) == 0
+
+
+def test_layout_bfs_multi(gi, rmat):
+    g, host = rmat
+    srcs = O.pick_sources(host, 8, 11)
+    depth, levels = gi.bfs_multi(g, srcs, fused(gi))
+    want = O.bfs_multi_depth(host, srcs)
+    assert np.array_equal(depth, want) and levels == want.max() + 1
+
+
+def test_layout_cc(gi):
+    n = 40000
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, n, 25000)
+    b = rng.integers(0, n, 25000)
+    e = np.stack([a, b]).astype(np.int32)
+    g = gi.Graph.from_edges(n, e[0], e[1], True)
+    host = O.build_csr(n, e[0], e[1], True)
+    g.set_layout(gi.LAYOUT_DEGREE)
+    sched = gi.Schedule(direction=0, load_balancing=6, kernel_fusion=1)
+    for _ in range(2):
+        lab, _ = gi.cc(g, schedule=sched)
+        assert np.array_equal(lab, O.cc(host))
+
+
+def test_layout_directed_bfs(gi):
+    g = gi.Graph.rmat(13, 8, 4, False)
+    off, nb, _ = g.read_csr()
+    host = O.CSR(off, nb)
+    g.set_layout(gi.LAYOUT_DEGREE)
+    for src in O.pick_sources(host, 2, 3):
+        depth, _, _ = gi.bfs(g, int(src), fused(gi, t=0.02))
+        assert np.array_equal(depth, O.bfs_depth(host, int(src)))
+
+
+def test_layout_back_to_native(gi):
+    g = gi.Graph.rmat(12, 8, 9, True)
+    off, nb, _ = g.read_csr()
+    host = O.CSR(off, nb)
+    g.set_layout(gi.LAYOUT_DEGREE)
+    g.set_layout(gi.LAYOUT_NATIVE)
+    assert g.layout == gi.LAYOUT_NATIVE
+    depth, _, _ = gi.bfs(g, 0, fused(gi))
+    assert np.array_equal(depth, O.bfs_depth(host, 0))
+
+
+def test_layout_sweep_records(gi, rmat):
+    g, host = rmat
+    src = int(O.pick_sources(host, 4, 1)[0])
+    feats, times = gi.bfs_sweep(g, src, reps=1)
+    assert feats.shape[0] == O.bfs_depth(host, src).max() + 1
+    assert (times > 0).all()
