@@ -682,7 +682,7 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
     GI_LAUNCH_CHECK();
     int32_t block = w->last_slots > 0 ? w->last_slots : 16;
     for (int64_t used = 0; used < (1ll << 24);) {
-        if (block > 256) block = 256;
+        if (block > 64) block = 64;
         auto key = std::make_pair(block, d_dist);
         auto it = w->execs.find(key);
         if (it == w->execs.end()) {
