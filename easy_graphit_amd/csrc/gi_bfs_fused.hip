@@ -28,6 +28,7 @@
 //   * |F'| and the sum of its degrees are reduced per workgroup into a slot
 //     of a partial-sum array (plain stores) that the decide kernel adds up,
 //     so no counter is shared by the whole grid.
+#include <cstring>
 #include <vector>
 
 #include "gi_bfs_fused.h"
@@ -116,6 +117,53 @@ __device__ __forceinline__ void push_edge(const StepArgs &a, StepCtx &c, int32_t
     if (c.want_fdeg) c.fdeg += (unsigned long long)deg_of(a.off, d);
 }
 
+// Push discovery by a whole wavefront at once (every lane calls it; `active`
+// lanes carry an edge). Lanes whose targets share a bitmap word — runs of
+// neighbours of a sorted list, common among hubs in the degree-ordered
+// layout — combine their bits with a segmented OR, and the last lane of each
+// run claims them with one atomicOr, then tells the run which bits it won.
+__device__ __forceinline__ void push_edge_wave(const StepArgs &a, StepCtx &c, int32_t s, int32_t e,
+                                               bool active) {
+    const int32_t lane = lane_id();
+    const int32_t d = active ? a.nbr[e] : -1;
+    const int32_t key = d >= 0 ? (d >> 6) : -1;
+    const uint64_t bit = d >= 0 ? 1ull << (d & 63) : 0ull;
+    const bool want = d >= 0 && !(a.visited[key] & bit);  // a stale word only costs the atomic
+    // runs: maximal groups of adjacent lanes with the same word (the same
+    // word may come back in a later run: that run claims on its own)
+    const int32_t kp = __shfl_up(key, 1), kn = __shfl_down(key, 1);
+    const bool head = lane == 0 || kp != key;
+    const bool end = lane == 63 || kn != key;
+    const uint64_t heads = __ballot(head);
+    const int32_t my_head = 63 - __clzll((unsigned long long)(heads & ((2ull << lane) - 1)));
+    uint64_t x = want ? bit : 0ull;
+    for (int o = 1; o < 64; o <<= 1) {  // segmented OR inside the lane's own run
+        const uint64_t y = __shfl_up(x, o);
+        if (lane - o >= my_head) x |= y;
+    }
+    uint64_t won = 0;
+    if (end && key >= 0 && x) {
+        const unsigned long long old = atomicOr((unsigned long long *)&a.visited[key], x);
+        won = x & ~old;
+        if (won) {
+            const unsigned long long o2 = atomicOr((unsigned long long *)&c.bm_out[key], won);
+            if (o2 == 0) mark_chunk(c.sum_out, key);
+        }
+    }
+    const uint64_t ends = __ballot(end);
+    const int32_t leader = __ffsll((unsigned long long)(ends & (~0ull << lane))) - 1;
+    const uint64_t run_won = __shfl(won, leader);
+    // bits of the lanes before this one in its run: a target repeated in a
+    // run is recorded once, by its first lane
+    const uint64_t xp = __shfl_up(x, 1);
+    const uint64_t before = lane - 1 >= my_head ? xp : 0ull;
+    if (want && (run_won & bit) && !(before & bit)) {
+        if (a.parent) a.parent[d] = s;  // the depth is written in word order by the next step
+        c.cnt += 1;
+        if (c.want_fdeg) c.fdeg += (unsigned long long)deg_of(a.off, d);
+    }
+}
+
 __device__ __forceinline__ void lane_walk(const StepArgs &a, StepCtx &c, int32_t s, int32_t e,
                                           int32_t e1) {
     for (; e < e1; ++e) push_edge(a, c, s, e, true);
@@ -124,7 +172,7 @@ __device__ __forceinline__ void lane_walk(const StepArgs &a, StepCtx &c, int32_t
 __device__ __forceinline__ void wave_walk(const StepArgs &a, StepCtx &c, int32_t s, int32_t e0,
                                           int32_t e1) {
     int32_t lane = lane_id();
-    for (int32_t b = e0; b < e1; b += 64) push_edge(a, c, s, b + lane, b + lane < e1);
+    for (int32_t b = e0; b < e1; b += 64) push_edge_wave(a, c, s, b + lane, b + lane < e1);
 }
 
 // Workgroup reduction of the lane counters into this block's slot.
@@ -235,7 +283,7 @@ __device__ void push_batch(const StepArgs &a, StepCtx &c, int32_t mode, const in
                     v = list[j];
                     e = soff[j] + (k - sscan[j]);
                 }
-                push_edge(a, c, v, e, act);
+                push_edge_wave(a, c, v, e, act);
             }
         } else {
             // TWCE: workgroup granule (degree >= block), wave granule (>= 64), lanes
@@ -243,7 +291,7 @@ __device__ void push_batch(const StepArgs &a, StepCtx &c, int32_t mode, const in
                 int32_t dj = (j + 1 < nb ? sscan[j + 1] : total) - sscan[j];
                 if (dj < kStepBlock) continue;  // uniform
                 int32_t v = list[j], o = soff[j];
-                for (int32_t b = 0; b < dj; b += kStepBlock) push_edge(a, c, v, o + b + tid, b + tid < dj);
+                for (int32_t b = 0; b < dj; b += kStepBlock) push_edge_wave(a, c, v, o + b + tid, b + tid < dj);
             }
             for (int32_t j = wid; j < nb; j += kStepBlock / 64) {
                 int32_t dj = (j + 1 < nb ? sscan[j + 1] : total) - sscan[j];
@@ -383,6 +431,63 @@ __device__ __forceinline__ bool pull_words(const StepArgs &a, StepCtx &c, int32_
     return any;
 }
 
+// Grid bin of PUSH_TWC: the edges of the deferred hubs, spread evenly over
+// the grid. Every workgroup scans the hub degrees in LDS and walks its share
+// of the concatenated edge range.
+__device__ __forceinline__ void hub_body(const StepArgs &a0) {
+    BfsState *st = a0.st;
+    if (st->done || st->mode >= BFS_PULL_THREAD) return;
+    int32_t nh = st->hub_count;
+    if (nh <= 0) return;  // slots stay zero (the decide kernel clears them)
+    if (nh > kMaxHubs) nh = kMaxHubs;
+    StepArgs a = a0;
+    a.depth = st->depth;
+    a.parent = st->parent;
+    StepCtx c;
+    c.level = st->level;
+    c.lane_max = st->lane_max;
+    c.hub_min = st->hub_min;
+    c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], c.level % 3);
+    c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], (c.level + 1) % 3);
+    c.sum_out = pick3(a0.sum[0], a0.sum[1], a0.sum[2], (c.level + 1) % 3);
+    c.want_fdeg = st->want_fdeg;
+    c.cnt = 0;
+    c.fdeg = 0;
+    __shared__ int32_t hscan[kMaxHubs + 1];
+    __shared__ int32_t hv[kMaxHubs];
+    int32_t per = (nh + kStepBlock - 1) / kStepBlock;
+    int32_t lo = threadIdx.x * per, hi = lo + per < nh ? lo + per : nh;
+    int32_t part = 0;
+    for (int32_t i = lo; i < hi; ++i) {
+        int32_t s = st->hubs[i];
+        hv[i] = s;
+        part += deg_of(a.off, s);
+    }
+    int32_t total = 0;
+    int32_t ex = block_exclusive_scan(part, &total);
+    for (int32_t i = lo; i < hi; ++i) {
+        hscan[i] = ex;
+        ex += deg_of(a.off, hv[i]);
+    }
+    __syncthreads();
+    int32_t chunk = (total + gridDim.x - 1) / gridDim.x;
+    int32_t k0 = blockIdx.x * chunk, k1 = k0 + chunk < total ? k0 + chunk : total;
+    for (int32_t kb = k0; kb < k1; kb += kStepBlock) {  // block-uniform trip count
+        const int32_t k = kb + (int32_t)threadIdx.x;
+        const bool act = k < k1;
+        int32_t s = 0, e = 0;
+        if (act) {
+            const int32_t j = upper_bound_i32(hscan, 0, nh, k) - 1;
+            s = hv[j];
+            e = a.off[s] + (k - hscan[j]);
+        }
+        push_edge_wave(a, c, s, e, act);
+    }
+    flush_counts(a.pcount + kMaxGrid, a.pfdeg + kMaxGrid, c);
+}
+
+__global__ void __launch_bounds__(kStepBlock) bfs_hub_kernel(StepArgs a0) { hub_body(a0); }
+
 // ---------------------------------------------------------------- step kernels
 // One level's step. The push and pull paths are separate kernels (each with
 // its own register and LDS budget, hence occupancy); a captured level
@@ -491,62 +596,13 @@ __device__ __forceinline__ void step_body(const StepArgs &a0) {
     flush_counts(a.pcount, a.pfdeg, c);
 }
 
-// (the pull kernels fit 64 VGPRs without spilling: 8 waves per SIMD)
-#define GI_PULL_ATTR __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(8)))
-__global__ void __launch_bounds__(kStepBlock) bfs_push_step(StepArgs a) { step_body<kPushStep>(a); }
-__global__ void GI_PULL_ATTR bfs_pull_step(StepArgs a) { step_body<kPullStep>(a); }
-__global__ void __launch_bounds__(kStepBlock) bfs_push_step_traced(StepArgs a) { step_body<kPushStep>(a); }
-__global__ void GI_PULL_ATTR bfs_pull_step_traced(StepArgs a) { step_body<kPullStep>(a); }
-#undef GI_PULL_ATTR
-
-// Grid bin of PUSH_TWC: the edges of the deferred hubs, spread evenly over
-// the grid. Every workgroup scans the hub degrees in LDS and walks its share
-// of the concatenated edge range.
-__global__ void __launch_bounds__(kStepBlock) bfs_hub_kernel(StepArgs a0) {
-    BfsState *st = a0.st;
-    if (st->done || st->mode >= BFS_PULL_THREAD) return;
-    int32_t nh = st->hub_count;
-    if (nh <= 0) return;  // slots stay zero (the decide kernel clears them)
-    if (nh > kMaxHubs) nh = kMaxHubs;
-    StepArgs a = a0;
-    a.depth = st->depth;
-    a.parent = st->parent;
-    StepCtx c;
-    c.level = st->level;
-    c.lane_max = st->lane_max;
-    c.hub_min = st->hub_min;
-    c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], c.level % 3);
-    c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], (c.level + 1) % 3);
-    c.sum_out = pick3(a0.sum[0], a0.sum[1], a0.sum[2], (c.level + 1) % 3);
-    c.want_fdeg = st->want_fdeg;
-    c.cnt = 0;
-    c.fdeg = 0;
-    __shared__ int32_t hscan[kMaxHubs + 1];
-    __shared__ int32_t hv[kMaxHubs];
-    int32_t per = (nh + kStepBlock - 1) / kStepBlock;
-    int32_t lo = threadIdx.x * per, hi = lo + per < nh ? lo + per : nh;
-    int32_t part = 0;
-    for (int32_t i = lo; i < hi; ++i) {
-        int32_t s = st->hubs[i];
-        hv[i] = s;
-        part += deg_of(a.off, s);
-    }
-    int32_t total = 0;
-    int32_t ex = block_exclusive_scan(part, &total);
-    for (int32_t i = lo; i < hi; ++i) {
-        hscan[i] = ex;
-        ex += deg_of(a.off, hv[i]);
-    }
-    __syncthreads();
-    int32_t chunk = (total + gridDim.x - 1) / gridDim.x;
-    int32_t k0 = blockIdx.x * chunk, k1 = k0 + chunk < total ? k0 + chunk : total;
-    for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += kStepBlock) {
-        int32_t j = upper_bound_i32(hscan, 0, nh, k) - 1;
-        int32_t s = hv[j];
-        push_edge(a, c, s, a.off[s] + (k - hscan[j]), true);
-    }
-    flush_counts(a.pcount + kMaxGrid, a.pfdeg + kMaxGrid, c);
-}
+// (both fit the register budget of 8 waves per SIMD without spilling)
+#define GI_STEP_ATTR __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(8)))
+__global__ void GI_STEP_ATTR bfs_push_step(StepArgs a) { step_body<kPushStep>(a); }
+__global__ void GI_STEP_ATTR bfs_pull_step(StepArgs a) { step_body<kPullStep>(a); }
+__global__ void GI_STEP_ATTR bfs_push_step_traced(StepArgs a) { step_body<kPushStep>(a); }
+__global__ void GI_STEP_ATTR bfs_pull_step_traced(StepArgs a) { step_body<kPullStep>(a); }
+#undef GI_STEP_ATTR
 
 // ---------------------------------------------------------------- decide kernel
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
@@ -860,6 +916,8 @@ struct FusedWork {
     const int32_t *poff = nullptr;   // in-edge CSR (the graph itself when symmetric)
     const int32_t *pnbr = nullptr;
     SelectorView *d_sel = nullptr;   // selector of the current run (device copy)
+    SelectorView last_sel{};         // host copy of what d_sel holds
+    bool sel_valid = false;
     int32_t *d_src = nullptr;        // sources of a multi-source run
     int32_t src_cap = 0;
     int32_t *host_flag = nullptr;    // pinned copy of `done`
@@ -1058,7 +1116,11 @@ static void launch_level_traced(gi_graph *g, FusedWork *w, const StepArgs &a, in
 static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t n_sources,
                     int32_t single, int32_t *depth, int32_t *parent, const SelectorView &sel,
                     bool want_fdeg, hipStream_t s) {
-    GI_HIP(hipMemcpyAsync(w->d_sel, &sel, sizeof(SelectorView), hipMemcpyHostToDevice, s));
+    if (!w->sel_valid || memcmp(&w->last_sel, &sel, sizeof(SelectorView)) != 0) {
+        GI_HIP(hipMemcpyAsync(w->d_sel, &sel, sizeof(SelectorView), hipMemcpyHostToDevice, s));
+        w->last_sel = sel;
+        w->sel_valid = true;
+    }
     SeedArgs sa{depth, parent, w->lane_max, w->hub_min, want_fdeg ? 1 : 0};
     ResetArgs r;
     r.depth = depth;

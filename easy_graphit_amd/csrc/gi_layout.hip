@@ -5,8 +5,9 @@
 // and depths mostly hit a few cache lines. Results are mapped back to the
 // original ids, so callers see no difference.
 //
-// Neighbour lists of the view keep the original order of each list (mapped
-// to new ids); no kernel here needs them sorted.
+// Neighbour lists of the view are sorted by new id, so the widest
+// neighbours come first: pull scans meet them early, and a wavefront walking
+// one list claims runs of neighbours that share a bitmap word.
 #include <climits>
 #include <vector>
 
@@ -32,19 +33,25 @@ __global__ void layout_inverse_kernel(const int32_t *old_of_new, const int32_t *
     }
 }
 
-// Edge-parallel over the original edges: edge e of old row o lands at the
-// same position k of new row new_of_old[o].
-__global__ void layout_edges_kernel(const int32_t *esrc, const int32_t *off, const int32_t *nbr,
-                                    const int32_t *w, const int32_t *new_of_old, const int32_t *off_new,
-                                    int64_t E, int32_t *nbr_new, int32_t *w_new) {
+// Every edge as a packed (new source, new target) key; sorted, the keys are
+// the view's CSR with each neighbour list in increasing new id (hubs first).
+__global__ void layout_edge_keys_kernel(const int32_t *esrc, const int32_t *nbr, const int32_t *new_of_old,
+                                        int64_t E, uint64_t *keys) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
-         e += (int64_t)gridDim.x * blockDim.x) {
-        const int32_t o = esrc[e];
-        const int32_t k = (int32_t)(e - off[o]);
-        const int32_t pos = off_new[new_of_old[o]] + k;
-        nbr_new[pos] = new_of_old[nbr[e]];
-        if (w) w_new[pos] = w[e];
-    }
+         e += (int64_t)gridDim.x * blockDim.x)
+        keys[e] = ((uint64_t)(uint32_t)new_of_old[esrc[e]] << 32) | (uint32_t)new_of_old[nbr[e]];
+}
+
+__global__ void layout_unpack_kernel(const uint64_t *keys, int64_t E, int32_t *nbr_new) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
+         e += (int64_t)gridDim.x * blockDim.x)
+        nbr_new[e] = (int32_t)(uint32_t)keys[e];
+}
+
+static int bits_of(int32_t n) {
+    int b = 1;
+    while (b < 31 && (1ll << b) < (int64_t)n) ++b;
+    return b;
 }
 
 __global__ void layout_gather_kernel(const int32_t *in_new, const int32_t *new_of_old, int32_t V,
@@ -170,9 +177,46 @@ static int build_layout(gi_graph *g, GraphLayout **out) {
             rc = fill_i32(deg + V, 1, 0, s);  // the scan's last term: offsets[V] = E
             if (rc == GI_OK) rc = exclusive_scan_i32(deg, t->d_off, (int64_t)V + 1, s);
         }
-        if (rc == GI_OK && E > 0)
-            layout_edges_kernel<<<grid_of(E), 256, 0, s>>>(g->d_esrc, g->d_off, g->d_nbr, g->d_w, L->new_of_old,
-                                                           t->d_off, E, t->d_nbr, t->d_w);
+        if (rc == GI_OK && E > 0) {
+            uint64_t *ka = nullptr, *kb2 = nullptr;
+            int32_t *wa = nullptr, *wb = nullptr;
+            rc = dalloc(&ka, (size_t)E + 1);
+            if (rc == GI_OK) rc = dalloc(&kb2, (size_t)E + 1);
+            if (rc == GI_OK && g->d_w) rc = dalloc(&wb, (size_t)E + 1);
+            if (rc == GI_OK) {
+                layout_edge_keys_kernel<<<grid_of(E), 256, 0, s>>>(g->d_esrc, g->d_nbr, L->new_of_old, E, ka);
+                const int end_bit = 32 + bits_of(V);
+                rocprim::double_buffer<uint64_t> ek(ka, kb2);
+                size_t eb = 0;
+                void *etmp = nullptr;
+                if (g->d_w) {  // weights travel with their edges
+                    rc = dalloc(&wa, (size_t)E + 1);
+                    if (rc == GI_OK && hipMemcpyAsync(wa, g->d_w, sizeof(int32_t) * E, hipMemcpyDeviceToDevice, s) !=
+                                           hipSuccess)
+                        rc = fail(GI_ERR_HIP, "layout weights");
+                    rocprim::double_buffer<int32_t> ew(wa, wb);
+                    (void)rocprim::radix_sort_pairs(nullptr, eb, ek, ew, (size_t)E, 0, end_bit, s);
+                    if (rc == GI_OK) rc = dmalloc(&etmp, eb);
+                    if (rc == GI_OK && rocprim::radix_sort_pairs(etmp, eb, ek, ew, (size_t)E, 0, end_bit, s) != hipSuccess)
+                        rc = fail(GI_ERR_HIP, "layout edge sort");
+                    if (rc == GI_OK &&
+                        hipMemcpyAsync(t->d_w, ew.current(), sizeof(int32_t) * E, hipMemcpyDeviceToDevice, s) != hipSuccess)
+                        rc = fail(GI_ERR_HIP, "layout weights");
+                } else {
+                    (void)rocprim::radix_sort_keys(nullptr, eb, ek, (size_t)E, 0, end_bit, s);
+                    rc = dmalloc(&etmp, eb);
+                    if (rc == GI_OK && rocprim::radix_sort_keys(etmp, eb, ek, (size_t)E, 0, end_bit, s) != hipSuccess)
+                        rc = fail(GI_ERR_HIP, "layout edge sort");
+                }
+                if (rc == GI_OK) layout_unpack_kernel<<<grid_of(E), 256, 0, s>>>(ek.current(), E, t->d_nbr);
+                if (rc == GI_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(GI_ERR_HIP, "layout sync");
+                dfree(etmp);
+            }
+            dfree(ka);
+            dfree(kb2);
+            dfree(wa);
+            dfree(wb);
+        }
         if (rc == GI_OK && hipGetLastError() != hipSuccess) rc = fail(GI_ERR_HIP, "layout kernels");
         if (rc == GI_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(GI_ERR_HIP, "layout sync");
     }
