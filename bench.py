@@ -161,7 +161,7 @@ def bench_apps(gi, g, sources, edges_of, log) -> dict:
                  "alg_bytes": "per round 8 E + 12 V (hook) + 8 V (shortcut)", "config": "RMAT-24 sym"}
     log(f"cc: {out['cc']}")
     # PageRank: RMAT-22 directed, 20 iterations, FP32 pull
-    gp = gi.Graph.rmat(22, 16, SEED, False)
+    gp = gi.Graph.rmat(22, 16, SEED, False).set_layout(g.layout)
     fp = gi.Schedule(direction=gi.PULL, load_balancing=gi.LB_VERTEX_BASED, kernel_fusion=1)
     gi.pagerank(gp, 20, schedule=fp)
     ts = []
@@ -173,7 +173,8 @@ def bench_apps(gi, g, sources, edges_of, log) -> dict:
     out["pr"] = {"gteps": round(20 * Ep / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3),
                  "iterations_per_s": round(20 / t, 1),
                  "roofline_frac": alg_frac(20 * (8 * Ep + 20 * Vp), t),
-                 "alg_bytes": "20 (8 E + 20 V)", "config": "RMAT-22 directed, 20 iterations, d 0.85"}
+                 "alg_bytes": "20 (8 E + 20 V)",
+                 "config": f"RMAT-22 directed, 20 iterations, d 0.85, {'degree' if gp.layout else 'native'} layout"}
     log(f"pr: {out['pr']}")
     gp.destroy()
     # Batched Brandes BC: RMAT-20, 256 sources, batched vs one source at a time

@@ -635,7 +635,14 @@ int gi_pagerank(gi_graph *g, int32_t iters, float damping, const gi_schedule *sc
         // fused engine: deterministic pull over the in-edges, one captured graph
         if (sched->direction != GI_PULL) return fail(GI_ERR_UNSUPPORTED, "fused PageRank pulls");
         EventTimer timer(s);
-        GI_TRY(pr_fused_run(g, iters, damping, d_rank, s));
+        if (g->layout) {  // degree-ordered view; ranks gathered back (bit copies)
+            float *rv = reinterpret_cast<float *>(layout_tmp(g, 0));
+            GI_TRY(pr_fused_run(layout_view(g), iters, damping, rv, s));
+            GI_TRY(layout_map_back(g, reinterpret_cast<const int32_t *>(rv), reinterpret_cast<int32_t *>(d_rank),
+                                   false, s));
+        } else {
+            GI_TRY(pr_fused_run(g, iters, damping, d_rank, s));
+        }
         double ms = timer.stop();
         if (stats) {
             stats->iterations = iters;

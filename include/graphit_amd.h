@@ -162,9 +162,10 @@ int gi_graph_destroy(gi_graph *g);
 /* Vertex layout used by the fused engines (no reference counterpart: the
  * reference keeps the file's vertex order, graph.h:236-338). DEGREE builds an
  * internal copy renumbered by decreasing degree; fused BFS (single- and
- * multi-source) and fused CC then run on it and map their results back to
- * the original ids, so outputs are unchanged. NATIVE drops the copy. Costs
- * one more CSR in device memory. */
+ * multi-source), fused CC and fused PageRank then run on it and map their
+ * results back to the original ids, so outputs are unchanged (fused SSSP
+ * keeps the native order). NATIVE drops the copy. Costs one more CSR in
+ * device memory. */
 enum gi_layout { GI_LAYOUT_NATIVE = 0, GI_LAYOUT_DEGREE = 1 };
 int gi_graph_set_layout(gi_graph *g, int32_t layout);
 int32_t gi_graph_layout(const gi_graph *g);

@@ -90,3 +90,16 @@ def test_layout_sweep_records(gi, rmat):
     feats, times = gi.bfs_sweep(g, src, reps=1)
     assert feats.shape[0] == O.bfs_depth(host, src).max() + 1
     assert (times > 0).all()
+
+
+def test_layout_pagerank(gi):
+    g = gi.Graph.rmat(14, 16, SEED, False)
+    off, nb, _ = g.read_csr()
+    want = O.pagerank(O.CSR(off, nb), 20)
+    sched = gi.Schedule(direction=1, load_balancing=0, kernel_fusion=1)
+    native, _ = gi.pagerank(g, 20, schedule=sched)
+    g.set_layout(gi.LAYOUT_DEGREE)
+    r, _ = gi.pagerank(g, 20, schedule=sched)
+    rel = np.abs(r - want).sum() / np.abs(want).sum()
+    assert rel <= 1e-5
+    assert np.abs(r - native).sum() / np.abs(native).sum() <= 1e-6  # only the summation order differs
