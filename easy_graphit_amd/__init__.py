@@ -202,6 +202,7 @@ def lib():
         "gi_pq_cutoff": (C.c_int, [vp, C.POINTER(i32)]),
         "gi_pq_destroy": (C.c_int, [vp]),
         "gi_sssp": (C.c_int, [vp, i32, i32, C.POINTER(Schedule), vp, C.POINTER(RunStats), vp]),
+        "gi_sssp_trace": (C.c_int, [vp, i32, i32, i32, vp, i32, vp, vp, C.POINTER(i32), vp]),
         "gi_cc": (C.c_int, [vp, C.POINTER(Schedule), vp, C.POINTER(RunStats), vp]),
         "gi_pagerank": (C.c_int, [vp, i32, C.c_float, C.POINTER(Schedule), vp,
                                   C.POINTER(RunStats), vp]),
@@ -683,6 +684,18 @@ def sssp(g: Graph, source: int, delta: int, schedule: Schedule | None = None, st
     check(lib().gi_sssp(g.h, source, delta, C.byref(schedule) if schedule is not None else None,
                         dist.ptr, C.byref(st), stream))
     return dist.numpy(stream), st.as_dict()
+
+
+def sssp_trace(g: Graph, source: int, delta: int, pull_min: int = -1, max_slots: int = 4096,
+               stream=None):
+    """Fused SSSP run slot by slot: (dist, kernel_ms[slots, 6], info[slots, 5])."""
+    dist = DeviceArray(g.num_vertices, np.int32)
+    ms = np.zeros((max_slots, 6), np.float32)
+    info = np.zeros((max_slots, 5), np.int64)
+    n = C.c_int32()
+    check(lib().gi_sssp_trace(g.h, source, delta, pull_min, dist.ptr, max_slots, ms.ctypes.data,
+                              info.ctypes.data, C.byref(n), stream))
+    return dist.numpy(stream), ms[: n.value].copy(), info[: n.value].copy()
 
 
 def cc(g: Graph, schedule: Schedule | None = None, stream=None):

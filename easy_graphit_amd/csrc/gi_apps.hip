@@ -504,13 +504,15 @@ int gi_sssp(gi_graph *g, int32_t source, int32_t delta, const gi_schedule *sched
     if (sched->kernel_fusion == GI_FUSION_ENABLED) {
         // fused engine: windows and phases decided on the device
         EventTimer timer(s);
-        int32_t steps = 0;
-        GI_TRY(sssp_fused_run(g, source, delta, d_dist, &steps, s));
+        int32_t counts[4] = {0, 0, 0, 0};
+        // native vertex order even when a degree-ordered copy exists: there the
+        // hubs share a few cache lines and the relaxations' atomics serialize
+        GI_TRY(sssp_fused_run(g, source, delta, d_dist, counts, s));
         double ms = timer.stop();
         if (stats) {
-            stats->iterations = steps;
-            stats->push_iterations = steps;
-            stats->pull_iterations = 0;
+            stats->iterations = counts[0];
+            stats->push_iterations = counts[0] - counts[1];
+            stats->pull_iterations = counts[1];
             stats->ms = ms;
             stats->reached = 0;
             stats->edges_traversed = 0;

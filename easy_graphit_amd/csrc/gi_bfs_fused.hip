@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "gi_bfs_fused.h"
+#include "gi_bitset.h"
 #include "gi_lb.h"
 
 namespace gi {
@@ -62,6 +63,7 @@ struct StepArgs {
     uint64_t *visited;
     uint64_t *bm[3];
     uint64_t *sum[3];
+    uint8_t *cflag;             // chunk flags of the output bitmap (bs_flag_chunk)
     int32_t *pcount;            // [2][kMaxGrid]: step slots, then hub slots
     unsigned long long *pfdeg;  // same layout
 };
@@ -73,7 +75,7 @@ struct StepCtx {
     int32_t want_fdeg;
     const uint64_t *bm_in;
     uint64_t *bm_out;
-    uint64_t *sum_out;
+    uint8_t *cflag;
     int32_t cnt;               // per-lane |F'| contribution
     unsigned long long fdeg;   // per-lane sum of degrees of F'
 };
@@ -94,12 +96,6 @@ __device__ __forceinline__ bool in_bitmap(const uint64_t *bm, int32_t v) {
     return (bm[v >> 6] >> (v & 63)) & 1ull;
 }
 
-__device__ __forceinline__ void mark_chunk(uint64_t *sum, int32_t word) {
-    int32_t ch = word / kChunkWords;
-    uint64_t bit = 1ull << (ch & 63);
-    if (!(sum[ch >> 6] & bit)) atomicOr((unsigned long long *)&sum[ch >> 6], bit);
-}
-
 // Push discovery: test, claim in `visited`, record in the output bitmap.
 __device__ __forceinline__ void push_edge(const StepArgs &a, StepCtx &c, int32_t s, int32_t e,
                                           bool active) {
@@ -111,7 +107,7 @@ __device__ __forceinline__ void push_edge(const StepArgs &a, StepCtx &c, int32_t
     unsigned long long old = atomicOr((unsigned long long *)vw, bit);
     if (old & bit) return;
     unsigned long long o2 = atomicOr((unsigned long long *)&c.bm_out[d >> 6], bit);
-    if (o2 == 0) mark_chunk(c.sum_out, d >> 6);
+    if (o2 == 0) bs_flag_chunk(c.cflag, d >> 6);
     if (a.parent) a.parent[d] = s;  // the depth is written in word order by the next step
     c.cnt += 1;
     if (c.want_fdeg) c.fdeg += (unsigned long long)deg_of(a.off, d);
@@ -147,7 +143,7 @@ __device__ __forceinline__ void push_edge_wave(const StepArgs &a, StepCtx &c, in
         won = x & ~old;
         if (won) {
             const unsigned long long o2 = atomicOr((unsigned long long *)&c.bm_out[key], won);
-            if (o2 == 0) mark_chunk(c.sum_out, key);
+            if (o2 == 0) bs_flag_chunk(c.cflag, key);
         }
     }
     const uint64_t ends = __ballot(end);
@@ -449,7 +445,7 @@ __device__ __forceinline__ void hub_body(const StepArgs &a0) {
     c.hub_min = st->hub_min;
     c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], c.level % 3);
     c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], (c.level + 1) % 3);
-    c.sum_out = pick3(a0.sum[0], a0.sum[1], a0.sum[2], (c.level + 1) % 3);
+    c.cflag = a0.cflag;
     c.want_fdeg = st->want_fdeg;
     c.cnt = 0;
     c.fdeg = 0;
@@ -514,7 +510,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a0) {
     const int32_t iin = c.level % 3, iout = (c.level + 1) % 3, iclr = (c.level + 2) % 3;
     c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iin);
     c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iout);
-    c.sum_out = pick3(a0.sum[0], a0.sum[1], a0.sum[2], iout);
+    c.cflag = a0.cflag;
     c.cnt = 0;
     c.fdeg = 0;
     const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -590,7 +586,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a0) {
             }
             bool any = false;
             if (all != ~0ull) any = pull_words(a, c, mode, w0, vis);  // wave-uniform
-            if (__syncthreads_or(any) && threadIdx.x == 0) mark_chunk(c.sum_out, un * kPullUnit);
+            if (__syncthreads_or(any) && threadIdx.x == 0) bs_flag_chunk(c.cflag, un * kPullUnit);
         }
     }
     flush_counts(a.pcount, a.pfdeg, c);
@@ -619,7 +615,8 @@ struct SumSet {
 
 __global__ void __launch_bounds__(kDecideBlock)
 bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E, int32_t *pcount,
-                  unsigned long long *pfdeg, int32_t slots, SumSet sums, int32_t sum_words) {
+                  unsigned long long *pfdeg, int32_t slots, SumSet sums, int32_t sum_words, uint8_t *cflag,
+                  int32_t chunks) {
     if (st->done) return;  // uniform
     const SelectorView sel = *selp;
     // after step `level`, the bitmap two levels ahead of it was cleared by the
@@ -670,6 +667,8 @@ bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E, 
             rf[wid] = fdeg;
         }
         for (int32_t i = threadIdx.x; i < sum_words; i += kDecideBlock) sum_clear[i] = 0;
+        // the summary of F', from the flags its chunks set
+        bs_build_summary<kDecideBlock>(cflag, chunks, pick3(sums.s[0], sums.s[1], sums.s[2], next_level % 3));
         __syncthreads();
         n = 0;
         fdeg = 0;
@@ -908,6 +907,7 @@ struct FusedWork {
     BfsState *st = nullptr;
     uint64_t *bm[3] = {nullptr, nullptr, nullptr};
     uint64_t *sum[3] = {nullptr, nullptr, nullptr};
+    uint8_t *cflag = nullptr;        // chunk flags (bs_flag_chunk), zero between steps
     int32_t *pcount = nullptr;
     unsigned long long *pfdeg = nullptr;
     int32_t *reached = nullptr;
@@ -946,6 +946,7 @@ static void free_work(FusedWork *w) {
     for (auto p : w->sum) dfree(p);
     dfree(w->pcount);
     dfree(w->pfdeg);
+    dfree(w->cflag);
     dfree(w->reached);
     dfree(w->dead);
     dfree(w->cand);
@@ -981,6 +982,9 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->sum[i], (size_t)sum_words + 1);
     if (rc == GI_OK) rc = dalloc(&w->pcount, 2 * (size_t)kMaxGrid);
     if (rc == GI_OK) rc = dalloc(&w->pfdeg, 2 * (size_t)kMaxGrid);
+    if (rc == GI_OK) rc = dalloc(&w->cflag, (size_t)bs_flag_bytes(chunks));
+    if (rc == GI_OK && hipMemset(w->cflag, 0, (size_t)bs_flag_bytes(chunks)) != hipSuccess)
+        rc = fail(GI_ERR_HIP, "flag init");
     if (rc == GI_OK) rc = dalloc(&w->reached, 1);
     if (rc == GI_OK) rc = dalloc(&w->dead, (size_t)words + 1);
     if (rc == GI_OK) rc = dalloc(&w->cand, (size_t)g->V + 1);
@@ -1078,6 +1082,7 @@ static StepArgs step_args(gi_graph *g, FusedWork *w) {
         a.bm[i] = w->bm[i];
         a.sum[i] = w->sum[i];
     }
+    a.cflag = w->cflag;
     a.pcount = w->pcount;
     a.pfdeg = w->pfdeg;
     return a;
@@ -1087,7 +1092,8 @@ static void launch_decide(gi_graph *g, FusedWork *w, int32_t level, hipStream_t 
     (void)level;  // the kernel reads the level from the device state
     SumSet sums{{w->sum[0], w->sum[1], w->sum[2]}};
     bfs_decide_kernel<<<1, kDecideBlock, 0, s>>>(w->st, w->d_sel, g->V, g->E, w->pcount, w->pfdeg,
-                                                 step_slots(w), sums, w->sum_words);
+                                                 step_slots(w), sums, w->sum_words, w->cflag,
+                                                 (int32_t)ceil_div(bitmap_words(g->V), kChunkWords));
 }
 
 // A captured level: both step kernels (one returns at once), the hub kernel
@@ -1406,6 +1412,7 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
                 GI_HIP(hipMemcpyAsync(g->d_visited, vis_bak, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, s));
                 GI_TRY(fill_u64(w->bm[iout], words, 0, s));
                 GI_TRY(fill_u64(w->sum[iout], w->sum_words, 0, s));
+                GI_TRY(fill_u64(reinterpret_cast<uint64_t *>(w->cflag), bs_flag_bytes(a.chunks) / 8, 0, s));
                 zero_slots_kernel<<<8, 256, 0, s>>>(w->pcount, w->pfdeg);
                 bfs_force_mode_kernel<<<1, 1, 0, s>>>(w->st, run);
                 GI_HIP(hipEventRecord(e0, s));

@@ -22,6 +22,45 @@ __device__ __forceinline__ bool bs_test(const uint64_t *bm, int32_t v) {
     return (bm[v >> 6] >> (v & 63)) & 1ull;
 }
 
+// Chunk flags: a kernel that makes a chunk non-empty stores 1 into the
+// chunk's byte, and one workgroup later rebuilds the summary from the flags
+// (bs_build_summary). Marking the summary from every workgroup instead puts
+// all their atomics on the summary's few cache lines, where they serialize.
+__device__ __forceinline__ void bs_flag_chunk(uint8_t *flags, int32_t word) {
+    flags[word / kBsChunkWords] = 1;
+}
+
+// Bytes of flag storage for `chunks` chunks (whole 16-byte groups).
+__host__ __device__ constexpr int64_t bs_flag_bytes(int64_t chunks) { return (chunks + 15) / 16 * 16; }
+
+// 4 flag bytes (each 0 or 1) -> 4-bit mask.
+__device__ __forceinline__ uint32_t bs_byte_mask(uint32_t x) {
+    return (x & 1u) | ((x >> 7) & 2u) | ((x >> 14) & 4u) | ((x >> 21) & 8u);
+}
+
+// Rewrites every word of summary `sum` from the chunk flags and clears the
+// flags; one workgroup of Block threads, every thread calls it. Thread i
+// reads the 16 flags of chunks 16i..16i+15 and four neighbouring threads make
+// one 64-bit summary word.
+template <int Block>
+__device__ __forceinline__ void bs_build_summary(uint8_t *flags, int32_t chunks, uint64_t *sum) {
+    static_assert(Block % 64 == 0, "whole wavefronts");
+    uint4 *f4 = reinterpret_cast<uint4 *>(flags);
+    const int32_t n16 = (chunks + 15) / 16;
+    for (int32_t base = 0; base < n16; base += Block) {  // uniform
+        const int32_t i = base + (int32_t)threadIdx.x;
+        uint4 q = make_uint4(0u, 0u, 0u, 0u);
+        if (i < n16) q = f4[i];
+        const uint32_t m16 = bs_byte_mask(q.x) | (bs_byte_mask(q.y) << 4) | (bs_byte_mask(q.z) << 8) |
+                             (bs_byte_mask(q.w) << 12);
+        if (m16) f4[i] = make_uint4(0u, 0u, 0u, 0u);
+        uint64_t word = (uint64_t)m16 << (16 * (threadIdx.x & 3));
+        word |= __shfl_xor(word, 1);
+        word |= __shfl_xor(word, 2);
+        if ((threadIdx.x & 3) == 0 && i < n16) sum[i >> 2] = word;
+    }
+}
+
 // Marks the chunk holding `word` in the summary (test first: most calls find
 // the bit already set).
 __device__ __forceinline__ void bs_mark_chunk(uint64_t *sum, int32_t word) {

@@ -570,11 +570,20 @@ int gi_graph_set_weights(gi_graph *g, uint64_t seed, int32_t wmin, int32_t wmax)
     if (g->E) weights_kernel<<<grid_for(g->E), 256, 0, s>>>(g->d_esrc, g->d_nbr, (int32_t)g->E, seed, wmin, wmax, g->d_w);
     GI_LAUNCH_CHECK();
     GI_HIP(hipStreamSynchronize(s));
-    // cached derived graphs carry old weights: drop them
+    // cached derived graphs carry old weights, and the fused engines' state
+    // points into them: drop both
+    release_fused(g);
+    release_pr(g);
+    release_cc(g);
+    release_sssp(g);
     if (g->transposed && g->transposed != g) { gi_graph *t = g->transposed; g->transposed = nullptr; t->borrowed = false; gi_graph_destroy(t); }
     g->transposed = nullptr;
     for (auto &kv : g->blocked) { kv.second->borrowed = false; gi_graph_destroy(kv.second); }
     g->blocked.clear();
+    if (g->layout) {  // the degree-ordered copy carries the weights too
+        release_layout(g);
+        GI_TRY(build_layout(g));
+    }
     return GI_OK;
 }
 

@@ -278,7 +278,8 @@ int pr_fused_run(gi_graph *g, int32_t iters, float damping, float *d_rank, hipSt
 void release_pr(const gi_graph *g);
 int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds, hipStream_t s);
 void release_cc(const gi_graph *g);
-int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *steps,
+// counts (optional): relax steps, pull steps, windows entered, captured slots.
+int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,
                    hipStream_t s);
 void release_sssp(const gi_graph *g);
 
@@ -286,6 +287,7 @@ void release_sssp(const gi_graph *g);
 // source mapping, two V-sized scratch arrays in view order, and the
 // mappings of results back to the original ids.
 void release_layout(gi_graph *g);
+int build_layout(gi_graph *g);  // degree-ordered copy (with the weights, if any)
 gi_graph *layout_view(const gi_graph *g);
 int32_t layout_map_source(const gi_graph *g, int32_t v);
 int32_t *layout_tmp(const gi_graph *g, int i);

@@ -132,7 +132,7 @@ void release_layout(gi_graph *g) {
     g->layout = nullptr;
 }
 
-static int build_layout(gi_graph *g, GraphLayout **out) {
+int build_layout(gi_graph *g) {
     const int32_t V = g->V;
     const int64_t E = g->E;
     hipStream_t s = 0;
@@ -237,7 +237,6 @@ static int build_layout(gi_graph *g, GraphLayout **out) {
         release_layout(g);
         return rc;
     }
-    *out = L;
     return GI_OK;
 }
 
@@ -282,8 +281,7 @@ int gi_graph_set_layout(gi_graph *g, int32_t layout) {
     }
     if (layout != GI_LAYOUT_DEGREE) return fail(GI_ERR_RANGE, "unknown layout %d", layout);
     if (g->layout) return GI_OK;
-    GraphLayout *L = nullptr;
-    return build_layout(g, &L);
+    return build_layout(g);
 }
 
 int32_t gi_graph_layout(const gi_graph *g) { return g && g->layout ? GI_LAYOUT_DEGREE : GI_LAYOUT_NATIVE; }

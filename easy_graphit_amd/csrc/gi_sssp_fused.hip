@@ -4,22 +4,28 @@
 // phases captured as hipGraphs. Shortest-path distances are unique, so the
 // order of relaxations does not change the result.
 //
-// Sets (bitmaps over the vertices):
-//   * the near frontier F: vertices improved below the window's upper bound
-//     `hi`, three rotating bitmaps with chunk summaries (as in the fused BFS);
-//   * the far set: vertices improved to a distance >= hi, waiting for a
-//     later window (the reference's priority queue, gpu_priority_queue.h).
+// The window is [lo, hi). Every vertex below lo is settled; the near
+// frontier F holds the vertices improved into the window since their last
+// expansion (three rotating bitmaps with chunk summaries, as in the fused
+// BFS). Vertices improved to a distance >= hi are pending: they are not kept
+// in a set of their own (the reference's priority queue,
+// gpu_priority_queue.h), because a vertex is pending exactly when
+// hi <= dist < INT_MAX; a window advance finds them with one coalesced scan
+// of dist, so relaxations into later windows cost no atomics beyond the
+// atomicMin on dist.
 // A captured slot is six launches, each a no-op unless its mode is active:
-//   ss_step    push: relax the out-edges of F (lane / wave / grid-wide hub
-//              bins); an improvement below hi joins F', else the far set;
+//   ss_step    push: relax the out-edges of F, spread evenly over each
+//              workgroup (vertices of at least kSsHubMin edges go to the
+//              grid-wide bin); an improvement below hi joins F';
 //   ss_pull    pull, for a large F: each unsettled vertex takes the minimum
 //              over its in-neighbours in F (no atomics on distances);
 //   ss_hub     the grid-wide bin: edges of very wide frontier vertices;
-//   ss_decide  count F'; empty -> advance the window [hi, hi + delta);
-//   ss_farmove move far vertices below the new hi into F (wave per word,
-//              coalesced distance loads);
-//   ss_decide2 count the moves; none -> next window, or done when the far
-//              set is empty.
+//   ss_decide  count F'; empty -> advance to [hi, hi + delta);
+//   ss_farmove scan dist: [lo, hi) joins F; also the least distance >= hi;
+//   ss_decide2 count the joins; none -> jump to the window that starts at
+//              that least distance (gpu_priority_queue.h:126-146 starts a
+//              window at the minimum remaining priority too), or done when
+//              nothing is pending.
 #include <climits>
 #include <map>
 #include <vector>
@@ -30,11 +36,11 @@
 namespace gi {
 
 constexpr int kSsBlock = 256;
-constexpr int32_t kSsLaneMax = 64;    // narrower vertices: one lane walks the edges
 constexpr int32_t kSsHubMin = 4096;   // at least this wide: grid-wide bin
 constexpr int32_t kSsMaxHubs = 1024;
 constexpr int32_t kSsMaxGrid = 8192;  // partial-sum slots per kernel
 constexpr int32_t kSsPullDiv = 8;     // a step pulls when |F| > V / kSsPullDiv
+constexpr int32_t kSsMinSlots = 9;    // slot row of the window scan's least pending distances
 
 enum SsMode : int32_t { SS_RELAX = 0, SS_ADVANCE = 1 };
 
@@ -45,13 +51,16 @@ struct SsState {
     int32_t steps;       // relax steps run
     int32_t windows;     // windows entered after the first
     int32_t slots;       // captured slots consumed
+    int32_t pulls;       // relax steps that pulled
     int32_t hub_count;
-    int32_t hi;          // window upper bound (exclusive)
+    int32_t lo;          // window [lo, hi)
+    int32_t hi;
     int32_t delta;
     int32_t front;       // |F|
-    long long far;       // |far set|
+    int32_t next_min;    // least pending distance >= hi seen by the last scan
     int32_t pull;        // the next relax step pulls (large frontier)
     int32_t pull_min;    // |F| above which a step pulls
+    long long fdeg;      // out-edges of F
     int32_t hubs[kSsMaxHubs];
 };
 
@@ -69,8 +78,9 @@ struct SsArgs {
     int32_t sum_words;
     uint64_t *bm[3];
     uint64_t *sum[3];
-    uint64_t *far;
-    int32_t *slots;      // [3 kernels][3 counters][kSsMaxGrid]
+    uint8_t *cflag;      // chunk flags of the bitmap being filled (bs_flag_chunk)
+    int32_t chunks;
+    int32_t *slots;      // [3 kernels][3 counters][kSsMaxGrid], then row kSsMinSlots
     int32_t wide_chunks;       // pull: chunks of the in-lists of wide vertices
     const int32_t *chunk_v;    // vertex of each chunk
     const int32_t *chunk_e0;   // first in-edge of each chunk
@@ -79,47 +89,50 @@ struct SsArgs {
 struct SsCtx {
     int32_t hi;
     uint64_t *bm_out;
-    uint64_t *sum_out;
     int32_t nf;    // per-lane: new members of F'
-    int32_t nfar;  // per-lane: new members of the far set
+    int32_t fdeg;  // per-lane: their out-edges
 };
 
-// writeMin relaxation (apps/sssp.cpp:24-27): a strict improvement joins F'
-// when below hi, else the far set.
+// writeMin (apps/sssp.cpp:24-27) of dist[v] to nd, known to be below the
+// value last read: a strict improvement below hi joins F'; one at or above
+// hi leaves the vertex pending.
+__device__ __forceinline__ void ss_claim(const SsArgs &a, SsCtx &c, int32_t v, int32_t nd) {
+    const int32_t old = atomicMin(&a.dist[v], nd);
+    if (nd >= old || nd >= c.hi) return;
+    const uint64_t bit = 1ull << (v & 63);
+    const unsigned long long o = atomicOr((unsigned long long *)&c.bm_out[v >> 6], bit);
+    if (!(o & bit)) {
+        c.nf += 1;
+        c.fdeg += a.off[v + 1] - a.off[v];
+        if (o == 0) bs_flag_chunk(a.cflag, v >> 6);
+    }
+}
+
+// Relaxation of edge e from a vertex at distance du.
 __device__ __forceinline__ void ss_relax(const SsArgs &a, SsCtx &c, int32_t du, int32_t e, bool active) {
     if (!active) return;
     const int32_t v = a.nbr[e];
     const int32_t nd = du + a.wt[e];
     if (nd >= a.dist[v]) return;  // test before the atomic
-    const int32_t old = atomicMin(&a.dist[v], nd);
-    if (nd >= old) return;
-    const uint64_t bit = 1ull << (v & 63);
-    if (nd < c.hi) {
-        const unsigned long long o = atomicOr((unsigned long long *)&c.bm_out[v >> 6], bit);
-        if (!(o & bit)) {
-            c.nf += 1;
-            if (o == 0) bs_mark_chunk(c.sum_out, v >> 6);
-        }
-    } else {
-        const unsigned long long o = atomicOr((unsigned long long *)&a.far[v >> 6], bit);
-        if (!(o & bit)) c.nfar += 1;
-    }
+    ss_claim(a, c, v, nd);
 }
+
+constexpr int kSsBatch = 4;  // edges per thread with their loads in flight together
 
 __global__ void __launch_bounds__(kSsBlock) ss_step_kernel(SsArgs a) {
     SsState *st = a.st;
     if (st->done || st->mode != SS_RELAX || st->pull) return;  // uniform
     __shared__ int32_t pre[kBsMaxSumWords + 1];
     __shared__ int32_t list[kBsChunkVerts];
+    __shared__ int32_t pre_s[kSsBlock], e0_s[kSsBlock], du_s[kSsBlock];
     __shared__ int32_t n_s;
     const int32_t phase = st->phase;
     const int32_t iin = phase % 3, iout = (phase + 1) % 3, iclr = (phase + 2) % 3;
     SsCtx c;
     c.hi = st->hi;
     c.bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iout);
-    c.sum_out = bs_pick3(a.sum[0], a.sum[1], a.sum[2], iout);
     c.nf = 0;
-    c.nfar = 0;
+    c.fdeg = 0;
     // the bitmap two phases ahead is free (its summary is cleared by ss_decide)
     bs_clear_chunks(bs_pick3(a.bm[0], a.bm[1], a.bm[2], iclr), bs_pick3(a.sum[0], a.sum[1], a.sum[2], iclr),
                     a.words, a.sum_words, pre);
@@ -148,31 +161,50 @@ __global__ void __launch_bounds__(kSsBlock) ss_step_kernel(SsArgs a) {
         }
         __syncthreads();
         const int32_t n = n_s;
+        // rounds of 256 members: their edges (hubs aside) are spread evenly
+        // over the workgroup, each thread finding its edge's owner in LDS
         for (int32_t b0 = 0; b0 < n; b0 += kSsBlock) {
             const int32_t j = b0 + (int32_t)threadIdx.x;
             const bool has = j < n;
             const int32_t s = has ? list[j] : 0;
             const int32_t e0 = has ? a.off[s] : 0, e1 = has ? a.off[s + 1] : 0;
-            const int32_t du = has ? a.dist[s] : 0;
             const int32_t dg = e1 - e0;
             const bool hub = has && dg >= kSsHubMin;
             const int32_t hpos = wave_atomic_add(&st->hub_count, hub);
             if (hub && hpos < kSsMaxHubs) st->hubs[hpos] = s;
-            const bool wide = has && dg >= kSsLaneMax && (!hub || hpos >= kSsMaxHubs);
-            uint64_t wm = __ballot(wide);
-            while (wm) {  // wave bin
-                const int32_t l = __ffsll((unsigned long long)wm) - 1;
-                wm &= wm - 1;
-                const int32_t w0 = __shfl(e0, l), w1 = __shfl(e1, l), wd = __shfl(du, l);
-                for (int32_t bb = w0; bb < w1; bb += 64) ss_relax(a, c, wd, bb + lane, bb + lane < w1);
+            const bool mine = has && (!hub || hpos >= kSsMaxHubs);
+            int32_t total = 0;
+            const int32_t ex = block_exclusive_scan(mine ? dg : 0, &total);
+            pre_s[threadIdx.x] = ex;
+            e0_s[threadIdx.x] = e0;
+            du_s[threadIdx.x] = mine ? a.dist[s] : 0;
+            __syncthreads();
+            for (int32_t k0 = 0; k0 < total; k0 += kSsBlock * kSsBatch) {
+                int32_t v[kSsBatch], nd[kSsBatch], dv[kSsBatch];
+#pragma unroll
+                for (int p = 0; p < kSsBatch; ++p) {
+                    const int32_t k = k0 + p * kSsBlock + (int32_t)threadIdx.x;
+                    v[p] = -1;
+                    nd[p] = 0;
+                    if (k < total) {  // owner: the last member whose prefix is <= k
+                        const int32_t o = upper_bound_i32(pre_s, 0, kSsBlock, k) - 1;
+                        const int32_t e = e0_s[o] + (k - pre_s[o]);
+                        v[p] = a.nbr[e];
+                        nd[p] = du_s[o] + a.wt[e];
+                    }
+                }
+#pragma unroll
+                for (int p = 0; p < kSsBatch; ++p) dv[p] = v[p] >= 0 ? a.dist[v[p]] : INT_MIN;
+#pragma unroll
+                for (int p = 0; p < kSsBatch; ++p)
+                    if (nd[p] < dv[p]) ss_claim(a, c, v[p], nd[p]);
             }
-            if (has && dg < kSsLaneMax)  // lane bin
-                for (int32_t e = e0; e < e1; ++e) ss_relax(a, c, du, e, true);
+            __syncthreads();
         }
         __syncthreads();
     }
     (void)wid;
-    bs_flush3<kSsBlock>(a.slots, kSsMaxGrid, c.nf, c.nfar, 0);
+    bs_flush3<kSsBlock>(a.slots, kSsMaxGrid, c.nf, c.fdeg, 0);
 }
 
 // Pull relax step for a large frontier: every vertex not yet settled below
@@ -190,16 +222,15 @@ __global__ void __launch_bounds__(kSsBlock) ss_pull_kernel(SsArgs a) {
     if (st->done || st->mode != SS_RELAX || !st->pull) return;  // uniform
     __shared__ int32_t pre[kBsMaxSumWords + 1];
     __shared__ int32_t best_s[kSsBlock / 64][64];
-    const int32_t phase = st->phase, hi = st->hi, lo = st->hi - st->delta;
+    const int32_t phase = st->phase, hi = st->hi, lo = st->lo;
     const int32_t iin = phase % 3, iout = (phase + 1) % 3, iclr = (phase + 2) % 3;
     bs_clear_chunks(bs_pick3(a.bm[0], a.bm[1], a.bm[2], iclr), bs_pick3(a.sum[0], a.sum[1], a.sum[2], iclr),
                     a.words, a.sum_words, pre);
     const uint64_t *bm_in = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iin);
     uint64_t *bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iout);
-    uint64_t *sum_out = bs_pick3(a.sum[0], a.sum[1], a.sum[2], iout);
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
     int32_t *best = best_s[wid];
-    int32_t nf = 0, nfar = 0;
+    int32_t nf = 0, fdeg = 0;
     const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
     for (int32_t un = blockIdx.x; un < units; un += gridDim.x) {
         bool any = false;
@@ -253,25 +284,22 @@ __global__ void __launch_bounds__(kSsBlock) ss_pull_kernel(SsArgs a) {
             const int32_t nd = best[lane];
             const bool imp = open && nd < dv;
             if (imp) a.dist[v] = nd;
-            const uint64_t near = __ballot(imp && nd < hi);
-            const uint64_t farb = __ballot(imp && nd >= hi);
-            if (lane == 0) {  // atomics: a wide vertex of the same word may join too
-                if (near) {
-                    const uint64_t old = atomicOr((unsigned long long *)&bm_out[w], near);
-                    nf += __popcll(near & ~old);
-                    any = true;
-                }
-                if (farb) {
-                    const uint64_t old = atomicOr((unsigned long long *)&a.far[w], farb);
-                    nfar += __popcll(farb & ~old);
-                }
+            const bool join = imp && nd < hi;
+            const uint64_t near = __ballot(join);
+            uint64_t old = 0;
+            if (lane == 0 && near) {  // atomic: a wide vertex of the same word may join too
+                old = atomicOr((unsigned long long *)&bm_out[w], near);
+                nf += __popcll(near & ~old);
+                any = true;
             }
+            old = __shfl(old, 0);
+            if (join && !((old >> lane) & 1ull)) fdeg += a.off[v + 1] - a.off[v];
             __builtin_amdgcn_wave_barrier();
         }
-        if (__syncthreads_or(any) && threadIdx.x == 0) bs_mark_chunk(sum_out, un * kSsPullUnit);
+        if (__syncthreads_or(any) && threadIdx.x == 0) bs_flag_chunk(a.cflag, un * kSsPullUnit);
     }
     // wide vertices: each chunk of an in-list is a workgroup item; its minimum
-    // joins dist[v] by atomicMin, and an improvement joins F' or the far set
+    // joins dist[v] by atomicMin, and an improvement below hi joins F'
     const int32_t per_wave = kSsPullChunk / (kSsBlock / 64);
     for (int32_t k = blockIdx.x; k < a.wide_chunks; k += gridDim.x) {
         const int32_t v = a.chunk_v[k];
@@ -295,22 +323,18 @@ __global__ void __launch_bounds__(kSsBlock) ss_pull_kernel(SsArgs a) {
         }
         if (lane == 0 && m < dv) {
             const int32_t old = atomicMin(&a.dist[v], m);
-            if (m < old) {
+            if (m < old && m < hi) {
                 const uint64_t bit = 1ull << (v & 63);
-                if (m < hi) {
-                    const unsigned long long o = atomicOr((unsigned long long *)&bm_out[v >> 6], bit);
-                    if (!(o & bit)) {
-                        nf += 1;
-                        if (o == 0) bs_mark_chunk(sum_out, v >> 6);
-                    }
-                } else {
-                    const unsigned long long o = atomicOr((unsigned long long *)&a.far[v >> 6], bit);
-                    if (!(o & bit)) nfar += 1;
+                const unsigned long long o = atomicOr((unsigned long long *)&bm_out[v >> 6], bit);
+                if (!(o & bit)) {
+                    nf += 1;
+                    fdeg += a.off[v + 1] - a.off[v];
+                    if (o == 0) bs_flag_chunk(a.cflag, v >> 6);
                 }
             }
         }
     }
-    bs_flush3<kSsBlock>(a.slots, kSsMaxGrid, nf, nfar, 0);
+    bs_flush3<kSsBlock>(a.slots, kSsMaxGrid, nf, fdeg, 0);
 }
 
 // Grid-wide bin: the concatenated edges of the deferred hubs, spread evenly.
@@ -324,9 +348,8 @@ __global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a) {
     SsCtx c;
     c.hi = st->hi;
     c.bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], (phase + 1) % 3);
-    c.sum_out = bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 1) % 3);
     c.nf = 0;
-    c.nfar = 0;
+    c.fdeg = 0;
     __shared__ int32_t hscan[kSsMaxHubs + 1];
     __shared__ int32_t hv[kSsMaxHubs];
     __shared__ int32_t hd[kSsMaxHubs];
@@ -352,7 +375,7 @@ __global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a) {
         const int32_t j = upper_bound_i32(hscan, 0, nh, k) - 1;
         ss_relax(a, c, hd[j], a.off[hv[j]] + (k - hscan[j]), true);
     }
-    bs_flush3<kSsBlock>(a.slots + 3 * kSsMaxGrid, kSsMaxGrid, c.nf, c.nfar, 0);
+    bs_flush3<kSsBlock>(a.slots + 3 * kSsMaxGrid, kSsMaxGrid, c.nf, c.fdeg, 0);
 }
 
 // Adds and zeroes counter `which` of slot region `region` over `slots` blocks.
@@ -374,70 +397,90 @@ __device__ __forceinline__ long long ss_collect(int32_t *slot_base, int32_t regi
     return s;
 }
 
+__device__ __forceinline__ int32_t ss_window_end(int32_t lo, int32_t delta) {
+    return lo > INT_MAX - delta ? INT_MAX : lo + delta;
+}
+
 __global__ void __launch_bounds__(kSsBlock) ss_decide_kernel(SsArgs a, int32_t nslots) {
     SsState *st = a.st;
     if (st->done || st->mode != SS_RELAX) return;  // uniform
     const long long nf = ss_collect(a.slots, 0, 0, nslots) + ss_collect(a.slots, 1, 0, nslots);
-    const long long nfar = ss_collect(a.slots, 0, 1, nslots) + ss_collect(a.slots, 1, 1, nslots);
+    const long long fdeg = ss_collect(a.slots, 0, 1, nslots) + ss_collect(a.slots, 1, 1, nslots);
     const int32_t phase = st->phase;
     uint64_t *sclr = bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 2) % 3);
     for (int32_t i = threadIdx.x; i <= a.sum_words; i += kSsBlock) sclr[i] = 0;
+    // the summary of F', from the flags its chunks set
+    bs_build_summary<kSsBlock>(a.cflag, a.chunks, bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 1) % 3));
     if (threadIdx.x == 0) {
-        st->far += nfar;
         st->phase = phase + 1;
         st->steps += 1;
+        st->pulls += st->pull;
         st->front = (int32_t)nf;
+        st->fdeg = fdeg;
         st->hub_count = 0;
         st->pull = nf > st->pull_min;
-        if (nf == 0) {
-            if (st->far == 0) {
-                st->done = 1;
-            } else {  // the window is settled: the next one is [hi, hi + delta)
-                st->mode = SS_ADVANCE;
-                st->hi += st->delta;
-            }
+        if (nf == 0) {  // the window is settled: scan for [hi, hi + delta)
+            st->mode = SS_ADVANCE;
+            st->lo = st->hi;
+            st->hi = ss_window_end(st->hi, st->delta);
+            st->next_min = INT_MAX;
         }
     }
 }
 
+// Window scan: every vertex with lo <= dist < hi joins F (each wave owns its
+// words: plain stores into the empty bitmap); the least distance >= hi of a
+// reached vertex goes to next_min; the out-edges of the joined vertices are
+// counted. Units of 16 words per workgroup (4 per
+// wave, loads in flight together); a unit's summary bit costs one atomic.
 __global__ void __launch_bounds__(kSsBlock) ss_farmove_kernel(SsArgs a) {
     SsState *st = a.st;
     if (st->done || st->mode != SS_ADVANCE) return;
-    const int32_t hi = st->hi, phase = st->phase;
+    __shared__ int32_t red[kSsBlock / 64];
+    const int32_t lo = st->lo, hi = st->hi, phase = st->phase;
     uint64_t *f = bs_pick3(a.bm[0], a.bm[1], a.bm[2], phase % 3);  // empty: the last step found nothing
-    uint64_t *sf = bs_pick3(a.sum[0], a.sum[1], a.sum[2], phase % 3);
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    int32_t moved = 0;
-    // units of 16 words per workgroup (4 per wave, loads in flight together);
-    // a unit's summary bit costs at most one atomic
+    int32_t moved = 0, fdeg = 0;
+    int32_t beyond = INT_MAX;
     const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
     for (int32_t un = blockIdx.x; un < units; un += gridDim.x) {
         const int32_t w0 = un * kSsPullUnit + wid * 4;
-        uint64_t fw[4];
         int32_t dv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) fw[i] = w0 + i < a.words ? a.far[w0 + i] : 0ull;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int32_t v = (w0 + i) * 64 + lane;
-            dv[i] = ((fw[i] >> lane) & 1ull) ? a.dist[v] : INT_MAX;
+            dv[i] = v < a.V ? a.dist[v] : INT_MAX;
         }
         bool any = false;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const uint64_t m = __ballot(dv[i] < hi);
+            const int32_t v = (w0 + i) * 64 + lane;
+            const bool in = dv[i] >= lo && dv[i] < hi;  // dv is INT_MAX past V
+            const uint64_t m = __ballot(in);
+            if (in) fdeg += a.off[v + 1] - a.off[v];
+            if (dv[i] >= hi && dv[i] < beyond) beyond = dv[i];
             if (m) {
                 any = true;
                 if (lane == 0) {
-                    a.far[w0 + i] = fw[i] & ~m;  // this wave owns the word
                     f[w0 + i] = m;
                     moved += __popcll(m);
                 }
             }
         }
-        if (__syncthreads_or(any) && threadIdx.x == 0) bs_mark_chunk(sf, un * kSsPullUnit);
+        if (__syncthreads_or(any) && threadIdx.x == 0) bs_flag_chunk(a.cflag, un * kSsPullUnit);
     }
-    bs_flush3<kSsBlock>(a.slots + 6 * kSsMaxGrid, kSsMaxGrid, moved, 0, 0);
+    for (int o = 32; o > 0; o >>= 1) {
+        const int32_t y = __shfl_xor(beyond, o);
+        beyond = y < beyond ? y : beyond;
+    }
+    if (lane == 0) red[wid] = beyond;
+    __syncthreads();
+    if (threadIdx.x == 0) {  // one slot per workgroup: no atomics on a single word
+        int32_t m = red[0];
+        for (int i = 1; i < kSsBlock / 64; ++i) m = red[i] < m ? red[i] : m;
+        a.slots[kSsMinSlots * kSsMaxGrid + blockIdx.x] = m;
+    }
+    bs_flush3<kSsBlock>(a.slots + 6 * kSsMaxGrid, kSsMaxGrid, moved, fdeg, 0);
 }
 
 __global__ void __launch_bounds__(kSsBlock) ss_decide2_kernel(SsArgs a, int32_t nslots) {
@@ -446,17 +489,37 @@ __global__ void __launch_bounds__(kSsBlock) ss_decide2_kernel(SsArgs a, int32_t 
     if (threadIdx.x == 0) st->slots += 1;
     if (st->mode != SS_ADVANCE) return;  // uniform
     const long long moved = ss_collect(a.slots, 2, 0, nslots);
+    const long long fdeg = ss_collect(a.slots, 2, 1, nslots);
+    // the summary of the window's F
+    bs_build_summary<kSsBlock>(a.cflag, a.chunks, bs_pick3(a.sum[0], a.sum[1], a.sum[2], st->phase % 3));
+    // least pending distance over the window scan's workgroups
+    __shared__ int32_t red[kSsBlock / 64];
+    int32_t m = INT_MAX;
+    for (int32_t i = threadIdx.x; i < nslots; i += kSsBlock) {
+        const int32_t x = a.slots[kSsMinSlots * kSsMaxGrid + i];
+        m = x < m ? x : m;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int32_t y = __shfl_xor(m, o);
+        m = y < m ? y : m;
+    }
+    if (lane_id() == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
     if (threadIdx.x == 0) {
-        st->far -= moved;
+        for (int i = 0; i < kSsBlock / 64; ++i) m = red[i] < m ? red[i] : m;
+        st->next_min = m;
         if (moved > 0) {
             st->mode = SS_RELAX;
             st->front = (int32_t)moved;
+            st->fdeg = fdeg;
             st->pull = moved > st->pull_min;
             st->windows += 1;
-        } else if (st->far == 0) {
-            st->done = 1;
-        } else {
-            st->hi += st->delta;  // empty window
+        } else if (st->next_min == INT_MAX) {
+            st->done = 1;  // nothing pending
+        } else {  // empty window: the next one starts at the least pending distance
+            st->lo = st->next_min;
+            st->hi = ss_window_end(st->next_min, st->delta);
+            st->next_min = INT_MAX;
         }
     }
 }
@@ -476,7 +539,6 @@ __global__ void ss_reset_kernel(SsArgs a, SsSeed sd) {
             a.bm[0][i] = s;
             a.bm[1][i] = 0;
             a.bm[2][i] = 0;
-            a.far[i] = 0;
         }
         if (i <= a.sum_words) {
             const int32_t ch = (sd.source >> 6) / kBsChunkWords;
@@ -493,12 +555,15 @@ __global__ void ss_reset_kernel(SsArgs a, SsSeed sd) {
         st->steps = 0;
         st->windows = 0;
         st->slots = 0;
+        st->pulls = 0;
         st->hub_count = 0;
-        st->hi = sd.delta;
+        st->lo = 0;
+        st->hi = ss_window_end(0, sd.delta);
         st->delta = sd.delta;
         st->front = 1;
-        st->far = 0;
+        st->next_min = INT_MAX;
         st->pull = 0;
+        st->fdeg = 0;
         st->pull_min = sd.pull_min;
     }
 }
@@ -512,10 +577,11 @@ struct SsWork {
     SsState *st = nullptr;
     uint64_t *bm[3] = {nullptr, nullptr, nullptr};
     uint64_t *sum[3] = {nullptr, nullptr, nullptr};
-    uint64_t *far = nullptr;
+    uint8_t *cflag = nullptr;
+    int32_t chunks = 0;
     int32_t *slots = nullptr;
     int32_t sum_words = 0;
-    int32_t *host = nullptr;  // pinned: done, mode, phase, steps, windows, slots
+    int32_t *host = nullptr;  // pinned: done, mode, phase, steps, windows, slots, pulls
     const int32_t *poff = nullptr, *pnbr = nullptr, *pwt = nullptr;
     int32_t pull_min = 0;
     int32_t wide_chunks = 0;
@@ -533,8 +599,8 @@ static void free_ss(SsWork *w) {
     dfree(w->st);
     for (auto p : w->bm) dfree(p);
     for (auto p : w->sum) dfree(p);
-    dfree(w->far);
     dfree(w->slots);
+    dfree(w->cflag);
     dfree(w->chunk_v);
     dfree(w->chunk_e0);
     if (w->host) (void)hipHostFree(w->host);
@@ -573,8 +639,11 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
     int rc = dalloc(&w->st, 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->bm[i], (size_t)words + 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->sum[i], (size_t)sum_words + 1);
-    if (rc == GI_OK) rc = dalloc(&w->far, (size_t)words + 1);
-    if (rc == GI_OK) rc = dalloc(&w->slots, 9 * (size_t)kSsMaxGrid);
+    if (rc == GI_OK) rc = dalloc(&w->slots, (kSsMinSlots + 1) * (size_t)kSsMaxGrid);
+    w->chunks = (int32_t)ceil_div(words, kBsChunkWords);
+    const size_t flag_bytes = (size_t)bs_flag_bytes(w->chunks);
+    if (rc == GI_OK) rc = dalloc(&w->cflag, flag_bytes);
+    if (rc == GI_OK && hipMemset(w->cflag, 0, flag_bytes) != hipSuccess) rc = fail(GI_ERR_HIP, "flag init");
     if (rc == GI_OK && hipHostMalloc((void **)&w->host, 8 * sizeof(int32_t)) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
     if (rc == GI_OK && hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking) != hipSuccess)
@@ -599,7 +668,7 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
             rc = fail(GI_ERR_HIP, "plan upload");
     }
     if (rc == GI_OK) {
-        ss_zero_slots_kernel<<<64, 256>>>(w->slots, 9 * kSsMaxGrid);
+        ss_zero_slots_kernel<<<64, 256>>>(w->slots, (kSsMinSlots + 1) * kSsMaxGrid);
         if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
             rc = fail(GI_ERR_HIP, "slot init");
     }
@@ -641,8 +710,9 @@ static SsArgs ss_args(gi_graph *g, SsWork *w, int32_t *dist) {
         a.bm[i] = w->bm[i];
         a.sum[i] = w->sum[i];
     }
-    a.far = w->far;
     a.slots = w->slots;
+    a.cflag = w->cflag;
+    a.chunks = w->chunks;
     a.wide_chunks = w->wide_chunks;
     a.chunk_v = w->chunk_v;
     a.chunk_e0 = w->chunk_e0;
@@ -671,7 +741,7 @@ static void launch_slot(const SsArgs &a, hipStream_t s) {
     ss_decide2_kernel<<<1, kSsBlock, 0, s>>>(a, grid);
 }
 
-int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *steps_out,
+int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,
                    hipStream_t s) {
     SsWork *w = nullptr;
     GI_TRY(ensure_ss(g, &w));
@@ -702,12 +772,17 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
             it = w->execs.emplace(key, exec).first;
         }
         GI_HIP(hipGraphLaunch(it->second, s));
-        GI_HIP(hipMemcpyAsync(w->host, w->st, 6 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GI_HIP(hipMemcpyAsync(w->host, w->st, 7 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
         GI_HIP(hipStreamSynchronize(s));
         used += block;
         if (w->host[0]) {  // done
             w->last_slots = w->host[5];
-            if (steps_out) *steps_out = w->host[3];
+            if (counts) {  // steps, pull steps, windows entered, slots
+                counts[0] = w->host[3];
+                counts[1] = w->host[6];
+                counts[2] = w->host[4];
+                counts[3] = w->host[5];
+            }
             return GI_OK;
         }
         block *= 2;
@@ -715,4 +790,72 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
     return fail(GI_ERR_RANGE, "SSSP did not finish");
 }
 
+// Eager slot loop for measurement: the six launches of every slot bracketed
+// by HIP events; the state is read before each slot.
+static int sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_min, int32_t *d_dist,
+                      int32_t max_slots, float *kernel_ms, int64_t *info, int32_t *num_slots, hipStream_t s) {
+    SsWork *w = nullptr;
+    GI_TRY(ensure_ss(g, &w));
+    SsArgs a = ss_args(g, w, d_dist);
+    const int32_t n = g->V > w->sum_words + 1 ? g->V : w->sum_words + 1;
+    int rg = (int)ceil_div(n, 256);
+    ss_reset_kernel<<<rg < 8192 ? rg : 8192, 256, 0, s>>>(a, SsSeed{source, delta, pull_min >= 0 ? pull_min : w->pull_min});
+    GI_LAUNCH_CHECK();
+    hipEvent_t ev[7] = {};
+    int rc = GI_OK;
+    for (int i = 0; i < 7 && rc == GI_OK; ++i)
+        if (hipEventCreate(&ev[i]) != hipSuccess) rc = fail(GI_ERR_HIP, "event create");
+    const int grid = ss_grid(), pgrid = ss_pull_grid();
+    SsState h{};
+    int32_t k = 0;
+    for (; rc == GI_OK && k < max_slots; ++k) {
+        if (hipMemcpyAsync(&h, w->st, offsetof(SsState, hubs), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) {
+            rc = fail(GI_ERR_HIP, "state read");
+            break;
+        }
+        if (h.done) break;
+        info[5 * k + 0] = h.mode;
+        info[5 * k + 1] = h.pull;
+        info[5 * k + 2] = h.front;
+        info[5 * k + 3] = h.hi;
+        info[5 * k + 4] = h.fdeg;
+        (void)hipEventRecord(ev[0], s);
+        ss_step_kernel<<<grid, kSsBlock, 0, s>>>(a);
+        (void)hipEventRecord(ev[1], s);
+        ss_pull_kernel<<<pgrid, kSsBlock, 0, s>>>(a);
+        (void)hipEventRecord(ev[2], s);
+        ss_hub_kernel<<<grid, kSsBlock, 0, s>>>(a);
+        (void)hipEventRecord(ev[3], s);
+        ss_decide_kernel<<<1, kSsBlock, 0, s>>>(a, grid > pgrid ? grid : pgrid);
+        (void)hipEventRecord(ev[4], s);
+        ss_farmove_kernel<<<grid, kSsBlock, 0, s>>>(a);
+        (void)hipEventRecord(ev[5], s);
+        ss_decide2_kernel<<<1, kSsBlock, 0, s>>>(a, grid);
+        (void)hipEventRecord(ev[6], s);
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(ev[6]) != hipSuccess) {
+            rc = fail(GI_ERR_HIP, "slot launch");
+            break;
+        }
+        for (int i = 0; i < 6; ++i) (void)hipEventElapsedTime(&kernel_ms[6 * k + i], ev[i], ev[i + 1]);
+    }
+    for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc == GI_OK && k == max_slots && !h.done) rc = fail(GI_ERR_RANGE, "more than %d slots", max_slots);
+    *num_slots = k;
+    return rc;
+}
+
 }  // namespace gi
+
+extern "C" int gi_sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_min, int32_t *d_dist,
+                             int32_t max_slots, float *kernel_ms, int64_t *info, int32_t *num_slots,
+                             void *stream) {
+    if (!g || !d_dist || !kernel_ms || !info || !num_slots) return gi::fail(GI_ERR_INVALID, "null argument");
+    if (!g->d_off) return gi::fail(GI_ERR_UNSUPPORTED, "pass the base graph, not a blocked view");
+    if (source < 0 || source >= g->V) return gi::fail(GI_ERR_RANGE, "source %d out of range", source);
+    if (delta < 1) return gi::fail(GI_ERR_RANGE, "delta must be positive");
+    if (!g->d_w) return gi::fail(GI_ERR_INVALID, "SSSP needs edge weights");
+    return gi::sssp_trace(g, source, delta, pull_min, d_dist, max_slots, kernel_ms, info, num_slots,
+                          (hipStream_t)stream);
+}

@@ -411,6 +411,16 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
  * set as bitmaps, captured phases, no host synchronisation per phase). */
 int gi_sssp(gi_graph *g, int32_t source, int32_t delta, const gi_schedule *s,
             int32_t *d_dist, gi_run_stats *stats, void *stream);
+/* Measurement hook of the fused SSSP engine (no reference counterpart):
+ * runs the slots of the device-side window loop one at a time, uncaptured,
+ * and times each of the six launches of a slot with HIP events. pull_min:
+ * |F| above which a relax step pulls (< 0: the engine's default).
+ * kernel_ms[max_slots][6]: push step, pull step, hub bin, decide, window
+ * scan, decide2. info[max_slots][5]: the state before the slot (mode: 0
+ * relax / 1 advance, pull flag, |F|, window end hi, out-edges of F).
+ * Synchronises. */
+int gi_sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_min, int32_t *d_dist,
+                  int32_t max_slots, float *kernel_ms, int64_t *info, int32_t *num_slots, void *stream);
 /* Connected components, hook + pointer jumping (apps/cc.cpp:53-78); labels
  * are the minimum vertex id of each component. Default schedule (s == NULL):
  * the app's PUSH EDGE_ONLY (apps/cc.cpp:84-86). With kernel_fusion ENABLED

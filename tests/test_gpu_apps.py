@@ -58,6 +58,67 @@ def test_sssp_fused_grid_and_unreachable(gi):
     assert np.array_equal(d2, O.sssp(h2, 0)) and d2[3] == np.iinfo(np.int32).max
 
 
+def test_sssp_fused_skips_empty_windows(gi):
+    # distances 0, 3, 503, 505 with delta 4: after [0, 4) the next window
+    # must start at 503, the least pending distance
+    off = np.array([0, 1, 3, 5, 6], np.int32)
+    nb = np.array([1, 0, 2, 1, 3, 2], np.int32)
+    w = np.array([3, 3, 500, 500, 2, 2], np.int32)
+    g = gi.Graph.from_csr(off, nb, w)
+    assert O.sssp(host_csr(g, True), 0).tolist() == [0, 3, 503, 505]
+    for _ in range(2):
+        d, _ = gi.sssp(g, 0, 4, _fused_sssp(gi))
+        assert d.tolist() == [0, 3, 503, 505]
+    d, _, info = gi.sssp_trace(g, 0, 4)
+    assert d.tolist() == [0, 3, 503, 505]
+    ends = set(info[:, 3].tolist())
+    assert 507 in ends and not any(8 < h < 503 for h in ends)
+
+
+def test_sssp_trace_matches_and_counts(gi):
+    g = gi.Graph.rmat(13, 16, SEED, True).set_weights(SEED, 1, 255)
+    host = host_csr(g, True)
+    src = int(O.pick_sources(host, 1, 4)[0])
+    want = O.sssp(host, src)
+    _, st = gi.sssp(g, src, 32, _fused_sssp(gi))
+    for pull_min in (-1, 0, 1 << 30):  # default rule, pull whenever F is non-empty, push only
+        d, ms, info = gi.sssp_trace(g, src, 32, pull_min)
+        assert np.array_equal(d, want), pull_min
+        assert (ms >= 0).all() and len(ms) == len(info)
+        relax = info[:, 0] == 0
+        assert info[relax, 2].min() > 0  # a relax slot has a non-empty F
+        if pull_min == 0:
+            assert info[relax, 1][1:].all()
+        if pull_min == 1 << 30:
+            assert not info[relax, 1].any()
+    assert st["iterations"] == st["push_iterations"] + st["pull_iterations"]
+
+
+def test_set_weights_after_fused_runs_directed(gi):
+    # new weights drop the transposed graph; the fused engines must not keep
+    # pointers into it
+    g = gi.Graph.rmat(12, 8, 4, False).set_weights(1, 1, 100)
+    fs = _fused_sssp(gi)
+    fp = gi.Schedule(direction=gi.PULL, load_balancing=gi.LB_VERTEX_BASED, kernel_fusion=1)
+    s1 = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWCE, kernel_fusion=1)
+    s2 = gi.Schedule(direction=gi.PULL, load_balancing=gi.LB_VERTEX_BASED, kernel_fusion=1,
+                     pull_frontier_rep=gi.PULL_BITMAP, frontier_creation=gi.FC_BITMAP)
+    hy = gi.HybridSchedule(s1, s2, 0.02, gi.SELECT_HEURISTIC)
+    host = host_csr(g, True)
+    src = int(O.pick_sources(host, 1, 9)[0])
+    for seed in (1, 2):
+        if seed == 2:
+            g.set_weights(seed, 1, 100)
+            host = host_csr(g, True)
+        d, _ = gi.sssp(g, src, 16, fs)
+        assert np.array_equal(d, O.sssp(host, src)), seed
+        r, _ = gi.pagerank(g, 10, schedule=fp)
+        want = O.pagerank(host, 10)
+        assert np.abs(r - want).sum() / np.abs(want).sum() <= 1e-5
+        depth, _, _ = gi.bfs(g, src, hy)
+        assert np.array_equal(depth, O.bfs_depth(host, src))
+
+
 def test_sssp_fused_hub_bin(gi):
     # a centre wider than the grid-bin threshold, with a tail of long edges
     n = 9003

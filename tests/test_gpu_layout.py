@@ -1,6 +1,6 @@
 """Degree-ordered layout: fused BFS (single- and multi-source, parents, the
-learned selector) and fused CC run on the renumbered copy and must give the
-oracle's results in the original vertex ids."""
+learned selector), CC, PageRank and SSSP run on the renumbered copy and must
+give the oracle's results in the original vertex ids."""
 import numpy as np
 import pytest
 
@@ -103,3 +103,25 @@ def test_layout_pagerank(gi):
     rel = np.abs(r - want).sum() / np.abs(want).sum()
     assert rel <= 1e-5
     assert np.abs(r - native).sum() / np.abs(native).sum() <= 1e-6  # only the summation order differs
+
+
+def test_layout_sssp(gi):
+    g = gi.Graph.rmat(14, 16, SEED, True)
+    g.set_layout(gi.LAYOUT_DEGREE)
+    g.set_weights(SEED, 1, 255)  # after the layout: the renumbered copy takes the weights
+    sched = gi.Schedule(direction=0, load_balancing=1, kernel_fusion=1)
+    off, nb, w = g.read_csr(weights=True)
+    host = O.CSR(off, nb, w)
+    srcs = [int(x) for x in O.pick_sources(host, 2, 7)]
+    for src in srcs:
+        for delta in (8, 64):
+            d, _ = gi.sssp(g, src, delta, sched)
+            assert np.array_equal(d, O.sssp(host, src)), (src, delta)
+    g.set_weights(SEED + 1, 1, 9)  # new weights reach the renumbered copy too
+    off, nb, w = g.read_csr(weights=True)
+    host = O.CSR(off, nb, w)
+    want = O.sssp(host, srcs[0])
+    d, _ = gi.sssp(g, srcs[0], 4, sched)
+    assert np.array_equal(d, want)
+    d, _, _ = gi.sssp_trace(g, srcs[0], 4)
+    assert np.array_equal(d, want)
