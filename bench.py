@@ -131,22 +131,46 @@ def bench_apps(gi, g, sources, edges_of, log) -> dict:
     from easy_graphit_amd import autotune
     out = {}
     V, E = g.num_vertices, g.num_edges
-    # SSSP: RMAT-24 with int32 weights U[1,255], delta-stepping (delta 32)
+    # SSSP: RMAT-24 with int32 weights U[1,255], delta-stepping; the delta is
+    # the fastest of autotune.DELTAS on the first source
     g.set_weights(SEED, 1, 255)
     fs = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWC, kernel_fusion=1)
     srcs = [int(s) for s in sources[:4]]
-    t, edges, vr = 0.0, 0, 0
+    tuned = autotune.tune_delta(g, srcs[0], log=log)
+    delta = tuned["delta"]
+    t, edges, vr, steps = 0.0, 0, 0, 0
     for s in srcs:
-        gi.sssp(g, s, 32, fs)  # warm-up
-        dist, st = gi.sssp(g, s, 32, fs)
+        gi.sssp(g, s, delta, fs)  # warm-up
+        dist, st = gi.sssp(g, s, delta, fs)
         t += st["ms"] * 1e-3
         edges += edges_of[s]
         vr += int((dist < np.iinfo(np.int32).max).sum())
+        steps += st["iterations"]
     out["sssp"] = {"gteps": round(edges / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3 / len(srcs), 3),
-                   "roofline_frac": alg_frac(16 * vr + 12 * edges, t),
+                   "roofline_frac": alg_frac(16 * vr + 12 * edges, t), "relax_steps": steps,
                    "alg_bytes": "16 V_reached + 12 E_reached", "config": "RMAT-24 sym, w U[1,255], "
-                   f"delta 32, {len(srcs)} sources"}
+                   f"delta {delta} (fastest of {list(autotune.DELTAS)} on the first source), "
+                   f"{len(srcs)} sources",
+                   "delta_ms": {r["delta"]: round(r["ms"], 3) for r in tuned["table"]}}
     log(f"sssp: {out['sssp']}")
+    # SSSP on the grid workload: 2048 x 2048 4-neighbour grid plus 1% random
+    # long edges, w U[1,1000], source at the corner, delta 64
+    gr = gi.Graph.grid(2048, 2 * 2048 * 2047 // 100, SEED).set_weights(SEED, 1, 1000)
+    gi.sssp(gr, 0, 64, fs)
+    ts = []
+    for _ in range(3):
+        dist, st = gi.sssp(gr, 0, 64, fs)
+        ts.append(st["ms"] * 1e-3)
+    t = float(np.median(ts))
+    ge = gr.num_edges
+    tuned_g = autotune.tune_delta(gr, 0, deltas=(64, 256, 1024), reps=1)
+    out["sssp_grid"] = {"gteps": round(ge / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3),
+                        "relax_steps": st["iterations"],
+                        "roofline_frac": alg_frac(16 * gr.num_vertices + 12 * ge, t),
+                        "config": "2048x2048 grid + 1% random edges, w U[1,1000], source 0, delta 64",
+                        "delta_ms": {r["delta"]: round(r["ms"], 3) for r in tuned_g["table"]}}
+    log(f"sssp_grid: {out['sssp_grid']}")
+    gr.destroy()
     # CC: RMAT-24, hook + shortcutting rounds
     fc = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_EDGE_ONLY, kernel_fusion=1)
     gi.cc(g, fc)

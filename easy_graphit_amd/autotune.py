@@ -1,16 +1,22 @@
-"""Autotuner for the fused BFS engine's bins and launch shape.
+"""Autotuners for the fused engines.
 
-The fused engine walks a push frontier vertex in one lane when its degree is
-below `lane_max`, with a whole wavefront up to `hub_min`, and spreads wider
-vertices over the whole grid; its persistent step kernels run `blocks_per_cu`
-workgroups of 256 threads per CU (gi_bfs_set_bins). The best values depend on
-the degree distribution of the workload: many small uniform graphs packed in
-one CSR want different bins than one RMAT graph with hubs.
+Bins and launch shape of the fused BFS engine: it walks a push frontier
+vertex in one lane when its degree is below `lane_max`, with a whole
+wavefront up to `hub_min`, and spreads wider vertices over the whole grid;
+its persistent step kernels run `blocks_per_cu` workgroups of 256 threads per
+CU (gi_bfs_set_bins). The best values depend on the degree distribution of
+the workload: many small uniform graphs packed in one CSR want different bins
+than one RMAT graph with hubs. tune_bins() times the real workload (a
+multi-source fused BFS over the packed graphs, one source per graph) under
+every candidate setting, checks that every setting gives the same depths,
+keeps the fastest and leaves it applied to the graph.
 
-tune_bins() times the real workload (a multi-source fused BFS over the
-packed graphs, one source per graph) under every candidate setting, checks
-that every setting gives the same depths, keeps the fastest and leaves it
-applied to the graph.
+Delta of the fused SSSP engine: a small delta settles vertices in many thin
+windows (more launches, fewer repeated relaxations), a large one in few wide
+windows (each vertex relaxed more often). The delta is a run-time argument
+of the reference app (apps/sssp.cpp:34); tune_delta() times the candidates
+on one source, checks that all give the same distances and returns the
+fastest.
 """
 from __future__ import annotations
 
@@ -87,3 +93,28 @@ def tune_bins(g: gi.Graph, sources, lane_max=LANE_MAX, hub_min=HUB_MIN,
     out["default_seconds"] = default[0]["seconds"] if default else None
     out["table"] = table
     return out
+
+
+DELTAS = (4, 8, 16, 32, 64, 128, 256)
+
+
+def tune_delta(g: gi.Graph, source: int, deltas=DELTAS, reps: int = 2, log=None) -> dict:
+    """Fastest delta of a fused SSSP from `source`: {"delta", "ms", "table"}."""
+    sched = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWC, kernel_fusion=1)
+    ref = None
+    table = []
+    for d in deltas:
+        gi.sssp(g, source, d, sched)  # warm-up (captures the slots)
+        best = float("inf")
+        for _ in range(reps):
+            dist, st = gi.sssp(g, source, d, sched)
+            best = min(best, st["ms"])
+        if ref is None:
+            ref = dist
+        elif not np.array_equal(dist, ref):
+            raise AssertionError(f"delta {d} changed the distances")
+        table.append({"delta": d, "ms": best})
+        if log:
+            log(f"  delta {d}: {best:.3f} ms")
+    top = min(table, key=lambda r: r["ms"])
+    return {"delta": top["delta"], "ms": top["ms"], "table": table}

@@ -119,6 +119,17 @@ def test_set_weights_after_fused_runs_directed(gi):
         assert np.array_equal(depth, O.bfs_depth(host, src))
 
 
+def test_autotune_delta_keeps_distances(gi):
+    from easy_graphit_amd import autotune
+    g = gi.Graph.rmat(12, 16, SEED, True).set_weights(SEED, 1, 255)
+    host = host_csr(g, True)
+    src = int(O.pick_sources(host, 1, 2)[0])
+    best = autotune.tune_delta(g, src, deltas=(4, 32, 256), reps=1)
+    assert len(best["table"]) == 3 and best["ms"] == min(r["ms"] for r in best["table"])
+    d, _ = gi.sssp(g, src, best["delta"], _fused_sssp(gi))
+    assert np.array_equal(d, O.sssp(host, src))
+
+
 def test_sssp_fused_hub_bin(gi):
     # a centre wider than the grid-bin threshold, with a tail of long edges
     n = 9003
