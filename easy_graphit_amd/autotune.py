@@ -28,7 +28,7 @@ import numpy as np
 import easy_graphit_amd as gi
 
 LANE_MAX = (16, 32, 64, 128)
-HUB_MIN = (1024, 4096, 16384)
+HUB_MIN = (256, 512, 2048, 8192)
 BLOCKS_PER_CU = (4, 8)
 PUSH_LB = (gi.LB_TWC, gi.LB_TWCE, gi.LB_CM)
 
@@ -61,7 +61,7 @@ def tune_bins(g: gi.Graph, sources, lane_max=LANE_MAX, hub_min=HUB_MIN,
 
     Returns {"lane_max", "hub_min", "blocks_per_cu", "push_lb", "seconds",
     "default_seconds", "table"}; "default_seconds" is the engine's default
-    setting (64, 4096, 8, TWC) on the same workload."""
+    setting (64, 512, 8, TWC) on the same workload."""
     depth = gi.DeviceArray(g.num_vertices, np.int32)
     ref = None
     table = []
@@ -71,7 +71,7 @@ def tune_bins(g: gi.Graph, sources, lane_max=LANE_MAX, hub_min=HUB_MIN,
             cands += [(lm, hm, bpc, lb) for lm, hm, bpc in itertools.product(lane_max, hub_min, blocks_per_cu)
                       if hm > lm]
         else:                # TWCE and CM bin inside the workgroup: launch shape only
-            cands += [(64, 4096, bpc, lb) for bpc in blocks_per_cu]
+            cands += [(64, 512, bpc, lb) for bpc in blocks_per_cu]
     for lm, hm, bpc, lb in cands:
         g.set_bins(lm, hm, bpc)
         sched = schedule(lb, threshold)
@@ -87,7 +87,7 @@ def tune_bins(g: gi.Graph, sources, lane_max=LANE_MAX, hub_min=HUB_MIN,
             log(f"  bins lane<{lm} hub>={hm} {bpc}/CU {gi.LB_NAMES[lb]}: {t * 1e3:.3f} ms")
     best = min(table, key=lambda r: r["seconds"])
     default = [r for r in table if (r["lane_max"], r["hub_min"], r["blocks_per_cu"], r["push_lb"])
-               == (64, 4096, 8, gi.LB_TWC)]
+               == (64, 512, 8, gi.LB_TWC)]
     g.set_bins(best["lane_max"], best["hub_min"], best["blocks_per_cu"])
     out = dict(best)
     out["default_seconds"] = default[0]["seconds"] if default else None

@@ -23,7 +23,10 @@ constexpr int kBfsMaxLevels = 4096;
 constexpr int kSelInputs = 16;
 constexpr int kSelHidden = 64;
 constexpr int kSelMaxOut = 8;
-constexpr int32_t kHubDegree = 4096;  // push vertices at least this wide go to the grid bin
+// Push vertices at least this wide go to the grid-wide bin (TWC). Low
+// enough that a small frontier of long lists (the source's neighbours) is
+// spread over the whole GPU instead of one wavefront per list.
+constexpr int32_t kHubDegree = 512;
 constexpr int32_t kMaxHubs = 1024;
 constexpr int32_t kMaxGrid = 8192;          // partial-sum slots per kernel
 constexpr int32_t kMaxSumWords = 256;       // summary words (V up to 2^26)

@@ -1318,6 +1318,8 @@ extern "C" {
 
 int gi_bfs_set_bins(gi_graph *g, int32_t lane_max, int32_t hub_min, int32_t blocks_per_cu) {
     if (!g) return fail(GI_ERR_INVALID, "null graph");
+    // the degree-ordered view is where the fused engine runs when there is one
+    if (g->layout) GI_TRY(fused_set_bins(layout_view(g), lane_max, hub_min, blocks_per_cu));
     return fused_set_bins(g, lane_max, hub_min, blocks_per_cu);
 }
 

@@ -162,7 +162,7 @@ def test_packed_small_graphs_bfs(gi):
     assert np.array_equal(off, host.offsets) and np.array_equal(nb, host.nbrs)
     srcs = offs[:-1] + (np.arange(64) * 7919) % np.diff(offs)
     h = gi.HybridSchedule(fused(gi, 4), fused(gi, 5), 0.02, gi.SELECT_HEURISTIC)
-    for bins in [(64, 4096, 8), (16, 1024, 4), (128, 16384, 8)]:
+    for bins in [(64, 512, 8), (64, 4096, 8), (16, 1024, 4), (128, 16384, 8), (16, 64, 8)]:
         g.set_bins(*bins)
         depth, _ = gi.bfs_multi(g, srcs, h)
         assert np.array_equal(depth, O.bfs_multi_depth(host, srcs)), bins

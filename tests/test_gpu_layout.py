@@ -125,3 +125,16 @@ def test_layout_sssp(gi):
     assert np.array_equal(d, want)
     d, _, _ = gi.sssp_trace(g, srcs[0], 4)
     assert np.array_equal(d, want)
+
+
+def test_layout_bins_reach_the_view(gi, rmat):
+    # bins set on the graph apply where the fused engine runs (the view);
+    # a tiny grid-bin threshold sends most frontier vertices to the grid bin
+    g, host = rmat
+    srcs = O.pick_sources(host, 3, 13)
+    for bins in [(16, 64, 4), (64, 512, 8)]:
+        g.set_bins(*bins)
+        for src in srcs:
+            depth, parent, _ = gi.bfs(g, int(src), fused(gi, 4), want_parent=True)
+            assert np.array_equal(depth, O.bfs_depth(host, int(src))), bins
+            assert O.bfs_parent_violations(host, int(src), depth, parent) == 0
