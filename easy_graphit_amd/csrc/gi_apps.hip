@@ -160,12 +160,12 @@ static int bfs_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int
             rparent = d_parent ? layout_tmp(g, 1) : nullptr;
         }
         int32_t levels = 0;
+        FusedEpilogue back = [&](hipStream_t st) {  // queued behind the level loop
+            GI_TRY(layout_depth_back(g, rdepth, source, d_depth, st));
+            return d_parent ? layout_map_back(g, rparent, d_parent, true, st) : GI_OK;
+        };
         int rc = bfs_fused_run(rg, rsrc, h, sel, rdepth, rparent, s, want_trace ? &ft : nullptr,
-                               trace != nullptr, &levels);
-        if (rc == GI_OK && g->layout) {
-            rc = layout_map_back(g, rdepth, d_depth, false, s);
-            if (rc == GI_OK && d_parent) rc = layout_map_back(g, rparent, d_parent, true, s);
-        }
+                               trace != nullptr, &levels, g->layout ? &back : nullptr);
         float ms = 0.f;
         if (stats && rc == GI_OK) {
             GI_HIP(hipEventRecord(t1, s));

@@ -1,6 +1,7 @@
 // Shared definitions of the fused BFS engine and the schedule selector.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 #include "gi_internal.h"
@@ -108,16 +109,21 @@ struct FusedTrace {
 };
 
 int32_t fused_mode_of(const gi_schedule &s);
+// Work queued behind the level loop (e.g. mapping results back to original
+// ids). It is enqueued before the host waits for the loop's done flag, so it
+// overlaps that wait; if the loop needs another block of levels it is
+// enqueued again after it.
+using FusedEpilogue = std::function<int(hipStream_t)>;
 int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, SelectorView sel,
                   int32_t *d_depth, int32_t *d_parent, hipStream_t s, FusedTrace *trace,
-                  bool timed, int32_t *levels_out);
+                  bool timed, int32_t *levels_out, const FusedEpilogue *epilogue = nullptr);
 void release_fused(const gi_graph *g);
 int fused_reached(gi_graph *g, const int32_t *depth, int32_t *out, hipStream_t s);
 int fused_set_bins(gi_graph *g, int32_t lane_max, int32_t hub_min, int32_t blocks_per_cu);
 int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources, int32_t single,
                         const gi_hybrid_schedule *h, SelectorView sel, int32_t *d_depth,
                         int32_t *d_parent, hipStream_t s, FusedTrace *trace, bool timed,
-                        int32_t *levels_out);
+                        int32_t *levels_out, const FusedEpilogue *epilogue = nullptr);
 // The process-wide selector model (null weights when none is loaded).
 SelectorView selector_view();
 

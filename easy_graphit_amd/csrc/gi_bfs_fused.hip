@@ -920,7 +920,8 @@ struct FusedWork {
     bool sel_valid = false;
     int32_t *d_src = nullptr;        // sources of a multi-source run
     int32_t src_cap = 0;
-    int32_t *host_flag = nullptr;    // pinned copy of `done`
+    int32_t *host_flag = nullptr;    // pinned copy of {done, level, levels, mode}
+    hipEvent_t flag_ev = nullptr;    // recorded after the copy into host_flag
     int32_t sum_words = 0;
     int32_t lane_max = 64;           // bin thresholds (gi_bfs_set_bins)
     int32_t hub_min = kHubDegree;
@@ -953,6 +954,7 @@ static void free_work(FusedWork *w) {
     dfree(w->d_sel);
     dfree(w->d_src);
     if (w->host_flag) (void)hipHostFree(w->host_flag);
+    if (w->flag_ev) (void)hipEventDestroy(w->flag_ev);
     delete w;
 }
 
@@ -994,6 +996,8 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
     if (rc == GI_OK && hipStreamCreateWithFlags(&w->cap_stream, hipStreamNonBlocking) != hipSuccess)
         rc = fail(GI_ERR_HIP, "stream create failed");
+    if (rc == GI_OK && hipEventCreateWithFlags(&w->flag_ev, hipEventDisableTiming) != hipSuccess)
+        rc = fail(GI_ERR_HIP, "event create failed");
     if (rc == GI_OK) rc = fill_i32(w->pcount, 2 * kMaxGrid, 0, 0);
     if (rc == GI_OK) rc = fill_u64((uint64_t *)w->pfdeg, 2 * kMaxGrid, 0, 0);
     if (rc == GI_OK) {
@@ -1189,9 +1193,12 @@ static int level_graph(gi_graph *g, FusedWork *w, int32_t levels, hipGraphExec_t
     return GI_OK;
 }
 
-static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trace, bool timed) {
+static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trace, bool timed,
+                      const FusedEpilogue *epilogue, int32_t *levels) {
     StepArgs a = step_args(g, w);
     int32_t *done_flag = &w->st->done;
+    static_assert(offsetof(BfsState, levels) == offsetof(BfsState, done) + 2 * sizeof(int32_t),
+                  "done, level, levels are contiguous");
     if (trace && timed) {
         // per-level launches with events around each step; the host reads
         // {done, level, levels, mode} after every decide
@@ -1200,7 +1207,10 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
         for (int32_t level = 0; level < kBfsMaxLevels; ++level) {
             GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
             GI_HIP(hipStreamSynchronize(s));
-            if (w->host_flag[0]) return GI_OK;
+            if (w->host_flag[0]) {
+                *levels = w->host_flag[2];
+                return epilogue ? (*epilogue)(s) : GI_OK;
+            }
             hipEvent_t e0, e1;
             GI_HIP(hipEventCreate(&e0));
             GI_HIP(hipEventCreate(&e1));
@@ -1212,8 +1222,10 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
         return fail(GI_ERR_RANGE, "BFS deeper than %d levels", kBfsMaxLevels);
     }
     // Graph mode: replay a captured block of levels sized by the previous run
-    // on this graph, check `done` once per replay, continue with longer
-    // blocks for deep graphs (levels past the end are no-op launches).
+    // on this graph, read {done, level, levels} once per replay (the host
+    // waits for that copy only, while the epilogue is already queued),
+    // continue with longer blocks for deep graphs (levels past the end are
+    // no-op launches).
     int32_t block = w->last_levels > 0 ? w->last_levels : 8;
     if (block > 48) block = 48;
     int32_t level = 0;
@@ -1222,9 +1234,14 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
         GI_TRY(level_graph(g, w, block, &exec));
         GI_HIP(hipGraphLaunch(exec, s));
         level += block;
-        GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        GI_HIP(hipStreamSynchronize(s));
-        if (*w->host_flag) return GI_OK;
+        GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        GI_HIP(hipEventRecord(w->flag_ev, s));
+        if (epilogue) GI_TRY((*epilogue)(s));
+        GI_HIP(hipEventSynchronize(w->flag_ev));
+        if (w->host_flag[0]) {
+            *levels = w->host_flag[2];
+            return GI_OK;
+        }
         block = 48;
     }
     return fail(GI_ERR_RANGE, "BFS deeper than %d levels", kBfsMaxLevels);
@@ -1233,7 +1250,7 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
 int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources, int32_t single,
                         const gi_hybrid_schedule *h, SelectorView sel, int32_t *d_depth,
                         int32_t *d_parent, hipStream_t s, FusedTrace *trace, bool timed,
-                        int32_t *levels_out) {
+                        int32_t *levels_out, const FusedEpilogue *epilogue) {
     FusedWork *w = nullptr;
     GI_TRY(ensure_fused(g, &w));
     sel.mode_s1 = fused_mode_of(h->s1);
@@ -1242,10 +1259,8 @@ int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources
     if (sel.mode != GI_SELECT_LEARNED) sel.mode = h->selector;
     GI_TRY(seed_run(g, w, h_sources, n_sources, single, d_depth, d_parent, sel,
                     sel.mode == GI_SELECT_LEARNED, s));
-    GI_TRY(run_levels(g, w, s, trace, timed));
     int32_t levels = 0;
-    GI_HIP(hipMemcpyAsync(&levels, &w->st->levels, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-    GI_HIP(hipStreamSynchronize(s));
+    GI_TRY(run_levels(g, w, s, trace, timed, epilogue, &levels));
     w->last_levels = levels;
     if (levels_out) *levels_out = levels;
     if (trace) {
@@ -1260,9 +1275,9 @@ int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources
 
 int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, SelectorView sel,
                   int32_t *d_depth, int32_t *d_parent, hipStream_t s, FusedTrace *trace,
-                  bool timed, int32_t *levels_out) {
+                  bool timed, int32_t *levels_out, const FusedEpilogue *epilogue) {
     return bfs_fused_run_multi(g, nullptr, 0, source, h, sel, d_depth, d_parent, s, trace, timed,
-                               levels_out);
+                               levels_out, epilogue);
 }
 
 int fused_reached(gi_graph *g, const int32_t *depth, int32_t *out, hipStream_t s) {

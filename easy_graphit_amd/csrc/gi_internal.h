@@ -292,5 +292,9 @@ gi_graph *layout_view(const gi_graph *g);
 int32_t layout_map_source(const gi_graph *g, int32_t v);
 int32_t *layout_tmp(const gi_graph *g, int i);
 int layout_map_back(const gi_graph *g, const int32_t *in_new, int32_t *out_old, bool ids, hipStream_t s);
+// BFS depths of a single-source run back to original ids; on a symmetric
+// graph the edgeless tail of the view is filled without reading depth_new.
+int layout_depth_back(const gi_graph *g, const int32_t *depth_new, int32_t source_old, int32_t *out_old,
+                      hipStream_t s);
 int layout_labels_back(const gi_graph *g, const int32_t *labels_new, int32_t *out_old, hipStream_t s);
 }  // namespace gi

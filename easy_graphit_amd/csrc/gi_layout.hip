@@ -69,6 +69,30 @@ __global__ void layout_gather_ids_kernel(const int32_t *in_new, const int32_t *n
     }
 }
 
+// BFS depths of a symmetric graph: a vertex past iso_begin in the view has
+// no edges, so its depth is -1 unless it is the source (src_old); its random
+// read of in_new is skipped.
+__global__ void layout_gather_depth_kernel(const int32_t *in_new, const int32_t *new_of_old, int32_t V,
+                                           int32_t iso_begin, int32_t src_old, int32_t *out_old) {
+    for (int32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < V; o += gridDim.x * blockDim.x) {
+        const int32_t nw = new_of_old[o];
+        out_old[o] = nw < iso_begin ? in_new[nw] : (o == src_old ? 0 : -1);
+    }
+}
+
+// First vertex of the view with no edges: degrees decrease along the view,
+// so it is the first v with off[v] == E (V when every vertex has edges).
+__global__ void layout_iso_kernel(const int32_t *off, int32_t V, int32_t *out) {
+    const int64_t E = off[V];
+    int32_t lo = 0, hi = V;  // answer in [lo, hi]
+    while (lo < hi) {
+        const int32_t mid = lo + (hi - lo) / 2;
+        if ((int64_t)off[mid] >= E && off[mid + 1] == off[mid]) hi = mid;
+        else lo = mid + 1;
+    }
+    *out = lo;
+}
+
 __global__ void layout_fill_kernel(int32_t *a, int32_t V, int32_t x) {
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) a[i] = x;
 }
@@ -115,6 +139,7 @@ struct GraphLayout {
     int32_t *old_of_new = nullptr;
     std::vector<int32_t> h_new_of_old;  // for mapping host-side sources
     int32_t *tmp[2] = {nullptr, nullptr};  // results in view order
+    int32_t iso_begin = 0;                 // first view vertex without edges (symmetric graphs)
 };
 
 void release_layout(gi_graph *g) {
@@ -220,6 +245,12 @@ int build_layout(gi_graph *g) {
         if (rc == GI_OK && hipGetLastError() != hipSuccess) rc = fail(GI_ERR_HIP, "layout kernels");
         if (rc == GI_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(GI_ERR_HIP, "layout sync");
     }
+    if (rc == GI_OK) {  // the edgeless tail of the view (only used for symmetric graphs)
+        layout_iso_kernel<<<1, 1, 0, s>>>(t->d_off, V, L->tmp[1]);
+        if (hipGetLastError() != hipSuccess ||
+            hipMemcpy(&L->iso_begin, L->tmp[1], sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(GI_ERR_HIP, "layout tail");
+    }
     if (rc == GI_OK) {
         L->h_new_of_old.resize((size_t)V);
         if (hipMemcpy(L->h_new_of_old.data(), L->new_of_old, sizeof(int32_t) * V, hipMemcpyDeviceToHost) !=
@@ -252,6 +283,16 @@ int layout_map_back(const gi_graph *g, const int32_t *in_new, int32_t *out_old, 
         layout_gather_ids_kernel<<<grid_of(g->V), 256, 0, s>>>(in_new, L->new_of_old, L->old_of_new, g->V, out_old);
     else
         layout_gather_kernel<<<grid_of(g->V), 256, 0, s>>>(in_new, L->new_of_old, g->V, out_old);
+    GI_LAUNCH_CHECK();
+    return GI_OK;
+}
+
+int layout_depth_back(const gi_graph *g, const int32_t *depth_new, int32_t source_old, int32_t *out_old,
+                      hipStream_t s) {
+    const GraphLayout *L = g->layout;
+    if (g->directed) return layout_map_back(g, depth_new, out_old, false, s);  // a sink may still be reached
+    layout_gather_depth_kernel<<<grid_of(g->V), 256, 0, s>>>(depth_new, L->new_of_old, g->V, L->iso_begin,
+                                                             source_old, out_old);
     GI_LAUNCH_CHECK();
     return GI_OK;
 }

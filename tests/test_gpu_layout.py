@@ -138,3 +138,14 @@ def test_layout_bins_reach_the_view(gi, rmat):
             depth, parent, _ = gi.bfs(g, int(src), fused(gi, 4), want_parent=True)
             assert np.array_equal(depth, O.bfs_depth(host, int(src))), bins
             assert O.bfs_parent_violations(host, int(src), depth, parent) == 0
+
+
+def test_layout_bfs_isolated_tail(gi, rmat):
+    # edgeless vertices sit at the end of the view; their depths come back
+    # as -1 without being read, except for a source among them
+    g, host = rmat
+    iso = np.flatnonzero(np.diff(host.offsets) == 0)
+    assert len(iso) > 0
+    for src in (int(iso[0]), int(iso[-1]), int(O.pick_sources(host, 1, 17)[0])):
+        depth, _, _ = gi.bfs(g, src, fused(gi))
+        assert np.array_equal(depth, O.bfs_depth(host, src)), src
