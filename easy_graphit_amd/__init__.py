@@ -97,6 +97,16 @@ class LevelRecord(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class HostCSR(C.Structure):
+    _fields_ = [("num_vertices", C.c_int32), ("num_edges", C.c_int64), ("directed", C.c_int32),
+                ("offsets", C.POINTER(C.c_int32)), ("nbrs", C.POINTER(C.c_int32)),
+                ("weights", C.POINTER(C.c_int32))]
+
+
+LOAD_SYMMETRIZE = 1
+LOAD_CACHE = 2
+
+
 class RunStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("push_iterations", C.c_int32),
                 ("pull_iterations", C.c_int32), ("reached", C.c_int32),
@@ -148,6 +158,10 @@ def lib():
         "gi_schedule_init": (None, [C.POINTER(Schedule)]),
         "gi_graph_from_csr": (C.c_int, [i32, i64, vp, vp, vp, i32, pp]),
         "gi_graph_from_edges": (C.c_int, [i32, i64, vp, vp, i32, pp]),
+        "gi_read_graph_file": (C.c_int, [C.c_char_p, i32, C.POINTER(HostCSR)]),
+        "gi_host_csr_free": (None, [C.POINTER(HostCSR)]),
+        "gi_write_graphit_bin": (C.c_int, [C.POINTER(HostCSR), C.c_char_p]),
+        "gi_graph_load": (C.c_int, [C.c_char_p, i32, pp]),
         "gi_graph_rmat": (C.c_int, [i32, i32, u64, i32, pp]),
         "gi_graph_uniform": (C.c_int, [i32, i64, u64, i32, pp]),
         "gi_graph_grid": (C.c_int, [i32, i64, u64, pp]),
@@ -350,6 +364,15 @@ class Graph:
         check(lib().gi_graph_from_csr(len(off) - 1, len(nb), off.ctypes.data, nb.ctypes.data,
                                       None if w is None else w.ctypes.data, int(directed),
                                       C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def load(cls, path: str, symmetrize: bool = False, cache: bool = False):
+        """A graph file (.el .wel .gr .graph .mtx .sg .wsg .graphit_bin) on
+        the device; `cache` reads / writes the .graphit_bin cache beside it."""
+        h = C.c_void_p()
+        flags = (LOAD_SYMMETRIZE if symmetrize else 0) | (LOAD_CACHE if cache else 0)
+        check(lib().gi_graph_load(os.fsencode(path), flags, C.byref(h)))
         return cls(h.value)
 
     @classmethod
@@ -675,6 +698,35 @@ class PriorityQueue:
                 lib().gi_pq_destroy(self.h)
         except Exception:
             pass
+
+
+def read_graph_file(path: str, symmetrize: bool = False):
+    """Host CSR of a graph file, read without a device:
+    (offsets, nbrs, weights or None, directed)."""
+    c = HostCSR()
+    check(lib().gi_read_graph_file(os.fsencode(path), LOAD_SYMMETRIZE if symmetrize else 0,
+                                   C.byref(c)))
+    try:
+        n, m = c.num_vertices, c.num_edges
+        off = np.ctypeslib.as_array(c.offsets, (n + 1,)).copy()
+        nb = np.ctypeslib.as_array(c.nbrs, (max(m, 1),))[:m].copy()
+        w = None
+        if c.weights:
+            w = np.ctypeslib.as_array(c.weights, (max(m, 1),))[:m].copy()
+        return off, nb, w, bool(c.directed)
+    finally:
+        lib().gi_host_csr_free(C.byref(c))
+
+
+def write_graphit_bin(path: str, offsets, nbrs, weights=None) -> None:
+    """The reference's .graphit_bin cache of a host CSR."""
+    off = np.ascontiguousarray(offsets, np.int32)
+    nb = np.ascontiguousarray(nbrs, np.int32)
+    w = None if weights is None else np.ascontiguousarray(weights, np.int32)
+    i32p = C.POINTER(C.c_int32)
+    c = HostCSR(len(off) - 1, len(nb), 1, off.ctypes.data_as(i32p), nb.ctypes.data_as(i32p),
+                w.ctypes.data_as(i32p) if w is not None else None)
+    check(lib().gi_write_graphit_bin(C.byref(c), os.fsencode(path)))
 
 
 def sssp(g: Graph, source: int, delta: int, schedule: Schedule | None = None, stream=None):

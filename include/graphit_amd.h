@@ -159,6 +159,40 @@ int gi_graph_transpose(gi_graph *g, gi_graph **out);
 int gi_graph_blocked(gi_graph *g, int32_t block_size, gi_graph **out);
 int gi_graph_destroy(gi_graph *g);
 
+/* ------------------------------------------------------------- graph files
+ * Replaces load_graph (runtime/infra_gpu/graph.h:236-303) and the GAPBS
+ * reader and builder it calls (runtime/infra_gapbs/reader.h:51-375,
+ * builder.h:62-246). The format comes from the suffix: .el ("u v", weight
+ * 1), .wel ("u v w"), .gr (DIMACS "a u v w" lines, ids as written), .graph
+ * (METIS, ids from 1, fmt 1 = weights), .mtx (MatrixMarket coordinate, ids
+ * from 1, symmetric matrices both ways), .sg / .wsg (GAPBS serialized CSR,
+ * taken as stored), .graphit_bin (the reference's cache). Text formats are
+ * built as GAPBS builds them: V = largest id + 1, lists sorted, a repeated
+ * neighbour kept once with its least weight, self-loops dropped; directed
+ * unless GI_LOAD_SYMMETRIZE (load_graph never symmetrizes). Host code only:
+ * no device is touched by gi_read_graph_file. */
+enum gi_load_flags { GI_LOAD_SYMMETRIZE = 1, GI_LOAD_CACHE = 2 };
+typedef struct gi_host_csr {
+    int32_t num_vertices;
+    int64_t num_edges;
+    int32_t directed;
+    int32_t *offsets;  /* num_vertices + 1, malloc'd */
+    int32_t *nbrs;     /* num_edges */
+    int32_t *weights;  /* num_edges; NULL for .sg */
+} gi_host_csr;
+/* Read a graph file into host arrays (free with gi_host_csr_free). */
+int gi_read_graph_file(const char *path, int32_t flags, gi_host_csr *out);
+void gi_host_csr_free(gi_host_csr *c);
+/* Write the reference's .graphit_bin cache (graph.h:286-293): int32 V,
+ * int32 E, sources[E], neighbours[E], weights[E] (1 when absent),
+ * offsets[V+1]. */
+int gi_write_graphit_bin(const gi_host_csr *c, const char *path);
+/* Read a graph file and build the device graph. With GI_LOAD_CACHE the
+ * reference's cache next to the file ("<path>.i.graphit_bin", or
+ * "<path>.sym.i.graphit_bin" when symmetrized) is read when present and
+ * written when not (graph.h:239-293). */
+int gi_graph_load(const char *path, int32_t flags, gi_graph **out);
+
 /* Vertex layout used by the fused engines (no reference counterpart: the
  * reference keeps the file's vertex order, graph.h:236-338). DEGREE builds an
  * internal copy renumbered by decreasing degree; fused BFS (single- and

@@ -7,7 +7,8 @@ The product library never imports it.
 numpy wrappers over ``liboracle.so`` (serial C, ``oracle.c``). Each C function
 cites the reference file:line it restates; how each is pinned is listed at the
 top of ``oracle.c``. ``ref()`` loads ``_ref/libgapbs_ref.so``, the reference's
-own GAPBS builder compiled from its sources, when it was built.
+own GAPBS builder and reader compiled from its sources, when it was built;
+``files.py`` restates the graph file readers in plain Python.
 """
 from __future__ import annotations
 
@@ -88,8 +89,45 @@ def ref():
         R.ref_build_csr.argtypes = [C.c_int32, C.c_int64, i32p, i32p, C.c_int, i32p, i32p]
         R.ref_rmat_edges.restype = C.c_int64
         R.ref_rmat_edges.argtypes = [C.c_int, C.c_int, i32p, i32p]
+        R.ref_read_graph.restype = C.c_int64
+        R.ref_read_graph.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_int)]
+        R.ref_read_graph_copy.restype = None
+        R.ref_read_graph_copy.argtypes = [i32p, i32p, i32p]
+        R.ref_write_serialized.restype = None
+        R.ref_write_serialized.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_char_p]
         _REF = R
     return _REF
+
+
+def ref_read_graph(path: str, symmetrize: bool = False):
+    """A graph file read by the reference's own GAPBS reader and builder
+    (oracle/_ref), as load_graph reads it: (offsets, nbrs, weights or None,
+    directed), or None when _ref was not built. Only for valid files: the
+    reference exits the process on errors."""
+    R = ref()
+    if R is None:
+        return None
+    e, d = C.c_int64(), C.c_int()
+    n = R.ref_read_graph(os.fsencode(path), int(symmetrize), C.byref(e), C.byref(d))
+    m = e.value
+    off = np.empty(n + 1, np.int32)
+    nb = np.empty(max(m, 1), np.int32)
+    w = np.empty(max(m, 1), np.int32)
+    R.ref_read_graph_copy(off, nb, w)
+    weighted = not path.endswith(".sg")
+    return off, nb[:m], (w[:m] if weighted else None), bool(d.value)
+
+
+def ref_write_serialized(text_path: str, symmetrize: bool, weighted: bool, out_path: str) -> bool:
+    """GAPBS builds a text graph file and writes it serialized (.sg/.wsg)."""
+    R = ref()
+    if R is None:
+        return False
+    R.ref_write_serialized(os.fsencode(text_path), int(symmetrize), int(weighted), os.fsencode(out_path))
+    return True
+
+
+from .files import graphit_bin_bytes, read_graph_file  # noqa: E402
 
 
 # ---------------------------------------------------------------- generators
