@@ -567,11 +567,12 @@ int gi_cc(gi_graph *g, const gi_schedule *sched, int32_t *d_labels, gi_run_stats
         // fused engine: hook sweeps and shortcutting with device-side termination
         EventTimer timer(s);
         int32_t rounds = 0;
+        const bool sampled = sched->load_balancing == GI_LB_VERTEX_BASED;
         if (g->layout) {  // degree-ordered view: labels come back as original minimum ids
-            GI_TRY(cc_fused_run(layout_view(g), layout_tmp(g, 0), &rounds, s));
+            GI_TRY(cc_fused_run(layout_view(g), layout_tmp(g, 0), &rounds, sampled, s));
             GI_TRY(layout_labels_back(g, layout_tmp(g, 0), d_labels, s));
         } else {
-            GI_TRY(cc_fused_run(g, d_labels, &rounds, s));
+            GI_TRY(cc_fused_run(g, d_labels, &rounds, sampled, s));
         }
         double ms = timer.stop();
         if (stats) {
@@ -580,7 +581,7 @@ int gi_cc(gi_graph *g, const gi_schedule *sched, int32_t *d_labels, gi_run_stats
             stats->pull_iterations = 0;
             stats->ms = ms;
             stats->reached = g->V;
-            stats->edges_traversed = (int64_t)rounds * g->E;
+            stats->edges_traversed = sampled ? 0 : (int64_t)rounds * g->E;  // sampled rounds skip most edges
         }
         return GI_OK;
     }

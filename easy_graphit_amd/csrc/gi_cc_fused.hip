@@ -3,7 +3,15 @@
 // stable, until a sweep changes nothing) with every decision on the device
 // and the rounds captured as a hipGraph.
 //
-// Per round (three launches):
+// Two schedules. EDGE_ONLY (the app's own, apps/cc.cpp:84-86) sweeps every
+// edge each round. VERTEX_BASED first hooks the first two neighbours of every
+// vertex (two vertex passes, each followed by shortcutting), samples the most
+// frequent label, and then runs rounds in which only vertices outside that
+// label hook their edges; on graphs with a giant component (RMAT) those
+// rounds touch a small part of the graph. Both stop when a round lowers no
+// label and give the same labels.
+//
+// Per EDGE_ONLY round (three launches):
 //   * cc_hook: an edge-parallel sweep of the CSR. Edges are cut into chunks
 //     of kCcChunk; the source of each chunk's first edge is precomputed once
 //     per graph. A wave takes 256 edges at a time (4 per lane, loads in
@@ -19,6 +27,7 @@
 // labels are vertex ids of the same component and only decrease, and the
 // minimum vertex never takes another label.
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "gi_internal.h"
@@ -33,7 +42,7 @@ struct CcState {
     int32_t changed;   // set by a hook sweep that lowered a label
     int32_t done;
     int32_t rounds;
-    int32_t pad;
+    int32_t giant;     // label whose vertices the filtered rounds skip (-1: none)
 };
 
 struct CcArgs {
@@ -67,6 +76,7 @@ __global__ void cc_init_kernel(int32_t *id, int32_t V, CcState *st) {
         st->changed = 0;
         st->done = 0;
         st->rounds = 0;
+        st->giant = -1;
     }
 }
 
@@ -141,6 +151,101 @@ __global__ void __launch_bounds__(kCcBlock) cc_hook_kernel(CcArgs a) {
     if (__ballot(changed) && lane == 0 && !a.st->changed) a.st->changed = 1;
 }
 
+// One hook of edge (u, v) as the reference's UDF does it: ID[ID[u]] and
+// ID[ID[v]] compared, the larger parent lowered by atomicMin
+// (apps/cc.cpp:23-29). lu is ID[u], already read. True when it lowered.
+__device__ __forceinline__ bool cc_hook(int32_t *id, int32_t lu, int32_t v) {
+    const int32_t lv = id[v];
+    const int32_t ps = id[lu], pd = id[lv];
+    if (pd < ps) return atomicMin(&id[lu], pd) > pd;
+    if (ps < pd) return atomicMin(&id[lv], ps) > ps;
+    return false;
+}
+
+// Vertex-parallel hooks (VERTEX_BASED). Sampling pass r: every vertex hooks
+// its r-th neighbour, so two passes join most of a giant component at the
+// cost of two reads per vertex.
+__global__ void __launch_bounds__(kCcBlock) cc_sample_kernel(CcArgs a, int32_t r) {
+    if (a.st->done) return;
+    for (int32_t u = blockIdx.x * blockDim.x + threadIdx.x; u < a.V; u += gridDim.x * blockDim.x) {
+        const int32_t e = a.off[u] + r;
+        if (e < a.off[u + 1]) (void)cc_hook(a.id, a.id[u], a.nbr[e]);
+    }
+}
+
+// The most frequent label among 1024 spread-out vertices (one workgroup).
+__global__ void __launch_bounds__(kCcBlock) cc_giant_kernel(CcArgs a) {
+    constexpr int kSamples = 1024;
+    __shared__ int32_t lab[kSamples];
+    __shared__ int32_t best_n[kCcBlock / 64], best_l[kCcBlock / 64];
+    for (int32_t i = threadIdx.x; i < kSamples; i += kCcBlock)
+        lab[i] = a.id[(int32_t)(((uint64_t)i * 2654435761ull + 12345ull) % (uint64_t)a.V)];
+    __syncthreads();
+    int32_t bn = 0, bl = -1;
+    for (int32_t i = threadIdx.x; i < kSamples; i += kCcBlock) {
+        int32_t n = 0;
+        for (int32_t j = 0; j < kSamples; ++j) n += lab[j] == lab[i];
+        if (n > bn || (n == bn && lab[i] < bl)) {
+            bn = n;
+            bl = lab[i];
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const int32_t on = __shfl_xor(bn, o), ol = __shfl_xor(bl, o);
+        if (on > bn || (on == bn && ol < bl)) {
+            bn = on;
+            bl = ol;
+        }
+    }
+    if (lane_id() == 0) {
+        best_n[threadIdx.x >> 6] = bn;
+        best_l[threadIdx.x >> 6] = bl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kCcBlock / 64; ++w)
+            if (best_n[w] > bn || (best_n[w] == bn && best_l[w] < bl)) {
+                bn = best_n[w];
+                bl = best_l[w];
+            }
+        a.st->giant = bl;
+    }
+}
+
+// Filtered round: every vertex outside the giant label hooks all its edges;
+// vertices inside it are skipped. In a symmetric graph an edge whose ends
+// still differ has an end outside the giant label, whose own list holds the
+// edge, so a round that lowers nothing still proves every edge's ends equal.
+// Short lists are walked by their lane, long ones by the whole wave.
+__global__ void __launch_bounds__(kCcBlock) cc_rest_kernel(CcArgs a) {
+    if (a.st->done) return;
+    const int32_t giant = a.st->giant, lane = lane_id();
+    bool changed = false;
+    const int32_t stride = gridDim.x * blockDim.x;
+    for (int32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63); base < a.V; base += stride) {
+        const int32_t u = base + lane;
+        int32_t lu = 0, e0 = 0, e1 = 0;
+        if (u < a.V) {
+            lu = a.id[u];
+            if (lu != giant) {
+                e0 = a.off[u];
+                e1 = a.off[u + 1];
+            }
+        }
+        const bool wide = e1 - e0 >= 64;
+        if (!wide)
+            for (int32_t e = e0; e < e1; ++e) changed |= cc_hook(a.id, lu, a.nbr[e]);
+        uint64_t wm = __ballot(wide);
+        while (wm) {
+            const int32_t l = __ffsll((unsigned long long)wm) - 1;
+            wm &= wm - 1;
+            const int32_t w0 = __shfl(e0, l), w1 = __shfl(e1, l), wu = __shfl(lu, l);
+            for (int32_t e = w0 + lane; e < w1; e += 64) changed |= cc_hook(a.id, wu, a.nbr[e]);
+        }
+    }
+    if (__ballot(changed) && lane == 0 && !a.st->changed) a.st->changed = 1;
+}
+
 __global__ void cc_compress_kernel(CcArgs a) {
     if (a.st->done) return;
     for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.V; v += gridDim.x * blockDim.x) {
@@ -168,7 +273,7 @@ struct CcPlan {
     CcState *st = nullptr;
     int32_t *host = nullptr;  // pinned copy of the state
     hipStream_t cap = nullptr;
-    std::map<std::pair<int32_t, int32_t *>, hipGraphExec_t> execs;  // (rounds, labels)
+    std::map<std::tuple<int32_t, int32_t *, bool>, hipGraphExec_t> execs;  // (rounds, labels, sampled)
     int32_t last_rounds = 0;
 };
 
@@ -230,14 +335,20 @@ static int hook_grid() {
     return device_info().num_cus * per_cu;
 }
 
-static void launch_round(const CcArgs &a, hipStream_t s) {
-    cc_hook_kernel<<<hook_grid(), kCcBlock, 0, s>>>(a);
+static int vertex_grid(int32_t V) {
+    const int g = (int)ceil_div(V, kCcBlock);
+    return g < 8192 ? (g < 1 ? 1 : g) : 8192;
+}
+
+static void launch_round(const CcArgs &a, bool sampled, hipStream_t s) {
+    if (sampled) cc_rest_kernel<<<vertex_grid(a.V), kCcBlock, 0, s>>>(a);
+    else cc_hook_kernel<<<hook_grid(), kCcBlock, 0, s>>>(a);
     int cg = (int)ceil_div(a.V, 256);
     cc_compress_kernel<<<cg < 8192 ? cg : 8192, 256, 0, s>>>(a);
     cc_control_kernel<<<1, 1, 0, s>>>(a.st);
 }
 
-int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, hipStream_t s) {
+int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, bool sampled, hipStream_t s) {
     if (g->V == 0) {
         if (rounds_out) *rounds_out = 0;
         return GI_OK;
@@ -256,11 +367,18 @@ int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, hipStream_
     a.chunks = p->chunks;
     int ig = (int)ceil_div(g->V, 256);
     cc_init_kernel<<<ig < 8192 ? ig : 8192, 256, 0, s>>>(d_labels, g->V, p->st);
+    if (sampled) {  // two sampling passes, each followed by full shortcutting
+        for (int32_t r = 0; r < 2; ++r) {
+            cc_sample_kernel<<<vertex_grid(g->V), kCcBlock, 0, s>>>(a, r);
+            cc_compress_kernel<<<ig < 8192 ? ig : 8192, 256, 0, s>>>(a);
+        }
+        cc_giant_kernel<<<1, kCcBlock, 0, s>>>(a);
+    }
     GI_LAUNCH_CHECK();
     int32_t block = p->last_rounds > 0 ? p->last_rounds : 4;
     for (int32_t done_rounds = 0; done_rounds < (1 << 20);) {
         if (block > 32) block = 32;
-        auto key = std::make_pair(block, d_labels);
+        auto key = std::make_tuple(block, d_labels, sampled);
         auto it = p->execs.find(key);
         if (it == p->execs.end()) {
             if (p->execs.size() >= 8) {
@@ -269,7 +387,7 @@ int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, hipStream_
             }
             hipGraph_t graph = nullptr;
             GI_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-            for (int32_t r = 0; r < block; ++r) launch_round(a, p->cap);
+            for (int32_t r = 0; r < block; ++r) launch_round(a, sampled, p->cap);
             hipError_t e = hipStreamEndCapture(p->cap, &graph);
             if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
             hipGraphExec_t exec = nullptr;

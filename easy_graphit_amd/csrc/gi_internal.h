@@ -276,7 +276,9 @@ int read_count(const int32_t *d_count, int32_t *out, hipStream_t s);
 // gi_graph_destroy).
 int pr_fused_run(gi_graph *g, int32_t iters, float damping, float *d_rank, hipStream_t s);
 void release_pr(const gi_graph *g);
-int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds, hipStream_t s);
+// sampled: the VERTEX_BASED schedule (neighbour sampling, then rounds over
+// the vertices outside the most frequent label); else full EDGE_ONLY sweeps.
+int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds, bool sampled, hipStream_t s);
 void release_cc(const gi_graph *g);
 // counts (optional): relax steps, pull steps, windows entered, captured slots.
 int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,

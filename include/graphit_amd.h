@@ -459,7 +459,11 @@ int gi_sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_min, 
  * are the minimum vertex id of each component. Default schedule (s == NULL):
  * the app's PUSH EDGE_ONLY (apps/cc.cpp:84-86). With kernel_fusion ENABLED
  * (symmetric graphs) the rounds run as a captured graph with device-side
- * termination and full shortcutting per round. */
+ * termination and full shortcutting per round: EDGE_ONLY sweeps every edge
+ * each round; VERTEX_BASED first hooks every vertex's first two neighbours,
+ * then runs rounds over the vertices outside the most frequent label only
+ * (an edge whose ends differ always has such an end). Same labels either
+ * way. */
 int gi_cc(gi_graph *g, const gi_schedule *s, int32_t *d_labels, gi_run_stats *stats,
           void *stream);
 /* PageRank (apps/pr.cpp:49-69): iters iterations, FP32; d_rank = old_rank.

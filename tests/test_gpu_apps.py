@@ -191,8 +191,9 @@ def test_pagerank_schedules(gi, direction, lb):
     assert rel_l1(r, O.pagerank(host_csr(g), 20)) <= 1e-5
 
 
+@pytest.mark.parametrize("lb", [6, 0])  # EDGE_ONLY sweeps; VERTEX_BASED sampled + filtered rounds
 @pytest.mark.parametrize("graph", ["rmat", "grid", "forest"])
-def test_cc_fused_matches_oracle(gi, graph):
+def test_cc_fused_matches_oracle(gi, graph, lb):
     if graph == "rmat":
         g = gi.Graph.rmat(15, 8, SEED, True)
     elif graph == "grid":
@@ -207,7 +208,7 @@ def test_cc_fused_matches_oracle(gi, graph):
         g = gi.Graph.from_edges(n, e[0], e[1], True)
     host = host_csr(g)
     want = O.cc(host)
-    sched = gi.Schedule(direction=0, load_balancing=6, kernel_fusion=1)
+    sched = gi.Schedule(direction=0, load_balancing=lb, kernel_fusion=1)
     for _ in range(2):  # second run replays the captured rounds
         lab, st = gi.cc(g, schedule=sched)
         assert np.array_equal(lab, want)

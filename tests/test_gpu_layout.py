@@ -57,10 +57,11 @@ def test_layout_cc(gi):
     g = gi.Graph.from_edges(n, e[0], e[1], True)
     host = O.build_csr(n, e[0], e[1], True)
     g.set_layout(gi.LAYOUT_DEGREE)
-    sched = gi.Schedule(direction=0, load_balancing=6, kernel_fusion=1)
-    for _ in range(2):
-        lab, _ = gi.cc(g, schedule=sched)
-        assert np.array_equal(lab, O.cc(host))
+    for lb in (6, 0):
+        sched = gi.Schedule(direction=0, load_balancing=lb, kernel_fusion=1)
+        for _ in range(2):
+            lab, _ = gi.cc(g, schedule=sched)
+            assert np.array_equal(lab, O.cc(host)), lb
 
 
 def test_layout_directed_bfs(gi):
