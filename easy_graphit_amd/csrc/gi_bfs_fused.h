@@ -53,21 +53,27 @@ struct BfsState {
 };
 
 // Device view of the selector network (BF16 weights, FP32 accumulation).
+// Passed to the decide kernel by value. The BF16 weights are stored one row
+// per hidden unit, so that a lane (one hidden unit) reads its own weights
+// with 16-byte loads: w1 [hid][in], w2 [hid][hid] (row = the unit fed), w3
+// [hid][kSelMaxOut] (row = the unit feeding, zero-padded), biases b1 [hid],
+// b2 [hid], b3 [out].
 struct SelectorView {
-    const uint16_t *w1, *b1, *w2, *b2, *w3, *b3;  // [in][hid], [hid], [hid][hid], [hid], [hid][out], [out]
+    const uint16_t *w1, *b1, *w2, *b2, *w3, *b3;
     int32_t n_out;
     int32_t allowed;     // bit mask of modes the selector may pick
     int32_t mode;        // gi_selector_mode
     int32_t mode_s1;     // heuristic's s1 / s2 modes
     int32_t mode_s2;
     float threshold;
-    const float *feat_mean;  // feature normalisation (kSelInputs)
-    const float *feat_scale;
+    float feat_mean[kSelInputs];   // feature normalisation
+    float feat_scale[kSelInputs];
 };
 
-// The selector's 16 input features, computed from device counters (no host
-// sync). Shared by the decide kernel (one thread per feature) and the dataset
-// recorder, so training and inference see identical inputs.
+// The selector's 16 input features from the run's counters, in double: the
+// dataset recorder's definition (training inputs). The decide kernel
+// evaluates the same formulas in FP32 with the hardware log2
+// (bfs_features_f32), equal to these within FP32 rounding.
 __host__ __device__ inline float bfs_raw_feature(int k, int32_t n, unsigned long long fdeg,
                                                  long long unvisited, long long unvisited_edges,
                                                  int32_t level, int32_t prev_mode,

@@ -29,6 +29,7 @@
 //     of a partial-sum array (plain stores) that the decide kernel adds up,
 //     so no counter is shared by the whole grid.
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "gi_bfs_fused.h"
@@ -613,12 +614,47 @@ struct SumSet {
     uint64_t *s[3];
 };
 
+// The selector's 16 inputs in FP32 with the hardware log2 (v_log_f32): the
+// values of bfs_raw_feature (double, used by the host to record training
+// data) to within FP32 rounding, at a few dozen instructions instead of the
+// hundreds of FP64 ones its logarithms and divisions take on one wavefront.
+__device__ __forceinline__ void bfs_features_f32(float *f, int32_t n, unsigned long long fdeg, long long unv,
+                                                 long long unv_e, int32_t level, int32_t prev_mode,
+                                                 int32_t prev_n, int32_t V, int64_t E) {
+    const float dV = V > 0 ? (float)V : 1.f, dE = E > 0 ? (float)E : 1.f;
+    const float nn = (float)n, fd = (float)fdeg, u = (float)unv, ue = (float)unv_e;
+    const float l_n = __log2f(1.f + nn);
+    f[0] = nn / dV;
+    f[1] = fd / dE;
+    f[2] = u / dV;
+    f[3] = ue / dE;
+    f[4] = l_n / 32.f;
+    f[5] = __log2f(1.f + fd) / 40.f;
+    f[6] = __log2f(1.f + fd / (n > 0 ? nn : 1.f)) / 20.f;
+    f[7] = __log2f(1.f + dE / dV) / 20.f;
+    f[8] = __log2f(dV) / 32.f;
+    f[9] = level < 64 ? (float)level / 64.f : 1.f;
+    f[10] = prev_mode >= BFS_PULL_THREAD ? 1.f : 0.f;
+    f[11] = (l_n - __log2f(1.f + (float)prev_n)) / 16.f;
+    f[12] = fd / (1.f + ue);
+    f[13] = nn / (1.f + u);
+    f[14] = __log2f(1.f + u) / 32.f;
+    f[15] = 1.f;
+}
+
+// BF16 weight k (unrolled loops make k a constant) of a row held in registers.
+__device__ __forceinline__ float bf16_in(const uint4 *q, int k) {
+    const uint4 v = q[k >> 3];
+    const int j = (k >> 1) & 3;
+    const uint32_t word = j == 0 ? v.x : (j == 1 ? v.y : (j == 2 ? v.z : v.w));
+    return __uint_as_float((k & 1) ? (word & 0xffff0000u) : (word << 16));
+}
+
 __global__ void __launch_bounds__(kDecideBlock)
-bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E, int32_t *pcount,
+bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int64_t E, int32_t *pcount,
                   unsigned long long *pfdeg, int32_t slots, SumSet sums, int32_t sum_words, uint8_t *cflag,
                   int32_t chunks) {
     if (st->done) return;  // uniform
-    const SelectorView sel = *selp;
     // after step `level`, the bitmap two levels ahead of it was cleared by the
     // step; its summary is cleared here
     uint64_t *sum_clear = pick3(sums.s[0], sums.s[1], sums.s[2], (st->level + 2 + 3) % 3);
@@ -626,22 +662,28 @@ bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E, 
     const int32_t next_level = st->level + 1;
     const int32_t p = next_level & 1;
     const bool learned = sel.mode == GI_SELECT_LEARNED && sel.w1 != nullptr;
-    // the network's BF16 parameters (one contiguous blob: w1 b1 w2 b2 w3 b3)
-    // go to LDS with 16-byte loads while the partial sums are added
-    constexpr int kBlobMax = kSelInputs * kSelHidden + kSelHidden + kSelHidden * kSelHidden +
-                             kSelHidden + kSelHidden * kSelMaxOut + kSelMaxOut;
-    __shared__ __align__(16) uint16_t blob[(kBlobMax + 7) / 8 * 8];
-    const int32_t n_params = kSelInputs * kSelHidden + kSelHidden + kSelHidden * kSelHidden +
-                             kSelHidden + (kSelHidden + 1) * sel.n_out;
-    if (learned) {
-        const uint4 *src = reinterpret_cast<const uint4 *>(sel.w1);  // 256-byte aligned allocation
-        uint4 *dst = reinterpret_cast<uint4 *>(blob);
-        for (int i = threadIdx.x; i < n_params / 8; i += kDecideBlock) dst[i] = src[i];
-        for (int i = n_params / 8 * 8 + (int)threadIdx.x; i < n_params; i += kDecideBlock) blob[i] = sel.w1[i];
+    // the network runs on wave 0, one lane per hidden unit; each lane's
+    // weights are loaded first, so their latency hides behind the partial sums
+    static_assert(kSelMaxOut == 8, "one 16-byte row of output weights per hidden unit");
+    uint4 w1q[kSelInputs / 8], w2q[kSelHidden / 8], w3q[1];
+    uint16_t b1v = 0, b2v = 0, b3v = 0;
+#pragma unroll
+    for (int i = 0; i < kSelInputs / 8; ++i) w1q[i] = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+    for (int i = 0; i < kSelHidden / 8; ++i) w2q[i] = make_uint4(0u, 0u, 0u, 0u);
+    w3q[0] = make_uint4(0u, 0u, 0u, 0u);
+    if (learned && wid == 0) {
+        const uint4 *r1 = reinterpret_cast<const uint4 *>(sel.w1 + lane * kSelInputs);
+        const uint4 *r2 = reinterpret_cast<const uint4 *>(sel.w2 + lane * kSelHidden);
+#pragma unroll
+        for (int i = 0; i < kSelInputs / 8; ++i) w1q[i] = r1[i];
+#pragma unroll
+        for (int i = 0; i < kSelHidden / 8; ++i) w2q[i] = r2[i];
+        w3q[0] = *reinterpret_cast<const uint4 *>(sel.w3 + lane * kSelMaxOut);
+        b1v = sel.b1[lane];
+        b2v = sel.b2[lane];
+        if (lane < sel.n_out) b3v = sel.b3[lane];
     }
-    const uint16_t *sw1 = blob, *sb1 = sw1 + kSelInputs * kSelHidden;
-    const uint16_t *sw2 = sb1 + kSelHidden, *sb2 = sw2 + kSelHidden * kSelHidden;
-    const uint16_t *sw3 = sb2 + kSelHidden, *sb3 = sw3 + kSelHidden * sel.n_out;
     __shared__ int32_t rn[kDecideBlock / 64];
     __shared__ unsigned long long rf[kDecideBlock / 64];
     int32_t n = 0;
@@ -680,11 +722,6 @@ bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E, 
     const int32_t prev_n = next_level > 0 ? st->count[p ^ 1] : 0;
     const long long unv = st->unvisited - n;
     const long long unv_e = st->unvisited_edges - (long long)fdeg;
-    __shared__ float feat[kSelInputs];
-    __shared__ float h1[kSelHidden];
-    __shared__ float h2[kSelHidden];
-    __shared__ float y[kSelMaxOut];
-    __shared__ float part[kDecideBlock / 64][64];
     if (n == 0) {  // uniform
         if (threadIdx.x == 0) {
             st->done = 1;
@@ -693,67 +730,39 @@ bfs_decide_kernel(BfsState *st, const SelectorView *selp, int32_t V, int64_t E, 
         return;
     }
     int32_t mode;
-    const bool w0 = threadIdx.x < 64;
     if (!learned) {
         if (sel.mode == GI_SELECT_S1) mode = sel.mode_s1;
         else if (sel.mode == GI_SELECT_S2) mode = sel.mode_s2;
         else mode = ((double)n <= (double)sel.threshold * (double)V) ? sel.mode_s1 : sel.mode_s2;
     } else {
-        if (threadIdx.x < kSelInputs) {
-            const int k = threadIdx.x;
-            const float raw = bfs_raw_feature(k, n, fdeg, unv, unv_e, next_level, st->mode, prev_n, V, E);
-            feat[k] = (raw - sel.feat_mean[k]) * sel.feat_scale[k];
-        }
-        __syncthreads();
-        // each layer: wave `wid` sums its quarter of the inputs for unit `lane`
-        constexpr int kWaves = kDecideBlock / 64;
-        {
-            float acc = 0.f;
-            for (int k = wid; k < kSelInputs; k += kWaves)
-                acc += feat[k] * bf16_to_f32(sw1[k * kSelHidden + lane]);
-            part[wid][lane] = acc;
-        }
-        __syncthreads();
-        if (w0) {
-            float acc = bf16_to_f32(sb1[lane]);
-            for (int q = 0; q < kWaves; ++q) acc += part[q][lane];
-            h1[lane] = acc > 0.f ? acc : 0.f;
-        }
-        __syncthreads();
-        {
-            float acc = 0.f;
-            for (int k = wid; k < kSelHidden; k += kWaves)
-                acc += h1[k] * bf16_to_f32(sw2[k * kSelHidden + lane]);
-            part[wid][lane] = acc;
-        }
-        __syncthreads();
-        if (w0) {
-            float acc = bf16_to_f32(sb2[lane]);
-            for (int q = 0; q < kWaves; ++q) acc += part[q][lane];
-            h2[lane] = acc > 0.f ? acc : 0.f;
-        }
-        __syncthreads();
-        {
-            float acc = 0.f;
-            if (lane < sel.n_out)
-                for (int k = wid; k < kSelHidden; k += kWaves)
-                    acc += h2[k] * bf16_to_f32(sw3[k * sel.n_out + lane]);
-            part[wid][lane] = acc;
-        }
-        __syncthreads();
-        if (w0 && lane < sel.n_out) {
-            float acc = bf16_to_f32(sb3[lane]);
-            for (int q = 0; q < kWaves; ++q) acc += part[q][lane];
-            y[lane] = acc;
-        }
-        __syncthreads();
+        if (wid != 0) return;  // the state is written by thread 0
+        // every lane computes the whole input vector (the same values, so no
+        // lane waits on another's branch of the feature switch)
+        float feats[kSelInputs];
+        bfs_features_f32(feats, n, fdeg, unv, unv_e, next_level, st->mode, prev_n, V, E);
+        float a1 = bf16_to_f32(b1v);
+#pragma unroll
+        for (int k = 0; k < kSelInputs; ++k)
+            a1 += (feats[k] - sel.feat_mean[k]) * sel.feat_scale[k] * bf16_in(w1q, k);
+        const float h1 = a1 > 0.f ? a1 : 0.f;
+        float a2 = bf16_to_f32(b2v);
+#pragma unroll
+        for (int k = 0; k < kSelHidden; ++k) a2 += __shfl(h1, k) * bf16_in(w2q, k);
+        const float h2 = a2 > 0.f ? a2 : 0.f;
+        // output o = b3[o] + sum over the hidden units of h2 * w3[unit][o]
         mode = sel.mode_s1;
         float best = 3.4e38f;
-        for (int j = 0; j < sel.n_out; ++j)
-            if (((sel.allowed >> j) & 1) && y[j] < best) {
-                best = y[j];
-                mode = j;
+#pragma unroll
+        for (int o = 0; o < kSelMaxOut; ++o) {
+            if (o >= sel.n_out) break;  // uniform
+            float t = h2 * bf16_in(w3q, o);
+            for (int x = 32; x > 0; x >>= 1) t += __shfl_xor(t, x);
+            const float yo = t + bf16_to_f32((uint16_t)__shfl((int32_t)b3v, o));
+            if (((sel.allowed >> o) & 1) && yo < best) {
+                best = yo;
+                mode = o;
             }
+        }
     }
     if (threadIdx.x == 0) {
         st->count[p] = n;
@@ -915,9 +924,7 @@ struct FusedWork {
     int32_t *cand = nullptr;
     const int32_t *poff = nullptr;   // in-edge CSR (the graph itself when symmetric)
     const int32_t *pnbr = nullptr;
-    SelectorView *d_sel = nullptr;   // selector of the current run (device copy)
-    SelectorView last_sel{};         // host copy of what d_sel holds
-    bool sel_valid = false;
+    SelectorView last_sel{};         // selector of the current run (passed to the decide kernel)
     int32_t *d_src = nullptr;        // sources of a multi-source run
     int32_t src_cap = 0;
     int32_t *host_flag = nullptr;    // pinned copy of {done, level, levels, mode}
@@ -928,7 +935,9 @@ struct FusedWork {
     int32_t blocks_per_cu = 8;
     // level loop captured as a hipGraph, one exec per level count
     hipStream_t cap_stream = nullptr;
-    std::map<int32_t, hipGraphExec_t> execs;
+    // captured level loops by (level count, selector view bytes): the decide
+    // kernel takes the view by value
+    std::map<std::pair<int32_t, std::string>, hipGraphExec_t> execs;
     int32_t last_levels = 0;
 };
 
@@ -951,7 +960,6 @@ static void free_work(FusedWork *w) {
     dfree(w->reached);
     dfree(w->dead);
     dfree(w->cand);
-    dfree(w->d_sel);
     dfree(w->d_src);
     if (w->host_flag) (void)hipHostFree(w->host_flag);
     if (w->flag_ev) (void)hipEventDestroy(w->flag_ev);
@@ -990,7 +998,6 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
     if (rc == GI_OK) rc = dalloc(&w->reached, 1);
     if (rc == GI_OK) rc = dalloc(&w->dead, (size_t)words + 1);
     if (rc == GI_OK) rc = dalloc(&w->cand, (size_t)g->V + 1);
-    if (rc == GI_OK) rc = dalloc(&w->d_sel, 1);
     if (rc == GI_OK && !g->d_visited) rc = dalloc(&g->d_visited, (size_t)words + 1);
     if (rc == GI_OK && hipHostMalloc((void **)&w->host_flag, sizeof(int32_t) * 4) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
@@ -1095,7 +1102,7 @@ static StepArgs step_args(gi_graph *g, FusedWork *w) {
 static void launch_decide(gi_graph *g, FusedWork *w, int32_t level, hipStream_t s) {
     (void)level;  // the kernel reads the level from the device state
     SumSet sums{{w->sum[0], w->sum[1], w->sum[2]}};
-    bfs_decide_kernel<<<1, kDecideBlock, 0, s>>>(w->st, w->d_sel, g->V, g->E, w->pcount, w->pfdeg,
+    bfs_decide_kernel<<<1, kDecideBlock, 0, s>>>(w->st, w->last_sel, g->V, g->E, w->pcount, w->pfdeg,
                                                  step_slots(w), sums, w->sum_words, w->cflag,
                                                  (int32_t)ceil_div(bitmap_words(g->V), kChunkWords));
 }
@@ -1126,11 +1133,7 @@ static void launch_level_traced(gi_graph *g, FusedWork *w, const StepArgs &a, in
 static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t n_sources,
                     int32_t single, int32_t *depth, int32_t *parent, const SelectorView &sel,
                     bool want_fdeg, hipStream_t s) {
-    if (!w->sel_valid || memcmp(&w->last_sel, &sel, sizeof(SelectorView)) != 0) {
-        GI_HIP(hipMemcpyAsync(w->d_sel, &sel, sizeof(SelectorView), hipMemcpyHostToDevice, s));
-        w->last_sel = sel;
-        w->sel_valid = true;
-    }
+    w->last_sel = sel;  // captured level graphs are kept per selector view (level_graph)
     SeedArgs sa{depth, parent, w->lane_max, w->hub_min, want_fdeg ? 1 : 0};
     ResetArgs r;
     r.depth = depth;
@@ -1173,7 +1176,9 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
 // kernel reads its level from the device state, so one capture serves every
 // run on this graph.
 static int level_graph(gi_graph *g, FusedWork *w, int32_t levels, hipGraphExec_t *out) {
-    auto it = w->execs.find(levels);
+    const std::pair<int32_t, std::string> key(
+        levels, std::string(reinterpret_cast<const char *>(&w->last_sel), sizeof(SelectorView)));
+    auto it = w->execs.find(key);
     if (it != w->execs.end()) {
         *out = it->second;
         return GI_OK;
@@ -1188,7 +1193,8 @@ static int level_graph(gi_graph *g, FusedWork *w, int32_t levels, hipGraphExec_t
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
-    w->execs[levels] = exec;
+    if (w->execs.size() >= 16) drop_execs(w);
+    w->execs[key] = exec;
     *out = exec;
     return GI_OK;
 }

@@ -14,8 +14,8 @@ namespace gi {
 
 // ------------------------------------------------------------ model storage
 struct SelectorModel {
-    uint16_t *w = nullptr;  // all BF16 parameters, packed in layer order
-    float *norm = nullptr;  // feat_mean[16], feat_scale[16]
+    uint16_t *w = nullptr;  // BF16 inference blob: w1 [hid][in], w2 [hid][hid], w3 [hid][kSelMaxOut], b1, b2, b3
+    float norm[2 * kSelInputs] = {};  // feat_mean, feat_scale
     int32_t n_out = 0;
     int32_t allowed = 0;
     bool loaded = false;
@@ -38,18 +38,20 @@ static int64_t param_count(int32_t n_out) {
 SelectorView selector_view() {
     SelectorView v{};
     if (!g_model.loaded) return v;
-    const uint16_t *p = g_model.w;
+    const uint16_t *p = g_model.w;  // 16-byte rows: w1 at 0, w2 at 2048, w3 at 10240 bytes
     v.w1 = p; p += kSelInputs * kSelHidden;
-    v.b1 = p; p += kSelHidden;
     v.w2 = p; p += kSelHidden * kSelHidden;
+    v.w3 = p; p += kSelHidden * kSelMaxOut;
+    v.b1 = p; p += kSelHidden;
     v.b2 = p; p += kSelHidden;
-    v.w3 = p; p += kSelHidden * g_model.n_out;
     v.b3 = p;
     v.n_out = g_model.n_out;
     v.allowed = g_model.allowed;
     v.mode = GI_SELECT_LEARNED;
-    v.feat_mean = g_model.norm;
-    v.feat_scale = g_model.norm + kSelInputs;
+    for (int k = 0; k < kSelInputs; ++k) {
+        v.feat_mean[k] = g_model.norm[k];
+        v.feat_scale[k] = g_model.norm[kSelInputs + k];
+    }
     return v;
 }
 
@@ -197,16 +199,31 @@ int gi_selector_load(const float *params, int32_t n_out, int32_t allowed_mask,
     if (n_out < 1 || n_out > kSelMaxOut) return fail(GI_ERR_RANGE, "n_out must be in [1,%d]", kSelMaxOut);
     if ((allowed_mask & ((1 << n_out) - 1)) == 0) return fail(GI_ERR_INVALID, "no allowed mode");
     int64_t np = param_count(n_out);
-    std::vector<uint16_t> h((size_t)np);
-    for (int64_t i = 0; i < np; ++i) h[(size_t)i] = f32_to_bf16(params[i]);
+    // params in layer order: w1 [in][hid], b1, w2 [hid][hid], b2, w3 [hid][out], b3;
+    // the blob keeps one row per unit of the layer each weight feeds
+    const float *w1 = params, *b1 = w1 + kSelInputs * kSelHidden, *w2 = b1 + kSelHidden;
+    const float *b2 = w2 + kSelHidden * kSelHidden, *w3 = b2 + kSelHidden, *b3 = w3 + kSelHidden * n_out;
+    // w3 rows padded to kSelMaxOut outputs: one 16-byte load per hidden unit
+    std::vector<uint16_t> h((size_t)(kSelInputs * kSelHidden + kSelHidden * kSelHidden + kSelHidden * kSelMaxOut +
+                                     2 * kSelHidden + kSelMaxOut), 0);
+    uint16_t *o = h.data();
+    for (int j = 0; j < kSelHidden; ++j)
+        for (int k = 0; k < kSelInputs; ++k) *o++ = f32_to_bf16(w1[k * kSelHidden + j]);
+    for (int j = 0; j < kSelHidden; ++j)
+        for (int k = 0; k < kSelHidden; ++k) *o++ = f32_to_bf16(w2[k * kSelHidden + j]);
+    for (int j = 0; j < kSelHidden; ++j)
+        for (int k = 0; k < kSelMaxOut; ++k) *o++ = k < n_out ? f32_to_bf16(w3[j * n_out + k]) : 0;
+    for (int i = 0; i < kSelHidden; ++i) *o++ = f32_to_bf16(b1[i]);
+    for (int i = 0; i < kSelHidden; ++i) *o++ = f32_to_bf16(b2[i]);
+    for (int i = 0; i < n_out; ++i) *o++ = f32_to_bf16(b3[i]);
     if (g_model.w) dfree(g_model.w);
-    if (g_model.norm) dfree(g_model.norm);
     g_model = SelectorModel();
-    GI_TRY(dalloc(&g_model.w, (size_t)np));
-    GI_TRY(dalloc(&g_model.norm, 2 * kSelInputs));
-    GI_HIP(hipMemcpy(g_model.w, h.data(), sizeof(uint16_t) * np, hipMemcpyHostToDevice));
-    GI_HIP(hipMemcpy(g_model.norm, feat_mean, sizeof(float) * kSelInputs, hipMemcpyHostToDevice));
-    GI_HIP(hipMemcpy(g_model.norm + kSelInputs, feat_scale, sizeof(float) * kSelInputs, hipMemcpyHostToDevice));
+    GI_TRY(dalloc(&g_model.w, h.size()));
+    GI_HIP(hipMemcpy(g_model.w, h.data(), sizeof(uint16_t) * h.size(), hipMemcpyHostToDevice));
+    for (int k = 0; k < kSelInputs; ++k) {
+        g_model.norm[k] = feat_mean[k];
+        g_model.norm[kSelInputs + k] = feat_scale[k];
+    }
     g_model.n_out = n_out;
     g_model.allowed = allowed_mask & ((1 << n_out) - 1);
     g_model.loaded = true;
@@ -215,7 +232,6 @@ int gi_selector_load(const float *params, int32_t n_out, int32_t allowed_mask,
 
 int gi_selector_clear(void) {
     if (g_model.w) dfree(g_model.w);
-    if (g_model.norm) dfree(g_model.norm);
     g_model = SelectorModel();
     return GI_OK;
 }

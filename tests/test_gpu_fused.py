@@ -144,6 +144,67 @@ def test_learned_selector_outputs_stay_exact(gi, c1):
         gi.selector_clear()
 
 
+def _bf16(a):
+    """float32 -> BF16 (round to nearest even) -> float64."""
+    u = np.asarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _raw_features(n, fdeg, unv, unv_e, level, prev_pull, prev_n, V, E):
+    """The 16 selector inputs (the decide kernel's bfs_raw_feature)."""
+    lg = np.log2
+    return np.array([n / V, fdeg / E, unv / V, unv_e / E, lg(1 + n) / 32, lg(1 + fdeg) / 40,
+                     lg(1 + fdeg / max(n, 1)) / 20, lg(1 + E / V) / 20, lg(V) / 32,
+                     min(level / 64, 1.0), float(prev_pull), (lg(1 + n) - lg(1 + prev_n)) / 16,
+                     fdeg / (1 + unv_e), n / (1 + unv), lg(1 + unv) / 32, 1.0])
+
+
+def test_learned_selector_picks_match_host_inference(gi, c1):
+    """Every level's mode picked by the MLP in the decide kernel equals the
+    argmin of the same BF16 network evaluated on the host (float64) from the
+    oracle's level sets, wherever the two best outputs are not a near-tie."""
+    g, host, _ = c1
+    V, E = host.num_vertices, len(host.nbrs)
+    deg = np.diff(host.offsets).astype(np.int64)
+    rng = np.random.default_rng(23)
+    n_out, allowed = 7, 0x7F
+    p = _init_params(rng, n_out)
+    p[1] = rng.normal(size=64) * 0.1
+    p[5] = rng.normal(size=n_out) * 0.1
+    mean, scale = rng.normal(size=16) * 0.05, rng.uniform(0.5, 3.0, 16)
+    gi.selector_load(_flat(p), n_out, allowed, mean, scale)
+    try:
+        h = gi.HybridSchedule(fused(gi, 4), fused(gi, 5), 0.05, gi.SELECT_LEARNED)
+        depth = gi.DeviceArray(V, np.int32)
+        checked = 0
+        for src in O.pick_sources(host, 6, 29):
+            recs = gi.bfs_profile(g, int(src), h, depth)
+            d = O.bfs_depth(host, int(src))
+            w = [_bf16(x) for x in p]
+            prev_mode, prev_n, seen, seen_e = 0, 0, 0, 0
+            for r in recs:
+                lvl = r["level"]
+                F = np.flatnonzero(d == lvl)
+                n, fdeg = len(F), int(deg[F].sum())
+                seen += n
+                seen_e += fdeg
+                x = (_raw_features(n, fdeg, V - seen, E - seen_e, lvl, prev_mode >= 5, prev_n, V, E)
+                     .astype(np.float32).astype(np.float64) - mean.astype(np.float32)) * scale.astype(np.float32)
+                h1 = np.maximum(x @ w[0] + w[1], 0)
+                h2 = np.maximum(h1 @ w[2] + w[3], 0)
+                y = h2 @ w[4] + w[5]
+                order = np.argsort(y)
+                got = r["load_balancing"] - 100
+                if y[order[1]] - y[order[0]] > 1e-3 * max(1.0, abs(y[order[0]])):
+                    assert got == order[0], (src, lvl, got, y)
+                    checked += 1
+                prev_mode, prev_n = got, n
+        assert checked >= 10
+    finally:
+        gi.selector_clear()
+
+
 def test_multi_source_bfs_matches_oracle(gi, c1):
     g, host, _ = c1
     srcs = O.pick_sources(host, 21, 9)
