@@ -493,15 +493,16 @@ __global__ void __launch_bounds__(kStepBlock) bfs_hub_kernel(StepArgs a0) { hub_
 // their own symbols so their launches can be told apart in a profile.
 enum StepKind : int { kPushStep = 0, kPullStep = 1 };
 
+// `pre` is LDS of kMaxSumWords + 1 words owned by the calling kernel, so a
+// kernel that holds both step bodies allocates it once.
 template <int Kind>
-__device__ __forceinline__ void step_body(const StepArgs &a0) {
+__device__ __forceinline__ void step_body(const StepArgs &a0, int32_t *pre) {
     BfsState *st = a0.st;
     if (st->done) return;
     if ((Kind == kPullStep) != (st->mode >= BFS_PULL_THREAD)) return;  // uniform
     StepArgs a = a0;
     a.depth = st->depth;
     a.parent = st->parent;
-    __shared__ int32_t pre[kMaxSumWords + 1];
     StepCtx c;
     c.level = st->level;
     c.lane_max = st->lane_max;
@@ -593,12 +594,28 @@ __device__ __forceinline__ void step_body(const StepArgs &a0) {
     flush_counts(a.pcount, a.pfdeg, c);
 }
 
-// (both fit the register budget of 8 waves per SIMD without spilling)
+// (the four single-body kernels fit the register budget of 8 waves per SIMD
+// without spilling)
 #define GI_STEP_ATTR __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(8)))
-__global__ void GI_STEP_ATTR bfs_push_step(StepArgs a) { step_body<kPushStep>(a); }
-__global__ void GI_STEP_ATTR bfs_pull_step(StepArgs a) { step_body<kPullStep>(a); }
-__global__ void GI_STEP_ATTR bfs_push_step_traced(StepArgs a) { step_body<kPushStep>(a); }
-__global__ void GI_STEP_ATTR bfs_pull_step_traced(StepArgs a) { step_body<kPullStep>(a); }
+#define GI_STEP_KERNEL(name, kind)                  \
+    __global__ void GI_STEP_ATTR name(StepArgs a) { \
+        __shared__ int32_t pre[kMaxSumWords + 1];   \
+        step_body<kind>(a, pre);                    \
+    }
+GI_STEP_KERNEL(bfs_push_step, kPushStep)
+GI_STEP_KERNEL(bfs_pull_step, kPullStep)
+GI_STEP_KERNEL(bfs_push_step_traced, kPushStep)
+GI_STEP_KERNEL(bfs_pull_step_traced, kPullStep)
+#undef GI_STEP_KERNEL
+// The captured level loop launches one step kernel per level: the body the
+// decided mode needs (one launch fewer per level than a push and a pull
+// kernel of which one returns at once). Both bodies together need 72 VGPRs:
+// 7 waves per SIMD without spilling measured faster than 8 with a spill.
+__global__ void __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(7))) bfs_step(StepArgs a) {
+    __shared__ int32_t pre[kMaxSumWords + 1];
+    if (a.st->mode >= BFS_PULL_THREAD) step_body<kPullStep>(a, pre);  // uniform
+    else step_body<kPushStep>(a, pre);
+}
 #undef GI_STEP_ATTR
 
 // ---------------------------------------------------------------- decide kernel
@@ -1069,9 +1086,15 @@ static int pull_grid(const FusedWork *w) {
 }
 
 // Partial-sum slots the decide kernel adds up: every block of either step.
+static int step_grid(const FusedWork *w) {
+    static int resident = resident_per_cu((const void *)bfs_step);
+    return grid_of(w, resident);
+}
+
 static int step_slots(const FusedWork *w) {
-    int a = push_grid(w), b = pull_grid(w);
-    return a > b ? a : b;
+    int a = push_grid(w), b = pull_grid(w), c = step_grid(w);
+    a = a > b ? a : b;
+    return a > c ? a : c;
 }
 
 static StepArgs step_args(gi_graph *g, FusedWork *w) {
@@ -1107,11 +1130,10 @@ static void launch_decide(gi_graph *g, FusedWork *w, int32_t level, hipStream_t 
                                                  (int32_t)ceil_div(bitmap_words(g->V), kChunkWords));
 }
 
-// A captured level: both step kernels (one returns at once), the hub kernel
-// (a no-op after a pull step), the decide kernel.
+// A captured level: the step kernel (push or pull body as decided), the hub
+// kernel (a no-op after a pull step), the decide kernel.
 static void launch_level(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t level, hipStream_t s) {
-    bfs_push_step<<<push_grid(w), kStepBlock, 0, s>>>(a);
-    bfs_pull_step<<<pull_grid(w), kStepBlock, 0, s>>>(a);
+    bfs_step<<<step_grid(w), kStepBlock, 0, s>>>(a);
     bfs_hub_kernel<<<push_grid(w), kStepBlock, 0, s>>>(a);
     launch_decide(g, w, level, s);
 }
