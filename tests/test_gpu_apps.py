@@ -267,6 +267,21 @@ def test_bc_batched_matches_oracle(gi, batch):
     assert rel_l1(sc, O.bc(host, srcs)) <= 1e-4
 
 
+@pytest.mark.parametrize("graph", ["grid", "directed"])
+def test_bc_batched_long_and_directed(gi, graph):
+    # a grid: many levels, pushed and pulled in both passes; a directed graph:
+    # push only (the pulled levels need the symmetric graph's in-neighbours)
+    if graph == "grid":
+        g = gi.Graph.grid(48, 30, 3)
+    else:
+        g = gi.Graph.rmat(11, 8, 5, False)
+    host = host_csr(g)
+    srcs = O.pick_sources(host, 13, 32)
+    for batch in (1, 32):
+        sc, _ = gi.bc(g, srcs, batch=batch)
+        assert rel_l1(sc, O.bc(host, srcs)) <= 1e-4, (graph, batch)
+
+
 def test_bc_batched_equals_sequential_on_fixture(gi):
     e = np.array([[0, 1], [1, 2], [2, 3], [0, 4], [4, 3], [3, 5]], np.int32)
     g = gi.Graph.from_edges(7, e[:, 0], e[:, 1], True)  # vertex 6 isolated
