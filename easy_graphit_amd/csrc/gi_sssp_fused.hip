@@ -13,13 +13,14 @@
 // hi <= dist < INT_MAX; a window advance finds them with one coalesced scan
 // of dist, so relaxations into later windows cost no atomics beyond the
 // atomicMin on dist.
-// A captured slot is six launches, each a no-op unless its mode is active:
-//   ss_step    push: relax the out-edges of F, spread evenly over each
+// A captured slot is five launches, each a no-op unless its mode is active:
+//   ss_relax   push: relax the out-edges of F, spread evenly over each
 //              workgroup (vertices of at least kSsHubMin edges go to the
 //              grid-wide bin); an improvement below hi joins F';
-//   ss_pull    pull, for a large F: each unsettled vertex takes the minimum
-//              over its in-neighbours in F (no atomics on distances);
-//   ss_hub     the grid-wide bin: edges of very wide frontier vertices;
+//              or pull, for a large F: each unsettled vertex takes the
+//              minimum over its in-neighbours in F (no atomics on distances);
+//   ss_hub     the grid-wide bin: edges of very wide frontier vertices (not
+//              launched for a graph without such vertices);
 //   ss_decide  count F'; empty -> advance to [hi, hi + delta);
 //   ss_farmove scan dist: [lo, hi) joins F; also the least distance >= hi;
 //   ss_decide2 count the joins; none -> jump to the window that starts at
@@ -36,9 +37,10 @@
 namespace gi {
 
 constexpr int kSsBlock = 256;
+constexpr int kSsDecBlock = 1024;     // the single-workgroup decide kernels
 constexpr int32_t kSsHubMin = 4096;   // at least this wide: grid-wide bin
 constexpr int32_t kSsMaxHubs = 1024;
-constexpr int32_t kSsMaxGrid = 8192;  // partial-sum slots per kernel
+constexpr int32_t kSsMaxGrid = 8192;  // partial-sum slots per kernel (one per workgroup)
 constexpr int32_t kSsPullDiv = 8;     // a step pulls when |F| > V / kSsPullDiv
 constexpr int32_t kSsMinSlots = 9;    // slot row of the window scan's least pending distances
 
@@ -119,11 +121,10 @@ __device__ __forceinline__ void ss_relax(const SsArgs &a, SsCtx &c, int32_t du, 
 
 constexpr int kSsBatch = 4;  // edges per thread with their loads in flight together
 
-__global__ void __launch_bounds__(kSsBlock) ss_step_kernel(SsArgs a) {
+// Push relax step. `pre` holds kBsMaxSumWords + 1 entries, `list`
+// kBsChunkVerts (both LDS).
+__device__ __forceinline__ void ss_push_body(const SsArgs &a, int32_t *pre, int32_t *list) {
     SsState *st = a.st;
-    if (st->done || st->mode != SS_RELAX || st->pull) return;  // uniform
-    __shared__ int32_t pre[kBsMaxSumWords + 1];
-    __shared__ int32_t list[kBsChunkVerts];
     __shared__ int32_t pre_s[kSsBlock], e0_s[kSsBlock], du_s[kSsBlock];
     __shared__ int32_t n_s;
     const int32_t phase = st->phase;
@@ -217,11 +218,9 @@ constexpr int32_t kSsPullWide = 4096;            // in-degree at least this: chu
 constexpr int32_t kSsPullChunk = 16 * 1024;      // in-edges per chunk (a workgroup's item)
 constexpr int kSsPasses = 4;                     // cooperative passes in flight per wave
 
-__global__ void __launch_bounds__(kSsBlock) ss_pull_kernel(SsArgs a) {
+// Pull relax step; `best_s` holds kSsBlock entries of LDS.
+__device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int32_t *best_s) {
     SsState *st = a.st;
-    if (st->done || st->mode != SS_RELAX || !st->pull) return;  // uniform
-    __shared__ int32_t pre[kBsMaxSumWords + 1];
-    __shared__ int32_t best_s[kSsBlock / 64][64];
     const int32_t phase = st->phase, hi = st->hi, lo = st->lo;
     const int32_t iin = phase % 3, iout = (phase + 1) % 3, iclr = (phase + 2) % 3;
     bs_clear_chunks(bs_pick3(a.bm[0], a.bm[1], a.bm[2], iclr), bs_pick3(a.sum[0], a.sum[1], a.sum[2], iclr),
@@ -229,7 +228,7 @@ __global__ void __launch_bounds__(kSsBlock) ss_pull_kernel(SsArgs a) {
     const uint64_t *bm_in = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iin);
     uint64_t *bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iout);
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    int32_t *best = best_s[wid];
+    int32_t *best = best_s + 64 * wid;
     int32_t nf = 0, fdeg = 0;
     const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
     for (int32_t un = blockIdx.x; un < units; un += gridDim.x) {
@@ -337,6 +336,19 @@ __global__ void __launch_bounds__(kSsBlock) ss_pull_kernel(SsArgs a) {
     bs_flush3<kSsBlock>(a.slots, kSsMaxGrid, nf, fdeg, 0);
 }
 
+// One relax step, push or pull as the decide kernels set `pull` (one launch
+// either way; the body not taken costs nothing).
+__global__ void __launch_bounds__(kSsBlock) ss_relax_kernel(SsArgs a) {
+    SsState *st = a.st;
+    if (st->done || st->mode != SS_RELAX) return;  // uniform
+    __shared__ int32_t pre[kBsMaxSumWords + 1];
+    __shared__ int32_t lds[kBsChunkVerts];
+    if (st->pull)
+        ss_pull_body(a, pre, lds);
+    else
+        ss_push_body(a, pre, lds);
+}
+
 // Grid-wide bin: the concatenated edges of the deferred hubs, spread evenly.
 __global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a) {
     SsState *st = a.st;
@@ -378,39 +390,58 @@ __global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a) {
     bs_flush3<kSsBlock>(a.slots + 3 * kSsMaxGrid, kSsMaxGrid, c.nf, c.fdeg, 0);
 }
 
-// Adds and zeroes counter `which` of slot region `region` over `slots` blocks.
-__device__ __forceinline__ long long ss_collect(int32_t *slot_base, int32_t region, int32_t which,
-                                                int32_t nslots) {
-    __shared__ long long red[kSsBlock / 64];
-    int32_t *p = slot_base + (region * 3 + which) * kSsMaxGrid;
-    long long t = 0;
-    for (int32_t i = threadIdx.x; i < nslots; i += kSsBlock) {
-        t += p[i];
-        p[i] = 0;
+// Sums rows r[0..N) of the slot array (row = region * 3 + counter) over
+// nslots workgroups in one pass, zeroing them for the next kernel.
+template <int N>
+__device__ __forceinline__ void ss_collect(int32_t *slot_base, const int32_t (&r)[N], int32_t nslots,
+                                          long long (&out)[N]) {
+    __shared__ long long red[N][kSsDecBlock / 64];
+    long long t[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) t[j] = 0;
+#pragma unroll 1
+    for (int32_t i = threadIdx.x; i < nslots; i += kSsDecBlock) {
+        int32_t x[N];
+#pragma unroll
+        for (int j = 0; j < N; ++j) x[j] = slot_base[r[j] * kSsMaxGrid + i];
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            t[j] += x[j];
+            slot_base[r[j] * kSsMaxGrid + i] = 0;
+        }
     }
-    for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
-    if (lane_id() == 0) red[threadIdx.x >> 6] = t;
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        for (int o = 32; o > 0; o >>= 1) t[j] += __shfl_xor(t[j], o);
+        if (lane_id() == 0) red[j][threadIdx.x >> 6] = t[j];
+    }
     __syncthreads();
-    long long s = 0;
-    for (int i = 0; i < kSsBlock / 64; ++i) s += red[i];
+    const int32_t lane = lane_id();
+#pragma unroll
+    for (int j = 0; j < N; ++j) {  // every wave reduces the per-wave sums
+        long long v = lane < kSsDecBlock / 64 ? red[j][lane] : 0;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        out[j] = v;
+    }
     __syncthreads();
-    return s;
 }
 
 __device__ __forceinline__ int32_t ss_window_end(int32_t lo, int32_t delta) {
     return lo > INT_MAX - delta ? INT_MAX : lo + delta;
 }
 
-__global__ void __launch_bounds__(kSsBlock) ss_decide_kernel(SsArgs a, int32_t nslots) {
+__global__ void __launch_bounds__(kSsDecBlock) ss_decide_kernel(SsArgs a, int32_t nslots) {
     SsState *st = a.st;
     if (st->done || st->mode != SS_RELAX) return;  // uniform
-    const long long nf = ss_collect(a.slots, 0, 0, nslots) + ss_collect(a.slots, 1, 0, nslots);
-    const long long fdeg = ss_collect(a.slots, 0, 1, nslots) + ss_collect(a.slots, 1, 1, nslots);
+    const int32_t rows[4] = {0, 3, 1, 4};  // |F'| and its out-edges from the relax and hub kernels
+    long long c[4];
+    ss_collect(a.slots, rows, nslots, c);
+    const long long nf = c[0] + c[1], fdeg = c[2] + c[3];
     const int32_t phase = st->phase;
     uint64_t *sclr = bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 2) % 3);
-    for (int32_t i = threadIdx.x; i <= a.sum_words; i += kSsBlock) sclr[i] = 0;
+    for (int32_t i = threadIdx.x; i <= a.sum_words; i += kSsDecBlock) sclr[i] = 0;
     // the summary of F', from the flags its chunks set
-    bs_build_summary<kSsBlock>(a.cflag, a.chunks, bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 1) % 3));
+    bs_build_summary<kSsDecBlock>(a.cflag, a.chunks, bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 1) % 3));
     if (threadIdx.x == 0) {
         st->phase = phase + 1;
         st->steps += 1;
@@ -483,19 +514,21 @@ __global__ void __launch_bounds__(kSsBlock) ss_farmove_kernel(SsArgs a) {
     bs_flush3<kSsBlock>(a.slots + 6 * kSsMaxGrid, kSsMaxGrid, moved, fdeg, 0);
 }
 
-__global__ void __launch_bounds__(kSsBlock) ss_decide2_kernel(SsArgs a, int32_t nslots) {
+__global__ void __launch_bounds__(kSsDecBlock) ss_decide2_kernel(SsArgs a, int32_t nslots) {
     SsState *st = a.st;
     if (st->done) return;
     if (threadIdx.x == 0) st->slots += 1;
     if (st->mode != SS_ADVANCE) return;  // uniform
-    const long long moved = ss_collect(a.slots, 2, 0, nslots);
-    const long long fdeg = ss_collect(a.slots, 2, 1, nslots);
+    const int32_t rows[2] = {6, 7};  // joined vertices and their out-edges
+    long long c[2];
+    ss_collect(a.slots, rows, nslots, c);
+    const long long moved = c[0], fdeg = c[1];
     // the summary of the window's F
-    bs_build_summary<kSsBlock>(a.cflag, a.chunks, bs_pick3(a.sum[0], a.sum[1], a.sum[2], st->phase % 3));
+    bs_build_summary<kSsDecBlock>(a.cflag, a.chunks, bs_pick3(a.sum[0], a.sum[1], a.sum[2], st->phase % 3));
     // least pending distance over the window scan's workgroups
-    __shared__ int32_t red[kSsBlock / 64];
+    __shared__ int32_t red[kSsDecBlock / 64];
     int32_t m = INT_MAX;
-    for (int32_t i = threadIdx.x; i < nslots; i += kSsBlock) {
+    for (int32_t i = threadIdx.x; i < nslots; i += kSsDecBlock) {
         const int32_t x = a.slots[kSsMinSlots * kSsMaxGrid + i];
         m = x < m ? x : m;
     }
@@ -506,7 +539,7 @@ __global__ void __launch_bounds__(kSsBlock) ss_decide2_kernel(SsArgs a, int32_t 
     if (lane_id() == 0) red[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int i = 0; i < kSsBlock / 64; ++i) m = red[i] < m ? red[i] : m;
+        for (int i = 0; i < kSsDecBlock / 64; ++i) m = red[i] < m ? red[i] : m;
         st->next_min = m;
         if (moved > 0) {
             st->mode = SS_RELAX;
@@ -584,6 +617,7 @@ struct SsWork {
     int32_t *host = nullptr;  // pinned: done, mode, phase, steps, windows, slots, pulls
     const int32_t *poff = nullptr, *pnbr = nullptr, *pwt = nullptr;
     int32_t pull_min = 0;
+    bool hubs = false;  // some out-degree reaches kSsHubMin: the hub kernel is launched
     int32_t wide_chunks = 0;
     int32_t *chunk_v = nullptr, *chunk_e0 = nullptr;
     hipStream_t cap = nullptr;
@@ -630,7 +664,9 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
         pg = t;
     }
     if (!pg->d_w) return fail(GI_ERR_INVALID, "SSSP needs edge weights");
+    GI_TRY(ensure_max_degree(g, nullptr));
     SsWork *w = new SsWork();
+    w->hubs = g->max_degree >= kSsHubMin;
     w->sum_words = (int32_t)sum_words;
     w->poff = pg->d_off;
     w->pnbr = pg->d_nbr;
@@ -685,7 +721,7 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
 static int ss_grid() {
     static int per_cu = -1;
     if (per_cu < 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ss_step_kernel, kSsBlock, 0) != hipSuccess ||
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ss_relax_kernel, kSsBlock, 0) != hipSuccess ||
             per_cu < 1)
             per_cu = 4;
     }
@@ -719,26 +755,13 @@ static SsArgs ss_args(gi_graph *g, SsWork *w, int32_t *dist) {
     return a;
 }
 
-static int ss_pull_grid() {
-    static int per_cu = -1;
-    if (per_cu < 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ss_pull_kernel, kSsBlock, 0) != hipSuccess ||
-            per_cu < 1)
-            per_cu = 4;
-    }
-    int g = device_info().num_cus * per_cu;
-    return g < kSsMaxGrid ? g : kSsMaxGrid;
-}
-
-static void launch_slot(const SsArgs &a, hipStream_t s) {
+static void launch_slot(const SsArgs &a, bool hubs, hipStream_t s) {
     const int grid = ss_grid();
-    const int pgrid = ss_pull_grid();
-    ss_step_kernel<<<grid, kSsBlock, 0, s>>>(a);
-    ss_pull_kernel<<<pgrid, kSsBlock, 0, s>>>(a);
-    ss_hub_kernel<<<grid, kSsBlock, 0, s>>>(a);
-    ss_decide_kernel<<<1, kSsBlock, 0, s>>>(a, grid > pgrid ? grid : pgrid);
+    ss_relax_kernel<<<grid, kSsBlock, 0, s>>>(a);
+    if (hubs) ss_hub_kernel<<<grid, kSsBlock, 0, s>>>(a);
+    ss_decide_kernel<<<1, kSsDecBlock, 0, s>>>(a, grid);
     ss_farmove_kernel<<<grid, kSsBlock, 0, s>>>(a);
-    ss_decide2_kernel<<<1, kSsBlock, 0, s>>>(a, grid);
+    ss_decide2_kernel<<<1, kSsDecBlock, 0, s>>>(a, grid);
 }
 
 int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,
@@ -762,7 +785,7 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
             }
             hipGraph_t graph = nullptr;
             GI_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
-            for (int32_t r = 0; r < block; ++r) launch_slot(a, w->cap);
+            for (int32_t r = 0; r < block; ++r) launch_slot(a, w->hubs, w->cap);
             hipError_t e = hipStreamEndCapture(w->cap, &graph);
             if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
             hipGraphExec_t exec = nullptr;
@@ -790,8 +813,9 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
     return fail(GI_ERR_RANGE, "SSSP did not finish");
 }
 
-// Eager slot loop for measurement: the six launches of every slot bracketed
-// by HIP events; the state is read before each slot.
+// Eager slot loop for measurement: the launches of every slot bracketed by
+// HIP events; the state is read before each slot. The relax launch is
+// reported as a push or a pull step by the state's pull flag.
 static int sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_min, int32_t *d_dist,
                       int32_t max_slots, float *kernel_ms, int64_t *info, int32_t *num_slots, hipStream_t s) {
     SsWork *w = nullptr;
@@ -801,11 +825,11 @@ static int sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_m
     int rg = (int)ceil_div(n, 256);
     ss_reset_kernel<<<rg < 8192 ? rg : 8192, 256, 0, s>>>(a, SsSeed{source, delta, pull_min >= 0 ? pull_min : w->pull_min});
     GI_LAUNCH_CHECK();
-    hipEvent_t ev[7] = {};
+    hipEvent_t ev[6] = {};
     int rc = GI_OK;
-    for (int i = 0; i < 7 && rc == GI_OK; ++i)
+    for (int i = 0; i < 6 && rc == GI_OK; ++i)
         if (hipEventCreate(&ev[i]) != hipSuccess) rc = fail(GI_ERR_HIP, "event create");
-    const int grid = ss_grid(), pgrid = ss_pull_grid();
+    const int grid = ss_grid();
     SsState h{};
     int32_t k = 0;
     for (; rc == GI_OK && k < max_slots; ++k) {
@@ -821,23 +845,27 @@ static int sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_m
         info[5 * k + 3] = h.hi;
         info[5 * k + 4] = h.fdeg;
         (void)hipEventRecord(ev[0], s);
-        ss_step_kernel<<<grid, kSsBlock, 0, s>>>(a);
+        ss_relax_kernel<<<grid, kSsBlock, 0, s>>>(a);
         (void)hipEventRecord(ev[1], s);
-        ss_pull_kernel<<<pgrid, kSsBlock, 0, s>>>(a);
+        if (w->hubs) ss_hub_kernel<<<grid, kSsBlock, 0, s>>>(a);
         (void)hipEventRecord(ev[2], s);
-        ss_hub_kernel<<<grid, kSsBlock, 0, s>>>(a);
+        ss_decide_kernel<<<1, kSsDecBlock, 0, s>>>(a, grid);
         (void)hipEventRecord(ev[3], s);
-        ss_decide_kernel<<<1, kSsBlock, 0, s>>>(a, grid > pgrid ? grid : pgrid);
-        (void)hipEventRecord(ev[4], s);
         ss_farmove_kernel<<<grid, kSsBlock, 0, s>>>(a);
+        (void)hipEventRecord(ev[4], s);
+        ss_decide2_kernel<<<1, kSsDecBlock, 0, s>>>(a, grid);
         (void)hipEventRecord(ev[5], s);
-        ss_decide2_kernel<<<1, kSsBlock, 0, s>>>(a, grid);
-        (void)hipEventRecord(ev[6], s);
-        if (hipGetLastError() != hipSuccess || hipEventSynchronize(ev[6]) != hipSuccess) {
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(ev[5]) != hipSuccess) {
             rc = fail(GI_ERR_HIP, "slot launch");
             break;
         }
-        for (int i = 0; i < 6; ++i) (void)hipEventElapsedTime(&kernel_ms[6 * k + i], ev[i], ev[i + 1]);
+        float t[5];
+        for (int i = 0; i < 5; ++i) (void)hipEventElapsedTime(&t[i], ev[i], ev[i + 1]);
+        const bool pulled = h.mode == SS_RELAX && h.pull;
+        float *row = kernel_ms + 6 * k;
+        row[0] = pulled ? 0.f : t[0];
+        row[1] = pulled ? t[0] : 0.f;
+        for (int i = 1; i < 5; ++i) row[i + 1] = t[i];
     }
     for (auto e : ev)
         if (e) (void)hipEventDestroy(e);

@@ -447,9 +447,11 @@ int gi_sssp(gi_graph *g, int32_t source, int32_t delta, const gi_schedule *s,
             int32_t *d_dist, gi_run_stats *stats, void *stream);
 /* Measurement hook of the fused SSSP engine (no reference counterpart):
  * runs the slots of the device-side window loop one at a time, uncaptured,
- * and times each of the six launches of a slot with HIP events. pull_min:
- * |F| above which a relax step pulls (< 0: the engine's default).
- * kernel_ms[max_slots][6]: push step, pull step, hub bin, decide, window
+ * and times each launch of a slot with HIP events. pull_min: |F| above
+ * which a relax step pulls (< 0: the engine's default).
+ * kernel_ms[max_slots][6]: push step, pull step (one relax launch, counted
+ * in the column of its direction, 0 in the other), hub bin (0 for a graph
+ * whose out-degrees stay below the hub width: not launched), decide, window
  * scan, decide2. info[max_slots][5]: the state before the slot (mode: 0
  * relax / 1 advance, pull flag, |F|, window end hi, out-edges of F).
  * Synchronises. */
