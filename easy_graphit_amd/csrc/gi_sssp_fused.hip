@@ -10,9 +10,13 @@
 // BFS). Vertices improved to a distance >= hi are pending: they are not kept
 // in a set of their own (the reference's priority queue,
 // gpu_priority_queue.h), because a vertex is pending exactly when
-// hi <= dist < INT_MAX; a window advance finds them with one coalesced scan
-// of dist, so relaxations into later windows cost no atomics beyond the
-// atomicMin on dist.
+// hi <= dist < INT_MAX. Instead every unit of 1024 vertices keeps a lower
+// bound of its pending distances (umin, lowered with atomicMin when a
+// relaxation leaves a vertex pending; most relaxations find the bound
+// already lower and skip the atomic). A window advance scans dist only in
+// the units whose bound falls inside the new window, and rewrites their
+// bounds exactly; the other units are skipped, so later windows read a small
+// part of dist instead of all of it.
 // A captured slot is five launches, each a no-op unless its mode is active:
 //   ss_relax   push: relax the out-edges of F, spread evenly over each
 //              workgroup (vertices of at least kSsHubMin edges go to the
@@ -22,7 +26,8 @@
 //   ss_hub     the grid-wide bin: edges of very wide frontier vertices (not
 //              launched for a graph without such vertices);
 //   ss_decide  count F'; empty -> advance to [hi, hi + delta);
-//   ss_farmove scan dist: [lo, hi) joins F; also the least distance >= hi;
+//   ss_farmove scan dist in the units whose bound is below hi: [lo, hi)
+//              joins F; also the least pending distance >= hi;
 //   ss_decide2 count the joins; none -> jump to the window that starts at
 //              that least distance (gpu_priority_queue.h:126-146 starts a
 //              window at the minimum remaining priority too), or done when
@@ -43,6 +48,8 @@ constexpr int32_t kSsMaxHubs = 1024;
 constexpr int32_t kSsMaxGrid = 8192;  // partial-sum slots per kernel (one per workgroup)
 constexpr int32_t kSsPullDiv = 8;     // a step pulls when |F| > V / kSsPullDiv
 constexpr int32_t kSsMinSlots = 9;    // slot row of the window scan's least pending distances
+
+constexpr int kSsPullUnit = 4 * (kSsBlock / 64);  // words per unit (pull work, window scan, umin)
 
 enum SsMode : int32_t { SS_RELAX = 0, SS_ADVANCE = 1 };
 
@@ -81,6 +88,7 @@ struct SsArgs {
     uint64_t *bm[3];
     uint64_t *sum[3];
     uint8_t *cflag;      // chunk flags of the bitmap being filled (bs_flag_chunk)
+    int32_t *umin;       // per unit of kSsPullUnit words: lower bound of its pending distances
     int32_t chunks;
     int32_t *slots;      // [3 kernels][3 counters][kSsMaxGrid], then row kSsMinSlots
     int32_t wide_chunks;       // pull: chunks of the in-lists of wide vertices
@@ -100,7 +108,12 @@ struct SsCtx {
 // hi leaves the vertex pending.
 __device__ __forceinline__ void ss_claim(const SsArgs &a, SsCtx &c, int32_t v, int32_t nd) {
     const int32_t old = atomicMin(&a.dist[v], nd);
-    if (nd >= old || nd >= c.hi) return;
+    if (nd >= old) return;
+    if (nd >= c.hi) {  // pending: keep its unit's bound at or below nd
+        int32_t *m = &a.umin[v / (64 * kSsPullUnit)];
+        if (nd < *m) atomicMin(m, nd);
+        return;
+    }
     const uint64_t bit = 1ull << (v & 63);
     const unsigned long long o = atomicOr((unsigned long long *)&c.bm_out[v >> 6], bit);
     if (!(o & bit)) {
@@ -213,7 +226,6 @@ __device__ __forceinline__ void ss_push_body(const SsArgs &a, int32_t *pre, int3
 // (no atomics on dist: a vertex's word of 64 vertices belongs to one wave).
 // Work is in units of 16 words per workgroup (4 per wave), so a unit's
 // summary bit costs at most one atomic.
-constexpr int kSsPullUnit = 4 * (kSsBlock / 64);
 constexpr int32_t kSsPullWide = 4096;            // in-degree at least this: chunked pull
 constexpr int32_t kSsPullChunk = 16 * 1024;      // in-edges per chunk (a workgroup's item)
 constexpr int kSsPasses = 4;                     // cooperative passes in flight per wave
@@ -284,6 +296,12 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int3
             const bool imp = open && nd < dv;
             if (imp) a.dist[v] = nd;
             const bool join = imp && nd < hi;
+            int32_t pend = imp && !join ? nd : INT_MAX;  // least pending result of the word
+            for (int o = 32; o > 0; o >>= 1) {
+                const int32_t y = __shfl_xor(pend, o);
+                pend = y < pend ? y : pend;
+            }
+            if (lane == 0 && pend < a.umin[un]) atomicMin(&a.umin[un], pend);
             const uint64_t near = __ballot(join);
             uint64_t old = 0;
             if (lane == 0 && near) {  // atomic: a wide vertex of the same word may join too
@@ -322,6 +340,10 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int3
         }
         if (lane == 0 && m < dv) {
             const int32_t old = atomicMin(&a.dist[v], m);
+            if (m < old && m >= hi) {
+                int32_t *um = &a.umin[v / (64 * kSsPullUnit)];
+                if (m < *um) atomicMin(um, m);
+            }
             if (m < old && m < hi) {
                 const uint64_t bit = 1ull << (v & 63);
                 const unsigned long long o = atomicOr((unsigned long long *)&bm_out[v >> 6], bit);
@@ -459,22 +481,31 @@ __global__ void __launch_bounds__(kSsDecBlock) ss_decide_kernel(SsArgs a, int32_
     }
 }
 
-// Window scan: every vertex with lo <= dist < hi joins F (each wave owns its
-// words: plain stores into the empty bitmap); the least distance >= hi of a
-// reached vertex goes to next_min; the out-edges of the joined vertices are
-// counted. Units of 16 words per workgroup (4 per
-// wave, loads in flight together); a unit's summary bit costs one atomic.
+// Window scan over the units whose pending bound is below hi: every vertex
+// with lo <= dist < hi joins F (each wave owns its words: plain stores into
+// the empty bitmap), the unit's bound becomes its least distance >= hi, and
+// the out-edges of the joined vertices are counted. A unit whose bound is at
+// least hi holds nothing below hi and is skipped. The least bound over all
+// units (the least pending distance) goes to the workgroup's min slot. Units
+// of 16 words per workgroup (4 per wave, loads in flight together); a unit's
+// chunk flag costs one store.
 __global__ void __launch_bounds__(kSsBlock) ss_farmove_kernel(SsArgs a) {
     SsState *st = a.st;
     if (st->done || st->mode != SS_ADVANCE) return;
-    __shared__ int32_t red[kSsBlock / 64];
+    __shared__ int32_t ured[2][kSsBlock / 64];
     const int32_t lo = st->lo, hi = st->hi, phase = st->phase;
     uint64_t *f = bs_pick3(a.bm[0], a.bm[1], a.bm[2], phase % 3);  // empty: the last step found nothing
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
     int32_t moved = 0, fdeg = 0;
-    int32_t beyond = INT_MAX;
+    int32_t beyond = INT_MAX;  // thread 0: least bound over this workgroup's units
+    int32_t scanned = 0;       // block-uniform: LDS buffer of the unit reduction
     const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
     for (int32_t un = blockIdx.x; un < units; un += gridDim.x) {
+        const int32_t um = a.umin[un];  // block-uniform
+        if (um >= hi) {
+            beyond = um < beyond ? um : beyond;
+            continue;
+        }
         const int32_t w0 = un * kSsPullUnit + wid * 4;
         int32_t dv[4];
 #pragma unroll
@@ -483,13 +514,14 @@ __global__ void __launch_bounds__(kSsBlock) ss_farmove_kernel(SsArgs a) {
             dv[i] = v < a.V ? a.dist[v] : INT_MAX;
         }
         bool any = false;
+        int32_t ub = INT_MAX;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int32_t v = (w0 + i) * 64 + lane;
             const bool in = dv[i] >= lo && dv[i] < hi;  // dv is INT_MAX past V
             const uint64_t m = __ballot(in);
             if (in) fdeg += a.off[v + 1] - a.off[v];
-            if (dv[i] >= hi && dv[i] < beyond) beyond = dv[i];
+            if (dv[i] >= hi && dv[i] < ub) ub = dv[i];
             if (m) {
                 any = true;
                 if (lane == 0) {
@@ -498,19 +530,23 @@ __global__ void __launch_bounds__(kSsBlock) ss_farmove_kernel(SsArgs a) {
                 }
             }
         }
+        for (int o = 32; o > 0; o >>= 1) {
+            const int32_t y = __shfl_xor(ub, o);
+            ub = y < ub ? y : ub;
+        }
+        int32_t *r = ured[scanned & 1];
+        if (lane == 0) r[wid] = ub;
+        ++scanned;
         if (__syncthreads_or(any) && threadIdx.x == 0) bs_flag_chunk(a.cflag, un * kSsPullUnit);
+        if (threadIdx.x == 0) {  // the unit's exact bound
+            int32_t m = r[0];
+            for (int i = 1; i < kSsBlock / 64; ++i) m = r[i] < m ? r[i] : m;
+            a.umin[un] = m;
+            beyond = m < beyond ? m : beyond;
+        }
     }
-    for (int o = 32; o > 0; o >>= 1) {
-        const int32_t y = __shfl_xor(beyond, o);
-        beyond = y < beyond ? y : beyond;
-    }
-    if (lane == 0) red[wid] = beyond;
-    __syncthreads();
-    if (threadIdx.x == 0) {  // one slot per workgroup: no atomics on a single word
-        int32_t m = red[0];
-        for (int i = 1; i < kSsBlock / 64; ++i) m = red[i] < m ? red[i] : m;
-        a.slots[kSsMinSlots * kSsMaxGrid + blockIdx.x] = m;
-    }
+    if (threadIdx.x == 0)  // one slot per workgroup: no atomics on a single word
+        a.slots[kSsMinSlots * kSsMaxGrid + blockIdx.x] = beyond;
     bs_flush3<kSsBlock>(a.slots + 6 * kSsMaxGrid, kSsMaxGrid, moved, fdeg, 0);
 }
 
@@ -565,8 +601,10 @@ struct SsSeed {
 
 __global__ void ss_reset_kernel(SsArgs a, SsSeed sd) {
     const int32_t n = a.V > a.sum_words + 1 ? a.V : a.sum_words + 1;
+    const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (i < a.V) a.dist[i] = i == sd.source ? 0 : INT_MAX;
+        if (i < units) a.umin[i] = INT_MAX;  // nothing pending
         if (i < a.words) {
             const uint64_t s = (i == (sd.source >> 6)) ? (1ull << (sd.source & 63)) : 0ull;
             a.bm[0][i] = s;
@@ -612,6 +650,7 @@ struct SsWork {
     uint64_t *sum[3] = {nullptr, nullptr, nullptr};
     uint8_t *cflag = nullptr;
     int32_t chunks = 0;
+    int32_t *umin = nullptr;
     int32_t *slots = nullptr;
     int32_t sum_words = 0;
     int32_t *host = nullptr;  // pinned: done, mode, phase, steps, windows, slots, pulls
@@ -635,6 +674,7 @@ static void free_ss(SsWork *w) {
     for (auto p : w->sum) dfree(p);
     dfree(w->slots);
     dfree(w->cflag);
+    dfree(w->umin);
     dfree(w->chunk_v);
     dfree(w->chunk_e0);
     if (w->host) (void)hipHostFree(w->host);
@@ -676,6 +716,7 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->bm[i], (size_t)words + 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->sum[i], (size_t)sum_words + 1);
     if (rc == GI_OK) rc = dalloc(&w->slots, (kSsMinSlots + 1) * (size_t)kSsMaxGrid);
+    if (rc == GI_OK) rc = dalloc(&w->umin, (size_t)ceil_div(words, kSsPullUnit) + 1);
     w->chunks = (int32_t)ceil_div(words, kBsChunkWords);
     const size_t flag_bytes = (size_t)bs_flag_bytes(w->chunks);
     if (rc == GI_OK) rc = dalloc(&w->cflag, flag_bytes);
@@ -748,6 +789,7 @@ static SsArgs ss_args(gi_graph *g, SsWork *w, int32_t *dist) {
     }
     a.slots = w->slots;
     a.cflag = w->cflag;
+    a.umin = w->umin;
     a.chunks = w->chunks;
     a.wide_chunks = w->wide_chunks;
     a.chunk_v = w->chunk_v;

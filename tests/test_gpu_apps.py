@@ -58,6 +58,37 @@ def test_sssp_fused_grid_and_unreachable(gi):
     assert np.array_equal(d2, O.sssp(h2, 0)) and d2[3] == np.iinfo(np.int32).max
 
 
+def test_sssp_fused_chain_unit_bounds(gi):
+    # a weighted chain with long edges: each window holds a few consecutive
+    # vertices, so the window scan skips most 1024-vertex units; long edges
+    # leave far vertices pending at distances that later improve (stale unit
+    # bounds), and some long edges are shortcuts
+    n = 12000
+    rng = np.random.default_rng(5)
+    a = np.arange(n - 1)
+    la = rng.integers(0, n, 200)
+    lb = rng.integers(0, n, 200)
+    src = np.concatenate([a, la]).astype(np.int32)
+    dst = np.concatenate([a + 1, lb]).astype(np.int32)
+    w = np.concatenate([rng.integers(1, 11, n - 1), rng.integers(1, 30000, 200)]).astype(np.int32)
+    # symmetric, no self-loops, one entry per neighbour with its least weight
+    s2, d2, w2 = np.concatenate([src, dst]), np.concatenate([dst, src]), np.concatenate([w, w])
+    keep = s2 != d2
+    s2, d2, w2 = s2[keep], d2[keep], w2[keep]
+    order = np.lexsort((w2, d2, s2))
+    s2, d2, w2 = s2[order], d2[order], w2[order]
+    first = np.ones(len(s2), bool)
+    first[1:] = (s2[1:] != s2[:-1]) | (d2[1:] != d2[:-1])
+    s2, nb, wt = s2[first], d2[first].astype(np.int32), w2[first].astype(np.int32)
+    off = np.concatenate([[0], np.cumsum(np.bincount(s2, minlength=n))]).astype(np.int32)
+    g = gi.Graph.from_csr(off, nb, wt)
+    want = O.sssp(O.CSR(off, nb, wt), 0)
+    for delta in (8, 64):
+        for _ in range(2):
+            d, _ = gi.sssp(g, 0, delta, _fused_sssp(gi))
+            assert np.array_equal(d, want), delta
+
+
 def test_sssp_fused_skips_empty_windows(gi):
     # distances 0, 3, 503, 505 with delta 4: after [0, 4) the next window
     # must start at 503, the least pending distance
