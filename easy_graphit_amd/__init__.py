@@ -221,6 +221,7 @@ def lib():
         "gi_pagerank": (C.c_int, [vp, i32, C.c_float, C.POINTER(Schedule), vp,
                                   C.POINTER(RunStats), vp]),
         "gi_bc": (C.c_int, [vp, vp, i32, i32, vp, C.POINTER(RunStats), vp]),
+        "gi_bc_schedule": (C.c_int, [vp, vp, i32, vp, vp, C.POINTER(RunStats), vp]),
     }
     for name, (res, args) in sig.items():
         if hasattr(L, name):
@@ -770,10 +771,20 @@ def pagerank(g: Graph, iters: int = 20, damping: float = PR_DAMP,
     return rank.numpy(stream), st.as_dict()
 
 
-def bc(g: Graph, sources, batch: int = 0, stream=None):
+def bc(g: Graph, sources, batch: int = 0, schedule: Schedule | None = None, stream=None):
+    """Brandes BC summed over `sources`. batch > 1: batches of sources on the
+    device-driven engine; otherwise one source at a time, by `schedule` when
+    given (kernel_fusion=1: the fused single-source mode), else the app's
+    unfused push TWCE loop."""
     n = g.num_vertices
     src = np.ascontiguousarray(sources, np.int32)
     scores = DeviceArray(n, np.float32)
     st = RunStats()
-    check(lib().gi_bc(g.h, src.ctypes.data, len(src), batch, scores.ptr, C.byref(st), stream))
+    if schedule is not None:
+        if batch > 1:
+            raise ValueError("pass either batch > 1 or a schedule")
+        check(lib().gi_bc_schedule(g.h, src.ctypes.data, len(src), C.byref(schedule), scores.ptr,
+                                   C.byref(st), stream))
+    else:
+        check(lib().gi_bc(g.h, src.ctypes.data, len(src), batch, scores.ptr, C.byref(st), stream))
     return scores.numpy(stream), st.as_dict()

@@ -695,40 +695,39 @@ int gi_pagerank(gi_graph *g, int32_t iters, float damping, const gi_schedule *sc
     return GI_OK;
 }
 
-int gi_bc(gi_graph *g, const int32_t *h_sources, int32_t num_sources, int32_t batch,
-          float *d_scores, gi_run_stats *stats, void *stream) {
+static int bc_check_args(gi_graph *g, const int32_t *h_sources, int32_t num_sources, float *d_scores) {
     if (!g || !d_scores || (num_sources > 0 && !h_sources)) return fail(GI_ERR_INVALID, "null argument");
+    if (!g->d_off) return fail(GI_ERR_UNSUPPORTED, "pass the base graph, not a blocked view");
     for (int32_t i = 0; i < num_sources; ++i)
         if (h_sources[i] < 0 || h_sources[i] >= g->V) return fail(GI_ERR_RANGE, "source %d out of range", h_sources[i]);
-    if (batch > 1) {
-        // batched: groups of up to 256 sources advance level-synchronously
-        hipStream_t bs = (hipStream_t)stream;
-        if (batch > 256) batch = 256;
-        EventTimer timer(bs);
-        GI_TRY(fill_f32(d_scores, g->V, 0.f, bs));
-        int32_t total_levels = 0;
-        for (int32_t i = 0; i < num_sources; i += batch) {
-            int32_t nb = num_sources - i < batch ? num_sources - i : batch;
-            int32_t lv = 0;
-            GI_TRY(bc_batch(g, h_sources + i, nb, d_scores, bs, &lv));
-            total_levels += lv;
-        }
-        double ms = timer.stop();
-        if (stats) {
-            stats->iterations = num_sources;
-            stats->push_iterations = total_levels;
-            stats->pull_iterations = 0;
-            stats->ms = ms;
-            stats->reached = 0;
-            stats->edges_traversed = 0;
-        }
-        return GI_OK;
+    return GI_OK;
+}
+
+// Sources in batches on the device-driven engine (batch 1: fused single
+// source); d_scores zeroed first.
+static int bc_on_device(gi_graph *g, const int32_t *h_sources, int32_t num_sources, int32_t batch,
+                        float *d_scores, gi_run_stats *stats, hipStream_t s) {
+    EventTimer timer(s);
+    GI_TRY(fill_f32(d_scores, g->V, 0.f, s));
+    int64_t levels = 0;
+    GI_TRY(bc_batched(g, h_sources, num_sources, batch, d_scores, s, &levels));
+    double ms = timer.stop();
+    if (stats) {
+        stats->iterations = num_sources;
+        stats->push_iterations = levels;
+        stats->pull_iterations = 0;
+        stats->ms = ms;
+        stats->reached = 0;
+        stats->edges_traversed = 0;
     }
-    hipStream_t s = (hipStream_t)stream;
-    gi_schedule sched;  // apps/bc.cpp:235-237
-    gi_schedule_init(&sched);
-    sched.direction = GI_PUSH;
-    sched.load_balancing = GI_LB_TWCE;
+    return GI_OK;
+}
+
+// One source at a time with a host round trip per level and per list
+// operation, as the app runs unfused (apps/bc.cpp:64-120): push steps with
+// the schedule's load balancing.
+static int bc_unfused(gi_graph *g, const int32_t *h_sources, int32_t num_sources, const gi_schedule &sched,
+                      float *d_scores, gi_run_stats *stats, hipStream_t s) {
     double *paths = nullptr;
     float *dep = nullptr;
     uint8_t *visited = nullptr;
@@ -808,6 +807,34 @@ int gi_bc(gi_graph *g, const int32_t *h_sources, int32_t num_sources, int32_t ba
         stats->edges_traversed = 0;
     }
     return GI_OK;
+}
+
+int gi_bc(gi_graph *g, const int32_t *h_sources, int32_t num_sources, int32_t batch,
+          float *d_scores, gi_run_stats *stats, void *stream) {
+    GI_TRY(bc_check_args(g, h_sources, num_sources, d_scores));
+    hipStream_t s = (hipStream_t)stream;
+    if (batch > 1) return bc_on_device(g, h_sources, num_sources, batch, d_scores, stats, s);
+    gi_schedule sched;  // the app's schedule, apps/bc.cpp:149-151
+    gi_schedule_init(&sched);
+    sched.direction = GI_PUSH;
+    sched.load_balancing = GI_LB_TWCE;
+    return bc_unfused(g, h_sources, num_sources, sched, d_scores, stats, s);
+}
+
+int gi_bc_schedule(gi_graph *g, const int32_t *h_sources, int32_t num_sources, const gi_schedule *sched,
+                   float *d_scores, gi_run_stats *stats, void *stream) {
+    GI_TRY(bc_check_args(g, h_sources, num_sources, d_scores));
+    hipStream_t s = (hipStream_t)stream;
+    gi_schedule def;
+    gi_schedule_init(&def);
+    def.direction = GI_PUSH;
+    def.load_balancing = GI_LB_TWCE;
+    const gi_schedule &sc = sched ? *sched : def;
+    if (sc.kernel_fusion == GI_FUSION_ENABLED) return bc_on_device(g, h_sources, num_sources, 1, d_scores, stats, s);
+    if (sc.direction != GI_PUSH)
+        return fail(GI_ERR_UNSUPPORTED, "unfused BC runs push steps; pulled levels need kernel_fusion");
+    GI_TRY(validate_schedule(&sc));
+    return bc_unfused(g, h_sources, num_sources, sc, d_scores, stats, s);
 }
 
 }  // extern "C"

@@ -1,30 +1,44 @@
 // Batched Brandes betweenness: up to 256 sources advance level-synchronously
 // in one set of launches per level (apps/bc.cpp:150-209 computes one source
-// per run, with a host round trip per level and per list operation).
+// per run, with a host round trip per level and per list operation). With
+// one source per batch this is the fused single-source mode (fuse_kernel,
+// apps/bc.cpp:92-113): the level loops run on the device.
 //
 // Layout (source-major, B sources, V vertices):
 //   depth  int32 [B][V]   level of v from source b (-1 unreached)
 //   sigma  f64   [B][V]   shortest-path counts (apps/bc.cpp:20 uses double)
 //   dep    f32   [B][V]   D(v) of the backward recurrence (apps/bc.cpp:21 float)
-//   F[l]   u64   [B][V/64] frontier bitmap of every level (kept for the
-//                          backward pass, one 64-bit word per wave of vertices)
-//   visited u64  [B][V/64]
+//   bm[3]  u64   [B][V/64] rotating frontier bitmaps: F[l] = bm[l % 3]
+//   snd    u64   [B][V/64] senders of a backward step (level l + 1)
+//   visited u64  [B][V/64] levels before the current one
 // Forward level l: for each (b,u) in F[l]: for each neighbour v not seen by
 // b: claim v in F[l+1][b] (first claimer records depth) and add sigma[b][u]
-// into sigma[b][v]. Backward level l (deepest first): for each (b,w) in
-// F[l+1]: add dep[b][w] into dep[b][v] for neighbours with 0 <= depth <= l
-// (the reference's "not yet processed" filter, apps/bc.cpp:193-200), then
-// dep += 1/sigma over F[l]. Score: sum_b (dep - 1/sigma) * sigma where
+// into sigma[b][v]; F[l-1] is cleared meanwhile for reuse as F[l+2].
+// Backward level l (deepest first): dep += 1/sigma over level l + 1, whose
+// bitmap is rebuilt from depth (no bitmap per level is kept, so memory does
+// not grow with the number of levels); then for each (b,w) at level l + 1:
+// add dep[b][w] into dep[b][v] for neighbours with 0 <= depth <= l (the
+// reference's "not yet processed" filter, apps/bc.cpp:193-200); a last pass
+// adds 1/sigma over level 0. Score: sum_b (dep - 1/sigma) * sigma where
 // sigma != 0 (apps/bc.cpp:58-63), the source keeping its own term.
 // Push steps extract (b,u) pairs block-locally from 64-word chunks of the
 // bitmaps through one atomic chunk cursor per step (chunks differ widely in
 // work; dealing them out dynamically beats a fixed round-robin). On symmetric
-// graphs a
-// level can instead be pulled (the reference's hybrid BC schedule, push /
-// pull, apps/bc.cpp:131-147): each wavefront owns one 64-vertex word of one
-// source and reads its neighbours' values, so sigma, depth and D take no
-// atomics. The host picks the direction per level from the level sizes it
-// reads back anyway.
+// graphs a level can instead be pulled (the reference's hybrid BC schedule,
+// push / pull, apps/bc.cpp:131-147): each wavefront owns one 64-vertex word
+// of one source and reads its neighbours' values, so sigma, depth and D take
+// no atomics.
+// Control: the state (level, direction, counts) lives on the device; a
+// forward slot is a step and a decide launch, a backward slot a level, a step
+// and a decide launch; slots are captured as hipGraphs and replayed in
+// blocks, the host reading the state once per block. Direction rules, from
+// level sizes summed over the batch: pull a forward level whose frontier
+// holds more than V/16 vertices per source, and a backward level whose
+// receiving levels are under 1/64 of the level that sends; push otherwise,
+// and on directed graphs (measured on RMAT-20, 256 sources: 285 ms all
+// pushed, 218 ms with these rules).
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include "gi_lb.h"
@@ -36,22 +50,46 @@ constexpr int kBcChunkWords = 64;
 constexpr int kBcChunkVerts = kBcChunkWords * 64;
 constexpr int kBcMaxBatch = 256;
 
+enum BcPhase : int32_t { BC_FWD = 0, BC_BWD = 1, BC_DONE = 2 };
+
+struct BcState {
+    int32_t phase;
+    int32_t level;       // forward: the level expanded; backward: l (senders at l + 1)
+    int32_t levels;      // L, set when the forward pass ends
+    int32_t pull;        // forward: the next step pulls
+    int32_t cursor;      // chunk cursor of push steps
+    int32_t pad;
+    unsigned long long count;  // forward: |F[l+1]|; backward: |level l+1|
+    long long remaining;       // vertices at levels not yet sent from (backward)
+};
+
 struct BcArgs {
     const int32_t *off;
     const int32_t *nbr;
     int32_t V;
     int32_t words;       // bitmap words per source
     int32_t B;
-    int32_t level;
+    int32_t can_pull;    // symmetric graph: in-neighbours are the neighbours
+    BcState *st;
     int32_t *depth;
     double *sigma;
     float *dep;
     uint64_t *visited;
-    const uint64_t *fin;  // F[level]
-    uint64_t *fout;       // F[level+1] (forward) / unused
-    int32_t *cursor;      // chunk cursor (push steps)
-    unsigned long long *count;  // next-frontier size (forward)
+    uint64_t *bm[3];
+    uint64_t *snd;
+    int32_t *cursor;            // &st->cursor
+    unsigned long long *count;  // &st->count
+    // bound by each step from the state
+    int32_t level;
+    const uint64_t *fin;  // forward: F[level]; backward: the senders
+    uint64_t *fout;       // forward: F[level+1]
 };
+
+// Pointers passed by value: selecting through a reference to the kernel's
+// argument copy puts the whole copy in scratch memory.
+__device__ __forceinline__ uint64_t *bc_pick3(uint64_t *b0, uint64_t *b1, uint64_t *b2, int32_t i) {
+    return i == 0 ? b0 : (i == 1 ? b1 : b2);
+}
 
 // Block-local extraction of one chunk (64 words of one source) into LDS.
 // Returns the number of members; wave 0 does the work.
@@ -96,8 +134,7 @@ __device__ __forceinline__ void bc_forward_edge(const BcArgs &a, int32_t b, doub
     atomicAdd(&a.sigma[(int64_t)b * a.V + v], su);
 }
 
-__global__ void __launch_bounds__(kBcBlock) bc_forward_kernel(BcArgs a) {
-    __shared__ int32_t list[kBcChunkVerts];
+__device__ __forceinline__ void bc_forward_push(const BcArgs &a, int32_t *list) {
     __shared__ int32_t chunk_s, n_s;
     const int32_t cps = (a.words + kBcChunkWords - 1) / kBcChunkWords;  // chunks per source
     const int32_t nchunks = cps * a.B;
@@ -139,10 +176,9 @@ __global__ void __launch_bounds__(kBcBlock) bc_forward_kernel(BcArgs a) {
     (void)wid;
 }
 
-// Backward push from F[level+1] (passed as fin) into neighbours at depth
+// Backward push from the senders (fin, level + 1) into neighbours at depth
 // 0..level of the same source.
-__global__ void __launch_bounds__(kBcBlock) bc_backward_kernel(BcArgs a) {
-    __shared__ int32_t list[kBcChunkVerts];
+__device__ __forceinline__ void bc_backward_push(const BcArgs &a, int32_t *list) {
     __shared__ int32_t chunk_s, n_s;
     const int32_t cps = (a.words + kBcChunkWords - 1) / kBcChunkWords;
     const int32_t nchunks = cps * a.B;
@@ -221,7 +257,7 @@ __device__ __forceinline__ T bc_pull_sum(const BcArgs &a, const uint64_t *bits, 
 // sigma over its neighbours in F[level] of b (forward_update, apps/bc.cpp:33-35,
 // pulled). sigma holds integer path counts, so the order of the sum does not
 // change it. F[level] joins the seen set here.
-__global__ void __launch_bounds__(kBcBlock) bc_forward_pull_kernel(BcArgs a) {
+__device__ __forceinline__ void bc_forward_pull(const BcArgs &a) {
     const int32_t lane = lane_id();
     const int64_t units = (int64_t)a.B * a.words;
     const int64_t waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -253,9 +289,9 @@ __global__ void __launch_bounds__(kBcBlock) bc_forward_pull_kernel(BcArgs a) {
 }
 
 // Pull backward level: every vertex at depth 0..level of source b adds the
-// dependencies of its neighbours in F[level+1] (backward_update,
+// dependencies of its neighbours among the senders (backward_update,
 // apps/bc.cpp:52-54, pulled; the same "not yet processed" set).
-__global__ void __launch_bounds__(kBcBlock) bc_backward_pull_kernel(BcArgs a) {
+__device__ __forceinline__ void bc_backward_pull(const BcArgs &a) {
     const int32_t lane = lane_id();
     const int64_t units = (int64_t)a.B * a.words;
     const int64_t waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -272,23 +308,98 @@ __global__ void __launch_bounds__(kBcBlock) bc_backward_pull_kernel(BcArgs a) {
     }
 }
 
-// dep += 1/sigma for every (b, v) of F[level] (apps/bc.cpp:49-52).
-__global__ void bc_vertex_kernel(const uint64_t *f, int32_t words, int32_t B, int32_t V,
-                                 const double *sigma, float *dep) {
-    int64_t total = (int64_t)words * B;
-    for (int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; wi < total;
-         wi += (int64_t)gridDim.x * blockDim.x) {
-        uint64_t word = f[wi];
-        if (!word) continue;
-        int32_t b = (int32_t)(wi / words);
-        int32_t base = (int32_t)(wi % words) * 64;
-        while (word) {
-            int32_t bit = __ffsll((unsigned long long)word) - 1;
-            word &= word - 1;
-            int64_t k = (int64_t)b * V + base + bit;
-            dep[k] = (float)((double)dep[k] + 1.0 / sigma[k]);
+// Forward step of the state's level l: push from F[l] into F[l+1], or pull
+// (st->pull, set by the decide launch); F[l-1], already folded into visited,
+// is cleared for reuse as F[l+2].
+__global__ void __launch_bounds__(kBcBlock) bc_fwd_kernel(BcArgs a) {
+    BcState *st = a.st;
+    if (st->phase != BC_FWD) return;  // uniform
+    __shared__ int32_t list[kBcChunkVerts];
+    const int32_t l = st->level;
+    a.level = l;
+    a.fin = bc_pick3(a.bm[0], a.bm[1], a.bm[2], l % 3);
+    a.fout = bc_pick3(a.bm[0], a.bm[1], a.bm[2], (l + 1) % 3);
+    uint64_t *clr = bc_pick3(a.bm[0], a.bm[1], a.bm[2], (l + 2) % 3);
+    const int64_t n = (int64_t)a.B * a.words;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        clr[i] = 0;
+    if (st->pull)
+        bc_forward_pull(a);
+    else
+        bc_forward_push(a, list);
+}
+
+// After a forward step: an empty F[l+1] ends the pass with L = l + 1 levels
+// and starts the backward pass at l = L - 2; otherwise the next level, and
+// its direction.
+__global__ void bc_fwd_decide_kernel(BcArgs a) {
+    BcState *st = a.st;
+    if (st->phase != BC_FWD || threadIdx.x != 0) return;
+    const unsigned long long n = st->count;
+    st->count = 0;
+    st->cursor = 0;
+    if (n == 0) {
+        st->levels = st->level + 1;
+        st->level = st->levels - 2;
+        st->phase = BC_BWD;
+    } else {
+        st->remaining += (long long)n;
+        st->level += 1;
+        st->pull = a.can_pull && 16ull * n > (unsigned long long)a.B * (unsigned long long)a.V;
+    }
+}
+
+// Backward bookkeeping for the state's l: dep += 1/sigma over level l + 1
+// (backward_vertex_f, apps/bc.cpp:49-52), and for l >= 0 the bitmap of level
+// l + 1 (the senders of the step) with its size. One wave per word of one
+// source; every sender word is written, so snd needs no clearing.
+__global__ void __launch_bounds__(kBcBlock) bc_level_kernel(BcArgs a) {
+    BcState *st = a.st;
+    if (st->phase != BC_BWD) return;  // uniform
+    const int32_t l = st->level;
+    const int32_t lane = lane_id();
+    const int64_t units = (int64_t)a.B * a.words;
+    const int64_t waves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    unsigned long long cnt = 0;
+    for (int64_t un = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; un < units; un += waves) {
+        const int32_t b = (int32_t)(un / a.words), w = (int32_t)(un % a.words);
+        const int32_t v = w * 64 + lane;
+        const int64_t k = (int64_t)b * a.V + v;
+        const bool in = v < a.V && a.depth[k] == l + 1;
+        if (in) a.dep[k] = (float)((double)a.dep[k] + 1.0 / a.sigma[k]);
+        const uint64_t m = __ballot(in);
+        if (l >= 0 && lane == 0) {
+            a.snd[un] = m;
+            cnt += (unsigned long long)__popcll(m);
         }
     }
+    for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+    if (lane == 0 && cnt) atomicAdd(a.count, cnt);
+}
+
+// Backward step of the state's l >= 0 from the senders of level l + 1.
+__global__ void __launch_bounds__(kBcBlock) bc_bwd_kernel(BcArgs a) {
+    BcState *st = a.st;
+    if (st->phase != BC_BWD || st->level < 0) return;  // uniform
+    __shared__ int32_t list[kBcChunkVerts];
+    a.level = st->level;
+    a.fin = a.snd;
+    const long long c = (long long)st->count;  // senders
+    const long long below = st->remaining - c;   // vertices at levels 0..l
+    if (a.can_pull && 64 * below < c)
+        bc_backward_pull(a);
+    else
+        bc_backward_push(a, list);
+}
+
+__global__ void bc_bwd_decide_kernel(BcArgs a) {
+    BcState *st = a.st;
+    if (st->phase != BC_BWD || threadIdx.x != 0) return;
+    st->remaining -= (long long)st->count;
+    st->count = 0;
+    st->cursor = 0;
+    st->level -= 1;
+    if (st->level < -1) st->phase = BC_DONE;
 }
 
 // scores[v] += sum_b (dep - 1/sigma) * sigma over sigma != 0 (apps/bc.cpp:58-63).
@@ -305,6 +416,33 @@ __global__ void bc_final_kernel(const double *sigma, const float *dep, int32_t B
     }
 }
 
+// Fresh arrays and state for a batch of B sources (reset, apps/bc.cpp:24-27).
+__global__ void bc_reset_kernel(BcArgs a) {
+    const int64_t BV = (int64_t)a.B * a.V, BW = (int64_t)a.B * a.words;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < BV; i += (int64_t)gridDim.x * blockDim.x) {
+        a.depth[i] = -1;
+        a.sigma[i] = 0.0;
+        a.dep[i] = 0.f;
+        if (i < BW) {
+            a.visited[i] = 0;
+            a.bm[0][i] = 0;
+            a.bm[1][i] = 0;
+            a.bm[2][i] = 0;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        BcState *st = a.st;
+        st->phase = BC_FWD;
+        st->level = 0;
+        st->levels = 0;
+        st->pull = 0;
+        st->cursor = 0;
+        st->count = 0;
+        st->remaining = a.B;  // the sources, level 0
+    }
+}
+
+// Level 0: each source in its own row (one thread per source).
 __global__ void bc_seed_kernel_b(const int32_t *src, int32_t B, int32_t V, int32_t words,
                                  int32_t *depth, double *sigma, uint64_t *f0) {
     int32_t b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -312,7 +450,7 @@ __global__ void bc_seed_kernel_b(const int32_t *src, int32_t B, int32_t V, int32
     int32_t s = src[b];
     depth[(int64_t)b * V + s] = 0;
     sigma[(int64_t)b * V + s] = 1.0;
-    f0[(int64_t)b * words + (s >> 6)] |= 1ull << (s & 63);  // one source per lane-private row
+    f0[(int64_t)b * words + (s >> 6)] |= 1ull << (s & 63);
 }
 
 static int grid_for(int64_t n, int block = 256) {
@@ -321,116 +459,145 @@ static int grid_for(int64_t n, int block = 256) {
     return (int)(want < 1 ? 1 : (want < cap ? want : cap));
 }
 
-// One batch of B sources.
-int bc_batch(gi_graph *g, const int32_t *h_src, int32_t B, float *d_scores, hipStream_t s,
-             int32_t *levels_out) {
-    const int32_t V = g->V;
-    const int32_t words = (int32_t)bitmap_words(V);
-    const int64_t BV = (int64_t)B * V, BW = (int64_t)B * words;
-    int32_t *depth = nullptr, *d_src = nullptr, *cursor = nullptr;
+// Arrays, state and captured slot blocks of one BC call (batches of up to
+// `cap` sources); freed when the call ends.
+struct BcRun {
+    int32_t cap = 0;
+    int32_t *depth = nullptr, *d_src = nullptr;
     double *sigma = nullptr;
     float *dep = nullptr;
-    uint64_t *visited = nullptr;
-    unsigned long long *count = nullptr;
-    std::vector<uint64_t *> F;
-    int rc = dalloc(&depth, (size_t)BV);
-    if (rc == GI_OK) rc = dalloc(&sigma, (size_t)BV);
-    if (rc == GI_OK) rc = dalloc(&dep, (size_t)BV);
-    if (rc == GI_OK) rc = dalloc(&visited, (size_t)BW);
-    if (rc == GI_OK) rc = dalloc(&d_src, (size_t)B);
-    if (rc == GI_OK) rc = dalloc(&cursor, 1);
-    if (rc == GI_OK) rc = dalloc(&count, 1);
-    auto new_level = [&](uint64_t **out) -> int {
-        uint64_t *p = nullptr;
-        GI_TRY(dalloc(&p, (size_t)BW));
-        F.push_back(p);
-        *out = p;
-        return fill_u64(p, BW, 0, s);
-    };
-    uint64_t *f0 = nullptr;
-    if (rc == GI_OK) rc = new_level(&f0);
-    if (rc == GI_OK) rc = fill_i32(depth, BV, -1, s);
-    if (rc == GI_OK) rc = fill_f64(sigma, BV, 0.0, s);
-    if (rc == GI_OK) rc = fill_f32(dep, BV, 0.f, s);
-    if (rc == GI_OK) rc = fill_u64(visited, BW, 0, s);
-    if (rc == GI_OK && hipMemcpyAsync(d_src, h_src, sizeof(int32_t) * B, hipMemcpyHostToDevice, s) != hipSuccess)
-        rc = fail(GI_ERR_HIP, "copy sources");
-    // one kernel per source would race on shared words only if two sources
-    // shared a row; rows are per source, so the seed kernel is race free
-    if (rc == GI_OK) bc_seed_kernel_b<<<(B + 63) / 64, 64, 0, s>>>(d_src, B, V, words, depth, sigma, f0);
+    uint64_t *visited = nullptr, *snd = nullptr;
+    uint64_t *bm[3] = {nullptr, nullptr, nullptr};
+    BcState *st = nullptr;
+    BcState *host = nullptr;  // pinned
+    hipStream_t capture = nullptr;
+    std::map<std::tuple<int32_t, int32_t, int32_t>, hipGraphExec_t> execs;  // (phase, slots, B)
+    int32_t hint = 8;  // levels of the last batch
+    ~BcRun() {
+        for (auto &kv : execs) (void)hipGraphExecDestroy(kv.second);
+        if (capture) (void)hipStreamDestroy(capture);
+        dfree(depth);
+        dfree(d_src);
+        dfree(sigma);
+        dfree(dep);
+        dfree(visited);
+        dfree(snd);
+        for (auto p : bm) dfree(p);
+        dfree(st);
+        if (host) (void)hipHostFree(host);
+    }
+    int init(int32_t V, int32_t B) {
+        cap = B;
+        const int64_t BV = (int64_t)B * V, BW = (int64_t)B * bitmap_words(V);
+        GI_TRY(dalloc(&depth, (size_t)BV));
+        GI_TRY(dalloc(&sigma, (size_t)BV));
+        GI_TRY(dalloc(&dep, (size_t)BV));
+        GI_TRY(dalloc(&visited, (size_t)BW));
+        GI_TRY(dalloc(&snd, (size_t)BW));
+        for (auto &p : bm) GI_TRY(dalloc(&p, (size_t)BW));
+        GI_TRY(dalloc(&d_src, (size_t)B));
+        GI_TRY(dalloc(&st, 1));
+        if (hipHostMalloc((void **)&host, sizeof(BcState)) != hipSuccess) return fail(GI_ERR_OOM, "pinned allocation failed");
+        if (hipStreamCreateWithFlags(&capture, hipStreamNonBlocking) != hipSuccess)
+            return fail(GI_ERR_HIP, "stream create failed");
+        return GI_OK;
+    }
+};
+
+static void bc_launch_slot(const BcArgs &a, int32_t phase, hipStream_t s) {
+    const int grid = persistent_grid(kBcBlock, 8);
+    if (phase == BC_FWD) {
+        bc_fwd_kernel<<<grid, kBcBlock, 0, s>>>(a);
+        bc_fwd_decide_kernel<<<1, 64, 0, s>>>(a);
+    } else {
+        bc_level_kernel<<<grid, kBcBlock, 0, s>>>(a);
+        bc_bwd_kernel<<<grid, kBcBlock, 0, s>>>(a);
+        bc_bwd_decide_kernel<<<1, 64, 0, s>>>(a);
+    }
+}
+
+// Replays blocks of captured slots of one pass until the state leaves it
+// (a slot past the end is a no-op).
+static int bc_pass(BcRun &r, const BcArgs &a, int32_t phase, int32_t first, hipStream_t s) {
+    int32_t block = first < 4 ? 4 : (first + 3) / 4 * 4;
+    for (int64_t used = 0; used < (1ll << 24);) {
+        if (block > 512) block = 512;
+        const auto key = std::make_tuple(phase, block, a.B);
+        auto it = r.execs.find(key);
+        if (it == r.execs.end()) {
+            hipGraph_t graph = nullptr;
+            GI_HIP(hipStreamBeginCapture(r.capture, hipStreamCaptureModeThreadLocal));
+            for (int32_t i = 0; i < block; ++i) bc_launch_slot(a, phase, r.capture);
+            hipError_t e = hipStreamEndCapture(r.capture, &graph);
+            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            hipGraphExec_t exec = nullptr;
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+            it = r.execs.emplace(key, exec).first;
+        }
+        GI_HIP(hipGraphLaunch(it->second, s));
+        GI_HIP(hipMemcpyAsync(r.host, r.st, sizeof(BcState), hipMemcpyDeviceToHost, s));
+        GI_HIP(hipStreamSynchronize(s));
+        used += block;
+        if (r.host->phase != phase) return GI_OK;
+        block *= 2;
+    }
+    return fail(GI_ERR_RANGE, "BC did not finish");
+}
+
+// One batch of B <= r.cap sources: scores += their BC terms; *levels = L.
+static int bc_batch_run(gi_graph *g, BcRun &r, const int32_t *h_src, int32_t B, float *d_scores, hipStream_t s,
+                        int32_t *levels) {
     BcArgs a{};
     a.off = g->d_off;
     a.nbr = g->d_nbr;
-    a.V = V;
-    a.words = words;
+    a.V = g->V;
+    a.words = (int32_t)bitmap_words(g->V);
     a.B = B;
-    a.depth = depth;
-    a.sigma = sigma;
-    a.dep = dep;
-    a.visited = visited;
-    a.cursor = cursor;
-    a.count = count;
-    int grid = persistent_grid(kBcBlock, 8);
-    // level sizes summed over the batch decide each level's direction: pull a
-    // forward level whose frontier holds more than V/16 vertices per source,
-    // and a backward level whose receiving levels are under 1/64 of the level
-    // that sends; push otherwise, and on directed graphs (measured on RMAT-20,
-    // 256 sources: 285 ms all pushed, 218 ms with these rules)
-    const bool can_pull = !g->directed;
-    std::vector<int64_t> sizes(1, B);
-    int64_t seen = B;
-    int32_t level = 0;
-    while (rc == GI_OK) {  // forward
-        uint64_t *fn = nullptr;
-        rc = new_level(&fn);
-        if (rc != GI_OK) break;
-        a.level = level;
-        a.fin = F[level];
-        a.fout = fn;
-        rc = fill_i32(cursor, 1, 0, s);
-        if (rc == GI_OK) rc = fill_u64((uint64_t *)count, 1, 0, s);
-        if (rc != GI_OK) break;
-        if (can_pull && 16 * sizes[level] > (int64_t)B * V) bc_forward_pull_kernel<<<grid, kBcBlock, 0, s>>>(a);
-        else bc_forward_kernel<<<grid, kBcBlock, 0, s>>>(a);
-        unsigned long long n = 0;
-        if (hipMemcpyAsync(&n, count, sizeof(n), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
-            rc = fail(GI_ERR_HIP, "bc forward");
-            break;
-        }
-        ++level;
-        sizes.push_back((int64_t)n);
-        seen += (int64_t)n;
-        if (n == 0) break;
+    a.can_pull = !g->directed;
+    a.st = r.st;
+    a.depth = r.depth;
+    a.sigma = r.sigma;
+    a.dep = r.dep;
+    a.visited = r.visited;
+    for (int i = 0; i < 3; ++i) a.bm[i] = r.bm[i];
+    a.snd = r.snd;
+    a.cursor = &r.st->cursor;
+    a.count = &r.st->count;
+    GI_HIP(hipMemcpyAsync(r.d_src, h_src, sizeof(int32_t) * B, hipMemcpyHostToDevice, s));
+    bc_reset_kernel<<<grid_for((int64_t)B * g->V), 256, 0, s>>>(a);
+    bc_seed_kernel_b<<<(B + 63) / 64, 64, 0, s>>>(r.d_src, B, g->V, a.words, r.depth, r.sigma, r.bm[0]);
+    GI_LAUNCH_CHECK();
+    GI_TRY(bc_pass(r, a, BC_FWD, r.hint, s));
+    const int32_t L = r.host->levels;
+    GI_TRY(bc_pass(r, a, BC_BWD, L, s));  // l = L-2 .. -1: L slots
+    bc_final_kernel<<<grid_for(g->V), 256, 0, s>>>(r.sigma, r.dep, B, g->V, d_scores);
+    GI_LAUNCH_CHECK();
+    r.hint = L;
+    if (levels) *levels = L;
+    return GI_OK;
+}
+
+// Sources in batches of `batch` (1: the fused single-source mode); scores are
+// added into d_scores. *levels: the levels summed over the batches.
+int bc_batched(gi_graph *g, const int32_t *h_src, int32_t num_sources, int32_t batch, float *d_scores,
+               hipStream_t s, int64_t *levels) {
+    if (batch < 1) batch = 1;
+    if (batch > kBcMaxBatch) batch = kBcMaxBatch;
+    if (batch > num_sources) batch = num_sources;
+    if (levels) *levels = 0;
+    if (num_sources <= 0) return GI_OK;
+    BcRun r;
+    GI_TRY(r.init(g->V, batch));
+    for (int32_t i = 0; i < num_sources; i += batch) {
+        const int32_t nb = num_sources - i < batch ? num_sources - i : batch;
+        int32_t lv = 0;
+        GI_TRY(bc_batch_run(g, r, h_src + i, nb, d_scores, s, &lv));
+        if (levels) *levels += lv;
     }
-    // F[0..level-1] hold the levels; F[level] is empty
-    const int32_t L = level;
-    if (rc == GI_OK && L > 0)
-        bc_vertex_kernel<<<grid_for(BW), 256, 0, s>>>(F[L - 1], words, B, V, sigma, dep);
-    std::vector<int64_t> upto(sizes.size() + 1, 0);  // upto[l + 1] = sizes of levels 0..l
-    for (size_t i = 0; i < sizes.size(); ++i) upto[i + 1] = upto[i] + sizes[i];
-    for (int32_t l = L - 2; l >= 0 && rc == GI_OK; --l) {  // backward
-        a.level = l;
-        a.fin = F[l + 1];
-        rc = fill_i32(cursor, 1, 0, s);
-        if (rc != GI_OK) break;
-        if (can_pull && 64 * upto[l + 1] < sizes[l + 1]) bc_backward_pull_kernel<<<grid, kBcBlock, 0, s>>>(a);
-        else bc_backward_kernel<<<grid, kBcBlock, 0, s>>>(a);
-        bc_vertex_kernel<<<grid_for(BW), 256, 0, s>>>(F[l], words, B, V, sigma, dep);
-        if (hipGetLastError() != hipSuccess) rc = fail(GI_ERR_HIP, "bc backward launch");
-    }
-    if (rc == GI_OK) bc_final_kernel<<<grid_for(V), 256, 0, s>>>(sigma, dep, B, V, d_scores);
-    if (rc == GI_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(GI_ERR_HIP, "bc sync");
-    for (uint64_t *p : F) dfree(p);
-    dfree(depth);
-    dfree(sigma);
-    dfree(dep);
-    dfree(visited);
-    dfree(d_src);
-    dfree(cursor);
-    dfree(count);
-    if (levels_out) *levels_out = L;
-    return rc;
+    GI_HIP(hipStreamSynchronize(s));
+    return GI_OK;
 }
 
 }  // namespace gi

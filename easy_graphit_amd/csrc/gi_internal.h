@@ -251,8 +251,8 @@ namespace gi {
 int ensure_coo(gi_graph *g, hipStream_t s);
 int ensure_max_degree(gi_graph *g, hipStream_t s);
 int ensure_scratch(gi_graph *g, int64_t len);
-int bc_batch(gi_graph *g, const int32_t *h_src, int32_t B, float *d_scores, hipStream_t s,
-             int32_t *levels_out);
+int bc_batched(gi_graph *g, const int32_t *h_src, int32_t num_sources, int32_t batch, float *d_scores,
+               hipStream_t s, int64_t *levels);
 // Scratch layout per graph: load-balance region first (TWC bins 3V+3, or the
 // STRICT prefix V + chunk sums), then the pull enumerator (count, queue).
 inline int64_t scratch_lb_len(int64_t V) {

@@ -476,9 +476,20 @@ int gi_cc(gi_graph *g, const gi_schedule *s, int32_t *d_labels, gi_run_stats *st
 int gi_pagerank(gi_graph *g, int32_t iters, float damping, const gi_schedule *s,
                 float *d_rank, gi_run_stats *stats, void *stream);
 /* Brandes BC from num_sources sources (apps/bc.cpp:150-209), summed into
- * d_scores (float, zeroed first); batch = sources per launch group. */
+ * d_scores (float, zeroed first). batch > 1: up to 256 sources advance
+ * level-synchronously, the level loops on the device (captured); batch <= 1:
+ * one source at a time as the app runs unfused (push TWCE, a host round trip
+ * per level, apps/bc.cpp:149-151). */
 int gi_bc(gi_graph *g, const int32_t *h_sources, int32_t num_sources, int32_t batch,
           float *d_scores, gi_run_stats *stats, void *stream);
+/* BC one source at a time under a schedule (apps/bc.cpp:64-120 with its s1
+ * and to_fuse). s == NULL: the app's push TWCE, unfused. kernel_fusion
+ * ENABLED: the fused mode (fuse_kernel, apps/bc.cpp:92-113): the forward and
+ * backward level loops run on the device, push or pull per level (pull only
+ * on symmetric graphs). Otherwise push steps with s's load balancing (a PULL
+ * direction needs kernel_fusion: GI_ERR_UNSUPPORTED). */
+int gi_bc_schedule(gi_graph *g, const int32_t *h_sources, int32_t num_sources, const gi_schedule *s,
+                   float *d_scores, gi_run_stats *stats, void *stream);
 
 #ifdef __cplusplus
 }

@@ -313,6 +313,36 @@ def test_bc_batched_long_and_directed(gi, graph):
         assert rel_l1(sc, O.bc(host, srcs)) <= 1e-4, (graph, batch)
 
 
+@pytest.mark.parametrize("graph", ["rmat", "grid", "directed"])
+def test_bc_fused_single_source(gi, graph):
+    # the fused mode: one source at a time with the level loops on the device
+    if graph == "rmat":
+        g = gi.Graph.rmat(12, 8, SEED, True)
+    elif graph == "grid":
+        g = gi.Graph.grid(48, 30, 3)
+    else:
+        g = gi.Graph.rmat(11, 8, 5, False)
+    host = host_csr(g)
+    srcs = O.pick_sources(host, 9, 12)
+    want = O.bc(host, srcs)
+    fused = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWCE, kernel_fusion=1)
+    sc, st = gi.bc(g, srcs, schedule=fused)
+    assert rel_l1(sc, want) <= 1e-4, graph
+    assert st["iterations"] == len(srcs)
+
+
+@pytest.mark.parametrize("lb", [0, 1, 2])
+def test_bc_schedule_unfused(gi, lb):
+    # unfused, one source at a time with the schedule's push load balancing
+    g = gi.Graph.rmat(11, 8, SEED, True)
+    host = host_csr(g)
+    srcs = O.pick_sources(host, 5, 8)
+    sc, _ = gi.bc(g, srcs, schedule=gi.Schedule(direction=gi.PUSH, load_balancing=lb))
+    assert rel_l1(sc, O.bc(host, srcs)) <= 1e-4, lb
+    with pytest.raises(gi.GIError):  # pulled levels need the fused mode
+        gi.bc(g, srcs, schedule=gi.Schedule(direction=gi.PULL, load_balancing=lb))
+
+
 def test_bc_batched_equals_sequential_on_fixture(gi):
     e = np.array([[0, 1], [1, 2], [2, 3], [0, 4], [4, 3], [3, 5]], np.int32)
     g = gi.Graph.from_edges(7, e[:, 0], e[:, 1], True)  # vertex 6 isolated
@@ -321,4 +351,6 @@ def test_bc_batched_equals_sequential_on_fixture(gi):
     want = O.bc(host, srcs)
     seq, _ = gi.bc(g, srcs, batch=1)
     bat, _ = gi.bc(g, srcs, batch=4)
-    assert np.allclose(seq, want, rtol=1e-5) and np.allclose(bat, want, rtol=1e-5)
+    fus, _ = gi.bc(g, srcs, schedule=gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWCE, kernel_fusion=1))
+    for got in (seq, bat, fus):
+        assert np.allclose(got, want, rtol=1e-5)
