@@ -228,15 +228,20 @@ def bench_apps(gi, g, sources, edges_of, log) -> dict:
     _, st_b = gi.bc(gb, bsrc, 256)
     gi.bc(gb, bsrc[:2], 1)
     _, st_s = gi.bc(gb, bsrc, 1)
+    fused1 = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWCE, kernel_fusion=1)
+    gi.bc(gb, bsrc[:2], schedule=fused1)
+    _, st_f = gi.bc(gb, bsrc, schedule=fused1)
     dtmp = gi.DeviceArray(gb.num_vertices, np.int32)
     be = 0
     for s in bsrc:
         gi.bfs_device(gb, int(s), None, dtmp)
         be += gb.reached_edges(dtmp)
-    tb, tsq = st_b["ms"] * 1e-3, st_s["ms"] * 1e-3
+    tb, tsq, tf = st_b["ms"] * 1e-3, st_s["ms"] * 1e-3, st_f["ms"] * 1e-3
     out["bc"] = {"sources": len(bsrc), "batched_ms": round(tb * 1e3, 2), "sequential_ms": round(tsq * 1e3, 2),
                  "speedup": round(tsq / tb, 2), "gteps": round(be / 2 / tb / 1e9, 3),
-                 "config": "RMAT-20 sym, 256 sources, batch 256 vs 1"}
+                 "fused_single_source_ms": round(tf * 1e3, 2), "speedup_vs_fused_single": round(tf / tb, 2),
+                 "config": "RMAT-20 sym, 256 sources: batch 256 vs one source at a time (unfused per-level "
+                           "host loop, and the fused single-source mode)"}
     log(f"bc: {out['bc']}")
     gb.destroy()
     # 1,024 uniform graphs (V log-uniform in [1k, 64k], average degree 8), one
