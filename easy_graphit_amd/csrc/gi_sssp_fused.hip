@@ -17,6 +17,15 @@
 // the units whose bound falls inside the new window, and rewrites their
 // bounds exactly; the other units are skipped, so later windows read a small
 // part of dist instead of all of it.
+// Light and heavy edges (the delta-stepping split of Meyer and Sanders):
+// each out-list is copied with its light edges (w < delta) first. An edge
+// from the window [lo, hi) that is heavy lands at or beyond hi, so it can
+// never put a vertex back into the window. A push step whose F has at least
+// E / kSsDeferDiv out-edges relaxes light edges only and records F in the
+// window set W; when the window is settled, one heavy step relaxes the heavy
+// edges of W once, from final distances, instead of once per improvement of
+// each vertex within the window. Steps with smaller F relax all edges (no
+// extra step for windows that settle in one step).
 // A captured slot is five launches, each a no-op unless its mode is active:
 //   ss_relax   push: relax the out-edges of F, spread evenly over each
 //              workgroup (vertices of at least kSsHubMin edges go to the
@@ -51,7 +60,9 @@ constexpr int32_t kSsMinSlots = 9;    // slot row of the window scan's least pen
 
 constexpr int kSsPullUnit = 4 * (kSsBlock / 64);  // words per unit (pull work, window scan, umin)
 
-enum SsMode : int32_t { SS_RELAX = 0, SS_ADVANCE = 1 };
+constexpr long long kSsDeferDiv = 16;  // F with E / kSsDeferDiv out-edges or more: heavy edges wait
+
+enum SsMode : int32_t { SS_RELAX = 0, SS_ADVANCE = 1, SS_HEAVY = 2 };
 
 struct SsState {
     int32_t done;
@@ -69,6 +80,8 @@ struct SsState {
     int32_t next_min;    // least pending distance >= hi seen by the last scan
     int32_t pull;        // the next relax step pulls (large frontier)
     int32_t pull_min;    // |F| above which a step pulls
+    int32_t defer;       // the next push step relaxes light edges only and records F in W
+    int32_t ws_any;      // W holds vertices whose heavy edges wait
     long long fdeg;      // out-edges of F
     int32_t hubs[kSsMaxHubs];
 };
@@ -76,8 +89,12 @@ struct SsState {
 struct SsArgs {
     SsState *st;
     const int32_t *off;
-    const int32_t *nbr;
+    const int32_t *nbr;   // out-lists
     const int32_t *wt;
+    const int32_t *snbr;  // the same lists, light edges first (light or heavy steps)
+    const int32_t *swt;
+    const int32_t *lend;  // end of each light part (null: no light/heavy split)
+    long long defer_min;  // out-edges of F from which its heavy edges wait
     const int32_t *poff;  // in-edges with their weights (the graph itself when symmetric)
     const int32_t *pnbr;
     const int32_t *pwt;
@@ -89,6 +106,9 @@ struct SsArgs {
     uint64_t *sum[3];
     uint8_t *cflag;      // chunk flags of the bitmap being filled (bs_flag_chunk)
     int32_t *umin;       // per unit of kSsPullUnit words: lower bound of its pending distances
+    uint64_t *ws;        // W: vertices of the window whose heavy edges wait
+    uint64_t *wsum;      // its chunk summary (built when the heavy step starts)
+    uint8_t *wsflag;     // its chunk flags
     int32_t chunks;
     int32_t *slots;      // [3 kernels][3 counters][kSsMaxGrid], then row kSsMinSlots
     int32_t wide_chunks;       // pull: chunks of the in-lists of wide vertices
@@ -123,11 +143,12 @@ __device__ __forceinline__ void ss_claim(const SsArgs &a, SsCtx &c, int32_t v, i
     }
 }
 
-// Relaxation of edge e from a vertex at distance du.
-__device__ __forceinline__ void ss_relax(const SsArgs &a, SsCtx &c, int32_t du, int32_t e, bool active) {
+// Relaxation of edge e (of lists nbr, wt) from a vertex at distance du.
+__device__ __forceinline__ void ss_relax(const SsArgs &a, SsCtx &c, const int32_t *nbr, const int32_t *wt,
+                                         int32_t du, int32_t e, bool active) {
     if (!active) return;
-    const int32_t v = a.nbr[e];
-    const int32_t nd = du + a.wt[e];
+    const int32_t v = nbr[e];
+    const int32_t nd = du + wt[e];
     if (nd >= a.dist[v]) return;  // test before the atomic
     ss_claim(a, c, v, nd);
 }
@@ -140,6 +161,10 @@ __device__ __forceinline__ void ss_push_body(const SsArgs &a, int32_t *pre, int3
     SsState *st = a.st;
     __shared__ int32_t pre_s[kSsBlock], e0_s[kSsBlock], du_s[kSsBlock];
     __shared__ int32_t n_s;
+    const bool heavy = st->mode == SS_HEAVY;  // F is W, edges from lend
+    const bool defer = !heavy && st->defer;   // edges up to lend; F joins W
+    const int32_t *nbr = heavy || defer ? a.snbr : a.nbr;  // split lists only where the split is used
+    const int32_t *wt = heavy || defer ? a.swt : a.wt;
     const int32_t phase = st->phase;
     const int32_t iin = phase % 3, iout = (phase + 1) % 3, iclr = (phase + 2) % 3;
     SsCtx c;
@@ -150,8 +175,8 @@ __device__ __forceinline__ void ss_push_body(const SsArgs &a, int32_t *pre, int3
     // the bitmap two phases ahead is free (its summary is cleared by ss_decide)
     bs_clear_chunks(bs_pick3(a.bm[0], a.bm[1], a.bm[2], iclr), bs_pick3(a.sum[0], a.sum[1], a.sum[2], iclr),
                     a.words, a.sum_words, pre);
-    const uint64_t *bm_in = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iin);
-    const uint64_t *si = bs_pick3(a.sum[0], a.sum[1], a.sum[2], iin);
+    const uint64_t *bm_in = heavy ? a.ws : bs_pick3(a.bm[0], a.bm[1], a.bm[2], iin);
+    const uint64_t *si = heavy ? a.wsum : bs_pick3(a.sum[0], a.sum[1], a.sum[2], iin);
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
     const int32_t nz = bs_summary_prefix(si, a.sum_words, pre);
     for (int32_t k = blockIdx.x; k < nz; k += gridDim.x) {
@@ -159,6 +184,15 @@ __device__ __forceinline__ void ss_push_body(const SsArgs &a, int32_t *pre, int3
         if (threadIdx.x < 64) {  // wave 0 lists the chunk's members
             const int32_t w = chunk * kBsChunkWords + (int32_t)threadIdx.x;
             uint64_t word = w < a.words ? bm_in[w] : 0ull;
+            if (word) {  // this lane alone owns word w of W in this step
+                if (heavy) {
+                    a.ws[w] = 0;  // consumed
+                } else if (defer) {
+                    const uint64_t o = a.ws[w];
+                    a.ws[w] = o | word;
+                    if (!o) bs_flag_chunk(a.wsflag, w);
+                }
+            }
             const int32_t pc = __popcll(word);
             int32_t incl = pc;
             for (int o = 1; o < 64; o <<= 1) {
@@ -181,7 +215,9 @@ __device__ __forceinline__ void ss_push_body(const SsArgs &a, int32_t *pre, int3
             const int32_t j = b0 + (int32_t)threadIdx.x;
             const bool has = j < n;
             const int32_t s = has ? list[j] : 0;
-            const int32_t e0 = has ? a.off[s] : 0, e1 = has ? a.off[s + 1] : 0;
+            int32_t e0 = has ? a.off[s] : 0, e1 = has ? a.off[s + 1] : 0;
+            if (has && heavy) e0 = a.lend[s];
+            if (has && defer) e1 = a.lend[s];
             const int32_t dg = e1 - e0;
             const bool hub = has && dg >= kSsHubMin;
             const int32_t hpos = wave_atomic_add(&st->hub_count, hub);
@@ -203,8 +239,8 @@ __device__ __forceinline__ void ss_push_body(const SsArgs &a, int32_t *pre, int3
                     if (k < total) {  // owner: the last member whose prefix is <= k
                         const int32_t o = upper_bound_i32(pre_s, 0, kSsBlock, k) - 1;
                         const int32_t e = e0_s[o] + (k - pre_s[o]);
-                        v[p] = a.nbr[e];
-                        nd[p] = du_s[o] + a.wt[e];
+                        v[p] = nbr[e];
+                        nd[p] = du_s[o] + wt[e];
                     }
                 }
 #pragma unroll
@@ -362,7 +398,7 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int3
 // either way; the body not taken costs nothing).
 __global__ void __launch_bounds__(kSsBlock) ss_relax_kernel(SsArgs a) {
     SsState *st = a.st;
-    if (st->done || st->mode != SS_RELAX) return;  // uniform
+    if (st->done || st->mode == SS_ADVANCE) return;  // uniform
     __shared__ int32_t pre[kBsMaxSumWords + 1];
     __shared__ int32_t lds[kBsChunkVerts];
     if (st->pull)
@@ -374,10 +410,13 @@ __global__ void __launch_bounds__(kSsBlock) ss_relax_kernel(SsArgs a) {
 // Grid-wide bin: the concatenated edges of the deferred hubs, spread evenly.
 __global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a) {
     SsState *st = a.st;
-    if (st->done || st->mode != SS_RELAX || st->pull) return;
+    if (st->done || st->mode == SS_ADVANCE || st->pull) return;
     int32_t nh = st->hub_count;
     if (nh <= 0) return;  // slots stay zero
     if (nh > kSsMaxHubs) nh = kSsMaxHubs;
+    const bool heavy = st->mode == SS_HEAVY, defer = !heavy && st->defer;  // the step's edge range
+    const int32_t *nbr = heavy || defer ? a.snbr : a.nbr;
+    const int32_t *wt = heavy || defer ? a.swt : a.wt;
     const int32_t phase = st->phase;
     SsCtx c;
     c.hi = st->hi;
@@ -385,29 +424,31 @@ __global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a) {
     c.nf = 0;
     c.fdeg = 0;
     __shared__ int32_t hscan[kSsMaxHubs + 1];
-    __shared__ int32_t hv[kSsMaxHubs];
+    __shared__ int32_t hs[kSsMaxHubs];  // first edge
     __shared__ int32_t hd[kSsMaxHubs];
     const int32_t per = (nh + kSsBlock - 1) / kSsBlock;
     const int32_t lo = threadIdx.x * per, hi = lo + per < nh ? lo + per : nh;
     int32_t part = 0;
     for (int32_t i = lo; i < hi; ++i) {
         const int32_t s = st->hubs[i];
-        hv[i] = s;
+        const int32_t e0 = heavy ? a.lend[s] : a.off[s], e1 = defer ? a.lend[s] : a.off[s + 1];
+        hs[i] = e0;
         hd[i] = a.dist[s];
-        part += a.off[s + 1] - a.off[s];
+        part += e1 - e0;
     }
     int32_t total = 0;
     int32_t ex = block_exclusive_scan(part, &total);
     for (int32_t i = lo; i < hi; ++i) {
+        const int32_t s = st->hubs[i];
         hscan[i] = ex;
-        ex += a.off[hv[i] + 1] - a.off[hv[i]];
+        ex += (defer ? a.lend[s] : a.off[s + 1]) - hs[i];
     }
     __syncthreads();
     const int32_t chunk = (total + gridDim.x - 1) / gridDim.x;
     const int32_t k0 = blockIdx.x * chunk, k1 = k0 + chunk < total ? k0 + chunk : total;
     for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += kSsBlock) {
         const int32_t j = upper_bound_i32(hscan, 0, nh, k) - 1;
-        ss_relax(a, c, hd[j], a.off[hv[j]] + (k - hscan[j]), true);
+        ss_relax(a, c, nbr, wt, hd[j], hs[j] + (k - hscan[j]), true);
     }
     bs_flush3<kSsBlock>(a.slots + 3 * kSsMaxGrid, kSsMaxGrid, c.nf, c.fdeg, 0);
 }
@@ -454,16 +495,25 @@ __device__ __forceinline__ int32_t ss_window_end(int32_t lo, int32_t delta) {
 
 __global__ void __launch_bounds__(kSsDecBlock) ss_decide_kernel(SsArgs a, int32_t nslots) {
     SsState *st = a.st;
-    if (st->done || st->mode != SS_RELAX) return;  // uniform
+    // every thread reads the state before the barriers in ss_collect; thread 0
+    // rewrites it after them
+    const int32_t mode = st->mode;
+    if (st->done || mode == SS_ADVANCE) return;  // uniform
+    const int32_t phase = st->phase;
+    const bool ws_any = st->ws_any || (mode == SS_RELAX && st->defer);
     const int32_t rows[4] = {0, 3, 1, 4};  // |F'| and its out-edges from the relax and hub kernels
     long long c[4];
     ss_collect(a.slots, rows, nslots, c);
     const long long nf = c[0] + c[1], fdeg = c[2] + c[3];
-    const int32_t phase = st->phase;
     uint64_t *sclr = bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 2) % 3);
     for (int32_t i = threadIdx.x; i <= a.sum_words; i += kSsDecBlock) sclr[i] = 0;
     // the summary of F', from the flags its chunks set
     bs_build_summary<kSsDecBlock>(a.cflag, a.chunks, bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 1) % 3));
+    const bool to_heavy = nf == 0 && mode == SS_RELAX && ws_any;  // settled, heavy edges waiting
+    if (to_heavy)
+        bs_build_summary<kSsDecBlock>(a.wsflag, a.chunks, a.wsum);
+    else if (mode == SS_HEAVY)  // the heavy step consumed W
+        for (int32_t i = threadIdx.x; i <= a.sum_words; i += kSsDecBlock) a.wsum[i] = 0;
     if (threadIdx.x == 0) {
         st->phase = phase + 1;
         st->steps += 1;
@@ -472,11 +522,19 @@ __global__ void __launch_bounds__(kSsDecBlock) ss_decide_kernel(SsArgs a, int32_
         st->fdeg = fdeg;
         st->hub_count = 0;
         st->pull = nf > st->pull_min;
-        if (nf == 0) {  // the window is settled: scan for [hi, hi + delta)
+        st->defer = a.lend != nullptr && !st->pull && fdeg >= a.defer_min;
+        st->ws_any = ws_any;
+        if (to_heavy) {
+            st->mode = SS_HEAVY;
+            st->ws_any = 0;
+            st->defer = 0;
+        } else if (nf == 0) {  // the window is settled: scan for [hi, hi + delta)
             st->mode = SS_ADVANCE;
             st->lo = st->hi;
             st->hi = ss_window_end(st->hi, st->delta);
             st->next_min = INT_MAX;
+        } else if (mode == SS_HEAVY) {
+            st->mode = SS_RELAX;  // not reached: heavy edges land at or beyond hi
         }
     }
 }
@@ -582,6 +640,7 @@ __global__ void __launch_bounds__(kSsDecBlock) ss_decide2_kernel(SsArgs a, int32
             st->front = (int32_t)moved;
             st->fdeg = fdeg;
             st->pull = moved > st->pull_min;
+            st->defer = a.lend != nullptr && !st->pull && fdeg >= a.defer_min;
             st->windows += 1;
         } else if (st->next_min == INT_MAX) {
             st->done = 1;  // nothing pending
@@ -605,6 +664,9 @@ __global__ void ss_reset_kernel(SsArgs a, SsSeed sd) {
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (i < a.V) a.dist[i] = i == sd.source ? 0 : INT_MAX;
         if (i < units) a.umin[i] = INT_MAX;  // nothing pending
+        if (i < a.words) a.ws[i] = 0;
+        if (i <= a.sum_words) a.wsum[i] = 0;
+        if (i < a.chunks) a.wsflag[i] = 0;
         if (i < a.words) {
             const uint64_t s = (i == (sd.source >> 6)) ? (1ull << (sd.source & 63)) : 0ull;
             a.bm[0][i] = s;
@@ -634,6 +696,8 @@ __global__ void ss_reset_kernel(SsArgs a, SsSeed sd) {
         st->front = 1;
         st->next_min = INT_MAX;
         st->pull = 0;
+        st->defer = 0;
+        st->ws_any = 0;
         st->fdeg = 0;
         st->pull_min = sd.pull_min;
     }
@@ -641,6 +705,33 @@ __global__ void ss_reset_kernel(SsArgs a, SsSeed sd) {
 
 __global__ void ss_zero_slots_kernel(int32_t *slots, int32_t n) {
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) slots[i] = 0;
+}
+
+// Light/heavy split of the out-lists for one delta: flag[e] = w < delta,
+// scanned into L (L[e] = light edges before e); each list keeps its order
+// within the light part and within the heavy part.
+__global__ void ss_light_flag_kernel(const int32_t *wt, int64_t E, int32_t delta, int32_t *flag) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e <= E; e += (int64_t)gridDim.x * blockDim.x)
+        flag[e] = e < E && wt[e] < delta ? 1 : 0;
+}
+
+__global__ void ss_split_kernel(const int32_t *esrc, const int32_t *off, const int32_t *nbr, const int32_t *wt,
+                                const int32_t *L, int64_t E, int32_t *snbr, int32_t *swt) {
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E; e += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t v = esrc[e];
+        const int32_t base = off[v];
+        const int32_t lb = L[e] - L[base];               // light edges of v before e
+        const int32_t nl = L[off[v + 1]] - L[base];      // light edges of v
+        const bool light = L[e + 1] > L[e];
+        const int32_t pos = light ? base + lb : base + nl + ((int32_t)e - base - lb);
+        snbr[pos] = nbr[e];
+        swt[pos] = wt[e];
+    }
+}
+
+__global__ void ss_lend_kernel(const int32_t *off, const int32_t *L, int32_t V, int32_t *lend) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
+        lend[v] = off[v] + (L[off[v + 1]] - L[off[v]]);
 }
 
 // ---------------------------------------------------------------- host side
@@ -651,6 +742,11 @@ struct SsWork {
     uint8_t *cflag = nullptr;
     int32_t chunks = 0;
     int32_t *umin = nullptr;
+    uint64_t *ws = nullptr, *wsum = nullptr;
+    uint8_t *wsflag = nullptr;
+    int32_t *snbr = nullptr, *swt = nullptr, *lend = nullptr;  // light/heavy split
+    int32_t split_delta = -1;  // delta of the split (-1: none built)
+    bool split = false;        // the split has heavy edges and is used
     int32_t *slots = nullptr;
     int32_t sum_words = 0;
     int32_t *host = nullptr;  // pinned: done, mode, phase, steps, windows, slots, pulls
@@ -675,6 +771,12 @@ static void free_ss(SsWork *w) {
     dfree(w->slots);
     dfree(w->cflag);
     dfree(w->umin);
+    dfree(w->ws);
+    dfree(w->wsum);
+    dfree(w->wsflag);
+    dfree(w->snbr);
+    dfree(w->swt);
+    dfree(w->lend);
     dfree(w->chunk_v);
     dfree(w->chunk_e0);
     if (w->host) (void)hipHostFree(w->host);
@@ -717,10 +819,14 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->sum[i], (size_t)sum_words + 1);
     if (rc == GI_OK) rc = dalloc(&w->slots, (kSsMinSlots + 1) * (size_t)kSsMaxGrid);
     if (rc == GI_OK) rc = dalloc(&w->umin, (size_t)ceil_div(words, kSsPullUnit) + 1);
+    if (rc == GI_OK) rc = dalloc(&w->ws, (size_t)words + 1);
+    if (rc == GI_OK) rc = dalloc(&w->wsum, (size_t)sum_words + 1);
     w->chunks = (int32_t)ceil_div(words, kBsChunkWords);
     const size_t flag_bytes = (size_t)bs_flag_bytes(w->chunks);
     if (rc == GI_OK) rc = dalloc(&w->cflag, flag_bytes);
     if (rc == GI_OK && hipMemset(w->cflag, 0, flag_bytes) != hipSuccess) rc = fail(GI_ERR_HIP, "flag init");
+    if (rc == GI_OK) rc = dalloc(&w->wsflag, flag_bytes);
+    if (rc == GI_OK && hipMemset(w->wsflag, 0, flag_bytes) != hipSuccess) rc = fail(GI_ERR_HIP, "flag init");
     if (rc == GI_OK && hipHostMalloc((void **)&w->host, 8 * sizeof(int32_t)) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
     if (rc == GI_OK && hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking) != hipSuccess)
@@ -776,6 +882,10 @@ static SsArgs ss_args(gi_graph *g, SsWork *w, int32_t *dist) {
     a.off = g->d_off;
     a.nbr = g->d_nbr;
     a.wt = g->d_w;
+    a.snbr = w->split ? w->snbr : nullptr;
+    a.swt = w->split ? w->swt : nullptr;
+    a.lend = w->split ? w->lend : nullptr;
+    a.defer_min = g->E / kSsDeferDiv > 1 ? g->E / kSsDeferDiv : 1;
     a.poff = w->poff;
     a.pnbr = w->pnbr;
     a.pwt = w->pwt;
@@ -790,11 +900,56 @@ static SsArgs ss_args(gi_graph *g, SsWork *w, int32_t *dist) {
     a.slots = w->slots;
     a.cflag = w->cflag;
     a.umin = w->umin;
+    a.ws = w->ws;
+    a.wsum = w->wsum;
+    a.wsflag = w->wsflag;
     a.chunks = w->chunks;
     a.wide_chunks = w->wide_chunks;
     a.chunk_v = w->chunk_v;
     a.chunk_e0 = w->chunk_e0;
     return a;
+}
+
+// The light/heavy split for `delta` (built on first use of a delta; the
+// captured slots are dropped when the arrays they name change).
+static int ensure_split(gi_graph *g, SsWork *w, int32_t delta, hipStream_t s) {
+    if (w->split_delta == delta) return GI_OK;
+    const int64_t E = g->E;
+    bool use = false;
+    if (E > 0) {
+        GI_TRY(ensure_coo(g, s));
+        int32_t *flag = nullptr, *L = nullptr;
+        int rc = dalloc(&flag, (size_t)E + 1);
+        if (rc == GI_OK) rc = dalloc(&L, (size_t)E + 1);
+        if (rc == GI_OK && !w->snbr) rc = dalloc(&w->snbr, (size_t)E);
+        if (rc == GI_OK && !w->swt) rc = dalloc(&w->swt, (size_t)E);
+        if (rc == GI_OK && !w->lend) rc = dalloc(&w->lend, (size_t)g->V + 1);
+        int32_t light = 0;
+        if (rc == GI_OK) {
+            const int grid = persistent_grid(256);
+            ss_light_flag_kernel<<<grid, 256, 0, s>>>(g->d_w, E, delta, flag);
+            rc = exclusive_scan_i32(flag, L, E + 1, s);
+            if (rc == GI_OK) rc = read_count(L + E, &light, s);
+        }
+        if (rc == GI_OK && light < E) {  // some edge is heavy: build and use the split
+            const int grid = persistent_grid(256);
+            ss_split_kernel<<<grid, 256, 0, s>>>(g->d_esrc, g->d_off, g->d_nbr, g->d_w, L, E, w->snbr, w->swt);
+            ss_lend_kernel<<<grid, 256, 0, s>>>(g->d_off, L, g->V, w->lend);
+            if (hipGetLastError() != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+                rc = fail(GI_ERR_HIP, "light/heavy split");
+            use = true;
+        }
+        dfree(flag);
+        dfree(L);
+        GI_TRY(rc);
+    }
+    if (use != w->split || use) {  // the arrays or their use changed
+        for (auto &kv : w->execs) (void)hipGraphExecDestroy(kv.second);
+        w->execs.clear();
+    }
+    w->split = use;
+    w->split_delta = delta;
+    return GI_OK;
 }
 
 static void launch_slot(const SsArgs &a, bool hubs, hipStream_t s) {
@@ -810,6 +965,7 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
                    hipStream_t s) {
     SsWork *w = nullptr;
     GI_TRY(ensure_ss(g, &w));
+    GI_TRY(ensure_split(g, w, delta, s));
     SsArgs a = ss_args(g, w, d_dist);
     const int32_t n = g->V > w->sum_words + 1 ? g->V : w->sum_words + 1;
     int rg = (int)ceil_div(n, 256);
@@ -862,6 +1018,7 @@ static int sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_m
                       int32_t max_slots, float *kernel_ms, int64_t *info, int32_t *num_slots, hipStream_t s) {
     SsWork *w = nullptr;
     GI_TRY(ensure_ss(g, &w));
+    GI_TRY(ensure_split(g, w, delta, s));
     SsArgs a = ss_args(g, w, d_dist);
     const int32_t n = g->V > w->sum_words + 1 ? g->V : w->sum_words + 1;
     int rg = (int)ceil_div(n, 256);

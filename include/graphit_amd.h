@@ -453,7 +453,7 @@ int gi_sssp(gi_graph *g, int32_t source, int32_t delta, const gi_schedule *s,
  * in the column of its direction, 0 in the other), hub bin (0 for a graph
  * whose out-degrees stay below the hub width: not launched), decide, window
  * scan, decide2. info[max_slots][5]: the state before the slot (mode: 0
- * relax / 1 advance, pull flag, |F|, window end hi, out-edges of F).
+ * relax / 1 advance / 2 heavy-edge step of a settled window, pull flag, |F|, window end hi, out-edges of F).
  * Synchronises. */
 int gi_sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_min, int32_t *d_dist,
                   int32_t max_slots, float *kernel_ms, int64_t *info, int32_t *num_slots, void *stream);

@@ -122,6 +122,8 @@ def test_sssp_trace_matches_and_counts(gi):
             assert info[relax, 1][1:].all()
         if pull_min == 1 << 30:
             assert not info[relax, 1].any()
+            # wide windows deferred their heavy edges to a heavy step (mode 2)
+            assert (info[:, 0] == 2).any()
     assert st["iterations"] == st["push_iterations"] + st["pull_iterations"]
 
 
