@@ -34,9 +34,10 @@
 // blocks, the host reading the state once per block. Direction rules, from
 // level sizes summed over the batch: pull a forward level whose frontier
 // holds more than V/16 vertices per source, and a backward level whose
-// receiving levels are under 1/64 of the level that sends; push otherwise,
-// and on directed graphs (measured on RMAT-20, 256 sources: 285 ms all
-// pushed, 218 ms with these rules).
+// lower levels together hold under 1/64 of the vertices of the level that
+// sends (fewer than it, for a single source); push otherwise, and on
+// directed graphs (RMAT-20, 256 sources: 285 ms all pushed, 218 ms with
+// these rules).
 #include <map>
 #include <tuple>
 #include <vector>
@@ -288,9 +289,12 @@ __device__ __forceinline__ void bc_forward_pull(const BcArgs &a) {
     if (lane == 0 && cnt) atomicAdd(a.count, cnt);
 }
 
-// Pull backward level: every vertex at depth 0..level of source b adds the
-// dependencies of its neighbours among the senders (backward_update,
-// apps/bc.cpp:52-54, pulled; the same "not yet processed" set).
+// Pull backward level (symmetric graphs only): every vertex at depth `level`
+// of source b adds the dependencies of its neighbours among the senders
+// (backward_update, apps/bc.cpp:52-54, pulled). The reference's receivers
+// are all vertices at depth 0..level ("not yet processed"), but on a
+// symmetric graph a neighbour of a vertex at level + 1 lies at level,
+// level + 1 or level + 2, so only depth == level can receive.
 __device__ __forceinline__ void bc_backward_pull(const BcArgs &a) {
     const int32_t lane = lane_id();
     const int64_t units = (int64_t)a.B * a.words;
@@ -299,7 +303,7 @@ __device__ __forceinline__ void bc_backward_pull(const BcArgs &a) {
         const int32_t b = (int32_t)(un / a.words), w = (int32_t)(un % a.words);
         const int32_t v = w * 64 + lane;
         const int32_t d = v < a.V ? a.depth[(int64_t)b * a.V + v] : -1;
-        const bool open = d >= 0 && d <= a.level;
+        const bool open = d == a.level;
         if (!__ballot(open)) continue;
         const int32_t e0 = open ? a.off[v] : 0, e1 = open ? a.off[v + 1] : 0;
         float *D = a.dep + (int64_t)b * a.V;
@@ -385,8 +389,13 @@ __global__ void __launch_bounds__(kBcBlock) bc_bwd_kernel(BcArgs a) {
     a.level = st->level;
     a.fin = a.snd;
     const long long c = (long long)st->count;  // senders
-    const long long below = st->remaining - c;   // vertices at levels 0..l
-    if (a.can_pull && 64 * below < c)
+    const long long below = st->remaining - c;   // vertices at levels 0..l (>= the receivers)
+    // one source: its sender bitmap (V/8 bytes) stays in L2 and pulling pays
+    // once the lower levels are smaller than the senders; a batch's bitmaps
+    // and values leave L2, and pulling pays only for far smaller receivers
+    // (RMAT-20: one source 859 -> 820 ms per 256; a batch of 256 223 ms with
+    // the 1/64 rule, 389 ms with the other)
+    if (a.can_pull && (a.B == 1 ? below < c : 64 * below < c))
         bc_backward_pull(a);
     else
         bc_backward_push(a, list);
