@@ -43,7 +43,7 @@ __device__ __forceinline__ int32_t deg_of(const int32_t *off, int32_t v) {
 }
 
 constexpr int kStepBlock = 256;
-constexpr int kDecideBlock = 256;
+constexpr int kDecideBlock = 1024;  // one workgroup: few loads of the slot rows per thread
 constexpr int kChunkWords = 64;                  // chunk: 64 words = 4096 vertices
 constexpr int kChunkVerts = kChunkWords * 64;
 constexpr int kCandScan = 128;                   // in-neighbours examined per candidate
