@@ -150,3 +150,26 @@ def test_layout_bfs_isolated_tail(gi, rmat):
     for src in (int(iso[0]), int(iso[-1]), int(O.pick_sources(host, 1, 17)[0])):
         depth, _, _ = gi.bfs(g, src, fused(gi))
         assert np.array_equal(depth, O.bfs_depth(host, src)), src
+
+
+def test_layout_bfs_deep_and_shallow(gi):
+    # a 700-vertex path hung off a random graph, with isolated vertices: deep
+    # runs (more than 255 levels) from the path's far end and shallow ones
+    # from the random part map back alike
+    n_path, n_rand = 700, 3000
+    rng = np.random.default_rng(8)
+    a = rng.integers(0, n_rand, 12000)
+    b = rng.integers(0, n_rand, 12000)
+    pa = np.arange(n_rand - 1, n_rand + n_path - 1)
+    src = np.concatenate([a, pa]).astype(np.int32)
+    dst = np.concatenate([b, pa + 1]).astype(np.int32)
+    n = n_rand + n_path + 50  # 50 isolated vertices
+    g = gi.Graph.from_edges(n, src, dst, True)
+    host = O.build_csr(n, src, dst, True)
+    g.set_layout(gi.LAYOUT_DEGREE)
+    for s0 in (n_rand + n_path - 1, 0, n - 1):
+        for _ in range(2):
+            depth, _, _ = gi.bfs(g, int(s0), fused(gi))
+            want = O.bfs_depth(host, int(s0))
+            assert np.array_equal(depth, want), s0
+    assert O.bfs_depth(host, n_rand + n_path - 1).max() > 255
