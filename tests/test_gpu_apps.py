@@ -356,3 +356,36 @@ def test_bc_batched_equals_sequential_on_fixture(gi):
     fus, _ = gi.bc(g, srcs, schedule=gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWCE, kernel_fusion=1))
     for got in (seq, bat, fus):
         assert np.allclose(got, want, rtol=1e-5)
+
+
+def test_sssp_fused_tiny_graphs(gi):
+    # no edges; one edge that is light (delta 2) or heavy (delta 1); a source
+    # whose only edge is heavy and wide windows do not appear
+    g0 = gi.Graph.from_csr(np.array([0, 0], np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32))
+    d, _ = gi.sssp(g0, 0, 4, _fused_sssp(gi))
+    assert d.tolist() == [0]
+    off = np.array([0, 1, 2], np.int32)
+    nb = np.array([1, 0], np.int32)
+    w = np.array([1, 1], np.int32)
+    g1 = gi.Graph.from_csr(off, nb, w)
+    for delta in (1, 2, 1000):
+        for src in (0, 1):
+            d, _ = gi.sssp(g1, src, delta, _fused_sssp(gi))
+            assert d.tolist() == ([0, 1] if src == 0 else [1, 0]), (delta, src)
+
+
+def test_bc_tiny_and_empty(gi):
+    fused = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWCE, kernel_fusion=1)
+    g0 = gi.Graph.from_edges(1, np.zeros(0, np.int32), np.zeros(0, np.int32), True)
+    for kw in ({"batch": 4}, {"schedule": fused}, {}):
+        sc, _ = gi.bc(g0, [0, 0], **kw)
+        assert sc.tolist() == [0.0], kw
+        sc, st = gi.bc(g0, np.zeros(0, np.int32), **kw)
+        assert sc.tolist() == [0.0] and st["iterations"] == 0, kw
+    # a path 0-1-2: vertex 1 lies on the 0..2 paths
+    e = np.array([[0, 1], [1, 2]], np.int32)
+    g = gi.Graph.from_edges(3, e[:, 0], e[:, 1], True)
+    want = O.bc(host_csr(g), np.array([0, 1, 2], np.int32))
+    for kw in ({"batch": 3}, {"schedule": fused}, {}):
+        sc, _ = gi.bc(g, [0, 1, 2], **kw)
+        assert np.allclose(sc, want, rtol=1e-6), kw
