@@ -124,172 +124,178 @@ def alg_frac(nbytes: float, seconds: float) -> float:
     return round(nbytes / seconds / (HBM_PEAK_GBS * 1e9), 4)
 
 
-def bench_apps(gi, g, sources, edges_of, log) -> dict:
+def bench_apps(gi, g, sources, edges_of, log, only: str | None = None) -> dict:
     """The other apps on their configurations, each through its fused
     schedule, with GTEPS and the fraction of the HBM roofline of their
     algorithmic bytes (one definition per app, stated with each)."""
     from easy_graphit_amd import autotune
     out = {}
     V, E = g.num_vertices, g.num_edges
-    # SSSP: RMAT-24 with int32 weights U[1,255], delta-stepping; the delta is
-    # the fastest of autotune.DELTAS on the first source
-    g.set_weights(SEED, 1, 255)
     fs = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWC, kernel_fusion=1)
-    srcs = [int(s) for s in sources[:4]]
-    tuned = autotune.tune_delta(g, srcs, log=log)
-    delta = tuned["delta"]
-    t, edges, vr, steps = 0.0, 0, 0, 0
-    for s in srcs:
-        gi.sssp(g, s, delta, fs)  # warm-up
-        dist, st = gi.sssp(g, s, delta, fs)
-        t += st["ms"] * 1e-3
-        edges += edges_of[s]
-        vr += int((dist < np.iinfo(np.int32).max).sum())
-        steps += st["iterations"]
-    out["sssp"] = {"gteps": round(edges / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3 / len(srcs), 3),
-                   "roofline_frac": alg_frac(16 * vr + 12 * edges, t), "relax_steps": steps,
-                   "alg_bytes": "16 V_reached + 12 E_reached", "config": "RMAT-24 sym, w U[1,255], "
-                   f"delta {delta} (fastest of {list(autotune.DELTAS)} summed over these sources), "
-                   f"{len(srcs)} sources",
-                   "delta_ms_summed": {r["delta"]: round(r["ms"], 3) for r in tuned["table"]}}
-    log(f"sssp: {out['sssp']}")
-    # SSSP on the grid workload: 2048 x 2048 4-neighbour grid plus 1% random
-    # long edges, w U[1,1000], source at the corner, delta 64
-    gr = gi.Graph.grid(2048, 2 * 2048 * 2047 // 100, SEED).set_weights(SEED, 1, 1000)
-    gi.sssp(gr, 0, 64, fs)
-    ts = []
-    for _ in range(3):
-        dist, st = gi.sssp(gr, 0, 64, fs)
-        ts.append(st["ms"] * 1e-3)
-    t = float(np.median(ts))
-    ge = gr.num_edges
-    tuned_g = autotune.tune_delta(gr, 0, deltas=(64, 256, 1024), reps=1)
-    out["sssp_grid"] = {"gteps": round(ge / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3),
-                        "relax_steps": st["iterations"],
-                        "roofline_frac": alg_frac(16 * gr.num_vertices + 12 * ge, t),
-                        "config": "2048x2048 grid + 1% random edges, w U[1,1000], source 0, delta 64",
-                        "delta_ms": {r["delta"]: round(r["ms"], 3) for r in tuned_g["table"]}}
-    log(f"sssp_grid: {out['sssp_grid']}")
-    gr.destroy()
-    # CC: RMAT-24, hook + shortcutting rounds
-    fc = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_EDGE_ONLY, kernel_fusion=1)
-    gi.cc(g, fc)
-    ts, rounds = [], 0
-    for _ in range(3):
-        _, st = gi.cc(g, fc)
-        ts.append(st["ms"] * 1e-3)
-        rounds = st["iterations"]
-    t = float(np.median(ts))
-    out["cc"] = {"gteps": round(E / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3), "rounds": rounds,
-                 "roofline_frac": alg_frac(rounds * (8 * E + 12 * V) + rounds * 8 * V, t),
-                 "alg_bytes": "per round 8 E + 12 V (hook) + 8 V (shortcut)", "config": "RMAT-24 sym"}
-    log(f"cc: {out['cc']}")
-    # the same, VERTEX_BASED: two neighbour-sampling passes, then rounds over
-    # the vertices outside the most frequent label only
-    fcs = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_VERTEX_BASED, kernel_fusion=1)
-    want_lab, _ = gi.cc(g, fc)
-    lab, _ = gi.cc(g, fcs)
-    if not np.array_equal(lab, want_lab):
-        raise SystemExit("sampled CC labels differ from the full sweeps'")
-    ts, rounds_s = [], 0
-    for _ in range(3):
-        _, st = gi.cc(g, fcs)
-        ts.append(st["ms"] * 1e-3)
-        rounds_s = st["iterations"]
-    t = float(np.median(ts))
-    out["cc_sampled"] = {"gteps": round(E / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3),
-                         "filtered_rounds": rounds_s,
-                         "roofline_frac": alg_frac(2 * (16 * V) + 2 * 8 * V + (rounds_s + 1) * 8 * V, t),
-                         "alg_bytes": "2 sampling passes x 16 V, 2 x 8 V shortcut, filtered rounds x 8 V "
-                                      "(+ the edges of vertices outside the giant label, not counted)",
-                         "config": "RMAT-24 sym, VERTEX_BASED fused schedule; labels equal the full sweeps'"}
-    log(f"cc_sampled: {out['cc_sampled']}")
-    # PageRank: RMAT-22 directed, 20 iterations, FP32 pull
-    gp = gi.Graph.rmat(22, 16, SEED, False).set_layout(g.layout)
-    fp = gi.Schedule(direction=gi.PULL, load_balancing=gi.LB_VERTEX_BASED, kernel_fusion=1)
-    gi.pagerank(gp, 20, schedule=fp)
-    ts = []
-    for _ in range(3):
-        _, st = gi.pagerank(gp, 20, schedule=fp)
-        ts.append(st["ms"] * 1e-3)
-    t = float(np.median(ts))
-    Vp, Ep = gp.num_vertices, gp.num_edges
-    out["pr"] = {"gteps": round(20 * Ep / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3),
-                 "iterations_per_s": round(20 / t, 1),
-                 "roofline_frac": alg_frac(20 * (8 * Ep + 20 * Vp), t),
-                 "alg_bytes": "20 (8 E + 20 V)",
-                 "config": f"RMAT-22 directed, 20 iterations, d 0.85, {'degree' if gp.layout else 'native'} layout"}
-    log(f"pr: {out['pr']}")
-    gp.destroy()
-    # Batched Brandes BC: RMAT-20, 256 sources, batched vs one source at a time
-    gb = gi.Graph.rmat(20, 16, SEED, True)
-    bsrc = gb.pick_sources(SEED, 256)
-    gi.bc(gb, bsrc[:8], 256)
-    _, st_b = gi.bc(gb, bsrc, 256)
-    gi.bc(gb, bsrc[:2], 1)
-    _, st_s = gi.bc(gb, bsrc, 1)
-    fused1 = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWCE, kernel_fusion=1)
-    gi.bc(gb, bsrc[:2], schedule=fused1)
-    _, st_f = gi.bc(gb, bsrc, schedule=fused1)
-    dtmp = gi.DeviceArray(gb.num_vertices, np.int32)
-    be = 0
-    for s in bsrc:
-        gi.bfs_device(gb, int(s), None, dtmp)
-        be += gb.reached_edges(dtmp)
-    tb, tsq, tf = st_b["ms"] * 1e-3, st_s["ms"] * 1e-3, st_f["ms"] * 1e-3
-    out["bc"] = {"sources": len(bsrc), "batched_ms": round(tb * 1e3, 2), "sequential_ms": round(tsq * 1e3, 2),
-                 "speedup": round(tsq / tb, 2), "gteps": round(be / 2 / tb / 1e9, 3),
-                 "fused_single_source_ms": round(tf * 1e3, 2), "speedup_vs_fused_single": round(tf / tb, 2),
-                 "config": "RMAT-20 sym, 256 sources: batch 256 vs one source at a time (unfused per-level "
-                           "host loop, and the fused single-source mode)"}
-    log(f"bc: {out['bc']}")
-    gb.destroy()
-    # 1,024 uniform graphs (V log-uniform in [1k, 64k], average degree 8), one
-    # fused multi-source BFS over the packed CSR with autotuned bins
-    gm, offs = gi.Graph.uniform_batch(1024, 1024, 65536, 8, SEED)
-    moff, _, _ = gm.read_csr()
-    msrc = []
-    for i in range(1024):  # first vertex with edges in each graph
-        deg = np.diff(moff[offs[i]:offs[i + 1] + 1])
-        nz = np.flatnonzero(deg)
-        msrc.append(int(offs[i] + (nz[0] if len(nz) else 0)))
-    msrc = np.array(msrc, np.int32)
-    tuned = autotune.tune_bins(gm, msrc)
-    sched = autotune.schedule(tuned["push_lb"])
-    depth = gi.DeviceArray(gm.num_vertices, np.int32)
-    t = autotune.time_multi(gm, msrc, sched, depth, 5)
-    me = gm.reached_edges(depth)
-    # one graph at a time: 32 of the graphs built on their own, scaled to 1,024
-    rng = np.random.default_rng(SEED)
-    pick = rng.choice(1024, 32, replace=False)
-    mnb = None
-    tone = 0.0
-    for i in pick:
-        v0, v1 = int(offs[i]), int(offs[i + 1])
-        if mnb is None:
-            _, mnb, _ = gm.read_csr()
-        sub_off = moff[v0:v1 + 1] - moff[v0]
-        sub_nb = mnb[moff[v0]:moff[v1]] - v0
-        gs = gi.Graph.from_csr(sub_off, sub_nb)
-        ds = gi.DeviceArray(v1 - v0, np.int32)
-        sc1 = autotune.schedule(gi.LB_TWC)
-        gi.bfs_device(gs, int(msrc[i] - v0), sc1, ds)
-        gi.synchronize()
-        t0 = time.perf_counter()
-        gi.bfs_device(gs, int(msrc[i] - v0), sc1, ds)
-        gi.synchronize()
-        tone += time.perf_counter() - t0
-        gs.destroy()
-    tone *= 1024 / len(pick)
-    out["many_graphs"] = {"graphs": 1024, "ms": round(t * 1e3, 3), "graphs_per_s": round(1024 / t, 1),
-                          "gteps": round(me / 2 / t / 1e9, 3),
-                          "one_at_a_time_ms": round(tone * 1e3, 2), "speedup": round(tone / t, 2),
-                          "bins": {k: tuned[k] for k in ("lane_max", "hub_min", "blocks_per_cu", "push_lb")},
-                          "default_bins_ms": round(tuned["default_seconds"] * 1e3, 3)
-                          if tuned["default_seconds"] else None,
-                          "config": "1,024 uniform graphs, V log-uniform [1k,64k], avg degree 8, packed CSR"}
-    log(f"many graphs: {out['many_graphs']}")
-    gm.destroy()
+    if only in (None, "sssp"):
+        # SSSP: RMAT-24 with int32 weights U[1,255], delta-stepping; the delta is
+        # the fastest of autotune.DELTAS summed over the timed sources
+        g.set_weights(SEED, 1, 255)
+        srcs = [int(s) for s in sources[:4]]
+        tuned = autotune.tune_delta(g, srcs, log=log)
+        delta = tuned["delta"]
+        t, edges, vr, steps = 0.0, 0, 0, 0
+        for s in srcs:
+            gi.sssp(g, s, delta, fs)  # warm-up
+            dist, st = gi.sssp(g, s, delta, fs)
+            t += st["ms"] * 1e-3
+            edges += edges_of[s]
+            vr += int((dist < np.iinfo(np.int32).max).sum())
+            steps += st["iterations"]
+        out["sssp"] = {"gteps": round(edges / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3 / len(srcs), 3),
+                       "roofline_frac": alg_frac(16 * vr + 12 * edges, t), "relax_steps": steps,
+                       "alg_bytes": "16 V_reached + 12 E_reached", "config": "RMAT-24 sym, w U[1,255], "
+                       f"delta {delta} (fastest of {list(autotune.DELTAS)} summed over these sources), "
+                       f"{len(srcs)} sources",
+                       "delta_ms_summed": {r["delta"]: round(r["ms"], 3) for r in tuned["table"]}}
+        log(f"sssp: {out['sssp']}")
+    if only in (None, "sssp_grid"):
+        # SSSP on the grid workload: 2048 x 2048 4-neighbour grid plus 1% random
+        # long edges, w U[1,1000], source at the corner, delta 64
+        gr = gi.Graph.grid(2048, 2 * 2048 * 2047 // 100, SEED).set_weights(SEED, 1, 1000)
+        gi.sssp(gr, 0, 64, fs)
+        ts = []
+        for _ in range(3):
+            dist, st = gi.sssp(gr, 0, 64, fs)
+            ts.append(st["ms"] * 1e-3)
+        t = float(np.median(ts))
+        ge = gr.num_edges
+        tuned_g = autotune.tune_delta(gr, 0, deltas=(64, 256, 1024), reps=1)
+        out["sssp_grid"] = {"gteps": round(ge / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3),
+                            "relax_steps": st["iterations"],
+                            "roofline_frac": alg_frac(16 * gr.num_vertices + 12 * ge, t),
+                            "config": "2048x2048 grid + 1% random edges, w U[1,1000], source 0, delta 64",
+                            "delta_ms": {r["delta"]: round(r["ms"], 3) for r in tuned_g["table"]}}
+        log(f"sssp_grid: {out['sssp_grid']}")
+        gr.destroy()
+    if only in (None, "cc"):
+        # CC: RMAT-24, hook + shortcutting rounds
+        fc = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_EDGE_ONLY, kernel_fusion=1)
+        gi.cc(g, fc)
+        ts, rounds = [], 0
+        for _ in range(3):
+            _, st = gi.cc(g, fc)
+            ts.append(st["ms"] * 1e-3)
+            rounds = st["iterations"]
+        t = float(np.median(ts))
+        out["cc"] = {"gteps": round(E / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3), "rounds": rounds,
+                     "roofline_frac": alg_frac(rounds * (8 * E + 12 * V) + rounds * 8 * V, t),
+                     "alg_bytes": "per round 8 E + 12 V (hook) + 8 V (shortcut)", "config": "RMAT-24 sym"}
+        log(f"cc: {out['cc']}")
+        # the same, VERTEX_BASED: two neighbour-sampling passes, then rounds over
+        # the vertices outside the most frequent label only
+        fcs = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_VERTEX_BASED, kernel_fusion=1)
+        want_lab, _ = gi.cc(g, fc)
+        lab, _ = gi.cc(g, fcs)
+        if not np.array_equal(lab, want_lab):
+            raise SystemExit("sampled CC labels differ from the full sweeps'")
+        ts, rounds_s = [], 0
+        for _ in range(3):
+            _, st = gi.cc(g, fcs)
+            ts.append(st["ms"] * 1e-3)
+            rounds_s = st["iterations"]
+        t = float(np.median(ts))
+        out["cc_sampled"] = {"gteps": round(E / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3),
+                             "filtered_rounds": rounds_s,
+                             "roofline_frac": alg_frac(2 * (16 * V) + 2 * 8 * V + (rounds_s + 1) * 8 * V, t),
+                             "alg_bytes": "2 sampling passes x 16 V, 2 x 8 V shortcut, filtered rounds x 8 V "
+                                          "(+ the edges of vertices outside the giant label, not counted)",
+                             "config": "RMAT-24 sym, VERTEX_BASED fused schedule; labels equal the full sweeps'"}
+        log(f"cc_sampled: {out['cc_sampled']}")
+    if only in (None, "pr"):
+        # PageRank: RMAT-22 directed, 20 iterations, FP32 pull
+        gp = gi.Graph.rmat(22, 16, SEED, False).set_layout(g.layout)
+        fp = gi.Schedule(direction=gi.PULL, load_balancing=gi.LB_VERTEX_BASED, kernel_fusion=1)
+        gi.pagerank(gp, 20, schedule=fp)
+        ts = []
+        for _ in range(3):
+            _, st = gi.pagerank(gp, 20, schedule=fp)
+            ts.append(st["ms"] * 1e-3)
+        t = float(np.median(ts))
+        Vp, Ep = gp.num_vertices, gp.num_edges
+        out["pr"] = {"gteps": round(20 * Ep / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3),
+                     "iterations_per_s": round(20 / t, 1),
+                     "roofline_frac": alg_frac(20 * (8 * Ep + 20 * Vp), t),
+                     "alg_bytes": "20 (8 E + 20 V)",
+                     "config": f"RMAT-22 directed, 20 iterations, d 0.85, {'degree' if gp.layout else 'native'} layout"}
+        log(f"pr: {out['pr']}")
+        gp.destroy()
+    if only in (None, "bc"):
+        # Batched Brandes BC: RMAT-20, 256 sources, batched vs one source at a time
+        gb = gi.Graph.rmat(20, 16, SEED, True)
+        bsrc = gb.pick_sources(SEED, 256)
+        gi.bc(gb, bsrc[:8], 256)
+        _, st_b = gi.bc(gb, bsrc, 256)
+        gi.bc(gb, bsrc[:2], 1)
+        _, st_s = gi.bc(gb, bsrc, 1)
+        fused1 = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWCE, kernel_fusion=1)
+        gi.bc(gb, bsrc[:2], schedule=fused1)
+        _, st_f = gi.bc(gb, bsrc, schedule=fused1)
+        dtmp = gi.DeviceArray(gb.num_vertices, np.int32)
+        be = 0
+        for s in bsrc:
+            gi.bfs_device(gb, int(s), None, dtmp)
+            be += gb.reached_edges(dtmp)
+        tb, tsq, tf = st_b["ms"] * 1e-3, st_s["ms"] * 1e-3, st_f["ms"] * 1e-3
+        out["bc"] = {"sources": len(bsrc), "batched_ms": round(tb * 1e3, 2), "sequential_ms": round(tsq * 1e3, 2),
+                     "speedup": round(tsq / tb, 2), "gteps": round(be / 2 / tb / 1e9, 3),
+                     "fused_single_source_ms": round(tf * 1e3, 2), "speedup_vs_fused_single": round(tf / tb, 2),
+                     "config": "RMAT-20 sym, 256 sources: batch 256 vs one source at a time (unfused per-level "
+                               "host loop, and the fused single-source mode)"}
+        log(f"bc: {out['bc']}")
+        gb.destroy()
+    if only in (None, "many"):
+        # 1,024 uniform graphs (V log-uniform in [1k, 64k], average degree 8), one
+        # fused multi-source BFS over the packed CSR with autotuned bins
+        gm, offs = gi.Graph.uniform_batch(1024, 1024, 65536, 8, SEED)
+        moff, _, _ = gm.read_csr()
+        msrc = []
+        for i in range(1024):  # first vertex with edges in each graph
+            deg = np.diff(moff[offs[i]:offs[i + 1] + 1])
+            nz = np.flatnonzero(deg)
+            msrc.append(int(offs[i] + (nz[0] if len(nz) else 0)))
+        msrc = np.array(msrc, np.int32)
+        tuned = autotune.tune_bins(gm, msrc)
+        sched = autotune.schedule(tuned["push_lb"])
+        depth = gi.DeviceArray(gm.num_vertices, np.int32)
+        t = autotune.time_multi(gm, msrc, sched, depth, 5)
+        me = gm.reached_edges(depth)
+        # one graph at a time: 32 of the graphs built on their own, scaled to 1,024
+        rng = np.random.default_rng(SEED)
+        pick = rng.choice(1024, 32, replace=False)
+        mnb = None
+        tone = 0.0
+        for i in pick:
+            v0, v1 = int(offs[i]), int(offs[i + 1])
+            if mnb is None:
+                _, mnb, _ = gm.read_csr()
+            sub_off = moff[v0:v1 + 1] - moff[v0]
+            sub_nb = mnb[moff[v0]:moff[v1]] - v0
+            gs = gi.Graph.from_csr(sub_off, sub_nb)
+            ds = gi.DeviceArray(v1 - v0, np.int32)
+            sc1 = autotune.schedule(gi.LB_TWC)
+            gi.bfs_device(gs, int(msrc[i] - v0), sc1, ds)
+            gi.synchronize()
+            t0 = time.perf_counter()
+            gi.bfs_device(gs, int(msrc[i] - v0), sc1, ds)
+            gi.synchronize()
+            tone += time.perf_counter() - t0
+            gs.destroy()
+        tone *= 1024 / len(pick)
+        out["many_graphs"] = {"graphs": 1024, "ms": round(t * 1e3, 3), "graphs_per_s": round(1024 / t, 1),
+                              "gteps": round(me / 2 / t / 1e9, 3),
+                              "one_at_a_time_ms": round(tone * 1e3, 2), "speedup": round(tone / t, 2),
+                              "bins": {k: tuned[k] for k in ("lane_max", "hub_min", "blocks_per_cu", "push_lb")},
+                              "default_bins_ms": round(tuned["default_seconds"] * 1e3, 3)
+                              if tuned["default_seconds"] else None,
+                              "config": "1,024 uniform graphs, V log-uniform [1k,64k], avg degree 8, packed CSR"}
+        log(f"many graphs: {out['many_graphs']}")
+        gm.destroy()
     return out
 
 
@@ -331,6 +337,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-apps", action="store_true", help="BFS line only (no SSSP/CC/PR/BC/many-graph runs)")
     ap.add_argument("--sources", type=int, default=64, help="distinct BFS sources cycled over")
+    ap.add_argument("--app", choices=["sssp", "sssp_grid", "cc", "pr", "bc", "many"], default=None,
+                    help="only this app's section (for profiling; prints its results, no bench line)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run three device-to-device copies of MIB MiB (HBM counter scale)")
     args = ap.parse_args()
@@ -396,6 +404,10 @@ def main():
                 raise SystemExit(f"learned and heuristic BFS disagree from source {src}")
         if i % 16 == 15:
             log(f"reference runs {i + 1}/{ns} ({time.perf_counter() - t0:.1f}s)")
+    if args.app:
+        print(json.dumps({"app": args.app, "results": bench_apps(gi, g, sources, edges_of, log, args.app)}),
+              flush=True)
+        return
     for k in range(args.warmup):
         gi.bfs_device(g, int(sources[k % ns]), sched, depth)
     gi.synchronize()
