@@ -57,6 +57,7 @@ def lib():
             "or_transpose": (None, [i32, i32p, i32p, C.c_void_p, i32p, i32p, C.c_void_p]),
             "or_pick_sources": (i32, [i32, i32p, u64, i32, i32p]),
             "or_bfs_depth": (i32, [i32, i32p, i32p, i32, i32p]),
+            "or_bfs_depth_omp": (i32, [i32, i32p, i32p, i32, i32, i32, i32p, C.POINTER(C.c_int32)]),
             "or_bfs_parent_valid": (i32, [i32, i32p, i32p, i32, i32p, i32p]),
             "or_bfs_multi_depth": (i32, [i32, i32p, i32p, i32p, i32, i32p]),
             "or_bfs_pull_examined": (i64, [i32, i32p, i32p, i32p, i32]),
@@ -240,6 +241,16 @@ def pick_sources(g: CSR, seed: int, count: int):
 
 
 # ---------------------------------------------------------------- algorithms
+def bfs_depth_omp(g: CSR, source: int, threads: int = 0):
+    """Direction-optimizing OpenMP BFS (the bench's CPU baseline): (depth,
+    threads used). Bottom-up steps only on symmetric graphs."""
+    d = np.empty(g.num_vertices, np.int32)
+    used = C.c_int32(0)
+    lib().or_bfs_depth_omp(g.num_vertices, g.offsets, g.nbrs, source, int(not g.directed), threads, d,
+                           C.byref(used))
+    return d, used.value
+
+
 def bfs_depth(g: CSR, source: int):
     d = np.empty(g.num_vertices, np.int32)
     lib().or_bfs_depth(g.num_vertices, g.offsets, g.nbrs, source, d)

@@ -19,6 +19,8 @@
  *   or_transpose                   scipy.sparse transpose; double transpose = identity.
  *   or_bfs_depth                   scipy.sparse.csgraph.breadth_first_order and
  *                                  shortest_path(unweighted); fixtures in tests/golden.
+ *   or_bfs_depth_omp               equal to or_bfs_depth on every test graph and
+ *                                  thread count (the CPU baseline of the bench; OpenMP).
  *   or_sssp_dijkstra               scipy.sparse.csgraph.dijkstra; or_sssp_delta_stepping.
  *   or_cc_labels                   scipy connected_components (mapped to min id);
  *                                  or_cc_label_propagation (apps/cc.cpp restated).
@@ -32,6 +34,7 @@
 #include "oracle.h"
 
 #include <limits.h>
+#include <omp.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -304,6 +307,121 @@ int32_t or_bfs_depth(int32_t num_vertices, const int32_t *offsets, const int32_t
     }
     free(queue);
     return levels;
+}
+
+/* The bench's CPU baseline: direction-optimizing BFS (Beamer, Asanovic and
+ * Patterson, SC'12: top-down while the frontier's out-edges are at most
+ * 1/alpha of the unexplored edges, bottom-up until the frontier shrinks
+ * below V/beta; alpha 15, beta 18, the published defaults GAPBS uses) over
+ * OpenMP threads. Top-down claims with compare-and-swap on depth and
+ * per-thread queue blocks; bottom-up scans the unvisited vertices' lists for
+ * a frontier member (a bitmap). Bottom-up needs in-neighbours, so a directed
+ * graph (symmetric == 0) runs top-down only. nthreads <= 0: OpenMP's default.
+ * Returns the number of levels; *threads_used gets the team size. */
+int32_t or_bfs_depth_omp(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,
+                         int32_t source, int32_t symmetric, int32_t nthreads, int32_t *depth,
+                         int32_t *threads_used) {
+    const int32_t V = num_vertices;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    int32_t team = 1;
+#pragma omp parallel
+    {
+#pragma omp single
+        team = omp_get_num_threads();
+    }
+    if (threads_used) *threads_used = team;
+#pragma omp parallel for schedule(static)
+    for (int32_t v = 0; v < V; ++v) depth[v] = -1;
+    if (source < 0 || source >= V) return 0;
+    const int64_t words = ((int64_t)V + 63) / 64;
+    int32_t *queue = (int32_t *)malloc((size_t)V * sizeof(int32_t));
+    uint64_t *front = (uint64_t *)calloc((size_t)words, sizeof(uint64_t));
+    uint64_t *next = (uint64_t *)calloc((size_t)words, sizeof(uint64_t));
+    int64_t qn = 1;  /* frontier size (queue form during top-down) */
+    queue[0] = source;
+    depth[source] = 0;
+    int64_t edges_left = (int64_t)offsets[V] - (offsets[source + 1] - offsets[source]);
+    int64_t mf = offsets[source + 1] - offsets[source];
+    int32_t level = 0;
+    int bottom_up = 0;
+    while (qn > 0) {
+        if (!bottom_up && symmetric && mf > edges_left / 15) {
+            /* switch: queue -> bitmap */
+            memset(front, 0, (size_t)words * sizeof(uint64_t));
+            for (int64_t i = 0; i < qn; ++i) front[queue[i] >> 6] |= 1ull << (queue[i] & 63);
+            bottom_up = 1;
+        } else if (bottom_up && qn < V / 18) {
+            /* switch back: bitmap -> queue */
+            int64_t n = 0;
+            for (int64_t w = 0; w < words; ++w)
+                for (uint64_t m = front[w]; m; m &= m - 1) queue[n++] = (int32_t)(w * 64 + __builtin_ctzll(m));
+            qn = n;
+            bottom_up = 0;
+        }
+        int64_t next_n = 0, next_mf = 0;
+        if (bottom_up) {
+            memset(next, 0, (size_t)words * sizeof(uint64_t));
+#pragma omp parallel for schedule(dynamic, 1024) reduction(+ : next_n, next_mf)
+            for (int32_t v = 0; v < V; ++v) {
+                if (depth[v] != -1) continue;
+                for (int32_t e = offsets[v]; e < offsets[v + 1]; ++e) {
+                    const int32_t u = nbrs[e];
+                    if ((front[u >> 6] >> (u & 63)) & 1ull) {
+                        depth[v] = level + 1;
+                        __atomic_fetch_or(&next[v >> 6], 1ull << (v & 63), __ATOMIC_RELAXED);
+                        next_n += 1;
+                        next_mf += offsets[v + 1] - offsets[v];
+                        break;
+                    }
+                }
+            }
+            uint64_t *t = front;
+            front = next;
+            next = t;
+        } else {
+            int32_t *out = (int32_t *)malloc((size_t)(V > 0 ? V : 1) * sizeof(int32_t));
+            int64_t out_n = 0;
+#pragma omp parallel reduction(+ : next_mf)
+            {
+                int32_t local[1024];
+                int32_t ln = 0;
+#pragma omp for schedule(dynamic, 64) nowait
+                for (int64_t i = 0; i < qn; ++i) {
+                    const int32_t u = queue[i];
+                    for (int32_t e = offsets[u]; e < offsets[u + 1]; ++e) {
+                        const int32_t v = nbrs[e];
+                        int32_t expect = -1;
+                        if (__atomic_load_n(&depth[v], __ATOMIC_RELAXED) == -1 &&
+                            __atomic_compare_exchange_n(&depth[v], &expect, level + 1, 0, __ATOMIC_RELAXED,
+                                                        __ATOMIC_RELAXED)) {
+                            next_mf += offsets[v + 1] - offsets[v];
+                            local[ln++] = v;
+                            if (ln == 1024) {
+                                const int64_t at = __atomic_fetch_add(&out_n, ln, __ATOMIC_RELAXED);
+                                memcpy(out + at, local, sizeof(int32_t) * (size_t)ln);
+                                ln = 0;
+                            }
+                        }
+                    }
+                }
+                if (ln) {
+                    const int64_t at = __atomic_fetch_add(&out_n, ln, __ATOMIC_RELAXED);
+                    memcpy(out + at, local, sizeof(int32_t) * (size_t)ln);
+                }
+            }
+            memcpy(queue, out, sizeof(int32_t) * (size_t)out_n);
+            free(out);
+            next_n = out_n;
+        }
+        edges_left -= next_mf;
+        mf = next_mf;
+        qn = next_n;
+        ++level;
+    }
+    free(queue);
+    free(front);
+    free(next);
+    return level;
 }
 
 /* Multi-source BFS: distance to the nearest source (queue seeded with every

@@ -48,6 +48,9 @@ int32_t or_pick_sources(int32_t num_vertices, const int32_t *offsets, uint64_t s
 /* ---- algorithms ------------------------------------------------------- */
 int32_t or_bfs_depth(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,
                      int32_t source, int32_t *depth);
+int32_t or_bfs_depth_omp(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,
+                         int32_t source, int32_t symmetric, int32_t nthreads, int32_t *depth,
+                         int32_t *threads_used);
 int32_t or_bfs_multi_depth(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,
                            const int32_t *sources, int32_t num_sources, int32_t *depth);
 int32_t or_bfs_parent_valid(int32_t num_vertices, const int32_t *offsets,

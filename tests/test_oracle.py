@@ -424,3 +424,22 @@ def test_uniform_batch_sizes_ranges_and_degree():
     o2, s2, _ = O.uniform_batch_edges(G, vmin, vmax, deg, 5)
     assert np.array_equal(o2, offs) and np.array_equal(s2, s)
     assert not np.array_equal(O.uniform_batch_edges(G, vmin, vmax, deg, 6)[0], offs)
+
+
+def test_openmp_bfs_matches_serial():
+    # the bench's CPU baseline: direction-optimizing OpenMP BFS, equal depths
+    # to the serial queue BFS for every thread count, top-down only on a
+    # directed graph
+    s, d = O.rmat_edges(13, 16, 5)
+    sym = O.build_csr(1 << 13, s, d, True)
+    dirg = O.build_csr(1 << 13, s, d, False)
+    gs, gd = O.grid_edges(60, 40, 2)
+    grid = O.build_csr(60 * 60, gs, gd, True)
+    path = O.build_csr(3000, np.arange(2999), np.arange(1, 3000), True)
+    for g in (sym, dirg, grid, path):
+        for src in list(O.pick_sources(g, 3, 3)) + [0]:
+            want = O.bfs_depth(g, int(src))
+            for threads in (1, 2, 8):
+                got, used = O.bfs_depth_omp(g, int(src), threads)
+                assert np.array_equal(got, want), (g.directed, src, threads)
+                assert used == threads
