@@ -3,8 +3,9 @@
  * (Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg;
  * never by the product library.)
  *
- * Serial C restatement of what the reference computes. Citations are
- * reference-relative file:line. How each function is pinned:
+ * Serial C restatement of what the reference computes, plus the bench's
+ * OpenMP CPU baseline (or_bfs_depth_omp). Citations are reference-relative
+ * file:line. How each function is pinned:
  *
  *   or_splitmix64_mix / or_hash64  SplitMix64 published output vectors
  *                                  (seed 0 and seed 1234567) in tests/test_oracle.py.
