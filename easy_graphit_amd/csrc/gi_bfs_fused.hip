@@ -310,7 +310,7 @@ __device__ __forceinline__ uint64_t wave_or(uint64_t x) {
 constexpr int kPullWords = 4;     // words a wave works on at once (memory-level parallelism; pick4)
 constexpr int kPullUnit = kPullWords * (kStepBlock / 64);  // words per workgroup unit
 static_assert(kChunkWords % kPullUnit == 0, "a pull unit lies inside one chunk");
-constexpr int kPullLaneSteps = 8; // PULL_GROUP: lane-serial edges before the cooperative scan
+constexpr int kPullLaneSteps = 4; // PULL_GROUP: lane-serial edges before the cooperative scan (4: RMAT-24 group levels 120 -> 115 us against 8; 2, 3, 5, 6, 16, 32 no better)
 
 // Cooperative scan of the open vertices of one word: their remaining in-edge
 // lists [e, e1) concatenated, 64 edges per pass, until every open vertex has
