@@ -955,7 +955,8 @@ struct FusedWork {
     // captured level loops by (level count, selector view bytes): the decide
     // kernel takes the view by value
     std::map<std::pair<int32_t, std::string>, hipGraphExec_t> execs;
-    int32_t last_levels = 0;
+    int32_t recent_levels[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // depths of the last runs (ring)
+    int32_t recent_n = 0;
 };
 
 static std::map<const gi_graph *, FusedWork *> g_fused;
@@ -1249,13 +1250,17 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
         }
         return fail(GI_ERR_RANGE, "BFS deeper than %d levels", kBfsMaxLevels);
     }
-    // Graph mode: replay a captured block of levels sized by the previous run
-    // on this graph, read {done, level, levels} once per replay (the host
-    // waits for that copy only, while the epilogue is already queued),
-    // continue with longer blocks for deep graphs (levels past the end are
-    // no-op launches).
-    int32_t block = w->last_levels > 0 ? w->last_levels : 8;
+    // Graph mode: replay a captured block of levels sized by the deepest of
+    // the last runs on this graph (a level past the end costs three no-op
+    // launches, a block too short a whole extra replay), read {done, level,
+    // levels} once per replay (the host waits for that copy only, while the
+    // epilogue is already queued), continue with blocks of 8, 16, 32, 48
+    // levels for deeper runs.
+    int32_t block = 0;
+    for (int32_t l : w->recent_levels) block = l > block ? l : block;
+    if (block <= 0) block = 8;
     if (block > 48) block = 48;
+    int32_t next_block = 8;
     int32_t level = 0;
     while (level < kBfsMaxLevels) {
         hipGraphExec_t exec = nullptr;
@@ -1270,7 +1275,8 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
             *levels = w->host_flag[2];
             return GI_OK;
         }
-        block = 48;
+        block = next_block;
+        next_block = next_block < 48 ? (next_block * 2 < 48 ? next_block * 2 : 48) : 48;
     }
     return fail(GI_ERR_RANGE, "BFS deeper than %d levels", kBfsMaxLevels);
 }
@@ -1289,7 +1295,7 @@ int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources
                     sel.mode == GI_SELECT_LEARNED, s));
     int32_t levels = 0;
     GI_TRY(run_levels(g, w, s, trace, timed, epilogue, &levels));
-    w->last_levels = levels;
+    w->recent_levels[w->recent_n++ % 8] = levels;
     if (levels_out) *levels_out = levels;
     if (trace) {
         int32_t n = levels < kBfsMaxLevels ? levels : kBfsMaxLevels;
