@@ -481,7 +481,13 @@ struct BcRun {
     BcState *host = nullptr;  // pinned
     hipStream_t capture = nullptr;
     std::map<std::tuple<int32_t, int32_t, int32_t>, hipGraphExec_t> execs;  // (phase, slots, B)
-    int32_t hint = 8;  // levels of the last batch
+    int32_t recent[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // levels of the last batches (ring)
+    int32_t recent_n = 0;
+    int32_t hint() const {  // the deepest of them (8 before the first)
+        int32_t m = 0;
+        for (int32_t l : recent) m = l > m ? l : m;
+        return m > 0 ? m : 8;
+    }
     ~BcRun() {
         for (auto &kv : execs) (void)hipGraphExecDestroy(kv.second);
         if (capture) (void)hipStreamDestroy(capture);
@@ -525,8 +531,9 @@ static void bc_launch_slot(const BcArgs &a, int32_t phase, hipStream_t s) {
     }
 }
 
-// Replays blocks of captured slots of one pass until the state leaves it
-// (a slot past the end is a no-op).
+// Replays blocks of captured slots of one pass until the state leaves it:
+// first `first` slots (rounded up to 4), then 4 at a time (a slot past the
+// end costs its no-op launches, a block too short a host round trip).
 static int bc_pass(BcRun &r, const BcArgs &a, int32_t phase, int32_t first, hipStream_t s) {
     int32_t block = first < 4 ? 4 : (first + 3) / 4 * 4;
     for (int64_t used = 0; used < (1ll << 24);) {
@@ -550,7 +557,7 @@ static int bc_pass(BcRun &r, const BcArgs &a, int32_t phase, int32_t first, hipS
         GI_HIP(hipStreamSynchronize(s));
         used += block;
         if (r.host->phase != phase) return GI_OK;
-        block *= 2;
+        block = 4;
     }
     return fail(GI_ERR_RANGE, "BC did not finish");
 }
@@ -578,12 +585,12 @@ static int bc_batch_run(gi_graph *g, BcRun &r, const int32_t *h_src, int32_t B, 
     bc_reset_kernel<<<grid_for((int64_t)B * g->V), 256, 0, s>>>(a);
     bc_seed_kernel_b<<<(B + 63) / 64, 64, 0, s>>>(r.d_src, B, g->V, a.words, r.depth, r.sigma, r.bm[0]);
     GI_LAUNCH_CHECK();
-    GI_TRY(bc_pass(r, a, BC_FWD, r.hint, s));
+    GI_TRY(bc_pass(r, a, BC_FWD, r.hint(), s));
     const int32_t L = r.host->levels;
     GI_TRY(bc_pass(r, a, BC_BWD, L, s));  // l = L-2 .. -1: L slots
     bc_final_kernel<<<grid_for(g->V), 256, 0, s>>>(r.sigma, r.dep, B, g->V, d_scores);
     GI_LAUNCH_CHECK();
-    r.hint = L;
+    r.recent[r.recent_n++ % 8] = L;
     if (levels) *levels = L;
     return GI_OK;
 }

@@ -83,6 +83,7 @@ struct SsState {
     int32_t defer;       // the next push step relaxes light edges only and records F in W
     int32_t ws_any;      // W holds vertices whose heavy edges wait
     long long fdeg;      // out-edges of F
+    int32_t *dist;       // the run's distances (captured slots read it from here)
     int32_t hubs[kSsMaxHubs];
 };
 
@@ -399,6 +400,7 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int3
 __global__ void __launch_bounds__(kSsBlock) ss_relax_kernel(SsArgs a) {
     SsState *st = a.st;
     if (st->done || st->mode == SS_ADVANCE) return;  // uniform
+    a.dist = st->dist;
     __shared__ int32_t pre[kBsMaxSumWords + 1];
     __shared__ int32_t lds[kBsChunkVerts];
     if (st->pull)
@@ -413,6 +415,7 @@ __global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a) {
     if (st->done || st->mode == SS_ADVANCE || st->pull) return;
     int32_t nh = st->hub_count;
     if (nh <= 0) return;  // slots stay zero
+    a.dist = st->dist;
     if (nh > kSsMaxHubs) nh = kSsMaxHubs;
     const bool heavy = st->mode == SS_HEAVY, defer = !heavy && st->defer;  // the step's edge range
     const int32_t *nbr = heavy || defer ? a.snbr : a.nbr;
@@ -550,6 +553,7 @@ __global__ void __launch_bounds__(kSsDecBlock) ss_decide_kernel(SsArgs a, int32_
 __global__ void __launch_bounds__(kSsBlock) ss_farmove_kernel(SsArgs a) {
     SsState *st = a.st;
     if (st->done || st->mode != SS_ADVANCE) return;
+    a.dist = st->dist;
     __shared__ int32_t ured[2][kSsBlock / 64];
     const int32_t lo = st->lo, hi = st->hi, phase = st->phase;
     uint64_t *f = bs_pick3(a.bm[0], a.bm[1], a.bm[2], phase % 3);  // empty: the last step found nothing
@@ -700,6 +704,7 @@ __global__ void ss_reset_kernel(SsArgs a, SsSeed sd) {
         st->ws_any = 0;
         st->fdeg = 0;
         st->pull_min = sd.pull_min;
+        st->dist = a.dist;
     }
 }
 
@@ -756,8 +761,9 @@ struct SsWork {
     int32_t wide_chunks = 0;
     int32_t *chunk_v = nullptr, *chunk_e0 = nullptr;
     hipStream_t cap = nullptr;
-    std::map<std::pair<int32_t, int32_t *>, hipGraphExec_t> execs;  // (slots, dist)
-    int32_t last_slots = 0;
+    std::map<int32_t, hipGraphExec_t> execs;  // by slot count (the run's dist is read from the state)
+    int32_t recent_slots[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // slots of the last runs (ring)
+    int32_t recent_n = 0;
 };
 
 static std::map<const gi_graph *, SsWork *> g_ss;
@@ -971,11 +977,19 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
     int rg = (int)ceil_div(n, 256);
     ss_reset_kernel<<<rg < 8192 ? rg : 8192, 256, 0, s>>>(a, SsSeed{source, delta, w->pull_min});
     GI_LAUNCH_CHECK();
-    int32_t block = w->last_slots > 0 ? w->last_slots : 16;
-    for (int64_t used = 0; used < (1ll << 24);) {
-        if (block > 64) block = 64;
-        auto key = std::make_pair(block, d_dist);
-        auto it = w->execs.find(key);
+    // the first block holds as many slots as the longest of the last runs
+    // (rounded up to 8; a slot past the end is five no-op launches, a block
+    // too short a host round trip), later ones 16
+    auto first_block = [&]() {
+        int32_t b = 0;
+        for (int32_t k : w->recent_slots) b = k > b ? k : b;
+        b = b > 0 ? (b + 7) / 8 * 8 : 16;
+        return b < 1024 ? b : 1024;
+    };
+    SsArgs ca = a;
+    ca.dist = nullptr;  // captured kernels take the run's dist from the state
+    auto exec_of = [&](int32_t block, hipGraphExec_t *out) -> int {
+        auto it = w->execs.find(block);
         if (it == w->execs.end()) {
             if (w->execs.size() >= 8) {
                 for (auto &kv : w->execs) (void)hipGraphExecDestroy(kv.second);
@@ -983,30 +997,40 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
             }
             hipGraph_t graph = nullptr;
             GI_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
-            for (int32_t r = 0; r < block; ++r) launch_slot(a, w->hubs, w->cap);
+            for (int32_t r = 0; r < block; ++r) launch_slot(ca, w->hubs, w->cap);
             hipError_t e = hipStreamEndCapture(w->cap, &graph);
             if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
             hipGraphExec_t exec = nullptr;
             e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
-            it = w->execs.emplace(key, exec).first;
+            it = w->execs.emplace(block, exec).first;
         }
-        GI_HIP(hipGraphLaunch(it->second, s));
+        *out = it->second;
+        return GI_OK;
+    };
+    int32_t block = first_block();
+    for (int64_t used = 0; used < (1ll << 24);) {
+        hipGraphExec_t exec = nullptr;
+        GI_TRY(exec_of(block, &exec));
+        GI_HIP(hipGraphLaunch(exec, s));
         GI_HIP(hipMemcpyAsync(w->host, w->st, 7 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
         GI_HIP(hipStreamSynchronize(s));
         used += block;
         if (w->host[0]) {  // done
-            w->last_slots = w->host[5];
+            w->recent_slots[w->recent_n++ % 8] = w->host[5];
             if (counts) {  // steps, pull steps, windows entered, slots
                 counts[0] = w->host[3];
                 counts[1] = w->host[6];
                 counts[2] = w->host[4];
                 counts[3] = w->host[5];
             }
-            return GI_OK;
+            // a longer run than before: capture the next run's first block
+            // now, so that its launch does not wait on the capture
+            hipGraphExec_t next = nullptr;
+            return exec_of(first_block(), &next);
         }
-        block *= 2;
+        block = 16;
     }
     return fail(GI_ERR_RANGE, "SSSP did not finish");
 }
