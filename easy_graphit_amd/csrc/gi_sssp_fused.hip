@@ -955,6 +955,7 @@ static int ensure_split(gi_graph *g, SsWork *w, int32_t delta, hipStream_t s) {
     }
     w->split = use;
     w->split_delta = delta;
+    for (int32_t &k : w->recent_slots) k = 0;  // slot counts of other deltas do not carry over
     return GI_OK;
 }
 
