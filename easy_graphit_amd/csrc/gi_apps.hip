@@ -164,8 +164,11 @@ static int bfs_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int
             GI_TRY(layout_depth_back(g, rdepth, source, d_depth, st));
             return d_parent ? layout_map_back(g, rparent, d_parent, true, st) : GI_OK;
         };
+        // on a symmetric graph's view the map-back reads the depths of the
+        // vertices with edges only: the edgeless tail needs no clearing
+        const int32_t depth_len = g->layout && !g->directed ? layout_iso_begin(g) : -1;
         int rc = bfs_fused_run(rg, rsrc, h, sel, rdepth, rparent, s, want_trace ? &ft : nullptr,
-                               trace != nullptr, &levels, g->layout ? &back : nullptr);
+                               trace != nullptr, &levels, g->layout ? &back : nullptr, depth_len);
         float ms = 0.f;
         if (stats && rc == GI_OK) {
             GI_HIP(hipEventRecord(t1, s));

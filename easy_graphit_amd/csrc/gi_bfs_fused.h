@@ -122,14 +122,16 @@ int32_t fused_mode_of(const gi_schedule &s);
 using FusedEpilogue = std::function<int(hipStream_t)>;
 int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, SelectorView sel,
                   int32_t *d_depth, int32_t *d_parent, hipStream_t s, FusedTrace *trace,
-                  bool timed, int32_t *levels_out, const FusedEpilogue *epilogue = nullptr);
+                  bool timed, int32_t *levels_out, const FusedEpilogue *epilogue = nullptr,
+                  int32_t depth_len = -1);
 void release_fused(const gi_graph *g);
 int fused_reached(gi_graph *g, const int32_t *depth, int32_t *out, hipStream_t s);
 int fused_set_bins(gi_graph *g, int32_t lane_max, int32_t hub_min, int32_t blocks_per_cu);
 int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources, int32_t single,
                         const gi_hybrid_schedule *h, SelectorView sel, int32_t *d_depth,
                         int32_t *d_parent, hipStream_t s, FusedTrace *trace, bool timed,
-                        int32_t *levels_out, const FusedEpilogue *epilogue = nullptr);
+                        int32_t *levels_out, const FusedEpilogue *epilogue = nullptr,
+                        int32_t depth_len = -1);
 // The process-wide selector model (null weights when none is loaded).
 SelectorView selector_view();
 

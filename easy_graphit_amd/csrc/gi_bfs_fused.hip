@@ -806,6 +806,7 @@ struct SeedArgs {
 
 struct ResetArgs {
     int32_t *depth;
+    int32_t depth_len;      // depth entries to clear (the rest is never read)
     int32_t *parent;
     uint64_t *visited;
     const uint64_t *dead;   // vertices without in-edges, and the padding past V
@@ -821,10 +822,8 @@ __global__ void bfs_reset_kernel(BfsState *st, SeedArgs sa, ResetArgs r, int64_t
     const int32_t stride = gridDim.x * blockDim.x;
     const int32_t n = r.V > r.sum_words + 1 ? r.V : r.sum_words + 1;  // words <= V
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        if (i < r.V) {
-            r.depth[i] = -1;
-            if (r.parent) r.parent[i] = -1;
-        }
+        if (i < r.depth_len) r.depth[i] = -1;
+        if (i < r.V && r.parent) r.parent[i] = -1;
         if (i < r.words) {
             r.visited[i] = r.dead[i];
             r.bm[0][i] = 0;
@@ -1155,11 +1154,12 @@ static void launch_level_traced(gi_graph *g, FusedWork *w, const StepArgs &a, in
 // sources == null seeds the single vertex `single`.
 static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t n_sources,
                     int32_t single, int32_t *depth, int32_t *parent, const SelectorView &sel,
-                    bool want_fdeg, hipStream_t s) {
+                    bool want_fdeg, hipStream_t s, int32_t depth_len = -1) {
     w->last_sel = sel;  // captured level graphs are kept per selector view (level_graph)
     SeedArgs sa{depth, parent, w->lane_max, w->hub_min, want_fdeg ? 1 : 0};
     ResetArgs r;
     r.depth = depth;
+    r.depth_len = depth_len >= 0 && depth_len < g->V ? depth_len : g->V;
     r.parent = parent;
     r.visited = g->d_visited;
     r.dead = w->dead;
@@ -1284,7 +1284,7 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
 int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources, int32_t single,
                         const gi_hybrid_schedule *h, SelectorView sel, int32_t *d_depth,
                         int32_t *d_parent, hipStream_t s, FusedTrace *trace, bool timed,
-                        int32_t *levels_out, const FusedEpilogue *epilogue) {
+                        int32_t *levels_out, const FusedEpilogue *epilogue, int32_t depth_len) {
     FusedWork *w = nullptr;
     GI_TRY(ensure_fused(g, &w));
     sel.mode_s1 = fused_mode_of(h->s1);
@@ -1292,7 +1292,7 @@ int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources
     sel.threshold = h->threshold;
     if (sel.mode != GI_SELECT_LEARNED) sel.mode = h->selector;
     GI_TRY(seed_run(g, w, h_sources, n_sources, single, d_depth, d_parent, sel,
-                    sel.mode == GI_SELECT_LEARNED, s));
+                    sel.mode == GI_SELECT_LEARNED, s, depth_len));
     int32_t levels = 0;
     GI_TRY(run_levels(g, w, s, trace, timed, epilogue, &levels));
     w->recent_levels[w->recent_n++ % 8] = levels;
@@ -1309,9 +1309,9 @@ int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources
 
 int bfs_fused_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, SelectorView sel,
                   int32_t *d_depth, int32_t *d_parent, hipStream_t s, FusedTrace *trace,
-                  bool timed, int32_t *levels_out, const FusedEpilogue *epilogue) {
+                  bool timed, int32_t *levels_out, const FusedEpilogue *epilogue, int32_t depth_len) {
     return bfs_fused_run_multi(g, nullptr, 0, source, h, sel, d_depth, d_parent, s, trace, timed,
-                               levels_out, epilogue);
+                               levels_out, epilogue, depth_len);
 }
 
 int fused_reached(gi_graph *g, const int32_t *depth, int32_t *out, hipStream_t s) {

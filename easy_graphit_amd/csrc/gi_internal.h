@@ -293,6 +293,8 @@ int build_layout(gi_graph *g);  // degree-ordered copy (with the weights, if any
 gi_graph *layout_view(const gi_graph *g);
 int32_t layout_map_source(const gi_graph *g, int32_t v);
 int32_t *layout_tmp(const gi_graph *g, int i);
+// First vertex of the view without edges (V when all have edges).
+int32_t layout_iso_begin(const gi_graph *g);
 int layout_map_back(const gi_graph *g, const int32_t *in_new, int32_t *out_old, bool ids, hipStream_t s);
 // BFS depths of a single-source run back to original ids; on a symmetric
 // graph the edgeless tail of the view is filled without reading depth_new.

@@ -273,6 +273,8 @@ int build_layout(gi_graph *g) {
 
 gi_graph *layout_view(const gi_graph *g) { return g->layout ? g->layout->view : nullptr; }
 
+int32_t layout_iso_begin(const gi_graph *g) { return g->layout ? g->layout->iso_begin : g->V; }
+
 int32_t layout_map_source(const gi_graph *g, int32_t v) { return g->layout->h_new_of_old[(size_t)v]; }
 
 int32_t *layout_tmp(const gi_graph *g, int i) { return g->layout->tmp[i]; }
