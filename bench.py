@@ -184,9 +184,12 @@ def bench_apps(gi, g, sources, edges_of, log, only: str | None = None) -> dict:
             rounds = st["iterations"]
         t = float(np.median(ts))
         out["cc"] = {"gteps": round(E / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3), "rounds": rounds,
-                     "roofline_frac": alg_frac(rounds * (6 * E + 12 * V) + rounds * 8 * V, t),
-                     "alg_bytes": "per round 6 E + 12 V (hook: every neighbour id, the label of one end of each "
-                                  "symmetric pair) + 8 V (shortcut)", "config": "RMAT-24 sym"}
+                     "roofline_frac": alg_frac(rounds * (8 * E + 12 * V) + rounds * 8 * V, t),
+                     "alg_bytes": "per round 8 E + 12 V (hook) + 8 V (shortcut)",
+                     "roofline_frac_bytes_read": alg_frac(rounds * (6 * E + 12 * V) + rounds * 8 * V, t),
+                     "bytes_read": "per round 6 E + 12 V (the sweep hooks one direction of each symmetric pair: "
+                                   "every neighbour id, half the label gathers) + 8 V",
+                     "config": "RMAT-24 sym"}
         log(f"cc: {out['cc']}")
         # the same, VERTEX_BASED: two neighbour-sampling passes, then rounds over
         # the vertices outside the most frequent label only
