@@ -479,7 +479,6 @@ struct BcRun {
     uint64_t *bm[3] = {nullptr, nullptr, nullptr};
     BcState *st = nullptr;
     BcState *host = nullptr;  // pinned
-    hipStream_t capture = nullptr;
     std::map<std::tuple<int32_t, int32_t, int32_t>, hipGraphExec_t> execs;  // (phase, slots, B)
     int32_t recent[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // levels of the last batches (ring)
     int32_t recent_n = 0;
@@ -490,7 +489,6 @@ struct BcRun {
     }
     ~BcRun() {
         for (auto &kv : execs) (void)hipGraphExecDestroy(kv.second);
-        if (capture) (void)hipStreamDestroy(capture);
         dfree(depth);
         dfree(d_src);
         dfree(sigma);
@@ -513,21 +511,19 @@ struct BcRun {
         GI_TRY(dalloc(&d_src, (size_t)B));
         GI_TRY(dalloc(&st, 1));
         if (hipHostMalloc((void **)&host, sizeof(BcState)) != hipSuccess) return fail(GI_ERR_OOM, "pinned allocation failed");
-        if (hipStreamCreateWithFlags(&capture, hipStreamNonBlocking) != hipSuccess)
-            return fail(GI_ERR_HIP, "stream create failed");
         return GI_OK;
     }
 };
 
-static void bc_launch_slot(const BcArgs &a, int32_t phase, hipStream_t s) {
+static void bc_launch_slot(const BcArgs &a, int32_t phase, Launcher &L) {
     const int grid = persistent_grid(kBcBlock, 8);
     if (phase == BC_FWD) {
-        bc_fwd_kernel<<<grid, kBcBlock, 0, s>>>(a);
-        bc_fwd_decide_kernel<<<1, 64, 0, s>>>(a);
+        L(bc_fwd_kernel, grid, kBcBlock, a);
+        L(bc_fwd_decide_kernel, 1, 64, a);
     } else {
-        bc_level_kernel<<<grid, kBcBlock, 0, s>>>(a);
-        bc_bwd_kernel<<<grid, kBcBlock, 0, s>>>(a);
-        bc_bwd_decide_kernel<<<1, 64, 0, s>>>(a);
+        L(bc_level_kernel, grid, kBcBlock, a);
+        L(bc_bwd_kernel, grid, kBcBlock, a);
+        L(bc_bwd_decide_kernel, 1, 64, a);
     }
 }
 
@@ -541,15 +537,12 @@ static int bc_pass(BcRun &r, const BcArgs &a, int32_t phase, int32_t first, hipS
         const auto key = std::make_tuple(phase, block, a.B);
         auto it = r.execs.find(key);
         if (it == r.execs.end()) {
-            hipGraph_t graph = nullptr;
-            GI_HIP(hipStreamBeginCapture(r.capture, hipStreamCaptureModeThreadLocal));
-            for (int32_t i = 0; i < block; ++i) bc_launch_slot(a, phase, r.capture);
-            hipError_t e = hipStreamEndCapture(r.capture, &graph);
-            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
             hipGraphExec_t exec = nullptr;
-            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+            GI_TRY(build_graph(
+                [&](Launcher &L) {
+                    for (int32_t i = 0; i < block; ++i) bc_launch_slot(a, phase, L);
+                },
+                &exec));
             it = r.execs.emplace(key, exec).first;
         }
         GI_HIP(hipGraphLaunch(it->second, s));

@@ -950,7 +950,6 @@ struct FusedWork {
     int32_t hub_min = kHubDegree;
     int32_t blocks_per_cu = 8;
     // level loop captured as a hipGraph, one exec per level count
-    hipStream_t cap_stream = nullptr;
     // captured level loops by (level count, selector view bytes): the decide
     // kernel takes the view by value
     std::map<std::pair<int32_t, std::string>, hipGraphExec_t> execs;
@@ -958,7 +957,7 @@ struct FusedWork {
     int32_t recent_n = 0;
 };
 
-static std::map<const gi_graph *, FusedWork *> g_fused;
+static PlanRegistry<FusedWork> g_fused;
 
 static void drop_execs(FusedWork *w) {
     for (auto &kv : w->execs) (void)hipGraphExecDestroy(kv.second);
@@ -967,7 +966,6 @@ static void drop_execs(FusedWork *w) {
 
 static void free_work(FusedWork *w) {
     drop_execs(w);
-    if (w->cap_stream) (void)hipStreamDestroy(w->cap_stream);
     dfree(w->st);
     for (auto p : w->bm) dfree(p);
     for (auto p : w->sum) dfree(p);
@@ -984,11 +982,7 @@ static void free_work(FusedWork *w) {
 }
 
 static int ensure_fused(gi_graph *g, FusedWork **out) {
-    auto it = g_fused.find(g);
-    if (it != g_fused.end()) {
-        *out = it->second;
-        return GI_OK;
-    }
+    if ((*out = g_fused.find(g)) != nullptr) return GI_OK;
     int64_t words = bitmap_words(g->V);
     int64_t chunks = ceil_div(words, kChunkWords);
     int64_t sum_words = ceil_div(chunks, 64);
@@ -1018,8 +1012,6 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
     if (rc == GI_OK && !g->d_visited) rc = dalloc(&g->d_visited, (size_t)words + 1);
     if (rc == GI_OK && hipHostMalloc((void **)&w->host_flag, sizeof(int32_t) * 4) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
-    if (rc == GI_OK && hipStreamCreateWithFlags(&w->cap_stream, hipStreamNonBlocking) != hipSuccess)
-        rc = fail(GI_ERR_HIP, "stream create failed");
     if (rc == GI_OK && hipEventCreateWithFlags(&w->flag_ev, hipEventDisableTiming) != hipSuccess)
         rc = fail(GI_ERR_HIP, "event create failed");
     if (rc == GI_OK) rc = fill_i32(w->pcount, 2 * kMaxGrid, 0, 0);
@@ -1036,16 +1028,13 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
         free_work(w);
         return rc;
     }
-    g_fused[g] = w;
+    g_fused.put(g, w);
     *out = w;
     return GI_OK;
 }
 
 void release_fused(const gi_graph *g) {
-    auto it = g_fused.find(g);
-    if (it == g_fused.end()) return;
-    free_work(it->second);
-    g_fused.erase(it);
+    if (FusedWork *p = g_fused.take(g)) free_work(p);
 }
 
 int32_t fused_mode_of(const gi_schedule &s) {
@@ -1062,13 +1051,6 @@ int32_t fused_mode_of(const gi_schedule &s) {
 // Persistent grids: as many workgroups as can be resident at once (the
 // step kernels assign work statically, so a second round of workgroups
 // would serialise), capped by the tuned blocks per CU.
-static int resident_per_cu(const void *kernel) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kStepBlock, 0) != hipSuccess || per_cu < 1)
-        per_cu = 4;
-    return per_cu;
-}
-
 static int grid_of(const FusedWork *w, int resident) {
     int per = w->blocks_per_cu < resident ? w->blocks_per_cu : resident;
     int g = device_info().num_cus * per;
@@ -1076,18 +1058,18 @@ static int grid_of(const FusedWork *w, int resident) {
 }
 
 static int push_grid(const FusedWork *w) {
-    static int resident = resident_per_cu((const void *)bfs_push_step);
+    static const int resident = resident_per_cu((const void *)bfs_push_step, kStepBlock);
     return grid_of(w, resident);
 }
 
 static int pull_grid(const FusedWork *w) {
-    static int resident = resident_per_cu((const void *)bfs_pull_step);
+    static const int resident = resident_per_cu((const void *)bfs_pull_step, kStepBlock);
     return grid_of(w, resident);
 }
 
 // Partial-sum slots the decide kernel adds up: every block of either step.
 static int step_grid(const FusedWork *w) {
-    static int resident = resident_per_cu((const void *)bfs_step);
+    static const int resident = resident_per_cu((const void *)bfs_step, kStepBlock);
     return grid_of(w, resident);
 }
 
@@ -1122,32 +1104,36 @@ static StepArgs step_args(gi_graph *g, FusedWork *w) {
     return a;
 }
 
-static void launch_decide(gi_graph *g, FusedWork *w, int32_t level, hipStream_t s) {
-    (void)level;  // the kernel reads the level from the device state
+static void launch_decide(gi_graph *g, FusedWork *w, Launcher &L) {
+    // the kernel reads the level from the device state
     SumSet sums{{w->sum[0], w->sum[1], w->sum[2]}};
-    bfs_decide_kernel<<<1, kDecideBlock, 0, s>>>(w->st, w->last_sel, g->V, g->E, w->pcount, w->pfdeg,
-                                                 step_slots(w), sums, w->sum_words, w->cflag,
-                                                 (int32_t)ceil_div(bitmap_words(g->V), kChunkWords));
+    L(bfs_decide_kernel, 1, kDecideBlock, w->st, w->last_sel, g->V, g->E, w->pcount, w->pfdeg, step_slots(w), sums,
+      w->sum_words, w->cflag, (int32_t)ceil_div(bitmap_words(g->V), kChunkWords));
+}
+
+static void launch_decide(gi_graph *g, FusedWork *w, hipStream_t s) {
+    Launcher L(s);
+    launch_decide(g, w, L);
 }
 
 // A captured level: the step kernel (push or pull body as decided), the hub
 // kernel (a no-op after a pull step), the decide kernel.
-static void launch_level(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t level, hipStream_t s) {
-    bfs_step<<<step_grid(w), kStepBlock, 0, s>>>(a);
-    bfs_hub_kernel<<<push_grid(w), kStepBlock, 0, s>>>(a);
-    launch_decide(g, w, level, s);
+static void launch_level(gi_graph *g, FusedWork *w, const StepArgs &a, Launcher &L) {
+    L(bfs_step, step_grid(w), kStepBlock, a);
+    L(bfs_hub_kernel, push_grid(w), kStepBlock, a);
+    launch_decide(g, w, L);
 }
 
 // A traced level with the mode already known on the host: the matching step
 // kernel alone between the two events.
-static void launch_level_traced(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t level,
-                                int32_t mode, hipStream_t s, hipEvent_t *ev) {
+static void launch_level_traced(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t mode, hipStream_t s,
+                                hipEvent_t *ev) {
     (void)hipEventRecord(ev[0], s);
     if (mode >= BFS_PULL_THREAD) bfs_pull_step_traced<<<pull_grid(w), kStepBlock, 0, s>>>(a);
     else bfs_push_step_traced<<<push_grid(w), kStepBlock, 0, s>>>(a);
     (void)hipEventRecord(ev[1], s);
     if (mode < BFS_PULL_THREAD) bfs_hub_kernel<<<push_grid(w), kStepBlock, 0, s>>>(a);
-    launch_decide(g, w, level, s);
+    launch_decide(g, w, s);
 }
 
 // Seed: clear the arrays, set the state, mark the sources, decide level 0.
@@ -1189,7 +1175,7 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
         bfs_seed_sources_kernel<<<1, 64, 0, s>>>(w->st, nullptr, 1, single, w->bm[0], w->sum[0], g->d_visited,
                                                  g->d_off);
     }
-    launch_decide(g, w, -1, s);
+    launch_decide(g, w, s);
     GI_LAUNCH_CHECK();
     if (h_sources) GI_HIP(hipStreamSynchronize(s));  // h_sources may go away on return
     return GI_OK;
@@ -1207,15 +1193,12 @@ static int level_graph(gi_graph *g, FusedWork *w, int32_t levels, hipGraphExec_t
         return GI_OK;
     }
     StepArgs a = step_args(g, w);
-    hipGraph_t graph = nullptr;
-    GI_HIP(hipStreamBeginCapture(w->cap_stream, hipStreamCaptureModeThreadLocal));
-    for (int32_t l = 0; l < levels; ++l) launch_level(g, w, a, l, w->cap_stream);
-    hipError_t e = hipStreamEndCapture(w->cap_stream, &graph);
-    if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
     hipGraphExec_t exec = nullptr;
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+    GI_TRY(build_graph(
+        [&](Launcher &L) {
+            for (int32_t l = 0; l < levels; ++l) launch_level(g, w, a, L);
+        },
+        &exec));
     if (w->execs.size() >= 16) drop_execs(w);
     w->execs[key] = exec;
     *out = exec;
@@ -1245,7 +1228,7 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
             GI_HIP(hipEventCreate(&e1));
             trace->ev.push_back(e0);
             trace->ev.push_back(e1);
-            launch_level_traced(g, w, a, level, w->host_flag[3], s, &trace->ev[trace->ev.size() - 2]);
+            launch_level_traced(g, w, a, w->host_flag[3], s, &trace->ev[trace->ev.size() - 2]);
             GI_LAUNCH_CHECK();
         }
         return fail(GI_ERR_RANGE, "BFS deeper than %d levels", kBfsMaxLevels);
@@ -1486,7 +1469,7 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
                     best_mode = m;
                 }
             } else {
-                launch_decide(g, w, h.level, s);
+                launch_decide(g, w, s);
                 GI_LAUNCH_CHECK();
                 prev_mode = run;
             }

@@ -279,16 +279,14 @@ struct CcPlan {
     int32_t *chunk_v = nullptr;
     CcState *st = nullptr;
     int32_t *host = nullptr;  // pinned copy of the state
-    hipStream_t cap = nullptr;
     std::map<std::tuple<int32_t, int32_t *, bool>, hipGraphExec_t> execs;  // (rounds, labels, sampled)
     int32_t last_rounds = 0;
 };
 
-static std::map<const gi_graph *, CcPlan *> g_cc;
+static PlanRegistry<CcPlan> g_cc;
 
 static void free_cc(CcPlan *p) {
     for (auto &kv : p->execs) (void)hipGraphExecDestroy(kv.second);
-    if (p->cap) (void)hipStreamDestroy(p->cap);
     dfree(p->chunk_v);
     dfree(p->st);
     if (p->host) (void)hipHostFree(p->host);
@@ -296,18 +294,11 @@ static void free_cc(CcPlan *p) {
 }
 
 void release_cc(const gi_graph *g) {
-    auto it = g_cc.find(g);
-    if (it == g_cc.end()) return;
-    free_cc(it->second);
-    g_cc.erase(it);
+    if (CcPlan *p = g_cc.take(g)) free_cc(p);
 }
 
 static int ensure_cc(gi_graph *g, CcPlan **out) {
-    auto it = g_cc.find(g);
-    if (it != g_cc.end()) {
-        *out = it->second;
-        return GI_OK;
-    }
+    if ((*out = g_cc.find(g)) != nullptr) return GI_OK;
     if (g->E >= 0x7fffffffLL - kCcChunk) return fail(GI_ERR_RANGE, "fused CC needs E < 2^31");
     CcPlan *p = new CcPlan();
     p->chunks = (int32_t)ceil_div(g->E, kCcChunk);
@@ -315,8 +306,6 @@ static int ensure_cc(gi_graph *g, CcPlan **out) {
     if (rc == GI_OK) rc = dalloc(&p->st, 1);
     if (rc == GI_OK && hipHostMalloc((void **)&p->host, sizeof(CcState)) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
-    if (rc == GI_OK && hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess)
-        rc = fail(GI_ERR_HIP, "stream create failed");
     if (rc == GI_OK && p->chunks > 0) {
         int grid = (int)ceil_div(p->chunks, 256);
         cc_chunk_owner_kernel<<<grid, 256>>>(g->d_off, g->V, p->chunks, p->chunk_v);
@@ -327,18 +316,13 @@ static int ensure_cc(gi_graph *g, CcPlan **out) {
         free_cc(p);
         return rc;
     }
-    g_cc[g] = p;
+    g_cc.put(g, p);
     *out = p;
     return GI_OK;
 }
 
 static int hook_grid() {
-    static int per_cu = -1;
-    if (per_cu < 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, cc_hook_kernel, kCcBlock, 0) != hipSuccess ||
-            per_cu < 1)
-            per_cu = 4;
-    }
+    static const int per_cu = resident_per_cu((const void *)cc_hook_kernel, kCcBlock);
     return device_info().num_cus * per_cu;
 }
 
@@ -347,12 +331,12 @@ static int vertex_grid(int32_t V) {
     return g < 8192 ? (g < 1 ? 1 : g) : 8192;
 }
 
-static void launch_round(const CcArgs &a, bool sampled, hipStream_t s) {
-    if (sampled) cc_rest_kernel<<<vertex_grid(a.V), kCcBlock, 0, s>>>(a);
-    else cc_hook_kernel<<<hook_grid(), kCcBlock, 0, s>>>(a);
+static void launch_round(const CcArgs &a, bool sampled, Launcher &L) {
+    if (sampled) L(cc_rest_kernel, vertex_grid(a.V), kCcBlock, a);
+    else L(cc_hook_kernel, hook_grid(), kCcBlock, a);
     int cg = (int)ceil_div(a.V, 256);
-    cc_compress_kernel<<<cg < 8192 ? cg : 8192, 256, 0, s>>>(a);
-    cc_control_kernel<<<1, 1, 0, s>>>(a.st);
+    L(cc_compress_kernel, cg < 8192 ? cg : 8192, 256, a);
+    L(cc_control_kernel, 1, 1, a.st);
 }
 
 int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, bool sampled, hipStream_t s) {
@@ -392,15 +376,12 @@ int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, bool sampl
                 for (auto &kv : p->execs) (void)hipGraphExecDestroy(kv.second);
                 p->execs.clear();
             }
-            hipGraph_t graph = nullptr;
-            GI_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-            for (int32_t r = 0; r < block; ++r) launch_round(a, sampled, p->cap);
-            hipError_t e = hipStreamEndCapture(p->cap, &graph);
-            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
             hipGraphExec_t exec = nullptr;
-            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+            GI_TRY(build_graph(
+                [&](Launcher &L) {
+                    for (int32_t r = 0; r < block; ++r) launch_round(a, sampled, L);
+                },
+                &exec));
             it = p->execs.emplace(key, exec).first;
         }
         GI_HIP(hipGraphLaunch(it->second, s));

@@ -7,7 +7,13 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <tuple>
+#include <type_traits>
+#include <utility>
+#include <vector>
 
 #include "../../include/graphit_amd.h"
 
@@ -32,6 +38,113 @@ int fail(int code, const char *fmt, ...);
     } while (0)
 
 #define GI_LAUNCH_CHECK() GI_HIP(hipGetLastError())
+
+// ------------------------------------------------------------ launch graphs
+// Where a sequence of kernel launches goes: queued on a stream, or appended
+// to a hipGraph built node by node, each node after the one before (the
+// order the stream gives). The fused engines build the graphs they replay
+// this way, not by stream capture: this HIP runtime invalidates a capture in
+// progress when another thread makes a device-wide synchronising call (a
+// null-stream copy, a device synchronise), so captures failed when two
+// threads ran engines on two graphs.
+struct Launcher {
+    hipStream_t stream = nullptr;   // eager launches when graph is null
+    hipGraph_t graph = nullptr;     // the graph being built
+    hipGraphNode_t last = nullptr;  // its last node
+    hipError_t err = hipSuccess;    // first failure while adding nodes
+    std::vector<std::shared_ptr<void>> keep;  // argument values, alive until instantiation
+
+    explicit Launcher(hipStream_t s = nullptr) : stream(s) {}
+
+    template <class... P, class... A>
+    void operator()(void (*kernel)(P...), dim3 grid, dim3 block, A &&...args) {
+        using Vals = std::tuple<std::decay_t<P>...>;
+        void *ptrs[sizeof...(P) + 1];
+        if (!graph) {  // a failure is left for hipGetLastError, as a <<<>>> launch leaves it
+            Vals vals(std::forward<A>(args)...);
+            set_ptrs(vals, ptrs, std::index_sequence_for<P...>{});
+            (void)hipLaunchKernel(reinterpret_cast<const void *>(kernel), grid, block, ptrs, 0, stream);
+            return;
+        }
+        if (err != hipSuccess) return;
+        auto vals = std::make_shared<Vals>(std::forward<A>(args)...);
+        keep.push_back(vals);
+        set_ptrs(*vals, ptrs, std::index_sequence_for<P...>{});
+        hipKernelNodeParams np = {};
+        np.func = reinterpret_cast<void *>(kernel);
+        np.gridDim = grid;
+        np.blockDim = block;
+        np.sharedMemBytes = 0;
+        np.kernelParams = ptrs;
+        np.extra = nullptr;
+        hipGraphNode_t node = nullptr;
+        err = hipGraphAddKernelNode(&node, graph, last ? &last : nullptr, last ? 1 : 0, &np);
+        if (err == hipSuccess) last = node;
+    }
+
+  private:
+    template <class T, size_t... I>
+    static void set_ptrs(T &t, void **p, std::index_sequence<I...>) {
+        ((p[I] = (void *)&std::get<I>(t)), ...);
+    }
+};
+
+// Instantiates the graph of the launches `record(Launcher &)` makes.
+template <class F>
+int build_graph(F &&record, hipGraphExec_t *out) {
+    Launcher L;
+    hipError_t e = hipGraphCreate(&L.graph, 0);
+    if (e != hipSuccess) return fail(GI_ERR_HIP, "graph create failed: %s", hipGetErrorString(e));
+    record(L);
+    e = L.err;
+    hipGraphExec_t exec = nullptr;
+    if (e == hipSuccess) e = hipGraphInstantiate(&exec, L.graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(L.graph);
+    if (e != hipSuccess) return fail(GI_ERR_HIP, "graph build failed: %s", hipGetErrorString(e));
+    *out = exec;
+    return GI_OK;
+}
+
+// Workgroups of `kernel` (launched with `block` threads) resident per CU, 4
+// when the runtime cannot say. Callers keep it in a function-local static,
+// whose initialisation is thread-safe.
+inline int resident_per_cu(const void *kernel, int block) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu < 1)
+        per_cu = 4;
+    return per_cu;
+}
+
+// Per-graph plans of one engine, keyed by the graph handle. The map itself
+// is locked, so threads that each use their own graph may call in parallel;
+// a plan belongs to its graph and is used by that graph's one thread at a
+// time.
+template <class Plan>
+class PlanRegistry {
+  public:
+    Plan *find(const gi_graph *g) {
+        std::lock_guard<std::mutex> lock(mu_);
+        auto it = map_.find(g);
+        return it == map_.end() ? nullptr : it->second;
+    }
+    void put(const gi_graph *g, Plan *p) {
+        std::lock_guard<std::mutex> lock(mu_);
+        map_[g] = p;
+    }
+    // removes the graph's plan and hands it to the caller (nullptr: none)
+    Plan *take(const gi_graph *g) {
+        std::lock_guard<std::mutex> lock(mu_);
+        auto it = map_.find(g);
+        if (it == map_.end()) return nullptr;
+        Plan *p = it->second;
+        map_.erase(it);
+        return p;
+    }
+
+  private:
+    std::mutex mu_;
+    std::map<const gi_graph *, Plan *> map_;
+};
 
 // ----------------------------------------------------------------- device
 struct DeviceInfo {

@@ -143,16 +143,14 @@ struct PrPlan {
     int32_t *wide = nullptr, *wide_c0 = nullptr, *chunk_b = nullptr, *chunk_e0 = nullptr;
     float *partial = nullptr;
     float *contrib[2] = {nullptr, nullptr};
-    hipStream_t cap = nullptr;
     // captured runs keyed by (iterations, output array, damping bits)
     std::map<std::tuple<int32_t, const float *, uint32_t>, hipGraphExec_t> execs;
 };
 
-static std::map<const gi_graph *, PrPlan *> g_pr;
+static PlanRegistry<PrPlan> g_pr;
 
 static void free_plan(PrPlan *p) {
     for (auto &kv : p->execs) (void)hipGraphExecDestroy(kv.second);
-    if (p->cap) (void)hipStreamDestroy(p->cap);
     dfree(p->wide);
     dfree(p->wide_c0);
     dfree(p->chunk_b);
@@ -164,18 +162,11 @@ static void free_plan(PrPlan *p) {
 }
 
 void release_pr(const gi_graph *g) {
-    auto it = g_pr.find(g);
-    if (it == g_pr.end()) return;
-    free_plan(it->second);
-    g_pr.erase(it);
+    if (PrPlan *p = g_pr.take(g)) free_plan(p);
 }
 
 static int ensure_plan(gi_graph *g, PrPlan **out) {
-    auto it = g_pr.find(g);
-    if (it != g_pr.end()) {
-        *out = it->second;
-        return GI_OK;
-    }
+    if ((*out = g_pr.find(g)) != nullptr) return GI_OK;
     gi_graph *pg = g;
     if (g->directed) GI_TRY(gi_graph_transpose(g, &pg));
     const int32_t V = g->V;
@@ -214,29 +205,22 @@ static int ensure_plan(gi_graph *g, PrPlan **out) {
     up(p->wide_c0, wide_c0);
     up(p->chunk_b, chunk_b);
     up(p->chunk_e0, chunk_e0);
-    if (rc == GI_OK && hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking) != hipSuccess)
-        rc = fail(GI_ERR_HIP, "stream create failed");
     if (rc != GI_OK) {
         free_plan(p);
         return rc;
     }
-    g_pr[g] = p;
+    g_pr.put(g, p);
     *out = p;
     return GI_OK;
 }
 
 static int gather_grid() {
-    static int per_cu = -1;
-    if (per_cu < 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, pr_gather_kernel, kPrBlock, 0) != hipSuccess ||
-            per_cu < 1)
-            per_cu = 4;
-    }
+    static const int per_cu = resident_per_cu((const void *)pr_gather_kernel, kPrBlock);
     return device_info().num_cus * per_cu;
 }
 
 static void launch_iters(gi_graph *g, PrPlan *p, int32_t iters, float damping, float *rank,
-                         hipStream_t s) {
+                         Launcher &L) {
     PrArgs a;
     a.poff = p->poff;
     a.pnbr = p->pnbr;
@@ -254,15 +238,15 @@ static void launch_iters(gi_graph *g, PrPlan *p, int32_t iters, float damping, f
     a.damping = damping;
     a.beta = (float)((1.0 - (double)damping) / (double)g->V);  // apps/pr.cpp:36-41
     int ig = (int)ceil_div(g->V, 256);
-    pr_init_kernel<<<ig < 4096 ? ig : 4096, 256, 0, s>>>(g->d_off, g->V, p->contrib[0], rank);
+    L(pr_init_kernel, ig < 4096 ? ig : 4096, 256, g->d_off, g->V, p->contrib[0], rank);
     const int grid = gather_grid();
     const int fg = (int)ceil_div(p->nwide > 0 ? p->nwide : 1, 256);
     for (int32_t it = 0; it < iters; ++it) {
         a.contrib = p->contrib[it & 1];
         a.contrib_next = p->contrib[(it + 1) & 1];
         a.write_rank = it + 1 == iters;
-        pr_gather_kernel<<<grid, kPrBlock, 0, s>>>(a);
-        if (p->nwide > 0) pr_finalize_kernel<<<fg < 4096 ? fg : 4096, 256, 0, s>>>(a);
+        L(pr_gather_kernel, grid, kPrBlock, a);
+        if (p->nwide > 0) L(pr_finalize_kernel, fg < 4096 ? fg : 4096, 256, a);
     }
 }
 
@@ -286,15 +270,8 @@ int pr_fused_run(gi_graph *g, int32_t iters, float damping, float *d_rank, hipSt
             for (auto &kv : p->execs) (void)hipGraphExecDestroy(kv.second);
             p->execs.clear();
         }
-        hipGraph_t graph = nullptr;
-        GI_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
-        launch_iters(g, p, iters, damping, d_rank, p->cap);
-        hipError_t e = hipStreamEndCapture(p->cap, &graph);
-        if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
         hipGraphExec_t exec = nullptr;
-        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+        GI_TRY(build_graph([&](Launcher &L) { launch_iters(g, p, iters, damping, d_rank, L); }, &exec));
         it = p->execs.emplace(key, exec).first;
     }
     GI_HIP(hipGraphLaunch(it->second, s));

@@ -760,17 +760,15 @@ struct SsWork {
     bool hubs = false;  // some out-degree reaches kSsHubMin: the hub kernel is launched
     int32_t wide_chunks = 0;
     int32_t *chunk_v = nullptr, *chunk_e0 = nullptr;
-    hipStream_t cap = nullptr;
     std::map<int32_t, hipGraphExec_t> execs;  // by slot count (the run's dist is read from the state)
     int32_t recent_slots[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // slots of the last runs (ring)
     int32_t recent_n = 0;
 };
 
-static std::map<const gi_graph *, SsWork *> g_ss;
+static PlanRegistry<SsWork> g_ss;
 
 static void free_ss(SsWork *w) {
     for (auto &kv : w->execs) (void)hipGraphExecDestroy(kv.second);
-    if (w->cap) (void)hipStreamDestroy(w->cap);
     dfree(w->st);
     for (auto p : w->bm) dfree(p);
     for (auto p : w->sum) dfree(p);
@@ -790,18 +788,11 @@ static void free_ss(SsWork *w) {
 }
 
 void release_sssp(const gi_graph *g) {
-    auto it = g_ss.find(g);
-    if (it == g_ss.end()) return;
-    free_ss(it->second);
-    g_ss.erase(it);
+    if (SsWork *p = g_ss.take(g)) free_ss(p);
 }
 
 static int ensure_ss(gi_graph *g, SsWork **out) {
-    auto it = g_ss.find(g);
-    if (it != g_ss.end()) {
-        *out = it->second;
-        return GI_OK;
-    }
+    if ((*out = g_ss.find(g)) != nullptr) return GI_OK;
     const int64_t words = bitmap_words(g->V);
     const int64_t sum_words = ceil_div(ceil_div(words, kBsChunkWords), 64);
     if (sum_words > kBsMaxSumWords) return fail(GI_ERR_RANGE, "fused SSSP supports up to 2^26 vertices");
@@ -835,8 +826,6 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
     if (rc == GI_OK && hipMemset(w->wsflag, 0, flag_bytes) != hipSuccess) rc = fail(GI_ERR_HIP, "flag init");
     if (rc == GI_OK && hipHostMalloc((void **)&w->host, 8 * sizeof(int32_t)) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
-    if (rc == GI_OK && hipStreamCreateWithFlags(&w->cap, hipStreamNonBlocking) != hipSuccess)
-        rc = fail(GI_ERR_HIP, "stream create failed");
     if (rc == GI_OK) {  // chunks of the wide in-lists, for pull steps
         std::vector<int32_t> poff((size_t)g->V + 1), cv, ce;
         if (hipMemcpy(poff.data(), pg->d_off, sizeof(int32_t) * (g->V + 1), hipMemcpyDeviceToHost) != hipSuccess)
@@ -865,19 +854,14 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
         free_ss(w);
         return rc;
     }
-    g_ss[g] = w;
+    g_ss.put(g, w);
     *out = w;
     return GI_OK;
 }
 
 // Persistent grid: resident workgroups of the step kernel.
 static int ss_grid() {
-    static int per_cu = -1;
-    if (per_cu < 0) {
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, ss_relax_kernel, kSsBlock, 0) != hipSuccess ||
-            per_cu < 1)
-            per_cu = 4;
-    }
+    static const int per_cu = resident_per_cu((const void *)ss_relax_kernel, kSsBlock);
     int g = device_info().num_cus * per_cu;
     return g < kSsMaxGrid ? g : kSsMaxGrid;
 }
@@ -959,13 +943,13 @@ static int ensure_split(gi_graph *g, SsWork *w, int32_t delta, hipStream_t s) {
     return GI_OK;
 }
 
-static void launch_slot(const SsArgs &a, bool hubs, hipStream_t s) {
+static void launch_slot(const SsArgs &a, bool hubs, Launcher &L) {
     const int grid = ss_grid();
-    ss_relax_kernel<<<grid, kSsBlock, 0, s>>>(a);
-    if (hubs) ss_hub_kernel<<<grid, kSsBlock, 0, s>>>(a);
-    ss_decide_kernel<<<1, kSsDecBlock, 0, s>>>(a, grid);
-    ss_farmove_kernel<<<grid, kSsBlock, 0, s>>>(a);
-    ss_decide2_kernel<<<1, kSsDecBlock, 0, s>>>(a, grid);
+    L(ss_relax_kernel, grid, kSsBlock, a);
+    if (hubs) L(ss_hub_kernel, grid, kSsBlock, a);
+    L(ss_decide_kernel, 1, kSsDecBlock, a, grid);
+    L(ss_farmove_kernel, grid, kSsBlock, a);
+    L(ss_decide2_kernel, 1, kSsDecBlock, a, grid);
 }
 
 int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,
@@ -996,15 +980,12 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
                 for (auto &kv : w->execs) (void)hipGraphExecDestroy(kv.second);
                 w->execs.clear();
             }
-            hipGraph_t graph = nullptr;
-            GI_HIP(hipStreamBeginCapture(w->cap, hipStreamCaptureModeThreadLocal));
-            for (int32_t r = 0; r < block; ++r) launch_slot(ca, w->hubs, w->cap);
-            hipError_t e = hipStreamEndCapture(w->cap, &graph);
-            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
             hipGraphExec_t exec = nullptr;
-            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) return fail(GI_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+            GI_TRY(build_graph(
+                [&](Launcher &L) {
+                    for (int32_t r = 0; r < block; ++r) launch_slot(ca, w->hubs, L);
+                },
+                &exec));
             it = w->execs.emplace(block, exec).first;
         }
         *out = it->second;

@@ -17,7 +17,9 @@
  *     type and length num_vertices.
  *   - `stream` is a hipStream_t (NULL = default stream). Work is queued
  *     asynchronously on it; calls documented as "synchronises" wait for it.
- *   - A handle is used by one host thread at a time.
+ *   - A handle is used by one host thread at a time; calls on different
+ *     graphs may run on different threads at once. The selector model is
+ *     process-wide: gi_selector_load/unload must not overlap other calls.
  *   - int32 vertex ids and int32 CSR offsets, as the reference
  *     (runtime/infra_gpu/graph.h:20-35); graphs with >= 2^31 edge entries are
  *     rejected with GI_ERR_RANGE.
