@@ -389,3 +389,39 @@ def test_bc_tiny_and_empty(gi):
     for kw in ({"batch": 3}, {"schedule": fused}, {}):
         sc, _ = gi.bc(g, [0, 1, 2], **kw)
         assert np.allclose(sc, want, rtol=1e-6), kw
+
+
+def _tiny_graphs(gi):
+    """No edges, one edge, a full last bitmap word (path of 64), a ragged
+    one (star of 65), and two words apart joined by one edge."""
+    z = np.zeros(0, np.int32)
+    out = [gi.Graph.from_edges(1, z, z, True)]
+    out.append(gi.Graph.from_edges(2, np.array([0], np.int32), np.array([1], np.int32), True))
+    p = np.arange(63, dtype=np.int32)
+    out.append(gi.Graph.from_edges(64, p, p + 1, True))
+    out.append(gi.Graph.from_edges(65, np.full(64, 64, np.int32), np.arange(64, dtype=np.int32), True))
+    out.append(gi.Graph.from_edges(130, np.array([0], np.int32), np.array([129], np.int32), True))
+    return out
+
+
+@pytest.mark.parametrize("mode_sched", [(0, 0), (0, 3), (0, 2), (0, 4), (0, 1), (1, 0), (1, 2)])
+def test_fused_bfs_tiny_graphs(gi, mode_sched):
+    d, lb = mode_sched  # one schedule per fused step mode
+    s = gi.Schedule(direction=d, load_balancing=lb, kernel_fusion=1,
+                    pull_frontier_rep=gi.PULL_BITMAP, frontier_creation=gi.FC_BITMAP)
+    h = gi.HybridSchedule(s, s, selector=gi.SELECT_S1)
+    for g in _tiny_graphs(gi):
+        host = host_csr(g)
+        for src in sorted({0, g.num_vertices - 1}):
+            depth, _, _ = gi.bfs(g, src, h)
+            assert np.array_equal(depth, O.bfs_depth(host, src)), (g.num_vertices, src)
+
+
+def test_fused_cc_and_pagerank_tiny_graphs(gi):
+    for g in _tiny_graphs(gi):
+        host = host_csr(g)
+        for lb in (gi.LB_EDGE_ONLY, gi.LB_VERTEX_BASED):
+            lab, _ = gi.cc(g, schedule=gi.Schedule(direction=gi.PUSH, load_balancing=lb, kernel_fusion=1))
+            assert np.array_equal(lab, O.cc(host)), (g.num_vertices, lb)
+        r, _ = gi.pagerank(g, 20, schedule=_fused_pr(gi))
+        assert rel_l1(r, O.pagerank(host, 20)) <= 1e-5, g.num_vertices
