@@ -820,10 +820,31 @@ struct ResetArgs {
 // Clears every array of a run in one pass and initialises the state.
 __global__ void bfs_reset_kernel(BfsState *st, SeedArgs sa, ResetArgs r, int64_t E) {
     const int32_t stride = gridDim.x * blockDim.x;
-    const int32_t n = r.V > r.sum_words + 1 ? r.V : r.sum_words + 1;  // words <= V
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-        if (i < r.depth_len) r.depth[i] = -1;
-        if (i < r.V && r.parent) r.parent[i] = -1;
+    const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    // depths [0, depth_len) and parents [0, V): 16-byte stores where both
+    // arrays are 16-byte aligned, then the remainder one by one
+    const int32_t dlen = r.depth_len, plen = r.parent ? r.V : 0;
+    const int32_t vlen = dlen > plen ? dlen : plen;
+    const bool aligned = ((reinterpret_cast<uintptr_t>(r.depth) | reinterpret_cast<uintptr_t>(r.parent)) & 15) == 0;
+    const int32_t head = aligned ? vlen / 4 * 4 : 0;
+    const int4 none = make_int4(-1, -1, -1, -1);
+    for (int32_t q = tid; q < head / 4; q += stride) {
+        const int32_t i = 4 * q;
+        if (i + 4 <= dlen) reinterpret_cast<int4 *>(r.depth)[q] = none;
+        else
+            for (int k = 0; k < 4; ++k)
+                if (i + k < dlen) r.depth[i + k] = -1;
+        if (i + 4 <= plen) reinterpret_cast<int4 *>(r.parent)[q] = none;
+        else
+            for (int k = 0; k < 4; ++k)
+                if (i + k < plen) r.parent[i + k] = -1;
+    }
+    for (int32_t i = head + tid; i < vlen; i += stride) {
+        if (i < dlen) r.depth[i] = -1;
+        if (i < plen) r.parent[i] = -1;
+    }
+    const int32_t n = r.words > r.sum_words + 1 ? r.words : r.sum_words + 1;
+    for (int32_t i = tid; i < n; i += stride) {
         if (i < r.words) {
             r.visited[i] = r.dead[i];
             r.bm[0][i] = 0;
