@@ -67,7 +67,7 @@ def roofline(gi, g, sched, sources, depth):
 
     Traced runs launch one step kernel per level between two HIP events on
     the BFS stream (bfs_push_step_traced or bfs_pull_step_traced, the same
-    code as the captured kernels under a symbol of their own, so a rocprof
+    code as the recorded kernels under a symbol of their own, so a rocprof
     summary of this command lists exactly these launches). achieved =
     algorithmic bytes per launch / average launch duration."""
     kern, modes = {}, {}

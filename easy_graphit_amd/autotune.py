@@ -43,7 +43,7 @@ def schedule(push_lb: int = gi.LB_TWC, threshold: float = 0.02):
 
 def time_multi(g: gi.Graph, sources, sched, depth: gi.DeviceArray, reps: int) -> float:
     """Best wall time (s) of one multi-source fused BFS over `reps` runs."""
-    gi.bfs_multi(g, sources, sched, depth)  # warm-up (captures the level graph)
+    gi.bfs_multi(g, sources, sched, depth)  # warm-up (records the level graph)
     gi.synchronize()
     best = float("inf")
     for _ in range(reps):
@@ -108,7 +108,7 @@ def tune_delta(g: gi.Graph, source, deltas=DELTAS, reps: int = 2, log=None) -> d
     for d in deltas:
         total = 0.0
         for src in sources:
-            gi.sssp(g, src, d, sched)  # warm-up (builds the split, captures the slots)
+            gi.sssp(g, src, d, sched)  # warm-up (builds the split, records the slots)
             best = float("inf")
             for _ in range(reps):
                 dist, st = gi.sssp(g, src, d, sched)

@@ -638,7 +638,7 @@ int gi_pagerank(gi_graph *g, int32_t iters, float damping, const gi_schedule *sc
     }
     hipStream_t s = (hipStream_t)stream;
     if (sched->kernel_fusion == GI_FUSION_ENABLED) {
-        // fused engine: deterministic pull over the in-edges, one captured graph
+        // fused engine: deterministic pull over the in-edges, one recorded graph
         if (sched->direction != GI_PULL) return fail(GI_ERR_UNSUPPORTED, "fused PageRank pulls");
         EventTimer timer(s);
         if (g->layout) {  // degree-ordered view; ranks gathered back (bit copies)

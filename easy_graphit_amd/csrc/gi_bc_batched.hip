@@ -30,7 +30,7 @@
 // no atomics.
 // Control: the state (level, direction, counts) lives on the device; a
 // forward slot is a step and a decide launch, a backward slot a level, a step
-// and a decide launch; slots are captured as hipGraphs and replayed in
+// and a decide launch; slots are recorded as hipGraphs and replayed in
 // blocks, the host reading the state once per block. Direction rules, from
 // level sizes summed over the batch: pull a forward level whose frontier
 // holds more than V/16 vertices per source, and a backward level whose
@@ -468,7 +468,7 @@ static int grid_for(int64_t n, int block = 256) {
     return (int)(want < 1 ? 1 : (want < cap ? want : cap));
 }
 
-// Arrays, state and captured slot blocks of one BC call (batches of up to
+// Arrays, state and recorded slot blocks of one BC call (batches of up to
 // `cap` sources); freed when the call ends.
 struct BcRun {
     int32_t cap = 0;
@@ -527,7 +527,7 @@ static void bc_launch_slot(const BcArgs &a, int32_t phase, Launcher &L) {
     }
 }
 
-// Replays blocks of captured slots of one pass until the state leaves it:
+// Replays blocks of recorded slots of one pass until the state leaves it:
 // first `first` slots (rounded up to 4), then 4 at a time (a slot past the
 // end costs its no-op launches, a block too short a host round trip).
 static int bc_pass(BcRun &r, const BcArgs &a, int32_t phase, int32_t first, hipStream_t s) {

@@ -487,7 +487,7 @@ __global__ void __launch_bounds__(kStepBlock) bfs_hub_kernel(StepArgs a0) { hub_
 
 // ---------------------------------------------------------------- step kernels
 // One level's step. The push and pull paths are separate kernels (each with
-// its own register and LDS budget, hence occupancy); a captured level
+// its own register and LDS budget, hence occupancy); a recorded level
 // launches both and the one that does not match the decided mode returns at
 // once. Traced runs (per-level timing) launch only the matching one, under
 // their own symbols so their launches can be told apart in a profile.
@@ -607,7 +607,7 @@ GI_STEP_KERNEL(bfs_pull_step, kPullStep)
 GI_STEP_KERNEL(bfs_push_step_traced, kPushStep)
 GI_STEP_KERNEL(bfs_pull_step_traced, kPullStep)
 #undef GI_STEP_KERNEL
-// The captured level loop launches one step kernel per level: the body the
+// The recorded level loop launches one step kernel per level: the body the
 // decided mode needs (one launch fewer per level than a push and a pull
 // kernel of which one returns at once). Both bodies together need 72 VGPRs:
 // 7 waves per SIMD without spilling measured faster than 8 with a spill.
@@ -970,8 +970,8 @@ struct FusedWork {
     int32_t lane_max = 64;           // bin thresholds (gi_bfs_set_bins)
     int32_t hub_min = kHubDegree;
     int32_t blocks_per_cu = 8;
-    // level loop captured as a hipGraph, one exec per level count
-    // captured level loops by (level count, selector view bytes): the decide
+    // level loop recorded as a hipGraph, one exec per level count
+    // recorded level loops by (level count, selector view bytes): the decide
     // kernel takes the view by value
     std::map<std::pair<int32_t, std::string>, hipGraphExec_t> execs;
     int32_t recent_levels[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // depths of the last runs (ring)
@@ -1137,7 +1137,7 @@ static void launch_decide(gi_graph *g, FusedWork *w, hipStream_t s) {
     launch_decide(g, w, L);
 }
 
-// A captured level: the step kernel (push or pull body as decided), the hub
+// A recorded level: the step kernel (push or pull body as decided), the hub
 // kernel (a no-op after a pull step), the decide kernel.
 static void launch_level(gi_graph *g, FusedWork *w, const StepArgs &a, Launcher &L) {
     L(bfs_step, step_grid(w), kStepBlock, a);
@@ -1162,7 +1162,7 @@ static void launch_level_traced(gi_graph *g, FusedWork *w, const StepArgs &a, in
 static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t n_sources,
                     int32_t single, int32_t *depth, int32_t *parent, const SelectorView &sel,
                     bool want_fdeg, hipStream_t s, int32_t depth_len = -1) {
-    w->last_sel = sel;  // captured level graphs are kept per selector view (level_graph)
+    w->last_sel = sel;  // recorded level graphs are kept per selector view (level_graph)
     SeedArgs sa{depth, parent, w->lane_max, w->hub_min, want_fdeg ? 1 : 0};
     ResetArgs r;
     r.depth = depth;
@@ -1203,7 +1203,7 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
 }
 
 // A hipGraph of `levels` consecutive levels (step, hub, decide each); every
-// kernel reads its level from the device state, so one capture serves every
+// kernel reads its level from the device state, so one recording serves every
 // run on this graph.
 static int level_graph(gi_graph *g, FusedWork *w, int32_t levels, hipGraphExec_t *out) {
     const std::pair<int32_t, std::string> key(
@@ -1254,7 +1254,7 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
         }
         return fail(GI_ERR_RANGE, "BFS deeper than %d levels", kBfsMaxLevels);
     }
-    // Graph mode: replay a captured block of levels sized by the deepest of
+    // Graph mode: replay a recorded block of levels sized by the deepest of
     // the last runs on this graph (a level past the end costs three no-op
     // launches, a block too short a whole extra replay), read {done, level,
     // levels} once per replay (the host waits for that copy only, while the

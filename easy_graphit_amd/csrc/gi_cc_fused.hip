@@ -1,7 +1,7 @@
 // Fused connected components: the loop of apps/cc.cpp:53-78 (hook every
 // edge, ID[ID[s]] min= ID[ID[d]] both ways, then pointer jumping until
 // stable, until a sweep changes nothing) with every decision on the device
-// and the rounds captured as a hipGraph.
+// and the rounds recorded as a hipGraph.
 //
 // Two schedules. EDGE_ONLY (the app's own, apps/cc.cpp:84-86) sweeps every
 // edge each round. VERTEX_BASED first hooks the first two neighbours of every

@@ -393,7 +393,7 @@ void release_pr(const gi_graph *g);
 // the vertices outside the most frequent label); else full EDGE_ONLY sweeps.
 int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds, bool sampled, hipStream_t s);
 void release_cc(const gi_graph *g);
-// counts (optional): relax steps, pull steps, windows entered, captured slots.
+// counts (optional): relax steps, pull steps, windows entered, recorded slots.
 int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,
                    hipStream_t s);
 void release_sssp(const gi_graph *g);

@@ -1,7 +1,7 @@
 // Fused PageRank: the iteration loop of apps/pr.cpp:28-46 (contrib =
 // rank / out-degree; new = sum of in-neighbour contribs; rank = (1-d)/N +
 // d * new; no dangling redistribution) as a pull over the in-edge CSR with
-// no atomics, captured once as a hipGraph of all iterations.
+// no atomics, recorded once as a hipGraph of all iterations.
 //
 // Work per iteration (two launches):
 //   * pr_gather: work items are groups of 64 consecutive vertices and
@@ -143,7 +143,7 @@ struct PrPlan {
     int32_t *wide = nullptr, *wide_c0 = nullptr, *chunk_b = nullptr, *chunk_e0 = nullptr;
     float *partial = nullptr;
     float *contrib[2] = {nullptr, nullptr};
-    // captured runs keyed by (iterations, output array, damping bits)
+    // recorded runs keyed by (iterations, output array, damping bits)
     std::map<std::tuple<int32_t, const float *, uint32_t>, hipGraphExec_t> execs;
 };
 
@@ -260,7 +260,7 @@ int pr_fused_run(gi_graph *g, int32_t iters, float damping, float *d_rank, hipSt
         GI_LAUNCH_CHECK();
         return GI_OK;
     }
-    // the whole run is one captured graph per (iteration count, output, damping)
+    // the whole run is one recorded graph per (iteration count, output, damping)
     uint32_t dbits = 0;
     memcpy(&dbits, &damping, sizeof(dbits));
     auto key = std::make_tuple(iters, (const float *)d_rank, dbits);

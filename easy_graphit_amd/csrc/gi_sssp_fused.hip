@@ -1,7 +1,7 @@
 // Fused delta-stepping SSSP: the loop of apps/sssp.cpp:34-57 (dequeue the
 // current priority window, relax its out-edges with writeMin, re-enqueue
 // improved vertices) with the window bookkeeping on the device and the
-// phases captured as hipGraphs. Shortest-path distances are unique, so the
+// phases recorded as hipGraphs. Shortest-path distances are unique, so the
 // order of relaxations does not change the result.
 //
 // The window is [lo, hi). Every vertex below lo is settled; the near
@@ -26,7 +26,7 @@
 // edges of W once, from final distances, instead of once per improvement of
 // each vertex within the window. Steps with smaller F relax all edges (no
 // extra step for windows that settle in one step).
-// A captured slot is five launches, each a no-op unless its mode is active:
+// A recorded slot is five launches, each a no-op unless its mode is active:
 //   ss_relax   push: relax the out-edges of F, spread evenly over each
 //              workgroup (vertices of at least kSsHubMin edges go to the
 //              grid-wide bin); an improvement below hi joins F';
@@ -70,7 +70,7 @@ struct SsState {
     int32_t phase;       // F = bm[phase % 3]
     int32_t steps;       // relax steps run
     int32_t windows;     // windows entered after the first
-    int32_t slots;       // captured slots consumed
+    int32_t slots;       // recorded slots consumed
     int32_t pulls;       // relax steps that pulled
     int32_t hub_count;
     int32_t lo;          // window [lo, hi)
@@ -83,7 +83,7 @@ struct SsState {
     int32_t defer;       // the next push step relaxes light edges only and records F in W
     int32_t ws_any;      // W holds vertices whose heavy edges wait
     long long fdeg;      // out-edges of F
-    int32_t *dist;       // the run's distances (captured slots read it from here)
+    int32_t *dist;       // the run's distances (recorded slots read it from here)
     int32_t hubs[kSsMaxHubs];
 };
 
@@ -901,7 +901,7 @@ static SsArgs ss_args(gi_graph *g, SsWork *w, int32_t *dist) {
 }
 
 // The light/heavy split for `delta` (built on first use of a delta; the
-// captured slots are dropped when the arrays they name change).
+// recorded slots are dropped when the arrays they name change).
 static int ensure_split(gi_graph *g, SsWork *w, int32_t delta, hipStream_t s) {
     if (w->split_delta == delta) return GI_OK;
     const int64_t E = g->E;
@@ -972,7 +972,7 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
         return b < 1024 ? b : 1024;
     };
     SsArgs ca = a;
-    ca.dist = nullptr;  // captured kernels take the run's dist from the state
+    ca.dist = nullptr;  // recorded kernels take the run's dist from the state
     auto exec_of = [&](int32_t block, hipGraphExec_t *out) -> int {
         auto it = w->execs.find(block);
         if (it == w->execs.end()) {
@@ -1007,8 +1007,8 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
                 counts[2] = w->host[4];
                 counts[3] = w->host[5];
             }
-            // a longer run than before: capture the next run's first block
-            // now, so that its launch does not wait on the capture
+            // a longer run than before: record the next run's first block
+            // now, so that its launch does not wait on the recording
             hipGraphExec_t next = nullptr;
             return exec_of(first_block(), &next);
         }

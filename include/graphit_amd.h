@@ -418,7 +418,7 @@ int gi_bfs_set_bins(gi_graph *g, int32_t lane_max, int32_t hub_min, int32_t bloc
  * h->s1): d_depth[v] = distance to the nearest source. Over a graph that is
  * a disjoint union with one source per component (many small graphs packed
  * in one CSR) this runs every component's BFS in the same launches. The level
- * loop is a captured hipGraph replayed until the frontier is empty. The
+ * loop is a recorded hipGraph replayed until the frontier is empty. The
  * reference only seeds one source (apps/bfs.cpp:49-52). */
 int gi_bfs_multi(gi_graph *g, const int32_t *sources, int32_t num_sources,
                  const gi_hybrid_schedule *h, int32_t *d_depth, int32_t *d_parent,
@@ -444,11 +444,11 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
  * Default schedule (s == NULL): the app's PUSH TWCE with the bucketed
  * priority queue (apps/sssp.cpp:77-79). With kernel_fusion ENABLED the
  * windows and relaxation phases run on the device (near frontier and far
- * set as bitmaps, captured phases, no host synchronisation per phase). */
+ * set as bitmaps, recorded phases, no host synchronisation per phase). */
 int gi_sssp(gi_graph *g, int32_t source, int32_t delta, const gi_schedule *s,
             int32_t *d_dist, gi_run_stats *stats, void *stream);
 /* Measurement hook of the fused SSSP engine (no reference counterpart):
- * runs the slots of the device-side window loop one at a time, uncaptured,
+ * runs the slots of the device-side window loop one at a time, outside a graph,
  * and times each launch of a slot with HIP events. pull_min: |F| above
  * which a relax step pulls (< 0: the engine's default).
  * kernel_ms[max_slots][6]: push step, pull step (one relax launch, counted
@@ -462,7 +462,7 @@ int gi_sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_min, 
 /* Connected components, hook + pointer jumping (apps/cc.cpp:53-78); labels
  * are the minimum vertex id of each component. Default schedule (s == NULL):
  * the app's PUSH EDGE_ONLY (apps/cc.cpp:84-86). With kernel_fusion ENABLED
- * (symmetric graphs) the rounds run as a captured graph with device-side
+ * (symmetric graphs) the rounds run as a recorded graph with device-side
  * termination and full shortcutting per round: EDGE_ONLY sweeps every edge
  * each round; VERTEX_BASED first hooks every vertex's first two neighbours,
  * then runs rounds over the vertices outside the most frequent label only
@@ -473,13 +473,13 @@ int gi_cc(gi_graph *g, const gi_schedule *s, int32_t *d_labels, gi_run_stats *st
 /* PageRank (apps/pr.cpp:49-69): iters iterations, FP32; d_rank = old_rank.
  * Default schedule (s == NULL): the app's PULL EDGE_ONLY BLOCKED 0x42000
  * (apps/pr.cpp:78-81). With kernel_fusion ENABLED (direction PULL) the whole
- * run is one captured graph of deterministic pull sweeps over the in-edges
+ * run is one recorded graph of deterministic pull sweeps over the in-edges
  * (no atomics). */
 int gi_pagerank(gi_graph *g, int32_t iters, float damping, const gi_schedule *s,
                 float *d_rank, gi_run_stats *stats, void *stream);
 /* Brandes BC from num_sources sources (apps/bc.cpp:150-209), summed into
  * d_scores (float, zeroed first). batch > 1: up to 256 sources advance
- * level-synchronously, the level loops on the device (captured); batch <= 1:
+ * level-synchronously, the level loops on the device (recorded); batch <= 1:
  * one source at a time as the app runs unfused (push TWCE, a host round trip
  * per level, apps/bc.cpp:149-151). */
 int gi_bc(gi_graph *g, const int32_t *h_sources, int32_t num_sources, int32_t batch,
