@@ -55,6 +55,7 @@ struct StepArgs {
     const int32_t *poff;        // in-edges (pull); the same arrays on a symmetric graph
     const int32_t *pnbr;
     const int32_t *cand;        // candidate parent per vertex, -1 when none
+    int32_t cand_first;         // 1: the candidate is the vertex's first in-neighbour
     int32_t V;
     int32_t words;
     int32_t chunks;
@@ -377,6 +378,7 @@ __device__ __forceinline__ bool pull_words(const StepArgs &a, StepCtx &c, int32_
         const bool want = !((vis[i] >> lane) & 1ull);  // padding bits past V are set
         par[i] = (want && cd[i] >= 0 && in_bitmap(c.bm_in, cd[i])) ? cd[i] : -1;
         if (!want || par[i] >= 0) e1[i] = e[i];  // closed: nothing to scan
+        else if (a.cand_first && cd[i] >= 0) ++e[i];  // the candidate is the first in-neighbour: probed
     }
     bool any = false;
     // lane-serial scan, the lane's vertices interleaved
@@ -892,9 +894,13 @@ __global__ void bfs_dead_kernel(const int32_t *poff, int32_t V, int32_t words, u
 }
 
 __global__ void bfs_cand_kernel(const int32_t *poff, const int32_t *pnbr, const int32_t *off,
-                                int32_t V, int32_t *cand) {
+                                int32_t V, int32_t first, int32_t *cand) {
     for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
         int32_t e0 = poff[v], e1 = poff[v + 1];
+        if (first) {  // the first in-neighbour (a degree-ordered view lists the widest first)
+            cand[v] = e0 < e1 ? pnbr[e0] : -1;
+            continue;
+        }
         if (e1 - e0 > kCandScan) e1 = e0 + kCandScan;
         int32_t best = -1, bd = -1;
         for (int32_t e = e0; e < e1; ++e) {
@@ -959,6 +965,7 @@ struct FusedWork {
     int32_t *reached = nullptr;
     uint64_t *dead = nullptr;        // pull-side tables (see bfs_dead_kernel)
     int32_t *cand = nullptr;
+    int32_t cand_first = 0;          // cand holds first in-neighbours (degree-ordered views)
     const int32_t *poff = nullptr;   // in-edge CSR (the graph itself when symmetric)
     const int32_t *pnbr = nullptr;
     SelectorView last_sel{};         // selector of the current run (passed to the decide kernel)
@@ -1016,6 +1023,7 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
         pg = t;
     }
     FusedWork *w = new FusedWork();
+    w->cand_first = g->degree_view ? 1 : 0;
     w->sum_words = (int32_t)sum_words;
     w->poff = pg->d_off;
     w->pnbr = pg->d_nbr;
@@ -1041,7 +1049,7 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
         int grid = (int)ceil_div(g->V, 256);
         grid = grid < 4096 ? (grid < 1 ? 1 : grid) : 4096;
         bfs_dead_kernel<<<grid, 256>>>(w->poff, g->V, (int32_t)words, w->dead);
-        bfs_cand_kernel<<<grid, 256>>>(w->poff, w->pnbr, g->d_off, g->V, w->cand);
+        bfs_cand_kernel<<<grid, 256>>>(w->poff, w->pnbr, g->d_off, g->V, w->cand_first, w->cand);
         if (hipGetLastError() != hipSuccess) rc = fail(GI_ERR_HIP, "pull table launch failed");
     }
     if (rc == GI_OK && hipDeviceSynchronize() != hipSuccess) rc = fail(GI_ERR_HIP, "init sync");
@@ -1108,6 +1116,7 @@ static StepArgs step_args(gi_graph *g, FusedWork *w) {
     a.poff = w->poff;
     a.pnbr = w->pnbr;
     a.cand = w->cand;
+    a.cand_first = w->cand_first;
     a.V = g->V;
     a.words = (int32_t)bitmap_words(g->V);
     a.chunks = (int32_t)ceil_div(a.words, kChunkWords);

@@ -329,6 +329,7 @@ struct gi_graph {
     std::map<int32_t, gi_graph *> blocked;  // owned
     gi::GraphLayout *layout = nullptr;     // degree-ordered view (gi_graph_set_layout), owned
     bool borrowed = false;  // owned by another graph
+    bool degree_view = false;  // the degree-ordered view of another graph (lists in increasing new id)
 
     gi::DGraph view() const {
         gi::DGraph g;

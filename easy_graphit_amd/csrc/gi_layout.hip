@@ -192,6 +192,7 @@ int build_layout(gi_graph *g) {
     t->V = V;
     t->E = E;
     t->directed = g->directed;
+    t->degree_view = true;
     L->view = t;
     uint32_t *k0 = nullptr, *k1 = nullptr;
     int32_t *v0 = nullptr, *v1 = nullptr, *deg = nullptr;
