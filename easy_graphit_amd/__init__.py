@@ -308,9 +308,14 @@ def copy_device(dst: DeviceArray, src: DeviceArray, stream=None):
 class Graph:
     """Device CSR graph (GraphT<int> + load_graph, runtime/infra_gpu/graph.h)."""
 
-    def __init__(self, handle, owner=True):
+    def __init__(self, handle, owner=True, parent=None):
         self.h = handle
         self.owner = owner
+        # a derived view (transpose, blocked) lives inside its base graph:
+        # hold the base so it outlives the view, and let the base invalidate
+        # its views when it drops them (set_weights)
+        self._parent = parent
+        self._views = []
 
     @property
     def num_vertices(self) -> int:
@@ -397,8 +402,28 @@ class Graph:
         return e.value
 
     def set_weights(self, seed: int, wmin: int, wmax: int):
-        check(lib().gi_graph_set_weights(self.h, seed, wmin, wmax))
+        check(lib().gi_graph_set_weights(self._handle(), seed, wmin, wmax))
+        # the library rebuilt the derived views: the old handles are gone
+        for v in self._views:
+            v.h = None
+        self._views = []
         return self
+
+    def _handle(self):
+        if self.h is None:
+            raise GIError(-1, "graph handle is no longer valid (destroyed, or a view dropped "
+                              "by its base graph's set_weights)")
+        return self.h
+
+    def _view(self, h) -> "Graph":
+        if h == self.h:  # a symmetric graph is its own transpose
+            return self
+        for v in self._views:
+            if v.h == h:
+                return v
+        v = Graph(h, owner=False, parent=self)
+        self._views.append(v)
+        return v
 
     def read_csr(self, weights: bool = False):
         n, m = self.num_vertices, self.num_edges
@@ -411,13 +436,13 @@ class Graph:
 
     def transpose(self) -> "Graph":
         h = C.c_void_p()
-        check(lib().gi_graph_transpose(self.h, C.byref(h)))
-        return Graph(h.value, owner=False)
+        check(lib().gi_graph_transpose(self._handle(), C.byref(h)))
+        return self._view(h.value)
 
     def blocked(self, block_size: int) -> "Graph":
         h = C.c_void_p()
-        check(lib().gi_graph_blocked(self.h, block_size, C.byref(h)))
-        return Graph(h.value, owner=False)
+        check(lib().gi_graph_blocked(self._handle(), block_size, C.byref(h)))
+        return self._view(h.value)
 
     def set_layout(self, layout: int):
         """LAYOUT_DEGREE: fused BFS and CC run on a degree-ordered copy
@@ -431,6 +456,9 @@ class Graph:
 
     def destroy(self):
         if self.owner and self.h:
+            for v in self._views:
+                v.h = None
+            self._views = []
             lib().gi_graph_destroy(self.h)
         self.h = None
 

@@ -178,13 +178,14 @@ static int bfs_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int
         if (t0) (void)hipEventDestroy(t0);
         if (t1) (void)hipEventDestroy(t1);
         if (rc == GI_OK && trace) {
-            for (int32_t i = 0; i < levels && trace->n < trace->max; ++i) {
+            const int32_t recorded = (int32_t)ft.modes.size();  // the first kBfsMaxLevels levels
+            for (int32_t i = 0; i < levels && i < recorded && trace->n < trace->max; ++i) {
                 gi_level_record &r = trace->recs[trace->n++];
                 r.level = i;
                 r.direction = ft.modes[i] >= BFS_PULL_THREAD ? GI_PULL : GI_PUSH;
                 r.load_balancing = 100 + ft.modes[i];  // fused mode id
                 r.frontier_size = ft.sizes[i];
-                r.next_size = i + 1 < levels ? ft.sizes[i + 1] : 0;
+                r.next_size = i + 1 < levels && i + 1 < recorded ? ft.sizes[i + 1] : 0;
             }
             trace->ev.swap(ft.ev);
         }
@@ -192,9 +193,8 @@ static int bfs_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int
         GI_TRY(rc);
         if (stats) {
             stats->iterations = levels;
-            stats->push_iterations = 0;
-            stats->pull_iterations = 0;
-            for (int32_t m : ft.modes) (m >= BFS_PULL_THREAD ? stats->pull_iterations : stats->push_iterations)++;
+            stats->pull_iterations = ft.pull_levels;
+            stats->push_iterations = levels - ft.pull_levels;
             stats->ms = ms;
             stats->edges_traversed = 0;
             GI_TRY(fused_reached(g, d_depth, &stats->reached, s));
@@ -252,9 +252,11 @@ static int bfs_run(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int
         stats->edges_traversed = 0;
         int32_t *cnt = nullptr;
         GI_TRY(dalloc(&cnt, 1));
-        fill_i32(cnt, 1, 0, s);
-        count_reached_kernel<<<grid_for(g->V), 256, 0, s>>>(d_depth, g->V, cnt);
-        int rc = read_count(cnt, &stats->reached, s);
+        int rc = fill_i32(cnt, 1, 0, s);
+        if (rc == GI_OK) {
+            count_reached_kernel<<<grid_for(g->V), 256, 0, s>>>(d_depth, g->V, cnt);
+            rc = read_count(cnt, &stats->reached, s);
+        }
         dfree(cnt);
         GI_TRY(rc);
     }
@@ -598,13 +600,13 @@ int gi_cc(gi_graph *g, const gi_schedule *sched, int32_t *d_labels, gi_run_stats
     int32_t flags[2] = {1, 1};
     int32_t sweeps = 0;
     while (rc == GI_OK && flags[0]) {
-        fill_i32(upd, 2, 0, s);
-        rc = edgeset_apply(g, sched, nullptr, nullptr, GI_OP_CC_HOOK, a, s, nullptr);
+        rc = fill_i32(upd, 2, 0, s);
+        if (rc == GI_OK) rc = edgeset_apply(g, sched, nullptr, nullptr, GI_OP_CC_HOOK, a, s, nullptr);
         ++sweeps;
         flags[1] = 1;
         while (rc == GI_OK && flags[1]) {
-            fill_i32(upd + 1, 1, 0, s);
-            rc = gi_vertexset_apply(g, nullptr, GI_VOP_CC_PJUMP, &a, s);
+            rc = fill_i32(upd + 1, 1, 0, s);
+            if (rc == GI_OK) rc = gi_vertexset_apply(g, nullptr, GI_VOP_CC_PJUMP, &a, s);
             if (rc == GI_OK && hipMemcpyAsync(flags, upd, 2 * sizeof(int32_t), hipMemcpyDeviceToHost, s) != hipSuccess) rc = fail(GI_ERR_HIP, "read flags");
             if (rc == GI_OK && hipStreamSynchronize(s) != hipSuccess) rc = fail(GI_ERR_HIP, "sync");
         }

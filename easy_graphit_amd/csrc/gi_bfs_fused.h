@@ -45,9 +45,12 @@ struct BfsState {
     int32_t lane_max;               // push: degree < lane_max walked by one lane
     int32_t hub_min;                // push TWC: degree >= hub_min goes to the grid bin
     int32_t want_fdeg;              // accumulate frontier degrees (learned selector, sweeps)
+    int32_t pull_levels;            // levels expanded by a pull step so far
     int32_t *depth;                 // output arrays of the current run
     int32_t *parent;
     int32_t hubs[kMaxHubs];
+    // per-level history of the first kBfsMaxLevels levels (traces and
+    // statistics); deeper runs go on, unrecorded
     int32_t hist_mode[kBfsMaxLevels];
     int32_t hist_next[kBfsMaxLevels];
 };
@@ -110,8 +113,9 @@ __host__ __device__ inline void bfs_raw_features(float *f, int32_t n, unsigned l
 
 struct FusedTrace {
     std::vector<hipEvent_t> ev;   // two per level: around the step + hub kernels
-    std::vector<int32_t> modes;   // mode of each level
-    std::vector<int32_t> sizes;   // frontier size of each level
+    std::vector<int32_t> modes;   // mode of each level (the first kBfsMaxLevels)
+    std::vector<int32_t> sizes;   // frontier size of each level (the first kBfsMaxLevels)
+    int32_t pull_levels = 0;      // levels expanded by pull steps (all of them)
 };
 
 int32_t fused_mode_of(const gi_schedule &s);

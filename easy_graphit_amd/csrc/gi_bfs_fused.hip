@@ -791,6 +791,7 @@ bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int64_t E, int32_t 
         st->mode = mode;
         st->level = next_level;
         st->hub_count = 0;
+        if (mode >= BFS_PULL_THREAD) st->pull_levels += 1;
         if (next_level < kBfsMaxLevels) {
             st->hist_mode[next_level] = mode;
             st->hist_next[next_level] = n;
@@ -867,6 +868,7 @@ __global__ void bfs_reset_kernel(BfsState *st, SeedArgs sa, ResetArgs r, int64_t
         st->done = 0;
         st->level = -1;
         st->levels = 0;
+        st->pull_levels = 0;
         st->mode = BFS_PUSH_THREAD;
         st->unvisited = r.V;
         st->unvisited_edges = E;
@@ -1246,7 +1248,9 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
         // {done, level, levels, mode} after every decide
         static_assert(offsetof(BfsState, mode) == offsetof(BfsState, done) + 3 * sizeof(int32_t),
                       "done, level, levels, mode are contiguous");
-        for (int32_t level = 0; level < kBfsMaxLevels; ++level) {
+        // a BFS has at most V levels: the bound only guards against a state
+        // that never reports done
+        for (int64_t level = 0; level <= (int64_t)g->V + 1; ++level) {
             GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
             GI_HIP(hipStreamSynchronize(s));
             if (w->host_flag[0]) {
@@ -1261,7 +1265,7 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
             launch_level_traced(g, w, a, w->host_flag[3], s, &trace->ev[trace->ev.size() - 2]);
             GI_LAUNCH_CHECK();
         }
-        return fail(GI_ERR_RANGE, "BFS deeper than %d levels", kBfsMaxLevels);
+        return fail(GI_ERR_HIP, "BFS state never reported done");
     }
     // Graph mode: replay a recorded block of levels sized by the deepest of
     // the last runs on this graph (a level past the end costs three no-op
@@ -1274,8 +1278,8 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
     if (block <= 0) block = 8;
     if (block > 48) block = 48;
     int32_t next_block = 8;
-    int32_t level = 0;
-    while (level < kBfsMaxLevels) {
+    int64_t level = 0;
+    while (level <= (int64_t)g->V + 1) {  // no depth limit: a BFS has at most V levels
         hipGraphExec_t exec = nullptr;
         GI_TRY(level_graph(g, w, block, &exec));
         GI_HIP(hipGraphLaunch(exec, s));
@@ -1291,7 +1295,7 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
         block = next_block;
         next_block = next_block < 48 ? (next_block * 2 < 48 ? next_block * 2 : 48) : 48;
     }
-    return fail(GI_ERR_RANGE, "BFS deeper than %d levels", kBfsMaxLevels);
+    return fail(GI_ERR_HIP, "BFS state never reported done");
 }
 
 int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources, int32_t single,
@@ -1316,6 +1320,7 @@ int bfs_fused_run_multi(gi_graph *g, const int32_t *h_sources, int32_t n_sources
         trace->sizes.resize(n);
         GI_HIP(hipMemcpy(trace->modes.data(), w->st->hist_mode, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
         GI_HIP(hipMemcpy(trace->sizes.data(), w->st->hist_next, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+        GI_HIP(hipMemcpy(&trace->pull_levels, &w->st->pull_levels, sizeof(int32_t), hipMemcpyDeviceToHost));
     }
     return GI_OK;
 }
@@ -1445,25 +1450,25 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
     if (rc == GI_OK) rc = seed_run(g, w, nullptr, 0, source, depth, nullptr, sel, true, s);
     StepArgs a = step_args(g, w);
     int32_t nl = 0, prev_mode = BFS_PUSH_THREAD, prev_n = 0;
-    while (rc == GI_OK && nl < max_levels) {
+    // One level: snapshot the state, time every mode of the mask from it with
+    // the production launches (bfs_step + bfs_hub_kernel, as a recorded level
+    // has them), then advance by the followed mode. Returns 1 when the BFS
+    // is done; errors return through rc so the buffers below are freed.
+    auto one_level = [&]() -> int {
         SweepHost h;
-        if (hipMemcpyAsync(&h, w->st, sizeof(h), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
-            rc = fail(GI_ERR_HIP, "state read");
-            break;
-        }
-        if (h.done) break;
+        GI_HIP(hipMemcpyAsync(&h, w->st, sizeof(h), hipMemcpyDeviceToHost, s));
+        GI_HIP(hipStreamSynchronize(s));
+        if (h.done) return 1;
         int32_t p = h.level & 1;
         bfs_raw_features(features + (int64_t)nl * kSelInputs, h.count[p], h.fdeg[p], h.unvisited,
                          h.unvisited_edges, h.level, prev_mode, prev_n, g->V, g->E);
-        const int32_t iout = (h.level + 1) % 3, iclr = (h.level + 2) % 3;
-        // snapshot, then time every mode of the mask from the same state; the
-        // step clears bitmap iclr, which is already empty after the first run
+        const int32_t iout = (h.level + 1) % 3;
+        // the step clears bitmap (level + 2) % 3, already empty after the first run
         GI_HIP(hipMemcpyAsync(depth_bak, depth, sizeof(int32_t) * g->V, hipMemcpyDeviceToDevice, s));
         GI_HIP(hipMemcpyAsync(vis_bak, g->d_visited, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, s));
         float best = 3.4e38f;
         int32_t best_mode = -1;
-        for (int32_t m = 0; m <= BFS_NUM_MODES && rc == GI_OK; ++m) {
+        for (int32_t m = 0; m <= BFS_NUM_MODES; ++m) {
             bool final_run = m == BFS_NUM_MODES;
             if (!final_run && !((mode_mask >> m) & 1)) {
                 times_ms[(int64_t)nl * BFS_NUM_MODES + m] = -1.f;
@@ -1480,12 +1485,8 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
                 zero_slots_kernel<<<8, 256, 0, s>>>(w->pcount, w->pfdeg);
                 bfs_force_mode_kernel<<<1, 1, 0, s>>>(w->st, run);
                 GI_HIP(hipEventRecord(e0, s));
-                if (run >= BFS_PULL_THREAD) {
-                    bfs_pull_step<<<pull_grid(w), kStepBlock, 0, s>>>(a);
-                } else {
-                    bfs_push_step<<<push_grid(w), kStepBlock, 0, s>>>(a);
-                    bfs_hub_kernel<<<push_grid(w), kStepBlock, 0, s>>>(a);
-                }
+                bfs_step<<<step_grid(w), kStepBlock, 0, s>>>(a);
+                bfs_hub_kernel<<<push_grid(w), kStepBlock, 0, s>>>(a);
                 GI_HIP(hipEventRecord(e1, s));
                 GI_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
@@ -1504,9 +1505,14 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
                 prev_mode = run;
             }
         }
-        (void)iclr;
         prev_n = h.count[p];
         ++nl;
+        return GI_OK;
+    };
+    while (rc == GI_OK && nl < max_levels) {
+        int r = one_level();
+        if (r == 1) break;
+        rc = r;
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

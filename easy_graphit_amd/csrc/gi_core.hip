@@ -564,6 +564,9 @@ int gi_graph_grid(int32_t width, int64_t num_random, uint64_t seed, gi_graph **o
 
 int gi_graph_set_weights(gi_graph *g, uint64_t seed, int32_t wmin, int32_t wmax) {
     if (!g || wmax < wmin) return fail(GI_ERR_INVALID, "bad arguments");
+    // a transposed or blocked view is owned by its base graph, whose weights
+    // it copies: new weights go on the base graph, which rebuilds its views
+    if (g->borrowed || !g->d_off) return fail(GI_ERR_UNSUPPORTED, "set the weights of a base graph");
     hipStream_t s = 0;
     GI_TRY(ensure_coo(g, s));
     if (!g->d_w) GI_TRY(dalloc(&g->d_w, (size_t)g->E + 1));

@@ -356,6 +356,10 @@ int read_serialized(const std::vector<char> &buf, bool weighted, gi_host_csr *ou
     int64_t E = 0, V = 0;
     if (!take(&directed, 1) || !take(&E, 8) || !take(&V, 8)) return fail(GI_ERR_INVALID, "truncated serialized graph");
     if (V < 0 || V > INT32_MAX - 1 || E < 0 || E > INT32_MAX) return fail(GI_ERR_RANGE, "serialized graph too large");
+    // the header's sizes are checked against the bytes present before any
+    // allocation: a truncated or corrupt header must not request gigabytes
+    const uint64_t need = 8ull * ((uint64_t)V + 1) + (weighted ? 8ull : 4ull) * (uint64_t)E;
+    if (need > (uint64_t)(buf.size() - at)) return fail(GI_ERR_INVALID, "truncated serialized graph");
     std::vector<int64_t> off((size_t)V + 1);
     if (!take(off.data(), sizeof(int64_t) * ((size_t)V + 1))) return fail(GI_ERR_INVALID, "truncated offsets");
     out->num_vertices = (int32_t)V;

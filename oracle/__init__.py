@@ -65,6 +65,8 @@ def lib():
             "or_sssp_delta_stepping": (i64, [i32, i32p, i32p, i32p, i32, i32, i32p]),
             "or_cc_labels": (None, [i32, i32p, i32p, i32p]),
             "or_cc_label_propagation": (i32, [i32, i32p, i32p, i32p]),
+            "or_set_threads": (None, [i32]),
+            "or_max_threads": (i32, []),
             "or_pagerank": (None, [i32, i32p, i32p, i32, C.c_double, f64p]),
             "or_bc": (None, [i32, i32p, i32p, i32p, i32, f64p]),
             "or_mlp_forward": (None, [i32, i32, i32, i32] + [f64p] * 10),
@@ -307,6 +309,14 @@ def pagerank(g: CSR, iters: int = 20, damping: float = PR_DAMP):
     r = np.empty(g.num_vertices, np.float64)
     lib().or_pagerank(g.num_vertices, g.offsets, g.nbrs, iters, damping, r)
     return r
+
+
+def set_threads(n: int) -> int:
+    """Threads of the OpenMP oracle functions (pagerank, bc); returns the
+    previous count."""
+    prev = int(lib().or_max_threads())
+    lib().or_set_threads(int(n))
+    return prev
 
 
 def bc(g: CSR, sources):

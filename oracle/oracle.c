@@ -27,9 +27,11 @@
  *                                  or_cc_label_propagation (apps/cc.cpp restated).
  *   or_pagerank                    closed forms (directed cycle: uniform 1/N; star);
  *                                  networkx.pagerank on dangling-free graphs at
- *                                  convergence; dense-matrix power iteration.
+ *                                  convergence; dense-matrix power iteration; the
+ *                                  serial push loop restated in Python (bit-equal).
  *   or_bc                          brute-force all-shortest-path enumeration on tiny
- *                                  graphs; networkx single-source dependencies.
+ *                                  graphs; networkx single-source dependencies; equal
+ *                                  (to FP64 rounding) at 1 and 8 OpenMP threads.
  *   or_mlp_*                       central finite differences of the loss.
  */
 #include "oracle.h"
@@ -41,6 +43,13 @@
 #include <string.h>
 
 #define GOLDEN 0x9e3779b97f4a7c15ULL
+
+/* Threads of the OpenMP functions (or_pagerank, or_bc); 0 = OpenMP's default. */
+void or_set_threads(int32_t n) {
+    if (n > 0) omp_set_num_threads(n);
+}
+
+int32_t or_max_threads(void) { return omp_get_max_threads(); }
 
 /* ------------------------------------------------------------------------
  * Seeded generators. The reference generator (runtime/infra_gapbs/generator.h:
@@ -682,24 +691,42 @@ int32_t or_cc_label_propagation(int32_t num_vertices, const int32_t *offsets,
  * ---------------------------------------------------------------------- */
 void or_pagerank(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,
                  int32_t iters, double damping, double *rank) {
+    /* The serial loop adds contrib[u] into new[v] for u ascending. Pulling
+     * over the in-neighbours of v in ascending order (the transpose built
+     * below lists them so) performs the same additions in the same order,
+     * so every vertex's sum is bit-identical to the serial push, and the
+     * vertices can be summed in parallel (OpenMP). */
     int64_t n = num_vertices;
-    double *nw = (double *)calloc((size_t)n, sizeof(double));
-    double *contrib = (double *)calloc((size_t)n, sizeof(double));
+    int64_t m = n ? offsets[n] : 0;
+    int32_t *toff = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+    int32_t *tsrc = (int32_t *)malloc((size_t)(m > 0 ? m : 1) * sizeof(int32_t));
+    int32_t *fill = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+    for (int64_t e = 0; e < m; ++e) toff[nbrs[e] + 1]++;
+    for (int64_t v = 0; v < n; ++v) toff[v + 1] += toff[v];
+    for (int64_t u = 0; u < n; ++u) /* u ascending: each in-list sorted by source */
+        for (int32_t e = offsets[u]; e < offsets[u + 1]; ++e) {
+            int32_t v = nbrs[e];
+            tsrc[toff[v] + fill[v]++] = (int32_t)u;
+        }
+    free(fill);
+    double *contrib = (double *)calloc((size_t)(n > 0 ? n : 1), sizeof(double));
     for (int64_t v = 0; v < n; ++v) rank[v] = 1.0 / (double)n;
     double beta = (1.0 - damping) / (double)n;
     for (int32_t it = 0; it < iters; ++it) {
+#pragma omp parallel for schedule(static)
         for (int64_t v = 0; v < n; ++v) {
             int32_t deg = offsets[v + 1] - offsets[v];
             contrib[v] = deg ? rank[v] / deg : 0.0; /* never read when deg==0 */
         }
-        for (int64_t u = 0; u < n; ++u)
-            for (int32_t e = offsets[u]; e < offsets[u + 1]; ++e) nw[nbrs[e]] += contrib[u];
+#pragma omp parallel for schedule(dynamic, 4096)
         for (int64_t v = 0; v < n; ++v) {
-            rank[v] = beta + damping * nw[v];
-            nw[v] = 0.0;
+            double acc = 0.0;
+            for (int32_t e = toff[v]; e < toff[v + 1]; ++e) acc += contrib[tsrc[e]];
+            rank[v] = beta + damping * acc;
         }
     }
-    free(nw);
+    free(toff);
+    free(tsrc);
     free(contrib);
 }
 
@@ -710,60 +737,86 @@ void or_pagerank(int32_t num_vertices, const int32_t *offsets, const int32_t *nb
  * processed; final (D - 1/sigma) * sigma, 0 where sigma == 0. The source
  * keeps its own term. Scores are summed over sources.
  * ---------------------------------------------------------------------- */
+static void bc_one_source(int64_t n, const int32_t *offsets, const int32_t *nbrs, int32_t s,
+                          double *sigma, double *D, int32_t *depth, int32_t *order,
+                          int32_t *lvl_start, unsigned char *done, double *bc) {
+    for (int64_t v = 0; v < n; ++v) { sigma[v] = 0.0; D[v] = 0.0; depth[v] = -1; }
+    sigma[s] = 1.0;
+    depth[s] = 0;
+    int64_t head = 0, tail = 0;
+    int32_t nl = 0;
+    order[tail++] = s;
+    lvl_start[0] = 0;
+    /* level-synchronous forward pass */
+    while (head < tail) {
+        int64_t lend = tail;
+        lvl_start[++nl] = (int32_t)lend;
+        for (int64_t i = head; i < lend; ++i) {
+            int32_t u = order[i];
+            for (int32_t e = offsets[u]; e < offsets[u + 1]; ++e) {
+                int32_t v = nbrs[e];
+                if (depth[v] == -1) { depth[v] = depth[u] + 1; order[tail++] = v; }
+                if (depth[v] == depth[u] + 1) sigma[v] += sigma[u];
+            }
+        }
+        head = lend;
+    }
+    /* levels 0..nl-1 occupy order[lvl_start[l] .. lvl_start[l+1]) */
+    memset(done, 0, (size_t)n);
+    for (int32_t l = nl - 1; l >= 0; --l) {
+        if (l < nl - 1) {
+            /* push D from level l+1 to its not-yet-processed out-neighbours */
+            for (int32_t i = lvl_start[l + 1]; i < lvl_start[l + 2]; ++i) {
+                int32_t u = order[i];
+                for (int32_t e = offsets[u]; e < offsets[u + 1]; ++e)
+                    if (!done[nbrs[e]]) D[nbrs[e]] += D[u];
+            }
+        }
+        for (int32_t i = lvl_start[l]; i < lvl_start[l + 1]; ++i) {
+            int32_t u = order[i];
+            done[u] = 1;
+            D[u] += 1.0 / sigma[u];
+        }
+    }
+    /* only the reached vertices have sigma != 0 */
+    for (int64_t i = 0; i < tail; ++i) {
+        int32_t v = order[i];
+        bc[v] += (D[v] - 1.0 / sigma[v]) * sigma[v];
+    }
+}
+
+/* Sources run in parallel (OpenMP, a static split so a thread count fixes
+ * the order): each thread sums its sources into a private array in source
+ * order, and the private arrays are added in thread order. With one thread
+ * this is the serial loop. */
 void or_bc(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,
            const int32_t *sources, int32_t num_sources, double *bc) {
     int64_t n = num_vertices;
-    double *sigma = (double *)malloc((size_t)n * sizeof(double));
-    double *D = (double *)malloc((size_t)n * sizeof(double));
-    int32_t *depth = (int32_t *)malloc((size_t)n * sizeof(int32_t));
-    int32_t *order = (int32_t *)malloc((size_t)n * sizeof(int32_t));
-    int32_t *lvl_start = (int32_t *)malloc((size_t)(n + 2) * sizeof(int32_t));
-    unsigned char *done = (unsigned char *)malloc((size_t)n);
     for (int64_t v = 0; v < n; ++v) bc[v] = 0.0;
-    for (int32_t si = 0; si < num_sources; ++si) {
-        int32_t s = sources[si];
-        for (int64_t v = 0; v < n; ++v) { sigma[v] = 0.0; D[v] = 0.0; depth[v] = -1; }
-        sigma[s] = 1.0;
-        depth[s] = 0;
-        int64_t head = 0, tail = 0;
-        int32_t nl = 0;
-        order[tail++] = s;
-        lvl_start[0] = 0;
-        /* level-synchronous forward pass */
-        while (head < tail) {
-            int64_t lend = tail;
-            lvl_start[++nl] = (int32_t)lend;
-            for (int64_t i = head; i < lend; ++i) {
-                int32_t u = order[i];
-                for (int32_t e = offsets[u]; e < offsets[u + 1]; ++e) {
-                    int32_t v = nbrs[e];
-                    if (depth[v] == -1) { depth[v] = depth[u] + 1; order[tail++] = v; }
-                    if (depth[v] == depth[u] + 1) sigma[v] += sigma[u];
-                }
-            }
-            head = lend;
-        }
-        /* levels 0..nl-1 occupy order[lvl_start[l] .. lvl_start[l+1]) */
-        memset(done, 0, (size_t)n);
-        for (int32_t l = nl - 1; l >= 0; --l) {
-            if (l < nl - 1) {
-                /* push D from level l+1 to its not-yet-processed out-neighbours */
-                for (int32_t i = lvl_start[l + 1]; i < lvl_start[l + 2]; ++i) {
-                    int32_t u = order[i];
-                    for (int32_t e = offsets[u]; e < offsets[u + 1]; ++e)
-                        if (!done[nbrs[e]]) D[nbrs[e]] += D[u];
-                }
-            }
-            for (int32_t i = lvl_start[l]; i < lvl_start[l + 1]; ++i) {
-                int32_t u = order[i];
-                done[u] = 1;
-                D[u] += 1.0 / sigma[u];
-            }
-        }
-        for (int64_t v = 0; v < n; ++v)
-            if (sigma[v] != 0.0) bc[v] += (D[v] - 1.0 / sigma[v]) * sigma[v];
+    int nt = omp_get_max_threads();
+    if (nt > num_sources) nt = num_sources > 0 ? num_sources : 1;
+    double **part = (double **)calloc((size_t)nt, sizeof(double *));
+#pragma omp parallel num_threads(nt)
+    {
+        int t = omp_get_thread_num();
+        double *sigma = (double *)malloc((size_t)n * sizeof(double));
+        double *D = (double *)malloc((size_t)n * sizeof(double));
+        int32_t *depth = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+        int32_t *order = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+        int32_t *lvl_start = (int32_t *)malloc((size_t)(n + 2) * sizeof(int32_t));
+        unsigned char *done = (unsigned char *)malloc((size_t)n);
+        double *mine = t == 0 ? bc : (double *)calloc((size_t)n, sizeof(double));
+        part[t] = mine;
+#pragma omp for schedule(static)
+        for (int32_t si = 0; si < num_sources; ++si)
+            bc_one_source(n, offsets, nbrs, sources[si], sigma, D, depth, order, lvl_start, done, mine);
+        free(sigma); free(D); free(depth); free(order); free(lvl_start); free(done);
     }
-    free(sigma); free(D); free(depth); free(order); free(lvl_start); free(done);
+    for (int t = 1; t < nt; ++t) {
+        for (int64_t v = 0; v < n; ++v) bc[v] += part[t][v];
+        free(part[t]);
+    }
+    free(part);
 }
 
 /* ------------------------------------------------------------------------

@@ -65,6 +65,8 @@ void or_cc_labels(int32_t num_vertices, const int32_t *offsets, const int32_t *n
                   int32_t *labels);
 int32_t or_cc_label_propagation(int32_t num_vertices, const int32_t *offsets,
                                 const int32_t *nbrs, int32_t *labels);
+void or_set_threads(int32_t n);
+int32_t or_max_threads(void);
 void or_pagerank(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,
                  int32_t iters, double damping, double *rank);
 void or_bc(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,

@@ -70,6 +70,32 @@ def test_fused_grid_graph_many_levels(gi):
     assert np.array_equal(depth, O.bfs_depth(host, 0)) and st["iterations"] == 399
 
 
+@pytest.mark.parametrize("layout", [0, 1])
+def test_fused_bfs_deeper_than_level_history(gi, layout):
+    # no depth limit (the reference's fused loop has none, apps/bfs.cpp:53-57):
+    # a path of 9,000 vertices with a side branch every 1,000 is 9,000 levels
+    # deep, past the 4,096 levels whose per-level history is kept; the
+    # statistics still count every level
+    n = 9000
+    a = np.arange(n - 1, dtype=np.int32)
+    side = np.arange(0, n, 1000, dtype=np.int32)
+    src = np.concatenate([a, side]).astype(np.int32)
+    dst = np.concatenate([a + 1, n + np.arange(len(side))]).astype(np.int32)
+    V = n + len(side)
+    g = gi.Graph.from_edges(V, src, dst, True).set_layout(layout)
+    host = O.build_csr(V, src, dst, True)
+    for mode_pair in ((4, 5), (0, 6)):
+        h = gi.HybridSchedule(fused(gi, mode_pair[0]), fused(gi, mode_pair[1]), 0.05, gi.SELECT_HEURISTIC)
+        for s in (0, n - 1):
+            want = O.bfs_depth(host, s)
+            depth, _, st = gi.bfs(g, s, h)
+            assert np.array_equal(depth, want), (mode_pair, s)
+            assert st["iterations"] == want.max() + 1
+            assert st["push_iterations"] + st["pull_iterations"] == st["iterations"]
+    recs = gi.bfs_profile(g, 0, h, gi.DeviceArray(V, np.int32))
+    assert len(recs) == 4096  # per-level records: the first 4,096 levels
+
+
 def test_sweep_records_every_mode(gi, c1):
     g, host, want = c1
     feats, times = gi.bfs_sweep(g, 0, reps=1)

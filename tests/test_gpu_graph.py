@@ -62,3 +62,27 @@ def test_transpose_bit_exact(gi):
     same_csr(t, th, weights=True)
     sym = gi.Graph.rmat(8, 4, 9, True)
     assert sym.transpose().h == sym.h  # symmetric CSR is its own transpose
+
+
+def test_set_weights_on_views_fails_and_base_stays_usable(gi):
+    # a transpose or blocked view belongs to its base graph: new weights go on
+    # the base, the view refuses them, and the base keeps working
+    g = gi.Graph.rmat(10, 8, 3, False).set_weights(1, 1, 50)
+    t = g.transpose()
+    b = g.blocked(256)
+    for view in (t, b):
+        with pytest.raises(gi.GIError):
+            view.set_weights(2, 1, 50)
+    off, nb, w = g.read_csr(weights=True)
+    assert len(nb) == g.num_edges and w.min() >= 1
+    # new weights on the base drop its views: the old handles are refused
+    g.set_weights(3, 1, 50)
+    with pytest.raises(gi.GIError):
+        t.set_weights(4, 1, 50)
+    t2 = g.transpose()
+    toff, tnb, tw = t2.read_csr(weights=True)
+    host = O.transpose(O.CSR(*g.read_csr(weights=True)))
+    assert np.array_equal(toff, host.offsets) and np.array_equal(tnb, host.nbrs)
+    # a view outlives a base the caller no longer holds
+    v = gi.Graph.rmat(9, 8, 5, False).transpose()
+    assert v.num_vertices == 512 and v.read_csr()[0][-1] == v.num_edges

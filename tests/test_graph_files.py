@@ -187,3 +187,15 @@ def test_empty_and_large_ids(lib, tmp_path):
     off, nb, _, _ = lib.read_graph_file(str(p), True)
     assert len(off) == 100001 and nb.tolist() == [99999, 0]
     assert same(lib.read_graph_file(str(p), True), O.read_graph_file(str(p), True))
+
+
+@pytest.mark.parametrize("suffix", ["sg", "wsg"])
+def test_truncated_serialized_headers(lib, tmp_path, suffix):
+    # a header whose V and E ask for far more bytes than the file holds is an
+    # error before anything is allocated (not a multi-gigabyte allocation)
+    import struct
+    for V, E, body in ((1 << 30, 1 << 30, b""), (10, 1 << 31 - 1, b"\0" * 88), (3, 4, b"\0" * 20)):
+        p = tmp_path / f"t{V}_{E}.{suffix}"
+        p.write_bytes(struct.pack("<Bqq", 0, E, V) + body)
+        with pytest.raises(lib.GIError):
+            lib.read_graph_file(str(p))

@@ -443,3 +443,41 @@ def test_openmp_bfs_matches_serial():
                 got, used = O.bfs_depth_omp(g, int(src), threads)
                 assert np.array_equal(got, want), (g.directed, src, threads)
                 assert used == threads
+
+
+def test_pagerank_pull_equals_serial_push_bit_for_bit():
+    # the oracle pulls over in-neighbours in ascending source order (in
+    # parallel); the serial push of apps/pr.cpp adds the same terms in the
+    # same order, restated here with np.add.at (unbuffered, in index order)
+    s, d = O.rmat_edges(10, 8, 21)
+    g = O.build_csr(1024, s, d, False)
+    n = g.num_vertices
+    deg = np.diff(g.offsets)
+    src_of_edge = np.repeat(np.arange(n), deg)
+    rank = np.full(n, 1.0 / n)
+    beta = (1.0 - O.PR_DAMP) / n
+    for _ in range(7):
+        contrib = np.where(deg > 0, rank / np.maximum(deg, 1), 0.0)
+        nw = np.zeros(n)
+        np.add.at(nw, g.nbrs, contrib[src_of_edge])
+        rank = beta + O.PR_DAMP * nw
+    for threads in (1, 8):
+        prev = O.set_threads(threads)
+        got = O.pagerank(g, 7)
+        O.set_threads(prev)
+        assert np.array_equal(got, rank), threads
+
+
+def test_bc_thread_counts_agree():
+    # sources split over threads: per-thread sums in source order, added in
+    # thread order; equal to the serial loop up to FP64 rounding
+    s, d = O.rmat_edges(10, 8, 31)
+    g = O.build_csr(1024, s, d, True)
+    srcs = O.pick_sources(g, 3, 37)
+    prev = O.set_threads(1)
+    one = O.bc(g, srcs)
+    O.set_threads(8)
+    many = O.bc(g, srcs)
+    O.set_threads(prev)
+    assert np.allclose(one, many, rtol=1e-12, atol=1e-12)
+    assert np.allclose(one, O.bc(g, srcs[:1]) + O.bc(g, srcs[1:]), rtol=1e-12, atol=1e-12)
