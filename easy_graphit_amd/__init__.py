@@ -195,6 +195,7 @@ def lib():
         "gi_frontier_list_levels": (C.c_int, [vp, C.POINTER(i32)]),
         "gi_frontier_list_destroy": (C.c_int, [vp]),
         "gi_bfs": (C.c_int, [vp, i32, C.POINTER(HybridSchedule), vp, vp, C.POINTER(RunStats), vp]),
+        "gi_bfs_static": (C.c_int, [vp, i32, C.POINTER(Schedule), vp, vp, vp]),
         "gi_bfs_profile": (C.c_int, [vp, i32, C.POINTER(HybridSchedule), vp,
                                      C.POINTER(LevelRecord), i32, C.POINTER(i32), vp]),
         "gi_graph_pick_sources": (C.c_int, [vp, u64, i32, vp]),
@@ -612,6 +613,18 @@ def bfs_device(g: Graph, source: int, schedule, depth: DeviceArray, parent=None,
                        depth.ptr, parent.ptr if parent else None,
                        C.byref(st) if st is not None else None, stream))
     return st.as_dict() if st is not None else None
+
+
+BFS_STATIC_MAX_VERTICES = 1 << 21  # GI_BFS_STATIC_MAX_VERTICES
+
+
+def bfs_static(g: Graph, source: int, schedule: Schedule | None, depth: DeviceArray,
+               parent: DeviceArray | None = None, stream=None):
+    """BFS that allocates nothing (gi_bfs_static): the unfused push loop with
+    its frontier and visited bitmap in the library's fixed device array;
+    results go into the caller's device arrays."""
+    check(lib().gi_bfs_static(g.h, source, C.byref(schedule) if schedule is not None else None,
+                              depth.ptr, parent.ptr if parent else None, stream))
 
 
 def bfs_multi(g: Graph, sources, schedule: HybridSchedule, depth: DeviceArray | None = None,

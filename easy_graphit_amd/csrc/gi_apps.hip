@@ -843,3 +843,126 @@ int gi_bc_schedule(gi_graph *g, const int32_t *h_sources, int32_t num_sources, c
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------- BFS without allocation
+// gi_bfs_static: the unfused push BFS of gi_bfs (the same edgeset_apply and
+// load-balanced kernels) with its frontier queues, counters and visited
+// bitmap in a fixed device array instead of memory the library allocates,
+// for callers that cannot allocate. Layout of `scratch` (int32 units):
+//   [0, 64)                      frontier counts and overflow flag
+//   [64, 64 + Vmax)              queue 0
+//   [64 + Vmax, 64 + 2 Vmax)     queue 1
+//   [64 + 2 Vmax, ...)           visited bitmap, Vmax / 64 words of 64 bits
+// Each GPU holds one copy until the process ends; every call on a GPU uses
+// that GPU's copy, so calls that overlap on one GPU race.
+namespace gi {
+namespace {
+
+constexpr int64_t kStaticCounts = 64;
+constexpr int64_t kStaticQueue = GI_BFS_STATIC_MAX_VERTICES;
+constexpr int64_t kStaticWords = kStaticCounts + 2 * kStaticQueue + 2 * (kStaticQueue / 64);
+static_assert(kStaticQueue % 64 == 0, "whole bitmap words");
+static_assert((kStaticCounts + 2 * kStaticQueue) % 2 == 0, "the bitmap starts 8-byte aligned");
+
+}  // namespace
+}  // namespace gi
+
+static __device__ __align__(256) int32_t scratch[gi::kStaticWords];
+
+namespace gi {
+namespace {
+
+__global__ void set_one_kernel_static(int32_t *q, int32_t *count, int32_t v) {
+    q[0] = v;
+    *count = 1;
+}
+
+// Device address of this GPU's `scratch` (the array's name is a host-side
+// stand-in): looked up once per device, with that device current.
+int static_scratch(int32_t **out) {
+    static std::mutex mu;
+    static std::map<int, int32_t *> by_device;
+    int dev = 0;
+    GI_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = by_device.find(dev);
+    if (it == by_device.end()) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(GI_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        it = by_device.emplace(dev, static_cast<int32_t *>(p)).first;
+    }
+    *out = it->second;
+    return GI_OK;
+}
+
+}  // namespace
+}  // namespace gi
+
+extern "C" {
+
+int gi_bfs_static(gi_graph *g, int32_t source, const gi_schedule *sched, int32_t *d_depth,
+                  int32_t *d_parent, void *stream) {
+    if (!g || !d_depth) return fail(GI_ERR_INVALID, "null argument");
+    if (!g->d_off) return fail(GI_ERR_UNSUPPORTED, "pass the base graph, not a blocked view");
+    if (g->V > GI_BFS_STATIC_MAX_VERTICES)
+        return fail(GI_ERR_RANGE, "gi_bfs_static handles up to %d vertices (graph has %d)",
+                    GI_BFS_STATIC_MAX_VERTICES, g->V);
+    if (source < 0 || source >= g->V) return fail(GI_ERR_RANGE, "source %d out of range [0,%d)", source, g->V);
+    gi_schedule s1;
+    if (sched) {
+        s1 = *sched;
+    } else {  // the app's schedule, apps/bfs.cpp:86-88
+        gi_schedule_init(&s1);
+        s1.direction = GI_PUSH;
+        s1.load_balancing = GI_LB_TWCE;
+    }
+    GI_TRY(validate_schedule(&s1));
+    // push steps whose load balancing needs no per-graph memory: TWC and
+    // STRICT bin into graph-owned scratch, EDGE_ONLY builds a COO array
+    if (s1.direction != GI_PUSH || s1.kernel_fusion == GI_FUSION_ENABLED ||
+        s1.frontier_creation != GI_FC_SPARSE ||
+        (s1.load_balancing != GI_LB_VERTEX_BASED && s1.load_balancing != GI_LB_TWCE &&
+         s1.load_balancing != GI_LB_CM && s1.load_balancing != GI_LB_WM))
+        return fail(GI_ERR_UNSUPPORTED,
+                    "gi_bfs_static runs unfused push steps with a sparse frontier and VERTEX_BASED, WM, "
+                    "TWCE or CM load balancing");
+    s1.deduplication = GI_DEDUP_DISABLED;  // the visited bitmap already admits each vertex once
+    hipStream_t s = (hipStream_t)stream;
+    int32_t *base = nullptr;
+    GI_TRY(static_scratch(&base));
+    // a frontier over the fixed array, built on the host stack
+    gi_frontier f;
+    f.V = g->V;
+    f.cap = g->V;
+    f.fmt = GI_FMT_SPARSE;
+    f.side = 0;
+    f.counts = base;
+    f.q[0] = base + kStaticCounts;
+    f.q[1] = base + kStaticCounts + kStaticQueue;
+    f.words = bitmap_words(g->V);
+    uint64_t *visited = reinterpret_cast<uint64_t *>(base + kStaticCounts + 2 * kStaticQueue);
+    // the array keeps the previous call's contents: clear what starts at zero
+    GI_TRY(fill_i32(base, kStaticCounts, 0, s));
+    GI_TRY(fill_u64(visited, f.words, 0, s));
+    GI_TRY(fill_i32(d_depth, g->V, -1, s));
+    if (d_parent) GI_TRY(fill_i32(d_parent, g->V, -1, s));
+    bfs_seed_kernel<<<1, 1, 0, s>>>(d_depth, d_parent, visited, source);
+    set_one_kernel_static<<<1, 1, 0, s>>>(f.q[0], f.counts, source);
+    GI_LAUNCH_CHECK();
+    int32_t n = 1, level = 0;
+    gi_op_args dummy{};
+    while (n > 0) {
+        OpBfsVisit op{visited, d_depth, d_parent, level + 1};
+        GI_TRY(edgeset_apply(g, &s1, &f, &f, GI_OP_BFS_DEPTH, dummy, s, &op, nullptr));
+        GI_TRY(read_count(f.count_in(), &n, s));
+        ++level;
+    }
+    int32_t ovf = 0;
+    GI_TRY(read_count(f.overflow(), &ovf, s));
+    if (ovf) return fail(GI_ERR_RANGE, "frontier overflow");
+    return GI_OK;
+}
+
+}  // extern "C"

@@ -348,6 +348,21 @@ typedef struct gi_run_stats {
  * the reference UDF). h may be NULL: push TWCE (apps/bfs.cpp:86-88). */
 int gi_bfs(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int32_t *d_depth,
            int32_t *d_parent, gi_run_stats *stats, void *stream);
+/* BFS that allocates nothing. No counterpart in the reference: it is for
+ * callers that cannot allocate device or host memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It runs
+ * gi_bfs's unfused push loop (s: push, sparse frontier creation, load
+ * balancing VERTEX_BASED, WM, TWCE or CM; NULL = push TWCE) with the
+ * frontier queues, counters and visited bitmap in a fixed device array of
+ * the library (about 17 MB per GPU, held until the process ends), so the
+ * graph may have at most GI_BFS_STATIC_MAX_VERTICES vertices (larger graphs
+ * and other schedules return an error). The caller passes d_depth (and
+ * optionally d_parent); results are gi_bfs's. Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU race. Synchronises once
+ * per level (a 4-byte read of the frontier size). */
+#define GI_BFS_STATIC_MAX_VERTICES (1 << 21)
+int gi_bfs_static(gi_graph *g, int32_t source, const gi_schedule *s, int32_t *d_depth,
+                  int32_t *d_parent, void *stream);
 /* BFS with a record per level: direction and load balancing used, frontier
  * sizes, the traversal kernel's device time (hipEvents around that launch)
  * and the step's algorithmic HBM bytes:
