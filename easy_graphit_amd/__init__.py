@@ -195,6 +195,7 @@ def lib():
         "gi_frontier_list_levels": (C.c_int, [vp, C.POINTER(i32)]),
         "gi_frontier_list_destroy": (C.c_int, [vp]),
         "gi_bfs": (C.c_int, [vp, i32, C.POINTER(HybridSchedule), vp, vp, C.POINTER(RunStats), vp]),
+        "gi_bfs_batched": (C.c_int, [vp, vp, i32, vp, C.POINTER(RunStats), vp]),
         "gi_bfs_static": (C.c_int, [vp, i32, C.POINTER(Schedule), vp, vp, vp]),
         "gi_bfs_profile": (C.c_int, [vp, i32, C.POINTER(HybridSchedule), vp,
                                      C.POINTER(LevelRecord), i32, C.POINTER(i32), vp]),
@@ -613,6 +614,22 @@ def bfs_device(g: Graph, source: int, schedule, depth: DeviceArray, parent=None,
                        depth.ptr, parent.ptr if parent else None,
                        C.byref(st) if st is not None else None, stream))
     return st.as_dict() if st is not None else None
+
+
+def bfs_batched(g: Graph, sources, depth: DeviceArray | None = None, stream=None):
+    """Per-source depths [len(sources), V] (gi_bfs_batched: 64 sources per
+    pass). With `depth` (a device array of len(sources) * V int32) the result
+    stays on the device and only the statistics are returned."""
+    src = np.ascontiguousarray(sources, np.int32)
+    n, V = len(src), g.num_vertices
+    own = depth is None
+    if own:
+        depth = DeviceArray(max(n, 1) * V, np.int32)
+    st = RunStats()
+    check(lib().gi_bfs_batched(g.h, src.ctypes.data, n, depth.ptr, C.byref(st), stream))
+    if own:
+        return depth.numpy(stream)[: n * V].reshape(n, V), st.as_dict()
+    return None, st.as_dict()
 
 
 BFS_STATIC_MAX_VERTICES = 1 << 21  # GI_BFS_STATIC_MAX_VERTICES

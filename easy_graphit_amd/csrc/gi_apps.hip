@@ -16,24 +16,6 @@ static int ensure_work(gi_graph *g) {
     return GI_OK;
 }
 
-struct EventTimer {
-    hipEvent_t a = nullptr, b = nullptr;
-    hipStream_t s;
-    explicit EventTimer(hipStream_t st) : s(st) {
-        if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) (void)hipEventRecord(a, s);
-    }
-    double stop() {
-        float ms = 0.f;
-        if (a && b && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess)
-            (void)hipEventElapsedTime(&ms, a, b);
-        return ms;
-    }
-    ~EventTimer() {
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
-    }
-};
-
 __global__ void bfs_seed_kernel(int32_t *depth, int32_t *parent, uint64_t *visited, int32_t src) {
     depth[src] = 0;
     if (parent) parent[src] = src;

@@ -89,6 +89,25 @@ struct Launcher {
     }
 };
 
+// Device time between construction and stop() on one stream (HIP events).
+struct EventTimer {
+    hipEvent_t a = nullptr, b = nullptr;
+    hipStream_t s;
+    explicit EventTimer(hipStream_t st) : s(st) {
+        if (hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) (void)hipEventRecord(a, s);
+    }
+    double stop() {
+        float ms = 0.f;
+        if (a && b && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess)
+            (void)hipEventElapsedTime(&ms, a, b);
+        return ms;
+    }
+    ~EventTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
 // Instantiates the graph of the launches `record(Launcher &)` makes.
 template <class F>
 int build_graph(F &&record, hipGraphExec_t *out) {

@@ -28,7 +28,9 @@ MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "selector_
 N_OUT = gi.BFS_MODES
 ALL_MODES = (1 << N_OUT) - 1
 
-# Calibration graphs: (kind, args). None of them is the RMAT-24 test graph.
+# Calibration graphs: (kind, args). None of them is a test graph: not the
+# RMAT-24 benchmark graph (seed 27491095) nor anything of its scale, and not
+# HELD_OUT below.
 CALIBRATION = [
     ("rmat", dict(scale=16, edge_factor=16, seed=11)),
     ("rmat", dict(scale=18, edge_factor=16, seed=12)),
@@ -39,11 +41,20 @@ CALIBRATION = [
     ("rmat", dict(scale=22, edge_factor=16, seed=20)),
     ("rmat", dict(scale=22, edge_factor=8, seed=21)),
     ("rmat", dict(scale=23, edge_factor=16, seed=24)),
-    ("rmat", dict(scale=24, edge_factor=16, seed=25)),
     ("uniform", dict(num_vertices=1 << 20, num_pairs=8 << 20, seed=17)),
     ("uniform", dict(num_vertices=1 << 18, num_pairs=4 << 18, seed=18)),
     ("grid", dict(width=512, num_random=2600, seed=19)),
 ]
+
+# The graph the gain test holds out of calibration (and of the rule's tuning).
+HELD_OUT = ("rmat", dict(scale=22, edge_factor=24, seed=31))
+
+# Fused step modes and the schedule that selects each (fused_mode_of).
+MODE_SCHED = {0: (gi.PUSH, gi.LB_VERTEX_BASED), 1: (gi.PUSH, gi.LB_WM), 2: (gi.PUSH, gi.LB_TWCE),
+              3: (gi.PUSH, gi.LB_CM), 4: (gi.PUSH, gi.LB_TWC), 5: (gi.PULL, gi.LB_VERTEX_BASED),
+              6: (gi.PULL, gi.LB_TWCE)}
+PUSH_MODES = (0, 1, 2, 3, 4)
+PULL_MODES = (5, 6)
 
 
 # Vertex layout the engine runs on; the selector is trained and judged on it.
@@ -136,6 +147,8 @@ def predict(model, feats):
 
 def save(model, path: str = MODEL_PATH):
     extra = {"threshold": np.float32(model["threshold"])} if "threshold" in model else {}
+    if "rule" in model:
+        extra["rule"] = np.array(model["rule"], np.float64)
     np.savez(path, params=model["params"], mean=model["mean"], scale=model["scale"],
              loss=model["loss"], n_out=np.int32(model["n_out"]),
              allowed=np.int32(model["allowed"]), **extra)
@@ -147,12 +160,22 @@ def load(path: str = MODEL_PATH):
              "loss": z["loss"], "n_out": int(z["n_out"]), "allowed": int(z["allowed"])}
         if "threshold" in z.files:
             m["threshold"] = float(z["threshold"])
+        if "rule" in z.files:  # the strongest fixed rule: push mode, pull mode, t
+            r = z["rule"]
+            m["rule"] = (int(r[0]), int(r[1]), float(r[2]))
         return m
 
 
 def install(model):
     gi.selector_load(model["params"], model["n_out"], model["allowed"], model["mean"],
                      model["scale"])
+
+
+def rule_schedule(push_mode: int, pull_mode: int, threshold: float,
+                  selector: int = gi.SELECT_HEURISTIC):
+    """GraphIt's hybrid rule (|F| <= t*V: push, else pull) over two fused
+    step modes."""
+    return fused_schedule(MODE_SCHED[push_mode][1], MODE_SCHED[pull_mode][1], threshold, selector)
 
 
 def fused_schedule(push_mode_lb: int = gi.LB_TWCE, pull_lb: int = gi.LB_VERTEX_BASED,
@@ -194,6 +217,78 @@ def tune_threshold(specs=CALIBRATION, grid=THRESHOLDS, sources_per_graph: int = 
     best = min(total, key=total.get)
     log(f"threshold sweep: {total} -> {best}")
     return best, total
+
+
+def tune_rule(specs=CALIBRATION, grid=THRESHOLDS, push_modes=PUSH_MODES, pull_modes=PULL_MODES,
+              sources_per_graph: int = 4, log=print):
+    """The strongest fixed rule: GraphIt's |F| <= t*V rule over every pair
+    of a push and a pull step mode and every threshold of `grid`, the triple
+    with the least total time over the calibration graphs.
+    Returns ((push_mode, pull_mode, t), {triple: seconds})."""
+    total = {}
+    for kind, args in specs:
+        g = build_graph(kind, args)
+        srcs = g.pick_sources(args["seed"] * 7 + 1, sources_per_graph)
+        depth = gi.DeviceArray(g.num_vertices, np.int32)
+        for pm in push_modes:
+            for qm in pull_modes:
+                for t in grid:
+                    key = (pm, qm, t)
+                    total[key] = total.get(key, 0.0) + time_policy(g, rule_schedule(pm, qm, t), srcs, depth)
+        g.destroy()
+        log(f"rule sweep: {kind} {args} done")
+    best = min(total, key=total.get)
+    log(f"strongest fixed rule on the calibration graphs: push mode {best[0]}, pull mode {best[1]}, "
+        f"t={best[2]} ({total[best]:.4f} s)")
+    return best, total
+
+
+def compare(model, g: gi.Graph, srcs, rule, app_threshold: float, grid=THRESHOLDS, reps: int = 3,
+            test_tuned: bool = True, log=print):
+    """Learned selector vs fixed rules on graph g (the caller's layout):
+    the calibration-tuned strongest rule `rule` = (push mode, pull mode, t),
+    the reference app's rule (PUSH TWCE / PULL VERTEX_BASED,
+    apps/bfs.cpp:69-80) at `app_threshold`, and, when test_tuned, the best
+    rule of every mode pair and threshold on g itself (tuned on the test
+    graph: an upper bound on what any fixed rule reaches). Depths of every
+    policy are checked equal to the learned run's before timing."""
+    depth = gi.DeviceArray(g.num_vertices, np.int32)
+    install(model)
+    learned = fused_schedule(threshold=app_threshold, selector=gi.SELECT_LEARNED)
+    strongest = rule_schedule(*rule)
+    app = rule_schedule(2, 5, app_threshold)
+    for s in srcs:
+        d_l, _, _ = gi.bfs(g, int(s), learned)
+        for other in (strongest, app):
+            d_o, _, _ = gi.bfs(g, int(s), other)
+            if not np.array_equal(d_o, d_l):
+                raise AssertionError(f"schedules disagree from source {s}")
+    times = {"learned": [], "strongest_rule": [], "app_rule": []}
+    for _ in range(reps):  # interleaved, so drift on the device hits all alike
+        times["learned"].append(time_policy(g, learned, srcs, depth))
+        times["strongest_rule"].append(time_policy(g, strongest, srcs, depth))
+        times["app_rule"].append(time_policy(g, app, srcs, depth))
+    t = {k: min(v) for k, v in times.items()}
+    res = {"learned_s": t["learned"], "strongest_rule_s": t["strongest_rule"], "app_rule_s": t["app_rule"],
+           "speedup_vs_strongest_rule": t["strongest_rule"] / t["learned"],
+           "speedup_vs_app_rule": t["app_rule"] / t["learned"],
+           "strongest_rule": {"push_mode": gi.BFS_MODE_NAMES[rule[0]], "pull_mode": gi.BFS_MODE_NAMES[rule[1]],
+                              "threshold": rule[2]}}
+    if test_tuned:
+        best, best_t = None, float("inf")
+        for pm in PUSH_MODES:
+            for qm in PULL_MODES:
+                for th in grid:
+                    tt = min(time_policy(g, rule_schedule(pm, qm, th), srcs, depth) for _ in range(2))
+                    if tt < best_t:
+                        best, best_t = (pm, qm, th), tt
+                log(f"  test-tuned sweep: push {pm} pull {qm} done (best so far {best} {best_t:.5f} s)")
+        res["test_tuned_rule"] = {"push_mode": gi.BFS_MODE_NAMES[best[0]],
+                                  "pull_mode": gi.BFS_MODE_NAMES[best[1]], "threshold": best[2],
+                                  "seconds": best_t}
+        res["speedup_vs_test_tuned_rule"] = best_t / t["learned"]
+    log(f"compare: {res}")
+    return res
 
 
 def evaluate(model, threshold: float, scale: int = 24, edge_factor: int = 16,
@@ -250,7 +345,7 @@ def main(argv=None):
     import json
     import sys
     ap = argparse.ArgumentParser(description="train / evaluate the BFS schedule selector")
-    ap.add_argument("cmd", choices=["train", "eval"])
+    ap.add_argument("cmd", choices=["train", "eval", "compare"])
     ap.add_argument("--sources", type=int, default=4)
     ap.add_argument("--epochs", type=int, default=3000)
     ap.add_argument("--scale", type=int, default=24)
@@ -268,6 +363,8 @@ def main(argv=None):
         model = fit(feats, times, epochs=args.epochs)
         t_best, sweep = tune_threshold(log=log)
         model["threshold"] = t_best
+        rule, _ = tune_rule(log=log)
+        model["rule"] = rule
         save(model, args.out)
         # offline choice quality on the training set
         pred = predict(model, feats)
@@ -279,6 +376,16 @@ def main(argv=None):
                           "first_loss": float(model["loss"][0]), "threshold": t_best,
                           "threshold_sweep": {str(k): v for k, v in sweep.items()},
                           "train_regret": regret}))
+    elif args.cmd == "compare":  # held-out graph and the benchmark graph
+        model = load(args.out)
+        out = {}
+        for name, (kind, gargs) in (("held_out", HELD_OUT),
+                                     ("rmat24", ("rmat", dict(scale=24, edge_factor=16, seed=27491095)))):
+            g = build_graph(kind, gargs)
+            srcs = g.pick_sources(gargs["seed"], 16)
+            out[name] = compare(model, g, srcs, model["rule"], model.get("threshold", 0.0005), log=log)
+            g.destroy()
+        print(json.dumps(out))
     else:
         model = load(args.out)
         print(json.dumps(evaluate(model, model.get("threshold", 0.0005), scale=args.scale, log=log)))

@@ -348,6 +348,15 @@ typedef struct gi_run_stats {
  * the reference UDF). h may be NULL: push TWCE (apps/bfs.cpp:86-88). */
 int gi_bfs(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int32_t *d_depth,
            int32_t *d_parent, gi_run_stats *stats, void *stream);
+/* Per-source BFS depths for many sources at once: d_depth is
+ * [num_sources][V] (source-major; -1 where a source does not reach v). The
+ * BFS app (apps/bfs.cpp:44-60) runs one source per call; here groups of up
+ * to 64 sources advance together, one 64-bit mask per vertex, with
+ * direction-optimizing push/pull levels recorded as hipGraph blocks. A
+ * directed graph pulls over its transpose. stats: iterations = levels summed
+ * over the groups, ms = device time. Synchronises. */
+int gi_bfs_batched(gi_graph *g, const int32_t *sources, int32_t num_sources, int32_t *d_depth,
+                   gi_run_stats *stats, void *stream);
 /* BFS that allocates nothing. No counterpart in the reference: it is for
  * callers that cannot allocate device or host memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc). It runs
