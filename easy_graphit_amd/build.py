@@ -64,6 +64,9 @@ def build(verbose: bool = False) -> str:
             raise RuntimeError(f"app build failed for {app}:\n{r.stderr}")
     if verbose:
         print(f"built {LIB} from {len(sources)} sources")
+    # the reference's apps through the staged front end (when its tree is here)
+    from easy_graphit_amd.frontend import build_apps
+    build_apps.build(verbose=verbose)
     return LIB
 
 

@@ -627,4 +627,12 @@ int run_bfs_family(int32_t op, const gi_op_args &a, const ApplyPlan &plan);
 int run_other_family(int32_t op, const gi_op_args &a, const ApplyPlan &plan);
 int run_bfs_visit(const OpBfsVisit &op, const ApplyPlan &plan);
 
+// The dispatcher's two halves (gi_ops.hip), for callers that bring their own
+// operator functor: resolve graph view, enumerator, input representation and
+// output sink (may_duplicate: the op can emit a vertex twice, so a sparse
+// output is deduplicated), then, after run_plan(op, plan), swap the output.
+int edgeset_plan(gi_graph *g, const gi_schedule *s, gi_frontier *from, gi_frontier *to, bool may_duplicate,
+                 hipStream_t st, hipEvent_t *events, ApplyPlan *out, int32_t *out_fmt);
+int edgeset_finish(gi_frontier *to, int32_t fmt, hipStream_t st);
+
 }  // namespace gi

@@ -32,9 +32,8 @@ int plan_grid(const gi_schedule *s) {
     return s->max_cta > 0 ? s->max_cta : persistent_grid(s->cta_size);
 }
 
-int edgeset_apply(gi_graph *g, const gi_schedule *s, gi_frontier *from, gi_frontier *to,
-                  int32_t op, const gi_op_args &a, hipStream_t st, const OpBfsVisit *visit,
-                  hipEvent_t *events) {
+int edgeset_plan(gi_graph *g, const gi_schedule *s, gi_frontier *from, gi_frontier *to, bool may_duplicate,
+                 hipStream_t st, hipEvent_t *events, ApplyPlan *out, int32_t *out_fmt) {
     GI_TRY(validate_schedule(s));
     if (from && from->V != g->V) return fail(GI_ERR_INVALID, "frontier size does not match graph");
     if (to && to->V != g->V) return fail(GI_ERR_INVALID, "output frontier size does not match graph");
@@ -84,17 +83,32 @@ int edgeset_apply(gi_graph *g, const gi_schedule *s, gi_frontier *from, gi_front
     if (to) {
         fmt = s->frontier_creation;
         GI_TRY(frontier_clear_out(to, fmt, st));
-        plan.out = make_sink(to, fmt, op_may_duplicate(op) || s->deduplication == GI_DEDUP_ENABLED);
+        plan.out = make_sink(to, fmt, may_duplicate || s->deduplication == GI_DEDUP_ENABLED);
     } else {
         plan.out = make_sink(nullptr, 0, false);
     }
+    *out = plan;
+    *out_fmt = fmt;
+    return GI_OK;
+}
+
+int edgeset_finish(gi_frontier *to, int32_t fmt, hipStream_t st) {
+    if (to) GI_TRY(gi_frontier_swap(to, fmt, st));
+    return GI_OK;
+}
+
+int edgeset_apply(gi_graph *g, const gi_schedule *s, gi_frontier *from, gi_frontier *to,
+                  int32_t op, const gi_op_args &a, hipStream_t st, const OpBfsVisit *visit,
+                  hipEvent_t *events) {
+    ApplyPlan plan{};
+    int32_t fmt = GI_FMT_SPARSE;
+    GI_TRY(edgeset_plan(g, s, from, to, op_may_duplicate(op), st, events, &plan, &fmt));
     int rc;
     if (visit) rc = run_bfs_visit(*visit, plan);
     else if (op == GI_OP_BFS_PARENT || op == GI_OP_BFS_DEPTH) rc = run_bfs_family(op, a, plan);
     else rc = run_other_family(op, a, plan);
     if (rc != GI_OK) return rc;
-    if (to) GI_TRY(gi_frontier_swap(to, fmt, st));
-    return GI_OK;
+    return edgeset_finish(to, fmt, st);
 }
 
 }  // namespace gi
