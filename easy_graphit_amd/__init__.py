@@ -196,6 +196,7 @@ def lib():
         "gi_frontier_list_destroy": (C.c_int, [vp]),
         "gi_bfs": (C.c_int, [vp, i32, C.POINTER(HybridSchedule), vp, vp, C.POINTER(RunStats), vp]),
         "gi_bfs_batched": (C.c_int, [vp, vp, i32, vp, C.POINTER(RunStats), vp]),
+        "gi_sssp_batched": (C.c_int, [vp, vp, i32, i32, i32, vp, C.POINTER(RunStats), vp]),
         "gi_bfs_static": (C.c_int, [vp, i32, C.POINTER(Schedule), vp, vp, vp]),
         "gi_bfs_profile": (C.c_int, [vp, i32, C.POINTER(HybridSchedule), vp,
                                      C.POINTER(LevelRecord), i32, C.POINTER(i32), vp]),
@@ -629,6 +630,23 @@ def bfs_batched(g: Graph, sources, depth: DeviceArray | None = None, stream=None
     check(lib().gi_bfs_batched(g.h, src.ctypes.data, n, depth.ptr, C.byref(st), stream))
     if own:
         return depth.numpy(stream)[: n * V].reshape(n, V), st.as_dict()
+    return None, st.as_dict()
+
+
+def sssp_batched(g: Graph, sources, delta: int, batch: int = 0, dist: DeviceArray | None = None,
+                 stream=None):
+    """Per-source SSSP distances [len(sources), V] (gi_sssp_batched). With
+    `dist` (len(sources) * V int32 on the device) only the statistics are
+    returned."""
+    src = np.ascontiguousarray(sources, np.int32)
+    n, V = len(src), g.num_vertices
+    own = dist is None
+    if own:
+        dist = DeviceArray(max(n, 1) * V, np.int32)
+    st = RunStats()
+    check(lib().gi_sssp_batched(g.h, src.ctypes.data, n, delta, batch, dist.ptr, C.byref(st), stream))
+    if own:
+        return dist.numpy(stream)[: n * V].reshape(n, V), st.as_dict()
     return None, st.as_dict()
 
 

@@ -357,6 +357,15 @@ int gi_bfs(gi_graph *g, int32_t source, const gi_hybrid_schedule *h, int32_t *d_
  * over the groups, ms = device time. Synchronises. */
 int gi_bfs_batched(gi_graph *g, const int32_t *sources, int32_t num_sources, int32_t *d_depth,
                    gi_run_stats *stats, void *stream);
+/* Delta-stepping SSSP from many sources at once: d_dist is
+ * [num_sources][V] (source-major; INT32_MAX where unreached). The SSSP app
+ * (apps/sssp.cpp:34-57, gpu_priority_queue.h:115-185) runs one source per
+ * call; here batches of up to `batch` sources (<= 256; 0 = 256) share one
+ * set of launches per step, each source with its own window of width
+ * delta and pending set. Distances equal gi_sssp's. stats: iterations = relax
+ * steps summed over the sources, ms = device time. Synchronises. */
+int gi_sssp_batched(gi_graph *g, const int32_t *sources, int32_t num_sources, int32_t delta, int32_t batch,
+                    int32_t *d_dist, gi_run_stats *stats, void *stream);
 /* BFS that allocates nothing. No counterpart in the reference: it is for
  * callers that cannot allocate device or host memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc). It runs

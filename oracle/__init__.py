@@ -64,6 +64,9 @@ def lib():
             "or_sssp_dijkstra": (None, [i32, i32p, i32p, i32p, i32, i32p]),
             "or_sssp_delta_stepping": (i64, [i32, i32p, i32p, i32p, i32, i32, i32p]),
             "or_cc_labels": (None, [i32, i32p, i32p, i32p]),
+            "or_sssp_delta_stepping_omp": (i64, [i32, i32p, i32p, i32p, i32, i32, i32, i32p,
+                                                 C.POINTER(C.c_int32)]),
+            "or_cc_omp": (i32, [i32, i32p, i32p, i32, i32p, C.POINTER(C.c_int32)]),
             "or_cc_label_propagation": (i32, [i32, i32p, i32p, i32p]),
             "or_set_threads": (None, [i32]),
             "or_max_threads": (i32, []),
@@ -288,6 +291,23 @@ def sssp_delta_stepping(g: CSR, source: int, delta: int):
     nb = lib().or_sssp_delta_stepping(g.num_vertices, g.offsets, g.nbrs, g.weights,
                                       source, delta, d)
     return d, nb
+
+
+def sssp_omp(g: CSR, source: int, delta: int, threads: int = 0):
+    """OpenMP delta-stepping (CPU baseline): (dist, threads used)."""
+    out = np.empty(g.num_vertices, np.int32)
+    used = C.c_int32()
+    lib().or_sssp_delta_stepping_omp(g.num_vertices, g.offsets, g.nbrs, g.weights, source, delta, threads, out,
+                                     C.byref(used))
+    return out, used.value
+
+
+def cc_omp(g: CSR, threads: int = 0):
+    """OpenMP label propagation with shortcutting (CPU baseline): (labels, threads used)."""
+    out = np.empty(g.num_vertices, np.int32)
+    used = C.c_int32()
+    lib().or_cc_omp(g.num_vertices, g.offsets, g.nbrs, threads, out, C.byref(used))
+    return out, used.value
 
 
 def cc(g: CSR):

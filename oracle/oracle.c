@@ -25,6 +25,8 @@
  *   or_sssp_dijkstra               scipy.sparse.csgraph.dijkstra; or_sssp_delta_stepping.
  *   or_cc_labels                   scipy connected_components (mapped to min id);
  *                                  or_cc_label_propagation (apps/cc.cpp restated).
+ *   or_cc_omp, or_sssp_delta_stepping_omp  (CPU baselines; OpenMP) equal to or_cc_labels /
+ *                                  or_sssp_dijkstra on every test graph at 1, 2 and 8 threads.
  *   or_pagerank                    closed forms (directed cycle: uniform 1/N; star);
  *                                  networkx.pagerank on dangling-free graphs at
  *                                  convergence; dense-matrix power iteration; the
@@ -628,6 +630,148 @@ int64_t or_sssp_delta_stepping(int32_t num_vertices, const int32_t *offsets,
     for (int64_t b = 0; b < nb; ++b) free(bucket[b]);
     free(bucket); free(bsz); free(bcap);
     return processed;
+}
+
+/* ------------------------------------------------------------------------
+ * OpenMP delta-stepping (the CPU baseline of the SSSP benchmarks), after the
+ * structure of the reference's ordered_processing.h:46-142: the current
+ * bucket's vertices are relaxed in parallel, each thread appends improved
+ * vertices to thread-local bins by bucket index, and the next round takes
+ * the least non-empty bucket index over all threads' bins. writeMin is a
+ * compare-and-swap loop. Distances equal or_sssp_dijkstra's for every
+ * thread count. Returns the number of rounds.
+ * ---------------------------------------------------------------------- */
+typedef struct {
+    int32_t *items;
+    int64_t n, cap;
+} i32vec;
+
+static void i32vec_push(i32vec *v, int32_t x) {
+    if (v->n == v->cap) {
+        v->cap = v->cap ? 2 * v->cap : 64;
+        v->items = (int32_t *)realloc(v->items, (size_t)v->cap * sizeof(int32_t));
+    }
+    v->items[v->n++] = x;
+}
+
+int64_t or_sssp_delta_stepping_omp(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,
+                                   const int32_t *w, int32_t source, int32_t delta, int32_t nthreads,
+                                   int32_t *dist, int32_t *threads_used) {
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+    if (threads_used) *threads_used = nt;
+    for (int32_t v = 0; v < num_vertices; ++v) dist[v] = INT_MAX;
+    dist[source] = 0;
+    /* a pending distance lies below (current bucket + 1) * delta + the largest
+     * weight, so nb rotating bins never hold two bucket indices at once */
+    int64_t maxw = 1;
+    const int64_t m = num_vertices ? offsets[num_vertices] : 0;
+    for (int64_t e = 0; e < m; ++e) maxw = w[e] > maxw ? w[e] : maxw;
+    const int64_t nb = maxw / delta + 2;
+    i32vec **bins = (i32vec **)calloc((size_t)nt, sizeof(i32vec *));
+    for (int t = 0; t < nt; ++t) bins[t] = (i32vec *)calloc((size_t)nb, sizeof(i32vec));
+    int64_t cap = 1024, fsize = 1;
+    int32_t *frontier = (int32_t *)malloc(sizeof(int32_t) * (size_t)cap);
+    frontier[0] = source;
+    int64_t cur = 0, rounds = 0;
+    while (fsize > 0) {
+        ++rounds;
+#pragma omp parallel num_threads(nt)
+        {
+            i32vec *mine = bins[omp_get_thread_num()];
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t i = 0; i < fsize; ++i) {
+                const int32_t u = frontier[i];
+                const int32_t du = __atomic_load_n(&dist[u], __ATOMIC_RELAXED);
+                if ((int64_t)du / delta != cur) continue; /* stale: it was improved into an earlier round */
+                for (int32_t e = offsets[u]; e < offsets[u + 1]; ++e) {
+                    const int32_t v = nbrs[e];
+                    const int64_t nd64 = (int64_t)du + w[e];
+                    if (nd64 >= INT_MAX) continue;
+                    const int32_t nd = (int32_t)nd64;
+                    int32_t old = __atomic_load_n(&dist[v], __ATOMIC_RELAXED);
+                    while (nd < old) {
+                        if (__atomic_compare_exchange_n(&dist[v], &old, nd, 0, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {
+                            i32vec_push(&mine[(nd / delta) % nb], v);
+                            break;
+                        }
+                    }
+                }
+            }
+        }
+        /* the next bucket: the current one when it was refilled, else the least non-empty */
+        int64_t next = -1;
+        for (int64_t b = cur; b < cur + nb && next < 0; ++b)
+            for (int t = 0; t < nt; ++t)
+                if (bins[t][b % nb].n) {
+                    next = b;
+                    break;
+                }
+        if (next < 0) break;
+        cur = next;
+        int64_t total = 0;
+        for (int t = 0; t < nt; ++t) total += bins[t][cur % nb].n;
+        if (total > cap) {
+            cap = total;
+            frontier = (int32_t *)realloc(frontier, sizeof(int32_t) * (size_t)cap);
+        }
+        fsize = 0;
+        for (int t = 0; t < nt; ++t) {
+            i32vec *bv = &bins[t][cur % nb];
+            memcpy(frontier + fsize, bv->items, (size_t)bv->n * sizeof(int32_t));
+            fsize += bv->n;
+            bv->n = 0;
+        }
+    }
+    for (int t = 0; t < nt; ++t) {
+        for (int64_t b = 0; b < nb; ++b) free(bins[t][b].items);
+        free(bins[t]);
+    }
+    free(bins);
+    free(frontier);
+    return rounds;
+}
+
+/* OpenMP label propagation with shortcutting (the CPU baseline of the CC
+ * benchmark): every vertex takes the least label among itself and its
+ * neighbours, then labels jump to their label's label, until nothing
+ * changes; the fixpoint is the least vertex id of each component, as
+ * or_cc_labels. Returns the rounds. */
+int32_t or_cc_omp(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs, int32_t nthreads,
+                  int32_t *labels, int32_t *threads_used) {
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+    if (threads_used) *threads_used = nt;
+#pragma omp parallel for num_threads(nt) schedule(static)
+    for (int32_t v = 0; v < num_vertices; ++v) labels[v] = v;
+    int32_t rounds = 0;
+    int changed = 1;
+    while (changed) {
+        changed = 0;
+        ++rounds;
+#pragma omp parallel for num_threads(nt) schedule(dynamic, 1024) reduction(| : changed)
+        for (int32_t v = 0; v < num_vertices; ++v) {
+            int32_t m = __atomic_load_n(&labels[v], __ATOMIC_RELAXED);
+            const int32_t m0 = m;
+            for (int32_t e = offsets[v]; e < offsets[v + 1]; ++e) {
+                const int32_t l = __atomic_load_n(&labels[nbrs[e]], __ATOMIC_RELAXED);
+                if (l < m) m = l;
+            }
+            if (m < m0) {
+                __atomic_store_n(&labels[v], m, __ATOMIC_RELAXED);
+                changed = 1;
+            }
+        }
+#pragma omp parallel for num_threads(nt) schedule(static)
+        for (int32_t v = 0; v < num_vertices; ++v) {
+            int32_t l = __atomic_load_n(&labels[v], __ATOMIC_RELAXED);
+            int32_t ll = __atomic_load_n(&labels[l], __ATOMIC_RELAXED);
+            while (ll < l) {
+                l = ll;
+                ll = __atomic_load_n(&labels[l], __ATOMIC_RELAXED);
+            }
+            __atomic_store_n(&labels[v], l, __ATOMIC_RELAXED);
+        }
+    }
+    return rounds;
 }
 
 /* ------------------------------------------------------------------------

@@ -61,6 +61,11 @@ void or_sssp_dijkstra(int32_t num_vertices, const int32_t *offsets, const int32_
 int64_t or_sssp_delta_stepping(int32_t num_vertices, const int32_t *offsets,
                                const int32_t *nbrs, const int32_t *w, int32_t source,
                                int32_t delta, int32_t *dist);
+int64_t or_sssp_delta_stepping_omp(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,
+                                   const int32_t *w, int32_t source, int32_t delta, int32_t nthreads,
+                                   int32_t *dist, int32_t *threads_used);
+int32_t or_cc_omp(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs, int32_t nthreads,
+                  int32_t *labels, int32_t *threads_used);
 void or_cc_labels(int32_t num_vertices, const int32_t *offsets, const int32_t *nbrs,
                   int32_t *labels);
 int32_t or_cc_label_propagation(int32_t num_vertices, const int32_t *offsets,

@@ -85,3 +85,52 @@ def test_bfs_batched_rmat20_256_sources(gi):
     assert (d[np.arange(256), srcs] == 0).all()
     for i in range(256):
         assert np.array_equal(d[i] >= 0, lab == lab[srcs[i]]), i
+
+
+@pytest.mark.parametrize("batch", [0, 7, 64])
+def test_sssp_batched_rmat(gi, batch):
+    g = gi.Graph.rmat(13, 16, SEED, True).set_weights(SEED, 1, 255)
+    host = host_csr(g, True)
+    srcs = O.pick_sources(host, 5, 70)
+    srcs[9] = srcs[2]
+    for delta in (16, 200):
+        dist, st = gi.sssp_batched(g, srcs, delta, batch)
+        for i, s in enumerate(srcs):
+            assert np.array_equal(dist[i], O.sssp(host, int(s))), (batch, delta, i)
+        assert st["iterations"] >= len(srcs)
+
+
+@pytest.mark.parametrize("graph", ["grid", "directed", "forest"])
+def test_sssp_batched_grid_directed_disconnected(gi, graph):
+    if graph == "grid":
+        g = gi.Graph.grid(96, 40, 3).set_weights(5, 1, 1000)
+    elif graph == "directed":
+        g = gi.Graph.rmat(12, 8, 4, False).set_weights(2, 1, 100)
+    else:
+        n = 20000
+        rng = np.random.default_rng(8)
+        a = rng.integers(0, n, 15000)
+        b = np.minimum(a + rng.integers(1, 4, 15000), n - 1)
+        e = np.stack([a, b]).astype(np.int32)
+        g = gi.Graph.from_edges(n, e[0], e[1], True).set_weights(3, 1, 50)
+    host = host_csr(g, True)
+    rng = np.random.default_rng(2)
+    srcs = rng.integers(0, g.num_vertices, 40).astype(np.int32)  # isolated sources too
+    dist, _ = gi.sssp_batched(g, srcs, 64)
+    for i, s in enumerate(srcs):
+        assert np.array_equal(dist[i], O.sssp(host, int(s))), (graph, i)
+
+
+def test_sssp_batched_matches_single_source_engine(gi):
+    # the batched distances equal the fused single-source engine's
+    g = gi.Graph.rmat(15, 16, SEED, True).set_weights(SEED, 1, 255)
+    srcs = g.pick_sources(SEED, 20)
+    dist, _ = gi.sssp_batched(g, srcs, 32)
+    fs = gi.Schedule(direction=gi.PUSH, load_balancing=gi.LB_TWC, kernel_fusion=1)
+    for i, s in enumerate(srcs):
+        d1, _ = gi.sssp(g, int(s), 32, fs)
+        assert np.array_equal(dist[i], d1), i
+    with pytest.raises(gi.GIError):
+        gi.sssp_batched(g, srcs, 0)
+    z, st = gi.sssp_batched(g, np.zeros(0, np.int32), 8)
+    assert z.shape == (0, g.num_vertices) and st["iterations"] == 0

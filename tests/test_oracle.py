@@ -481,3 +481,30 @@ def test_bc_thread_counts_agree():
     O.set_threads(prev)
     assert np.allclose(one, many, rtol=1e-12, atol=1e-12)
     assert np.allclose(one, O.bc(g, srcs[:1]) + O.bc(g, srcs[1:]), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("threads", [1, 2, 8])
+def test_openmp_sssp_and_cc_baselines_equal_the_serial_oracle(threads):
+    # symmetric RMAT, a directed graph, a grid and a weighted chain with long
+    # edges (pending distances far beyond the current bucket)
+    graphs = []
+    s, d = O.rmat_edges(12, 16, 7)
+    graphs.append(O.fill_weights(O.build_csr(4096, s, d, True), 3, 1, 255))
+    s, d = O.rmat_edges(11, 8, 8)
+    graphs.append(O.fill_weights(O.build_csr(2048, s, d, False), 4, 1, 1000))
+    s, d = O.grid_edges(40, 30, 5)
+    graphs.append(O.fill_weights(O.build_csr(1600, s, d, True), 5, 1, 1000))
+    n = 3000
+    a = np.arange(n - 1, dtype=np.int32)
+    g = O.build_csr(n, np.concatenate([a, [0, 5]]).astype(np.int32),
+                    np.concatenate([a + 1, [2999, 2000]]).astype(np.int32), True)
+    graphs.append(O.fill_weights(g, 6, 1, 30000))
+    for gi_, g in enumerate(graphs):
+        for src in O.pick_sources(g, 11, 2):
+            want = O.sssp(g, int(src))
+            for delta in (1, 16, 300):
+                got, used = O.sssp_omp(g, int(src), delta, threads)
+                assert used == threads and np.array_equal(got, want), (gi_, int(src), delta)
+        if gi_ != 1:  # components of the symmetric graphs
+            lab, _ = O.cc_omp(g, threads)
+            assert np.array_equal(lab, O.cc(g)), gi_
