@@ -11,8 +11,9 @@ Also reported on the same JSON line:
   roofline      the dominant traversal kernel's algorithmic bytes per launch
                 over its average duration (hipEvents around that launch),
                 against 8 TB/s HBM;
-  cpu_baseline  the oracle's serial BFS on the host, same graph and sources
-                (a bounded sample).
+  cpu_baseline  the oracle's OpenMP direction-optimizing BFS on the host's
+                cores, same graph and sources (a bounded sample); each app
+                section carries its own OpenMP oracle baseline.
 """
 from __future__ import annotations
 
@@ -124,7 +125,17 @@ def alg_frac(nbytes: float, seconds: float) -> float:
     return round(nbytes / seconds / (HBM_PEAK_GBS * 1e9), 4)
 
 
-def bench_apps(gi, g, sources, edges_of, log, only: str | None = None) -> dict:
+def cpu_leg(fn, edges: float, sample: str, unit_scale: float = 0.5):
+    """One OpenMP oracle run on this host's cores as the app's CPU baseline:
+    GTEPS over `edges` (edge entries; halved for undirected GTEPS)."""
+    t0 = time.perf_counter()
+    used = fn()
+    t = time.perf_counter() - t0
+    return {"value": round(edges * unit_scale / t / 1e9, 4), "unit": "GTEPS", "cores": used, "kind": "port",
+            "seconds": round(t, 3), "sample": sample}
+
+
+def bench_apps(gi, g, sources, edges_of, log, only: str | None = None, cpu_on: bool = True) -> dict:
     """The other apps on their configurations, each through its fused
     schedule, with GTEPS and the fraction of the HBM roofline of their
     algorithmic bytes (one definition per app, stated with each)."""
@@ -147,13 +158,41 @@ def bench_apps(gi, g, sources, edges_of, log, only: str | None = None) -> dict:
             edges += edges_of[s]
             vr += int((dist < np.iinfo(np.int32).max).sum())
             steps += st["iterations"]
+        cpu = None
+        if cpu_on:
+            import oracle
+            off, nb, wt = g.read_csr(weights=True)
+            host = oracle.CSR(off, nb, wt)
+            cpu = cpu_leg(lambda: oracle.sssp_omp(host, srcs[0], delta)[1], edges_of[srcs[0]],
+                          f"oracle OpenMP delta-stepping (delta {delta}) from the first of these sources")
+            del host, off, nb, wt
         out["sssp"] = {"gteps": round(edges / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3 / len(srcs), 3),
+                       "cpu_baseline": cpu,
                        "roofline_frac": alg_frac(16 * vr + 12 * edges, t), "relax_steps": steps,
                        "alg_bytes": "16 V_reached + 12 E_reached", "config": "RMAT-24 sym, w U[1,255], "
                        f"delta {delta} (fastest of {list(autotune.DELTAS)} summed over these sources), "
                        f"{len(srcs)} sources",
                        "delta_ms_summed": {r["delta"]: round(r["ms"], 3) for r in tuned["table"]}}
         log(f"sssp: {out['sssp']}")
+    if only == "sssp_trace":
+        # the SSSP slots launched one at a time with HIP events around each
+        # launch (gi_sssp_trace, no recorded graphs): the form a kernel-trace
+        # profile of the SSSP kernels is taken of
+        g.set_weights(SEED, 1, 255)
+        res = {}
+        for name, gg, src, delta in (("rmat24", g, int(sources[0]), 16),
+                                     ("grid2048", gi.Graph.grid(2048, 2 * 2048 * 2047 // 100, SEED)
+                                      .set_weights(SEED, 1, 1000), 0, 64)):
+            dist, ms, info = gi.sssp_trace(gg, src, delta, max_slots=8192)
+            d2, _ = gi.sssp(gg, src, delta, fs)
+            if not np.array_equal(dist, d2):
+                raise SystemExit("traced SSSP distances differ from the recorded engine's")
+            cols = ["push", "pull", "hub", "decide", "window_scan", "decide2"]
+            res[name] = {"slots": len(ms), "delta": delta,
+                         "kernel_ms_summed": dict(zip(cols, np.round(ms.sum(0), 3).tolist())),
+                         "heavy_steps": int((info[:, 0] == 2).sum())}
+            log(f"sssp_trace {name}: {res[name]}")
+        out["sssp_trace"] = res
     if only in (None, "sssp_grid"):
         # SSSP on the grid workload: 2048 x 2048 4-neighbour grid plus 1% random
         # long edges, w U[1,1000], source at the corner, delta 64
@@ -166,7 +205,15 @@ def bench_apps(gi, g, sources, edges_of, log, only: str | None = None) -> dict:
         t = float(np.median(ts))
         ge = gr.num_edges
         tuned_g = autotune.tune_delta(gr, 0, deltas=(64, 256, 1024), reps=1)
+        cpu = None
+        if cpu_on:
+            import oracle
+            off, nb, wt = gr.read_csr(weights=True)
+            host = oracle.CSR(off, nb, wt)
+            cpu = cpu_leg(lambda: oracle.sssp_omp(host, 0, 64)[1], ge,
+                          "oracle OpenMP delta-stepping, delta 64, from the corner")
         out["sssp_grid"] = {"gteps": round(ge / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3),
+                            "cpu_baseline": cpu,
                             "relax_steps": st["iterations"],
                             "roofline_frac": alg_frac(16 * gr.num_vertices + 12 * ge, t),
                             "config": "2048x2048 grid + 1% random edges, w U[1,1000], source 0, delta 64",
@@ -183,7 +230,15 @@ def bench_apps(gi, g, sources, edges_of, log, only: str | None = None) -> dict:
             ts.append(st["ms"] * 1e-3)
             rounds = st["iterations"]
         t = float(np.median(ts))
+        cpu = None
+        if cpu_on:
+            import oracle
+            off, nb, _ = g.read_csr()
+            host = oracle.CSR(off, nb)
+            cpu = cpu_leg(lambda: oracle.cc_omp(host)[1], E, "oracle OpenMP label propagation with shortcutting")
+            del host, off, nb
         out["cc"] = {"gteps": round(E / 2 / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3), "rounds": rounds,
+                     "cpu_baseline": cpu,
                      "roofline_frac": alg_frac(rounds * (8 * E + 12 * V) + rounds * 8 * V, t),
                      "alg_bytes": "per round 8 E + 12 V (hook) + 8 V (shortcut)",
                      "roofline_frac_bytes_read": alg_frac(rounds * (6 * E + 12 * V) + rounds * 8 * V, t),
@@ -222,7 +277,16 @@ def bench_apps(gi, g, sources, edges_of, log, only: str | None = None) -> dict:
             ts.append(st["ms"] * 1e-3)
         t = float(np.median(ts))
         Vp, Ep = gp.num_vertices, gp.num_edges
+        cpu = None
+        if cpu_on:
+            import oracle
+            off, nb, _ = gp.read_csr()
+            host = oracle.CSR(off, nb)
+            cpu = cpu_leg(lambda: (oracle.pagerank(host, 20), oracle.set_threads(0))[1], 20 * Ep,
+                          "oracle FP64 PageRank, 20 iterations (OpenMP pull over in-neighbours)", 1.0)
+            del host, off, nb
         out["pr"] = {"gteps": round(20 * Ep / t / 1e9, 3), "ms_per_run": round(t * 1e3, 3),
+                     "cpu_baseline": cpu,
                      "iterations_per_s": round(20 / t, 1),
                      "roofline_frac": alg_frac(20 * (8 * Ep + 20 * Vp), t),
                      "alg_bytes": "20 (8 E + 20 V)",
@@ -241,17 +305,84 @@ def bench_apps(gi, g, sources, edges_of, log, only: str | None = None) -> dict:
         gi.bc(gb, bsrc[:2], schedule=fused1)
         _, st_f = gi.bc(gb, bsrc, schedule=fused1)
         dtmp = gi.DeviceArray(gb.num_vertices, np.int32)
-        be = 0
+        be, bv = 0, 0
+        be_of = {}
         for s in bsrc:
             gi.bfs_device(gb, int(s), None, dtmp)
-            be += gb.reached_edges(dtmp)
+            be_of[int(s)] = gb.reached_edges(dtmp)
+            be += be_of[int(s)]
+            bv += int((dtmp.numpy() >= 0).sum())
         tb, tsq, tf = st_b["ms"] * 1e-3, st_s["ms"] * 1e-3, st_f["ms"] * 1e-3
+        Vb = gb.num_vertices
+        # effective bytes: per source, a push BFS (20 V_r + 8 E_r), sigma (4 E_r + 4 V), backward (8 E_r + 12 V)
+        bc_bytes = 20 * bv + 20 * be + 16 * Vb * len(bsrc)
+        cpu = None
+        if cpu_on:
+            import oracle
+            off, nb, _ = gb.read_csr()
+            host = oracle.CSR(off, nb)
+            sample = [int(x) for x in bsrc[:32]]
+            cpu = cpu_leg(lambda: (oracle.bc(host, sample), oracle.set_threads(0))[1],
+                          sum(be_of[x] for x in sample),
+                          f"oracle FP64 Brandes (OpenMP over sources) from {len(sample)} of the 256 sources")
         out["bc"] = {"sources": len(bsrc), "batched_ms": round(tb * 1e3, 2), "sequential_ms": round(tsq * 1e3, 2),
                      "speedup": round(tsq / tb, 2), "gteps": round(be / 2 / tb / 1e9, 3),
+                     "roofline_frac": alg_frac(bc_bytes, tb),
+                     "alg_bytes": "per source 20 V_reached + 20 E_reached + 16 V (forward push BFS, sigma, "
+                                  "backward), effective",
+                     "cpu_baseline": cpu,
                      "fused_single_source_ms": round(tf * 1e3, 2), "speedup_vs_fused_single": round(tf / tb, 2),
                      "config": "RMAT-20 sym, 256 sources: batch 256 vs one source at a time (unfused per-level "
                                "host loop, and the fused single-source mode)"}
         log(f"bc: {out['bc']}")
+        # batched BFS and SSSP on the same graph and sources: one launch set per
+        # step for 64 (BFS) / 256 (SSSP) sources against the fused single-source
+        # engines one source at a time
+        dall = gi.DeviceArray(len(bsrc) * Vb, np.int32)
+        gi.bfs_batched(gb, bsrc[:64], dall)
+        gi.synchronize()
+        t0 = time.perf_counter()
+        gi.bfs_batched(gb, bsrc, dall)
+        gi.synchronize()
+        t_mb = time.perf_counter() - t0
+        d_b = dall.numpy().reshape(len(bsrc), Vb)
+        one = autotune.schedule(gi.LB_TWC, 0.02)
+        gi.bfs_device(gb, int(bsrc[0]), one, dtmp)
+        gi.synchronize()
+        t0 = time.perf_counter()
+        for k, s in enumerate(bsrc):
+            gi.bfs_device(gb, int(s), one, dtmp)
+        gi.synchronize()
+        t_sq = time.perf_counter() - t0
+        for k in (0, 100, 255):  # rows checked against the single-source engine
+            gi.bfs_device(gb, int(bsrc[k]), one, dtmp)
+            if not np.array_equal(dtmp.numpy(), d_b[k]):
+                raise SystemExit("batched BFS depths differ from the single-source engine's")
+        out["bfs_batched"] = {"sources": len(bsrc), "ms": round(t_mb * 1e3, 2), "one_at_a_time_ms": round(t_sq * 1e3, 2),
+                              "speedup": round(t_sq / t_mb, 2), "gteps": round(be / 2 / t_mb / 1e9, 3),
+                              "config": "RMAT-20 sym, the 256 BC sources; depth[256][V] in 4 passes of 64 sources "
+                                        "vs the fused single-source engine one source at a time"}
+        log(f"bfs_batched: {out['bfs_batched']}")
+        gb.set_weights(SEED, 1, 255)
+        ss = bsrc[:64]
+        gi.sssp_batched(gb, ss[:8], 32, 0, dall)
+        _, st_sb = gi.sssp_batched(gb, ss, 32, 0, dall)
+        gi.sssp(gb, int(ss[0]), 32, fs)
+        t_ss = 0.0
+        for s in ss:
+            _, st1 = gi.sssp(gb, int(s), 32, fs)
+            t_ss += st1["ms"] * 1e-3
+        d_sb = dall.numpy()[: len(ss) * Vb].reshape(len(ss), Vb)
+        d1, _ = gi.sssp(gb, int(ss[5]), 32, fs)
+        if not np.array_equal(d1, d_sb[5]):
+            raise SystemExit("batched SSSP distances differ from the single-source engine's")
+        t_sb = st_sb["ms"] * 1e-3
+        out["sssp_batched"] = {"sources": len(ss), "ms": round(t_sb * 1e3, 2), "one_at_a_time_ms": round(t_ss * 1e3, 2),
+                               "speedup": round(t_ss / t_sb, 2),
+                               "gteps": round(sum(be_of[int(x)] for x in ss) / 2 / t_sb / 1e9, 3),
+                               "config": "RMAT-20 sym, w U[1,255], delta 32, 64 of the BC sources in one batch vs "
+                                         "the fused single-source engine one source at a time"}
+        log(f"sssp_batched: {out['sssp_batched']}")
         gb.destroy()
     if only in (None, "many"):
         # 1,024 uniform graphs (V log-uniform in [1k, 64k], average degree 8), one
@@ -343,8 +474,10 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-apps", action="store_true", help="BFS line only (no SSSP/CC/PR/BC/many-graph runs)")
+    ap.add_argument("--no-selector", action="store_true",
+                    help="skip the fixed-rule comparison (profiles of the timed BFS runs alone)")
     ap.add_argument("--sources", type=int, default=64, help="distinct BFS sources cycled over")
-    ap.add_argument("--app", choices=["sssp", "sssp_grid", "cc", "pr", "bc", "many"], default=None,
+    ap.add_argument("--app", choices=["sssp", "sssp_grid", "sssp_trace", "cc", "pr", "bc", "many"], default=None,
                     help="only this app's section (for profiling; prints its results, no bench line)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run three device-to-device copies of MIB MiB (HBM counter scale)")
@@ -412,7 +545,8 @@ def main():
         if i % 16 == 15:
             log(f"reference runs {i + 1}/{ns} ({time.perf_counter() - t0:.1f}s)")
     if args.app:
-        print(json.dumps({"app": args.app, "results": bench_apps(gi, g, sources, edges_of, log, args.app)}),
+        print(json.dumps({"app": args.app, "results": bench_apps(gi, g, sources, edges_of, log, args.app,
+                                                                 cpu_on=False)}),
               flush=True)
         return
     for k in range(args.warmup):
@@ -423,37 +557,53 @@ def main():
     log(f"{args.steps} BFS in {dt * 1e3:.1f} ms -> {gteps:.2f} GTEPS "
         f"({'learned selector' if use_learned else 'heuristic'})")
     selector_cmp = None
-    if learned is not None:
-        # the other policy on this layout, and GraphIt's rule also on the other
-        # layout: its best of the two is what the selector is judged against
-        other = heur if use_learned else learned
-        for k in range(args.warmup):
-            gi.bfs_device(g, int(sources[k % ns]), other, depth)
-        dt2, gteps2 = time_bfs(gi, g, other, sources, depth, edges_of, args.steps)
-        g_l, g_h = (gteps, gteps2) if use_learned else (gteps2, gteps)
-        h_by_layout = {args.layout: round(g_h, 3)}
-        alt = "native" if args.layout == "degree" else "degree"
-        g.set_layout(gi.LAYOUT_NATIVE if alt == "native" else gi.LAYOUT_DEGREE)
-        for k in range(args.warmup):
-            gi.bfs_device(g, int(sources[k % ns]), heur, depth)
-        _, g_h_alt = time_bfs(gi, g, heur, sources, depth, edges_of, args.steps)
-        h_by_layout[alt] = round(g_h_alt, 3)
+    if learned is not None and not args.no_selector:
+        # GraphIt's fixed rule |F| <= t*V, judged on both layouts: the
+        # strongest rule (the push/pull mode pair and t tuned on the
+        # calibration graphs, stored with the model) and the BFS app's own rule
+        # (PUSH TWCE / PULL VERTEX_BASED); the headline ratio is against the
+        # strongest rule on whichever layout suits it best
+        rules = {"strongest": sel_mod.rule_schedule(*model["rule"]) if "rule" in model else heur, "app": heur}
+        by_layout = {}
+        g_learned = {}
+        for lay_name, lay in (("degree", gi.LAYOUT_DEGREE), ("native", gi.LAYOUT_NATIVE)):
+            g.set_layout(lay)
+            g_learned[lay_name] = gteps if (lay == layout and use_learned) else None
+            if g_learned[lay_name] is None:
+                for k in range(args.warmup):
+                    gi.bfs_device(g, int(sources[k % ns]), learned, depth)
+                g_learned[lay_name] = time_bfs(gi, g, learned, sources, depth, edges_of, args.steps)[1]
+            for name, rs in rules.items():
+                gi.bfs_device(g, int(sources[0]), rs, depth)
+                _, gr = time_bfs(gi, g, rs, sources, depth, edges_of, min(args.steps, 16))
+                by_layout.setdefault(name, {})[lay_name] = round(gr, 3)
         g.set_layout(layout)
-        g_h = max(g_h, g_h_alt)
-        selector_cmp = {"learned_gteps": round(g_l, 3), "heuristic_gteps": round(g_h, 3),
-                        "heuristic_gteps_by_layout": h_by_layout,
-                        "speedup": round(g_l / g_h, 3),
-                        "heuristic": f"GraphIt hybrid |F| <= t*V: PUSH {gi.LB_NAMES[args.push_lb]} "
-                                     f"else PULL {gi.LB_NAMES[args.pull_lb]} bitmap, t={threshold:g}",
+        g_l = g_learned[args.layout]
+        best_rule = max(by_layout["strongest"].values())
+        rule_desc = (f"push {gi.BFS_MODE_NAMES[model['rule'][0]]}, pull {gi.BFS_MODE_NAMES[model['rule'][1]]}, "
+                     f"t={model['rule'][2]:g}") if "rule" in model else "the app's rule"
+        selector_cmp = {"learned_gteps": round(g_l, 3),
+                        "learned_gteps_by_layout": {k: round(v, 3) for k, v in g_learned.items()},
+                        "heuristic_gteps": best_rule,
+                        "heuristic_gteps_by_layout": by_layout["strongest"],
+                        "speedup": round(g_l / best_rule, 3),
+                        "speedup_same_layout": round(g_l / by_layout["strongest"][args.layout], 3),
+                        "app_rule_gteps_by_layout": by_layout["app"],
+                        "speedup_vs_app_rule": round(g_l / max(by_layout["app"].values()), 3),
+                        "heuristic": f"GraphIt hybrid |F| <= t*V with the mode pair and threshold tuned on the "
+                                     f"calibration graphs ({rule_desc}), on its best layout",
+                        "app_rule": f"PUSH {gi.LB_NAMES[args.push_lb]} else PULL {gi.LB_NAMES[args.pull_lb]} "
+                                    f"bitmap, t={threshold:g} (apps/bfs.cpp:69-80)",
                         "learned": "MLP 16-64-64-7 (BF16 weights, FP32 accumulation) in the decide "
-                                   "kernel, trained on calibration RMAT/uniform/grid graphs"}
+                                   "kernel, trained on the production step kernel's timings over calibration "
+                                   "RMAT/uniform/grid graphs (none of them RMAT-24)"}
         log(f"selector: {selector_cmp}")
 
     roof = roofline(gi, g, sched, sources[: args.profile_sources], depth) if args.profile_sources > 0 else None
     if roof is not None:
         roof["traffic"], roof["traffic_source"] = pmc_traffic(roof["kernel"], args.scale)
     log(f"roofline: {json.dumps(roof)}")
-    apps = None if args.no_apps else bench_apps(gi, g, sources, edges_of, log)
+    apps = None if args.no_apps else bench_apps(gi, g, sources, edges_of, log, cpu_on=not args.no_cpu)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(gi, g, sources, edges_of, args.cpu_budget)
