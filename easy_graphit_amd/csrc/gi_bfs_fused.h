@@ -24,9 +24,11 @@ constexpr int kBfsMaxLevels = 4096;
 constexpr int kSelInputs = 16;
 constexpr int kSelHidden = 64;
 constexpr int kSelMaxOut = 8;
-// Push vertices at least this wide go to the grid-wide bin (TWC). Low
-// enough that a small frontier of long lists (the source's neighbours) is
-// spread over the whole GPU instead of one wavefront per list.
+// Push vertices at least this wide go to the grid-wide bin (TWC): they are
+// listed when discovered, and the next step spreads their edges over its
+// whole grid before it walks its chunks. Low enough that a small frontier of
+// long lists (the source's neighbours) is spread over the whole GPU instead
+// of one wavefront per list.
 constexpr int32_t kHubDegree = 512;
 constexpr int32_t kMaxHubs = 1024;
 constexpr int32_t kMaxGrid = 8192;          // partial-sum slots per kernel
@@ -41,14 +43,17 @@ struct BfsState {
     int32_t mode;                   // mode of the next step
     long long unvisited;            // vertices not reached yet
     long long unvisited_edges;      // sum of their degrees
-    int32_t hub_count;              // deferred push vertices of the current step
+    int32_t hub_n[2];               // listed hubs of F_level (level & 1) and of F_level+1
+    int32_t want_hubs;              // list hubs at discovery (the selector may pick TWC)
+    int32_t hub_list_count;         // ... during push steps of at most this many vertices
+    long long hub_list_fdeg;        // ... and at most this many edges (when degrees are summed)
     int32_t lane_max;               // push: degree < lane_max walked by one lane
     int32_t hub_min;                // push TWC: degree >= hub_min goes to the grid bin
     int32_t want_fdeg;              // accumulate frontier degrees (learned selector, sweeps)
     int32_t pull_levels;            // levels expanded by a pull step so far
     int32_t *depth;                 // output arrays of the current run
     int32_t *parent;
-    int32_t hubs[kMaxHubs];
+    int32_t hubs[2][kMaxHubs];      // by level parity; more than kMaxHubs: the list is void
     // per-level history of the first kBfsMaxLevels levels (traces and
     // statistics); deeper runs go on, unrecorded
     int32_t hist_mode[kBfsMaxLevels];
@@ -56,7 +61,7 @@ struct BfsState {
 };
 
 // Device view of the selector network (BF16 weights, FP32 accumulation).
-// Passed to the decide kernel by value. The BF16 weights are stored one row
+// Passed to the decides by value. The BF16 weights are stored one row
 // per hidden unit, so that a lane (one hidden unit) reads its own weights
 // with 16-byte loads: w1 [hid][in], w2 [hid][hid] (row = the unit fed), w3
 // [hid][kSelMaxOut] (row = the unit feeding, zero-padded), biases b1 [hid],
@@ -74,7 +79,7 @@ struct SelectorView {
 };
 
 // The selector's 16 input features from the run's counters, in double: the
-// dataset recorder's definition (training inputs). The decide kernel
+// dataset recorder's definition (training inputs). The decide
 // evaluates the same formulas in FP32 with the hardware log2
 // (bfs_features_f32), equal to these within FP32 rounding.
 __host__ __device__ inline float bfs_raw_feature(int k, int32_t n, unsigned long long fdeg,
@@ -112,7 +117,7 @@ __host__ __device__ inline void bfs_raw_features(float *f, int32_t n, unsigned l
 }
 
 struct FusedTrace {
-    std::vector<hipEvent_t> ev;   // two per level: around the step + hub kernels
+    std::vector<hipEvent_t> ev;   // two per level: around the step kernel
     std::vector<int32_t> modes;   // mode of each level (the first kBfsMaxLevels)
     std::vector<int32_t> sizes;   // frontier size of each level (the first kBfsMaxLevels)
     int32_t pull_levels = 0;      // levels expanded by pull steps (all of them)

@@ -4,11 +4,18 @@
 // The reference's fused mode (fuse_kernel, operators.cpp:651-672) runs the
 // whole while-loop inside one cooperative kernel with grid barriers; its
 // hybrid schedule decides push vs pull on the host from a D2H copy of the
-// frontier size (operators.cpp:526). Here each level is three launches on one
-// stream: a persistent step kernel that switches on the chosen mode, a hub
-// kernel that spreads the edges of very wide push vertices over the whole
-// grid (a no-op on most levels), and a one-workgroup decide kernel (GraphIt's
-// hybrid rule or the MLP selector, from device counters).
+// frontier size (operators.cpp:526). Here each level is two launches on one
+// stream: a persistent step kernel that switches on the chosen mode, and a
+// one-workgroup decide kernel (GraphIt's hybrid rule or the MLP selector,
+// from device counters). Very wide push vertices (hubs) are listed when they
+// are discovered, and the next TWC step spreads their edges over its whole
+// grid before it walks its chunks.
+//
+// The decide stays a launch of its own: run by the last workgroup of a step
+// to finish, it would need every workgroup to publish its partial sums with
+// a device-scope release, and on this GPU (one L2 per XCD) each such fence
+// writes the XCD's L2 back — measured at ~110 us per level on RMAT-24,
+// against ~5 us for the extra launch.
 //
 // Frontiers are bitmaps only (one 64-bit word per wavefront of vertices),
 // each with a summary bitmap of its non-empty 64-word chunks:
@@ -66,7 +73,7 @@ struct StepArgs {
     uint64_t *bm[3];
     uint64_t *sum[3];
     uint8_t *cflag;             // chunk flags of the output bitmap (bs_flag_chunk)
-    int32_t *pcount;            // [2][kMaxGrid]: step slots, then hub slots
+    int32_t *pcount;            // [kMaxGrid]: one slot per step workgroup
     unsigned long long *pfdeg;  // same layout
 };
 
@@ -75,6 +82,9 @@ struct StepCtx {
     int32_t lane_max;          // degree < lane_max: one lane walks the vertex
     int32_t hub_min;           // TWC: degree >= hub_min goes to the grid bin
     int32_t want_fdeg;
+    int32_t want_hubs;         // list the hubs of F' as they are discovered
+    int32_t hub_out;           // their list: hubs[(level + 1) & 1]
+    int32_t hubs_listed;       // TWC step whose hubs (of F) are all listed: walked grid-wide first
     const uint64_t *bm_in;
     uint64_t *bm_out;
     uint8_t *cflag;
@@ -98,6 +108,28 @@ __device__ __forceinline__ bool in_bitmap(const uint64_t *bm, int32_t v) {
     return (bm[v >> 6] >> (v & 63)) & 1ull;
 }
 
+// A discovered vertex's degree, read when the frontier degree sum or the hub
+// list needs it.
+__device__ __forceinline__ int32_t note_degree(const StepArgs &a, StepCtx &c, int32_t d) {
+    if (!c.want_fdeg && !c.want_hubs) return 0;
+    const int32_t dd = deg_of(a.off, d);
+    if (c.want_fdeg) c.fdeg += (unsigned long long)dd;
+    return dd;
+}
+
+// Appends d to the hub list of F' (one lane).
+__device__ __forceinline__ void note_hub(const StepArgs &a, const StepCtx &c, int32_t d) {
+    const int32_t p = atomicAdd(&a.st->hub_n[c.hub_out], 1);
+    if (p < kMaxHubs) a.st->hubs[c.hub_out][p] = d;
+}
+
+// Appends the lanes' hubs to the list of F' with one atomic per wave (every
+// lane of the wave calls it).
+__device__ __forceinline__ void note_hubs_wave(const StepArgs &a, const StepCtx &c, int32_t d, bool hub) {
+    const int32_t p = wave_atomic_add(&a.st->hub_n[c.hub_out], hub);
+    if (hub && p < kMaxHubs) a.st->hubs[c.hub_out][p] = d;
+}
+
 // Push discovery: test, claim in `visited`, record in the output bitmap.
 __device__ __forceinline__ void push_edge(const StepArgs &a, StepCtx &c, int32_t s, int32_t e,
                                           bool active) {
@@ -112,7 +144,8 @@ __device__ __forceinline__ void push_edge(const StepArgs &a, StepCtx &c, int32_t
     if (o2 == 0) bs_flag_chunk(c.cflag, d >> 6);
     if (a.parent) a.parent[d] = s;  // the depth is written in word order by the next step
     c.cnt += 1;
-    if (c.want_fdeg) c.fdeg += (unsigned long long)deg_of(a.off, d);
+    const int32_t dd = note_degree(a, c, d);
+    if (c.want_hubs && dd >= c.hub_min) note_hub(a, c, d);
 }
 
 // Push discovery by a whole wavefront at once (every lane calls it; `active`
@@ -155,11 +188,14 @@ __device__ __forceinline__ void push_edge_wave(const StepArgs &a, StepCtx &c, in
     // run is recorded once, by its first lane
     const uint64_t xp = __shfl_up(x, 1);
     const uint64_t before = lane - 1 >= my_head ? xp : 0ull;
-    if (want && (run_won & bit) && !(before & bit)) {
+    const bool mine = want && (run_won & bit) && !(before & bit);
+    int32_t dd = 0;
+    if (mine) {
         if (a.parent) a.parent[d] = s;  // the depth is written in word order by the next step
         c.cnt += 1;
-        if (c.want_fdeg) c.fdeg += (unsigned long long)deg_of(a.off, d);
+        dd = note_degree(a, c, d);
     }
+    if (c.want_hubs) note_hubs_wave(a, c, d, mine && dd >= c.hub_min);  // uniform
 }
 
 __device__ __forceinline__ void lane_walk(const StepArgs &a, StepCtx &c, int32_t s, int32_t e,
@@ -252,11 +288,10 @@ __device__ void push_batch(const StepArgs &a, StepCtx &c, int32_t mode, const in
             wave_walk(a, c, v, a.off[v], a.off[v + 1]);
         }
     } else if (mode == BFS_PUSH_TWC) {
-        // lane bin, wave bin, grid bin (deferred to the hub kernel)
-        bool hub = has && dg >= c.hub_min;
-        int32_t hpos = wave_atomic_add(&a.st->hub_count, hub);
-        if (hub && hpos < kMaxHubs) a.st->hubs[hpos] = s;
-        bool wide = has && dg >= c.lane_max && (!hub || hpos >= kMaxHubs);
+        // lane bin, wave bin; the grid bin (listed hubs) was walked at the
+        // start of the step, unless the list overflowed
+        const bool todo = has && !(c.hubs_listed && dg >= c.hub_min);
+        bool wide = todo && dg >= c.lane_max;
         uint64_t wm = __ballot(wide);
         while (wm) {
             int32_t l = __ffsll((unsigned long long)wm) - 1;
@@ -264,7 +299,7 @@ __device__ void push_batch(const StepArgs &a, StepCtx &c, int32_t mode, const in
             int32_t ws = __shfl(s, l), w0 = __shfl(e0, l), w1 = __shfl(e1, l);
             wave_walk(a, c, ws, w0, w1);
         }
-        if (has && dg < c.lane_max) lane_walk(a, c, s, e0, e1);
+        if (todo && dg < c.lane_max) lane_walk(a, c, s, e0, e1);
     } else {
         // TWCE and CM need the workgroup's degree scan
         int32_t total = 0;
@@ -413,7 +448,7 @@ __device__ __forceinline__ bool pull_words(const StepArgs &a, StepCtx &c, int32_
         if (mode != BFS_PULL_THREAD && __ballot(ei < e1i)) coop = pull_coop(a, c, w, ei, e1i);
         const int32_t v = w * 64 + lane;
         const bool found_self = pi >= 0 || ((coop >> lane) & 1ull);
-        if (found_self) {
+        if (found_self) {  // (pull steps list no hubs)
             if (a.parent && pi >= 0) a.parent[v] = pi;
             if (c.want_fdeg) c.fdeg += (unsigned long long)deg_of(a.off, v);
         }
@@ -430,35 +465,21 @@ __device__ __forceinline__ bool pull_words(const StepArgs &a, StepCtx &c, int32_
     return any;
 }
 
-// Grid bin of PUSH_TWC: the edges of the deferred hubs, spread evenly over
-// the grid. Every workgroup scans the hub degrees in LDS and walks its share
-// of the concatenated edge range.
-__device__ __forceinline__ void hub_body(const StepArgs &a0) {
-    BfsState *st = a0.st;
-    if (st->done || st->mode >= BFS_PULL_THREAD) return;
-    int32_t nh = st->hub_count;
-    if (nh <= 0) return;  // slots stay zero (the decide kernel clears them)
-    if (nh > kMaxHubs) nh = kMaxHubs;
-    StepArgs a = a0;
-    a.depth = st->depth;
-    a.parent = st->parent;
-    StepCtx c;
-    c.level = st->level;
-    c.lane_max = st->lane_max;
-    c.hub_min = st->hub_min;
-    c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], c.level % 3);
-    c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], (c.level + 1) % 3);
-    c.cflag = a0.cflag;
-    c.want_fdeg = st->want_fdeg;
-    c.cnt = 0;
-    c.fdeg = 0;
-    __shared__ int32_t hscan[kMaxHubs + 1];
-    __shared__ int32_t hv[kMaxHubs];
-    int32_t per = (nh + kStepBlock - 1) / kStepBlock;
-    int32_t lo = threadIdx.x * per, hi = lo + per < nh ? lo + per : nh;
+// Grid bin of PUSH_TWC, at the start of the step: the edges of F's listed
+// hubs (degree >= hub_min, listed when they were discovered) spread evenly
+// over the grid. Every workgroup scans the hub degrees into LDS (`lds`: the
+// step's vertex list, free at this point) and walks its share of the
+// concatenated edge range. Every thread of the workgroup calls it.
+static_assert(kChunkVerts >= 2 * kMaxHubs + 1, "hub scan fits the vertex list");
+constexpr int32_t kHubListShare = 64;  // list hubs during push steps of <= V/64 vertices, <= E/64 edges
+__device__ __forceinline__ void hub_phase(const StepArgs &a, StepCtx &c, int32_t nh, int32_t *lds) {
+    const int32_t *hubs = a.st->hubs[c.level & 1];
+    int32_t *hv = lds, *hscan = lds + kMaxHubs;
+    const int32_t per = (nh + kStepBlock - 1) / kStepBlock;
+    const int32_t lo = threadIdx.x * per, hi = lo + per < nh ? lo + per : nh;
     int32_t part = 0;
     for (int32_t i = lo; i < hi; ++i) {
-        int32_t s = st->hubs[i];
+        const int32_t s = hubs[i];
         hv[i] = s;
         part += deg_of(a.off, s);
     }
@@ -469,8 +490,8 @@ __device__ __forceinline__ void hub_body(const StepArgs &a0) {
         ex += deg_of(a.off, hv[i]);
     }
     __syncthreads();
-    int32_t chunk = (total + gridDim.x - 1) / gridDim.x;
-    int32_t k0 = blockIdx.x * chunk, k1 = k0 + chunk < total ? k0 + chunk : total;
+    const int32_t chunk = (total + gridDim.x - 1) / gridDim.x;
+    const int32_t k0 = blockIdx.x * chunk, k1 = k0 + chunk < total ? k0 + chunk : total;
     for (int32_t kb = k0; kb < k1; kb += kStepBlock) {  // block-uniform trip count
         const int32_t k = kb + (int32_t)threadIdx.x;
         const bool act = k < k1;
@@ -482,26 +503,18 @@ __device__ __forceinline__ void hub_body(const StepArgs &a0) {
         }
         push_edge_wave(a, c, s, e, act);
     }
-    flush_counts(a.pcount + kMaxGrid, a.pfdeg + kMaxGrid, c);
+    __syncthreads();  // the list LDS is reused by the chunk loop
 }
 
-__global__ void __launch_bounds__(kStepBlock) bfs_hub_kernel(StepArgs a0) { hub_body(a0); }
-
-// ---------------------------------------------------------------- step kernels
-// One level's step. The push and pull paths are separate kernels (each with
-// its own register and LDS budget, hence occupancy); a recorded level
-// launches both and the one that does not match the decided mode returns at
-// once. Traced runs (per-level timing) launch only the matching one, under
-// their own symbols so their launches can be told apart in a profile.
-enum StepKind : int { kPushStep = 0, kPullStep = 1 };
-
+// ---------------------------------------------------------------- step
+// One level's step: the push or the pull body, as the state's mode says.
 // `pre` is LDS of kMaxSumWords + 1 words owned by the calling kernel, so a
 // kernel that holds both step bodies allocates it once.
+enum StepKind : int { kPushStep = 0, kPullStep = 1 };
+
 template <int Kind>
 __device__ __forceinline__ void step_body(const StepArgs &a0, int32_t *pre) {
     BfsState *st = a0.st;
-    if (st->done) return;
-    if ((Kind == kPullStep) != (st->mode >= BFS_PULL_THREAD)) return;  // uniform
     StepArgs a = a0;
     a.depth = st->depth;
     a.parent = st->parent;
@@ -511,18 +524,29 @@ __device__ __forceinline__ void step_body(const StepArgs &a0, int32_t *pre) {
     c.hub_min = st->hub_min;
     c.want_fdeg = st->want_fdeg;
     const int32_t mode = st->mode;
+    // F''s hubs are listed during small push steps only: after a large or a
+    // pull step the next step is a pull step or has too many hubs for the
+    // list (and the same-address atomics of listing them would cost more
+    // than the list saves); such a step voids the list instead
+    {
+        const int32_t p = c.level & 1;
+        c.want_hubs = st->want_hubs && mode < BFS_PULL_THREAD && st->count[p] <= st->hub_list_count &&
+                      (!c.want_fdeg || (long long)st->fdeg[p] <= st->hub_list_fdeg);
+    }
+    c.hub_out = (c.level + 1) & 1;
+    if (!c.want_hubs && blockIdx.x == 0 && threadIdx.x == 0) st->hub_n[c.hub_out] = kMaxHubs + 1;
+    const int32_t nh = st->hub_n[c.level & 1];
+    c.hubs_listed = mode == BFS_PUSH_TWC && st->want_hubs && nh <= kMaxHubs;  // else void or never kept
     const int32_t iin = c.level % 3, iout = (c.level + 1) % 3, iclr = (c.level + 2) % 3;
     c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iin);
     c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iout);
     c.cflag = a0.cflag;
     c.cnt = 0;
     c.fdeg = 0;
-    const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t threads = gridDim.x * blockDim.x;
     const int32_t lane = lane_id();
 
     // the bitmap two levels ahead is free: clear its non-empty chunks (its
-    // summary is cleared by the decide kernel after this step)
+    // summary is cleared by the decide after this step)
     {
         const uint64_t *sc = pick3(a0.sum[0], a0.sum[1], a0.sum[2], iclr);
         uint64_t *bc = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iclr);
@@ -540,6 +564,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a0, int32_t *pre) {
     if constexpr (Kind == kPushStep) {
         __shared__ int32_t list[kChunkVerts];
         __shared__ int32_t n_s;
+        if (c.hubs_listed && nh > 0) hub_phase(a, c, nh, list);  // uniform
         const uint64_t *si = pick3(a0.sum[0], a0.sum[1], a0.sum[2], iin);
         int32_t nz = summary_prefix(si, a.sum_words, pre);
         for (int32_t k = blockIdx.x; k < nz; k += gridDim.x) {
@@ -596,41 +621,32 @@ __device__ __forceinline__ void step_body(const StepArgs &a0, int32_t *pre) {
     flush_counts(a.pcount, a.pfdeg, c);
 }
 
-// (the four single-body kernels fit the register budget of 8 waves per SIMD
-// without spilling)
-#define GI_STEP_ATTR __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(8)))
-#define GI_STEP_KERNEL(name, kind)                  \
-    __global__ void GI_STEP_ATTR name(StepArgs a) { \
-        __shared__ int32_t pre[kMaxSumWords + 1];   \
-        step_body<kind>(a, pre);                    \
-    }
-GI_STEP_KERNEL(bfs_push_step, kPushStep)
-GI_STEP_KERNEL(bfs_pull_step, kPullStep)
-GI_STEP_KERNEL(bfs_push_step_traced, kPushStep)
-GI_STEP_KERNEL(bfs_pull_step_traced, kPullStep)
-#undef GI_STEP_KERNEL
-// The recorded level loop launches one step kernel per level: the body the
-// decided mode needs (one launch fewer per level than a push and a pull
-// kernel of which one returns at once). Both bodies together need 72 VGPRs:
-// 7 waves per SIMD without spilling measured faster than 8 with a spill.
-__global__ void __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(7))) bfs_step(StepArgs a) {
-    __shared__ int32_t pre[kMaxSumWords + 1];
-    if (a.st->mode >= BFS_PULL_THREAD) step_body<kPullStep>(a, pre);  // uniform
-    else step_body<kPushStep>(a, pre);
-}
-#undef GI_STEP_ATTR
-
-// ---------------------------------------------------------------- decide kernel
+// ---------------------------------------------------------------- decide
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
     return __uint_as_float(((uint32_t)h) << 16);
 }
 
-// One workgroup, after step `level` (or after seeding, level = -1): adds up
-// the step's partial sums, closes its bookkeeping, ends the run on an empty
-// frontier, else picks the next step's mode and clears what the next step
-// writes. The MLP runs on wave 0; every thread passes the same barriers.
-struct SumSet {
-    uint64_t *s[3];
+// The decide kernel: one workgroup, after step `level` (or after seeding,
+// level = -1), adds up the step's partial sums, closes its bookkeeping, ends
+// the run on an empty frontier, else picks the next step's mode and clears
+// what the next step writes. The MLP runs on wave 0; every thread passes the
+// same barriers.
+struct DecideArgs {
+    BfsState *st;
+    int32_t V;
+    int64_t E;
+    int32_t *pcount;
+    unsigned long long *pfdeg;
+    int32_t slots;          // step workgroups that wrote a slot
+    uint64_t *sum[3];
+    int32_t sum_words;
+    uint8_t *cflag;
+    int32_t chunks;
+    int32_t *host_flag;     // pinned: {done, level, levels} when the run ends
+    const int32_t *off;
+    uint64_t *visited;
+    uint64_t *bm0;          // the bitmap of F_0
+    int32_t seed;           // the decide after the reset: a single source to seed, or -1
 };
 
 // The selector's 16 inputs in FP32 with the hardware log2 (v_log_f32): the
@@ -669,14 +685,18 @@ __device__ __forceinline__ float bf16_in(const uint4 *q, int k) {
     return __uint_as_float((k & 1) ? (word & 0xffff0000u) : (word << 16));
 }
 
-__global__ void __launch_bounds__(kDecideBlock)
-bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int64_t E, int32_t *pcount,
-                  unsigned long long *pfdeg, int32_t slots, SumSet sums, int32_t sum_words, uint8_t *cflag,
-                  int32_t chunks) {
+template <int Block>
+__device__ __forceinline__ void decide_body(const DecideArgs &d, const SelectorView &sel) {
+    BfsState *st = d.st;
     if (st->done) return;  // uniform
+    const int32_t V = d.V;
+    const int64_t E = d.E;
+    int32_t *pcount = d.pcount;
+    unsigned long long *pfdeg = d.pfdeg;
+    int32_t *host_flag = d.host_flag;
     // after step `level`, the bitmap two levels ahead of it was cleared by the
     // step; its summary is cleared here
-    uint64_t *sum_clear = pick3(sums.s[0], sums.s[1], sums.s[2], (st->level + 2 + 3) % 3);
+    uint64_t *sum_clear = pick3(d.sum[0], d.sum[1], d.sum[2], (st->level + 2 + 3) % 3);
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
     const int32_t next_level = st->level + 1;
     const int32_t p = next_level & 1;
@@ -703,21 +723,38 @@ bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int64_t E, int32_t 
         b2v = sel.b2[lane];
         if (lane < sel.n_out) b3v = sel.b3[lane];
     }
-    __shared__ int32_t rn[kDecideBlock / 64];
-    __shared__ unsigned long long rf[kDecideBlock / 64];
+    __shared__ int32_t rn[Block / 64];
+    __shared__ unsigned long long rf[Block / 64];
     int32_t n = 0;
     unsigned long long fdeg = 0;
-    if (next_level == 0) {  // seeded frontier
+    if (next_level == 0 && d.seed >= 0) {
+        // a single source: seeded here (the reset cleared its entries); every
+        // thread counts it
+        const int32_t v = d.seed, dv = d.off[v + 1] - d.off[v];
+        n = 1;
+        fdeg = (unsigned long long)dv;
+        if (threadIdx.x == 0) {
+            const uint64_t bit = 1ull << (v & 63);
+            const int32_t ch = (v >> 6) / kChunkWords;
+            st->depth[v] = 0;
+            if (st->parent) st->parent[v] = v;
+            d.visited[v >> 6] |= bit;
+            d.bm0[v >> 6] |= bit;
+            d.sum[0][ch >> 6] |= 1ull << (ch & 63);
+            if (st->want_hubs && dv >= st->hub_min) {  // the first step's grid bin
+                st->hubs[0][0] = v;
+                st->hub_n[0] = 1;
+            }
+        }
+    } else if (next_level == 0) {  // seeded by the seed kernel
         n = st->count[0];
         fdeg = st->fdeg[0];
     } else {
-        for (int32_t i = threadIdx.x; i < slots; i += kDecideBlock) {
-            n += pcount[i] + pcount[kMaxGrid + i];
-            fdeg += pfdeg[i] + pfdeg[kMaxGrid + i];
+        for (int32_t i = threadIdx.x; i < d.slots; i += Block) {
+            n += pcount[i];
+            fdeg += pfdeg[i];
             pcount[i] = 0;
-            pcount[kMaxGrid + i] = 0;
             pfdeg[i] = 0;
-            pfdeg[kMaxGrid + i] = 0;
         }
         for (int o = 32; o > 0; o >>= 1) {
             n += __shfl_xor(n, o);
@@ -727,13 +764,13 @@ bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int64_t E, int32_t 
             rn[wid] = n;
             rf[wid] = fdeg;
         }
-        for (int32_t i = threadIdx.x; i < sum_words; i += kDecideBlock) sum_clear[i] = 0;
+        for (int32_t i = threadIdx.x; i < d.sum_words; i += Block) sum_clear[i] = 0;
         // the summary of F', from the flags its chunks set
-        bs_build_summary<kDecideBlock>(cflag, chunks, pick3(sums.s[0], sums.s[1], sums.s[2], next_level % 3));
+        bs_build_summary<Block>(d.cflag, d.chunks, pick3(d.sum[0], d.sum[1], d.sum[2], next_level % 3));
         __syncthreads();
         n = 0;
         fdeg = 0;
-        for (int i = 0; i < kDecideBlock / 64; ++i) {
+        for (int i = 0; i < Block / 64; ++i) {
             n += rn[i];
             fdeg += rf[i];
         }
@@ -745,6 +782,11 @@ bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int64_t E, int32_t 
         if (threadIdx.x == 0) {
             st->done = 1;
             st->levels = next_level;
+            if (host_flag) {  // the host waits on the replay's event, then reads these
+                host_flag[2] = next_level;
+                host_flag[1] = next_level;
+                host_flag[0] = 1;
+            }
         }
         return;
     }
@@ -790,7 +832,7 @@ bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int64_t E, int32_t 
         st->unvisited_edges = unv_e;
         st->mode = mode;
         st->level = next_level;
-        st->hub_count = 0;
+        st->hub_n[(next_level + 1) & 1] = 0;  // the list the step walked: the next step refills it
         if (mode >= BFS_PULL_THREAD) st->pull_levels += 1;
         if (next_level < kBfsMaxLevels) {
             st->hist_mode[next_level] = mode;
@@ -799,12 +841,42 @@ bfs_decide_kernel(BfsState *st, SelectorView sel, int32_t V, int64_t E, int32_t 
     }
 }
 
+__global__ void __launch_bounds__(kDecideBlock) bfs_decide_kernel(DecideArgs d, SelectorView sel) {
+    decide_body<kDecideBlock>(d, sel);
+}
+
+// A level's step: the push or the pull body, as the decided mode says.
+__device__ __forceinline__ void level_body(const StepArgs &a) {
+    __shared__ int32_t pre[kMaxSumWords + 1];
+    if (a.st->done) return;                                            // uniform: set by a decide kernel
+    if (a.st->mode >= BFS_PULL_THREAD) step_body<kPullStep>(a, pre);  // uniform
+    else step_body<kPushStep>(a, pre);
+}
+
+// The recorded level loop launches bfs_step. Traced levels (per-level
+// timing) and sweeps launch the same code, traced levels under two more
+// symbols, one per direction, so a profile tells their launches apart. Both
+// step bodies together need 72 VGPRs: 7 waves per SIMD without spilling
+// measured faster than 8 with a spill.
+#define GI_STEP_KERNEL(name)                                                               \
+    __global__ void __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(7))) \
+    name(StepArgs a) {                                                                    \
+        level_body(a);                                                                     \
+    }
+GI_STEP_KERNEL(bfs_step)
+GI_STEP_KERNEL(bfs_push_step_traced)
+GI_STEP_KERNEL(bfs_pull_step_traced)
+#undef GI_STEP_KERNEL
+
 struct SeedArgs {
     int32_t *depth;
     int32_t *parent;
     int32_t lane_max;
     int32_t hub_min;
     int32_t want_fdeg;
+    int32_t want_hubs;
+    int32_t hub_list_count;
+    long long hub_list_fdeg;
 };
 
 struct ResetArgs {
@@ -865,6 +937,11 @@ __global__ void bfs_reset_kernel(BfsState *st, SeedArgs sa, ResetArgs r, int64_t
         st->count[1] = 0;
         st->fdeg[0] = 0;
         st->fdeg[1] = 0;
+        st->hub_n[0] = 0;
+        st->hub_n[1] = 0;
+        st->want_hubs = sa.want_hubs;
+        st->hub_list_count = sa.hub_list_count;
+        st->hub_list_fdeg = sa.hub_list_fdeg;
         st->done = 0;
         st->level = -1;
         st->levels = 0;
@@ -872,7 +949,6 @@ __global__ void bfs_reset_kernel(BfsState *st, SeedArgs sa, ResetArgs r, int64_t
         st->mode = BFS_PUSH_THREAD;
         st->unvisited = r.V;
         st->unvisited_edges = E;
-        st->hub_count = 0;
         st->lane_max = sa.lane_max;
         st->hub_min = sa.hub_min;
         st->want_fdeg = sa.want_fdeg;
@@ -936,7 +1012,12 @@ __global__ void bfs_seed_sources_kernel(BfsState *st, const int32_t *src, int32_
         int32_t ch = (v >> 6) / kChunkWords;
         atomicOr((unsigned long long *)&sum0[ch >> 6], 1ull << (ch & 63));
         cnt += 1;
-        fd += (unsigned long long)(off[v + 1] - off[v]);
+        const int32_t dv = off[v + 1] - off[v];
+        fd += (unsigned long long)dv;
+        if (st->want_hubs && dv >= st->hub_min) {  // the first step's grid bin
+            const int32_t p = atomicAdd(&st->hub_n[0], 1);
+            if (p < kMaxHubs) st->hubs[0][p] = v;
+        }
     }
     for (int o = 32; o > 0; o >>= 1) {
         cnt += __shfl_xor(cnt, o);
@@ -970,11 +1051,12 @@ struct FusedWork {
     int32_t cand_first = 0;          // cand holds first in-neighbours (degree-ordered views)
     const int32_t *poff = nullptr;   // in-edge CSR (the graph itself when symmetric)
     const int32_t *pnbr = nullptr;
-    SelectorView last_sel{};         // selector of the current run (passed to the decide kernel)
+    SelectorView last_sel{};         // selector of the current run (passed to the decides)
     int32_t *d_src = nullptr;        // sources of a multi-source run
     int32_t src_cap = 0;
-    int32_t *host_flag = nullptr;    // pinned copy of {done, level, levels, mode}
-    hipEvent_t flag_ev = nullptr;    // recorded after the copy into host_flag
+    int32_t *host_flag = nullptr;    // pinned, coherent {done, level, levels, mode}: written by the
+                                     // decide that ends a run, or copied in by the traced path
+    hipEvent_t flag_ev = nullptr;    // recorded after each replay of a block of levels
     int32_t sum_words = 0;
     int32_t lane_max = 64;           // bin thresholds (gi_bfs_set_bins)
     int32_t hub_min = kHubDegree;
@@ -1032,8 +1114,8 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
     int rc = dalloc(&w->st, 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->bm[i], (size_t)words + 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->sum[i], (size_t)sum_words + 1);
-    if (rc == GI_OK) rc = dalloc(&w->pcount, 2 * (size_t)kMaxGrid);
-    if (rc == GI_OK) rc = dalloc(&w->pfdeg, 2 * (size_t)kMaxGrid);
+    if (rc == GI_OK) rc = dalloc(&w->pcount, (size_t)kMaxGrid);
+    if (rc == GI_OK) rc = dalloc(&w->pfdeg, (size_t)kMaxGrid);
     if (rc == GI_OK) rc = dalloc(&w->cflag, (size_t)bs_flag_bytes(chunks));
     if (rc == GI_OK && hipMemset(w->cflag, 0, (size_t)bs_flag_bytes(chunks)) != hipSuccess)
         rc = fail(GI_ERR_HIP, "flag init");
@@ -1041,12 +1123,12 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
     if (rc == GI_OK) rc = dalloc(&w->dead, (size_t)words + 1);
     if (rc == GI_OK) rc = dalloc(&w->cand, (size_t)g->V + 1);
     if (rc == GI_OK && !g->d_visited) rc = dalloc(&g->d_visited, (size_t)words + 1);
-    if (rc == GI_OK && hipHostMalloc((void **)&w->host_flag, sizeof(int32_t) * 4) != hipSuccess)
+    if (rc == GI_OK && hipHostMalloc((void **)&w->host_flag, sizeof(int32_t) * 4, hipHostMallocCoherent) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
     if (rc == GI_OK && hipEventCreateWithFlags(&w->flag_ev, hipEventDisableTiming) != hipSuccess)
         rc = fail(GI_ERR_HIP, "event create failed");
-    if (rc == GI_OK) rc = fill_i32(w->pcount, 2 * kMaxGrid, 0, 0);
-    if (rc == GI_OK) rc = fill_u64((uint64_t *)w->pfdeg, 2 * kMaxGrid, 0, 0);
+    if (rc == GI_OK) rc = fill_i32(w->pcount, kMaxGrid, 0, 0);
+    if (rc == GI_OK) rc = fill_u64((uint64_t *)w->pfdeg, kMaxGrid, 0, 0);
     if (rc == GI_OK) {
         int grid = (int)ceil_div(g->V, 256);
         grid = grid < 4096 ? (grid < 1 ? 1 : grid) : 4096;
@@ -1088,26 +1170,11 @@ static int grid_of(const FusedWork *w, int resident) {
     return g < kMaxGrid ? g : kMaxGrid;
 }
 
-static int push_grid(const FusedWork *w) {
-    static const int resident = resident_per_cu((const void *)bfs_push_step, kStepBlock);
-    return grid_of(w, resident);
-}
-
-static int pull_grid(const FusedWork *w) {
-    static const int resident = resident_per_cu((const void *)bfs_pull_step, kStepBlock);
-    return grid_of(w, resident);
-}
-
-// Partial-sum slots the decide kernel adds up: every block of either step.
+// Every level, traced or not, runs on the same persistent grid; each of its
+// workgroups writes one partial-sum slot.
 static int step_grid(const FusedWork *w) {
     static const int resident = resident_per_cu((const void *)bfs_step, kStepBlock);
     return grid_of(w, resident);
-}
-
-static int step_slots(const FusedWork *w) {
-    int a = push_grid(w), b = pull_grid(w), c = step_grid(w);
-    a = a > b ? a : b;
-    return a > c ? a : c;
 }
 
 static StepArgs step_args(gi_graph *g, FusedWork *w) {
@@ -1136,35 +1203,54 @@ static StepArgs step_args(gi_graph *g, FusedWork *w) {
     return a;
 }
 
-static void launch_decide(gi_graph *g, FusedWork *w, Launcher &L) {
+static DecideArgs decide_args(gi_graph *g, FusedWork *w) {
+    DecideArgs d;
+    d.st = w->st;
+    d.V = g->V;
+    d.E = g->E;
+    d.pcount = w->pcount;
+    d.pfdeg = w->pfdeg;
+    d.slots = step_grid(w);
+    for (int i = 0; i < 3; ++i) d.sum[i] = w->sum[i];
+    d.sum_words = w->sum_words;
+    d.cflag = w->cflag;
+    d.chunks = (int32_t)ceil_div(bitmap_words(g->V), kChunkWords);
+    d.host_flag = w->host_flag;
+    d.off = g->d_off;
+    d.visited = g->d_visited;
+    d.bm0 = w->bm[0];
+    d.seed = -1;
+    return d;
+}
+
+static void launch_decide(gi_graph *g, FusedWork *w, Launcher &L, int32_t seed = -1) {
     // the kernel reads the level from the device state
-    SumSet sums{{w->sum[0], w->sum[1], w->sum[2]}};
-    L(bfs_decide_kernel, 1, kDecideBlock, w->st, w->last_sel, g->V, g->E, w->pcount, w->pfdeg, step_slots(w), sums,
-      w->sum_words, w->cflag, (int32_t)ceil_div(bitmap_words(g->V), kChunkWords));
+    DecideArgs d = decide_args(g, w);
+    d.seed = seed;
+    L(bfs_decide_kernel, 1, kDecideBlock, d, w->last_sel);
 }
 
-static void launch_decide(gi_graph *g, FusedWork *w, hipStream_t s) {
+static void launch_decide(gi_graph *g, FusedWork *w, hipStream_t s, int32_t seed = -1) {
     Launcher L(s);
-    launch_decide(g, w, L);
+    launch_decide(g, w, L, seed);
 }
 
-// A recorded level: the step kernel (push or pull body as decided), the hub
-// kernel (a no-op after a pull step), the decide kernel.
+// A recorded level: the step kernel (push or pull body as decided, the TWC
+// grid bin included), the decide kernel.
 static void launch_level(gi_graph *g, FusedWork *w, const StepArgs &a, Launcher &L) {
     L(bfs_step, step_grid(w), kStepBlock, a);
-    L(bfs_hub_kernel, push_grid(w), kStepBlock, a);
     launch_decide(g, w, L);
 }
 
-// A traced level with the mode already known on the host: the matching step
-// kernel alone between the two events.
+// A traced level with the mode already known on the host: the step code
+// under its per-direction symbol alone between the two events, then the
+// decide kernel.
 static void launch_level_traced(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t mode, hipStream_t s,
                                 hipEvent_t *ev) {
     (void)hipEventRecord(ev[0], s);
-    if (mode >= BFS_PULL_THREAD) bfs_pull_step_traced<<<pull_grid(w), kStepBlock, 0, s>>>(a);
-    else bfs_push_step_traced<<<push_grid(w), kStepBlock, 0, s>>>(a);
+    if (mode >= BFS_PULL_THREAD) bfs_pull_step_traced<<<step_grid(w), kStepBlock, 0, s>>>(a);
+    else bfs_push_step_traced<<<step_grid(w), kStepBlock, 0, s>>>(a);
     (void)hipEventRecord(ev[1], s);
-    if (mode < BFS_PULL_THREAD) bfs_hub_kernel<<<push_grid(w), kStepBlock, 0, s>>>(a);
     launch_decide(g, w, s);
 }
 
@@ -1174,7 +1260,21 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
                     int32_t single, int32_t *depth, int32_t *parent, const SelectorView &sel,
                     bool want_fdeg, hipStream_t s, int32_t depth_len = -1) {
     w->last_sel = sel;  // recorded level graphs are kept per selector view (level_graph)
-    SeedArgs sa{depth, parent, w->lane_max, w->hub_min, want_fdeg ? 1 : 0};
+    // hubs are listed at discovery when a TWC step may use the list: the
+    // selector may pick TWC, or the degrees are read anyway
+    const bool learned = sel.mode == GI_SELECT_LEARNED && sel.w1 != nullptr;
+    const bool may_twc = learned ? ((sel.allowed >> BFS_PUSH_TWC) & 1) != 0
+                                 : (sel.mode_s1 == BFS_PUSH_TWC || sel.mode_s2 == BFS_PUSH_TWC);
+    // ... during push steps of at most V/64 vertices and E/64 edges
+    SeedArgs sa{depth,
+                parent,
+                w->lane_max,
+                w->hub_min,
+                want_fdeg ? 1 : 0,
+                (want_fdeg || may_twc) ? 1 : 0,
+                g->V / kHubListShare,
+                (long long)(g->E / kHubListShare)};
+    w->host_flag[0] = 0;  // no run of this graph's engine is in flight: set again by the final decide
     ResetArgs r;
     r.depth = depth;
     r.depth_len = depth_len >= 0 && depth_len < g->V ? depth_len : g->V;
@@ -1203,11 +1303,8 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
         int sg = (int)ceil_div(n_sources, 256);
         bfs_seed_sources_kernel<<<sg < 1024 ? sg : 1024, 256, 0, s>>>(w->st, w->d_src, n_sources, 0, w->bm[0],
                                                                       w->sum[0], g->d_visited, g->d_off);
-    } else {
-        bfs_seed_sources_kernel<<<1, 64, 0, s>>>(w->st, nullptr, 1, single, w->bm[0], w->sum[0], g->d_visited,
-                                                 g->d_off);
     }
-    launch_decide(g, w, s);
+    launch_decide(g, w, s, h_sources ? -1 : single);  // a single source is seeded by this decide
     GI_LAUNCH_CHECK();
     if (h_sources) GI_HIP(hipStreamSynchronize(s));  // h_sources may go away on return
     return GI_OK;
@@ -1268,11 +1365,11 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
         return fail(GI_ERR_HIP, "BFS state never reported done");
     }
     // Graph mode: replay a recorded block of levels sized by the deepest of
-    // the last runs on this graph (a level past the end costs three no-op
-    // launches, a block too short a whole extra replay), read {done, level,
-    // levels} once per replay (the host waits for that copy only, while the
-    // epilogue is already queued), continue with blocks of 8, 16, 32, 48
-    // levels for deeper runs.
+    // the last runs on this graph (a level past the end costs one step launch
+    // that returns at once, a block too short a whole extra replay), wait for
+    // each replay's event (the epilogue is already queued behind it) and read
+    // the {done, level, levels} the final decide wrote to pinned memory,
+    // continue with blocks of 8, 16, 32, 48 levels for deeper runs.
     int32_t block = 0;
     for (int32_t l : w->recent_levels) block = l > block ? l : block;
     if (block <= 0) block = 8;
@@ -1284,7 +1381,8 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
         GI_TRY(level_graph(g, w, block, &exec));
         GI_HIP(hipGraphLaunch(exec, s));
         level += block;
-        GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        // the decide that ends the run writes {done, level, levels}
+        // into the pinned host_flag itself (the host cleared done before the run)
         GI_HIP(hipEventRecord(w->flag_ev, s));
         if (epilogue) GI_TRY((*epilogue)(s));
         GI_HIP(hipEventSynchronize(w->flag_ev));
@@ -1336,7 +1434,7 @@ int fused_reached(gi_graph *g, const int32_t *depth, int32_t *out, hipStream_t s
     FusedWork *w = nullptr;
     GI_TRY(ensure_fused(g, &w));
     GI_TRY(fill_i32(w->reached, 1, 0, s));
-    count_reached_fused_kernel<<<push_grid(w), 256, 0, s>>>(depth, g->V, w->reached);
+    count_reached_fused_kernel<<<step_grid(w), 256, 0, s>>>(depth, g->V, w->reached);
     GI_LAUNCH_CHECK();
     return read_count(w->reached, out, s);
 }
@@ -1356,11 +1454,11 @@ int fused_set_bins(gi_graph *g, int32_t lane_max, int32_t hub_min, int32_t block
 // ---------------------------------------------------------------- sweep
 __global__ void bfs_force_mode_kernel(BfsState *st, int32_t mode) {
     st->mode = mode;
-    st->hub_count = 0;
+    st->hub_n[(st->level + 1) & 1] = 0;  // every replay lists the hubs of F' afresh
 }
 
 __global__ void zero_slots_kernel(int32_t *pcount, unsigned long long *pfdeg) {
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * kMaxGrid; i += gridDim.x * blockDim.x) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kMaxGrid; i += gridDim.x * blockDim.x) {
         pcount[i] = 0;
         pfdeg[i] = 0;
     }
@@ -1451,9 +1549,9 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
     StepArgs a = step_args(g, w);
     int32_t nl = 0, prev_mode = BFS_PUSH_THREAD, prev_n = 0;
     // One level: snapshot the state, time every mode of the mask from it with
-    // the production launches (bfs_step + bfs_hub_kernel, as a recorded level
-    // has them), then advance by the followed mode. Returns 1 when the BFS
-    // is done; errors return through rc so the buffers below are freed.
+    // the production step kernel (bfs_step), then advance by the followed
+    // mode. Returns 1 when the BFS is done; errors return
+    // through rc so the buffers below are freed.
     auto one_level = [&]() -> int {
         SweepHost h;
         GI_HIP(hipMemcpyAsync(&h, w->st, sizeof(h), hipMemcpyDeviceToHost, s));
@@ -1486,7 +1584,6 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
                 bfs_force_mode_kernel<<<1, 1, 0, s>>>(w->st, run);
                 GI_HIP(hipEventRecord(e0, s));
                 bfs_step<<<step_grid(w), kStepBlock, 0, s>>>(a);
-                bfs_hub_kernel<<<push_grid(w), kStepBlock, 0, s>>>(a);
                 GI_HIP(hipEventRecord(e1, s));
                 GI_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;

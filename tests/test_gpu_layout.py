@@ -173,3 +173,27 @@ def test_layout_bfs_deep_and_shallow(gi):
             want = O.bfs_depth(host, int(s0))
             assert np.array_equal(depth, want), s0
     assert O.bfs_depth(host, n_rand + n_path - 1).max() > 255
+
+
+def test_layout_bfs_runs_leave_no_stale_depths(gi):
+    # two components of different sizes: the view's depths are not cleared
+    # between runs, so a run from one component must report the other as
+    # unreached although an earlier run wrote depths there
+    rng = np.random.default_rng(21)
+    na, nb = 5000, 3000
+    a = rng.integers(0, na, 20000)
+    b = rng.integers(0, na, 20000)
+    c = na + rng.integers(0, nb, 12000)
+    d = na + rng.integers(0, nb, 12000)
+    src = np.concatenate([a, c]).astype(np.int32)
+    dst = np.concatenate([b, d]).astype(np.int32)
+    n = na + nb + 50  # and 50 isolated vertices
+    g = gi.Graph.from_edges(n, src, dst, True)
+    host = O.build_csr(n, src, dst, True)
+    g.set_layout(gi.LAYOUT_DEGREE)
+    sa, sb = int(a[0]), int(c[0])
+    for s in (sa, sb, sa, n - 1, sb):
+        depth, _, _ = gi.bfs(g, s, fused(gi))
+        want = O.bfs_depth(host, s)
+        assert np.array_equal(depth, want), s
+        assert (depth[:na] >= 0).any() == (s < na) and (depth[na:na + nb] >= 0).any() == (na <= s < na + nb)
