@@ -31,37 +31,17 @@ constexpr int kSelMaxOut = 8;
 // of one wavefront per list.
 constexpr int32_t kHubDegree = 512;
 constexpr int32_t kMaxHubs = 1024;
-constexpr int32_t kMaxGrid = 8192;          // partial-sum slots per kernel
+// Giant hubs (degree >= kGiantFactor * the hub threshold) are listed in a
+// list of their own, kept in every step: they are few, and one wavefront
+// walking a list of 10^5..10^6 edges (where the hub list was not kept or
+// overflowed) would take milliseconds.
+constexpr int32_t kGiantFactor = 64;
+constexpr int32_t kMaxGiants = 512;
+constexpr int32_t kMaxGrid = 8192;          // workgroups of a step launch at most
 constexpr int32_t kMaxSumWords = 256;       // summary words (V up to 2^26)
 
-struct BfsState {
-    int32_t count[2];               // frontier sizes by level parity
-    unsigned long long fdeg[2];     // sum of frontier degrees by parity
-    int32_t done;
-    int32_t level;                  // level of the frontier the next step expands
-    int32_t levels;                 // BFS depth + 1 when done
-    int32_t mode;                   // mode of the next step
-    long long unvisited;            // vertices not reached yet
-    long long unvisited_edges;      // sum of their degrees
-    int32_t hub_n[2];               // listed hubs of F_level (level & 1) and of F_level+1
-    int32_t want_hubs;              // list hubs at discovery (the selector may pick TWC)
-    int32_t hub_list_count;         // ... during push steps of at most this many vertices
-    long long hub_list_fdeg;        // ... and at most this many edges (when degrees are summed)
-    int32_t lane_max;               // push: degree < lane_max walked by one lane
-    int32_t hub_min;                // push TWC: degree >= hub_min goes to the grid bin
-    int32_t want_fdeg;              // accumulate frontier degrees (learned selector, sweeps)
-    int32_t pull_levels;            // levels expanded by a pull step so far
-    int32_t *depth;                 // output arrays of the current run
-    int32_t *parent;
-    int32_t hubs[2][kMaxHubs];      // by level parity; more than kMaxHubs: the list is void
-    // per-level history of the first kBfsMaxLevels levels (traces and
-    // statistics); deeper runs go on, unrecorded
-    int32_t hist_mode[kBfsMaxLevels];
-    int32_t hist_next[kBfsMaxLevels];
-};
-
-// Device view of the selector network (BF16 weights, FP32 accumulation).
-// Passed to the decides by value. The BF16 weights are stored one row
+// Device view of the selector network (BF16 weights, FP32 accumulation),
+// kept in the run's device state. The BF16 weights are stored one row
 // per hidden unit, so that a lane (one hidden unit) reads its own weights
 // with 16-byte loads: w1 [hid][in], w2 [hid][hid] (row = the unit fed), w3
 // [hid][kSelMaxOut] (row = the unit feeding, zero-padded), biases b1 [hid],
@@ -76,6 +56,55 @@ struct SelectorView {
     float threshold;
     float feat_mean[kSelInputs];   // feature normalisation
     float feat_scale[kSelInputs];
+};
+
+// One level's bookkeeping, written by the first workgroup of the level's
+// step launch and read by the next launch.
+struct BfsLevel {
+    int32_t level;
+    int32_t mode;                   // mode the level's step ran
+    int32_t count;                  // |F_level|
+    int32_t pull_levels;            // pull steps up to and including this one
+    unsigned long long fdeg;        // sum of the degrees of F_level
+    long long unvisited;            // vertices not reached once F_level is
+    long long unvisited_edges;      // the sum of their degrees
+};
+
+struct BfsState {
+    int32_t done;
+    int32_t levels;                 // BFS depth + 1 when done
+    int32_t pull_levels;            // levels expanded by a pull step, when done
+    int32_t force_mode;             // >= 0: the mode of the next step (traced levels, sweeps)
+    int32_t base;                   // level of the first launch of a replayed block of levels
+    int32_t lane_max;               // push: degree < lane_max walked by one lane
+    int32_t hub_min;                // push TWC: degree >= hub_min goes to the grid bin
+    int32_t want_fdeg;              // accumulate frontier degrees (learned selector, sweeps)
+    int32_t want_hubs;              // list hubs at discovery (the selector may pick TWC)
+    int32_t hub_list_count;         // ... during push steps of at most this many vertices
+    long long hub_list_fdeg;        // ... and at most this many edges (when degrees are summed)
+    int32_t *depth;                 // output arrays of the current run
+    int32_t *parent;
+    SelectorView sel;               // the run's selector (the reset copies it here)
+    BfsLevel lv[2];                 // by level parity; lv[1] holds level -1 after the reset
+    int32_t hub_n[3];               // listed hubs of F_l in hub_n[l % 3]; more than kMaxHubs: void
+    int32_t hubs[3][kMaxHubs];      // (below the giant threshold)
+    int32_t giant_n[3];             // listed giant hubs of F_l; more than kMaxGiants: void
+    int32_t giants[3][kMaxGiants];
+    // per-level history of the first kBfsMaxLevels levels (traces and
+    // statistics); deeper runs go on, unrecorded
+    int32_t hist_mode[kBfsMaxLevels];
+    int32_t hist_next[kBfsMaxLevels];
+};
+
+// A level's selector inputs as the step launch sees them (sweeps).
+struct BfsPeek {
+    int32_t n;
+    int32_t prev_mode;
+    int32_t prev_n;
+    int32_t level;
+    unsigned long long fdeg;
+    long long unvisited;
+    long long unvisited_edges;
 };
 
 // The selector's 16 input features from the run's counters, in double: the

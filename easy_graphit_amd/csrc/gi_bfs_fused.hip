@@ -50,10 +50,18 @@ __device__ __forceinline__ int32_t deg_of(const int32_t *off, int32_t v) {
 }
 
 constexpr int kStepBlock = 256;
-constexpr int kDecideBlock = 1024;  // one workgroup: few loads of the slot rows per thread
 constexpr int kChunkWords = 64;                  // chunk: 64 words = 4096 vertices
 constexpr int kChunkVerts = kChunkWords * 64;
 constexpr int kCandScan = 128;                   // in-neighbours examined per candidate
+static_assert(kChunkWords == kBsChunkWords, "chunk flags mark the step's chunks");
+// A level's |F| and degree sum are added up in kStripes stripes, each on a
+// 128-byte line of its own (one atomic per workgroup and counter, spread
+// over the lines by workgroup), read back by the next launch.
+constexpr int kStripes = 64;                     // one per lane of the reading wave
+constexpr int kStripeI32 = 32;
+constexpr int kStripeU64 = 16;
+constexpr int kCountSets = 3;                    // by level % 3: read, written, cleared
+constexpr int kFlagSets = 4;                     // by level % 4 (see level_body)
 
 struct StepArgs {
     BfsState *st;
@@ -64,17 +72,18 @@ struct StepArgs {
     const int32_t *cand;        // candidate parent per vertex, -1 when none
     int32_t cand_first;         // 1: the candidate is the vertex's first in-neighbour
     int32_t V;
+    int64_t E;
     int32_t words;
     int32_t chunks;
-    int32_t sum_words;
     int32_t *depth;
     int32_t *parent;
     uint64_t *visited;
-    uint64_t *bm[3];
-    uint64_t *sum[3];
-    uint8_t *cflag;             // chunk flags of the output bitmap (bs_flag_chunk)
-    int32_t *pcount;            // [kMaxGrid]: one slot per step workgroup
-    unsigned long long *pfdeg;  // same layout
+    uint64_t *bm[3];            // F_l in bm[l % 3]
+    uint8_t *flags;             // kFlagSets sets of chunk flags (bs_flag_chunk), flag_bytes apart:
+    int32_t flag_bytes;         // those of F_l in set (l - 1) % 4
+    int32_t *cnt;               // [kCountSets][kStripes] lines: |F_l| in set (l - 1) % 3
+    unsigned long long *fdg;    // the same for the degree sums
+    int32_t *host_flag;         // pinned {done, level, levels}: set when a run ends
 };
 
 struct StepCtx {
@@ -83,8 +92,12 @@ struct StepCtx {
     int32_t hub_min;           // TWC: degree >= hub_min goes to the grid bin
     int32_t want_fdeg;
     int32_t want_hubs;         // list the hubs of F' as they are discovered
-    int32_t hub_out;           // their list: hubs[(level + 1) & 1]
+    int32_t want_giants;       // list the giant hubs of F' (every step of a run that may use TWC)
+    int32_t giant_min;         // degree of a giant hub
+    int32_t hub_in;            // the hub lists of F: hubs[level % 3], giants[level % 3]
+    int32_t hub_out;           // the lists of F': [(level + 1) % 3]
     int32_t hubs_listed;       // TWC step whose hubs (of F) are all listed: walked grid-wide first
+    int32_t giants_listed;     // TWC step whose giant hubs are all listed: likewise
     const uint64_t *bm_in;
     uint64_t *bm_out;
     uint8_t *cflag;
@@ -99,6 +112,10 @@ __device__ __forceinline__ T pick3(T p0, T p1, T p2, int32_t i) {
     return i == 0 ? p0 : (i == 1 ? p1 : p2);
 }
 
+__device__ __forceinline__ uint8_t *flag_set(const StepArgs &a, int32_t i) {
+    return a.flags + (size_t)i * a.flag_bytes;
+}
+
 template <class T>
 __device__ __forceinline__ T pick4(const T *x, int32_t i) {
     return i == 0 ? x[0] : (i == 1 ? x[1] : (i == 2 ? x[2] : x[3]));
@@ -111,23 +128,22 @@ __device__ __forceinline__ bool in_bitmap(const uint64_t *bm, int32_t v) {
 // A discovered vertex's degree, read when the frontier degree sum or the hub
 // list needs it.
 __device__ __forceinline__ int32_t note_degree(const StepArgs &a, StepCtx &c, int32_t d) {
-    if (!c.want_fdeg && !c.want_hubs) return 0;
+    if (!c.want_fdeg && !c.want_hubs && !c.want_giants) return 0;
     const int32_t dd = deg_of(a.off, d);
     if (c.want_fdeg) c.fdeg += (unsigned long long)dd;
     return dd;
 }
 
-// Appends d to the hub list of F' (one lane).
-__device__ __forceinline__ void note_hub(const StepArgs &a, const StepCtx &c, int32_t d) {
-    const int32_t p = atomicAdd(&a.st->hub_n[c.hub_out], 1);
-    if (p < kMaxHubs) a.st->hubs[c.hub_out][p] = d;
-}
-
-// Appends the lanes' hubs to the list of F' with one atomic per wave (every
-// lane of the wave calls it).
-__device__ __forceinline__ void note_hubs_wave(const StepArgs &a, const StepCtx &c, int32_t d, bool hub) {
-    const int32_t p = wave_atomic_add(&a.st->hub_n[c.hub_out], hub);
-    if (hub && p < kMaxHubs) a.st->hubs[c.hub_out][p] = d;
+// Appends d (degree dd) to the giant or the hub list of F', as the step
+// keeps them (one lane; listed hubs are rare, one atomic each).
+__device__ __forceinline__ void note_hub(const StepArgs &a, const StepCtx &c, int32_t d, int32_t dd) {
+    if (c.want_giants && dd >= c.giant_min) {
+        const int32_t p = atomicAdd(&a.st->giant_n[c.hub_out], 1);
+        if (p < kMaxGiants) a.st->giants[c.hub_out][p] = d;
+    } else if (c.want_hubs && dd >= c.hub_min && dd < c.giant_min) {
+        const int32_t p = atomicAdd(&a.st->hub_n[c.hub_out], 1);
+        if (p < kMaxHubs) a.st->hubs[c.hub_out][p] = d;
+    }
 }
 
 // Push discovery: test, claim in `visited`, record in the output bitmap.
@@ -145,7 +161,7 @@ __device__ __forceinline__ void push_edge(const StepArgs &a, StepCtx &c, int32_t
     if (a.parent) a.parent[d] = s;  // the depth is written in word order by the next step
     c.cnt += 1;
     const int32_t dd = note_degree(a, c, d);
-    if (c.want_hubs && dd >= c.hub_min) note_hub(a, c, d);
+    if (dd >= c.hub_min) note_hub(a, c, d, dd);
 }
 
 // Push discovery by a whole wavefront at once (every lane calls it; `active`
@@ -188,14 +204,12 @@ __device__ __forceinline__ void push_edge_wave(const StepArgs &a, StepCtx &c, in
     // run is recorded once, by its first lane
     const uint64_t xp = __shfl_up(x, 1);
     const uint64_t before = lane - 1 >= my_head ? xp : 0ull;
-    const bool mine = want && (run_won & bit) && !(before & bit);
-    int32_t dd = 0;
-    if (mine) {
+    if (want && (run_won & bit) && !(before & bit)) {
         if (a.parent) a.parent[d] = s;  // the depth is written in word order by the next step
         c.cnt += 1;
-        dd = note_degree(a, c, d);
+        const int32_t dd = note_degree(a, c, d);
+        if (dd >= c.hub_min) note_hub(a, c, d, dd);  // rare: one atomic per listed hub
     }
-    if (c.want_hubs) note_hubs_wave(a, c, d, mine && dd >= c.hub_min);  // uniform
 }
 
 __device__ __forceinline__ void lane_walk(const StepArgs &a, StepCtx &c, int32_t s, int32_t e,
@@ -209,9 +223,8 @@ __device__ __forceinline__ void wave_walk(const StepArgs &a, StepCtx &c, int32_t
     for (int32_t b = e0; b < e1; b += 64) push_edge_wave(a, c, s, b + lane, b + lane < e1);
 }
 
-// Workgroup reduction of the lane counters into this block's slot.
-__device__ __forceinline__ void flush_counts(int32_t *pcount, unsigned long long *pfdeg,
-                                             StepCtx &c) {
+// Workgroup reduction of the lane counters into the level's stripes.
+__device__ __forceinline__ void flush_counts(const StepArgs &a, int32_t set, StepCtx &c) {
     __shared__ int32_t wc[kStepBlock / 64];
     __shared__ unsigned long long wf[kStepBlock / 64];
     int32_t cnt = c.cnt;
@@ -232,43 +245,10 @@ __device__ __forceinline__ void flush_counts(int32_t *pcount, unsigned long long
             tc += wc[i];
             tf += wf[i];
         }
-        pcount[blockIdx.x] = tc;
-        pfdeg[blockIdx.x] = tf;
+        const int32_t line = set * kStripes + (int32_t)(blockIdx.x % kStripes);
+        if (tc) atomicAdd(&a.cnt[line * kStripeI32], tc);
+        if (tf) atomicAdd(&a.fdg[line * kStripeU64], tf);
     }
-}
-
-// Prefix of set bits over the summary words (LDS), for k-th-chunk lookup.
-// Returns the number of non-empty chunks. Every thread of the block calls it.
-__device__ int32_t summary_prefix(const uint64_t *sum, int32_t sum_words, int32_t *pre) {
-    __shared__ int32_t total_s;
-    // one wave scans 64 words at a time
-    if (threadIdx.x < 64) {
-        int32_t carry = 0;
-        int32_t lane = lane_id();
-        for (int32_t base = 0; base < sum_words; base += 64) {
-            int32_t j = base + lane;
-            int32_t pc = j < sum_words ? __popcll(sum[j]) : 0;
-            int32_t incl = pc;
-            for (int o = 1; o < 64; o <<= 1) {
-                int32_t y = __shfl_up(incl, o);
-                if (lane >= o) incl += y;
-            }
-            if (j < sum_words) pre[j] = carry + incl - pc;
-            carry += __shfl(incl, 63);
-        }
-        if (lane == 0) total_s = carry;
-    }
-    __syncthreads();
-    return total_s;
-}
-
-// Chunk id of the k-th non-empty chunk.
-__device__ __forceinline__ int32_t kth_chunk(const uint64_t *sum, const int32_t *pre,
-                                             int32_t sum_words, int32_t k) {
-    int32_t j = upper_bound_i32(pre, 0, sum_words, k) - 1;
-    uint64_t w = sum[j];
-    for (int32_t r = k - pre[j]; r > 0; --r) w &= w - 1;
-    return j * 64 + (__ffsll((unsigned long long)w) - 1);
 }
 
 // One batch (<= kStepBlock vertices of the block's LDS list) in push mode.
@@ -290,7 +270,8 @@ __device__ void push_batch(const StepArgs &a, StepCtx &c, int32_t mode, const in
     } else if (mode == BFS_PUSH_TWC) {
         // lane bin, wave bin; the grid bin (listed hubs) was walked at the
         // start of the step, unless the list overflowed
-        const bool todo = has && !(c.hubs_listed && dg >= c.hub_min);
+        const bool todo = has && !(c.giants_listed && dg >= c.giant_min) &&
+                          !(c.hubs_listed && dg >= c.hub_min && dg < c.giant_min);
         bool wide = todo && dg >= c.lane_max;
         uint64_t wm = __ballot(wide);
         while (wm) {
@@ -448,7 +429,7 @@ __device__ __forceinline__ bool pull_words(const StepArgs &a, StepCtx &c, int32_
         if (mode != BFS_PULL_THREAD && __ballot(ei < e1i)) coop = pull_coop(a, c, w, ei, e1i);
         const int32_t v = w * 64 + lane;
         const bool found_self = pi >= 0 || ((coop >> lane) & 1ull);
-        if (found_self) {  // (pull steps list no hubs)
+        if (found_self) {  // (pull steps list no hubs: giant hubs are found by the first, push levels)
             if (a.parent && pi >= 0) a.parent[v] = pi;
             if (c.want_fdeg) c.fdeg += (unsigned long long)deg_of(a.off, v);
         }
@@ -466,20 +447,23 @@ __device__ __forceinline__ bool pull_words(const StepArgs &a, StepCtx &c, int32_
 }
 
 // Grid bin of PUSH_TWC, at the start of the step: the edges of F's listed
-// hubs (degree >= hub_min, listed when they were discovered) spread evenly
-// over the grid. Every workgroup scans the hub degrees into LDS (`lds`: the
-// step's vertex list, free at this point) and walks its share of the
-// concatenated edge range. Every thread of the workgroup calls it.
-static_assert(kChunkVerts >= 2 * kMaxHubs + 1, "hub scan fits the vertex list");
+// hubs (degree >= hub_min, listed when they were discovered: the giant
+// ones, then the others) spread evenly over the grid. Every workgroup scans
+// the hub degrees into LDS (`lds`: the step's vertex list, free at this
+// point) and walks its share of the concatenated edge range. Every thread of
+// the workgroup calls it.
+static_assert(kChunkVerts >= 2 * (kMaxHubs + kMaxGiants) + 1, "hub scan fits the vertex list");
 constexpr int32_t kHubListShare = 64;  // list hubs during push steps of <= V/64 vertices, <= E/64 edges
-__device__ __forceinline__ void hub_phase(const StepArgs &a, StepCtx &c, int32_t nh, int32_t *lds) {
-    const int32_t *hubs = a.st->hubs[c.level & 1];
-    int32_t *hv = lds, *hscan = lds + kMaxHubs;
+__device__ __forceinline__ void hub_phase(const StepArgs &a, StepCtx &c, int32_t ng, int32_t nhub, int32_t *lds) {
+    const int32_t *giants = a.st->giants[c.hub_in];
+    const int32_t *hubs = a.st->hubs[c.hub_in];
+    const int32_t nh = ng + nhub;
+    int32_t *hv = lds, *hscan = lds + kMaxHubs + kMaxGiants;
     const int32_t per = (nh + kStepBlock - 1) / kStepBlock;
     const int32_t lo = threadIdx.x * per, hi = lo + per < nh ? lo + per : nh;
     int32_t part = 0;
     for (int32_t i = lo; i < hi; ++i) {
-        const int32_t s = hubs[i];
+        const int32_t s = i < ng ? giants[i] : hubs[i - ng];
         hv[i] = s;
         part += deg_of(a.off, s);
     }
@@ -507,68 +491,70 @@ __device__ __forceinline__ void hub_phase(const StepArgs &a, StepCtx &c, int32_t
 }
 
 // ---------------------------------------------------------------- step
-// One level's step: the push or the pull body, as the state's mode says.
-// `pre` is LDS of kMaxSumWords + 1 words owned by the calling kernel, so a
-// kernel that holds both step bodies allocates it once.
+// Step l: F_l = bm[l % 3] expanded by the push or the pull body into
+// F_l+1 = bm[(l + 1) % 3], whose chunk flags go to flags[l % 4]; bm[(l + 2)
+// % 3] (F_l-1, done with) is cleared on the way. Chunk c belongs to
+// workgroup c % G (G the grid): a workgroup clears and lists only its own
+// chunks, from the bit masks `fl_clear` (flags of F_l-1) and `fl_list`
+// (flags of F_l) of its chunks, bit j for chunk blockIdx + j G, which the
+// caller loaded with the level's other inputs. `list` is LDS of the
+// calling kernel. `ph` is l mod 12 (the rotations' indices).
 enum StepKind : int { kPushStep = 0, kPullStep = 1 };
 
 template <int Kind>
-__device__ __forceinline__ void step_body(const StepArgs &a0, int32_t *pre) {
+__device__ __forceinline__ void step_body(const StepArgs &a0, int32_t *list, int32_t l, int32_t ph, int32_t mode,
+                                          int32_t n, unsigned long long nfdeg, uint64_t fl_clear,
+                                          uint64_t fl_list) {
     BfsState *st = a0.st;
     StepArgs a = a0;
     a.depth = st->depth;
     a.parent = st->parent;
+    const int32_t p3 = ph % 3;
     StepCtx c;
-    c.level = st->level;
+    c.level = l;
     c.lane_max = st->lane_max;
     c.hub_min = st->hub_min;
     c.want_fdeg = st->want_fdeg;
-    const int32_t mode = st->mode;
     // F''s hubs are listed during small push steps only: after a large or a
     // pull step the next step is a pull step or has too many hubs for the
     // list (and the same-address atomics of listing them would cost more
     // than the list saves); such a step voids the list instead
-    {
-        const int32_t p = c.level & 1;
-        c.want_hubs = st->want_hubs && mode < BFS_PULL_THREAD && st->count[p] <= st->hub_list_count &&
-                      (!c.want_fdeg || (long long)st->fdeg[p] <= st->hub_list_fdeg);
-    }
-    c.hub_out = (c.level + 1) & 1;
+    c.want_hubs = st->want_hubs && mode < BFS_PULL_THREAD && n <= st->hub_list_count &&
+                  (!c.want_fdeg || (long long)nfdeg <= st->hub_list_fdeg);
+    c.want_giants = st->want_hubs && mode < BFS_PULL_THREAD;
+    c.giant_min = st->hub_min * kGiantFactor;
+    c.hub_in = p3;
+    c.hub_out = (p3 + 1) % 3;
     if (!c.want_hubs && blockIdx.x == 0 && threadIdx.x == 0) st->hub_n[c.hub_out] = kMaxHubs + 1;
-    const int32_t nh = st->hub_n[c.level & 1];
+    const int32_t nh = st->hub_n[p3], ng = st->giant_n[p3];
     c.hubs_listed = mode == BFS_PUSH_TWC && st->want_hubs && nh <= kMaxHubs;  // else void or never kept
-    const int32_t iin = c.level % 3, iout = (c.level + 1) % 3, iclr = (c.level + 2) % 3;
-    c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iin);
-    c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iout);
-    c.cflag = a0.cflag;
+    c.giants_listed = mode == BFS_PUSH_TWC && st->want_hubs && ng <= kMaxGiants;
+    c.bm_in = pick3(a0.bm[0], a0.bm[1], a0.bm[2], p3);
+    c.bm_out = pick3(a0.bm[0], a0.bm[1], a0.bm[2], (p3 + 1) % 3);
+    c.cflag = flag_set(a0, ph & 3);
     c.cnt = 0;
     c.fdeg = 0;
     const int32_t lane = lane_id();
 
-    // the bitmap two levels ahead is free: clear its non-empty chunks (its
-    // summary is cleared by the decide after this step)
+    // the bitmap two levels ahead is free: clear this workgroup's non-empty
+    // chunks of it
     {
-        const uint64_t *sc = pick3(a0.sum[0], a0.sum[1], a0.sum[2], iclr);
-        uint64_t *bc = pick3(a0.bm[0], a0.bm[1], a0.bm[2], iclr);
-        int32_t nz = summary_prefix(sc, a.sum_words, pre);
-        for (int32_t k = blockIdx.x; k < nz; k += gridDim.x) {
-            int32_t ch = kth_chunk(sc, pre, a.sum_words, k);
+        uint64_t *bc = pick3(a0.bm[0], a0.bm[1], a0.bm[2], (p3 + 2) % 3);
+        for (uint64_t m = fl_clear; m; m &= m - 1) {  // uniform
+            const int32_t ch = (int32_t)blockIdx.x + (__ffsll((unsigned long long)m) - 1) * (int32_t)gridDim.x;
             if (threadIdx.x < kChunkWords) {
-                int32_t w = ch * kChunkWords + (int32_t)threadIdx.x;
+                const int32_t w = ch * kChunkWords + (int32_t)threadIdx.x;
                 if (w < a.words) bc[w] = 0;
             }
         }
-        __syncthreads();
     }
 
     if constexpr (Kind == kPushStep) {
-        __shared__ int32_t list[kChunkVerts];
         __shared__ int32_t n_s;
-        if (c.hubs_listed && nh > 0) hub_phase(a, c, nh, list);  // uniform
-        const uint64_t *si = pick3(a0.sum[0], a0.sum[1], a0.sum[2], iin);
-        int32_t nz = summary_prefix(si, a.sum_words, pre);
-        for (int32_t k = blockIdx.x; k < nz; k += gridDim.x) {
-            int32_t chunk = kth_chunk(si, pre, a.sum_words, k);
+        const int32_t nhw = c.hubs_listed ? nh : 0, ngw = c.giants_listed ? ng : 0;
+        if (nhw + ngw > 0) hub_phase(a, c, ngw, nhw, list);  // uniform
+        for (uint64_t m = fl_list; m; m &= m - 1) {  // uniform
+            const int32_t chunk = (int32_t)blockIdx.x + (__ffsll((unsigned long long)m) - 1) * (int32_t)gridDim.x;
             if (threadIdx.x < 64) {
                 // wave 0 lists the chunk's members
                 int32_t w = chunk * kChunkWords + (int32_t)threadIdx.x;
@@ -589,15 +575,15 @@ __device__ __forceinline__ void step_body(const StepArgs &a0, int32_t *pre) {
                 if (threadIdx.x == 63) n_s = incl;
             }
             __syncthreads();
-            const int32_t n = n_s;
-            for (int32_t b0 = 0; b0 < n; b0 += kStepBlock)
-                push_batch(a, c, mode, list + b0, n - b0 < kStepBlock ? n - b0 : kStepBlock);
+            const int32_t nl = n_s;
+            for (int32_t b0 = 0; b0 < nl; b0 += kStepBlock)
+                push_batch(a, c, mode, list + b0, nl - b0 < kStepBlock ? nl - b0 : kStepBlock);
             __syncthreads();
         }
     } else {
         // pull over all vertices in units of kPullUnit words, one unit per
         // workgroup at a time (each wave takes kPullWords of them), so a
-        // unit's summary bit costs at most one atomic
+        // unit's chunk flag costs at most one store
         const int32_t units = (a.words + kPullUnit - 1) / kPullUnit;
         const int32_t wid = threadIdx.x >> 6;
         for (int32_t un = blockIdx.x; un < units; un += gridDim.x) {
@@ -618,36 +604,13 @@ __device__ __forceinline__ void step_body(const StepArgs &a0, int32_t *pre) {
             if (__syncthreads_or(any) && threadIdx.x == 0) bs_flag_chunk(c.cflag, un * kPullUnit);
         }
     }
-    flush_counts(a.pcount, a.pfdeg, c);
+    flush_counts(a, p3, c);
 }
 
 // ---------------------------------------------------------------- decide
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
     return __uint_as_float(((uint32_t)h) << 16);
 }
-
-// The decide kernel: one workgroup, after step `level` (or after seeding,
-// level = -1), adds up the step's partial sums, closes its bookkeeping, ends
-// the run on an empty frontier, else picks the next step's mode and clears
-// what the next step writes. The MLP runs on wave 0; every thread passes the
-// same barriers.
-struct DecideArgs {
-    BfsState *st;
-    int32_t V;
-    int64_t E;
-    int32_t *pcount;
-    unsigned long long *pfdeg;
-    int32_t slots;          // step workgroups that wrote a slot
-    uint64_t *sum[3];
-    int32_t sum_words;
-    uint8_t *cflag;
-    int32_t chunks;
-    int32_t *host_flag;     // pinned: {done, level, levels} when the run ends
-    const int32_t *off;
-    uint64_t *visited;
-    uint64_t *bm0;          // the bitmap of F_0
-    int32_t seed;           // the decide after the reset: a single source to seed, or -1
-};
 
 // The selector's 16 inputs in FP32 with the hardware log2 (v_log_f32): the
 // values of bfs_raw_feature (double, used by the host to record training
@@ -656,24 +619,26 @@ struct DecideArgs {
 __device__ __forceinline__ void bfs_features_f32(float *f, int32_t n, unsigned long long fdeg, long long unv,
                                                  long long unv_e, int32_t level, int32_t prev_mode,
                                                  int32_t prev_n, int32_t V, int64_t E) {
+    // divisions as reciprocal-multiplies (__fdividef, a few ulp): the network
+    // runs at the start of every step, on its critical path
     const float dV = V > 0 ? (float)V : 1.f, dE = E > 0 ? (float)E : 1.f;
     const float nn = (float)n, fd = (float)fdeg, u = (float)unv, ue = (float)unv_e;
     const float l_n = __log2f(1.f + nn);
-    f[0] = nn / dV;
-    f[1] = fd / dE;
-    f[2] = u / dV;
-    f[3] = ue / dE;
-    f[4] = l_n / 32.f;
-    f[5] = __log2f(1.f + fd) / 40.f;
-    f[6] = __log2f(1.f + fd / (n > 0 ? nn : 1.f)) / 20.f;
-    f[7] = __log2f(1.f + dE / dV) / 20.f;
-    f[8] = __log2f(dV) / 32.f;
-    f[9] = level < 64 ? (float)level / 64.f : 1.f;
+    f[0] = __fdividef(nn, dV);
+    f[1] = __fdividef(fd, dE);
+    f[2] = __fdividef(u, dV);
+    f[3] = __fdividef(ue, dE);
+    f[4] = l_n * (1.f / 32.f);
+    f[5] = __log2f(1.f + fd) * (1.f / 40.f);
+    f[6] = __log2f(1.f + __fdividef(fd, n > 0 ? nn : 1.f)) * (1.f / 20.f);
+    f[7] = __log2f(1.f + __fdividef(dE, dV)) * (1.f / 20.f);
+    f[8] = __log2f(dV) * (1.f / 32.f);
+    f[9] = level < 64 ? (float)level * (1.f / 64.f) : 1.f;
     f[10] = prev_mode >= BFS_PULL_THREAD ? 1.f : 0.f;
-    f[11] = (l_n - __log2f(1.f + (float)prev_n)) / 16.f;
-    f[12] = fd / (1.f + ue);
-    f[13] = nn / (1.f + u);
-    f[14] = __log2f(1.f + u) / 32.f;
+    f[11] = (l_n - __log2f(1.f + (float)prev_n)) * (1.f / 16.f);
+    f[12] = __fdividef(fd, 1.f + ue);
+    f[13] = __fdividef(nn, 1.f + u);
+    f[14] = __log2f(1.f + u) * (1.f / 32.f);
     f[15] = 1.f;
 }
 
@@ -685,133 +650,92 @@ __device__ __forceinline__ float bf16_in(const uint4 *q, int k) {
     return __uint_as_float((k & 1) ? (word & 0xffff0000u) : (word << 16));
 }
 
-template <int Block>
-__device__ __forceinline__ void decide_body(const DecideArgs &d, const SelectorView &sel) {
-    BfsState *st = d.st;
-    if (st->done) return;  // uniform
-    const int32_t V = d.V;
-    const int64_t E = d.E;
-    int32_t *pcount = d.pcount;
-    unsigned long long *pfdeg = d.pfdeg;
-    int32_t *host_flag = d.host_flag;
-    // after step `level`, the bitmap two levels ahead of it was cleared by the
-    // step; its summary is cleared here
-    uint64_t *sum_clear = pick3(d.sum[0], d.sum[1], d.sum[2], (st->level + 2 + 3) % 3);
+// |F_l| and the sum of its degrees, from the stripes the previous launch
+// (or the seed) added them up in, count set `set` = (l - 1) % 3: lane i
+// reads stripe i. Every lane of the calling wave gets the totals.
+__device__ __forceinline__ void level_counts(const StepArgs &a, int32_t set, int32_t *n, unsigned long long *fdeg) {
+    static_assert(kStripes == 64, "one stripe per lane");
+    const int32_t line = set * kStripes + lane_id();
+    int32_t c = a.cnt[line * kStripeI32];
+    unsigned long long f = a.fdg[line * kStripeU64];
+    for (int o = 32; o > 0; o >>= 1) {
+        c += __shfl_xor(c, o);
+        f += __shfl_xor(f, o);
+    }
+    *n = c;
+    *fdeg = f;
+}
+
+// The non-empty chunks among this workgroup's (chunk blockIdx + j G at bit
+// j) in one set of chunk flags; every wave computes the same mask.
+__device__ __forceinline__ uint64_t wg_chunk_mask(const uint8_t *flags, int32_t chunks) {
+    const int32_t ch = (int32_t)blockIdx.x + lane_id() * (int32_t)gridDim.x;
+    return __ballot(ch < chunks && flags[ch] != 0);
+}
+
+// The selector network's weights a lane needs, loaded before the level's
+// counts are known so that the loads overlap: lane u of wave w holds the
+// first-layer row of hidden unit u and the second-layer weights of unit u
+// over inputs [16 w, 16 w + 16).
+struct SelWeights {
+    uint4 w1[kSelInputs / 8];
+    uint4 w2[2];
+};
+
+static_assert(kStepBlock == 4 * 64 && kSelHidden == 64 && kSelInputs == 16, "the network's split over 4 waves");
+__device__ __forceinline__ void load_sel_weights(const SelectorView &sel, SelWeights *q) {
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    const int32_t next_level = st->level + 1;
-    const int32_t p = next_level & 1;
-    const bool learned = sel.mode == GI_SELECT_LEARNED && sel.w1 != nullptr;
-    // the network runs on wave 0, one lane per hidden unit; each lane's
-    // weights are loaded first, so their latency hides behind the partial sums
+    const uint4 *r1 = reinterpret_cast<const uint4 *>(sel.w1 + lane * kSelInputs);
+    const uint4 *r2 = reinterpret_cast<const uint4 *>(sel.w2 + lane * kSelHidden + 16 * wid);
+#pragma unroll
+    for (int i = 0; i < kSelInputs / 8; ++i) q->w1[i] = r1[i];
+    q->w2[0] = r2[0];
+    q->w2[1] = r2[1];
+}
+
+__device__ __forceinline__ bool sel_learned(const SelectorView &sel) {
+    return sel.mode == GI_SELECT_LEARNED && sel.w1 != nullptr;
+}
+
+// The mode of step l (n = |F_l| > 0): GraphIt's hybrid rule or one of its
+// schedules, or the MLP. Every thread of the workgroup calls it and gets the
+// same mode. The network: every wave computes the first layer (lane u:
+// hidden unit u), wave w the part of the second layer over its 16 inputs
+// into `part` (LDS, 4 x 64 floats), wave 0 the sums (in wave order) and the
+// output layer.
+__device__ __forceinline__ int32_t select_mode(const SelectorView &sel, const SelWeights &q, int32_t n,
+                                               unsigned long long fdeg, long long unv, long long unv_e, int32_t l,
+                                               int32_t prev_mode, int32_t prev_n, int32_t V, int64_t E,
+                                               float *part, int32_t *mode_s) {
+    if (!sel_learned(sel)) {
+        if (sel.mode == GI_SELECT_S1) return sel.mode_s1;
+        if (sel.mode == GI_SELECT_S2) return sel.mode_s2;
+        return ((double)n <= (double)sel.threshold * (double)V) ? sel.mode_s1 : sel.mode_s2;
+    }
     static_assert(kSelMaxOut == 8, "one 16-byte row of output weights per hidden unit");
-    uint4 w1q[kSelInputs / 8], w2q[kSelHidden / 8], w3q[1];
-    uint16_t b1v = 0, b2v = 0, b3v = 0;
+    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    // every lane computes the whole input vector (the same values, so no
+    // lane waits on another's branch of the feature switch)
+    float feats[kSelInputs];
+    bfs_features_f32(feats, n, fdeg, unv, unv_e, l, prev_mode, prev_n, V, E);
+    float a1 = bf16_to_f32(sel.b1[lane]);
 #pragma unroll
-    for (int i = 0; i < kSelInputs / 8; ++i) w1q[i] = make_uint4(0u, 0u, 0u, 0u);
+    for (int k = 0; k < kSelInputs; ++k) a1 += (feats[k] - sel.feat_mean[k]) * sel.feat_scale[k] * bf16_in(q.w1, k);
+    const float h1 = a1 > 0.f ? a1 : 0.f;
+    float p = 0.f;
 #pragma unroll
-    for (int i = 0; i < kSelHidden / 8; ++i) w2q[i] = make_uint4(0u, 0u, 0u, 0u);
-    w3q[0] = make_uint4(0u, 0u, 0u, 0u);
-    if (learned && wid == 0) {
-        const uint4 *r1 = reinterpret_cast<const uint4 *>(sel.w1 + lane * kSelInputs);
-        const uint4 *r2 = reinterpret_cast<const uint4 *>(sel.w2 + lane * kSelHidden);
+    for (int k = 0; k < 8; ++k) p += __shfl(h1, 16 * wid + k) * bf16_in(&q.w2[0], k);
 #pragma unroll
-        for (int i = 0; i < kSelInputs / 8; ++i) w1q[i] = r1[i];
-#pragma unroll
-        for (int i = 0; i < kSelHidden / 8; ++i) w2q[i] = r2[i];
-        w3q[0] = *reinterpret_cast<const uint4 *>(sel.w3 + lane * kSelMaxOut);
-        b1v = sel.b1[lane];
-        b2v = sel.b2[lane];
-        if (lane < sel.n_out) b3v = sel.b3[lane];
-    }
-    __shared__ int32_t rn[Block / 64];
-    __shared__ unsigned long long rf[Block / 64];
-    int32_t n = 0;
-    unsigned long long fdeg = 0;
-    if (next_level == 0 && d.seed >= 0) {
-        // a single source: seeded here (the reset cleared its entries); every
-        // thread counts it
-        const int32_t v = d.seed, dv = d.off[v + 1] - d.off[v];
-        n = 1;
-        fdeg = (unsigned long long)dv;
-        if (threadIdx.x == 0) {
-            const uint64_t bit = 1ull << (v & 63);
-            const int32_t ch = (v >> 6) / kChunkWords;
-            st->depth[v] = 0;
-            if (st->parent) st->parent[v] = v;
-            d.visited[v >> 6] |= bit;
-            d.bm0[v >> 6] |= bit;
-            d.sum[0][ch >> 6] |= 1ull << (ch & 63);
-            if (st->want_hubs && dv >= st->hub_min) {  // the first step's grid bin
-                st->hubs[0][0] = v;
-                st->hub_n[0] = 1;
-            }
-        }
-    } else if (next_level == 0) {  // seeded by the seed kernel
-        n = st->count[0];
-        fdeg = st->fdeg[0];
-    } else {
-        for (int32_t i = threadIdx.x; i < d.slots; i += Block) {
-            n += pcount[i];
-            fdeg += pfdeg[i];
-            pcount[i] = 0;
-            pfdeg[i] = 0;
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            n += __shfl_xor(n, o);
-            fdeg += __shfl_xor(fdeg, o);
-        }
-        if (lane == 0) {
-            rn[wid] = n;
-            rf[wid] = fdeg;
-        }
-        for (int32_t i = threadIdx.x; i < d.sum_words; i += Block) sum_clear[i] = 0;
-        // the summary of F', from the flags its chunks set
-        bs_build_summary<Block>(d.cflag, d.chunks, pick3(d.sum[0], d.sum[1], d.sum[2], next_level % 3));
-        __syncthreads();
-        n = 0;
-        fdeg = 0;
-        for (int i = 0; i < Block / 64; ++i) {
-            n += rn[i];
-            fdeg += rf[i];
-        }
-    }
-    const int32_t prev_n = next_level > 0 ? st->count[p ^ 1] : 0;
-    const long long unv = st->unvisited - n;
-    const long long unv_e = st->unvisited_edges - (long long)fdeg;
-    if (n == 0) {  // uniform
-        if (threadIdx.x == 0) {
-            st->done = 1;
-            st->levels = next_level;
-            if (host_flag) {  // the host waits on the replay's event, then reads these
-                host_flag[2] = next_level;
-                host_flag[1] = next_level;
-                host_flag[0] = 1;
-            }
-        }
-        return;
-    }
-    int32_t mode;
-    if (!learned) {
-        if (sel.mode == GI_SELECT_S1) mode = sel.mode_s1;
-        else if (sel.mode == GI_SELECT_S2) mode = sel.mode_s2;
-        else mode = ((double)n <= (double)sel.threshold * (double)V) ? sel.mode_s1 : sel.mode_s2;
-    } else {
-        if (wid != 0) return;  // the state is written by thread 0
-        // every lane computes the whole input vector (the same values, so no
-        // lane waits on another's branch of the feature switch)
-        float feats[kSelInputs];
-        bfs_features_f32(feats, n, fdeg, unv, unv_e, next_level, st->mode, prev_n, V, E);
-        float a1 = bf16_to_f32(b1v);
-#pragma unroll
-        for (int k = 0; k < kSelInputs; ++k)
-            a1 += (feats[k] - sel.feat_mean[k]) * sel.feat_scale[k] * bf16_in(w1q, k);
-        const float h1 = a1 > 0.f ? a1 : 0.f;
-        float a2 = bf16_to_f32(b2v);
-#pragma unroll
-        for (int k = 0; k < kSelHidden; ++k) a2 += __shfl(h1, k) * bf16_in(w2q, k);
+    for (int k = 0; k < 8; ++k) p += __shfl(h1, 16 * wid + 8 + k) * bf16_in(&q.w2[1], k);
+    part[wid * 64 + lane] = p;
+    __syncthreads();
+    if (wid == 0) {
+        const uint4 w3q[1] = {*reinterpret_cast<const uint4 *>(sel.w3 + lane * kSelMaxOut)};
+        const uint16_t b3v = lane < sel.n_out ? sel.b3[lane] : (uint16_t)0;
+        const float a2 = bf16_to_f32(sel.b2[lane]) + part[lane] + part[64 + lane] + part[128 + lane] + part[192 + lane];
         const float h2 = a2 > 0.f ? a2 : 0.f;
         // output o = b3[o] + sum over the hidden units of h2 * w3[unit][o]
-        mode = sel.mode_s1;
+        int32_t mode = sel.mode_s1;
         float best = 3.4e38f;
 #pragma unroll
         for (int o = 0; o < kSelMaxOut; ++o) {
@@ -824,51 +748,155 @@ __device__ __forceinline__ void decide_body(const DecideArgs &d, const SelectorV
                 mode = o;
             }
         }
+        if (lane == 0) *mode_s = mode;
     }
-    if (threadIdx.x == 0) {
-        st->count[p] = n;
-        st->fdeg[p] = fdeg;
-        st->unvisited = unv;
-        st->unvisited_edges = unv_e;
-        st->mode = mode;
-        st->level = next_level;
-        st->hub_n[(next_level + 1) & 1] = 0;  // the list the step walked: the next step refills it
-        if (mode >= BFS_PULL_THREAD) st->pull_levels += 1;
-        if (next_level < kBfsMaxLevels) {
-            st->hist_mode[next_level] = mode;
-            st->hist_next[next_level] = n;
+    __syncthreads();
+    return *mode_s;
+}
+
+// The end of a run: F_l is empty. One thread.
+__device__ __forceinline__ void finish_run(const StepArgs &a, int32_t l, const BfsLevel &prev) {
+    BfsState *st = a.st;
+    st->levels = l;
+    st->pull_levels = prev.pull_levels;
+    st->done = 1;
+    if (a.host_flag) {  // the host waits on the replay's event, then reads these
+        a.host_flag[2] = l;
+        a.host_flag[1] = l;
+        a.host_flag[0] = 1;
+    }
+}
+
+// One level, one launch. Every workgroup first decides step l itself, from
+// what the previous launch left (the counts of F_l in the stripes, the
+// record of level l - 1) -- the same inputs and the same code, hence the
+// same mode in every workgroup -- so no workgroup waits on another and no
+// decide launch sits between two steps. Everything the decision and the
+// step's set-up read is loaded at once: the rotations' indices come from
+// the phase ph = l mod 12, a launch argument, not from the state. The first
+// workgroup writes the record of level l and clears what the next launch
+// adds into (count set (l + 1) % 3, flags (l + 1) % 4, hub list (l + 2) %
+// 3): each was last read by the launch before this one and is next written
+// by the one after. An empty F_l ends the run. Graph replays run launch k of
+// a block as level base + k (base kept in the state); eager launches pass
+// their level with base 0.
+__device__ __forceinline__ void level_body(const StepArgs &a, int32_t k, int32_t ph) {
+    __shared__ int32_t list[kChunkVerts];  // push: a chunk's vertices; first, the network's partial sums
+    __shared__ int32_t mode_s;
+    static_assert(sizeof(int32_t) * kChunkVerts >= sizeof(float) * kStepBlock, "the partial sums fit");
+    BfsState *st = a.st;
+    const SelectorView &sel = st->sel;  // read from memory where the network needs it
+    const int32_t p2 = ph & 1, p3 = ph % 3, p4 = ph & 3;
+    const bool learned = sel_learned(sel);
+    SelWeights q;
+    if (learned) load_sel_weights(sel, &q);
+    const int32_t done = st->done, base = st->base, forced = st->force_mode;
+    const BfsLevel prev = st->lv[p2 ^ 1];
+    int32_t n;
+    unsigned long long fdeg;
+    level_counts(a, (p3 + 2) % 3, &n, &fdeg);  // every wave: the same totals
+    const uint64_t fl_clear = wg_chunk_mask(flag_set(a, (p4 + 2) & 3), a.chunks);
+    const uint64_t fl_list = wg_chunk_mask(flag_set(a, (p4 + 3) & 3), a.chunks);
+    if (done) return;  // uniform: set only by the launch that found F empty
+    const int32_t l = base + k;
+    if (n == 0) {  // uniform
+        if (blockIdx.x == 0 && threadIdx.x == 0) finish_run(a, l, prev);
+        return;
+    }
+    const long long unv = prev.unvisited - n, unv_e = prev.unvisited_edges - (long long)fdeg;
+    const int32_t mode = forced >= 0 ? forced
+                                     : select_mode(sel, q, n, fdeg, unv, unv_e, l, prev.mode, prev.count, a.V, a.E,
+                                                   reinterpret_cast<float *>(list), &mode_s);
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
+            BfsLevel &cur = st->lv[p2];
+            cur.level = l;
+            cur.mode = mode;
+            cur.count = n;
+            cur.fdeg = fdeg;
+            cur.unvisited = unv;
+            cur.unvisited_edges = unv_e;
+            cur.pull_levels = prev.pull_levels + (mode >= BFS_PULL_THREAD ? 1 : 0);
+            if (l < kBfsMaxLevels) {
+                st->hist_mode[l] = mode;
+                st->hist_next[l] = n;
+            }
+            st->hub_n[(p3 + 2) % 3] = 0;
+            st->giant_n[(p3 + 2) % 3] = 0;
         }
+        const int32_t cs = (p3 + 1) % kCountSets;
+        if (threadIdx.x < kStripes) {
+            a.cnt[(cs * kStripes + threadIdx.x) * kStripeI32] = 0;
+            a.fdg[(cs * kStripes + threadIdx.x) * kStripeU64] = 0;
+        }
+        uint4 *f4 = reinterpret_cast<uint4 *>(flag_set(a, (p4 + 1) & 3));
+        const int32_t n16 = (a.chunks + 15) / 16;
+        for (int32_t i = threadIdx.x; i < n16; i += kStepBlock) f4[i] = make_uint4(0u, 0u, 0u, 0u);
     }
+    if (mode >= BFS_PULL_THREAD) step_body<kPullStep>(a, list, l, ph, mode, n, fdeg, fl_clear, fl_list);  // uniform
+    else step_body<kPushStep>(a, list, l, ph, mode, n, fdeg, fl_clear, fl_list);
 }
 
-__global__ void __launch_bounds__(kDecideBlock) bfs_decide_kernel(DecideArgs d, SelectorView sel) {
-    decide_body<kDecideBlock>(d, sel);
-}
-
-// A level's step: the push or the pull body, as the decided mode says.
-__device__ __forceinline__ void level_body(const StepArgs &a) {
-    __shared__ int32_t pre[kMaxSumWords + 1];
-    if (a.st->done) return;                                            // uniform: set by a decide kernel
-    if (a.st->mode >= BFS_PULL_THREAD) step_body<kPullStep>(a, pre);  // uniform
-    else step_body<kPushStep>(a, pre);
-}
-
-// The recorded level loop launches bfs_step. Traced levels (per-level
-// timing) and sweeps launch the same code, traced levels under two more
-// symbols, one per direction, so a profile tells their launches apart. Both
-// step bodies together need 72 VGPRs: 7 waves per SIMD without spilling
-// measured faster than 8 with a spill.
+// The recorded level loop launches bfs_step, block launch k as level
+// base + k with phase (base + k) mod 12. Traced levels (per-level timing)
+// and sweeps launch the same code with their level and base 0, traced
+// levels under two more symbols, one per direction, so a profile tells
+// their launches apart. Both step bodies together need 72 VGPRs: 7 waves
+// per SIMD without spilling measured faster than 8 with a spill.
 #define GI_STEP_KERNEL(name)                                                               \
     __global__ void __launch_bounds__(kStepBlock) __attribute__((amdgpu_waves_per_eu(7))) \
-    name(StepArgs a) {                                                                    \
-        level_body(a);                                                                     \
+    name(StepArgs a, int32_t k, int32_t ph) {                                             \
+        level_body(a, k, ph);                                                              \
     }
 GI_STEP_KERNEL(bfs_step)
 GI_STEP_KERNEL(bfs_push_step_traced)
 GI_STEP_KERNEL(bfs_pull_step_traced)
 #undef GI_STEP_KERNEL
 
+// Traced levels: the decision of step l alone (the same code as the step
+// launches'), stored as the forced mode the traced step launch then runs
+// with, so the host knows the direction before it launches; an empty F_l
+// ends the run.
+__global__ void __launch_bounds__(kStepBlock) bfs_decide_kernel(StepArgs a, int32_t l) {
+    __shared__ float part[kStepBlock];
+    __shared__ int32_t mode_s;
+    BfsState *st = a.st;
+    const SelectorView &sel = st->sel;
+    if (st->done) return;
+    const BfsLevel prev = st->lv[(l + 1) & 1];
+    int32_t n;
+    unsigned long long fdeg;
+    level_counts(a, (l + 2) % kCountSets, &n, &fdeg);
+    if (n == 0) {
+        if (threadIdx.x == 0) finish_run(a, l, prev);
+        return;
+    }
+    SelWeights q;
+    if (sel_learned(sel)) load_sel_weights(sel, &q);
+    const int32_t mode = select_mode(sel, q, n, fdeg, prev.unvisited - n, prev.unvisited_edges - (long long)fdeg, l,
+                                     prev.mode, prev.count, a.V, a.E, part, &mode_s);
+    if (threadIdx.x == 0) st->force_mode = mode;
+}
+
+// Sweeps: the selector inputs of step l.
+__global__ void __launch_bounds__(64) bfs_peek_kernel(StepArgs a, int32_t l, BfsPeek *out) {
+    const BfsLevel prev = a.st->lv[(l + 1) & 1];
+    int32_t n;
+    unsigned long long fdeg;
+    level_counts(a, (l + 2) % kCountSets, &n, &fdeg);
+    if (threadIdx.x == 0) {
+        out->n = n;
+        out->fdeg = fdeg;
+        out->unvisited = prev.unvisited - n;
+        out->unvisited_edges = prev.unvisited_edges - (long long)fdeg;
+        out->prev_mode = prev.mode;
+        out->prev_n = prev.count;
+        out->level = l;
+    }
+}
+
 struct SeedArgs {
+    SelectorView sel;
     int32_t *depth;
     int32_t *parent;
     int32_t lane_max;
@@ -886,10 +914,12 @@ struct ResetArgs {
     uint64_t *visited;
     const uint64_t *dead;   // vertices without in-edges, and the padding past V
     uint64_t *bm[3];
-    uint64_t *sum[3];
+    uint64_t *flags;        // every set, as 64-bit words
+    int32_t flag_words;
+    int32_t *cnt;
+    unsigned long long *fdg;
     int32_t V;
     int32_t words;
-    int32_t sum_words;
 };
 
 // Clears every array of a run in one pass and initialises the state.
@@ -918,7 +948,9 @@ __global__ void bfs_reset_kernel(BfsState *st, SeedArgs sa, ResetArgs r, int64_t
         if (i < dlen) r.depth[i] = -1;
         if (i < plen) r.parent[i] = -1;
     }
-    const int32_t n = r.words > r.sum_words + 1 ? r.words : r.sum_words + 1;
+    const int32_t lines = kCountSets * kStripes;
+    int32_t n = r.words > r.flag_words ? r.words : r.flag_words;
+    n = n > lines ? n : lines;
     for (int32_t i = tid; i < n; i += stride) {
         if (i < r.words) {
             r.visited[i] = r.dead[i];
@@ -926,34 +958,39 @@ __global__ void bfs_reset_kernel(BfsState *st, SeedArgs sa, ResetArgs r, int64_t
             r.bm[1][i] = 0;
             r.bm[2][i] = 0;
         }
-        if (i <= r.sum_words) {
-            r.sum[0][i] = 0;
-            r.sum[1][i] = 0;
-            r.sum[2][i] = 0;
+        if (i < r.flag_words) r.flags[i] = 0;
+        if (i < lines) {
+            r.cnt[i * kStripeI32] = 0;
+            r.fdg[i * kStripeU64] = 0;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st->count[0] = 0;
-        st->count[1] = 0;
-        st->fdeg[0] = 0;
-        st->fdeg[1] = 0;
-        st->hub_n[0] = 0;
-        st->hub_n[1] = 0;
+        for (int i = 0; i < 3; ++i) {
+            st->hub_n[i] = 0;
+            st->giant_n[i] = 0;
+        }
         st->want_hubs = sa.want_hubs;
         st->hub_list_count = sa.hub_list_count;
         st->hub_list_fdeg = sa.hub_list_fdeg;
         st->done = 0;
-        st->level = -1;
         st->levels = 0;
         st->pull_levels = 0;
-        st->mode = BFS_PUSH_THREAD;
-        st->unvisited = r.V;
-        st->unvisited_edges = E;
+        st->force_mode = -1;
+        st->base = 0;
+        BfsLevel &before = st->lv[1];  // level -1: nothing reached yet
+        before.level = -1;
+        before.mode = BFS_PUSH_THREAD;
+        before.count = 0;
+        before.pull_levels = 0;
+        before.fdeg = 0;
+        before.unvisited = r.V;
+        before.unvisited_edges = E;
         st->lane_max = sa.lane_max;
         st->hub_min = sa.hub_min;
         st->want_fdeg = sa.want_fdeg;
         st->depth = sa.depth;
         st->parent = sa.parent;
+        st->sel = sa.sel;
     }
 }
 
@@ -993,12 +1030,14 @@ __global__ void bfs_cand_kernel(const int32_t *poff, const int32_t *pnbr, const 
     }
 }
 
-// Seeds one or many sources (a multi-source BFS gives every vertex its
-// distance to the nearest source; over a disjoint union of graphs with one
-// source each it is every graph's own BFS). Repeated sources count once.
-__global__ void bfs_seed_sources_kernel(BfsState *st, const int32_t *src, int32_t n,
-                                        int32_t single, uint64_t *bm0, uint64_t *sum0,
-                                        uint64_t *visited, const int32_t *off) {
+// Seeds one or many sources into F_0 (a multi-source BFS gives every vertex
+// its distance to the nearest source; over a disjoint union of graphs with
+// one source each it is every graph's own BFS): the bitmap, the chunk flags
+// step 0 reads (flags3 = flags[3]), the counts it reads (count set 2, first
+// stripe), the hub list. Repeated sources count once.
+__global__ void bfs_seed_sources_kernel(BfsState *st, const int32_t *src, int32_t n, int32_t single,
+                                        uint64_t *bm0, uint8_t *flags3, uint64_t *visited, const int32_t *off,
+                                        int32_t *cnt2, unsigned long long *fdg2) {
     int32_t cnt = 0;
     unsigned long long fd = 0;
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1009,12 +1048,14 @@ __global__ void bfs_seed_sources_kernel(BfsState *st, const int32_t *src, int32_
         atomicOr((unsigned long long *)&visited[v >> 6], bit);
         st->depth[v] = 0;
         if (st->parent) st->parent[v] = v;
-        int32_t ch = (v >> 6) / kChunkWords;
-        atomicOr((unsigned long long *)&sum0[ch >> 6], 1ull << (ch & 63));
+        bs_flag_chunk(flags3, v >> 6);
         cnt += 1;
         const int32_t dv = off[v + 1] - off[v];
         fd += (unsigned long long)dv;
-        if (st->want_hubs && dv >= st->hub_min) {  // the first step's grid bin
+        if (st->want_hubs && dv >= st->hub_min * kGiantFactor) {  // the first step's grid bin
+            const int32_t p = atomicAdd(&st->giant_n[0], 1);
+            if (p < kMaxGiants) st->giants[0][p] = v;
+        } else if (st->want_hubs && dv >= st->hub_min) {
             const int32_t p = atomicAdd(&st->hub_n[0], 1);
             if (p < kMaxHubs) st->hubs[0][p] = v;
         }
@@ -1024,8 +1065,8 @@ __global__ void bfs_seed_sources_kernel(BfsState *st, const int32_t *src, int32_
         fd += __shfl_xor(fd, o);
     }
     if (lane_id() == 0 && cnt) {
-        atomicAdd(&st->count[0], cnt);
-        atomicAdd(&st->fdeg[0], fd);
+        atomicAdd(cnt2, cnt);
+        atomicAdd(fdg2, fd);
     }
 }
 
@@ -1041,30 +1082,27 @@ __global__ void count_reached_fused_kernel(const int32_t *depth, int32_t V, int3
 struct FusedWork {
     BfsState *st = nullptr;
     uint64_t *bm[3] = {nullptr, nullptr, nullptr};
-    uint64_t *sum[3] = {nullptr, nullptr, nullptr};
-    uint8_t *cflag = nullptr;        // chunk flags (bs_flag_chunk), zero between steps
-    int32_t *pcount = nullptr;
-    unsigned long long *pfdeg = nullptr;
+    uint8_t *flags = nullptr;        // kFlagSets sets of chunk flags, flag_bytes apart
+    int32_t flag_bytes = 0;
+    int32_t *cnt = nullptr;          // level counts: kCountSets x kStripes lines
+    unsigned long long *fdg = nullptr;
+    BfsPeek *peek = nullptr;         // sweeps: a level's selector inputs
     int32_t *reached = nullptr;
     uint64_t *dead = nullptr;        // pull-side tables (see bfs_dead_kernel)
     int32_t *cand = nullptr;
     int32_t cand_first = 0;          // cand holds first in-neighbours (degree-ordered views)
     const int32_t *poff = nullptr;   // in-edge CSR (the graph itself when symmetric)
     const int32_t *pnbr = nullptr;
-    SelectorView last_sel{};         // selector of the current run (passed to the decides)
     int32_t *d_src = nullptr;        // sources of a multi-source run
     int32_t src_cap = 0;
-    int32_t *host_flag = nullptr;    // pinned, coherent {done, level, levels, mode}: written by the
-                                     // decide that ends a run, or copied in by the traced path
+    int32_t *host_flag = nullptr;    // pinned, coherent: {done, level, levels} written by the launch
+                                     // that ends a run; [4] the next block's base level (copied in)
     hipEvent_t flag_ev = nullptr;    // recorded after each replay of a block of levels
-    int32_t sum_words = 0;
     int32_t lane_max = 64;           // bin thresholds (gi_bfs_set_bins)
     int32_t hub_min = kHubDegree;
     int32_t blocks_per_cu = 8;
-    // level loop recorded as a hipGraph, one exec per level count
-    // recorded level loops by (level count, selector view bytes): the decide
-    // kernel takes the view by value
-    std::map<std::pair<int32_t, std::string>, hipGraphExec_t> execs;
+    // recorded level loops by 12 x level count + phase of the first level
+    std::map<int32_t, hipGraphExec_t> execs;
     int32_t recent_levels[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // depths of the last runs (ring)
     int32_t recent_n = 0;
 };
@@ -1080,10 +1118,10 @@ static void free_work(FusedWork *w) {
     drop_execs(w);
     dfree(w->st);
     for (auto p : w->bm) dfree(p);
-    for (auto p : w->sum) dfree(p);
-    dfree(w->pcount);
-    dfree(w->pfdeg);
-    dfree(w->cflag);
+    dfree(w->flags);
+    dfree(w->cnt);
+    dfree(w->fdg);
+    dfree(w->peek);
     dfree(w->reached);
     dfree(w->dead);
     dfree(w->cand);
@@ -1097,9 +1135,11 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
     if ((*out = g_fused.find(g)) != nullptr) return GI_OK;
     int64_t words = bitmap_words(g->V);
     int64_t chunks = ceil_div(words, kChunkWords);
-    int64_t sum_words = ceil_div(chunks, 64);
-    if (sum_words > kMaxSumWords)
-        return fail(GI_ERR_RANGE, "fused BFS supports up to 2^26 vertices");
+    // a step workgroup owns at most 64 chunks (one bit each of a wave's mask)
+    // and the grid has at least one workgroup per CU
+    const int64_t max_chunks = 64LL * device_info().num_cus;
+    if (chunks > max_chunks)
+        return fail(GI_ERR_RANGE, "fused BFS supports up to %lld vertices on this GPU", (long long)max_chunks * kChunkVerts);
     const gi_graph *pg = g;  // in-edges for pull steps
     if (g->directed) {
         gi_graph *t = nullptr;
@@ -1108,27 +1148,23 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
     }
     FusedWork *w = new FusedWork();
     w->cand_first = g->degree_view ? 1 : 0;
-    w->sum_words = (int32_t)sum_words;
     w->poff = pg->d_off;
     w->pnbr = pg->d_nbr;
+    w->flag_bytes = (int32_t)bs_flag_bytes(chunks);  // a multiple of 16
     int rc = dalloc(&w->st, 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->bm[i], (size_t)words + 1);
-    for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->sum[i], (size_t)sum_words + 1);
-    if (rc == GI_OK) rc = dalloc(&w->pcount, (size_t)kMaxGrid);
-    if (rc == GI_OK) rc = dalloc(&w->pfdeg, (size_t)kMaxGrid);
-    if (rc == GI_OK) rc = dalloc(&w->cflag, (size_t)bs_flag_bytes(chunks));
-    if (rc == GI_OK && hipMemset(w->cflag, 0, (size_t)bs_flag_bytes(chunks)) != hipSuccess)
-        rc = fail(GI_ERR_HIP, "flag init");
+    if (rc == GI_OK) rc = dalloc(&w->flags, (size_t)kFlagSets * w->flag_bytes);
+    if (rc == GI_OK) rc = dalloc(&w->cnt, (size_t)kCountSets * kStripes * kStripeI32);
+    if (rc == GI_OK) rc = dalloc(&w->fdg, (size_t)kCountSets * kStripes * kStripeU64);
+    if (rc == GI_OK) rc = dalloc(&w->peek, 1);
     if (rc == GI_OK) rc = dalloc(&w->reached, 1);
     if (rc == GI_OK) rc = dalloc(&w->dead, (size_t)words + 1);
     if (rc == GI_OK) rc = dalloc(&w->cand, (size_t)g->V + 1);
     if (rc == GI_OK && !g->d_visited) rc = dalloc(&g->d_visited, (size_t)words + 1);
-    if (rc == GI_OK && hipHostMalloc((void **)&w->host_flag, sizeof(int32_t) * 4, hipHostMallocCoherent) != hipSuccess)
+    if (rc == GI_OK && hipHostMalloc((void **)&w->host_flag, sizeof(int32_t) * 8, hipHostMallocCoherent) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
     if (rc == GI_OK && hipEventCreateWithFlags(&w->flag_ev, hipEventDisableTiming) != hipSuccess)
         rc = fail(GI_ERR_HIP, "event create failed");
-    if (rc == GI_OK) rc = fill_i32(w->pcount, kMaxGrid, 0, 0);
-    if (rc == GI_OK) rc = fill_u64((uint64_t *)w->pfdeg, kMaxGrid, 0, 0);
     if (rc == GI_OK) {
         int grid = (int)ceil_div(g->V, 256);
         grid = grid < 4096 ? (grid < 1 ? 1 : grid) : 4096;
@@ -1170,14 +1206,14 @@ static int grid_of(const FusedWork *w, int resident) {
     return g < kMaxGrid ? g : kMaxGrid;
 }
 
-// Every level, traced or not, runs on the same persistent grid; each of its
-// workgroups writes one partial-sum slot.
+// Every level, traced or not, runs on the same persistent grid.
 static int step_grid(const FusedWork *w) {
     static const int resident = resident_per_cu((const void *)bfs_step, kStepBlock);
     return grid_of(w, resident);
 }
 
 static StepArgs step_args(gi_graph *g, FusedWork *w) {
+    // (ensure_fused checked that a workgroup owns at most 64 chunks)
     StepArgs a;
     a.st = w->st;
     a.off = g->d_off;
@@ -1187,86 +1223,59 @@ static StepArgs step_args(gi_graph *g, FusedWork *w) {
     a.cand = w->cand;
     a.cand_first = w->cand_first;
     a.V = g->V;
+    a.E = g->E;
     a.words = (int32_t)bitmap_words(g->V);
     a.chunks = (int32_t)ceil_div(a.words, kChunkWords);
-    a.sum_words = w->sum_words;
     a.depth = nullptr;   // read from the state inside the kernels
     a.parent = nullptr;
     a.visited = g->d_visited;
-    for (int i = 0; i < 3; ++i) {
-        a.bm[i] = w->bm[i];
-        a.sum[i] = w->sum[i];
-    }
-    a.cflag = w->cflag;
-    a.pcount = w->pcount;
-    a.pfdeg = w->pfdeg;
+    for (int i = 0; i < 3; ++i) a.bm[i] = w->bm[i];
+    a.flags = w->flags;
+    a.flag_bytes = w->flag_bytes;
+    a.cnt = w->cnt;
+    a.fdg = w->fdg;
+    a.host_flag = w->host_flag;
     return a;
 }
 
-static DecideArgs decide_args(gi_graph *g, FusedWork *w) {
-    DecideArgs d;
-    d.st = w->st;
-    d.V = g->V;
-    d.E = g->E;
-    d.pcount = w->pcount;
-    d.pfdeg = w->pfdeg;
-    d.slots = step_grid(w);
-    for (int i = 0; i < 3; ++i) d.sum[i] = w->sum[i];
-    d.sum_words = w->sum_words;
-    d.cflag = w->cflag;
-    d.chunks = (int32_t)ceil_div(bitmap_words(g->V), kChunkWords);
-    d.host_flag = w->host_flag;
-    d.off = g->d_off;
-    d.visited = g->d_visited;
-    d.bm0 = w->bm[0];
-    d.seed = -1;
-    return d;
+// A traced level: the decision alone (the host reads the mode it forces),
+// then the step code under its per-direction symbol between two events.
+// Returns 1 when the run has ended.
+static int launch_level_traced(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t l, hipStream_t s,
+                               FusedTrace *trace) {
+    bfs_decide_kernel<<<1, kStepBlock, 0, s>>>(a, l);
+    GI_LAUNCH_CHECK();
+    static_assert(offsetof(BfsState, force_mode) == 3 * sizeof(int32_t), "done, levels, pull_levels, force_mode");
+    int32_t h[4];
+    GI_HIP(hipMemcpyAsync(h, w->st, sizeof(h), hipMemcpyDeviceToHost, s));
+    GI_HIP(hipStreamSynchronize(s));
+    if (h[0]) return 1;
+    hipEvent_t e0, e1;
+    GI_HIP(hipEventCreate(&e0));
+    GI_HIP(hipEventCreate(&e1));
+    trace->ev.push_back(e0);
+    trace->ev.push_back(e1);
+    GI_HIP(hipEventRecord(e0, s));
+    if (h[3] >= BFS_PULL_THREAD) bfs_pull_step_traced<<<step_grid(w), kStepBlock, 0, s>>>(a, l, l % 12);
+    else bfs_push_step_traced<<<step_grid(w), kStepBlock, 0, s>>>(a, l, l % 12);
+    GI_HIP(hipEventRecord(e1, s));
+    GI_LAUNCH_CHECK();
+    return GI_OK;
 }
 
-static void launch_decide(gi_graph *g, FusedWork *w, Launcher &L, int32_t seed = -1) {
-    // the kernel reads the level from the device state
-    DecideArgs d = decide_args(g, w);
-    d.seed = seed;
-    L(bfs_decide_kernel, 1, kDecideBlock, d, w->last_sel);
-}
-
-static void launch_decide(gi_graph *g, FusedWork *w, hipStream_t s, int32_t seed = -1) {
-    Launcher L(s);
-    launch_decide(g, w, L, seed);
-}
-
-// A recorded level: the step kernel (push or pull body as decided, the TWC
-// grid bin included), the decide kernel.
-static void launch_level(gi_graph *g, FusedWork *w, const StepArgs &a, Launcher &L) {
-    L(bfs_step, step_grid(w), kStepBlock, a);
-    launch_decide(g, w, L);
-}
-
-// A traced level with the mode already known on the host: the step code
-// under its per-direction symbol alone between the two events, then the
-// decide kernel.
-static void launch_level_traced(gi_graph *g, FusedWork *w, const StepArgs &a, int32_t mode, hipStream_t s,
-                                hipEvent_t *ev) {
-    (void)hipEventRecord(ev[0], s);
-    if (mode >= BFS_PULL_THREAD) bfs_pull_step_traced<<<step_grid(w), kStepBlock, 0, s>>>(a);
-    else bfs_push_step_traced<<<step_grid(w), kStepBlock, 0, s>>>(a);
-    (void)hipEventRecord(ev[1], s);
-    launch_decide(g, w, s);
-}
-
-// Seed: clear the arrays, set the state, mark the sources, decide level 0.
+// Seed: clear the arrays, set the state, mark the sources (F_0).
 // sources == null seeds the single vertex `single`.
 static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t n_sources,
                     int32_t single, int32_t *depth, int32_t *parent, const SelectorView &sel,
                     bool want_fdeg, hipStream_t s, int32_t depth_len = -1) {
-    w->last_sel = sel;  // recorded level graphs are kept per selector view (level_graph)
     // hubs are listed at discovery when a TWC step may use the list: the
     // selector may pick TWC, or the degrees are read anyway
     const bool learned = sel.mode == GI_SELECT_LEARNED && sel.w1 != nullptr;
     const bool may_twc = learned ? ((sel.allowed >> BFS_PUSH_TWC) & 1) != 0
                                  : (sel.mode_s1 == BFS_PUSH_TWC || sel.mode_s2 == BFS_PUSH_TWC);
     // ... during push steps of at most V/64 vertices and E/64 edges
-    SeedArgs sa{depth,
+    SeedArgs sa{sel,
+                depth,
                 parent,
                 w->lane_max,
                 w->hub_min,
@@ -1274,23 +1283,25 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
                 (want_fdeg || may_twc) ? 1 : 0,
                 g->V / kHubListShare,
                 (long long)(g->E / kHubListShare)};
-    w->host_flag[0] = 0;  // no run of this graph's engine is in flight: set again by the final decide
+    w->host_flag[0] = 0;  // no run of this graph's engine is in flight: set again by the final launch
     ResetArgs r;
     r.depth = depth;
     r.depth_len = depth_len >= 0 && depth_len < g->V ? depth_len : g->V;
     r.parent = parent;
     r.visited = g->d_visited;
     r.dead = w->dead;
-    for (int i = 0; i < 3; ++i) {
-        r.bm[i] = w->bm[i];
-        r.sum[i] = w->sum[i];
-    }
+    for (int i = 0; i < 3; ++i) r.bm[i] = w->bm[i];
+    r.flags = reinterpret_cast<uint64_t *>(w->flags);
+    r.flag_words = kFlagSets * w->flag_bytes / 8;
+    r.cnt = w->cnt;
+    r.fdg = w->fdg;
     r.V = g->V;
     r.words = (int32_t)bitmap_words(g->V);
-    r.sum_words = w->sum_words;
-    int grid = (int)ceil_div(g->V > w->sum_words + 1 ? g->V : w->sum_words + 1, 256);
-    grid = grid < 4096 ? grid : 4096;
+    int grid = (int)ceil_div(g->V, 256);
+    grid = grid < 4096 ? (grid < 1 ? 1 : grid) : 4096;
     bfs_reset_kernel<<<grid, 256, 0, s>>>(w->st, sa, r, g->E);
+    int32_t *cnt2 = w->cnt + (size_t)2 * kStripes * kStripeI32;  // count set 2 (level -1), first stripe
+    unsigned long long *fdg2 = w->fdg + (size_t)2 * kStripes * kStripeU64;
     if (h_sources) {
         if (n_sources > w->src_cap) {
             dfree(w->d_src);
@@ -1301,21 +1312,24 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
         }
         GI_HIP(hipMemcpyAsync(w->d_src, h_sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice, s));
         int sg = (int)ceil_div(n_sources, 256);
-        bfs_seed_sources_kernel<<<sg < 1024 ? sg : 1024, 256, 0, s>>>(w->st, w->d_src, n_sources, 0, w->bm[0],
-                                                                      w->sum[0], g->d_visited, g->d_off);
+        bfs_seed_sources_kernel<<<sg < 1024 ? sg : 1024, 256, 0, s>>>(
+            w->st, w->d_src, n_sources, 0, w->bm[0], w->flags + 3 * w->flag_bytes, g->d_visited, g->d_off, cnt2, fdg2);
+    } else {
+        bfs_seed_sources_kernel<<<1, 64, 0, s>>>(w->st, nullptr, 1, single, w->bm[0], w->flags + 3 * w->flag_bytes,
+                                                 g->d_visited, g->d_off, cnt2, fdg2);
     }
-    launch_decide(g, w, s, h_sources ? -1 : single);  // a single source is seeded by this decide
     GI_LAUNCH_CHECK();
     if (h_sources) GI_HIP(hipStreamSynchronize(s));  // h_sources may go away on return
     return GI_OK;
 }
 
-// A hipGraph of `levels` consecutive levels (step, hub, decide each); every
-// kernel reads its level from the device state, so one recording serves every
-// run on this graph.
-static int level_graph(gi_graph *g, FusedWork *w, int32_t levels, hipGraphExec_t *out) {
-    const std::pair<int32_t, std::string> key(
-        levels, std::string(reinterpret_cast<const char *>(&w->last_sel), sizeof(SelectorView)));
+// A hipGraph of `levels` consecutive step launches, launch k running level
+// base + k (base from the device state) with phase (ph0 + k) mod 12, ph0 =
+// base mod 12, so one recording serves every run on this graph (the
+// selector is in the device state) and every block of a run that starts at
+// the same phase.
+static int level_graph(gi_graph *g, FusedWork *w, int32_t levels, int32_t ph0, hipGraphExec_t *out) {
+    const int32_t key = levels * 12 + ph0;
     auto it = w->execs.find(key);
     if (it != w->execs.end()) {
         *out = it->second;
@@ -1325,7 +1339,8 @@ static int level_graph(gi_graph *g, FusedWork *w, int32_t levels, hipGraphExec_t
     hipGraphExec_t exec = nullptr;
     GI_TRY(build_graph(
         [&](Launcher &L) {
-            for (int32_t l = 0; l < levels; ++l) launch_level(g, w, a, L);
+            for (int32_t k = 0; k < levels; ++k)
+                L(bfs_step, step_grid(w), kStepBlock, a, k, (ph0 + k) % 12);
         },
         &exec));
     if (w->execs.size() >= 16) drop_execs(w);
@@ -1337,52 +1352,40 @@ static int level_graph(gi_graph *g, FusedWork *w, int32_t levels, hipGraphExec_t
 static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trace, bool timed,
                       const FusedEpilogue *epilogue, int32_t *levels) {
     StepArgs a = step_args(g, w);
-    int32_t *done_flag = &w->st->done;
-    static_assert(offsetof(BfsState, levels) == offsetof(BfsState, done) + 2 * sizeof(int32_t),
-                  "done, level, levels are contiguous");
     if (trace && timed) {
-        // per-level launches with events around each step; the host reads
-        // {done, level, levels, mode} after every decide
-        static_assert(offsetof(BfsState, mode) == offsetof(BfsState, done) + 3 * sizeof(int32_t),
-                      "done, level, levels, mode are contiguous");
-        // a BFS has at most V levels: the bound only guards against a state
-        // that never reports done
+        // per-level launches with events around each step; a BFS has at most
+        // V levels: the bound only guards against a state that never
+        // reports done
         for (int64_t level = 0; level <= (int64_t)g->V + 1; ++level) {
-            GI_HIP(hipMemcpyAsync(w->host_flag, done_flag, 4 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-            GI_HIP(hipStreamSynchronize(s));
-            if (w->host_flag[0]) {
-                *levels = w->host_flag[2];
+            const int r = launch_level_traced(g, w, a, (int32_t)level, s, trace);
+            if (r == 1) {
+                GI_HIP(hipMemcpy(levels, &w->st->levels, sizeof(int32_t), hipMemcpyDeviceToHost));
                 return epilogue ? (*epilogue)(s) : GI_OK;
             }
-            hipEvent_t e0, e1;
-            GI_HIP(hipEventCreate(&e0));
-            GI_HIP(hipEventCreate(&e1));
-            trace->ev.push_back(e0);
-            trace->ev.push_back(e1);
-            launch_level_traced(g, w, a, w->host_flag[3], s, &trace->ev[trace->ev.size() - 2]);
-            GI_LAUNCH_CHECK();
+            GI_TRY(r);
         }
         return fail(GI_ERR_HIP, "BFS state never reported done");
     }
-    // Graph mode: replay a recorded block of levels sized by the deepest of
-    // the last runs on this graph (a level past the end costs one step launch
-    // that returns at once, a block too short a whole extra replay), wait for
-    // each replay's event (the epilogue is already queued behind it) and read
-    // the {done, level, levels} the final decide wrote to pinned memory,
-    // continue with blocks of 8, 16, 32, 48 levels for deeper runs.
+    // Graph mode: replay a recorded block of step launches sized by the
+    // deepest of the last runs on this graph (a launch past the end returns at
+    // once, a block too short costs a whole extra replay), wait for each
+    // replay's event (the epilogue is already queued behind it) and read the
+    // {done, level, levels} the final launch wrote to pinned memory; continue
+    // with blocks of 8, 16, 32, 48 levels for deeper runs, each after copying
+    // its base level into the state.
     int32_t block = 0;
-    for (int32_t l : w->recent_levels) block = l > block ? l : block;
-    if (block <= 0) block = 8;
+    for (int32_t l : w->recent_levels) block = l + 1 > block ? l + 1 : block;
+    if (block <= 1) block = 8;
     if (block > 48) block = 48;
     int32_t next_block = 8;
     int64_t level = 0;
     while (level <= (int64_t)g->V + 1) {  // no depth limit: a BFS has at most V levels
         hipGraphExec_t exec = nullptr;
-        GI_TRY(level_graph(g, w, block, &exec));
+        GI_TRY(level_graph(g, w, block, (int32_t)(level % 12), &exec));
         GI_HIP(hipGraphLaunch(exec, s));
         level += block;
-        // the decide that ends the run writes {done, level, levels}
-        // into the pinned host_flag itself (the host cleared done before the run)
+        // the launch that ends the run writes {done, level, levels} into the
+        // pinned host_flag itself (the host cleared done before the run)
         GI_HIP(hipEventRecord(w->flag_ev, s));
         if (epilogue) GI_TRY((*epilogue)(s));
         GI_HIP(hipEventSynchronize(w->flag_ev));
@@ -1390,6 +1393,9 @@ static int run_levels(gi_graph *g, FusedWork *w, hipStream_t s, FusedTrace *trac
             *levels = w->host_flag[2];
             return GI_OK;
         }
+        if (level > (int64_t)INT32_MAX - 64) break;
+        w->host_flag[4] = (int32_t)level;  // the next block's first level
+        GI_HIP(hipMemcpyAsync(&w->st->base, &w->host_flag[4], sizeof(int32_t), hipMemcpyHostToDevice, s));
         block = next_block;
         next_block = next_block < 48 ? (next_block * 2 < 48 ? next_block * 2 : 48) : 48;
     }
@@ -1452,28 +1458,18 @@ int fused_set_bins(gi_graph *g, int32_t lane_max, int32_t hub_min, int32_t block
 }
 
 // ---------------------------------------------------------------- sweep
-__global__ void bfs_force_mode_kernel(BfsState *st, int32_t mode) {
-    st->mode = mode;
-    st->hub_n[(st->level + 1) & 1] = 0;  // every replay lists the hubs of F' afresh
-}
-
-__global__ void zero_slots_kernel(int32_t *pcount, unsigned long long *pfdeg) {
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < kMaxGrid; i += gridDim.x * blockDim.x) {
-        pcount[i] = 0;
-        pfdeg[i] = 0;
+// Before each timed replay of step l: clear the counts it adds into and its
+// hub list (every replay lists the hubs of F' afresh), and force its mode.
+__global__ void __launch_bounds__(64) bfs_sweep_prep_kernel(StepArgs a, int32_t l, int32_t mode) {
+    const int32_t line = (l % kCountSets) * kStripes + (int32_t)threadIdx.x;
+    a.cnt[line * kStripeI32] = 0;
+    a.fdg[line * kStripeU64] = 0;
+    if (threadIdx.x == 0) {
+        a.st->force_mode = mode;
+        a.st->hub_n[(l + 1) % 3] = 0;
+        a.st->giant_n[(l + 1) % 3] = 0;
     }
 }
-
-struct SweepHost {
-    int32_t count[2];
-    unsigned long long fdeg[2];
-    int32_t done;
-    int32_t level;
-    int32_t levels;
-    int32_t mode;
-    long long unvisited;
-    long long unvisited_edges;
-};
 
 }  // namespace gi
 
@@ -1547,21 +1543,23 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
     sel.mode_s2 = BFS_PUSH_THREAD;
     if (rc == GI_OK) rc = seed_run(g, w, nullptr, 0, source, depth, nullptr, sel, true, s);
     StepArgs a = step_args(g, w);
-    int32_t nl = 0, prev_mode = BFS_PUSH_THREAD, prev_n = 0;
+    int32_t nl = 0;
     // One level: snapshot the state, time every mode of the mask from it with
-    // the production step kernel (bfs_step), then advance by the followed
-    // mode. Returns 1 when the BFS is done; errors return
+    // the production step kernel (bfs_step, its mode forced), then advance by
+    // the followed mode. Returns 1 when the BFS is done; errors return
     // through rc so the buffers below are freed.
     auto one_level = [&]() -> int {
-        SweepHost h;
-        GI_HIP(hipMemcpyAsync(&h, w->st, sizeof(h), hipMemcpyDeviceToHost, s));
+        const int32_t l = nl;  // level l: F_l from the previous level's final run
+        BfsPeek h;
+        bfs_peek_kernel<<<1, 64, 0, s>>>(a, l, w->peek);
+        GI_LAUNCH_CHECK();
+        GI_HIP(hipMemcpyAsync(&h, w->peek, sizeof(h), hipMemcpyDeviceToHost, s));
         GI_HIP(hipStreamSynchronize(s));
-        if (h.done) return 1;
-        int32_t p = h.level & 1;
-        bfs_raw_features(features + (int64_t)nl * kSelInputs, h.count[p], h.fdeg[p], h.unvisited,
-                         h.unvisited_edges, h.level, prev_mode, prev_n, g->V, g->E);
-        const int32_t iout = (h.level + 1) % 3;
-        // the step clears bitmap (level + 2) % 3, already empty after the first run
+        if (h.n == 0) return 1;
+        bfs_raw_features(features + (int64_t)nl * kSelInputs, h.n, h.fdeg, h.unvisited, h.unvisited_edges, l,
+                         h.prev_mode, h.prev_n, g->V, g->E);
+        const int32_t iout = (l + 1) % 3;
+        // the step clears bitmap (l + 2) % 3, already empty after the first run
         GI_HIP(hipMemcpyAsync(depth_bak, depth, sizeof(int32_t) * g->V, hipMemcpyDeviceToDevice, s));
         GI_HIP(hipMemcpyAsync(vis_bak, g->d_visited, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, s));
         float best = 3.4e38f;
@@ -1578,12 +1576,11 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
                 GI_HIP(hipMemcpyAsync(depth, depth_bak, sizeof(int32_t) * g->V, hipMemcpyDeviceToDevice, s));
                 GI_HIP(hipMemcpyAsync(g->d_visited, vis_bak, sizeof(uint64_t) * words, hipMemcpyDeviceToDevice, s));
                 GI_TRY(fill_u64(w->bm[iout], words, 0, s));
-                GI_TRY(fill_u64(w->sum[iout], w->sum_words, 0, s));
-                GI_TRY(fill_u64(reinterpret_cast<uint64_t *>(w->cflag), bs_flag_bytes(a.chunks) / 8, 0, s));
-                zero_slots_kernel<<<8, 256, 0, s>>>(w->pcount, w->pfdeg);
-                bfs_force_mode_kernel<<<1, 1, 0, s>>>(w->st, run);
+                GI_TRY(fill_u64(reinterpret_cast<uint64_t *>(w->flags + (l % kFlagSets) * w->flag_bytes),
+                                w->flag_bytes / 8, 0, s));
+                bfs_sweep_prep_kernel<<<1, 64, 0, s>>>(a, l, run);
                 GI_HIP(hipEventRecord(e0, s));
-                bfs_step<<<step_grid(w), kStepBlock, 0, s>>>(a);
+                bfs_step<<<step_grid(w), kStepBlock, 0, s>>>(a, l, l % 12);
                 GI_HIP(hipEventRecord(e1, s));
                 GI_HIP(hipEventSynchronize(e1));
                 float ms = 0.f;
@@ -1596,13 +1593,8 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
                     best = tmin;
                     best_mode = m;
                 }
-            } else {
-                launch_decide(g, w, s);
-                GI_LAUNCH_CHECK();
-                prev_mode = run;
             }
         }
-        prev_n = h.count[p];
         ++nl;
         return GI_OK;
     };

@@ -12,7 +12,8 @@
 4. save()/load(): an .npz of plain arrays (numpy, allow_pickle=False).
 
 The model is loaded into the runtime with gi_selector_load; inference then
-happens inside the BFS decide kernel with no host synchronisation.
+happens on the device at the start of every BFS step launch, with no host
+synchronisation.
 """
 from __future__ import annotations
 

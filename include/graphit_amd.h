@@ -415,7 +415,7 @@ int gi_reached_edges(const gi_graph *g, const int32_t *d_depth, int64_t *out, vo
 /* ------------------------------------------------------- learned selector
  * No counterpart in the reference (its only schedule choice is the fixed
  * hybrid rule, operators.cpp:526). With kernel_fusion ENABLED in s1, gi_bfs
- * runs the fused engine: per level a decide kernel on the device picks one of
+ * runs the fused engine: at the start of each level's step launch the device picks one of
  * GI_BFS_MODES step modes, either by GraphIt's rule over (s1, s2) or, with
  * selector GI_SELECT_LEARNED and a loaded model, by a 16-64-64-K MLP that
  * predicts each mode's log time (BF16 weights, FP32 accumulation); outputs

@@ -178,7 +178,7 @@ def _bf16(a):
 
 
 def _raw_features(n, fdeg, unv, unv_e, level, prev_pull, prev_n, V, E):
-    """The 16 selector inputs (the decide kernel's bfs_raw_feature)."""
+    """The 16 selector inputs (bfs_raw_feature, which the device evaluates in FP32)."""
     lg = np.log2
     return np.array([n / V, fdeg / E, unv / V, unv_e / E, lg(1 + n) / 32, lg(1 + fdeg) / 40,
                      lg(1 + fdeg / max(n, 1)) / 20, lg(1 + E / V) / 20, lg(V) / 32,
@@ -187,7 +187,7 @@ def _raw_features(n, fdeg, unv, unv_e, level, prev_pull, prev_n, V, E):
 
 
 def test_learned_selector_picks_match_host_inference(gi, c1):
-    """Every level's mode picked by the MLP in the decide kernel equals the
+    """Every level's mode picked by the MLP at the start of its step launch equals the
     argmin of the same BF16 network evaluated on the host (float64) from the
     oracle's level sets, wherever the two best outputs are not a near-tie."""
     g, host, _ = c1
