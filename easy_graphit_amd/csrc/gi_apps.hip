@@ -744,7 +744,7 @@ static int bc_unfused(gi_graph *g, const int32_t *h_sources, int32_t num_sources
         if (rc == GI_OK) rc = gi_frontier_add_vertex(f, src, s);
         if (rc == GI_OK) rc = gi_frontier_list_insert(list, f, s);
         int32_t round = 0, n = 1;
-        while (rc == GI_OK && n != 0) {  // forward, apps/bc.cpp:178-185
+        while (rc == GI_OK && n != 0) {  // forward, apps/bc.cpp:93-99
             ++round;
             gi_op_args fa{};
             fa.data0 = paths;
@@ -765,7 +765,7 @@ static int bc_unfused(gi_graph *g, const int32_t *h_sources, int32_t num_sources
         if (rc == GI_OK) rc = gi_frontier_list_retrieve(list, f, s);
         if (rc == GI_OK) rc = gi_vertexset_apply(g, f, GI_VOP_BC_BACKWARD_V, &va, s);
         --round;
-        while (rc == GI_OK && round > 0) {  // backward, apps/bc.cpp:193-200
+        while (rc == GI_OK && round > 0) {  // backward, apps/bc.cpp:108-114
             gi_op_args ba{};
             ba.data0 = dep;
             ba.data1 = visited;

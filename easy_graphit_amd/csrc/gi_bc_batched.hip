@@ -1,5 +1,5 @@
 // Batched Brandes betweenness: up to 256 sources advance level-synchronously
-// in one set of launches per level (apps/bc.cpp:150-209 computes one source
+// in one set of launches per level (apps/bc.cpp:64-125 computes one source
 // per run, with a host round trip per level and per list operation). With
 // one source per batch this is the fused single-source mode (fuse_kernel,
 // apps/bc.cpp:92-113): the level loops run on the device.
@@ -18,7 +18,7 @@
 // bitmap is rebuilt from depth (no bitmap per level is kept, so memory does
 // not grow with the number of levels); then for each (b,w) at level l + 1:
 // add dep[b][w] into dep[b][v] for neighbours with 0 <= depth <= l (the
-// reference's "not yet processed" filter, apps/bc.cpp:193-200); a last pass
+// reference's "not yet processed" filter, apps/bc.cpp:108-114); a last pass
 // adds 1/sigma over level 0. Score: sum_b (dep - 1/sigma) * sigma where
 // sigma != 0 (apps/bc.cpp:58-63), the source keeping its own term.
 // Push steps extract (b,u) pairs block-locally from 64-word chunks of the

@@ -510,7 +510,7 @@ int gi_cc(gi_graph *g, const gi_schedule *s, int32_t *d_labels, gi_run_stats *st
  * (no atomics). */
 int gi_pagerank(gi_graph *g, int32_t iters, float damping, const gi_schedule *s,
                 float *d_rank, gi_run_stats *stats, void *stream);
-/* Brandes BC from num_sources sources (apps/bc.cpp:150-209), summed into
+/* Brandes BC from num_sources sources (apps/bc.cpp:64-125), summed into
  * d_scores (float, zeroed first). batch > 1: up to 256 sources advance
  * level-synchronously, the level loops on the device (recorded); batch <= 1:
  * one source at a time as the app runs unfused (push TWCE, a host round trip

@@ -207,7 +207,7 @@ inline Udf pr_update_edge(VertexData<float> &new_rank, VertexData<float> &contri
 }  // namespace udf
 
 // ---------------------------------------------------------------- operators
-// edgeset_apply (include/graphit/operators.h:193-330).
+// edgeset_apply (include/graphit/operators.h:38-125).
 struct edgeset_apply {
     const gi_schedule *simple = nullptr;
     const gi_hybrid_schedule *hybrid = nullptr;
