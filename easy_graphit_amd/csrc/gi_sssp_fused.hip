@@ -6,17 +6,16 @@
 //
 // The window is [lo, hi). Every vertex below lo is settled; the near
 // frontier F holds the vertices improved into the window since their last
-// expansion (three rotating bitmaps with chunk summaries, as in the fused
-// BFS). Vertices improved to a distance >= hi are pending: they are not kept
-// in a set of their own (the reference's priority queue,
-// gpu_priority_queue.h), because a vertex is pending exactly when
-// hi <= dist < INT_MAX. Instead every unit of 1024 vertices keeps a lower
-// bound of its pending distances (umin, lowered with atomicMin when a
-// relaxation leaves a vertex pending; most relaxations find the bound
-// already lower and skip the atomic). A window advance scans dist only in
-// the units whose bound falls inside the new window, and rewrites their
-// bounds exactly; the other units are skipped, so later windows read a small
-// part of dist instead of all of it.
+// expansion (three rotating bitmaps with chunk flags, as in the fused BFS).
+// Vertices improved to a distance >= hi are pending: they are not kept in a
+// set of their own (the reference's priority queue, gpu_priority_queue.h),
+// because a vertex is pending exactly when hi <= dist < INT_MAX. Instead
+// every unit of 1024 vertices keeps lower bounds of its pending distances
+// (lowered with atomicMin when a relaxation leaves a vertex pending; most
+// relaxations find the bound already lower and skip the atomic). A window
+// scan reads dist only in the units whose bound falls inside the window,
+// and rewrites their bounds exactly; the other units are skipped, so later
+// windows read a small part of dist instead of all of it.
 // Light and heavy edges (the delta-stepping split of Meyer and Sanders):
 // each out-list is copied with its light edges (w < delta) first. An edge
 // from the window [lo, hi) that is heavy lands at or beyond hi, so it can
@@ -26,21 +25,31 @@
 // edges of W once, from final distances, instead of once per improvement of
 // each vertex within the window. Steps with smaller F relax all edges (no
 // extra step for windows that settle in one step).
-// A recorded slot is five launches, each a no-op unless its mode is active:
-//   ss_relax   push: relax the out-edges of F, spread evenly over each
-//              workgroup (vertices of at least kSsHubMin edges go to the
-//              grid-wide bin); an improvement below hi joins F';
-//              or pull, for a large F: each unsettled vertex takes the
-//              minimum over its in-neighbours in F (no atomics on distances);
-//   ss_hub     the grid-wide bin: edges of very wide frontier vertices (not
-//              launched for a graph without such vertices);
-//   ss_decide  count F'; empty -> advance to [hi, hi + delta);
-//   ss_farmove scan dist in the units whose bound is below hi: [lo, hi)
-//              joins F; also the least pending distance >= hi;
-//   ss_decide2 count the joins; none -> jump to the window that starts at
-//              that least distance (gpu_priority_queue.h:126-146 starts a
-//              window at the minimum remaining priority too), or done when
-//              nothing is pending.
+// One launch per step (plus, on graphs with vertices of at least kSsHubMin
+// edges, the step's grid-wide hub bin), recorded as hipGraph blocks. Every
+// workgroup of a step decides the step itself, from the previous step's
+// record and the counts it left in striped counters (the same inputs, hence
+// the same decision everywhere; no decide launch):
+//   relax      push: relax the out-edges of F (each workgroup over its own
+//              chunks of F), spread evenly over the workgroup (vertices of at
+//              least kSsHubMin edges go to the hub bin); an improvement below
+//              hi joins F'; or pull, for a large F: each unsettled vertex
+//              takes the minimum over its in-neighbours in F (no atomics on
+//              distances);
+//   heavy      the heavy edges of W, once per window: the same push, or,
+//              on a symmetric graph whose W holds E / kSsHeavyPullDiv edges
+//              or more, the pull over each list's heavy part;
+//   scan       the next window [lo, hi): scan dist in the units whose
+//              bound is below hi and relax the out-edges of the vertices in
+//              [lo, hi) at once; also the least pending distance >= hi;
+//   done       nothing pending.
+// After a relax step with an empty F' the window is settled: heavy step if W
+// holds vertices, else a scan of [hi, hi + delta). A scan that moved nothing
+// is followed by a scan at the least pending distance
+// (gpu_priority_queue.h:126-146 starts a window at the minimum remaining
+// priority too). A window is thus its scan-and-relax launch plus the relax
+// steps its improvements need.
+#include <algorithm>
 #include <climits>
 #include <map>
 #include <vector>
@@ -51,40 +60,60 @@
 namespace gi {
 
 constexpr int kSsBlock = 256;
-constexpr int kSsDecBlock = 1024;     // the single-workgroup decide kernels
 constexpr int32_t kSsHubMin = 4096;   // at least this wide: grid-wide bin
 constexpr int32_t kSsMaxHubs = 1024;
-constexpr int32_t kSsMaxGrid = 8192;  // partial-sum slots per kernel (one per workgroup)
+constexpr int32_t kSsMaxGrid = 8192;  // workgroups of a step launch at most
+constexpr int32_t kSsMaxBlock = 2048; // step launches of a recorded block at most
 constexpr int32_t kSsPullDiv = 8;     // a step pulls when |F| > V / kSsPullDiv
-constexpr int32_t kSsMinSlots = 9;    // slot row of the window scan's least pending distances
 
-constexpr int kSsPullUnit = 4 * (kSsBlock / 64);  // words per unit (pull work, window scan, umin)
+constexpr int kSsPullUnit = 4 * (kSsBlock / 64);  // words per unit (pull work, window scan, pending bounds)
 
 constexpr long long kSsDeferDiv = 16;  // F with E / kSsDeferDiv out-edges or more: heavy edges wait
+constexpr long long kSsHeavyPullDiv = 8;  // W with E / kSsHeavyPullDiv out-edges or more: the heavy step pulls
 
-enum SsMode : int32_t { SS_RELAX = 0, SS_ADVANCE = 1, SS_HEAVY = 2 };
+// A step's counts (|F'| or the vertices a scan moved, their out-edges, the
+// least pending distance a scan saw) are added up in kSsStripes stripes,
+// each on a 128-byte line of its own, one atomic per workgroup and counter,
+// in three rotating sets: read by the next step, written, zeroed.
+constexpr int kSsStripes = 64;
+constexpr int kSsLineI32 = 32;
+constexpr int kSsSets = 3;
+constexpr int kSsFlagSets = 4;
+
+enum SsMode : int32_t { SS_RELAX = 0, SS_ADVANCE = 1, SS_HEAVY = 2, SS_DONE = 3 };
+
+// One step's bookkeeping, by step parity: written by the first workgroup of
+// the step's launch, read by the next launch (and by the step's hub bin).
+struct SsRec {
+    int32_t mode;
+    int32_t lo;          // window [lo, hi)
+    int32_t hi;
+    int32_t pull;        // the step pulls (a large F, or a heavy step over a large W)
+    int32_t defer;       // the push step relaxes light edges only and records F in W
+    int32_t ws_any;      // W holds vertices whose heavy edges wait
+    int32_t front;       // |F|
+    int32_t steps;       // relax steps up to and including this one
+    int32_t windows;     // windows entered after the first
+    int32_t pulls;       // relax steps that pulled
+    long long fdeg;      // out-edges of F
+    long long ws_fdeg;   // out-edges of the F's recorded in W this window (W's size in edges, at most)
+    int32_t uq;          // scans so far mod 2: the pending bounds this step lowers
+    int32_t pad_;
+};
 
 struct SsState {
     int32_t done;
-    int32_t mode;
-    int32_t phase;       // F = bm[phase % 3]
-    int32_t steps;       // relax steps run
-    int32_t windows;     // windows entered after the first
-    int32_t slots;       // recorded slots consumed
-    int32_t pulls;       // relax steps that pulled
-    int32_t hub_count;
-    int32_t lo;          // window [lo, hi)
-    int32_t hi;
+    int32_t slots;       // step launches of the run, when done
+    int32_t steps;       // relax steps, windows, pull steps, when done
+    int32_t windows;
+    int32_t pulls;
+    int32_t base;        // step of the first launch of a replayed block
     int32_t delta;
-    int32_t front;       // |F|
-    int32_t next_min;    // least pending distance >= hi seen by the last scan
-    int32_t pull;        // the next relax step pulls (large frontier)
     int32_t pull_min;    // |F| above which a step pulls
-    int32_t defer;       // the next push step relaxes light edges only and records F in W
-    int32_t ws_any;      // W holds vertices whose heavy edges wait
-    long long fdeg;      // out-edges of F
-    int32_t *dist;       // the run's distances (recorded slots read it from here)
-    int32_t hubs[kSsMaxHubs];
+    int32_t *dist;       // the run's distances (recorded launches read it from here)
+    SsRec rec[2];        // rec[1] holds step -1 after the reset
+    int32_t hub_n[3];    // hubs listed by step s in hubs[s % 3]
+    int32_t hubs[3][kSsMaxHubs];
 };
 
 struct SsArgs {
@@ -96,51 +125,120 @@ struct SsArgs {
     const int32_t *swt;
     const int32_t *lend;  // end of each light part (null: no light/heavy split)
     long long defer_min;  // out-edges of F from which its heavy edges wait
+    long long heavy_pull_min;  // W of this many out-edges or more: the heavy step pulls (0: never)
     const int32_t *poff;  // in-edges with their weights (the graph itself when symmetric)
     const int32_t *pnbr;
     const int32_t *pwt;
     int32_t *dist;
     int32_t V;
     int32_t words;
-    int32_t sum_words;
-    uint64_t *bm[3];
-    uint64_t *sum[3];
-    uint8_t *cflag;      // chunk flags of the bitmap being filled (bs_flag_chunk)
-    int32_t *umin;       // per unit of kSsPullUnit words: lower bound of its pending distances
-    uint64_t *ws;        // W: vertices of the window whose heavy edges wait
-    uint64_t *wsum;      // its chunk summary (built when the heavy step starts)
-    uint8_t *wsflag;     // its chunk flags
+    uint64_t *bm[3];      // F_s in bm[s % 3]
+    uint8_t *flags;       // kSsFlagSets sets of chunk flags, flag_bytes apart: F_s's in set (s - 1) % 4
+    int32_t flag_bytes;
+    // Per unit of kSsPullUnit words, lower bounds of its pending distances
+    // (dist >= hi): the least of ubase[u] (written by the window scans) and
+    // ucl[q * ustride + u] for q = 0, 1 (lowered by the relaxations of the
+    // steps from a scan with q = scans mod 2 up to the next scan, which
+    // resets it for the units it scans: it read what those steps wrote).
+    int32_t *ubase;
+    int32_t *ucl;
+    int32_t ustride;
+    uint64_t *ws;         // W: vertices of the window whose heavy edges wait
+    uint8_t *wsflag;      // its chunk flags
     int32_t chunks;
-    int32_t *slots;      // [3 kernels][3 counters][kSsMaxGrid], then row kSsMinSlots
+    int32_t *cnt;         // [kSsSets][kSsStripes] lines: |F'| (or moved)
+    long long *fdg;       // the same, out-edges
+    int32_t *mn;          // the same, least pending distance seen by a scan (INT_MAX: none; -1: it found vertices)
+    int32_t *host_flag;   // pinned {done, slots, steps, windows, pulls}: written when the run ends
     int32_t wide_chunks;       // pull: chunks of the in-lists of wide vertices
     const int32_t *chunk_v;    // vertex of each chunk
     const int32_t *chunk_e0;   // first in-edge of each chunk
 };
 
+__device__ __forceinline__ uint8_t *ss_flag_set(const SsArgs &a, int32_t i) {
+    return a.flags + (size_t)i * a.flag_bytes;
+}
+
+// The non-empty chunks among this workgroup's (chunk blockIdx + j G at bit
+// j) in one set of chunk flags; every wave computes the same mask.
+__device__ __forceinline__ uint64_t ss_chunk_mask(const uint8_t *flags, int32_t chunks) {
+    const int32_t ch = (int32_t)blockIdx.x + lane_id() * (int32_t)gridDim.x;
+    return __ballot(ch < chunks && flags[ch] != 0);
+}
+
 struct SsCtx {
     int32_t hi;
     uint64_t *bm_out;
-    int32_t nf;    // per-lane: new members of F'
-    int32_t fdeg;  // per-lane: their out-edges
+    uint8_t *cflag;  // chunk flags of bm_out
+    int32_t *ucl;    // this step's pending bounds
+    int32_t nf;      // per-lane: new members of F'
+    int32_t fdeg;    // per-lane: their out-edges
 };
 
+__device__ __forceinline__ SsCtx ss_ctx(const SsArgs &a, const SsRec &cur, int32_t ph) {
+    SsCtx c;
+    c.hi = cur.hi;
+    c.bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], (ph + 1) % 3);
+    c.cflag = ss_flag_set(a, ph & 3);
+    c.ucl = a.ucl + cur.uq * a.ustride;
+    c.nf = 0;
+    c.fdeg = 0;
+    return c;
+}
+
+// Workgroup totals of the lane counters into the step's stripes (set `set`):
+// one atomic per workgroup and counter.
+__device__ __forceinline__ void ss_flush(const SsArgs &a, int32_t set, int32_t nf, long long fdeg, int32_t mn) {
+    __shared__ int32_t rn[kSsBlock / 64], rm[kSsBlock / 64];
+    __shared__ long long rf[kSsBlock / 64];
+    for (int o = 32; o > 0; o >>= 1) {
+        nf += __shfl_xor(nf, o);
+        fdeg += __shfl_xor(fdeg, o);
+        const int32_t y = __shfl_xor(mn, o);
+        mn = y < mn ? y : mn;
+    }
+    if (lane_id() == 0) {
+        rn[threadIdx.x >> 6] = nf;
+        rf[threadIdx.x >> 6] = fdeg;
+        rm[threadIdx.x >> 6] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int32_t tn = 0, tm = INT_MAX;
+        long long tf = 0;
+        for (int i = 0; i < kSsBlock / 64; ++i) {
+            tn += rn[i];
+            tf += rf[i];
+            tm = rm[i] < tm ? rm[i] : tm;
+        }
+        const int32_t line = set * kSsStripes + (int32_t)(blockIdx.x % kSsStripes);
+        if (tn) atomicAdd(&a.cnt[line * kSsLineI32], tn);
+        if (tf) atomicAdd((unsigned long long *)&a.fdg[line * (kSsLineI32 / 2)], (unsigned long long)tf);
+        if (tm < INT_MAX) atomicMin(&a.mn[line * kSsLineI32], tm);
+    }
+}
+
 // writeMin (apps/sssp.cpp:24-27) of dist[v] to nd, known to be below the
-// value last read: a strict improvement below hi joins F'; one at or above
-// hi leaves the vertex pending.
+// value last read in this step. Below hi the vertex joins F' whether or not
+// this atomic is the one that lowers dist (a lower value written in this
+// step came from a relaxation that joins it too), so the atomic's result is
+// not waited for, and the bitmap atomic and the degree loads are in flight
+// with it. At or above hi the vertex is pending: its unit's bound for this
+// step's parity is kept at or below nd.
 __device__ __forceinline__ void ss_claim(const SsArgs &a, SsCtx &c, int32_t v, int32_t nd) {
-    const int32_t old = atomicMin(&a.dist[v], nd);
-    if (nd >= old) return;
-    if (nd >= c.hi) {  // pending: keep its unit's bound at or below nd
-        int32_t *m = &a.umin[v / (64 * kSsPullUnit)];
+    atomicMin(&a.dist[v], nd);
+    if (nd >= c.hi) {
+        int32_t *m = &c.ucl[v / (64 * kSsPullUnit)];
         if (nd < *m) atomicMin(m, nd);
         return;
     }
+    const int32_t deg = a.off[v + 1] - a.off[v];
     const uint64_t bit = 1ull << (v & 63);
     const unsigned long long o = atomicOr((unsigned long long *)&c.bm_out[v >> 6], bit);
     if (!(o & bit)) {
         c.nf += 1;
-        c.fdeg += a.off[v + 1] - a.off[v];
-        if (o == 0) bs_flag_chunk(a.cflag, v >> 6);
+        c.fdeg += deg;
+        if (o == 0) bs_flag_chunk(c.cflag, v >> 6);
     }
 }
 
@@ -156,36 +254,77 @@ __device__ __forceinline__ void ss_relax(const SsArgs &a, SsCtx &c, const int32_
 
 constexpr int kSsBatch = 4;  // edges per thread with their loads in flight together
 
-// Push relax step. `pre` holds kBsMaxSumWords + 1 entries, `list`
-// kBsChunkVerts (both LDS).
-__device__ __forceinline__ void ss_push_body(const SsArgs &a, int32_t *pre, int32_t *list) {
+// Relaxes the out-edges of n listed vertices, in rounds of kSsBlock
+// vertices whose edges are spread evenly over the workgroup, each thread
+// finding its edge's owner by binary search in LDS; a vertex with at least
+// kSsHubMin edges in its range goes to the step's hub list (hubs[hl]) for
+// the grid-wide bin instead. get(j, s, e0, e1, du) gives entry j: vertex,
+// edge range [e0, e1) of (nbr, wt), distance. pre_s, e0_s, du_s hold
+// kSsBlock entries of LDS. Every thread of the workgroup calls it.
+template <class Get>
+__device__ __forceinline__ void ss_rounds(const SsArgs &a, SsCtx &c, int32_t n, const int32_t *nbr,
+                                          const int32_t *wt, int32_t hl, int32_t *pre_s, int32_t *e0_s,
+                                          int32_t *du_s, Get get) {
     SsState *st = a.st;
-    __shared__ int32_t pre_s[kSsBlock], e0_s[kSsBlock], du_s[kSsBlock];
+    for (int32_t b0 = 0; b0 < n; b0 += kSsBlock) {
+        const int32_t j = b0 + (int32_t)threadIdx.x;
+        const bool has = j < n;
+        int32_t s = 0, e0 = 0, e1 = 0, du = 0;
+        if (has) get(j, s, e0, e1, du);
+        const int32_t dg = e1 - e0;
+        const bool hub = has && dg >= kSsHubMin;
+        const int32_t hpos = wave_atomic_add(&st->hub_n[hl], hub);
+        if (hub && hpos < kSsMaxHubs) st->hubs[hl][hpos] = s;
+        const bool mine = has && (!hub || hpos >= kSsMaxHubs);
+        int32_t total = 0;
+        const int32_t ex = block_exclusive_scan(mine ? dg : 0, &total);
+        pre_s[threadIdx.x] = ex;
+        e0_s[threadIdx.x] = e0;
+        du_s[threadIdx.x] = du;
+        __syncthreads();
+        for (int32_t k0 = 0; k0 < total; k0 += kSsBlock * kSsBatch) {
+            int32_t v[kSsBatch], nd[kSsBatch], dv[kSsBatch];
+#pragma unroll
+            for (int p = 0; p < kSsBatch; ++p) {
+                const int32_t k = k0 + p * kSsBlock + (int32_t)threadIdx.x;
+                v[p] = -1;
+                nd[p] = 0;
+                if (k < total) {  // owner: the last entry whose prefix is <= k
+                    const int32_t o = upper_bound_i32(pre_s, 0, kSsBlock, k) - 1;
+                    const int32_t e = e0_s[o] + (k - pre_s[o]);
+                    v[p] = nbr[e];
+                    nd[p] = du_s[o] + wt[e];
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < kSsBatch; ++p) dv[p] = v[p] >= 0 ? a.dist[v[p]] : INT_MIN;
+#pragma unroll
+            for (int p = 0; p < kSsBatch; ++p)
+                if (nd[p] < dv[p]) ss_claim(a, c, v[p], nd[p]);
+        }
+        __syncthreads();
+    }
+}
+
+// Push relax step s (mode RELAX, or HEAVY: F is W and the edges are the
+// heavy ones), over this workgroup's chunks of F (`fl_in`: their mask).
+// `list` holds kBsChunkVerts entries (LDS).
+__device__ __forceinline__ void ss_push_body(const SsArgs &a, const SsRec &cur, int32_t ph, uint64_t fl_in,
+                                             int32_t *list, int32_t *pre_s, int32_t *e0_s, int32_t *du_s) {
     __shared__ int32_t n_s;
-    const bool heavy = st->mode == SS_HEAVY;  // F is W, edges from lend
-    const bool defer = !heavy && st->defer;   // edges up to lend; F joins W
+    const bool heavy = cur.mode == SS_HEAVY;  // F is W, edges from lend
+    const bool defer = !heavy && cur.defer;   // edges up to lend; F joins W
     const int32_t *nbr = heavy || defer ? a.snbr : a.nbr;  // split lists only where the split is used
     const int32_t *wt = heavy || defer ? a.swt : a.wt;
-    const int32_t phase = st->phase;
-    const int32_t iin = phase % 3, iout = (phase + 1) % 3, iclr = (phase + 2) % 3;
-    SsCtx c;
-    c.hi = st->hi;
-    c.bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iout);
-    c.nf = 0;
-    c.fdeg = 0;
-    // the bitmap two phases ahead is free (its summary is cleared by ss_decide)
-    bs_clear_chunks(bs_pick3(a.bm[0], a.bm[1], a.bm[2], iclr), bs_pick3(a.sum[0], a.sum[1], a.sum[2], iclr),
-                    a.words, a.sum_words, pre);
-    const uint64_t *bm_in = heavy ? a.ws : bs_pick3(a.bm[0], a.bm[1], a.bm[2], iin);
-    const uint64_t *si = heavy ? a.wsum : bs_pick3(a.sum[0], a.sum[1], a.sum[2], iin);
-    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    const int32_t nz = bs_summary_prefix(si, a.sum_words, pre);
-    for (int32_t k = blockIdx.x; k < nz; k += gridDim.x) {
-        const int32_t chunk = bs_kth_chunk(si, pre, a.sum_words, k);
+    SsCtx c = ss_ctx(a, cur, ph);
+    const uint64_t *bm_in = heavy ? a.ws : bs_pick3(a.bm[0], a.bm[1], a.bm[2], ph % 3);
+    const int32_t lane = lane_id();
+    for (uint64_t m = fl_in; m; m &= m - 1) {  // uniform
+        const int32_t chunk = (int32_t)blockIdx.x + (__ffsll((unsigned long long)m) - 1) * (int32_t)gridDim.x;
         if (threadIdx.x < 64) {  // wave 0 lists the chunk's members
             const int32_t w = chunk * kBsChunkWords + (int32_t)threadIdx.x;
             uint64_t word = w < a.words ? bm_in[w] : 0ull;
-            if (word) {  // this lane alone owns word w of W in this step
+            if (word) {  // this workgroup alone owns the chunk's words of W in this step
                 if (heavy) {
                     a.ws[w] = 0;  // consumed
                 } else if (defer) {
@@ -210,72 +349,46 @@ __device__ __forceinline__ void ss_push_body(const SsArgs &a, int32_t *pre, int3
         }
         __syncthreads();
         const int32_t n = n_s;
-        // rounds of 256 members: their edges (hubs aside) are spread evenly
-        // over the workgroup, each thread finding its edge's owner in LDS
-        for (int32_t b0 = 0; b0 < n; b0 += kSsBlock) {
-            const int32_t j = b0 + (int32_t)threadIdx.x;
-            const bool has = j < n;
-            const int32_t s = has ? list[j] : 0;
-            int32_t e0 = has ? a.off[s] : 0, e1 = has ? a.off[s + 1] : 0;
-            if (has && heavy) e0 = a.lend[s];
-            if (has && defer) e1 = a.lend[s];
-            const int32_t dg = e1 - e0;
-            const bool hub = has && dg >= kSsHubMin;
-            const int32_t hpos = wave_atomic_add(&st->hub_count, hub);
-            if (hub && hpos < kSsMaxHubs) st->hubs[hpos] = s;
-            const bool mine = has && (!hub || hpos >= kSsMaxHubs);
-            int32_t total = 0;
-            const int32_t ex = block_exclusive_scan(mine ? dg : 0, &total);
-            pre_s[threadIdx.x] = ex;
-            e0_s[threadIdx.x] = e0;
-            du_s[threadIdx.x] = mine ? a.dist[s] : 0;
-            __syncthreads();
-            for (int32_t k0 = 0; k0 < total; k0 += kSsBlock * kSsBatch) {
-                int32_t v[kSsBatch], nd[kSsBatch], dv[kSsBatch];
-#pragma unroll
-                for (int p = 0; p < kSsBatch; ++p) {
-                    const int32_t k = k0 + p * kSsBlock + (int32_t)threadIdx.x;
-                    v[p] = -1;
-                    nd[p] = 0;
-                    if (k < total) {  // owner: the last member whose prefix is <= k
-                        const int32_t o = upper_bound_i32(pre_s, 0, kSsBlock, k) - 1;
-                        const int32_t e = e0_s[o] + (k - pre_s[o]);
-                        v[p] = nbr[e];
-                        nd[p] = du_s[o] + wt[e];
-                    }
-                }
-#pragma unroll
-                for (int p = 0; p < kSsBatch; ++p) dv[p] = v[p] >= 0 ? a.dist[v[p]] : INT_MIN;
-#pragma unroll
-                for (int p = 0; p < kSsBatch; ++p)
-                    if (nd[p] < dv[p]) ss_claim(a, c, v[p], nd[p]);
-            }
-            __syncthreads();
-        }
+        // W's chunk flag is consumed only now: every wave of the workgroup
+        // took its ballot of the flags (at the start of the launch) before
+        // this barrier
+        if (heavy && threadIdx.x == 0) a.wsflag[chunk] = 0;
+        ss_rounds(a, c, n, nbr, wt, ph % 3, pre_s, e0_s, du_s,
+                  [&](int32_t j, int32_t &s, int32_t &e0, int32_t &e1, int32_t &du) {
+                      s = list[j];
+                      e0 = heavy ? a.lend[s] : a.off[s];
+                      e1 = defer ? a.lend[s] : a.off[s + 1];
+                      du = a.dist[s];
+                  });
         __syncthreads();
     }
-    (void)wid;
-    bs_flush3<kSsBlock>(a.slots, kSsMaxGrid, c.nf, c.fdeg, 0);
+    ss_flush(a, ph % kSsSets, c.nf, c.fdeg, INT_MAX);
 }
 
 // Pull relax step for a large frontier: every vertex not yet settled below
 // the window takes the minimum of dist[u] + w over its in-neighbours u in F
 // (no atomics on dist: a vertex's word of 64 vertices belongs to one wave).
 // Work is in units of 16 words per workgroup (4 per wave), so a unit's
-// summary bit costs at most one atomic.
+// chunk flag costs at most one store.
 constexpr int32_t kSsPullWide = 4096;            // in-degree at least this: chunked pull
 constexpr int32_t kSsPullChunk = 16 * 1024;      // in-edges per chunk (a workgroup's item)
 constexpr int kSsPasses = 4;                     // cooperative passes in flight per wave
 
-// Pull relax step; `best_s` holds kSsBlock entries of LDS.
-__device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int32_t *best_s) {
-    SsState *st = a.st;
-    const int32_t phase = st->phase, hi = st->hi, lo = st->lo;
-    const int32_t iin = phase % 3, iout = (phase + 1) % 3, iclr = (phase + 2) % 3;
-    bs_clear_chunks(bs_pick3(a.bm[0], a.bm[1], a.bm[2], iclr), bs_pick3(a.sum[0], a.sum[1], a.sum[2], iclr),
-                    a.words, a.sum_words, pre);
-    const uint64_t *bm_in = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iin);
-    uint64_t *bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], iout);
+// Pull relax step; `best_s` holds kSsBlock entries of LDS. The heavy step
+// pulls the same way (symmetric graphs, whose in-lists are the split
+// out-lists): F is W, the in-edges are each list's heavy part, and only
+// vertices beyond hi can improve (a heavy edge from [lo, hi) lands at or
+// beyond hi).
+__device__ __forceinline__ void ss_pull_body(const SsArgs &a, const SsRec &cur, int32_t ph, int32_t *best_s) {
+    const int32_t hi = cur.hi;
+    const bool heavy = cur.mode == SS_HEAVY;
+    const int32_t settled = heavy ? hi : cur.lo;  // vertices at or below this cannot improve
+    const int32_t *pnbr = heavy ? a.snbr : a.pnbr;
+    const int32_t *pwt = heavy ? a.swt : a.pwt;
+    const uint64_t *bm_in = heavy ? a.ws : bs_pick3(a.bm[0], a.bm[1], a.bm[2], ph % 3);
+    uint64_t *bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], (ph + 1) % 3);
+    uint8_t *cflag = ss_flag_set(a, ph & 3);
+    int32_t *ucl = a.ucl + cur.uq * a.ustride;  // this step's pending bounds
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
     int32_t *best = best_s + 64 * wid;
     int32_t nf = 0, fdeg = 0;
@@ -287,13 +400,11 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int3
             if (w >= a.words) break;  // wave-uniform
             const int32_t v = w * 64 + lane;
             const int32_t dv = v < a.V ? a.dist[v] : INT_MAX;
-            bool open = v < a.V && dv > lo;  // settled vertices cannot improve
-            const int32_t e0 = open ? a.poff[v] : 0;
-            int32_t len = open ? a.poff[v + 1] - e0 : 0;
-            if (len >= kSsPullWide) {  // wide: the chunk items below handle it
-                open = false;
-                len = 0;
-            }
+            bool open = v < a.V && dv > settled;
+            const int32_t p0 = open ? a.poff[v] : 0, p1 = open ? a.poff[v + 1] : 0;
+            if (p1 - p0 >= kSsPullWide) open = false;  // wide: the chunk items below handle it
+            const int32_t e0 = open ? (heavy ? a.lend[v] : p0) : 0;
+            const int32_t len = open ? p1 - e0 : 0;
             best[lane] = INT_MAX;
             int32_t incl = len;
             for (int o = 1; o < 64; o <<= 1) {
@@ -321,8 +432,8 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int3
                 }
 #pragma unroll
                 for (int p = 0; p < kSsPasses; ++p) {
-                    uu[p] = ee[p] >= 0 ? a.pnbr[ee[p]] : -1;
-                    ww[p] = ee[p] >= 0 ? a.pwt[ee[p]] : 0;
+                    uu[p] = ee[p] >= 0 ? pnbr[ee[p]] : -1;
+                    ww[p] = ee[p] >= 0 ? pwt[ee[p]] : 0;
                 }
 #pragma unroll
                 for (int p = 0; p < kSsPasses; ++p)
@@ -338,7 +449,7 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int3
                 const int32_t y = __shfl_xor(pend, o);
                 pend = y < pend ? y : pend;
             }
-            if (lane == 0 && pend < a.umin[un]) atomicMin(&a.umin[un], pend);
+            if (lane == 0 && pend < ucl[un]) atomicMin(&ucl[un], pend);
             const uint64_t near = __ballot(join);
             uint64_t old = 0;
             if (lane == 0 && near) {  // atomic: a wide vertex of the same word may join too
@@ -350,7 +461,7 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int3
             if (join && !((old >> lane) & 1ull)) fdeg += a.off[v + 1] - a.off[v];
             __builtin_amdgcn_wave_barrier();
         }
-        if (__syncthreads_or(any) && threadIdx.x == 0) bs_flag_chunk(a.cflag, un * kSsPullUnit);
+        if (__syncthreads_or(any) && threadIdx.x == 0) bs_flag_chunk(cflag, un * kSsPullUnit);
     }
     // wide vertices: each chunk of an in-list is a workgroup item; its minimum
     // joins dist[v] by atomicMin, and an improvement below hi joins F'
@@ -358,16 +469,17 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int3
     for (int32_t k = blockIdx.x; k < a.wide_chunks; k += gridDim.x) {
         const int32_t v = a.chunk_v[k];
         const int32_t dv = a.dist[v];
-        if (dv <= lo) continue;  // settled (block-uniform)
+        if (dv <= settled) continue;  // (a wave may see another workgroup's improvement: no barrier here)
         const int32_t end = a.poff[v + 1];
-        const int32_t e0 = a.chunk_e0[k] + wid * per_wave;
+        int32_t e0 = a.chunk_e0[k] + wid * per_wave;
         const int32_t e1 = e0 + per_wave < end ? e0 + per_wave : end;
+        if (heavy && e0 < a.lend[v]) e0 = a.lend[v];  // the heavy part of the list
         int32_t m = INT_MAX;
 #pragma unroll 4
         for (int32_t e = e0 + lane; e < e1; e += 64) {
-            const int32_t u = a.pnbr[e];
+            const int32_t u = pnbr[e];
             if (bs_test(bm_in, u)) {
-                const int32_t c = a.dist[u] + a.pwt[e];
+                const int32_t c = a.dist[u] + pwt[e];
                 m = c < m ? c : m;
             }
         }
@@ -378,7 +490,7 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int3
         if (lane == 0 && m < dv) {
             const int32_t old = atomicMin(&a.dist[v], m);
             if (m < old && m >= hi) {
-                int32_t *um = &a.umin[v / (64 * kSsPullUnit)];
+                int32_t *um = &ucl[v / (64 * kSsPullUnit)];
                 if (m < *um) atomicMin(um, m);
             }
             if (m < old && m < hi) {
@@ -387,45 +499,313 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, int32_t *pre, int3
                 if (!(o & bit)) {
                     nf += 1;
                     fdeg += a.off[v + 1] - a.off[v];
-                    if (o == 0) bs_flag_chunk(a.cflag, v >> 6);
+                    if (o == 0) bs_flag_chunk(cflag, v >> 6);
                 }
             }
         }
     }
-    bs_flush3<kSsBlock>(a.slots, kSsMaxGrid, nf, fdeg, 0);
+    ss_flush(a, ph % kSsSets, nf, fdeg, INT_MAX);
 }
 
-// One relax step, push or pull as the decide kernels set `pull` (one launch
-// either way; the body not taken costs nothing).
-__global__ void __launch_bounds__(kSsBlock) ss_relax_kernel(SsArgs a) {
-    SsState *st = a.st;
-    if (st->done || st->mode == SS_ADVANCE) return;  // uniform
-    a.dist = st->dist;
-    __shared__ int32_t pre[kBsMaxSumWords + 1];
+// Window scan-and-relax (mode ADVANCE), over the units whose pending bound
+// is below hi: each workgroup lists the vertices of its units with
+// lo <= dist < hi (4 words per wave, loads in flight together, their edge
+// ranges loaded with them) and relaxes their out-edges at once, as a push
+// step would in the next launch; improvements below hi join F_s+1. The
+// unit's base bound becomes its least distance >= hi and the bound lowered
+// since the previous scan is reset (those relaxations ran in earlier
+// launches, whose writes to dist this scan read); this step's own
+// relaxations lower the other bound, which the scan leaves alone, so no
+// bound is lost to a concurrent relaxation. A unit whose bound is at least hi holds
+// nothing below hi and is skipped. The min stripes get -1 when the
+// workgroup found vertices, else the least bound over its units (the least
+// pending distance, exact when no vertex was found anywhere). lds holds
+// kBsChunkVerts entries; pre_s, e0_s, du_s kSsBlock each.
+__device__ __forceinline__ void ss_scan_body(const SsArgs &a, const SsRec &cur, int32_t ph, int32_t *lds,
+                                             int32_t *pre_s, int32_t *e0_s, int32_t *du_s) {
+    __shared__ int32_t wcnt[kSsBlock / 64], ured[kSsBlock / 64];
+    const int32_t lo = cur.lo, hi = cur.hi;
+    SsCtx c = ss_ctx(a, cur, ph);
+    int32_t *uold = a.ucl + (cur.uq ^ 1) * a.ustride;  // bounds lowered since the previous scan
+    int32_t *lv = lds, *ldu = lds + 1024, *le0 = lds + 2048, *le1 = lds + 3072;  // a unit's 1024 vertices at most
+    static_assert(kSsPullUnit * 64 * 4 <= kBsChunkVerts, "a unit's list fits the LDS buffer");
+    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    int32_t beyond = INT_MAX;  // thread 0: least bound over this workgroup's units
+    bool found = false;        // block-uniform
+    const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
+    for (int32_t un = blockIdx.x; un < units; un += gridDim.x) {
+        // whether the bound is below hi is the same for every wave: this
+        // step's relaxations lower c.ucl only to values >= hi
+        const int32_t b0 = a.ubase[un], b1 = uold[un], b2 = c.ucl[un];
+        int32_t um = b0 < b1 ? b0 : b1;
+        um = b2 < um ? b2 : um;
+        if (um >= hi) {
+            beyond = um < beyond ? um : beyond;
+            continue;
+        }
+        const int32_t w0 = un * kSsPullUnit + wid * 4;
+        int32_t dv[4], d0[4], d1[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int32_t v = (w0 + i) * 64 + lane;
+            const bool ok = v < a.V;
+            dv[i] = ok ? a.dist[v] : INT_MAX;
+            d0[i] = ok ? a.off[v] : 0;
+            d1[i] = ok ? a.off[v + 1] : 0;
+        }
+        uint64_t m[4];
+        int32_t cnt = 0, ub = INT_MAX;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            m[i] = __ballot(dv[i] >= lo && dv[i] < hi);  // dv is INT_MAX past V
+            cnt += __popcll(m[i]);
+            if (dv[i] >= hi && dv[i] < ub) ub = dv[i];
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            const int32_t y = __shfl_xor(ub, o);
+            ub = y < ub ? y : ub;
+        }
+        if (lane == 0) {
+            wcnt[wid] = cnt;
+            ured[wid] = ub;
+        }
+        __syncthreads();
+        int32_t pos = 0, n = 0, mb = INT_MAX;
+        for (int i = 0; i < kSsBlock / 64; ++i) {
+            pos += i < wid ? wcnt[i] : 0;
+            n += wcnt[i];
+            mb = ured[i] < mb ? ured[i] : mb;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if ((m[i] >> lane) & 1ull) {
+                const int32_t q = pos + (int32_t)lanes_below(m[i]);
+                lv[q] = (w0 + i) * 64 + lane;
+                ldu[q] = dv[i];
+                le0[q] = d0[i];
+                le1[q] = d1[i];
+            }
+            pos += __popcll(m[i]);
+        }
+        if (threadIdx.x == 0) {  // the unit's exact bound
+            a.ubase[un] = mb;
+            uold[un] = INT_MAX;
+            beyond = mb < beyond ? mb : beyond;
+        }
+        __syncthreads();  // the list is complete; wcnt and ured are read
+        if (n > 0) {      // uniform
+            found = true;
+            ss_rounds(a, c, n, a.nbr, a.wt, ph % 3, pre_s, e0_s, du_s,
+                      [&](int32_t j, int32_t &v, int32_t &e0, int32_t &e1, int32_t &du) {
+                          v = lv[j];
+                          e0 = le0[j];
+                          e1 = le1[j];
+                          du = ldu[j];
+                      });  // ends with a barrier: the list may be rewritten
+        }
+    }
+    if (threadIdx.x != 0) beyond = INT_MAX;  // thread 0 holds the workgroup's least bound
+    ss_flush(a, ph % kSsSets, c.nf, c.fdeg, found ? -1 : beyond);
+}
+
+__device__ __forceinline__ int32_t ss_window_end(int32_t lo, int32_t delta) {
+    return lo > INT_MAX - delta ? INT_MAX : lo + delta;
+}
+
+// The counts of step s - 1 from its stripes (set (s - 1) % 3); every lane of
+// the calling wave gets the totals.
+__device__ __forceinline__ void ss_counts(const SsArgs &a, int32_t set, int32_t *n, long long *fdeg, int32_t *mn) {
+    static_assert(kSsStripes == 64, "one stripe per lane");
+    const int32_t line = set * kSsStripes + lane_id();
+    int32_t c = a.cnt[line * kSsLineI32];
+    long long f = a.fdg[line * (kSsLineI32 / 2)];
+    int32_t m = a.mn[line * kSsLineI32];
+    for (int o = 32; o > 0; o >>= 1) {
+        c += __shfl_xor(c, o);
+        f += __shfl_xor(f, o);
+        const int32_t y = __shfl_xor(m, o);
+        m = y < m ? y : m;
+    }
+    *n = c;
+    *fdeg = f;
+    *mn = m;
+}
+
+// The decision of step s from step s - 1's record and counts (what the
+// reference's queue does between two dequeues, gpu_priority_queue.h:115-185,
+// with the light/heavy split): a relax, heavy or scan-and-relax step whose
+// F' is not empty is followed by a relax step in the same window; a settled
+// window with heavy edges waiting by the heavy step; otherwise by the
+// scan-and-relax of the next window [hi, hi + delta). A scan that found no
+// vertex is followed by a scan at the least pending distance it saw (the
+// reference's queue starts a window at the least remaining priority too,
+// gpu_priority_queue.h:126-146), or by the end when nothing is pending.
+__device__ __forceinline__ SsRec ss_decide(const SsArgs &a, const SsRec &prev, int32_t n, long long fdeg,
+                                           int32_t mn) {
+    const SsState *st = a.st;
+    SsRec r = prev;
+    if (prev.mode == SS_DONE) return r;
+    auto relax_on = [&](int32_t steps_before) {
+        r.mode = SS_RELAX;
+        r.front = n;
+        r.fdeg = fdeg;
+        r.pull = n > st->pull_min;
+        r.defer = a.lend != nullptr && !r.pull && fdeg >= a.defer_min;
+        r.steps = steps_before + 1;
+        r.pulls = prev.pulls + r.pull;
+    };
+    auto scan_at = [&](int32_t lo) {
+        r.mode = SS_ADVANCE;
+        r.uq = prev.uq ^ 1;
+        r.lo = lo;
+        r.hi = ss_window_end(lo, st->delta);
+        r.pull = 0;
+        r.defer = 0;
+    };
+    if (prev.mode == SS_RELAX || prev.mode == SS_HEAVY) {
+        const bool deferred = prev.mode == SS_RELAX && prev.defer;
+        const int32_t ws_any = prev.ws_any || deferred;
+        r.ws_any = ws_any;
+        r.ws_fdeg = ws_any ? prev.ws_fdeg + (deferred ? prev.fdeg : 0) : 0;
+        if (n > 0) {
+            relax_on(prev.steps);
+        } else if (prev.mode == SS_RELAX && ws_any) {  // settled, heavy edges waiting
+            r.mode = SS_HEAVY;
+            // W's heavy edges pushed once each, or, when W holds a large part
+            // of the graph's edges, pulled: every vertex beyond hi reads its
+            // heavy in-edges in order (no random atomics on dist)
+            r.pull = a.heavy_pull_min > 0 && r.ws_fdeg >= a.heavy_pull_min;
+            r.ws_any = 0;
+            r.ws_fdeg = 0;
+            r.defer = 0;
+            r.steps = prev.steps + 1;
+        } else {  // the window is settled
+            scan_at(prev.hi);
+        }
+    } else {  // a scan-and-relax step; mn < 0: it found (and relaxed) vertices of its window
+        const bool found = mn < 0;
+        r.steps = prev.steps + (found ? 1 : 0);
+        if (found) r.windows = prev.windows + 1;
+        if (n > 0) {
+            relax_on(r.steps);
+        } else if (found) {  // the window settled in one step
+            scan_at(prev.hi);
+        } else if (mn == INT_MAX) {
+            r.mode = SS_DONE;  // nothing pending
+        } else {  // empty window: the next one starts at the least pending distance
+            scan_at(mn);
+        }
+    }
+    return r;
+}
+
+// One step, one launch: every workgroup decides the step from what the
+// previous launch left (the same inputs and code, hence the same decision),
+// the first writes the step's record and clears what the next launch adds
+// into (its count set, its flag set, the hub list of two steps before),
+// each workgroup clears its chunks of the bitmap two steps ahead, then the
+// push, pull, heavy or scan body runs. Graph replays run launch k of a block
+// as step base + k with phase (base + k) mod 12; eager launches pass their
+// step with base 0.
+__global__ void __launch_bounds__(kSsBlock) ss_step_kernel(SsArgs a, int32_t k, int32_t ph) {
     __shared__ int32_t lds[kBsChunkVerts];
-    if (st->pull)
-        ss_pull_body(a, pre, lds);
-    else
-        ss_push_body(a, pre, lds);
+    __shared__ int32_t pre_s[kSsBlock], e0_s[kSsBlock], du_s[kSsBlock];
+    SsState *st = a.st;
+    const int32_t p2 = ph & 1, p3 = ph % 3, p4 = ph & 3;
+    const int32_t done = st->done;
+    const SsRec prev = st->rec[p2 ^ 1];
+    int32_t n, mn;
+    long long fdeg;
+    ss_counts(a, (p3 + 2) % kSsSets, &n, &fdeg, &mn);  // every wave: the same totals
+    // every chunk mask the step may use, before any wave of the workgroup
+    // writes a flag
+    const uint64_t fl_clear = ss_chunk_mask(ss_flag_set(a, (p4 + 2) & 3), a.chunks);
+    const uint64_t fl_f = ss_chunk_mask(ss_flag_set(a, (p4 + 3) & 3), a.chunks);
+    const uint64_t fl_ws = ss_chunk_mask(a.wsflag, a.chunks);
+    if (done) return;  // uniform
+    a.dist = st->dist;
+    const SsRec cur = ss_decide(a, prev, n, fdeg, mn);
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) {
+            st->rec[p2] = cur;
+            st->hub_n[(p3 + 2) % 3] = 0;
+            if (cur.mode == SS_DONE) {
+                st->slots = st->base + k;
+                st->steps = cur.steps;
+                st->windows = cur.windows;
+                st->pulls = cur.pulls;
+                st->done = 1;
+                if (a.host_flag) {  // the host waits on the replay's event, then reads these
+                    a.host_flag[1] = st->base + k;
+                    a.host_flag[2] = cur.steps;
+                    a.host_flag[3] = cur.windows;
+                    a.host_flag[4] = cur.pulls;
+                    a.host_flag[0] = 1;
+                }
+            }
+        }
+        const int32_t cs = (p3 + 1) % kSsSets;
+        if (threadIdx.x < kSsStripes) {
+            const int32_t line = cs * kSsStripes + (int32_t)threadIdx.x;
+            a.cnt[line * kSsLineI32] = 0;
+            a.fdg[line * (kSsLineI32 / 2)] = 0;
+            a.mn[line * kSsLineI32] = INT_MAX;
+        }
+        uint4 *f4 = reinterpret_cast<uint4 *>(ss_flag_set(a, (p4 + 1) & 3));
+        const int32_t n16 = (a.chunks + 15) / 16;
+        for (int32_t i = threadIdx.x; i < n16; i += kSsBlock) f4[i] = make_uint4(0u, 0u, 0u, 0u);
+    }
+    if (cur.mode == SS_DONE) return;  // uniform
+    // the bitmap two steps ahead (F_s-1, done with) is free: clear this
+    // workgroup's non-empty chunks of it
+    {
+        uint64_t *bc = bs_pick3(a.bm[0], a.bm[1], a.bm[2], (p3 + 2) % 3);
+        for (uint64_t m = fl_clear; m; m &= m - 1) {  // uniform
+            const int32_t ch = (int32_t)blockIdx.x + (__ffsll((unsigned long long)m) - 1) * (int32_t)gridDim.x;
+            if (threadIdx.x < kBsChunkWords) {
+                const int32_t w = ch * kBsChunkWords + (int32_t)threadIdx.x;
+                if (w < a.words) bc[w] = 0;
+            }
+        }
+    }
+    if (prev.mode == SS_HEAVY && prev.pull) {  // uniform: a pulled W is done with
+        // (its bits were read by any workgroup, so the owners free it a step later)
+        for (uint64_t m = fl_ws; m; m &= m - 1) {
+            const int32_t ch = (int32_t)blockIdx.x + (__ffsll((unsigned long long)m) - 1) * (int32_t)gridDim.x;
+            if (threadIdx.x < kBsChunkWords) {
+                const int32_t w = ch * kBsChunkWords + (int32_t)threadIdx.x;
+                if (w < a.words) a.ws[w] = 0;
+            }
+        }
+        __syncthreads();  // (the flags were read at the start of the launch)
+        if (threadIdx.x < 64 && ((fl_ws >> threadIdx.x) & 1ull))
+            a.wsflag[blockIdx.x + threadIdx.x * gridDim.x] = 0;
+    }
+    if (cur.mode == SS_ADVANCE) {
+        ss_scan_body(a, cur, ph, lds, pre_s, e0_s, du_s);
+    } else if (cur.pull) {  // a pull relax step, or the heavy step pulled
+        ss_pull_body(a, cur, ph, lds);
+    } else {
+        ss_push_body(a, cur, ph, cur.mode == SS_HEAVY ? fl_ws : fl_f, lds, pre_s, e0_s, du_s);
+    }
 }
 
-// Grid-wide bin: the concatenated edges of the deferred hubs, spread evenly.
-__global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a) {
+// Grid-wide bin of step s (launched after it on graphs with hubs): the
+// concatenated edges of the hubs its push step deferred, spread evenly.
+__global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a, int32_t k, int32_t ph) {
+    (void)k;
     SsState *st = a.st;
-    if (st->done || st->mode == SS_ADVANCE || st->pull) return;
-    int32_t nh = st->hub_count;
-    if (nh <= 0) return;  // slots stay zero
+    const SsRec cur = st->rec[ph & 1];
+    if (cur.mode == SS_DONE || cur.pull) return;  // (a scan-and-relax lists hubs too)
+    const int32_t hl = ph % 3;
+    int32_t nh = st->hub_n[hl];
+    if (st->done || nh <= 0) return;
     a.dist = st->dist;
     if (nh > kSsMaxHubs) nh = kSsMaxHubs;
-    const bool heavy = st->mode == SS_HEAVY, defer = !heavy && st->defer;  // the step's edge range
+    const bool heavy = cur.mode == SS_HEAVY, defer = !heavy && cur.defer;  // the step's edge range
     const int32_t *nbr = heavy || defer ? a.snbr : a.nbr;
     const int32_t *wt = heavy || defer ? a.swt : a.wt;
-    const int32_t phase = st->phase;
-    SsCtx c;
-    c.hi = st->hi;
-    c.bm_out = bs_pick3(a.bm[0], a.bm[1], a.bm[2], (phase + 1) % 3);
-    c.nf = 0;
-    c.fdeg = 0;
+    SsCtx c = ss_ctx(a, cur, ph);
     __shared__ int32_t hscan[kSsMaxHubs + 1];
     __shared__ int32_t hs[kSsMaxHubs];  // first edge
     __shared__ int32_t hd[kSsMaxHubs];
@@ -433,7 +813,7 @@ __global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a) {
     const int32_t lo = threadIdx.x * per, hi = lo + per < nh ? lo + per : nh;
     int32_t part = 0;
     for (int32_t i = lo; i < hi; ++i) {
-        const int32_t s = st->hubs[i];
+        const int32_t s = st->hubs[hl][i];
         const int32_t e0 = heavy ? a.lend[s] : a.off[s], e1 = defer ? a.lend[s] : a.off[s + 1];
         hs[i] = e0;
         hd[i] = a.dist[s];
@@ -442,218 +822,18 @@ __global__ void __launch_bounds__(kSsBlock) ss_hub_kernel(SsArgs a) {
     int32_t total = 0;
     int32_t ex = block_exclusive_scan(part, &total);
     for (int32_t i = lo; i < hi; ++i) {
-        const int32_t s = st->hubs[i];
+        const int32_t s = st->hubs[hl][i];
         hscan[i] = ex;
         ex += (defer ? a.lend[s] : a.off[s + 1]) - hs[i];
     }
     __syncthreads();
     const int32_t chunk = (total + gridDim.x - 1) / gridDim.x;
     const int32_t k0 = blockIdx.x * chunk, k1 = k0 + chunk < total ? k0 + chunk : total;
-    for (int32_t k = k0 + (int32_t)threadIdx.x; k < k1; k += kSsBlock) {
-        const int32_t j = upper_bound_i32(hscan, 0, nh, k) - 1;
-        ss_relax(a, c, nbr, wt, hd[j], hs[j] + (k - hscan[j]), true);
+    for (int32_t e = k0 + (int32_t)threadIdx.x; e < k1; e += kSsBlock) {
+        const int32_t j = upper_bound_i32(hscan, 0, nh, e) - 1;
+        ss_relax(a, c, nbr, wt, hd[j], hs[j] + (e - hscan[j]), true);
     }
-    bs_flush3<kSsBlock>(a.slots + 3 * kSsMaxGrid, kSsMaxGrid, c.nf, c.fdeg, 0);
-}
-
-// Sums rows r[0..N) of the slot array (row = region * 3 + counter) over
-// nslots workgroups in one pass, zeroing them for the next kernel.
-template <int N>
-__device__ __forceinline__ void ss_collect(int32_t *slot_base, const int32_t (&r)[N], int32_t nslots,
-                                          long long (&out)[N]) {
-    __shared__ long long red[N][kSsDecBlock / 64];
-    long long t[N];
-#pragma unroll
-    for (int j = 0; j < N; ++j) t[j] = 0;
-#pragma unroll 1
-    for (int32_t i = threadIdx.x; i < nslots; i += kSsDecBlock) {
-        int32_t x[N];
-#pragma unroll
-        for (int j = 0; j < N; ++j) x[j] = slot_base[r[j] * kSsMaxGrid + i];
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-            t[j] += x[j];
-            slot_base[r[j] * kSsMaxGrid + i] = 0;
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < N; ++j) {
-        for (int o = 32; o > 0; o >>= 1) t[j] += __shfl_xor(t[j], o);
-        if (lane_id() == 0) red[j][threadIdx.x >> 6] = t[j];
-    }
-    __syncthreads();
-    const int32_t lane = lane_id();
-#pragma unroll
-    for (int j = 0; j < N; ++j) {  // every wave reduces the per-wave sums
-        long long v = lane < kSsDecBlock / 64 ? red[j][lane] : 0;
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-        out[j] = v;
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ int32_t ss_window_end(int32_t lo, int32_t delta) {
-    return lo > INT_MAX - delta ? INT_MAX : lo + delta;
-}
-
-__global__ void __launch_bounds__(kSsDecBlock) ss_decide_kernel(SsArgs a, int32_t nslots) {
-    SsState *st = a.st;
-    // every thread reads the state before the barriers in ss_collect; thread 0
-    // rewrites it after them
-    const int32_t mode = st->mode;
-    if (st->done || mode == SS_ADVANCE) return;  // uniform
-    const int32_t phase = st->phase;
-    const bool ws_any = st->ws_any || (mode == SS_RELAX && st->defer);
-    const int32_t rows[4] = {0, 3, 1, 4};  // |F'| and its out-edges from the relax and hub kernels
-    long long c[4];
-    ss_collect(a.slots, rows, nslots, c);
-    const long long nf = c[0] + c[1], fdeg = c[2] + c[3];
-    uint64_t *sclr = bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 2) % 3);
-    for (int32_t i = threadIdx.x; i <= a.sum_words; i += kSsDecBlock) sclr[i] = 0;
-    // the summary of F', from the flags its chunks set
-    bs_build_summary<kSsDecBlock>(a.cflag, a.chunks, bs_pick3(a.sum[0], a.sum[1], a.sum[2], (phase + 1) % 3));
-    const bool to_heavy = nf == 0 && mode == SS_RELAX && ws_any;  // settled, heavy edges waiting
-    if (to_heavy)
-        bs_build_summary<kSsDecBlock>(a.wsflag, a.chunks, a.wsum);
-    else if (mode == SS_HEAVY)  // the heavy step consumed W
-        for (int32_t i = threadIdx.x; i <= a.sum_words; i += kSsDecBlock) a.wsum[i] = 0;
-    if (threadIdx.x == 0) {
-        st->phase = phase + 1;
-        st->steps += 1;
-        st->pulls += st->pull;
-        st->front = (int32_t)nf;
-        st->fdeg = fdeg;
-        st->hub_count = 0;
-        st->pull = nf > st->pull_min;
-        st->defer = a.lend != nullptr && !st->pull && fdeg >= a.defer_min;
-        st->ws_any = ws_any;
-        if (to_heavy) {
-            st->mode = SS_HEAVY;
-            st->ws_any = 0;
-            st->defer = 0;
-        } else if (nf == 0) {  // the window is settled: scan for [hi, hi + delta)
-            st->mode = SS_ADVANCE;
-            st->lo = st->hi;
-            st->hi = ss_window_end(st->hi, st->delta);
-            st->next_min = INT_MAX;
-        } else if (mode == SS_HEAVY) {
-            st->mode = SS_RELAX;  // not reached: heavy edges land at or beyond hi
-        }
-    }
-}
-
-// Window scan over the units whose pending bound is below hi: every vertex
-// with lo <= dist < hi joins F (each wave owns its words: plain stores into
-// the empty bitmap), the unit's bound becomes its least distance >= hi, and
-// the out-edges of the joined vertices are counted. A unit whose bound is at
-// least hi holds nothing below hi and is skipped. The least bound over all
-// units (the least pending distance) goes to the workgroup's min slot. Units
-// of 16 words per workgroup (4 per wave, loads in flight together); a unit's
-// chunk flag costs one store.
-__global__ void __launch_bounds__(kSsBlock) ss_farmove_kernel(SsArgs a) {
-    SsState *st = a.st;
-    if (st->done || st->mode != SS_ADVANCE) return;
-    a.dist = st->dist;
-    __shared__ int32_t ured[2][kSsBlock / 64];
-    const int32_t lo = st->lo, hi = st->hi, phase = st->phase;
-    uint64_t *f = bs_pick3(a.bm[0], a.bm[1], a.bm[2], phase % 3);  // empty: the last step found nothing
-    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    int32_t moved = 0, fdeg = 0;
-    int32_t beyond = INT_MAX;  // thread 0: least bound over this workgroup's units
-    int32_t scanned = 0;       // block-uniform: LDS buffer of the unit reduction
-    const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
-    for (int32_t un = blockIdx.x; un < units; un += gridDim.x) {
-        const int32_t um = a.umin[un];  // block-uniform
-        if (um >= hi) {
-            beyond = um < beyond ? um : beyond;
-            continue;
-        }
-        const int32_t w0 = un * kSsPullUnit + wid * 4;
-        int32_t dv[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int32_t v = (w0 + i) * 64 + lane;
-            dv[i] = v < a.V ? a.dist[v] : INT_MAX;
-        }
-        bool any = false;
-        int32_t ub = INT_MAX;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int32_t v = (w0 + i) * 64 + lane;
-            const bool in = dv[i] >= lo && dv[i] < hi;  // dv is INT_MAX past V
-            const uint64_t m = __ballot(in);
-            if (in) fdeg += a.off[v + 1] - a.off[v];
-            if (dv[i] >= hi && dv[i] < ub) ub = dv[i];
-            if (m) {
-                any = true;
-                if (lane == 0) {
-                    f[w0 + i] = m;
-                    moved += __popcll(m);
-                }
-            }
-        }
-        for (int o = 32; o > 0; o >>= 1) {
-            const int32_t y = __shfl_xor(ub, o);
-            ub = y < ub ? y : ub;
-        }
-        int32_t *r = ured[scanned & 1];
-        if (lane == 0) r[wid] = ub;
-        ++scanned;
-        if (__syncthreads_or(any) && threadIdx.x == 0) bs_flag_chunk(a.cflag, un * kSsPullUnit);
-        if (threadIdx.x == 0) {  // the unit's exact bound
-            int32_t m = r[0];
-            for (int i = 1; i < kSsBlock / 64; ++i) m = r[i] < m ? r[i] : m;
-            a.umin[un] = m;
-            beyond = m < beyond ? m : beyond;
-        }
-    }
-    if (threadIdx.x == 0)  // one slot per workgroup: no atomics on a single word
-        a.slots[kSsMinSlots * kSsMaxGrid + blockIdx.x] = beyond;
-    bs_flush3<kSsBlock>(a.slots + 6 * kSsMaxGrid, kSsMaxGrid, moved, fdeg, 0);
-}
-
-__global__ void __launch_bounds__(kSsDecBlock) ss_decide2_kernel(SsArgs a, int32_t nslots) {
-    SsState *st = a.st;
-    if (st->done) return;
-    if (threadIdx.x == 0) st->slots += 1;
-    if (st->mode != SS_ADVANCE) return;  // uniform
-    const int32_t rows[2] = {6, 7};  // joined vertices and their out-edges
-    long long c[2];
-    ss_collect(a.slots, rows, nslots, c);
-    const long long moved = c[0], fdeg = c[1];
-    // the summary of the window's F
-    bs_build_summary<kSsDecBlock>(a.cflag, a.chunks, bs_pick3(a.sum[0], a.sum[1], a.sum[2], st->phase % 3));
-    // least pending distance over the window scan's workgroups
-    __shared__ int32_t red[kSsDecBlock / 64];
-    int32_t m = INT_MAX;
-    for (int32_t i = threadIdx.x; i < nslots; i += kSsDecBlock) {
-        const int32_t x = a.slots[kSsMinSlots * kSsMaxGrid + i];
-        m = x < m ? x : m;
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        const int32_t y = __shfl_xor(m, o);
-        m = y < m ? y : m;
-    }
-    if (lane_id() == 0) red[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < kSsDecBlock / 64; ++i) m = red[i] < m ? red[i] : m;
-        st->next_min = m;
-        if (moved > 0) {
-            st->mode = SS_RELAX;
-            st->front = (int32_t)moved;
-            st->fdeg = fdeg;
-            st->pull = moved > st->pull_min;
-            st->defer = a.lend != nullptr && !st->pull && fdeg >= a.defer_min;
-            st->windows += 1;
-        } else if (st->next_min == INT_MAX) {
-            st->done = 1;  // nothing pending
-        } else {  // empty window: the next one starts at the least pending distance
-            st->lo = st->next_min;
-            st->hi = ss_window_end(st->next_min, st->delta);
-            st->next_min = INT_MAX;
-        }
-    }
+    ss_flush(a, ph % kSsSets, c.nf, c.fdeg, INT_MAX);
 }
 
 struct SsSeed {
@@ -662,54 +842,63 @@ struct SsSeed {
     int32_t pull_min;
 };
 
+// Clears every array of a run, seeds the source as F_0 (bitmap 0, its chunk
+// flag in set 3, its count and out-degree in count set 2: what step 0 reads)
+// and writes the record of step -1 (a scan of the window [0, delta) that
+// found vertices and left the source in F', so step 0 relaxes it there).
 __global__ void ss_reset_kernel(SsArgs a, SsSeed sd) {
-    const int32_t n = a.V > a.sum_words + 1 ? a.V : a.sum_words + 1;
     const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
+    const int32_t lines = kSsSets * kSsStripes;
+    const int32_t fwords = kSsFlagSets * a.flag_bytes / 8;
+    const int32_t src = sd.source;
+    const int64_t fbyte = 3LL * a.flag_bytes + (src >> 6) / kBsChunkWords;  // the source's chunk flag in set 3
+    int32_t n = a.V > fwords ? a.V : fwords;
+    n = n > lines ? n : lines;
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (i < a.V) a.dist[i] = i == sd.source ? 0 : INT_MAX;
-        if (i < units) a.umin[i] = INT_MAX;  // nothing pending
-        if (i < a.words) a.ws[i] = 0;
-        if (i <= a.sum_words) a.wsum[i] = 0;
-        if (i < a.chunks) a.wsflag[i] = 0;
+        if (i < a.V) a.dist[i] = i == src ? 0 : INT_MAX;
+        if (i < units) {  // nothing pending
+            a.ubase[i] = INT_MAX;
+            a.ucl[i] = INT_MAX;
+            a.ucl[a.ustride + i] = INT_MAX;
+        }
         if (i < a.words) {
-            const uint64_t s = (i == (sd.source >> 6)) ? (1ull << (sd.source & 63)) : 0ull;
+            a.ws[i] = 0;
+            const uint64_t s = (i == (src >> 6)) ? (1ull << (src & 63)) : 0ull;
             a.bm[0][i] = s;
             a.bm[1][i] = 0;
             a.bm[2][i] = 0;
         }
-        if (i <= a.sum_words) {
-            const int32_t ch = (sd.source >> 6) / kBsChunkWords;
-            a.sum[0][i] = (i == (ch >> 6)) ? (1ull << (ch & 63)) : 0ull;
-            a.sum[1][i] = 0;
-            a.sum[2][i] = 0;
+        if (i < a.chunks) a.wsflag[i] = 0;
+        if (i < fwords)
+            reinterpret_cast<uint64_t *>(a.flags)[i] = (i == (int32_t)(fbyte >> 3)) ? 1ull << (8 * (fbyte & 7)) : 0ull;
+        if (i < lines) {
+            const bool seed = i == 2 * kSsStripes;  // count set 2, first stripe
+            a.cnt[i * kSsLineI32] = seed ? 1 : 0;
+            a.fdg[i * (kSsLineI32 / 2)] = seed ? (long long)(a.off[src + 1] - a.off[src]) : 0;
+            a.mn[i * kSsLineI32] = seed ? -1 : INT_MAX;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         SsState *st = a.st;
         st->done = 0;
-        st->mode = SS_RELAX;
-        st->phase = 0;
+        st->slots = 0;
         st->steps = 0;
         st->windows = 0;
-        st->slots = 0;
         st->pulls = 0;
-        st->hub_count = 0;
-        st->lo = 0;
-        st->hi = ss_window_end(0, sd.delta);
+        st->base = 0;
         st->delta = sd.delta;
-        st->front = 1;
-        st->next_min = INT_MAX;
-        st->pull = 0;
-        st->defer = 0;
-        st->ws_any = 0;
-        st->fdeg = 0;
         st->pull_min = sd.pull_min;
         st->dist = a.dist;
+        for (int i = 0; i < 3; ++i) st->hub_n[i] = 0;
+        SsRec r{};
+        r.mode = SS_ADVANCE;
+        r.lo = 0;
+        r.hi = ss_window_end(0, sd.delta);
+        r.steps = -1;    // the seeding scan is not counted as a step
+        r.windows = -1;  // nor is the first window counted as entered
+        st->rec[1] = r;
+        st->rec[0] = r;
     }
-}
-
-__global__ void ss_zero_slots_kernel(int32_t *slots, int32_t n) {
-    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) slots[i] = 0;
 }
 
 // Light/heavy split of the out-lists for one delta: flag[e] = w < delta,
@@ -743,47 +932,56 @@ __global__ void ss_lend_kernel(const int32_t *off, const int32_t *L, int32_t V, 
 struct SsWork {
     SsState *st = nullptr;
     uint64_t *bm[3] = {nullptr, nullptr, nullptr};
-    uint64_t *sum[3] = {nullptr, nullptr, nullptr};
-    uint8_t *cflag = nullptr;
+    uint8_t *flags = nullptr;  // kSsFlagSets sets of chunk flags, flag_bytes apart
+    int32_t flag_bytes = 0;
     int32_t chunks = 0;
-    int32_t *umin = nullptr;
-    uint64_t *ws = nullptr, *wsum = nullptr;
+    int32_t *ubase = nullptr, *ucl = nullptr;  // pending bounds
+    int32_t ustride = 0;
+    uint64_t *ws = nullptr;
     uint8_t *wsflag = nullptr;
+    int32_t *cnt = nullptr, *mn = nullptr;  // count stripes
+    long long *fdg = nullptr;
     int32_t *snbr = nullptr, *swt = nullptr, *lend = nullptr;  // light/heavy split
     int32_t split_delta = -1;  // delta of the split (-1: none built)
     bool split = false;        // the split has heavy edges and is used
-    int32_t *slots = nullptr;
-    int32_t sum_words = 0;
-    int32_t *host = nullptr;  // pinned: done, mode, phase, steps, windows, slots, pulls
+    int32_t *host_flag = nullptr;  // pinned, coherent: {done, slots, steps, windows, pulls} written by the last step
+    hipEvent_t flag_ev = nullptr;
     const int32_t *poff = nullptr, *pnbr = nullptr, *pwt = nullptr;
     int32_t pull_min = 0;
-    bool hubs = false;  // some out-degree reaches kSsHubMin: the hub kernel is launched
+    bool hubs = false;  // some out-degree reaches kSsHubMin: the hub kernel follows each step
     int32_t wide_chunks = 0;
     int32_t *chunk_v = nullptr, *chunk_e0 = nullptr;
-    std::map<int32_t, hipGraphExec_t> execs;  // by slot count (the run's dist is read from the state)
-    int32_t recent_slots[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // slots of the last runs (ring)
+    std::map<int32_t, hipGraphExec_t> execs;  // by steps * 12 + first phase (the run's dist is read from the state)
+    int32_t recent_steps[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // step launches of the last runs (ring)
     int32_t recent_n = 0;
 };
 
 static PlanRegistry<SsWork> g_ss;
 
-static void free_ss(SsWork *w) {
+static void drop_ss_execs(SsWork *w) {
     for (auto &kv : w->execs) (void)hipGraphExecDestroy(kv.second);
+    w->execs.clear();
+}
+
+static void free_ss(SsWork *w) {
+    drop_ss_execs(w);
     dfree(w->st);
     for (auto p : w->bm) dfree(p);
-    for (auto p : w->sum) dfree(p);
-    dfree(w->slots);
-    dfree(w->cflag);
-    dfree(w->umin);
+    dfree(w->flags);
+    dfree(w->ubase);
+    dfree(w->ucl);
     dfree(w->ws);
-    dfree(w->wsum);
     dfree(w->wsflag);
+    dfree(w->cnt);
+    dfree(w->mn);
+    dfree(w->fdg);
     dfree(w->snbr);
     dfree(w->swt);
     dfree(w->lend);
     dfree(w->chunk_v);
     dfree(w->chunk_e0);
-    if (w->host) (void)hipHostFree(w->host);
+    if (w->flag_ev) (void)hipEventDestroy(w->flag_ev);
+    if (w->host_flag) (void)hipHostFree(w->host_flag);
     delete w;
 }
 
@@ -791,11 +989,19 @@ void release_sssp(const gi_graph *g) {
     if (SsWork *p = g_ss.take(g)) free_ss(p);
 }
 
+// Persistent grid: resident workgroups of the step kernel.
+static int ss_grid() {
+    static const int per_cu = resident_per_cu((const void *)ss_step_kernel, kSsBlock);
+    int g = device_info().num_cus * per_cu;
+    return g < kSsMaxGrid ? g : kSsMaxGrid;
+}
+
 static int ensure_ss(gi_graph *g, SsWork **out) {
     if ((*out = g_ss.find(g)) != nullptr) return GI_OK;
     const int64_t words = bitmap_words(g->V);
-    const int64_t sum_words = ceil_div(ceil_div(words, kBsChunkWords), 64);
-    if (sum_words > kBsMaxSumWords) return fail(GI_ERR_RANGE, "fused SSSP supports up to 2^26 vertices");
+    const int64_t chunks = ceil_div(words, kBsChunkWords);
+    // every workgroup owns at most 64 chunks (one per lane of a ballot)
+    if (chunks > 64LL * ss_grid()) return fail(GI_ERR_RANGE, "fused SSSP: %d vertices exceed this device's grid", g->V);
     const gi_graph *pg = g;  // in-edges for pull steps
     if (g->directed) {
         gi_graph *t = nullptr;
@@ -806,26 +1012,29 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
     GI_TRY(ensure_max_degree(g, nullptr));
     SsWork *w = new SsWork();
     w->hubs = g->max_degree >= kSsHubMin;
-    w->sum_words = (int32_t)sum_words;
     w->poff = pg->d_off;
     w->pnbr = pg->d_nbr;
     w->pwt = pg->d_w;
     w->pull_min = g->V / kSsPullDiv;
+    w->chunks = (int32_t)chunks;
+    w->flag_bytes = (int32_t)bs_flag_bytes(chunks);  // a multiple of 16
+    const size_t lines = (size_t)kSsSets * kSsStripes;
     int rc = dalloc(&w->st, 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->bm[i], (size_t)words + 1);
-    for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->sum[i], (size_t)sum_words + 1);
-    if (rc == GI_OK) rc = dalloc(&w->slots, (kSsMinSlots + 1) * (size_t)kSsMaxGrid);
-    if (rc == GI_OK) rc = dalloc(&w->umin, (size_t)ceil_div(words, kSsPullUnit) + 1);
+    if (rc == GI_OK) rc = dalloc(&w->flags, (size_t)kSsFlagSets * w->flag_bytes);
+    w->ustride = (int32_t)ceil_div(words, kSsPullUnit) + 1;
+    if (rc == GI_OK) rc = dalloc(&w->ubase, (size_t)w->ustride);
+    if (rc == GI_OK) rc = dalloc(&w->ucl, 2 * (size_t)w->ustride);
     if (rc == GI_OK) rc = dalloc(&w->ws, (size_t)words + 1);
-    if (rc == GI_OK) rc = dalloc(&w->wsum, (size_t)sum_words + 1);
-    w->chunks = (int32_t)ceil_div(words, kBsChunkWords);
-    const size_t flag_bytes = (size_t)bs_flag_bytes(w->chunks);
-    if (rc == GI_OK) rc = dalloc(&w->cflag, flag_bytes);
-    if (rc == GI_OK && hipMemset(w->cflag, 0, flag_bytes) != hipSuccess) rc = fail(GI_ERR_HIP, "flag init");
-    if (rc == GI_OK) rc = dalloc(&w->wsflag, flag_bytes);
-    if (rc == GI_OK && hipMemset(w->wsflag, 0, flag_bytes) != hipSuccess) rc = fail(GI_ERR_HIP, "flag init");
-    if (rc == GI_OK && hipHostMalloc((void **)&w->host, 8 * sizeof(int32_t)) != hipSuccess)
+    if (rc == GI_OK) rc = dalloc(&w->wsflag, (size_t)w->flag_bytes);
+    if (rc == GI_OK) rc = dalloc(&w->cnt, lines * kSsLineI32);
+    if (rc == GI_OK) rc = dalloc(&w->mn, lines * kSsLineI32);
+    if (rc == GI_OK) rc = dalloc(&w->fdg, lines * (kSsLineI32 / 2));
+    if (rc == GI_OK &&
+        hipHostMalloc((void **)&w->host_flag, 8 * sizeof(int32_t), hipHostMallocCoherent) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
+    if (rc == GI_OK && hipEventCreateWithFlags(&w->flag_ev, hipEventDisableTiming) != hipSuccess)
+        rc = fail(GI_ERR_HIP, "event create");
     if (rc == GI_OK) {  // chunks of the wide in-lists, for pull steps
         std::vector<int32_t> poff((size_t)g->V + 1), cv, ce;
         if (hipMemcpy(poff.data(), pg->d_off, sizeof(int32_t) * (g->V + 1), hipMemcpyDeviceToHost) != hipSuccess)
@@ -845,11 +1054,6 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
              hipMemcpy(w->chunk_e0, ce.data(), sizeof(int32_t) * ce.size(), hipMemcpyHostToDevice) != hipSuccess))
             rc = fail(GI_ERR_HIP, "plan upload");
     }
-    if (rc == GI_OK) {
-        ss_zero_slots_kernel<<<64, 256>>>(w->slots, (kSsMinSlots + 1) * kSsMaxGrid);
-        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
-            rc = fail(GI_ERR_HIP, "slot init");
-    }
     if (rc != GI_OK) {
         free_ss(w);
         return rc;
@@ -857,13 +1061,6 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
     g_ss.put(g, w);
     *out = w;
     return GI_OK;
-}
-
-// Persistent grid: resident workgroups of the step kernel.
-static int ss_grid() {
-    static const int per_cu = resident_per_cu((const void *)ss_relax_kernel, kSsBlock);
-    int g = device_info().num_cus * per_cu;
-    return g < kSsMaxGrid ? g : kSsMaxGrid;
 }
 
 static SsArgs ss_args(gi_graph *g, SsWork *w, int32_t *dist) {
@@ -876,24 +1073,27 @@ static SsArgs ss_args(gi_graph *g, SsWork *w, int32_t *dist) {
     a.swt = w->split ? w->swt : nullptr;
     a.lend = w->split ? w->lend : nullptr;
     a.defer_min = g->E / kSsDeferDiv > 1 ? g->E / kSsDeferDiv : 1;
+    // symmetric graphs: the split out-lists are the split in-lists
+    a.heavy_pull_min = w->split && !g->directed ? std::max<long long>(g->E / kSsHeavyPullDiv, 1) : 0;
     a.poff = w->poff;
     a.pnbr = w->pnbr;
     a.pwt = w->pwt;
     a.dist = dist;
     a.V = g->V;
     a.words = (int32_t)bitmap_words(g->V);
-    a.sum_words = w->sum_words;
-    for (int i = 0; i < 3; ++i) {
-        a.bm[i] = w->bm[i];
-        a.sum[i] = w->sum[i];
-    }
-    a.slots = w->slots;
-    a.cflag = w->cflag;
-    a.umin = w->umin;
+    for (int i = 0; i < 3; ++i) a.bm[i] = w->bm[i];
+    a.flags = w->flags;
+    a.flag_bytes = w->flag_bytes;
+    a.ubase = w->ubase;
+    a.ucl = w->ucl;
+    a.ustride = w->ustride;
     a.ws = w->ws;
-    a.wsum = w->wsum;
     a.wsflag = w->wsflag;
     a.chunks = w->chunks;
+    a.cnt = w->cnt;
+    a.fdg = w->fdg;
+    a.mn = w->mn;
+    a.host_flag = w->host_flag;
     a.wide_chunks = w->wide_chunks;
     a.chunk_v = w->chunk_v;
     a.chunk_e0 = w->chunk_e0;
@@ -901,7 +1101,7 @@ static SsArgs ss_args(gi_graph *g, SsWork *w, int32_t *dist) {
 }
 
 // The light/heavy split for `delta` (built on first use of a delta; the
-// recorded slots are dropped when the arrays they name change).
+// recorded steps are dropped when the arrays they name change).
 static int ensure_split(gi_graph *g, SsWork *w, int32_t delta, hipStream_t s) {
     if (w->split_delta == delta) return GI_OK;
     const int64_t E = g->E;
@@ -933,23 +1133,56 @@ static int ensure_split(gi_graph *g, SsWork *w, int32_t delta, hipStream_t s) {
         dfree(L);
         GI_TRY(rc);
     }
-    if (use != w->split || use) {  // the arrays or their use changed
-        for (auto &kv : w->execs) (void)hipGraphExecDestroy(kv.second);
-        w->execs.clear();
-    }
+    if (use != w->split || use) drop_ss_execs(w);  // the arrays or their use changed
     w->split = use;
     w->split_delta = delta;
-    for (int32_t &k : w->recent_slots) k = 0;  // slot counts of other deltas do not carry over
+    for (int32_t &k : w->recent_steps) k = 0;  // step counts of other deltas do not carry over
     return GI_OK;
 }
 
-static void launch_slot(const SsArgs &a, bool hubs, Launcher &L) {
+static int ss_reset(SsWork *w, const SsArgs &a, int32_t source, int32_t delta, int32_t pull_min, hipStream_t s) {
+    w->host_flag[0] = 0;  // no run of this graph's engine is in flight: set again by the last step
+    const int64_t n = std::max<int64_t>({(int64_t)a.V, (int64_t)kSsFlagSets * a.flag_bytes / 8,
+                                         (int64_t)kSsSets * kSsStripes});
+    const int rg = (int)std::min<int64_t>(ceil_div(n, 256), 8192);
+    ss_reset_kernel<<<rg, 256, 0, s>>>(a, SsSeed{source, delta, pull_min});
+    GI_LAUNCH_CHECK();
+    return GI_OK;
+}
+
+// Launch k of a recorded block runs step base + k with phase (ph0 + k) mod
+// 12, ph0 = base mod 12: one recording serves every run on this graph and
+// delta and every block of a run that starts at the same phase. On a graph
+// with hubs each step is followed by its grid-wide hub bin.
+static int ss_step_graph(SsWork *w, const SsArgs &ca, int32_t steps, int32_t ph0, hipGraphExec_t *out) {
+    const int32_t key = steps * 12 + ph0;
+    auto it = w->execs.find(key);
+    if (it != w->execs.end()) {
+        *out = it->second;
+        return GI_OK;
+    }
     const int grid = ss_grid();
-    L(ss_relax_kernel, grid, kSsBlock, a);
-    if (hubs) L(ss_hub_kernel, grid, kSsBlock, a);
-    L(ss_decide_kernel, 1, kSsDecBlock, a, grid);
-    L(ss_farmove_kernel, grid, kSsBlock, a);
-    L(ss_decide2_kernel, 1, kSsDecBlock, a, grid);
+    hipGraphExec_t exec = nullptr;
+    GI_TRY(build_graph(
+        [&](Launcher &L) {
+            for (int32_t k = 0; k < steps; ++k) {
+                const int32_t ph = (ph0 + k) % 12;
+                L(ss_step_kernel, grid, kSsBlock, ca, k, ph);
+                if (w->hubs) L(ss_hub_kernel, grid, kSsBlock, ca, k, ph);
+            }
+        },
+        &exec));
+    if (w->execs.size() >= 12) drop_ss_execs(w);
+    w->execs[key] = exec;
+    *out = exec;
+    return GI_OK;
+}
+
+static int32_t ss_first_block(const SsWork *w) {
+    int32_t b = 0;
+    for (int32_t k : w->recent_steps) b = k + 1 > b ? k + 1 : b;
+    b = b > 1 ? (b + 7) / 8 * 8 : 16;
+    return b < kSsMaxBlock ? b : kSsMaxBlock;
 }
 
 int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,
@@ -958,123 +1191,93 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
     GI_TRY(ensure_ss(g, &w));
     GI_TRY(ensure_split(g, w, delta, s));
     SsArgs a = ss_args(g, w, d_dist);
-    const int32_t n = g->V > w->sum_words + 1 ? g->V : w->sum_words + 1;
-    int rg = (int)ceil_div(n, 256);
-    ss_reset_kernel<<<rg < 8192 ? rg : 8192, 256, 0, s>>>(a, SsSeed{source, delta, w->pull_min});
-    GI_LAUNCH_CHECK();
-    // the first block holds as many slots as the longest of the last runs
-    // (rounded up to 8; a slot past the end is five no-op launches, a block
-    // too short a host round trip), later ones 16
-    auto first_block = [&]() {
-        int32_t b = 0;
-        for (int32_t k : w->recent_slots) b = k > b ? k : b;
-        b = b > 0 ? (b + 7) / 8 * 8 : 16;
-        return b < 1024 ? b : 1024;
-    };
+    GI_TRY(ss_reset(w, a, source, delta, w->pull_min, s));
     SsArgs ca = a;
     ca.dist = nullptr;  // recorded kernels take the run's dist from the state
-    auto exec_of = [&](int32_t block, hipGraphExec_t *out) -> int {
-        auto it = w->execs.find(block);
-        if (it == w->execs.end()) {
-            if (w->execs.size() >= 8) {
-                for (auto &kv : w->execs) (void)hipGraphExecDestroy(kv.second);
-                w->execs.clear();
-            }
-            hipGraphExec_t exec = nullptr;
-            GI_TRY(build_graph(
-                [&](Launcher &L) {
-                    for (int32_t r = 0; r < block; ++r) launch_slot(ca, w->hubs, L);
-                },
-                &exec));
-            it = w->execs.emplace(block, exec).first;
-        }
-        *out = it->second;
-        return GI_OK;
-    };
-    int32_t block = first_block();
-    for (int64_t used = 0; used < (1ll << 24);) {
+    // Replay a block of step launches as long as the longest of the last runs
+    // (a launch past the end returns at once, a block too short costs a host
+    // round trip), wait on its event and read what the last step wrote to
+    // pinned memory; longer runs continue with blocks of 16, 32, ... 256
+    // steps, each after copying its first step into the state.
+    int32_t block = ss_first_block(w), next = 16;
+    for (int64_t step = 0; step < (1ll << 26);) {
         hipGraphExec_t exec = nullptr;
-        GI_TRY(exec_of(block, &exec));
+        GI_TRY(ss_step_graph(w, ca, block, (int32_t)(step % 12), &exec));
         GI_HIP(hipGraphLaunch(exec, s));
-        GI_HIP(hipMemcpyAsync(w->host, w->st, 7 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        GI_HIP(hipStreamSynchronize(s));
-        used += block;
-        if (w->host[0]) {  // done
-            w->recent_slots[w->recent_n++ % 8] = w->host[5];
-            if (counts) {  // steps, pull steps, windows entered, slots
-                counts[0] = w->host[3];
-                counts[1] = w->host[6];
-                counts[2] = w->host[4];
-                counts[3] = w->host[5];
+        step += block;
+        GI_HIP(hipEventRecord(w->flag_ev, s));
+        GI_HIP(hipEventSynchronize(w->flag_ev));
+        if (w->host_flag[0]) {
+            const int32_t *h = w->host_flag;
+            w->recent_steps[w->recent_n++ % 8] = h[1];
+            if (counts) {  // relax steps, pull steps, windows entered, step launches
+                counts[0] = h[2];
+                counts[1] = h[4];
+                counts[2] = h[3];
+                counts[3] = h[1];
             }
             // a longer run than before: record the next run's first block
             // now, so that its launch does not wait on the recording
-            hipGraphExec_t next = nullptr;
-            return exec_of(first_block(), &next);
+            hipGraphExec_t nx = nullptr;
+            return ss_step_graph(w, ca, ss_first_block(w), 0, &nx);
         }
-        block = 16;
+        w->host_flag[6] = (int32_t)step;  // the next block's first step
+        GI_HIP(hipMemcpyAsync(&w->st->base, &w->host_flag[6], sizeof(int32_t), hipMemcpyHostToDevice, s));
+        block = next;
+        next = next * 2 < 256 ? next * 2 : 256;
     }
     return fail(GI_ERR_RANGE, "SSSP did not finish");
 }
 
-// Eager slot loop for measurement: the launches of every slot bracketed by
-// HIP events; the state is read before each slot. The relax launch is
-// reported as a push or a pull step by the state's pull flag.
+// Eager step loop for measurement: each step launch (and its hub bin) with
+// HIP events around it; the step's record is read after it. A relax or heavy
+// step is reported as a push or a pull step by the record's pull flag.
 static int sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_min, int32_t *d_dist,
                       int32_t max_slots, float *kernel_ms, int64_t *info, int32_t *num_slots, hipStream_t s) {
     SsWork *w = nullptr;
     GI_TRY(ensure_ss(g, &w));
     GI_TRY(ensure_split(g, w, delta, s));
     SsArgs a = ss_args(g, w, d_dist);
-    const int32_t n = g->V > w->sum_words + 1 ? g->V : w->sum_words + 1;
-    int rg = (int)ceil_div(n, 256);
-    ss_reset_kernel<<<rg < 8192 ? rg : 8192, 256, 0, s>>>(a, SsSeed{source, delta, pull_min >= 0 ? pull_min : w->pull_min});
-    GI_LAUNCH_CHECK();
-    hipEvent_t ev[6] = {};
+    a.host_flag = nullptr;
+    GI_TRY(ss_reset(w, a, source, delta, pull_min >= 0 ? pull_min : w->pull_min, s));
+    hipEvent_t ev[3] = {};
     int rc = GI_OK;
-    for (int i = 0; i < 6 && rc == GI_OK; ++i)
+    for (int i = 0; i < 3 && rc == GI_OK; ++i)
         if (hipEventCreate(&ev[i]) != hipSuccess) rc = fail(GI_ERR_HIP, "event create");
     const int grid = ss_grid();
-    SsState h{};
+    SsRec r{};
     int32_t k = 0;
-    for (; rc == GI_OK && k < max_slots; ++k) {
-        if (hipMemcpyAsync(&h, w->st, offsetof(SsState, hubs), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
-            rc = fail(GI_ERR_HIP, "state read");
-            break;
-        }
-        if (h.done) break;
-        info[5 * k + 0] = h.mode;
-        info[5 * k + 1] = h.pull;
-        info[5 * k + 2] = h.front;
-        info[5 * k + 3] = h.hi;
-        info[5 * k + 4] = h.fdeg;
+    for (; rc == GI_OK && k <= max_slots; ++k) {
+        const int32_t ph = k % 12;
         (void)hipEventRecord(ev[0], s);
-        ss_relax_kernel<<<grid, kSsBlock, 0, s>>>(a);
+        ss_step_kernel<<<grid, kSsBlock, 0, s>>>(a, k, ph);
         (void)hipEventRecord(ev[1], s);
-        if (w->hubs) ss_hub_kernel<<<grid, kSsBlock, 0, s>>>(a);
+        if (w->hubs) ss_hub_kernel<<<grid, kSsBlock, 0, s>>>(a, k, ph);
         (void)hipEventRecord(ev[2], s);
-        ss_decide_kernel<<<1, kSsDecBlock, 0, s>>>(a, grid);
-        (void)hipEventRecord(ev[3], s);
-        ss_farmove_kernel<<<grid, kSsBlock, 0, s>>>(a);
-        (void)hipEventRecord(ev[4], s);
-        ss_decide2_kernel<<<1, kSsDecBlock, 0, s>>>(a, grid);
-        (void)hipEventRecord(ev[5], s);
-        if (hipGetLastError() != hipSuccess || hipEventSynchronize(ev[5]) != hipSuccess) {
-            rc = fail(GI_ERR_HIP, "slot launch");
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(ev[2]) != hipSuccess ||
+            hipMemcpy(&r, &w->st->rec[ph & 1], sizeof(SsRec), hipMemcpyDeviceToHost) != hipSuccess) {
+            rc = fail(GI_ERR_HIP, "step launch");
             break;
         }
-        float t[5];
-        for (int i = 0; i < 5; ++i) (void)hipEventElapsedTime(&t[i], ev[i], ev[i + 1]);
-        const bool pulled = h.mode == SS_RELAX && h.pull;
-        float *row = kernel_ms + 6 * k;
-        row[0] = pulled ? 0.f : t[0];
-        row[1] = pulled ? t[0] : 0.f;
-        for (int i = 1; i < 5; ++i) row[i + 1] = t[i];
+        if (r.mode == SS_DONE) break;
+        if (k == max_slots) break;
+        float t[2];
+        for (int i = 0; i < 2; ++i) (void)hipEventElapsedTime(&t[i], ev[i], ev[i + 1]);
+        info[5 * k + 0] = r.mode;
+        info[5 * k + 1] = r.pull;
+        info[5 * k + 2] = r.front;
+        info[5 * k + 3] = r.hi;
+        info[5 * k + 4] = r.fdeg;
+        float *row = kernel_ms + 6 * k;  // push, pull, hub, -, window scan, -
+        for (int i = 0; i < 6; ++i) row[i] = 0.f;
+        if (r.mode == SS_ADVANCE) row[4] = t[0];
+        else if (r.pull) row[1] = t[0];
+        else row[0] = t[0];
+        row[2] = t[1];
     }
     for (auto e : ev)
         if (e) (void)hipEventDestroy(e);
-    if (rc == GI_OK && k == max_slots && !h.done) rc = fail(GI_ERR_RANGE, "more than %d slots", max_slots);
+    if (rc == GI_OK && k >= max_slots && r.mode != SS_DONE) rc = fail(GI_ERR_RANGE, "more than %d steps", max_slots);
     *num_slots = k;
     return rc;
 }
@@ -1089,6 +1292,7 @@ extern "C" int gi_sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t
     if (source < 0 || source >= g->V) return gi::fail(GI_ERR_RANGE, "source %d out of range", source);
     if (delta < 1) return gi::fail(GI_ERR_RANGE, "delta must be positive");
     if (!g->d_w) return gi::fail(GI_ERR_INVALID, "SSSP needs edge weights");
+    if (max_slots < 1) return gi::fail(GI_ERR_RANGE, "max_slots must be positive");
     return gi::sssp_trace(g, source, delta, pull_min, d_dist, max_slots, kernel_ms, info, num_slots,
                           (hipStream_t)stream);
 }

@@ -127,6 +127,27 @@ def test_sssp_trace_matches_and_counts(gi):
     assert st["iterations"] == st["push_iterations"] + st["pull_iterations"]
 
 
+def test_sssp_heavy_step_pushed_and_pulled(gi):
+    # The heavy step relaxes W's heavy edges once per window: pushed, or, on a
+    # symmetric graph whose W holds an eighth of the edges or more, pulled
+    # over the split in-lists. Relax steps push here (pull_min huge), so wide
+    # frontiers defer their heavy edges.
+    seen = {}
+    for symmetric in (True, False):
+        g = gi.Graph.rmat(13, 16, SEED, symmetric).set_weights(SEED, 1, 255)
+        host = host_csr(g, True)
+        for src in O.pick_sources(host, 1, 3):
+            want = O.sssp(host, int(src))
+            d, _, info = gi.sssp_trace(g, int(src), 32, 1 << 30)
+            assert np.array_equal(d, want), (symmetric, int(src))
+            heavy = info[info[:, 0] == 2]
+            seen.setdefault(symmetric, set()).update(heavy[:, 1].tolist())
+            d2, _ = gi.sssp(g, int(src), 32, _fused_sssp(gi))
+            assert np.array_equal(d2, want), (symmetric, int(src))
+    assert 1 in seen[True]           # a symmetric graph's large W is pulled
+    assert seen[False] == {0}        # a directed graph's heavy steps push
+
+
 def test_set_weights_after_fused_runs_directed(gi):
     # new weights drop the transposed graph; the fused engines must not keep
     # pointers into it
