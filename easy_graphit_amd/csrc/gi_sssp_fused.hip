@@ -508,103 +508,121 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, const SsRec &cur, 
 }
 
 // Window scan-and-relax (mode ADVANCE), over the units whose pending bound
-// is below hi: each workgroup lists the vertices of its units with
-// lo <= dist < hi (4 words per wave, loads in flight together, their edge
-// ranges loaded with them) and relaxes their out-edges at once, as a push
-// step would in the next launch; improvements below hi join F_s+1. The
-// unit's base bound becomes its least distance >= hi and the bound lowered
-// since the previous scan is reset (those relaxations ran in earlier
-// launches, whose writes to dist this scan read); this step's own
-// relaxations lower the other bound, which the scan leaves alone, so no
-// bound is lost to a concurrent relaxation. A unit whose bound is at least hi holds
-// nothing below hi and is skipped. The min stripes get -1 when the
-// workgroup found vertices, else the least bound over its units (the least
-// pending distance, exact when no vertex was found anywhere). lds holds
-// kBsChunkVerts entries; pre_s, e0_s, du_s kSsBlock each.
+// is below hi (each workgroup loads the bounds of 64 of its units at once):
+// each workgroup lists the vertices of those units with lo <= dist < hi
+// (4 words per wave, loads in flight together, their edge ranges loaded
+// with them) and then relaxes all their out-edges, as a push step would in
+// the next launch; improvements below hi join F_s+1. The unit's base bound
+// becomes its least distance >= hi and the bound lowered since the previous
+// scan is reset (those relaxations ran in earlier launches, whose writes to
+// dist this scan read); this step's own relaxations lower the other bound,
+// which the scan leaves alone, so no bound is lost to a concurrent
+// relaxation. A unit whose bound is at least hi holds nothing below hi and
+// is skipped. The min stripes get -1 when the workgroup found vertices,
+// else the least bound over its units (the least pending distance, exact
+// when no vertex was found anywhere). lds holds kBsChunkVerts entries;
+// pre_s, e0_s, du_s kSsBlock each.
 __device__ __forceinline__ void ss_scan_body(const SsArgs &a, const SsRec &cur, int32_t ph, int32_t *lds,
                                              int32_t *pre_s, int32_t *e0_s, int32_t *du_s) {
     __shared__ int32_t wcnt[kSsBlock / 64], ured[kSsBlock / 64];
     const int32_t lo = cur.lo, hi = cur.hi;
     SsCtx c = ss_ctx(a, cur, ph);
     int32_t *uold = a.ucl + (cur.uq ^ 1) * a.ustride;  // bounds lowered since the previous scan
-    int32_t *lv = lds, *ldu = lds + 1024, *le0 = lds + 2048, *le1 = lds + 3072;  // a unit's 1024 vertices at most
-    static_assert(kSsPullUnit * 64 * 4 <= kBsChunkVerts, "a unit's list fits the LDS buffer");
+    constexpr int32_t kCap = kBsChunkVerts / 4;          // found vertices listed before they are relaxed
+    int32_t *lv = lds, *ldu = lds + kCap, *le0 = lds + 2 * kCap, *le1 = lds + 3 * kCap;
+    static_assert(kSsPullUnit * 64 <= kCap, "a unit's vertices fit the list");
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    int32_t beyond = INT_MAX;  // thread 0: least bound over this workgroup's units
+    auto get = [&](int32_t j, int32_t &v, int32_t &e0, int32_t &e1, int32_t &du) {
+        v = lv[j];
+        e0 = le0[j];
+        e1 = le1[j];
+        du = ldu[j];
+    };
+    int32_t beyond = INT_MAX;  // least bound over this workgroup's units
+    int32_t listed = 0;        // block-uniform: entries in the list
     bool found = false;        // block-uniform
     const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
-    for (int32_t un = blockIdx.x; un < units; un += gridDim.x) {
-        // whether the bound is below hi is the same for every wave: this
-        // step's relaxations lower c.ucl only to values >= hi
-        const int32_t b0 = a.ubase[un], b1 = uold[un], b2 = c.ucl[un];
-        int32_t um = b0 < b1 ? b0 : b1;
-        um = b2 < um ? b2 : um;
-        if (um >= hi) {
-            beyond = um < beyond ? um : beyond;
-            continue;
+    const int32_t G = (int32_t)gridDim.x;
+    for (int32_t g0 = 0; (int32_t)blockIdx.x + g0 * G < units; g0 += 64) {  // this workgroup's units, 64 at a time
+        // each lane loads the bounds of one unit; whether a bound is below hi
+        // is the same for every wave (this step's relaxations lower c.ucl only
+        // to values >= hi), so every wave gets the same mask
+        const int32_t ul = (int32_t)blockIdx.x + (g0 + lane) * G;
+        int32_t um = INT_MAX;
+        if (ul < units) {
+            const int32_t b0 = a.ubase[ul], b1 = uold[ul], b2 = c.ucl[ul];
+            um = b0 < b1 ? b0 : b1;
+            um = b2 < um ? b2 : um;
         }
-        const int32_t w0 = un * kSsPullUnit + wid * 4;
-        int32_t dv[4], d0[4], d1[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int32_t v = (w0 + i) * 64 + lane;
-            const bool ok = v < a.V;
-            dv[i] = ok ? a.dist[v] : INT_MAX;
-            d0[i] = ok ? a.off[v] : 0;
-            d1[i] = ok ? a.off[v + 1] : 0;
-        }
-        uint64_t m[4];
-        int32_t cnt = 0, ub = INT_MAX;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            m[i] = __ballot(dv[i] >= lo && dv[i] < hi);  // dv is INT_MAX past V
-            cnt += __popcll(m[i]);
-            if (dv[i] >= hi && dv[i] < ub) ub = dv[i];
-        }
+        const uint64_t todo = __ballot(um < hi);
+        int32_t skip = um >= hi ? um : INT_MAX;  // the bounds of the units not scanned
         for (int o = 32; o > 0; o >>= 1) {
-            const int32_t y = __shfl_xor(ub, o);
-            ub = y < ub ? y : ub;
+            const int32_t y = __shfl_xor(skip, o);
+            skip = y < skip ? y : skip;
         }
-        if (lane == 0) {
-            wcnt[wid] = cnt;
-            ured[wid] = ub;
-        }
-        __syncthreads();
-        int32_t pos = 0, n = 0, mb = INT_MAX;
-        for (int i = 0; i < kSsBlock / 64; ++i) {
-            pos += i < wid ? wcnt[i] : 0;
-            n += wcnt[i];
-            mb = ured[i] < mb ? ured[i] : mb;
-        }
+        beyond = skip < beyond ? skip : beyond;
+        for (uint64_t t = todo; t; t &= t - 1) {  // uniform
+            const int32_t un = (int32_t)blockIdx.x + (g0 + __ffsll((unsigned long long)t) - 1) * G;
+            const int32_t w0 = un * kSsPullUnit + wid * 4;
+            int32_t dv[4], d0[4], d1[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if ((m[i] >> lane) & 1ull) {
-                const int32_t q = pos + (int32_t)lanes_below(m[i]);
-                lv[q] = (w0 + i) * 64 + lane;
-                ldu[q] = dv[i];
-                le0[q] = d0[i];
-                le1[q] = d1[i];
+            for (int i = 0; i < 4; ++i) {
+                const int32_t v = (w0 + i) * 64 + lane;
+                const bool ok = v < a.V;
+                dv[i] = ok ? a.dist[v] : INT_MAX;
+                d0[i] = ok ? a.off[v] : 0;
+                d1[i] = ok ? a.off[v + 1] : 0;
             }
-            pos += __popcll(m[i]);
-        }
-        if (threadIdx.x == 0) {  // the unit's exact bound
-            a.ubase[un] = mb;
-            uold[un] = INT_MAX;
+            uint64_t m[4];
+            int32_t cnt = 0, ub = INT_MAX;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                m[i] = __ballot(dv[i] >= lo && dv[i] < hi);  // dv is INT_MAX past V
+                cnt += __popcll(m[i]);
+                if (dv[i] >= hi && dv[i] < ub) ub = dv[i];
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                const int32_t y = __shfl_xor(ub, o);
+                ub = y < ub ? y : ub;
+            }
+            if (lane == 0) {
+                wcnt[wid] = cnt;
+                ured[wid] = ub;
+            }
+            __syncthreads();
+            int32_t pos = 0, n = 0, mb = INT_MAX;
+            for (int i = 0; i < kSsBlock / 64; ++i) {
+                pos += i < wid ? wcnt[i] : 0;
+                n += wcnt[i];
+                mb = ured[i] < mb ? ured[i] : mb;
+            }
+            if (listed + n > kCap) {  // uniform: relax what is listed first
+                ss_rounds(a, c, listed, a.nbr, a.wt, ph % 3, pre_s, e0_s, du_s, get);
+                listed = 0;
+            }
+            pos += listed;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if ((m[i] >> lane) & 1ull) {
+                    const int32_t q = pos + (int32_t)lanes_below(m[i]);
+                    lv[q] = (w0 + i) * 64 + lane;
+                    ldu[q] = dv[i];
+                    le0[q] = d0[i];
+                    le1[q] = d1[i];
+                }
+                pos += __popcll(m[i]);
+            }
+            if (threadIdx.x == 0) {  // the unit's exact bound
+                a.ubase[un] = mb;
+                uold[un] = INT_MAX;
+            }
             beyond = mb < beyond ? mb : beyond;
-        }
-        __syncthreads();  // the list is complete; wcnt and ured are read
-        if (n > 0) {      // uniform
-            found = true;
-            ss_rounds(a, c, n, a.nbr, a.wt, ph % 3, pre_s, e0_s, du_s,
-                      [&](int32_t j, int32_t &v, int32_t &e0, int32_t &e1, int32_t &du) {
-                          v = lv[j];
-                          e0 = le0[j];
-                          e1 = le1[j];
-                          du = ldu[j];
-                      });  // ends with a barrier: the list may be rewritten
+            listed += n;
+            found = found || n > 0;
+            __syncthreads();  // the entries are listed; wcnt and ured are read
         }
     }
-    if (threadIdx.x != 0) beyond = INT_MAX;  // thread 0 holds the workgroup's least bound
+    if (listed > 0) ss_rounds(a, c, listed, a.nbr, a.wt, ph % 3, pre_s, e0_s, du_s, get);
     ss_flush(a, ph % kSsSets, c.nf, c.fdeg, found ? -1 : beyond);
 }
 
