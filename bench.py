@@ -121,6 +121,35 @@ def pmc_traffic(kernel: str, scale: int):
     return None, None
 
 
+def sssp_certificate_violations(g, source: int, dist) -> int:
+    """Vertices at which `dist` fails the shortest-path certificate for
+    positive integer weights: dist[source] == 0, dist[v] <= dist[u] + w for
+    every edge (u, v, w), and every other finite dist[v] is met with equality
+    by some in-edge (a tight edge leads back to the source, so dist is at
+    least the true distance; the edge inequalities make it at most that).
+    Checked in int64 over the CSR, in slices of vertices."""
+    off, nb, wt = g.read_csr(weights=True)
+    inf = np.iinfo(np.int32).max
+    d = dist.astype(np.int64)
+    tight = np.zeros(len(d), bool)
+    bad = int(d[source] != 0)
+    step = 1 << 20
+    for v0 in range(0, len(d), step):
+        v1 = min(v0 + step, len(d))
+        e0, e1 = int(off[v0]), int(off[v1])
+        src = np.repeat(np.arange(v0, v1, dtype=np.int64), np.diff(off[v0:v1 + 1]))
+        du = d[src]
+        reach = du < inf
+        cand = du[reach] + wt[e0:e1][reach]
+        tgt = nb[e0:e1][reach]
+        dt = d[tgt]
+        bad += int((cand < dt).sum())
+        tight[tgt[cand == dt]] = True
+    need = (d < inf) & ~tight
+    need[source] = False
+    return bad + int(need.sum())
+
+
 def alg_frac(nbytes: float, seconds: float) -> float:
     return round(nbytes / seconds / (HBM_PEAK_GBS * 1e9), 4)
 
@@ -184,12 +213,13 @@ def bench_apps(gi, g, sources, edges_of, log, only: str | None = None, cpu_on: b
                                      ("grid2048", gi.Graph.grid(2048, 2 * 2048 * 2047 // 100, SEED)
                                       .set_weights(SEED, 1, 1000), 0, 64)):
             dist, ms, info = gi.sssp_trace(gg, src, delta, max_slots=8192)
-            d2, _ = gi.sssp(gg, src, delta, fs)
-            if not np.array_equal(dist, d2):
-                raise SystemExit("traced SSSP distances differ from the recorded engine's")
-            cols = ["push", "pull", "hub", "decide", "window_scan", "decide2"]
-            res[name] = {"slots": len(ms), "delta": delta,
-                         "kernel_ms_summed": dict(zip(cols, np.round(ms.sum(0), 3).tolist())),
+            bad = sssp_certificate_violations(gg, src, dist)
+            if bad:
+                raise SystemExit(f"traced SSSP distances fail the shortest-path certificate ({bad} vertices)")
+            cols = {0: "push", 1: "pull", 2: "hub", 4: "scan_and_relax"}  # (columns 3 and 5 are unused)
+            tot = ms.sum(0)
+            res[name] = {"steps": len(ms), "delta": delta,
+                         "kernel_ms_summed": {c: round(float(tot[i]), 3) for i, c in cols.items()},
                          "heavy_steps": int((info[:, 0] == 2).sum())}
             log(f"sssp_trace {name}: {res[name]}")
         out["sssp_trace"] = res

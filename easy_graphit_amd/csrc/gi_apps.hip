@@ -494,7 +494,7 @@ int gi_sssp(gi_graph *g, int32_t source, int32_t delta, const gi_schedule *sched
         int32_t counts[4] = {0, 0, 0, 0};
         // native vertex order even when a degree-ordered copy exists: there the
         // hubs share a few cache lines and the relaxations' atomics serialize
-        GI_TRY(sssp_fused_run(g, source, delta, d_dist, counts, s));
+        GI_TRY(sssp_fused_run(g, source, delta, d_dist, counts, s, sched->max_cta));
         double ms = timer.stop();
         if (stats) {
             stats->iterations = counts[0];

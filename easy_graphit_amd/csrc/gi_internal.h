@@ -415,7 +415,7 @@ int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds, bool sampled, 
 void release_cc(const gi_graph *g);
 // counts (optional): relax steps, pull steps, windows entered, recorded slots.
 int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,
-                   hipStream_t s);
+                   hipStream_t s, int32_t max_cta = 0);
 void release_sssp(const gi_graph *g);
 
 // Degree-ordered layout (gi_layout.hip): the view the fused engines run on,

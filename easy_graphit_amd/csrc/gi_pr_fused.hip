@@ -26,6 +26,14 @@ namespace gi {
 constexpr int kPrBlock = 256;
 constexpr int32_t kPrNarrow = 64;      // in-degree below this: summed inside its group
 constexpr int32_t kPrChunk = 512;      // edges per chunk of a wide vertex
+constexpr int kPrPasses = 4;           // 64-edge passes of a group in flight together
+// The gather keeps the first kPrHubs contributions in LDS (128 KB, one
+// 1024-thread workgroup per CU): on a degree-ordered view these are the
+// hubs, the sources of about half the in-edges of an RMAT graph, whose
+// gathers then leave the L2 alone (random 4-byte gathers run at the L2's
+// request rate).
+constexpr int kPrGatherBlock = 1024;
+constexpr int32_t kPrHubs = 32768;
 
 struct PrArgs {
     const int32_t *poff;      // in-edge CSR
@@ -54,13 +62,29 @@ __device__ __forceinline__ void pr_store(const PrArgs &a, int32_t v, float sum) 
     a.contrib_next[v] = r / (float)(a.off[v + 1] - a.off[v]);  // apps/pr.cpp:28-30
 }
 
-__global__ void __launch_bounds__(kPrBlock) pr_gather_kernel(PrArgs a) {
-    __shared__ float acc_s[kPrBlock / 64][64];
+// contrib[u], from the workgroup's copy of the first kPrHubs entries when u
+// is one of them
+__device__ __forceinline__ float pr_contrib(const PrArgs &a, const float *hub, int32_t u) {
+    return u < kPrHubs ? hub[u] : a.contrib[u];
+}
+
+__global__ void __launch_bounds__(kPrGatherBlock) pr_gather_kernel(PrArgs a) {
+    __shared__ float acc_s[kPrGatherBlock / 64][64];
+    __shared__ float4 hub4[kPrHubs / 4];
+    float *hub = reinterpret_cast<float *>(hub4);
+    {  // the first kPrHubs contributions (the hubs, on a degree-ordered view)
+        const int32_t n4 = (a.V < kPrHubs ? a.V : kPrHubs) / 4;
+        const float4 *c4 = reinterpret_cast<const float4 *>(a.contrib);
+        for (int32_t i = threadIdx.x; i < n4; i += kPrGatherBlock) hub4[i] = c4[i];
+        for (int32_t i = 4 * n4 + (int32_t)threadIdx.x; i < a.V && i < kPrHubs; i += kPrGatherBlock)
+            hub[i] = a.contrib[i];
+        __syncthreads();
+    }
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
     float *acc = acc_s[wid];
-    const int32_t waves = gridDim.x * (kPrBlock / 64);
+    const int32_t waves = gridDim.x * (kPrGatherBlock / 64);
     const int32_t items = a.groups + a.chunks;
-    for (int32_t item = blockIdx.x * (kPrBlock / 64) + wid; item < items; item += waves) {
+    for (int32_t item = blockIdx.x * (kPrGatherBlock / 64) + wid; item < items; item += waves) {
         if (item < a.groups) {
             const int32_t v = item * 64 + lane;
             int32_t e0 = 0, len = 0;
@@ -80,28 +104,44 @@ __global__ void __launch_bounds__(kPrBlock) pr_gather_kernel(PrArgs a) {
             const int32_t total = __shfl(incl, 63);
             const int32_t pre = incl - len;
             __builtin_amdgcn_wave_barrier();
-            for (int32_t base = 0; base < total; base += 64) {
-                const int32_t g = base + lane;
-                int32_t j = 0;  // owner: last lane whose exclusive prefix is <= g
-                for (int32_t step = 32; step > 0; step >>= 1) {
-                    int32_t t = j + step;
-                    int32_t pt = __shfl(pre, t & 63);
-                    if (t < 64 && pt <= g) j = t;
+            // kPrPasses passes of 64 edges at a time: their index loads, then
+            // their gathers, are in flight together; the sums are added in
+            // pass order, as one pass at a time would
+            for (int32_t base = 0; base < total; base += 64 * kPrPasses) {
+                int32_t jj[kPrPasses], ee[kPrPasses], uu[kPrPasses];
+                float xx[kPrPasses];
+#pragma unroll
+                for (int p = 0; p < kPrPasses; ++p) {
+                    const int32_t g = base + 64 * p + lane;
+                    int32_t j = 0;  // owner: last lane whose exclusive prefix is <= g
+                    for (int32_t step = 32; step > 0; step >>= 1) {
+                        int32_t t = j + step;
+                        int32_t pt = __shfl(pre, t & 63);
+                        if (t < 64 && pt <= g) j = t;
+                    }
+                    const int32_t ej = __shfl(e0, j), pj = __shfl(pre, j);  // every lane takes part
+                    const bool valid = g < total;
+                    jj[p] = valid ? j : 64;  // past the last owner
+                    ee[p] = valid ? ej + (g - pj) : -1;
                 }
-                const bool valid = g < total;
-                const int32_t ej = __shfl(e0, j), pj = __shfl(pre, j);  // every lane takes part
-                float x = 0.f;
-                if (valid) x = a.contrib[a.pnbr[ej + (g - pj)]];
-                else j = 64;  // past the last owner
-                // segmented inclusive scan (segments = runs of equal owner)
-                for (int o = 1; o < 64; o <<= 1) {
-                    float y = __shfl_up(x, o);
-                    int32_t jo = __shfl_up(j, o);
-                    if (lane >= o && jo == j) x += y;
+#pragma unroll
+                for (int p = 0; p < kPrPasses; ++p) uu[p] = ee[p] >= 0 ? a.pnbr[ee[p]] : -1;
+#pragma unroll
+                for (int p = 0; p < kPrPasses; ++p) xx[p] = uu[p] >= 0 ? pr_contrib(a, hub, uu[p]) : 0.f;
+#pragma unroll
+                for (int p = 0; p < kPrPasses; ++p) {
+                    float x = xx[p];
+                    const int32_t j = jj[p];
+                    // segmented inclusive scan (segments = runs of equal owner)
+                    for (int o = 1; o < 64; o <<= 1) {
+                        float y = __shfl_up(x, o);
+                        int32_t jo = __shfl_up(j, o);
+                        if (lane >= o && jo == j) x += y;
+                    }
+                    const int32_t jn = __shfl_down(j, 1);
+                    if (j < 64 && (lane == 63 || jn != j)) acc[j] += x;  // segment end
+                    __builtin_amdgcn_wave_barrier();
                 }
-                const int32_t jn = __shfl_down(j, 1);
-                if (valid && (lane == 63 || jn != j)) acc[j] += x;  // segment end
-                __builtin_amdgcn_wave_barrier();
             }
             if (narrow) pr_store(a, v, acc[lane]);
             __builtin_amdgcn_wave_barrier();
@@ -112,8 +152,20 @@ __global__ void __launch_bounds__(kPrBlock) pr_gather_kernel(PrArgs a) {
             const int32_t e0 = a.chunk_e0[k];
             const int32_t end = a.poff[vw + 1];
             const int32_t e1 = e0 + kPrChunk < end ? e0 + kPrChunk : end;
+            // the chunk's kPrChunk / 64 edges per lane: index loads, then
+            // gathers, in flight together, added in edge order
+            int32_t uu[kPrChunk / 64];
+            float xx[kPrChunk / 64];
+#pragma unroll
+            for (int i = 0; i < kPrChunk / 64; ++i) {
+                const int32_t e = e0 + lane + 64 * i;
+                uu[i] = e < e1 ? a.pnbr[e] : -1;
+            }
+#pragma unroll
+            for (int i = 0; i < kPrChunk / 64; ++i) xx[i] = uu[i] >= 0 ? pr_contrib(a, hub, uu[i]) : 0.f;
             float x = 0.f;
-            for (int32_t e = e0 + lane; e < e1; e += 64) x += a.contrib[a.pnbr[e]];
+#pragma unroll
+            for (int i = 0; i < kPrChunk / 64; ++i) x += xx[i];
             for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
             if (lane == 0) a.partial[k] = x;
         }
@@ -215,7 +267,7 @@ static int ensure_plan(gi_graph *g, PrPlan **out) {
 }
 
 static int gather_grid() {
-    static const int per_cu = resident_per_cu((const void *)pr_gather_kernel, kPrBlock);
+    static const int per_cu = resident_per_cu((const void *)pr_gather_kernel, kPrGatherBlock);
     return device_info().num_cus * per_cu;
 }
 
@@ -245,7 +297,7 @@ static void launch_iters(gi_graph *g, PrPlan *p, int32_t iters, float damping, f
         a.contrib = p->contrib[it & 1];
         a.contrib_next = p->contrib[(it + 1) & 1];
         a.write_rank = it + 1 == iters;
-        L(pr_gather_kernel, grid, kPrBlock, a);
+        L(pr_gather_kernel, grid, kPrGatherBlock, a);
         if (p->nwide > 0) L(pr_finalize_kernel, fg < 4096 ? fg : 4096, 256, a);
     }
 }

@@ -73,8 +73,9 @@ constexpr long long kSsHeavyPullDiv = 8;  // W with E / kSsHeavyPullDiv out-edge
 
 // A step's counts (|F'| or the vertices a scan moved, their out-edges, the
 // least pending distance a scan saw) are added up in kSsStripes stripes,
-// each on a 128-byte line of its own, one atomic per workgroup and counter,
-// in three rotating sets: read by the next step, written, zeroed.
+// each on a 128-byte line of its own (holding all three), one atomic per
+// workgroup and counter, in three rotating sets: read by the next step,
+// written, zeroed.
 constexpr int kSsStripes = 64;
 constexpr int kSsLineI32 = 32;
 constexpr int kSsSets = 3;
@@ -98,7 +99,7 @@ struct SsRec {
     long long fdeg;      // out-edges of F
     long long ws_fdeg;   // out-edges of the F's recorded in W this window (W's size in edges, at most)
     int32_t uq;          // scans so far mod 2: the pending bounds this step lowers
-    int32_t pad_;
+    int32_t local;       // the step's workgroups relax their improvements below hi themselves
 };
 
 struct SsState {
@@ -166,6 +167,18 @@ __device__ __forceinline__ uint64_t ss_chunk_mask(const uint8_t *flags, int32_t 
     return __ballot(ch < chunks && flags[ch] != 0);
 }
 
+// Improvements below hi that the workgroup relaxes itself, in the same
+// launch (a light-edge chain inside a small window would otherwise cost a
+// launch per link): a queue in LDS of kSsQCap entries {vertex, distance,
+// first edge, end}, for vertices of at most kSsLocalDeg out-edges. A full
+// queue, or what is left after kSsDrainRounds rounds, joins F' instead.
+constexpr int32_t kSsQCap = 256;
+constexpr int32_t kSsLocalDeg = 32;
+constexpr int kSsDrainRounds = 8;
+constexpr int32_t kSsLocalMaxF = 16384;  // relax steps with |F| up to this ...
+constexpr long long kSsLocalMaxE = 262144;  // ... and this many out-edges drain their improvements
+constexpr int32_t kSsLocalScan = 256;    // a scan drains in the workgroups that found this many at most
+
 struct SsCtx {
     int32_t hi;
     uint64_t *bm_out;
@@ -173,6 +186,9 @@ struct SsCtx {
     int32_t *ucl;    // this step's pending bounds
     int32_t nf;      // per-lane: new members of F'
     int32_t fdeg;    // per-lane: their out-edges
+    bool local;      // workgroup-uniform: improvements below hi go to the queue q
+    int32_t *q;      // LDS: 4 * kSsQCap entries (vertex, distance, first edge, end)
+    int32_t *qn;     // LDS: entries claimed in q (may exceed kSsQCap: those joined F')
 };
 
 __device__ __forceinline__ SsCtx ss_ctx(const SsArgs &a, const SsRec &cur, int32_t ph) {
@@ -183,6 +199,9 @@ __device__ __forceinline__ SsCtx ss_ctx(const SsArgs &a, const SsRec &cur, int32
     c.ucl = a.ucl + cur.uq * a.ustride;
     c.nf = 0;
     c.fdeg = 0;
+    c.local = false;
+    c.q = nullptr;
+    c.qn = nullptr;
     return c;
 }
 
@@ -218,21 +237,8 @@ __device__ __forceinline__ void ss_flush(const SsArgs &a, int32_t set, int32_t n
     }
 }
 
-// writeMin (apps/sssp.cpp:24-27) of dist[v] to nd, known to be below the
-// value last read in this step. Below hi the vertex joins F' whether or not
-// this atomic is the one that lowers dist (a lower value written in this
-// step came from a relaxation that joins it too), so the atomic's result is
-// not waited for, and the bitmap atomic and the degree loads are in flight
-// with it. At or above hi the vertex is pending: its unit's bound for this
-// step's parity is kept at or below nd.
-__device__ __forceinline__ void ss_claim(const SsArgs &a, SsCtx &c, int32_t v, int32_t nd) {
-    atomicMin(&a.dist[v], nd);
-    if (nd >= c.hi) {
-        int32_t *m = &c.ucl[v / (64 * kSsPullUnit)];
-        if (nd < *m) atomicMin(m, nd);
-        return;
-    }
-    const int32_t deg = a.off[v + 1] - a.off[v];
+// v joins F' (the bitmap of the next step), counted once.
+__device__ __forceinline__ void ss_join(SsCtx &c, int32_t v, int32_t deg) {
     const uint64_t bit = 1ull << (v & 63);
     const unsigned long long o = atomicOr((unsigned long long *)&c.bm_out[v >> 6], bit);
     if (!(o & bit)) {
@@ -240,6 +246,40 @@ __device__ __forceinline__ void ss_claim(const SsArgs &a, SsCtx &c, int32_t v, i
         c.fdeg += deg;
         if (o == 0) bs_flag_chunk(c.cflag, v >> 6);
     }
+}
+
+// writeMin (apps/sssp.cpp:24-27) of dist[v] to nd, known to be below the
+// value last read in this step. Below hi the vertex is relaxed again, by
+// this workgroup from its queue or by the next step from F', whether or not
+// this atomic is the one that lowers dist (a lower value written meanwhile
+// came from a relaxation that schedules the vertex too), so the atomic's
+// result is not waited for, and the bitmap atomic and the degree loads are
+// in flight with it. At or above hi the vertex is pending: its unit's bound
+// for this step's parity is kept at or below nd.
+// With `known`, e0 and e1 are v's edge range, loaded with dist[v].
+__device__ __forceinline__ void ss_claim(const SsArgs &a, SsCtx &c, int32_t v, int32_t nd, bool known = false,
+                                         int32_t e0 = 0, int32_t e1 = 0) {
+    atomicMin(&a.dist[v], nd);
+    if (nd >= c.hi) {
+        int32_t *m = &c.ucl[v / (64 * kSsPullUnit)];
+        if (nd < *m) atomicMin(m, nd);
+        return;
+    }
+    if (!known) {
+        e0 = a.off[v];
+        e1 = a.off[v + 1];
+    }
+    if (c.local && e1 - e0 <= kSsLocalDeg) {
+        const int32_t p = atomicAdd(c.qn, 1);
+        if (p < kSsQCap) {
+            c.q[p] = v;
+            c.q[kSsQCap + p] = nd;
+            c.q[2 * kSsQCap + p] = e0;
+            c.q[3 * kSsQCap + p] = e1;
+            return;
+        }
+    }
+    ss_join(c, v, e1 - e0);
 }
 
 // Relaxation of edge e (of lists nbr, wt) from a vertex at distance du.
@@ -296,13 +336,67 @@ __device__ __forceinline__ void ss_rounds(const SsArgs &a, SsCtx &c, int32_t n, 
                     nd[p] = du_s[o] + wt[e];
                 }
             }
+            if (c.local) {  // uniform: few edges; the targets' edge ranges are loaded with their distances
+                int32_t o0[kSsBatch], o1[kSsBatch];
 #pragma unroll
-            for (int p = 0; p < kSsBatch; ++p) dv[p] = v[p] >= 0 ? a.dist[v[p]] : INT_MIN;
+                for (int p = 0; p < kSsBatch; ++p) {
+                    const bool ok = v[p] >= 0;
+                    dv[p] = ok ? a.dist[v[p]] : INT_MIN;
+                    o0[p] = ok ? a.off[v[p]] : 0;
+                    o1[p] = ok ? a.off[v[p] + 1] : 0;
+                }
 #pragma unroll
-            for (int p = 0; p < kSsBatch; ++p)
-                if (nd[p] < dv[p]) ss_claim(a, c, v[p], nd[p]);
+                for (int p = 0; p < kSsBatch; ++p)
+                    if (nd[p] < dv[p]) ss_claim(a, c, v[p], nd[p], true, o0[p], o1[p]);
+            } else {
+#pragma unroll
+                for (int p = 0; p < kSsBatch; ++p) dv[p] = v[p] >= 0 ? a.dist[v[p]] : INT_MIN;
+#pragma unroll
+                for (int p = 0; p < kSsBatch; ++p)
+                    if (nd[p] < dv[p]) ss_claim(a, c, v[p], nd[p]);
+            }
         }
         __syncthreads();
+    }
+}
+
+// The workgroup relaxes the entries of its queue (c.q with its count in
+// qn[0], filled by its claims in this step) in rounds, each round's improvements queued in the other
+// buffer (q1, kSsQCap * 4 entries of LDS not otherwise in use by now), all
+// edges of each vertex (a queued vertex has at most kSsLocalDeg). After
+// kSsDrainRounds rounds what is still queued joins F'. Every thread of the
+// workgroup calls it, with c.local set.
+__device__ __forceinline__ void ss_drain(const SsArgs &a, SsCtx &c, int32_t hl, int32_t *q1, int32_t *qn,
+                                         int32_t *pre_s, int32_t *e0_s, int32_t *du_s) {
+    int32_t *cur = c.q, *nxt = q1;
+    int32_t *ncur = qn, *nnxt = qn + 1;  // (c.qn is qn[0])
+    for (int r = 0;; ++r) {
+        __syncthreads();  // the claims that filled cur are done
+        int32_t n = *ncur;
+        n = n < kSsQCap ? n : kSsQCap;
+        if (n == 0) break;  // uniform
+        if (r == kSsDrainRounds) {  // what is left joins F'
+            for (int32_t j = threadIdx.x; j < n; j += kSsBlock)
+                ss_join(c, cur[j], cur[3 * kSsQCap + j] - cur[2 * kSsQCap + j]);
+            break;
+        }
+        if (threadIdx.x == 0) *nnxt = 0;  // (nxt was consumed by the previous round)
+        __syncthreads();
+        c.q = nxt;
+        c.qn = nnxt;
+        ss_rounds(a, c, n, a.nbr, a.wt, hl, pre_s, e0_s, du_s,
+                  [&](int32_t j, int32_t &v, int32_t &e0, int32_t &e1, int32_t &du) {
+                      v = cur[j];
+                      du = cur[kSsQCap + j];
+                      e0 = cur[2 * kSsQCap + j];
+                      e1 = cur[3 * kSsQCap + j];
+                  });
+        int32_t *t = cur;
+        cur = nxt;
+        nxt = t;
+        int32_t *tn = ncur;
+        ncur = nnxt;
+        nnxt = tn;
     }
 }
 
@@ -310,13 +404,17 @@ __device__ __forceinline__ void ss_rounds(const SsArgs &a, SsCtx &c, int32_t n, 
 // heavy ones), over this workgroup's chunks of F (`fl_in`: their mask).
 // `list` holds kBsChunkVerts entries (LDS).
 __device__ __forceinline__ void ss_push_body(const SsArgs &a, const SsRec &cur, int32_t ph, uint64_t fl_in,
-                                             int32_t *list, int32_t *pre_s, int32_t *e0_s, int32_t *du_s) {
+                                             int32_t *list, int32_t *pre_s, int32_t *e0_s, int32_t *du_s,
+                                             int32_t *q0, int32_t *qn) {
     __shared__ int32_t n_s;
     const bool heavy = cur.mode == SS_HEAVY;  // F is W, edges from lend
     const bool defer = !heavy && cur.defer;   // edges up to lend; F joins W
     const int32_t *nbr = heavy || defer ? a.snbr : a.nbr;  // split lists only where the split is used
     const int32_t *wt = heavy || defer ? a.swt : a.wt;
     SsCtx c = ss_ctx(a, cur, ph);
+    c.local = cur.local != 0;  // (a relax step of a small F: all edges, no hubs in the queue)
+    c.q = q0;
+    c.qn = qn;
     const uint64_t *bm_in = heavy ? a.ws : bs_pick3(a.bm[0], a.bm[1], a.bm[2], ph % 3);
     const int32_t lane = lane_id();
     for (uint64_t m = fl_in; m; m &= m - 1) {  // uniform
@@ -362,6 +460,7 @@ __device__ __forceinline__ void ss_push_body(const SsArgs &a, const SsRec &cur, 
                   });
         __syncthreads();
     }
+    if (c.local) ss_drain(a, c, ph % 3, list, qn, pre_s, e0_s, du_s);
     ss_flush(a, ph % kSsSets, c.nf, c.fdeg, INT_MAX);
 }
 
@@ -522,8 +621,11 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, const SsRec &cur, 
 // else the least bound over its units (the least pending distance, exact
 // when no vertex was found anywhere). lds holds kBsChunkVerts entries;
 // pre_s, e0_s, du_s kSsBlock each.
+// um0: the lane's bound for its first unit, loaded at the start of the
+// launch (the least of the unit's three bounds, whatever the parity).
 __device__ __forceinline__ void ss_scan_body(const SsArgs &a, const SsRec &cur, int32_t ph, int32_t *lds,
-                                             int32_t *pre_s, int32_t *e0_s, int32_t *du_s) {
+                                             int32_t *pre_s, int32_t *e0_s, int32_t *du_s, int32_t *q0,
+                                             int32_t *qn, int32_t um0) {
     __shared__ int32_t wcnt[kSsBlock / 64], ured[kSsBlock / 64];
     const int32_t lo = cur.lo, hi = cur.hi;
     SsCtx c = ss_ctx(a, cur, ph);
@@ -541,6 +643,7 @@ __device__ __forceinline__ void ss_scan_body(const SsArgs &a, const SsRec &cur, 
     int32_t beyond = INT_MAX;  // least bound over this workgroup's units
     int32_t listed = 0;        // block-uniform: entries in the list
     bool found = false;        // block-uniform
+    bool flushed = false;      // block-uniform: the list was relaxed before the end
     const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
     const int32_t G = (int32_t)gridDim.x;
     for (int32_t g0 = 0; (int32_t)blockIdx.x + g0 * G < units; g0 += 64) {  // this workgroup's units, 64 at a time
@@ -548,8 +651,8 @@ __device__ __forceinline__ void ss_scan_body(const SsArgs &a, const SsRec &cur, 
         // is the same for every wave (this step's relaxations lower c.ucl only
         // to values >= hi), so every wave gets the same mask
         const int32_t ul = (int32_t)blockIdx.x + (g0 + lane) * G;
-        int32_t um = INT_MAX;
-        if (ul < units) {
+        int32_t um = g0 == 0 ? um0 : INT_MAX;
+        if (g0 > 0 && ul < units) {
             const int32_t b0 = a.ubase[ul], b1 = uold[ul], b2 = c.ucl[ul];
             um = b0 < b1 ? b0 : b1;
             um = b2 < um ? b2 : um;
@@ -561,18 +664,43 @@ __device__ __forceinline__ void ss_scan_body(const SsArgs &a, const SsRec &cur, 
             skip = y < skip ? y : skip;
         }
         beyond = skip < beyond ? skip : beyond;
-        for (uint64_t t = todo; t; t &= t - 1) {  // uniform
-            const int32_t un = (int32_t)blockIdx.x + (g0 + __ffsll((unsigned long long)t) - 1) * G;
+        // a workgroup with many units to scan (a window whose pending
+        // vertices are everywhere) loads the edge ranges of the vertices it
+        // finds only, after their distances; one with few (a latency-bound
+        // window) loads them with the distances
+        const bool lazy = __popcll(todo) > 2;  // uniform
+        // the next unit's loads are issued before this unit is listed (a
+        // value read early is at worst higher than the latest: the launch's
+        // own relaxations schedule what they lower, as for any concurrent
+        // relaxation)
+        auto unit_of = [&](uint64_t t) {
+            return (int32_t)blockIdx.x + (g0 + __ffsll((unsigned long long)t) - 1) * G;
+        };
+        int32_t ndv[4], nd0[4], nd1[4];
+        auto load_unit = [&](int32_t un) {
             const int32_t w0 = un * kSsPullUnit + wid * 4;
-            int32_t dv[4], d0[4], d1[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int32_t v = (w0 + i) * 64 + lane;
                 const bool ok = v < a.V;
-                dv[i] = ok ? a.dist[v] : INT_MAX;
-                d0[i] = ok ? a.off[v] : 0;
-                d1[i] = ok ? a.off[v + 1] : 0;
+                ndv[i] = ok ? a.dist[v] : INT_MAX;
+                nd0[i] = ok && !lazy ? a.off[v] : 0;
+                nd1[i] = ok && !lazy ? a.off[v + 1] : 0;
             }
+        };
+        if (todo) load_unit(unit_of(todo));
+        for (uint64_t t = todo; t;) {  // uniform
+            const int32_t un = unit_of(t);
+            const int32_t w0 = un * kSsPullUnit + wid * 4;
+            int32_t dv[4], d0[4], d1[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                dv[i] = ndv[i];
+                d0[i] = nd0[i];
+                d1[i] = nd1[i];
+            }
+            t &= t - 1;
+            if (t) load_unit(unit_of(t));
             uint64_t m[4];
             int32_t cnt = 0, ub = INT_MAX;
 #pragma unroll
@@ -580,6 +708,15 @@ __device__ __forceinline__ void ss_scan_body(const SsArgs &a, const SsRec &cur, 
                 m[i] = __ballot(dv[i] >= lo && dv[i] < hi);  // dv is INT_MAX past V
                 cnt += __popcll(m[i]);
                 if (dv[i] >= hi && dv[i] < ub) ub = dv[i];
+            }
+            if (lazy) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    if ((m[i] >> lane) & 1ull) {
+                        const int32_t v = (w0 + i) * 64 + lane;
+                        d0[i] = a.off[v];
+                        d1[i] = a.off[v + 1];
+                    }
             }
             for (int o = 32; o > 0; o >>= 1) {
                 const int32_t y = __shfl_xor(ub, o);
@@ -599,6 +736,7 @@ __device__ __forceinline__ void ss_scan_body(const SsArgs &a, const SsRec &cur, 
             if (listed + n > kCap) {  // uniform: relax what is listed first
                 ss_rounds(a, c, listed, a.nbr, a.wt, ph % 3, pre_s, e0_s, du_s, get);
                 listed = 0;
+                flushed = true;
             }
             pos += listed;
 #pragma unroll
@@ -622,7 +760,15 @@ __device__ __forceinline__ void ss_scan_body(const SsArgs &a, const SsRec &cur, 
             __syncthreads();  // the entries are listed; wcnt and ured are read
         }
     }
-    if (listed > 0) ss_rounds(a, c, listed, a.nbr, a.wt, ph % 3, pre_s, e0_s, du_s, get);
+    if (listed > 0) {
+        // a workgroup that found few vertices relaxes what they improve
+        // into the window itself, in this launch
+        c.local = cur.local && !flushed && listed <= kSsLocalScan;
+        c.q = q0;
+        c.qn = qn;
+        ss_rounds(a, c, listed, a.nbr, a.wt, ph % 3, pre_s, e0_s, du_s, get);
+        if (c.local) ss_drain(a, c, ph % 3, lds, qn, pre_s, e0_s, du_s);
+    }
     ss_flush(a, ph % kSsSets, c.nf, c.fdeg, found ? -1 : beyond);
 }
 
@@ -669,6 +815,7 @@ __device__ __forceinline__ SsRec ss_decide(const SsArgs &a, const SsRec &prev, i
         r.fdeg = fdeg;
         r.pull = n > st->pull_min;
         r.defer = a.lend != nullptr && !r.pull && fdeg >= a.defer_min;
+        r.local = !r.pull && !r.defer && n <= kSsLocalMaxF && fdeg <= kSsLocalMaxE;
         r.steps = steps_before + 1;
         r.pulls = prev.pulls + r.pull;
     };
@@ -679,6 +826,7 @@ __device__ __forceinline__ SsRec ss_decide(const SsArgs &a, const SsRec &prev, i
         r.hi = ss_window_end(lo, st->delta);
         r.pull = 0;
         r.defer = 0;
+        r.local = 1;  // (in the workgroups that find few vertices)
     };
     if (prev.mode == SS_RELAX || prev.mode == SS_HEAVY) {
         const bool deferred = prev.mode == SS_RELAX && prev.defer;
@@ -696,6 +844,7 @@ __device__ __forceinline__ SsRec ss_decide(const SsArgs &a, const SsRec &prev, i
             r.ws_any = 0;
             r.ws_fdeg = 0;
             r.defer = 0;
+            r.local = 0;
             r.steps = prev.steps + 1;
         } else {  // the window is settled
             scan_at(prev.hi);
@@ -725,24 +874,60 @@ __device__ __forceinline__ SsRec ss_decide(const SsArgs &a, const SsRec &prev, i
 // push, pull, heavy or scan body runs. Graph replays run launch k of a block
 // as step base + k with phase (base + k) mod 12; eager launches pass their
 // step with base 0.
-__global__ void __launch_bounds__(kSsBlock) ss_step_kernel(SsArgs a, int32_t k, int32_t ph) {
+__global__ void __launch_bounds__(kSsBlock) __attribute__((amdgpu_waves_per_eu(5))) ss_step_kernel(SsArgs a, int32_t k,
+                                                                                           int32_t ph) {
     __shared__ int32_t lds[kBsChunkVerts];
     __shared__ int32_t pre_s[kSsBlock], e0_s[kSsBlock], du_s[kSsBlock];
+    __shared__ int32_t q0[4 * kSsQCap], qn[2];
+    static_assert(4 * kSsQCap <= kBsChunkVerts, "the second queue buffer fits the list space");
+    __shared__ SsRec cur_s;
+    __shared__ uint64_t mask_s[3];
+    __shared__ int32_t um_s[64];
+    __shared__ int32_t done_s, wsfree_s;
     SsState *st = a.st;
     const int32_t p2 = ph & 1, p3 = ph % 3, p4 = ph & 3;
-    const int32_t done = st->done;
-    const SsRec prev = st->rec[p2 ^ 1];
-    int32_t n, mn;
-    long long fdeg;
-    ss_counts(a, (p3 + 2) % kSsSets, &n, &fdeg, &mn);  // every wave: the same totals
-    // every chunk mask the step may use, before any wave of the workgroup
-    // writes a flag
-    const uint64_t fl_clear = ss_chunk_mask(ss_flag_set(a, (p4 + 2) & 3), a.chunks);
-    const uint64_t fl_f = ss_chunk_mask(ss_flag_set(a, (p4 + 3) & 3), a.chunks);
-    const uint64_t fl_ws = ss_chunk_mask(a.wsflag, a.chunks);
-    if (done) return;  // uniform
+    // Wave 0 loads what the decision and the bodies need and decides; the
+    // other waves take it from LDS (each of these loads touches 64 lines,
+    // so one wave per workgroup loading them keeps the launch's requests to
+    // the L2 down).
+    if (threadIdx.x < 64) {
+        const int32_t done = st->done;
+        const SsRec prev = st->rec[p2 ^ 1];
+        int32_t n, mn;
+        long long fdeg;
+        ss_counts(a, (p3 + 2) % kSsSets, &n, &fdeg, &mn);
+        // every chunk mask the step may use, before any wave of the
+        // workgroup writes a flag
+        const uint64_t fl_clear = ss_chunk_mask(ss_flag_set(a, (p4 + 2) & 3), a.chunks);
+        const uint64_t fl_f = ss_chunk_mask(ss_flag_set(a, (p4 + 3) & 3), a.chunks);
+        const uint64_t fl_ws = ss_chunk_mask(a.wsflag, a.chunks);
+        // the bounds of this lane's first unit for a scan (a relaxation of
+        // this launch may lower one, but only to a value at or above the
+        // window's end, which leaves the scan's choice alone)
+        int32_t um0 = INT_MAX;
+        const int32_t ul = (int32_t)blockIdx.x + lane_id() * (int32_t)gridDim.x;
+        if (ul < (a.words + kSsPullUnit - 1) / kSsPullUnit) {
+            const int32_t b0 = a.ubase[ul], b1 = a.ucl[ul], b2 = a.ucl[a.ustride + ul];
+            um0 = b0 < b1 ? b0 : b1;
+            um0 = b2 < um0 ? b2 : um0;
+        }
+        um_s[lane_id()] = um0;
+        if (threadIdx.x == 0) {
+            cur_s = done ? prev : ss_decide(a, prev, n, fdeg, mn);
+            done_s = done;
+            wsfree_s = prev.mode == SS_HEAVY && prev.pull;
+            mask_s[0] = fl_clear;
+            mask_s[1] = fl_f;
+            mask_s[2] = fl_ws;
+            qn[0] = 0;
+        }
+    }
+    __syncthreads();
+    if (done_s) return;  // uniform: the run has ended
+    const SsRec cur = cur_s;
+    const uint64_t fl_clear = mask_s[0], fl_f = mask_s[1], fl_ws = mask_s[2];
+    const int32_t um0 = um_s[lane_id()];
     a.dist = st->dist;
-    const SsRec cur = ss_decide(a, prev, n, fdeg, mn);
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) {
             st->rec[p2] = cur;
@@ -786,7 +971,7 @@ __global__ void __launch_bounds__(kSsBlock) ss_step_kernel(SsArgs a, int32_t k, 
             }
         }
     }
-    if (prev.mode == SS_HEAVY && prev.pull) {  // uniform: a pulled W is done with
+    if (wsfree_s) {  // uniform: the step before pulled W in a heavy step, which is done with
         // (its bits were read by any workgroup, so the owners free it a step later)
         for (uint64_t m = fl_ws; m; m &= m - 1) {
             const int32_t ch = (int32_t)blockIdx.x + (__ffsll((unsigned long long)m) - 1) * (int32_t)gridDim.x;
@@ -800,11 +985,11 @@ __global__ void __launch_bounds__(kSsBlock) ss_step_kernel(SsArgs a, int32_t k, 
             a.wsflag[blockIdx.x + threadIdx.x * gridDim.x] = 0;
     }
     if (cur.mode == SS_ADVANCE) {
-        ss_scan_body(a, cur, ph, lds, pre_s, e0_s, du_s);
+        ss_scan_body(a, cur, ph, lds, pre_s, e0_s, du_s, q0, qn, um0);
     } else if (cur.pull) {  // a pull relax step, or the heavy step pulled
         ss_pull_body(a, cur, ph, lds);
     } else {
-        ss_push_body(a, cur, ph, cur.mode == SS_HEAVY ? fl_ws : fl_f, lds, pre_s, e0_s, du_s);
+        ss_push_body(a, cur, ph, cur.mode == SS_HEAVY ? fl_ws : fl_f, lds, pre_s, e0_s, du_s, q0, qn);
     }
 }
 
@@ -971,6 +1156,7 @@ struct SsWork {
     int32_t *chunk_v = nullptr, *chunk_e0 = nullptr;
     std::map<int32_t, hipGraphExec_t> execs;  // by steps * 12 + first phase (the run's dist is read from the state)
     int32_t recent_steps[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // step launches of the last runs (ring)
+    int32_t grid = 0;  // workgroups of the recorded step launches
     int32_t recent_n = 0;
 };
 
@@ -990,9 +1176,7 @@ static void free_ss(SsWork *w) {
     dfree(w->ucl);
     dfree(w->ws);
     dfree(w->wsflag);
-    dfree(w->cnt);
-    dfree(w->mn);
-    dfree(w->fdg);
+    dfree(w->cnt);  // (mn and fdg point into it)
     dfree(w->snbr);
     dfree(w->swt);
     dfree(w->lend);
@@ -1045,9 +1229,13 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
     if (rc == GI_OK) rc = dalloc(&w->ucl, 2 * (size_t)w->ustride);
     if (rc == GI_OK) rc = dalloc(&w->ws, (size_t)words + 1);
     if (rc == GI_OK) rc = dalloc(&w->wsflag, (size_t)w->flag_bytes);
+    // a stripe's three counters share its line: |F'| at int 0, the least
+    // pending distance at int 1, the out-edges at ints 2-3
     if (rc == GI_OK) rc = dalloc(&w->cnt, lines * kSsLineI32);
-    if (rc == GI_OK) rc = dalloc(&w->mn, lines * kSsLineI32);
-    if (rc == GI_OK) rc = dalloc(&w->fdg, lines * (kSsLineI32 / 2));
+    if (rc == GI_OK) {
+        w->mn = w->cnt + 1;
+        w->fdg = reinterpret_cast<long long *>(w->cnt + 2);
+    }
     if (rc == GI_OK &&
         hipHostMalloc((void **)&w->host_flag, 8 * sizeof(int32_t), hipHostMallocCoherent) != hipSuccess)
         rc = fail(GI_ERR_OOM, "pinned allocation failed");
@@ -1179,7 +1367,7 @@ static int ss_step_graph(SsWork *w, const SsArgs &ca, int32_t steps, int32_t ph0
         *out = it->second;
         return GI_OK;
     }
-    const int grid = ss_grid();
+    const int grid = w->grid;
     hipGraphExec_t exec = nullptr;
     GI_TRY(build_graph(
         [&](Launcher &L) {
@@ -1203,11 +1391,25 @@ static int32_t ss_first_block(const SsWork *w) {
     return b < kSsMaxBlock ? b : kSsMaxBlock;
 }
 
+// Workgroups of the step launches: the resident grid, or max_cta (the
+// schedule's) when positive (at most kSsMaxGrid), but enough that every
+// workgroup owns at most 64 chunks. Recorded steps are dropped when it
+// changes.
+static void ss_set_grid(SsWork *w, int32_t max_cta) {
+    int32_t grid = ss_grid();
+    if (max_cta > 0) grid = max_cta < kSsMaxGrid ? max_cta : kSsMaxGrid;
+    const int32_t least = (int32_t)ceil_div(w->chunks, 64);
+    if (grid < least) grid = least;
+    if (grid != w->grid) drop_ss_execs(w);
+    w->grid = grid;
+}
+
 int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,
-                   hipStream_t s) {
+                   hipStream_t s, int32_t max_cta) {
     SsWork *w = nullptr;
     GI_TRY(ensure_ss(g, &w));
     GI_TRY(ensure_split(g, w, delta, s));
+    ss_set_grid(w, max_cta);
     SsArgs a = ss_args(g, w, d_dist);
     GI_TRY(ss_reset(w, a, source, delta, w->pull_min, s));
     SsArgs ca = a;

@@ -476,20 +476,22 @@ int gi_bfs_sweep(gi_graph *g, int32_t source, int32_t mode_mask, int32_t follow,
 /* Delta-stepping SSSP (apps/sssp.cpp:34-57); d_dist int32, INT_MAX unreached.
  * Default schedule (s == NULL): the app's PUSH TWCE with the bucketed
  * priority queue (apps/sssp.cpp:77-79). With kernel_fusion ENABLED the
- * windows and relaxation phases run on the device (near frontier and far
- * set as bitmaps, recorded phases, no host synchronisation per phase). */
+ * windows and relaxation phases run on the device (near frontier as a
+ * bitmap, pending vertices as per-unit bounds, one recorded launch per step,
+ * no host synchronisation per step; max_cta > 0 sets the workgroups of the
+ * step launches, 0 the resident grid). */
 int gi_sssp(gi_graph *g, int32_t source, int32_t delta, const gi_schedule *s,
             int32_t *d_dist, gi_run_stats *stats, void *stream);
 /* Measurement hook of the fused SSSP engine (no reference counterpart):
- * runs the slots of the device-side window loop one at a time, outside a graph,
- * and times each launch of a slot with HIP events. pull_min: |F| above
- * which a relax step pulls (< 0: the engine's default).
- * kernel_ms[max_slots][6]: push step, pull step (one relax launch, counted
- * in the column of its direction, 0 in the other), hub bin (0 for a graph
- * whose out-degrees stay below the hub width: not launched), decide, window
- * scan, decide2. info[max_slots][5]: the state before the slot (mode: 0
- * relax / 1 advance / 2 heavy-edge step of a settled window, pull flag, |F|, window end hi, out-edges of F).
- * Synchronises. */
+ * runs the engine's step launches one at a time, outside a graph, and times
+ * each with HIP events. pull_min: |F| above which a relax step pulls (< 0:
+ * the engine's default). kernel_ms[max_slots][6]: push relax step, pull
+ * relax step (one launch, in the column of its direction, 0 in the other),
+ * hub bin (0 for a graph whose out-degrees stay below the hub width: not
+ * launched), 0, scan-and-relax step of a new window, 0. info[max_slots][5]:
+ * the step's record: mode (0 relax / 1 scan-and-relax / 2 heavy-edge step
+ * of a settled window), pull flag, |F| of a relax step, window end hi,
+ * out-edges of F. Synchronises. */
 int gi_sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_min, int32_t *d_dist,
                   int32_t max_slots, float *kernel_ms, int64_t *info, int32_t *num_slots, void *stream);
 /* Connected components, hook + pointer jumping (apps/cc.cpp:53-78); labels
