@@ -160,11 +160,28 @@ __device__ __forceinline__ uint8_t *ss_flag_set(const SsArgs &a, int32_t i) {
     return a.flags + (size_t)i * a.flag_bytes;
 }
 
+// Workgroup b of a launch of G workgroups owns the chunks b + j G (j < 64)
+// and the units b + j G: their flags and pending bounds are stored at b * 64
+// + j (units in groups of 64 G), so that a workgroup reads its 64 with one
+// or two cache lines.
+__device__ __forceinline__ int32_t ss_fpos(int32_t chunk, int32_t G) { return (chunk % G) * 64 + chunk / G; }
+__device__ __forceinline__ int32_t ss_upos(int32_t unit, int32_t G) {
+    const int32_t j = unit / G;
+    return (j >> 6) * 64 * G + (unit % G) * 64 + (j & 63);
+}
+__device__ __forceinline__ int32_t ss_unit_of(int32_t v) { return v / (64 * kSsPullUnit); }
+
+// The flag of the chunk holding bitmap word `word`, in a step launch.
+__device__ __forceinline__ void ss_flag_chunk(uint8_t *flags, int32_t word) {
+    flags[ss_fpos(word / kBsChunkWords, (int32_t)gridDim.x)] = 1;
+}
+
 // The non-empty chunks among this workgroup's (chunk blockIdx + j G at bit
-// j) in one set of chunk flags; every wave computes the same mask.
+// j) in one set of chunk flags.
 __device__ __forceinline__ uint64_t ss_chunk_mask(const uint8_t *flags, int32_t chunks) {
-    const int32_t ch = (int32_t)blockIdx.x + lane_id() * (int32_t)gridDim.x;
-    return __ballot(ch < chunks && flags[ch] != 0);
+    const int32_t j = lane_id();
+    const int32_t ch = (int32_t)blockIdx.x + j * (int32_t)gridDim.x;
+    return __ballot(ch < chunks && flags[(int32_t)blockIdx.x * 64 + j] != 0);
 }
 
 // Improvements below hi that the workgroup relaxes itself, in the same
@@ -244,7 +261,7 @@ __device__ __forceinline__ void ss_join(SsCtx &c, int32_t v, int32_t deg) {
     if (!(o & bit)) {
         c.nf += 1;
         c.fdeg += deg;
-        if (o == 0) bs_flag_chunk(c.cflag, v >> 6);
+        if (o == 0) ss_flag_chunk(c.cflag, v >> 6);
     }
 }
 
@@ -261,7 +278,7 @@ __device__ __forceinline__ void ss_claim(const SsArgs &a, SsCtx &c, int32_t v, i
                                          int32_t e0 = 0, int32_t e1 = 0) {
     atomicMin(&a.dist[v], nd);
     if (nd >= c.hi) {
-        int32_t *m = &c.ucl[v / (64 * kSsPullUnit)];
+        int32_t *m = &c.ucl[ss_upos(ss_unit_of(v), (int32_t)gridDim.x)];
         if (nd < *m) atomicMin(m, nd);
         return;
     }
@@ -428,7 +445,7 @@ __device__ __forceinline__ void ss_push_body(const SsArgs &a, const SsRec &cur, 
                 } else if (defer) {
                     const uint64_t o = a.ws[w];
                     a.ws[w] = o | word;
-                    if (!o) bs_flag_chunk(a.wsflag, w);
+                    if (!o) ss_flag_chunk(a.wsflag, w);
                 }
             }
             const int32_t pc = __popcll(word);
@@ -450,7 +467,7 @@ __device__ __forceinline__ void ss_push_body(const SsArgs &a, const SsRec &cur, 
         // W's chunk flag is consumed only now: every wave of the workgroup
         // took its ballot of the flags (at the start of the launch) before
         // this barrier
-        if (heavy && threadIdx.x == 0) a.wsflag[chunk] = 0;
+        if (heavy && threadIdx.x == 0) a.wsflag[ss_fpos(chunk, (int32_t)gridDim.x)] = 0;
         ss_rounds(a, c, n, nbr, wt, ph % 3, pre_s, e0_s, du_s,
                   [&](int32_t j, int32_t &s, int32_t &e0, int32_t &e1, int32_t &du) {
                       s = list[j];
@@ -548,7 +565,8 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, const SsRec &cur, 
                 const int32_t y = __shfl_xor(pend, o);
                 pend = y < pend ? y : pend;
             }
-            if (lane == 0 && pend < ucl[un]) atomicMin(&ucl[un], pend);
+            int32_t *um = &ucl[ss_upos(un, (int32_t)gridDim.x)];
+            if (lane == 0 && pend < *um) atomicMin(um, pend);
             const uint64_t near = __ballot(join);
             uint64_t old = 0;
             if (lane == 0 && near) {  // atomic: a wide vertex of the same word may join too
@@ -560,7 +578,7 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, const SsRec &cur, 
             if (join && !((old >> lane) & 1ull)) fdeg += a.off[v + 1] - a.off[v];
             __builtin_amdgcn_wave_barrier();
         }
-        if (__syncthreads_or(any) && threadIdx.x == 0) bs_flag_chunk(cflag, un * kSsPullUnit);
+        if (__syncthreads_or(any) && threadIdx.x == 0) ss_flag_chunk(cflag, un * kSsPullUnit);
     }
     // wide vertices: each chunk of an in-list is a workgroup item; its minimum
     // joins dist[v] by atomicMin, and an improvement below hi joins F'
@@ -589,7 +607,7 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, const SsRec &cur, 
         if (lane == 0 && m < dv) {
             const int32_t old = atomicMin(&a.dist[v], m);
             if (m < old && m >= hi) {
-                int32_t *um = &ucl[v / (64 * kSsPullUnit)];
+                int32_t *um = &ucl[ss_upos(ss_unit_of(v), (int32_t)gridDim.x)];
                 if (m < *um) atomicMin(um, m);
             }
             if (m < old && m < hi) {
@@ -598,7 +616,7 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, const SsRec &cur, 
                 if (!(o & bit)) {
                     nf += 1;
                     fdeg += a.off[v + 1] - a.off[v];
-                    if (o == 0) bs_flag_chunk(cflag, v >> 6);
+                    if (o == 0) ss_flag_chunk(cflag, v >> 6);
                 }
             }
         }
@@ -607,166 +625,133 @@ __device__ __forceinline__ void ss_pull_body(const SsArgs &a, const SsRec &cur, 
 }
 
 // Window scan-and-relax (mode ADVANCE), over the units whose pending bound
-// is below hi (each workgroup loads the bounds of 64 of its units at once):
-// each workgroup lists the vertices of those units with lo <= dist < hi
-// (4 words per wave, loads in flight together, their edge ranges loaded
-// with them) and then relaxes all their out-edges, as a push step would in
-// the next launch; improvements below hi join F_s+1. The unit's base bound
-// becomes its least distance >= hi and the bound lowered since the previous
-// scan is reset (those relaxations ran in earlier launches, whose writes to
-// dist this scan read); this step's own relaxations lower the other bound,
-// which the scan leaves alone, so no bound is lost to a concurrent
-// relaxation. A unit whose bound is at least hi holds nothing below hi and
-// is skipped. The min stripes get -1 when the workgroup found vertices,
-// else the least bound over its units (the least pending distance, exact
-// when no vertex was found anywhere). lds holds kBsChunkVerts entries;
-// pre_s, e0_s, du_s kSsBlock each.
-// um0: the lane's bound for its first unit, loaded at the start of the
-// launch (the least of the unit's three bounds, whatever the parity).
+// is below hi (a unit whose bound is at least hi holds nothing below hi and
+// is skipped): each workgroup lists the vertices of those units with lo <=
+// dist < hi and then relaxes all their out-edges, as a push step would in
+// the next launch; improvements below hi join F_s+1 (or the workgroup's
+// queue). Each wave scans whole units of its own (the r-th unit to scan
+// goes to wave r mod 4), the 16 words of a unit with their loads in flight
+// together, and lists what it finds through an LDS counter, with no barrier
+// per unit; the edge ranges of the list are loaded afterwards, in one pass.
+// The unit's base bound becomes its least distance >= hi and the bound
+// lowered since the previous scan is reset (those relaxations ran in earlier
+// launches, whose writes to dist this scan read); this step's own
+// relaxations lower the other bound, which the scan leaves alone, so no
+// bound is lost to a concurrent relaxation. The min stripes get -1 when the
+// workgroup found vertices, else the least bound over its units (the least
+// pending distance, exact when no vertex was found anywhere). lds holds
+// kBsChunkVerts entries; pre_s, e0_s, du_s kSsBlock each; um_s the bounds
+// of the workgroup's first 64 units, loaded at the start of the launch (the
+// least of each unit's three bounds, whatever the parity).
 __device__ __forceinline__ void ss_scan_body(const SsArgs &a, const SsRec &cur, int32_t ph, int32_t *lds,
                                              int32_t *pre_s, int32_t *e0_s, int32_t *du_s, int32_t *q0,
-                                             int32_t *qn, int32_t um0) {
-    __shared__ int32_t wcnt[kSsBlock / 64], ured[kSsBlock / 64];
+                                             int32_t *qn, int32_t *um_s) {
+    __shared__ int32_t lcount;
     const int32_t lo = cur.lo, hi = cur.hi;
     SsCtx c = ss_ctx(a, cur, ph);
     int32_t *uold = a.ucl + (cur.uq ^ 1) * a.ustride;  // bounds lowered since the previous scan
     constexpr int32_t kCap = kBsChunkVerts / 4;          // found vertices listed before they are relaxed
     int32_t *lv = lds, *ldu = lds + kCap, *le0 = lds + 2 * kCap, *le1 = lds + 3 * kCap;
-    static_assert(kSsPullUnit * 64 <= kCap, "a unit's vertices fit the list");
+    static_assert(kSsPullUnit == 16, "a unit is 16 words: one wave loads it at once");
     const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    auto get = [&](int32_t j, int32_t &v, int32_t &e0, int32_t &e1, int32_t &du) {
-        v = lv[j];
-        e0 = le0[j];
-        e1 = le1[j];
-        du = ldu[j];
-    };
-    int32_t beyond = INT_MAX;  // least bound over this workgroup's units
-    int32_t listed = 0;        // block-uniform: entries in the list
-    bool found = false;        // block-uniform
-    bool flushed = false;      // block-uniform: the list was relaxed before the end
     const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
     const int32_t G = (int32_t)gridDim.x;
+    if (threadIdx.x == 0) lcount = 0;
+    int32_t beyond = INT_MAX;  // per wave: least bound over its units and the units not scanned
+    bool found = false;        // per wave
+    bool spilled = false;      // per wave: a found vertex did not fit the list and joined F'
     for (int32_t g0 = 0; (int32_t)blockIdx.x + g0 * G < units; g0 += 64) {  // this workgroup's units, 64 at a time
-        // each lane loads the bounds of one unit; whether a bound is below hi
-        // is the same for every wave (this step's relaxations lower c.ucl only
-        // to values >= hi), so every wave gets the same mask
-        const int32_t ul = (int32_t)blockIdx.x + (g0 + lane) * G;
-        int32_t um = g0 == 0 ? um0 : INT_MAX;
-        if (g0 > 0 && ul < units) {
-            const int32_t b0 = a.ubase[ul], b1 = uold[ul], b2 = c.ucl[ul];
-            um = b0 < b1 ? b0 : b1;
-            um = b2 < um ? b2 : um;
+        if (g0 > 0) {  // the bounds of the next 64 (the first 64 came with the launch)
+            __syncthreads();  // every wave has read um_s
+            if (threadIdx.x < 64) {
+                const int32_t ul = (int32_t)blockIdx.x + (g0 + lane) * G;
+                int32_t um = INT_MAX;
+                if (ul < units) {
+                    const int32_t up = ss_upos(ul, G);
+                    const int32_t b0 = a.ubase[up], b1 = uold[up], b2 = c.ucl[up];
+                    um = b0 < b1 ? b0 : b1;
+                    um = b2 < um ? b2 : um;
+                }
+                um_s[lane] = um;
+            }
         }
-        const uint64_t todo = __ballot(um < hi);
-        int32_t skip = um >= hi ? um : INT_MAX;  // the bounds of the units not scanned
+        __syncthreads();  // um_s is written (and lcount, the first time)
+        const int32_t um = um_s[lane];
+        const uint64_t todo = __ballot(um < hi);  // the same in every wave
+        int32_t skip = um >= hi ? um : INT_MAX;   // the bounds of the units not scanned
         for (int o = 32; o > 0; o >>= 1) {
             const int32_t y = __shfl_xor(skip, o);
             skip = y < skip ? y : skip;
         }
         beyond = skip < beyond ? skip : beyond;
-        // a workgroup with many units to scan (a window whose pending
-        // vertices are everywhere) loads the edge ranges of the vertices it
-        // finds only, after their distances; one with few (a latency-bound
-        // window) loads them with the distances
-        const bool lazy = __popcll(todo) > 2;  // uniform
-        // the next unit's loads are issued before this unit is listed (a
-        // value read early is at worst higher than the latest: the launch's
-        // own relaxations schedule what they lower, as for any concurrent
-        // relaxation)
-        auto unit_of = [&](uint64_t t) {
-            return (int32_t)blockIdx.x + (g0 + __ffsll((unsigned long long)t) - 1) * G;
-        };
-        int32_t ndv[4], nd0[4], nd1[4];
-        auto load_unit = [&](int32_t un) {
-            const int32_t w0 = un * kSsPullUnit + wid * 4;
+        int32_t r = 0;
+        for (uint64_t t = todo; t; t &= t - 1, ++r) {  // wave-uniform
+            if ((r & 3) != wid) continue;
+            const int32_t un = (int32_t)blockIdx.x + (g0 + __ffsll((unsigned long long)t) - 1) * G;
+            const int32_t v0 = un * kSsPullUnit * 64 + lane;
+            int32_t dv[kSsPullUnit];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int32_t v = (w0 + i) * 64 + lane;
-                const bool ok = v < a.V;
-                ndv[i] = ok ? a.dist[v] : INT_MAX;
-                nd0[i] = ok && !lazy ? a.off[v] : 0;
-                nd1[i] = ok && !lazy ? a.off[v + 1] : 0;
+            for (int i = 0; i < kSsPullUnit; ++i) {
+                const int32_t v = v0 + 64 * i;
+                dv[i] = v < a.V ? a.dist[v] : INT_MAX;
             }
-        };
-        if (todo) load_unit(unit_of(todo));
-        for (uint64_t t = todo; t;) {  // uniform
-            const int32_t un = unit_of(t);
-            const int32_t w0 = un * kSsPullUnit + wid * 4;
-            int32_t dv[4], d0[4], d1[4];
+            int32_t ub = INT_MAX;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                dv[i] = ndv[i];
-                d0[i] = nd0[i];
-                d1[i] = nd1[i];
-            }
-            t &= t - 1;
-            if (t) load_unit(unit_of(t));
-            uint64_t m[4];
-            int32_t cnt = 0, ub = INT_MAX;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                m[i] = __ballot(dv[i] >= lo && dv[i] < hi);  // dv is INT_MAX past V
-                cnt += __popcll(m[i]);
+            for (int i = 0; i < kSsPullUnit; ++i) {
+                const bool in = dv[i] >= lo && dv[i] < hi;  // dv is INT_MAX past V
                 if (dv[i] >= hi && dv[i] < ub) ub = dv[i];
-            }
-            if (lazy) {
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    if ((m[i] >> lane) & 1ull) {
-                        const int32_t v = (w0 + i) * 64 + lane;
-                        d0[i] = a.off[v];
-                        d1[i] = a.off[v + 1];
+                const uint64_t m = __ballot(in);
+                if (m) {  // wave-uniform
+                    found = true;
+                    int32_t base = 0;
+                    if (lane == 0) base = atomicAdd(&lcount, (int32_t)__popcll(m));
+                    base = __shfl(base, 0);
+                    if (in) {
+                        const int32_t q = base + (int32_t)lanes_below(m);
+                        const int32_t v = v0 + 64 * i;
+                        if (q < kCap) {
+                            lv[q] = v;
+                            ldu[q] = dv[i];
+                        } else {  // the list is full: the next step relaxes it from F'
+                            ss_join(c, v, a.off[v + 1] - a.off[v]);
+                            spilled = true;
+                        }
                     }
+                }
             }
             for (int o = 32; o > 0; o >>= 1) {
                 const int32_t y = __shfl_xor(ub, o);
                 ub = y < ub ? y : ub;
             }
-            if (lane == 0) {
-                wcnt[wid] = cnt;
-                ured[wid] = ub;
+            if (lane == 0) {  // the unit's exact bound
+                a.ubase[ss_upos(un, G)] = ub;
+                uold[ss_upos(un, G)] = INT_MAX;
             }
-            __syncthreads();
-            int32_t pos = 0, n = 0, mb = INT_MAX;
-            for (int i = 0; i < kSsBlock / 64; ++i) {
-                pos += i < wid ? wcnt[i] : 0;
-                n += wcnt[i];
-                mb = ured[i] < mb ? ured[i] : mb;
-            }
-            if (listed + n > kCap) {  // uniform: relax what is listed first
-                ss_rounds(a, c, listed, a.nbr, a.wt, ph % 3, pre_s, e0_s, du_s, get);
-                listed = 0;
-                flushed = true;
-            }
-            pos += listed;
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if ((m[i] >> lane) & 1ull) {
-                    const int32_t q = pos + (int32_t)lanes_below(m[i]);
-                    lv[q] = (w0 + i) * 64 + lane;
-                    ldu[q] = dv[i];
-                    le0[q] = d0[i];
-                    le1[q] = d1[i];
-                }
-                pos += __popcll(m[i]);
-            }
-            if (threadIdx.x == 0) {  // the unit's exact bound
-                a.ubase[un] = mb;
-                uold[un] = INT_MAX;
-            }
-            beyond = mb < beyond ? mb : beyond;
-            listed += n;
-            found = found || n > 0;
-            __syncthreads();  // the entries are listed; wcnt and ured are read
+            beyond = ub < beyond ? ub : beyond;
         }
     }
-    if (listed > 0) {
+    __syncthreads();  // the list is complete
+    int32_t listed = lcount;
+    listed = listed < kCap ? listed : kCap;
+    for (int32_t j = threadIdx.x; j < listed; j += kSsBlock) {  // the list's edge ranges, in one pass
+        const int32_t v = lv[j];
+        le0[j] = a.off[v];
+        le1[j] = a.off[v + 1];
+    }
+    const bool any_spill = __syncthreads_or(spilled);
+    if (listed > 0) {  // uniform
         // a workgroup that found few vertices relaxes what they improve
         // into the window itself, in this launch
-        c.local = cur.local && !flushed && listed <= kSsLocalScan;
+        c.local = cur.local && !any_spill && listed <= kSsLocalScan;
         c.q = q0;
         c.qn = qn;
-        ss_rounds(a, c, listed, a.nbr, a.wt, ph % 3, pre_s, e0_s, du_s, get);
+        ss_rounds(a, c, listed, a.nbr, a.wt, ph % 3, pre_s, e0_s, du_s,
+                  [&](int32_t j, int32_t &v, int32_t &e0, int32_t &e1, int32_t &du) {
+                      v = lv[j];
+                      e0 = le0[j];
+                      e1 = le1[j];
+                      du = ldu[j];
+                  });
         if (c.local) ss_drain(a, c, ph % 3, lds, qn, pre_s, e0_s, du_s);
     }
     ss_flush(a, ph % kSsSets, c.nf, c.fdeg, found ? -1 : beyond);
@@ -805,15 +790,14 @@ __device__ __forceinline__ void ss_counts(const SsArgs &a, int32_t set, int32_t 
 // reference's queue starts a window at the least remaining priority too,
 // gpu_priority_queue.h:126-146), or by the end when nothing is pending.
 __device__ __forceinline__ SsRec ss_decide(const SsArgs &a, const SsRec &prev, int32_t n, long long fdeg,
-                                           int32_t mn) {
-    const SsState *st = a.st;
+                                           int32_t mn, int32_t delta, int32_t pull_min) {
     SsRec r = prev;
     if (prev.mode == SS_DONE) return r;
     auto relax_on = [&](int32_t steps_before) {
         r.mode = SS_RELAX;
         r.front = n;
         r.fdeg = fdeg;
-        r.pull = n > st->pull_min;
+        r.pull = n > pull_min;
         r.defer = a.lend != nullptr && !r.pull && fdeg >= a.defer_min;
         r.local = !r.pull && !r.defer && n <= kSsLocalMaxF && fdeg <= kSsLocalMaxE;
         r.steps = steps_before + 1;
@@ -823,7 +807,7 @@ __device__ __forceinline__ SsRec ss_decide(const SsArgs &a, const SsRec &prev, i
         r.mode = SS_ADVANCE;
         r.uq = prev.uq ^ 1;
         r.lo = lo;
-        r.hi = ss_window_end(lo, st->delta);
+        r.hi = ss_window_end(lo, delta);
         r.pull = 0;
         r.defer = 0;
         r.local = 1;  // (in the workgroups that find few vertices)
@@ -869,8 +853,9 @@ __device__ __forceinline__ SsRec ss_decide(const SsArgs &a, const SsRec &prev, i
 // One step, one launch: every workgroup decides the step from what the
 // previous launch left (the same inputs and code, hence the same decision),
 // the first writes the step's record and clears what the next launch adds
-// into (its count set, its flag set, the hub list of two steps before),
-// each workgroup clears its chunks of the bitmap two steps ahead, then the
+// into (its count set, the hub list of two steps before), each workgroup
+// clears its flags of the set the next launch writes and its chunks of the
+// bitmap two steps ahead, then the
 // push, pull, heavy or scan body runs. Graph replays run launch k of a block
 // as step base + k with phase (base + k) mod 12; eager launches pass their
 // step with base 0.
@@ -884,6 +869,7 @@ __global__ void __launch_bounds__(kSsBlock) __attribute__((amdgpu_waves_per_eu(5
     __shared__ uint64_t mask_s[3];
     __shared__ int32_t um_s[64];
     __shared__ int32_t done_s, wsfree_s;
+    __shared__ int32_t *dist_s;
     SsState *st = a.st;
     const int32_t p2 = ph & 1, p3 = ph % 3, p4 = ph & 3;
     // Wave 0 loads what the decision and the bodies need and decides; the
@@ -893,6 +879,8 @@ __global__ void __launch_bounds__(kSsBlock) __attribute__((amdgpu_waves_per_eu(5
     if (threadIdx.x < 64) {
         const int32_t done = st->done;
         const SsRec prev = st->rec[p2 ^ 1];
+        const int32_t delta = st->delta, pull_min = st->pull_min;
+        int32_t *const dist = st->dist;
         int32_t n, mn;
         long long fdeg;
         ss_counts(a, (p3 + 2) % kSsSets, &n, &fdeg, &mn);
@@ -907,15 +895,17 @@ __global__ void __launch_bounds__(kSsBlock) __attribute__((amdgpu_waves_per_eu(5
         int32_t um0 = INT_MAX;
         const int32_t ul = (int32_t)blockIdx.x + lane_id() * (int32_t)gridDim.x;
         if (ul < (a.words + kSsPullUnit - 1) / kSsPullUnit) {
-            const int32_t b0 = a.ubase[ul], b1 = a.ucl[ul], b2 = a.ucl[a.ustride + ul];
+            const int32_t up = ss_upos(ul, (int32_t)gridDim.x);
+            const int32_t b0 = a.ubase[up], b1 = a.ucl[up], b2 = a.ucl[a.ustride + up];
             um0 = b0 < b1 ? b0 : b1;
             um0 = b2 < um0 ? b2 : um0;
         }
         um_s[lane_id()] = um0;
         if (threadIdx.x == 0) {
-            cur_s = done ? prev : ss_decide(a, prev, n, fdeg, mn);
+            cur_s = done ? prev : ss_decide(a, prev, n, fdeg, mn, delta, pull_min);
             done_s = done;
             wsfree_s = prev.mode == SS_HEAVY && prev.pull;
+            dist_s = dist;
             mask_s[0] = fl_clear;
             mask_s[1] = fl_f;
             mask_s[2] = fl_ws;
@@ -926,8 +916,7 @@ __global__ void __launch_bounds__(kSsBlock) __attribute__((amdgpu_waves_per_eu(5
     if (done_s) return;  // uniform: the run has ended
     const SsRec cur = cur_s;
     const uint64_t fl_clear = mask_s[0], fl_f = mask_s[1], fl_ws = mask_s[2];
-    const int32_t um0 = um_s[lane_id()];
-    a.dist = st->dist;
+    a.dist = dist_s;  // (the run's distances, from the state: one recording serves every run)
     if (blockIdx.x == 0) {
         if (threadIdx.x == 0) {
             st->rec[p2] = cur;
@@ -954,10 +943,10 @@ __global__ void __launch_bounds__(kSsBlock) __attribute__((amdgpu_waves_per_eu(5
             a.fdg[line * (kSsLineI32 / 2)] = 0;
             a.mn[line * kSsLineI32] = INT_MAX;
         }
-        uint4 *f4 = reinterpret_cast<uint4 *>(ss_flag_set(a, (p4 + 1) & 3));
-        const int32_t n16 = (a.chunks + 15) / 16;
-        for (int32_t i = threadIdx.x; i < n16; i += kSsBlock) f4[i] = make_uint4(0u, 0u, 0u, 0u);
     }
+    if (threadIdx.x < 4)  // this workgroup's 64 flags of the set the next launch writes
+        reinterpret_cast<uint4 *>(ss_flag_set(a, (p4 + 1) & 3) + (size_t)blockIdx.x * 64)[threadIdx.x] =
+            make_uint4(0u, 0u, 0u, 0u);
     if (cur.mode == SS_DONE) return;  // uniform
     // the bitmap two steps ahead (F_s-1, done with) is free: clear this
     // workgroup's non-empty chunks of it
@@ -981,11 +970,10 @@ __global__ void __launch_bounds__(kSsBlock) __attribute__((amdgpu_waves_per_eu(5
             }
         }
         __syncthreads();  // (the flags were read at the start of the launch)
-        if (threadIdx.x < 64 && ((fl_ws >> threadIdx.x) & 1ull))
-            a.wsflag[blockIdx.x + threadIdx.x * gridDim.x] = 0;
+        if (threadIdx.x < 64 && ((fl_ws >> threadIdx.x) & 1ull)) a.wsflag[blockIdx.x * 64 + threadIdx.x] = 0;
     }
     if (cur.mode == SS_ADVANCE) {
-        ss_scan_body(a, cur, ph, lds, pre_s, e0_s, du_s, q0, qn, um0);
+        ss_scan_body(a, cur, ph, lds, pre_s, e0_s, du_s, q0, qn, um_s);
     } else if (cur.pull) {  // a pull relax step, or the heavy step pulled
         ss_pull_body(a, cur, ph, lds);
     } else {
@@ -1043,6 +1031,7 @@ struct SsSeed {
     int32_t source;
     int32_t delta;
     int32_t pull_min;
+    int32_t grid;  // workgroups of the run's step launches (the flag layout)
 };
 
 // Clears every array of a run, seeds the source as F_0 (bitmap 0, its chunk
@@ -1050,16 +1039,18 @@ struct SsSeed {
 // and writes the record of step -1 (a scan of the window [0, delta) that
 // found vertices and left the source in F', so step 0 relaxes it there).
 __global__ void ss_reset_kernel(SsArgs a, SsSeed sd) {
-    const int32_t units = (a.words + kSsPullUnit - 1) / kSsPullUnit;
     const int32_t lines = kSsSets * kSsStripes;
     const int32_t fwords = kSsFlagSets * a.flag_bytes / 8;
     const int32_t src = sd.source;
-    const int64_t fbyte = 3LL * a.flag_bytes + (src >> 6) / kBsChunkWords;  // the source's chunk flag in set 3
+    // the source's chunk flag in set 3
+    const int64_t fbyte = 3LL * a.flag_bytes + ss_fpos((src >> 6) / kBsChunkWords, sd.grid);
     int32_t n = a.V > fwords ? a.V : fwords;
     n = n > lines ? n : lines;
+    n = n > a.ustride ? n : a.ustride;
+    n = n > a.flag_bytes ? n : a.flag_bytes;
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (i < a.V) a.dist[i] = i == src ? 0 : INT_MAX;
-        if (i < units) {  // nothing pending
+        if (i < a.ustride) {  // nothing pending
             a.ubase[i] = INT_MAX;
             a.ucl[i] = INT_MAX;
             a.ucl[a.ustride + i] = INT_MAX;
@@ -1071,7 +1062,7 @@ __global__ void ss_reset_kernel(SsArgs a, SsSeed sd) {
             a.bm[1][i] = 0;
             a.bm[2][i] = 0;
         }
-        if (i < a.chunks) a.wsflag[i] = 0;
+        if (i < a.flag_bytes) a.wsflag[i] = 0;
         if (i < fwords)
             reinterpret_cast<uint64_t *>(a.flags)[i] = (i == (int32_t)(fbyte >> 3)) ? 1ull << (8 * (fbyte & 7)) : 0ull;
         if (i < lines) {
@@ -1156,7 +1147,8 @@ struct SsWork {
     int32_t *chunk_v = nullptr, *chunk_e0 = nullptr;
     std::map<int32_t, hipGraphExec_t> execs;  // by steps * 12 + first phase (the run's dist is read from the state)
     int32_t recent_steps[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // step launches of the last runs (ring)
-    int32_t grid = 0;  // workgroups of the recorded step launches
+    int32_t grid = 0;  // workgroups of the recorded step launches (the layout of flags and bounds)
+    int32_t units = 0;
     int32_t recent_n = 0;
 };
 
@@ -1219,16 +1211,11 @@ static int ensure_ss(gi_graph *g, SsWork **out) {
     w->pwt = pg->d_w;
     w->pull_min = g->V / kSsPullDiv;
     w->chunks = (int32_t)chunks;
-    w->flag_bytes = (int32_t)bs_flag_bytes(chunks);  // a multiple of 16
+    w->units = (int32_t)ceil_div(words, kSsPullUnit);
     const size_t lines = (size_t)kSsSets * kSsStripes;
     int rc = dalloc(&w->st, 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->bm[i], (size_t)words + 1);
-    if (rc == GI_OK) rc = dalloc(&w->flags, (size_t)kSsFlagSets * w->flag_bytes);
-    w->ustride = (int32_t)ceil_div(words, kSsPullUnit) + 1;
-    if (rc == GI_OK) rc = dalloc(&w->ubase, (size_t)w->ustride);
-    if (rc == GI_OK) rc = dalloc(&w->ucl, 2 * (size_t)w->ustride);
     if (rc == GI_OK) rc = dalloc(&w->ws, (size_t)words + 1);
-    if (rc == GI_OK) rc = dalloc(&w->wsflag, (size_t)w->flag_bytes);
     // a stripe's three counters share its line: |F'| at int 0, the least
     // pending distance at int 1, the out-edges at ints 2-3
     if (rc == GI_OK) rc = dalloc(&w->cnt, lines * kSsLineI32);
@@ -1349,9 +1336,10 @@ static int ensure_split(gi_graph *g, SsWork *w, int32_t delta, hipStream_t s) {
 static int ss_reset(SsWork *w, const SsArgs &a, int32_t source, int32_t delta, int32_t pull_min, hipStream_t s) {
     w->host_flag[0] = 0;  // no run of this graph's engine is in flight: set again by the last step
     const int64_t n = std::max<int64_t>({(int64_t)a.V, (int64_t)kSsFlagSets * a.flag_bytes / 8,
-                                         (int64_t)kSsSets * kSsStripes});
+                                         (int64_t)kSsSets * kSsStripes, (int64_t)a.ustride,
+                                         (int64_t)a.flag_bytes});
     const int rg = (int)std::min<int64_t>(ceil_div(n, 256), 8192);
-    ss_reset_kernel<<<rg, 256, 0, s>>>(a, SsSeed{source, delta, pull_min});
+    ss_reset_kernel<<<rg, 256, 0, s>>>(a, SsSeed{source, delta, pull_min, w->grid});
     GI_LAUNCH_CHECK();
     return GI_OK;
 }
@@ -1393,15 +1381,31 @@ static int32_t ss_first_block(const SsWork *w) {
 
 // Workgroups of the step launches: the resident grid, or max_cta (the
 // schedule's) when positive (at most kSsMaxGrid), but enough that every
-// workgroup owns at most 64 chunks. Recorded steps are dropped when it
-// changes.
-static void ss_set_grid(SsWork *w, int32_t max_cta) {
+// workgroup owns at most 64 chunks. The chunk flags and the pending bounds
+// are laid out for it (workgroup b's 64 at b * 64), so they are allocated
+// again, and the recorded steps dropped, when it changes.
+static int ss_set_grid(SsWork *w, int32_t max_cta) {
     int32_t grid = ss_grid();
     if (max_cta > 0) grid = max_cta < kSsMaxGrid ? max_cta : kSsMaxGrid;
     const int32_t least = (int32_t)ceil_div(w->chunks, 64);
     if (grid < least) grid = least;
-    if (grid != w->grid) drop_ss_execs(w);
-    w->grid = grid;
+    if (grid == w->grid) return GI_OK;
+    drop_ss_execs(w);
+    dfree(w->flags);
+    dfree(w->wsflag);
+    dfree(w->ubase);
+    dfree(w->ucl);
+    w->flags = w->wsflag = nullptr;
+    w->ubase = w->ucl = nullptr;
+    w->grid = 0;
+    w->flag_bytes = 64 * grid;  // a multiple of 16
+    w->ustride = (int32_t)(ceil_div(w->units, 64LL * grid) * 64 * grid);
+    int rc = dalloc(&w->flags, (size_t)kSsFlagSets * w->flag_bytes);
+    if (rc == GI_OK) rc = dalloc(&w->wsflag, (size_t)w->flag_bytes);
+    if (rc == GI_OK) rc = dalloc(&w->ubase, (size_t)w->ustride);
+    if (rc == GI_OK) rc = dalloc(&w->ucl, 2 * (size_t)w->ustride);
+    if (rc == GI_OK) w->grid = grid;
+    return rc;
 }
 
 int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,
@@ -1409,7 +1413,7 @@ int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, 
     SsWork *w = nullptr;
     GI_TRY(ensure_ss(g, &w));
     GI_TRY(ensure_split(g, w, delta, s));
-    ss_set_grid(w, max_cta);
+    GI_TRY(ss_set_grid(w, max_cta));
     SsArgs a = ss_args(g, w, d_dist);
     GI_TRY(ss_reset(w, a, source, delta, w->pull_min, s));
     SsArgs ca = a;
@@ -1457,6 +1461,7 @@ static int sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_m
     SsWork *w = nullptr;
     GI_TRY(ensure_ss(g, &w));
     GI_TRY(ensure_split(g, w, delta, s));
+    GI_TRY(ss_set_grid(w, 0));
     SsArgs a = ss_args(g, w, d_dist);
     a.host_flag = nullptr;
     GI_TRY(ss_reset(w, a, source, delta, pull_min >= 0 ? pull_min : w->pull_min, s));
@@ -1464,7 +1469,7 @@ static int sssp_trace(gi_graph *g, int32_t source, int32_t delta, int32_t pull_m
     int rc = GI_OK;
     for (int i = 0; i < 3 && rc == GI_OK; ++i)
         if (hipEventCreate(&ev[i]) != hipSuccess) rc = fail(GI_ERR_HIP, "event create");
-    const int grid = ss_grid();
+    const int grid = w->grid;
     SsRec r{};
     int32_t k = 0;
     for (; rc == GI_OK && k <= max_slots; ++k) {
