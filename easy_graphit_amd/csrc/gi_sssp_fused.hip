@@ -68,7 +68,10 @@ constexpr int32_t kSsPullDiv = 8;     // a step pulls when |F| > V / kSsPullDiv
 
 constexpr int kSsPullUnit = 4 * (kSsBlock / 64);  // words per unit (pull work, window scan, pending bounds)
 
-constexpr long long kSsDeferDiv = 16;  // F with E / kSsDeferDiv out-edges or more: heavy edges wait
+// F with E / kSsDeferDiv out-edges or more: heavy edges wait (RMAT-24,
+// delta 16, four sources: E/16 5.49 ms, E/64 5.12, E/128 to E/4096 4.78 to
+// 4.81; the grid's windows stay below all of these)
+constexpr long long kSsDeferDiv = 256;
 constexpr long long kSsHeavyPullDiv = 8;  // W with E / kSsHeavyPullDiv out-edges or more: the heavy step pulls
 
 // A step's counts (|F'| or the vertices a scan moved, their out-edges, the
