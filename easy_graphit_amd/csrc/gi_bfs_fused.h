@@ -46,6 +46,14 @@ constexpr int32_t kMaxSumWords = 256;       // summary words (V up to 2^26)
 // with 16-byte loads: w1 [hid][in], w2 [hid][hid] (row = the unit fed), w3
 // [hid][kSelMaxOut] (row = the unit feeding, zero-padded), biases b1 [hid],
 // b2 [hid], b3 [out].
+// The blob holds them in that order from w1, at these uint16 offsets.
+constexpr int kSelOffW1 = 0;
+constexpr int kSelOffW2 = kSelOffW1 + kSelInputs * kSelHidden;
+constexpr int kSelOffW3 = kSelOffW2 + kSelHidden * kSelHidden;
+constexpr int kSelOffB1 = kSelOffW3 + kSelHidden * kSelMaxOut;
+constexpr int kSelOffB2 = kSelOffB1 + kSelHidden;
+constexpr int kSelOffB3 = kSelOffB2 + kSelHidden;
+constexpr int kSelBlob = kSelOffB3 + kSelMaxOut;  // uint16 entries
 struct SelectorView {
     const uint16_t *w1, *b1, *w2, *b2, *w3, *b3;
     int32_t n_out;
@@ -56,6 +64,7 @@ struct SelectorView {
     float threshold;
     float feat_mean[kSelInputs];   // feature normalisation
     float feat_scale[kSelInputs];
+    unsigned long long version;    // the installed model's (an engine copies the blob when it changes)
 };
 
 // One level's bookkeeping, written by the first workgroup of the level's

@@ -79,11 +79,12 @@ struct StepArgs {
     int32_t *parent;
     uint64_t *visited;
     uint64_t *bm[3];            // F_l in bm[l % 3]
-    uint8_t *flags;             // kFlagSets sets of chunk flags (bs_flag_chunk), flag_bytes apart:
+    uint8_t *flags;             // kFlagSets sets of chunk flags (flag_chunk), flag_bytes apart:
     int32_t flag_bytes;         // those of F_l in set (l - 1) % 4
     int32_t *cnt;               // [kCountSets][kStripes] lines: |F_l| in set (l - 1) % 3
-    unsigned long long *fdg;    // the same for the degree sums
+    unsigned long long *fdg;    // the same for the degree sums (in the same lines)
     int32_t *host_flag;         // pinned {done, level, levels}: set when a run ends
+    const uint16_t *selw;       // the engine's copy of the selector's BF16 blob (kSelBlob entries)
 };
 
 struct StepCtx {
@@ -114,6 +115,17 @@ __device__ __forceinline__ T pick3(T p0, T p1, T p2, int32_t i) {
 
 __device__ __forceinline__ uint8_t *flag_set(const StepArgs &a, int32_t i) {
     return a.flags + (size_t)i * a.flag_bytes;
+}
+
+// Workgroup b of a step launch of G workgroups owns the chunks b + j G (j <
+// 64); their flags are stored at b * 64 + j, so that a workgroup reads its
+// 64 with one line.
+__device__ __forceinline__ int32_t flag_pos(int32_t chunk, int32_t G) { return (chunk % G) * 64 + chunk / G; }
+
+// The flag of the chunk holding bitmap word `word`, from a launch of G
+// workgroups' layout.
+__device__ __forceinline__ void flag_chunk(uint8_t *flags, int32_t word, int32_t G) {
+    flags[flag_pos(word / kChunkWords, G)] = 1;
 }
 
 template <class T>
@@ -157,7 +169,7 @@ __device__ __forceinline__ void push_edge(const StepArgs &a, StepCtx &c, int32_t
     unsigned long long old = atomicOr((unsigned long long *)vw, bit);
     if (old & bit) return;
     unsigned long long o2 = atomicOr((unsigned long long *)&c.bm_out[d >> 6], bit);
-    if (o2 == 0) bs_flag_chunk(c.cflag, d >> 6);
+    if (o2 == 0) flag_chunk(c.cflag, d >> 6, (int32_t)gridDim.x);
     if (a.parent) a.parent[d] = s;  // the depth is written in word order by the next step
     c.cnt += 1;
     const int32_t dd = note_degree(a, c, d);
@@ -194,7 +206,7 @@ __device__ __forceinline__ void push_edge_wave(const StepArgs &a, StepCtx &c, in
         won = x & ~old;
         if (won) {
             const unsigned long long o2 = atomicOr((unsigned long long *)&c.bm_out[key], won);
-            if (o2 == 0) bs_flag_chunk(c.cflag, key);
+            if (o2 == 0) flag_chunk(c.cflag, key, (int32_t)gridDim.x);
         }
     }
     const uint64_t ends = __ballot(end);
@@ -601,7 +613,7 @@ __device__ __forceinline__ void step_body(const StepArgs &a0, int32_t *list, int
             }
             bool any = false;
             if (all != ~0ull) any = pull_words(a, c, mode, w0, vis);  // wave-uniform
-            if (__syncthreads_or(any) && threadIdx.x == 0) bs_flag_chunk(c.cflag, un * kPullUnit);
+            if (__syncthreads_or(any) && threadIdx.x == 0) flag_chunk(c.cflag, un * kPullUnit, (int32_t)gridDim.x);
         }
     }
     flush_counts(a, p3, c);
@@ -667,91 +679,110 @@ __device__ __forceinline__ void level_counts(const StepArgs &a, int32_t set, int
 }
 
 // The non-empty chunks among this workgroup's (chunk blockIdx + j G at bit
-// j) in one set of chunk flags; every wave computes the same mask.
+// j) in one set of chunk flags.
 __device__ __forceinline__ uint64_t wg_chunk_mask(const uint8_t *flags, int32_t chunks) {
-    const int32_t ch = (int32_t)blockIdx.x + lane_id() * (int32_t)gridDim.x;
-    return __ballot(ch < chunks && flags[ch] != 0);
+    const int32_t j = lane_id();
+    const int32_t ch = (int32_t)blockIdx.x + j * (int32_t)gridDim.x;
+    return __ballot(ch < chunks && flags[(int32_t)blockIdx.x * 64 + j] != 0);
 }
 
-// The selector network's weights a lane needs, loaded before the level's
-// counts are known so that the loads overlap: lane u of wave w holds the
-// first-layer row of hidden unit u and the second-layer weights of unit u
-// over inputs [16 w, 16 w + 16).
-struct SelWeights {
+// The selector network's parameters, loaded at the start of a launch with
+// everything else the decision reads (the engine's copy of the blob always
+// holds a model or zeros, so the loads need not wait on the mode): lane u of
+// wave w holds hidden unit u's second-layer weights over inputs [16 w, 16 w
+// + 16); lane u of the deciding wave also its first-layer row, its row of w3 (the 8
+// outputs it feeds), b1[u] and b2[u] (the low and high halves of b), and
+// b3[u] for u < kSelMaxOut.
+struct SelHead {
     uint4 w1[kSelInputs / 8];
-    uint4 w2[2];
+    uint4 w3[1];
+    uint32_t b;
+    uint32_t b3;
 };
 
 static_assert(kStepBlock == 4 * 64 && kSelHidden == 64 && kSelInputs == 16, "the network's split over 4 waves");
-__device__ __forceinline__ void load_sel_weights(const SelectorView &sel, SelWeights *q) {
-    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    const uint4 *r1 = reinterpret_cast<const uint4 *>(sel.w1 + lane * kSelInputs);
-    const uint4 *r2 = reinterpret_cast<const uint4 *>(sel.w2 + lane * kSelHidden + 16 * wid);
-#pragma unroll
-    for (int i = 0; i < kSelInputs / 8; ++i) q->w1[i] = r1[i];
-    q->w2[0] = r2[0];
-    q->w2[1] = r2[1];
+__device__ __forceinline__ void load_sel_slice(const uint16_t *blob, uint4 *w2) {
+    const int32_t u = lane_id(), wid = threadIdx.x >> 6;
+    const uint4 *r2 = reinterpret_cast<const uint4 *>(blob + kSelOffW2 + u * kSelHidden + 16 * wid);
+    w2[0] = r2[0];
+    w2[1] = r2[1];
+}
+
+__device__ __forceinline__ void load_sel_head(const uint16_t *blob, SelHead *q) {
+    const int32_t u = lane_id();
+    const uint4 *r1 = reinterpret_cast<const uint4 *>(blob + kSelOffW1 + u * kSelInputs);
+    q->w1[0] = r1[0];
+    q->w1[1] = r1[1];
+    q->w3[0] = *reinterpret_cast<const uint4 *>(blob + kSelOffW3 + u * kSelMaxOut);
+    q->b = (uint32_t)blob[kSelOffB1 + u] | ((uint32_t)blob[kSelOffB2 + u] << 16);
+    q->b3 = u < kSelMaxOut ? (uint32_t)blob[kSelOffB3 + u] : 0u;
 }
 
 __device__ __forceinline__ bool sel_learned(const SelectorView &sel) {
     return sel.mode == GI_SELECT_LEARNED && sel.w1 != nullptr;
 }
 
-// The mode of step l (n = |F_l| > 0): GraphIt's hybrid rule or one of its
-// schedules, or the MLP. Every thread of the workgroup calls it and gets the
-// same mode. The network: every wave computes the first layer (lane u:
-// hidden unit u), wave w the part of the second layer over its 16 inputs
-// into `part` (LDS, 4 x 64 floats), wave 0 the sums (in wave order) and the
-// output layer.
-__device__ __forceinline__ int32_t select_mode(const SelectorView &sel, const SelWeights &q, int32_t n,
-                                               unsigned long long fdeg, long long unv, long long unv_e, int32_t l,
-                                               int32_t prev_mode, int32_t prev_n, int32_t V, int64_t E,
-                                               float *part, int32_t *mode_s) {
-    if (!sel_learned(sel)) {
-        if (sel.mode == GI_SELECT_S1) return sel.mode_s1;
-        if (sel.mode == GI_SELECT_S2) return sel.mode_s2;
-        return ((double)n <= (double)sel.threshold * (double)V) ? sel.mode_s1 : sel.mode_s2;
-    }
-    static_assert(kSelMaxOut == 8, "one 16-byte row of output weights per hidden unit");
-    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
-    // every lane computes the whole input vector (the same values, so no
-    // lane waits on another's branch of the feature switch)
+// GraphIt's hybrid rule or one of its schedules (not the network).
+__device__ __forceinline__ int32_t rule_mode(const SelectorView &sel, int32_t n, int32_t V) {
+    if (sel.mode == GI_SELECT_S1) return sel.mode_s1;
+    if (sel.mode == GI_SELECT_S2) return sel.mode_s2;
+    return ((double)n <= (double)sel.threshold * (double)V) ? sel.mode_s1 : sel.mode_s2;
+}
+
+// The network's first layer (the deciding wave; lane u: hidden unit u). Every lane
+// computes the whole input vector (the same values, so no lane waits on
+// another's branch of the feature switch).
+__device__ __forceinline__ float sel_layer1(const SelectorView &sel, const SelHead &q, int32_t n,
+                                            unsigned long long fdeg, long long unv, long long unv_e, int32_t l,
+                                            int32_t prev_mode, int32_t prev_n, int32_t V, int64_t E) {
     float feats[kSelInputs];
     bfs_features_f32(feats, n, fdeg, unv, unv_e, l, prev_mode, prev_n, V, E);
-    float a1 = bf16_to_f32(sel.b1[lane]);
+    float a1 = bf16_to_f32((uint16_t)(q.b & 0xffffu));
 #pragma unroll
     for (int k = 0; k < kSelInputs; ++k) a1 += (feats[k] - sel.feat_mean[k]) * sel.feat_scale[k] * bf16_in(q.w1, k);
-    const float h1 = a1 > 0.f ? a1 : 0.f;
+    return a1 > 0.f ? a1 : 0.f;
+}
+
+// Wave w's part of the second layer over inputs [16 w, 16 w + 16) (h1 from
+// LDS, in input order).
+__device__ __forceinline__ float sel_layer2_part(const float *h1, const uint4 *w2) {
+    const int32_t wid = threadIdx.x >> 6;
     float p = 0.f;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) p += __shfl(h1, 16 * wid + k) * bf16_in(&q.w2[0], k);
+    for (int k = 0; k < 8; ++k) p += h1[16 * wid + k] * bf16_in(&w2[0], k);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) p += __shfl(h1, 16 * wid + 8 + k) * bf16_in(&q.w2[1], k);
-    part[wid * 64 + lane] = p;
-    __syncthreads();
-    if (wid == 0) {
-        const uint4 w3q[1] = {*reinterpret_cast<const uint4 *>(sel.w3 + lane * kSelMaxOut)};
-        const uint16_t b3v = lane < sel.n_out ? sel.b3[lane] : (uint16_t)0;
-        const float a2 = bf16_to_f32(sel.b2[lane]) + part[lane] + part[64 + lane] + part[128 + lane] + part[192 + lane];
-        const float h2 = a2 > 0.f ? a2 : 0.f;
-        // output o = b3[o] + sum over the hidden units of h2 * w3[unit][o]
-        int32_t mode = sel.mode_s1;
-        float best = 3.4e38f;
+    for (int k = 0; k < 8; ++k) p += h1[16 * wid + 8 + k] * bf16_in(&w2[1], k);
+    return p;
+}
+
+// The deciding wave: the second layer's sum (its parts in wave order) and
+// the output layer; the fastest allowed mode. Every lane of the wave gets it.
+__device__ __forceinline__ int32_t sel_output(const SelectorView &sel, const SelHead &q, const float *part) {
+    static_assert(kSelMaxOut == 8, "one 16-byte row of output weights per hidden unit");
+    const int32_t lane = lane_id();
+    const float a2 = bf16_to_f32((uint16_t)(q.b >> 16)) + part[lane] + part[64 + lane] + part[128 + lane] +
+                     part[192 + lane];
+    const float h2 = a2 > 0.f ? a2 : 0.f;
+    // output o = b3[o] + sum over the hidden units of h2 * w3[unit][o]; the
+    // eight butterfly sums run side by side (w3 is zero past n_out)
+    float t[kSelMaxOut];
 #pragma unroll
-        for (int o = 0; o < kSelMaxOut; ++o) {
-            if (o >= sel.n_out) break;  // uniform
-            float t = h2 * bf16_in(w3q, o);
-            for (int x = 32; x > 0; x >>= 1) t += __shfl_xor(t, x);
-            const float yo = t + bf16_to_f32((uint16_t)__shfl((int32_t)b3v, o));
-            if (((sel.allowed >> o) & 1) && yo < best) {
-                best = yo;
-                mode = o;
-            }
+    for (int o = 0; o < kSelMaxOut; ++o) t[o] = h2 * bf16_in(q.w3, o);
+#pragma unroll
+    for (int x = 32; x > 0; x >>= 1)
+#pragma unroll
+        for (int o = 0; o < kSelMaxOut; ++o) t[o] += __shfl_xor(t[o], x);
+    int32_t mode = sel.mode_s1;
+    float best = 3.4e38f;
+#pragma unroll
+    for (int o = 0; o < kSelMaxOut; ++o) {
+        const float yo = t[o] + bf16_to_f32((uint16_t)__builtin_amdgcn_readlane((int32_t)q.b3, o));
+        if (o < sel.n_out && ((sel.allowed >> o) & 1) && yo < best) {
+            best = yo;
+            mode = o;
         }
-        if (lane == 0) *mode_s = mode;
     }
-    __syncthreads();
-    return *mode_s;
+    return mode;
 }
 
 // The end of a run: F_l is empty. One thread.
@@ -775,40 +806,115 @@ __device__ __forceinline__ void finish_run(const StepArgs &a, int32_t l, const B
 // step's set-up read is loaded at once: the rotations' indices come from
 // the phase ph = l mod 12, a launch argument, not from the state. The first
 // workgroup writes the record of level l and clears what the next launch
-// adds into (count set (l + 1) % 3, flags (l + 1) % 4, hub list (l + 2) %
-// 3): each was last read by the launch before this one and is next written
-// by the one after. An empty F_l ends the run. Graph replays run launch k of
+// adds into (count set (l + 1) % 3, hub list (l + 2) % 3), each workgroup
+// its flags of set (l + 1) % 4: each was last read by the launch before this
+// one and is next written by the one after. An empty F_l ends the run. Graph replays run launch k of
 // a block as level base + k (base kept in the state); eager launches pass
 // their level with base 0.
-__device__ __forceinline__ void level_body(const StepArgs &a, int32_t k, int32_t ph) {
-    __shared__ int32_t list[kChunkVerts];  // push: a chunk's vertices; first, the network's partial sums
-    __shared__ int32_t mode_s;
-    static_assert(sizeof(int32_t) * kChunkVerts >= sizeof(float) * kStepBlock, "the partial sums fit");
-    BfsState *st = a.st;
-    const SelectorView &sel = st->sel;  // read from memory where the network needs it
-    const int32_t p2 = ph & 1, p3 = ph % 3, p4 = ph & 3;
-    const bool learned = sel_learned(sel);
-    SelWeights q;
-    if (learned) load_sel_weights(sel, &q);
-    const int32_t done = st->done, base = st->base, forced = st->force_mode;
-    const BfsLevel prev = st->lv[p2 ^ 1];
-    int32_t n;
+// The decision of step l = base + k (every thread of the workgroup calls
+// it; the result is uniform). One wave loads what the decision and the
+// step's set-up read (each of these loads touches up to 64 lines: one wave
+// per workgroup loading them keeps the launch's requests to the L2 down) and
+// decides; the network's second layer is split over the four waves. With
+// `step`, the first workgroup writes the level's record and clears the count
+// set the next launch adds into, and each workgroup its flags of the set
+// the next launch writes; otherwise (the traced levels' decision launch)
+// thread 0 stores the mode as the one the traced step launch runs with. An
+// empty F_l ends the run. `scratch`: 320 floats of LDS.
+struct LevelDec {
+    int32_t go, mode, n, l;
     unsigned long long fdeg;
-    level_counts(a, (p3 + 2) % 3, &n, &fdeg);  // every wave: the same totals
-    const uint64_t fl_clear = wg_chunk_mask(flag_set(a, (p4 + 2) & 3), a.chunks);
-    const uint64_t fl_list = wg_chunk_mask(flag_set(a, (p4 + 3) & 3), a.chunks);
-    if (done) return;  // uniform: set only by the launch that found F empty
-    const int32_t l = base + k;
-    if (n == 0) {  // uniform
-        if (blockIdx.x == 0 && threadIdx.x == 0) finish_run(a, l, prev);
-        return;
+    uint64_t fl_clear, fl_list;
+};
+
+__device__ __forceinline__ void level_decide(const StepArgs &a, int32_t k, int32_t ph, bool step, float *scratch,
+                                             LevelDec *d) {
+    __shared__ int32_t go_s, net_s, mode_s, n_s, l_s;
+    __shared__ unsigned long long fdeg_s;
+    __shared__ uint64_t mask_s[2];
+    float *h1_s = scratch, *part_s = scratch + 64;
+    BfsState *st = a.st;
+    const int32_t p2 = ph & 1, p3 = ph % 3, p4 = ph & 3;
+    const int32_t lane = lane_id(), wid = threadIdx.x >> 6;
+    constexpr int32_t dw = 0;  // the deciding wave
+    const SelectorView &sel = st->sel;
+    uint4 w2[2];
+    load_sel_slice(a.selw, w2);
+    SelHead q;
+    BfsLevel prev;
+    int32_t n = 0, mode = 0, l = 0;
+    unsigned long long fdeg = 0;
+    long long unv = 0, unv_e = 0;
+    if (wid == dw) {
+        load_sel_head(a.selw, &q);
+        const bool learned = sel_learned(sel);
+        const int32_t done = st->done, base = step ? st->base : 0, forced = step ? st->force_mode : -1;
+        prev = st->lv[p2 ^ 1];
+        level_counts(a, (p3 + 2) % 3, &n, &fdeg);
+        const uint64_t fl_clear = step ? wg_chunk_mask(flag_set(a, (p4 + 2) & 3), a.chunks) : 0ull;
+        const uint64_t fl_list = step ? wg_chunk_mask(flag_set(a, (p4 + 3) & 3), a.chunks) : 0ull;
+        l = base + k;
+        int32_t go = 0, net = 0;
+        if (!done && n == 0) {  // wave-uniform: F_l is empty
+            if (blockIdx.x == 0 && lane == 0) finish_run(a, l, prev);
+        } else if (!done) {
+            go = 1;
+            unv = prev.unvisited - n;
+            unv_e = prev.unvisited_edges - (long long)fdeg;
+            if (forced >= 0) {
+                mode = forced;
+            } else if (!learned) {
+                mode = rule_mode(sel, n, a.V);
+            } else {
+                net = 1;
+                h1_s[lane] = sel_layer1(sel, q, n, fdeg, unv, unv_e, l, prev.mode, prev.count, a.V, a.E);
+            }
+            if (step) {
+                if (blockIdx.x == 0) {
+                    static_assert(kStripes == 64, "one stripe per lane");
+                    const int32_t line = ((p3 + 1) % kCountSets) * kStripes + lane;
+                    a.cnt[line * kStripeI32] = 0;
+                    a.fdg[line * kStripeU64] = 0;
+                }
+                if (lane < 4)  // this workgroup's flags of the set the next launch writes
+                    reinterpret_cast<uint4 *>(flag_set(a, (p4 + 1) & 3) + (size_t)blockIdx.x * 64)[lane] =
+                        make_uint4(0u, 0u, 0u, 0u);
+            }
+        }
+        if (lane == 0) {
+            go_s = go;
+            net_s = net;
+            mode_s = mode;
+            n_s = n;
+            l_s = l;
+            fdeg_s = fdeg;
+            mask_s[0] = fl_clear;
+            mask_s[1] = fl_list;
+        }
     }
-    const long long unv = prev.unvisited - n, unv_e = prev.unvisited_edges - (long long)fdeg;
-    const int32_t mode = forced >= 0 ? forced
-                                     : select_mode(sel, q, n, fdeg, unv, unv_e, l, prev.mode, prev.count, a.V, a.E,
-                                                   reinterpret_cast<float *>(list), &mode_s);
-    if (blockIdx.x == 0) {
-        if (threadIdx.x == 0) {
+    __syncthreads();
+    d->go = go_s;
+    if (!d->go) return;  // uniform
+    if (net_s) {  // uniform: the network's second layer over the four waves, then its output in the deciding wave
+        part_s[wid * 64 + lane] = sel_layer2_part(h1_s, w2);
+        __syncthreads();
+        if (wid == dw) {
+            mode = sel_output(sel, q, part_s);
+            if (lane == 0) mode_s = mode;
+        }
+        __syncthreads();
+    }
+    d->mode = mode_s;
+    d->n = n_s;
+    d->l = l_s;
+    d->fdeg = fdeg_s;
+    d->fl_clear = mask_s[0];
+    d->fl_list = mask_s[1];
+    if (wid == dw && lane == 0) {
+        mode = d->mode;
+        if (!step) {
+            st->force_mode = mode;
+        } else if (blockIdx.x == 0) {
             BfsLevel &cur = st->lv[p2];
             cur.level = l;
             cur.mode = mode;
@@ -824,17 +930,19 @@ __device__ __forceinline__ void level_body(const StepArgs &a, int32_t k, int32_t
             st->hub_n[(p3 + 2) % 3] = 0;
             st->giant_n[(p3 + 2) % 3] = 0;
         }
-        const int32_t cs = (p3 + 1) % kCountSets;
-        if (threadIdx.x < kStripes) {
-            a.cnt[(cs * kStripes + threadIdx.x) * kStripeI32] = 0;
-            a.fdg[(cs * kStripes + threadIdx.x) * kStripeU64] = 0;
-        }
-        uint4 *f4 = reinterpret_cast<uint4 *>(flag_set(a, (p4 + 1) & 3));
-        const int32_t n16 = (a.chunks + 15) / 16;
-        for (int32_t i = threadIdx.x; i < n16; i += kStepBlock) f4[i] = make_uint4(0u, 0u, 0u, 0u);
     }
-    if (mode >= BFS_PULL_THREAD) step_body<kPullStep>(a, list, l, ph, mode, n, fdeg, fl_clear, fl_list);  // uniform
-    else step_body<kPushStep>(a, list, l, ph, mode, n, fdeg, fl_clear, fl_list);
+}
+
+__device__ __forceinline__ void level_body(const StepArgs &a, int32_t k, int32_t ph) {
+    __shared__ int32_t list[kChunkVerts];  // push: a chunk's vertices; first, the network's activations
+    static_assert(sizeof(int32_t) * kChunkVerts >= sizeof(float) * 5 * 64, "the activations fit");
+    LevelDec d;
+    level_decide(a, k, ph, true, reinterpret_cast<float *>(list), &d);
+    if (!d.go) return;  // uniform: the run has ended
+    if (d.mode >= BFS_PULL_THREAD)  // uniform
+        step_body<kPullStep>(a, list, d.l, ph, d.mode, d.n, d.fdeg, d.fl_clear, d.fl_list);
+    else
+        step_body<kPushStep>(a, list, d.l, ph, d.mode, d.n, d.fdeg, d.fl_clear, d.fl_list);
 }
 
 // The recorded level loop launches bfs_step, block launch k as level
@@ -858,24 +966,9 @@ GI_STEP_KERNEL(bfs_pull_step_traced)
 // with, so the host knows the direction before it launches; an empty F_l
 // ends the run.
 __global__ void __launch_bounds__(kStepBlock) bfs_decide_kernel(StepArgs a, int32_t l) {
-    __shared__ float part[kStepBlock];
-    __shared__ int32_t mode_s;
-    BfsState *st = a.st;
-    const SelectorView &sel = st->sel;
-    if (st->done) return;
-    const BfsLevel prev = st->lv[(l + 1) & 1];
-    int32_t n;
-    unsigned long long fdeg;
-    level_counts(a, (l + 2) % kCountSets, &n, &fdeg);
-    if (n == 0) {
-        if (threadIdx.x == 0) finish_run(a, l, prev);
-        return;
-    }
-    SelWeights q;
-    if (sel_learned(sel)) load_sel_weights(sel, &q);
-    const int32_t mode = select_mode(sel, q, n, fdeg, prev.unvisited - n, prev.unvisited_edges - (long long)fdeg, l,
-                                     prev.mode, prev.count, a.V, a.E, part, &mode_s);
-    if (threadIdx.x == 0) st->force_mode = mode;
+    __shared__ float scratch[5 * 64];
+    LevelDec d;
+    level_decide(a, l, l % 12, false, scratch, &d);
 }
 
 // Sweeps: the selector inputs of step l.
@@ -1037,7 +1130,7 @@ __global__ void bfs_cand_kernel(const int32_t *poff, const int32_t *pnbr, const 
 // stripe), the hub list. Repeated sources count once.
 __global__ void bfs_seed_sources_kernel(BfsState *st, const int32_t *src, int32_t n, int32_t single,
                                         uint64_t *bm0, uint8_t *flags3, uint64_t *visited, const int32_t *off,
-                                        int32_t *cnt2, unsigned long long *fdg2) {
+                                        int32_t *cnt2, unsigned long long *fdg2, int32_t step_grid) {
     int32_t cnt = 0;
     unsigned long long fd = 0;
     for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -1048,7 +1141,7 @@ __global__ void bfs_seed_sources_kernel(BfsState *st, const int32_t *src, int32_
         atomicOr((unsigned long long *)&visited[v >> 6], bit);
         st->depth[v] = 0;
         if (st->parent) st->parent[v] = v;
-        bs_flag_chunk(flags3, v >> 6);
+        flag_chunk(flags3, v >> 6, step_grid);
         cnt += 1;
         const int32_t dv = off[v + 1] - off[v];
         fd += (unsigned long long)dv;
@@ -1105,6 +1198,8 @@ struct FusedWork {
     std::map<int32_t, hipGraphExec_t> execs;
     int32_t recent_levels[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // depths of the last runs (ring)
     int32_t recent_n = 0;
+    uint16_t *selw = nullptr;                 // copy of the selector blob the step launches read
+    unsigned long long sel_version = ~0ull;   // the model it was copied from
 };
 
 static PlanRegistry<FusedWork> g_fused;
@@ -1119,9 +1214,9 @@ static void free_work(FusedWork *w) {
     dfree(w->st);
     for (auto p : w->bm) dfree(p);
     dfree(w->flags);
-    dfree(w->cnt);
-    dfree(w->fdg);
+    dfree(w->cnt);  // (fdg points into it)
     dfree(w->peek);
+    dfree(w->selw);
     dfree(w->reached);
     dfree(w->dead);
     dfree(w->cand);
@@ -1150,12 +1245,17 @@ static int ensure_fused(gi_graph *g, FusedWork **out) {
     w->cand_first = g->degree_view ? 1 : 0;
     w->poff = pg->d_off;
     w->pnbr = pg->d_nbr;
-    w->flag_bytes = (int32_t)bs_flag_bytes(chunks);  // a multiple of 16
+    // chunk flags laid out per workgroup (flag_pos) for the largest step grid
+    // (gi_bfs_set_bins allows up to 8 workgroups per CU)
+    w->flag_bytes = 64 * 8 * device_info().num_cus;  // a multiple of 16
     int rc = dalloc(&w->st, 1);
     for (int i = 0; i < 3 && rc == GI_OK; ++i) rc = dalloc(&w->bm[i], (size_t)words + 1);
     if (rc == GI_OK) rc = dalloc(&w->flags, (size_t)kFlagSets * w->flag_bytes);
+    // a stripe's count (int 0) and degree sum (ints 2-3) share its line
+    static_assert(kStripeI32 == 2 * kStripeU64, "stripes of both counters are lines of the same size");
     if (rc == GI_OK) rc = dalloc(&w->cnt, (size_t)kCountSets * kStripes * kStripeI32);
-    if (rc == GI_OK) rc = dalloc(&w->fdg, (size_t)kCountSets * kStripes * kStripeU64);
+    if (rc == GI_OK) w->fdg = reinterpret_cast<unsigned long long *>(w->cnt + 2);
+    if (rc == GI_OK) rc = dalloc(&w->selw, (size_t)kSelBlob);
     if (rc == GI_OK) rc = dalloc(&w->peek, 1);
     if (rc == GI_OK) rc = dalloc(&w->reached, 1);
     if (rc == GI_OK) rc = dalloc(&w->dead, (size_t)words + 1);
@@ -1235,6 +1335,7 @@ static StepArgs step_args(gi_graph *g, FusedWork *w) {
     a.cnt = w->cnt;
     a.fdg = w->fdg;
     a.host_flag = w->host_flag;
+    a.selw = w->selw;
     return a;
 }
 
@@ -1273,6 +1374,10 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
     const bool learned = sel.mode == GI_SELECT_LEARNED && sel.w1 != nullptr;
     const bool may_twc = learned ? ((sel.allowed >> BFS_PUSH_TWC) & 1) != 0
                                  : (sel.mode_s1 == BFS_PUSH_TWC || sel.mode_s2 == BFS_PUSH_TWC);
+    if (learned && w->sel_version != sel.version) {  // the step launches read the engine's copy
+        GI_HIP(hipMemcpyAsync(w->selw, sel.w1, sizeof(uint16_t) * kSelBlob, hipMemcpyDeviceToDevice, s));
+        w->sel_version = sel.version;
+    }
     // ... during push steps of at most V/64 vertices and E/64 edges
     SeedArgs sa{sel,
                 depth,
@@ -1313,10 +1418,11 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
         GI_HIP(hipMemcpyAsync(w->d_src, h_sources, sizeof(int32_t) * n_sources, hipMemcpyHostToDevice, s));
         int sg = (int)ceil_div(n_sources, 256);
         bfs_seed_sources_kernel<<<sg < 1024 ? sg : 1024, 256, 0, s>>>(
-            w->st, w->d_src, n_sources, 0, w->bm[0], w->flags + 3 * w->flag_bytes, g->d_visited, g->d_off, cnt2, fdg2);
+            w->st, w->d_src, n_sources, 0, w->bm[0], w->flags + 3 * w->flag_bytes, g->d_visited, g->d_off, cnt2, fdg2,
+            step_grid(w));
     } else {
         bfs_seed_sources_kernel<<<1, 64, 0, s>>>(w->st, nullptr, 1, single, w->bm[0], w->flags + 3 * w->flag_bytes,
-                                                 g->d_visited, g->d_off, cnt2, fdg2);
+                                                 g->d_visited, g->d_off, cnt2, fdg2, step_grid(w));
     }
     GI_LAUNCH_CHECK();
     if (h_sources) GI_HIP(hipStreamSynchronize(s));  // h_sources may go away on return

@@ -21,6 +21,7 @@ struct SelectorModel {
     bool loaded = false;
 };
 static SelectorModel g_model;
+static unsigned long long g_model_version = 0;  // advanced by every load and clear
 
 static uint16_t f32_to_bf16(float x) {
     uint32_t u;
@@ -39,12 +40,13 @@ SelectorView selector_view() {
     SelectorView v{};
     if (!g_model.loaded) return v;
     const uint16_t *p = g_model.w;  // 16-byte rows: w1 at 0, w2 at 2048, w3 at 10240 bytes
-    v.w1 = p; p += kSelInputs * kSelHidden;
-    v.w2 = p; p += kSelHidden * kSelHidden;
-    v.w3 = p; p += kSelHidden * kSelMaxOut;
-    v.b1 = p; p += kSelHidden;
-    v.b2 = p; p += kSelHidden;
-    v.b3 = p;
+    v.w1 = p + kSelOffW1;
+    v.w2 = p + kSelOffW2;
+    v.w3 = p + kSelOffW3;
+    v.b1 = p + kSelOffB1;
+    v.b2 = p + kSelOffB2;
+    v.b3 = p + kSelOffB3;
+    v.version = g_model_version;
     v.n_out = g_model.n_out;
     v.allowed = g_model.allowed;
     v.mode = GI_SELECT_LEARNED;
@@ -204,8 +206,7 @@ int gi_selector_load(const float *params, int32_t n_out, int32_t allowed_mask,
     const float *w1 = params, *b1 = w1 + kSelInputs * kSelHidden, *w2 = b1 + kSelHidden;
     const float *b2 = w2 + kSelHidden * kSelHidden, *w3 = b2 + kSelHidden, *b3 = w3 + kSelHidden * n_out;
     // w3 rows padded to kSelMaxOut outputs: one 16-byte load per hidden unit
-    std::vector<uint16_t> h((size_t)(kSelInputs * kSelHidden + kSelHidden * kSelHidden + kSelHidden * kSelMaxOut +
-                                     2 * kSelHidden + kSelMaxOut), 0);
+    std::vector<uint16_t> h((size_t)kSelBlob, 0);
     uint16_t *o = h.data();
     for (int j = 0; j < kSelHidden; ++j)
         for (int k = 0; k < kSelInputs; ++k) *o++ = f32_to_bf16(w1[k * kSelHidden + j]);
@@ -227,12 +228,14 @@ int gi_selector_load(const float *params, int32_t n_out, int32_t allowed_mask,
     g_model.n_out = n_out;
     g_model.allowed = allowed_mask & ((1 << n_out) - 1);
     g_model.loaded = true;
+    ++g_model_version;
     return GI_OK;
 }
 
 int gi_selector_clear(void) {
     if (g_model.w) dfree(g_model.w);
     g_model = SelectorModel();
+    ++g_model_version;
     return GI_OK;
 }
 
