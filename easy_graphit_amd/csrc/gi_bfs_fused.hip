@@ -1001,6 +1001,9 @@ struct SeedArgs {
 };
 
 struct ResetArgs {
+    int32_t source;         // >= 0: the run's one source, seeded here (F_0)
+    int32_t step_grid;      // workgroups of the step launches (the chunk flags' layout)
+    const int32_t *off;     // out-edge offsets (the source's degree)
     int32_t *depth;
     int32_t depth_len;      // depth entries to clear (the rest is never read)
     int32_t *parent;
@@ -1016,9 +1019,18 @@ struct ResetArgs {
 };
 
 // Clears every array of a run in one pass and initialises the state.
+// A single source (r.source >= 0) is seeded in the same pass by the threads
+// that clear its entries (depth 0, parent itself, its bit in visited and in
+// F_0, its chunk flag in set 3, its count and degree in count set 2 (what
+// step 0 reads), the first step's hub lists).
 __global__ void bfs_reset_kernel(BfsState *st, SeedArgs sa, ResetArgs r, int64_t E) {
     const int32_t stride = gridDim.x * blockDim.x;
     const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t src = r.source;
+    const int32_t sw = src >= 0 ? src >> 6 : -1;                // its bitmap word
+    const uint64_t sbit = src >= 0 ? 1ull << (src & 63) : 0ull;
+    // its chunk flag in set 3, as a 64-bit word of the flag array and a byte in it
+    const int64_t sfb = src >= 0 ? 3LL * (r.flag_words / kFlagSets) * 8 + flag_pos(sw / kChunkWords, r.step_grid) : -1;
     // depths [0, depth_len) and parents [0, V): 16-byte stores where both
     // arrays are 16-byte aligned, then the remainder one by one
     const int32_t dlen = r.depth_len, plen = r.parent ? r.V : 0;
@@ -1026,41 +1038,67 @@ __global__ void bfs_reset_kernel(BfsState *st, SeedArgs sa, ResetArgs r, int64_t
     const bool aligned = ((reinterpret_cast<uintptr_t>(r.depth) | reinterpret_cast<uintptr_t>(r.parent)) & 15) == 0;
     const int32_t head = aligned ? vlen / 4 * 4 : 0;
     const int4 none = make_int4(-1, -1, -1, -1);
+    const int32_t sq = src >= 0 && src < head ? src >> 2 : -1;  // the source's 16 bytes
     for (int32_t q = tid; q < head / 4; q += stride) {
         const int32_t i = 4 * q;
-        if (i + 4 <= dlen) reinterpret_cast<int4 *>(r.depth)[q] = none;
+        int4 dv = none, pv = none;
+        if (q == sq) {  // the source: depth 0, parent itself
+            const int32_t j = src & 3;
+            dv.x = j == 0 ? 0 : -1;
+            dv.y = j == 1 ? 0 : -1;
+            dv.z = j == 2 ? 0 : -1;
+            dv.w = j == 3 ? 0 : -1;
+            pv.x = j == 0 ? src : -1;
+            pv.y = j == 1 ? src : -1;
+            pv.z = j == 2 ? src : -1;
+            pv.w = j == 3 ? src : -1;
+        }
+        if (i + 4 <= dlen) reinterpret_cast<int4 *>(r.depth)[q] = dv;
         else
             for (int k = 0; k < 4; ++k)
-                if (i + k < dlen) r.depth[i + k] = -1;
-        if (i + 4 <= plen) reinterpret_cast<int4 *>(r.parent)[q] = none;
+                if (i + k < dlen) r.depth[i + k] = i + k == src ? 0 : -1;
+        if (i + 4 <= plen) reinterpret_cast<int4 *>(r.parent)[q] = pv;
         else
             for (int k = 0; k < 4; ++k)
-                if (i + k < plen) r.parent[i + k] = -1;
+                if (i + k < plen) r.parent[i + k] = i + k == src ? src : -1;
     }
     for (int32_t i = head + tid; i < vlen; i += stride) {
-        if (i < dlen) r.depth[i] = -1;
-        if (i < plen) r.parent[i] = -1;
+        if (i < dlen) r.depth[i] = i == src ? 0 : -1;
+        if (i < plen) r.parent[i] = i == src ? src : -1;
     }
+    if (tid == 0 && src >= dlen && src < r.V) r.depth[src] = 0;  // a source past the cleared depths
     const int32_t lines = kCountSets * kStripes;
     int32_t n = r.words > r.flag_words ? r.words : r.flag_words;
     n = n > lines ? n : lines;
     for (int32_t i = tid; i < n; i += stride) {
         if (i < r.words) {
-            r.visited[i] = r.dead[i];
-            r.bm[0][i] = 0;
+            const uint64_t b = i == sw ? sbit : 0ull;
+            r.visited[i] = r.dead[i] | b;
+            r.bm[0][i] = b;
             r.bm[1][i] = 0;
             r.bm[2][i] = 0;
         }
-        if (i < r.flag_words) r.flags[i] = 0;
+        if (i < r.flag_words) r.flags[i] = i == (int32_t)(sfb >> 3) ? 1ull << (8 * (sfb & 7)) : 0ull;
         if (i < lines) {
-            r.cnt[i * kStripeI32] = 0;
-            r.fdg[i * kStripeU64] = 0;
+            const bool seed = src >= 0 && i == 2 * kStripes;  // count set 2 (level -1), first stripe
+            r.cnt[i * kStripeI32] = seed ? 1 : 0;
+            r.fdg[i * kStripeU64] = seed ? (unsigned long long)(r.off[src + 1] - r.off[src]) : 0ull;
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         for (int i = 0; i < 3; ++i) {
             st->hub_n[i] = 0;
             st->giant_n[i] = 0;
+        }
+        if (src >= 0 && sa.want_hubs) {  // the first step's grid bin
+            const int32_t dv = r.off[src + 1] - r.off[src];
+            if (dv >= sa.hub_min * kGiantFactor) {
+                st->giants[0][0] = src;
+                st->giant_n[0] = 1;
+            } else if (dv >= sa.hub_min) {
+                st->hubs[0][0] = src;
+                st->hub_n[0] = 1;
+            }
         }
         st->want_hubs = sa.want_hubs;
         st->hub_list_count = sa.hub_list_count;
@@ -1390,6 +1428,9 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
                 (long long)(g->E / kHubListShare)};
     w->host_flag[0] = 0;  // no run of this graph's engine is in flight: set again by the final launch
     ResetArgs r;
+    r.source = h_sources ? -1 : single;
+    r.step_grid = step_grid(w);
+    r.off = g->d_off;
     r.depth = depth;
     r.depth_len = depth_len >= 0 && depth_len < g->V ? depth_len : g->V;
     r.parent = parent;
@@ -1420,10 +1461,7 @@ static int seed_run(gi_graph *g, FusedWork *w, const int32_t *h_sources, int32_t
         bfs_seed_sources_kernel<<<sg < 1024 ? sg : 1024, 256, 0, s>>>(
             w->st, w->d_src, n_sources, 0, w->bm[0], w->flags + 3 * w->flag_bytes, g->d_visited, g->d_off, cnt2, fdg2,
             step_grid(w));
-    } else {
-        bfs_seed_sources_kernel<<<1, 64, 0, s>>>(w->st, nullptr, 1, single, w->bm[0], w->flags + 3 * w->flag_bytes,
-                                                 g->d_visited, g->d_off, cnt2, fdg2, step_grid(w));
-    }
+    }  // (a single source is seeded by the reset)
     GI_LAUNCH_CHECK();
     if (h_sources) GI_HIP(hipStreamSynchronize(s));  // h_sources may go away on return
     return GI_OK;
