@@ -556,8 +556,12 @@ int gi_cc(gi_graph *g, const gi_schedule *sched, int32_t *d_labels, gi_run_stats
         int32_t rounds = 0;
         const bool sampled = sched->load_balancing == GI_LB_VERTEX_BASED;
         if (g->layout) {  // degree-ordered view: labels come back as original minimum ids
-            GI_TRY(cc_fused_run(layout_view(g), layout_tmp(g, 0), &rounds, sampled, s));
-            GI_TRY(layout_labels_back(g, layout_tmp(g, 0), d_labels, s));
+            gi_graph *view = layout_view(g);
+            const std::function<int(hipStream_t)> back = [&](hipStream_t st) {
+                // the sampled run's giant label is the most frequent one
+                return layout_labels_back(g, layout_tmp(g, 0), d_labels, st, sampled ? cc_giant_label(view) : nullptr);
+            };
+            GI_TRY(cc_fused_run(view, layout_tmp(g, 0), &rounds, sampled, s, &back));
         } else {
             GI_TRY(cc_fused_run(g, d_labels, &rounds, sampled, s));
         }

@@ -33,6 +33,7 @@
 // The result is the component's minimum vertex id, as in the reference:
 // labels are vertex ids of the same component and only decrease, and the
 // minimum vertex never takes another label.
+#include <functional>
 #include <map>
 #include <tuple>
 #include <vector>
@@ -180,23 +181,23 @@ __global__ void __launch_bounds__(kCcBlock) cc_sample_kernel(CcArgs a, int32_t r
     }
 }
 
-// The most frequent label among 1024 spread-out vertices (one workgroup).
-__global__ void __launch_bounds__(kCcBlock) cc_giant_kernel(CcArgs a) {
-    constexpr int kSamples = 1024;
+// The most frequent label among 1024 spread-out vertices (one workgroup of
+// kCcGiantBlock threads, one sample each, counted against all the others
+// from LDS; the label only steers which vertices the filtered rounds skip,
+// any label gives the same result).
+constexpr int kCcGiantBlock = 1024;
+__global__ void __launch_bounds__(kCcGiantBlock) cc_giant_kernel(CcArgs a) {
+    constexpr int kSamples = kCcGiantBlock;
     __shared__ int32_t lab[kSamples];
-    __shared__ int32_t best_n[kCcBlock / 64], best_l[kCcBlock / 64];
-    for (int32_t i = threadIdx.x; i < kSamples; i += kCcBlock)
-        lab[i] = a.id[(int32_t)(((uint64_t)i * 2654435761ull + 12345ull) % (uint64_t)a.V)];
+    __shared__ int32_t best_n[kCcGiantBlock / 64], best_l[kCcGiantBlock / 64];
+    const int32_t i = threadIdx.x;
+    const int32_t mine = a.id[(int32_t)(((uint64_t)i * 2654435761ull + 12345ull) % (uint64_t)a.V)];
+    lab[i] = mine;
     __syncthreads();
-    int32_t bn = 0, bl = -1;
-    for (int32_t i = threadIdx.x; i < kSamples; i += kCcBlock) {
-        int32_t n = 0;
-        for (int32_t j = 0; j < kSamples; ++j) n += lab[j] == lab[i];
-        if (n > bn || (n == bn && lab[i] < bl)) {
-            bn = n;
-            bl = lab[i];
-        }
-    }
+    int32_t n = 0;
+#pragma unroll 16
+    for (int32_t j = 0; j < kSamples; ++j) n += lab[j] == mine;  // (the same j in every lane: a broadcast read)
+    int32_t bn = n, bl = mine;
     for (int o = 32; o > 0; o >>= 1) {
         const int32_t on = __shfl_xor(bn, o), ol = __shfl_xor(bl, o);
         if (on > bn || (on == bn && ol < bl)) {
@@ -210,7 +211,7 @@ __global__ void __launch_bounds__(kCcBlock) cc_giant_kernel(CcArgs a) {
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        for (int w = 1; w < kCcBlock / 64; ++w)
+        for (int w = 1; w < kCcGiantBlock / 64; ++w)
             if (best_n[w] > bn || (best_n[w] == bn && best_l[w] < bl)) {
                 bn = best_n[w];
                 bl = best_l[w];
@@ -253,16 +254,34 @@ __global__ void __launch_bounds__(kCcBlock) cc_rest_kernel(CcArgs a) {
     if (__ballot(changed) && lane == 0 && !a.st->changed) a.st->changed = 1;
 }
 
+// Four consecutive vertices per thread, their chains followed side by side
+// (a step's four loads in flight together).
 __global__ void cc_compress_kernel(CcArgs a) {
     if (a.st->done) return;
-    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < a.V; v += gridDim.x * blockDim.x) {
-        const int32_t y0 = a.id[v];
-        int32_t y = y0, x = a.id[y];
-        while (x != y) {  // labels strictly decrease along a chain, so it ends at a root
-            y = x;
-            x = a.id[y];
+    const int32_t stride = gridDim.x * blockDim.x;
+    for (int32_t v0 = 4 * (blockIdx.x * blockDim.x + threadIdx.x); v0 < a.V; v0 += 4 * stride) {
+        int32_t y0[4], y[4], x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) y0[k] = v0 + k < a.V ? a.id[v0 + k] : 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            y[k] = y0[k];
+            x[k] = a.id[y[k]];
         }
-        if (y != y0) a.id[v] = y;
+        // labels strictly decrease along a chain, so each ends at a root
+        bool more = x[0] != y[0] || x[1] != y[1] || x[2] != y[2] || x[3] != y[3];
+        while (more) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (x[k] != y[k]) {
+                    y[k] = x[k];
+                    x[k] = a.id[y[k]];
+                }
+            more = x[0] != y[0] || x[1] != y[1] || x[2] != y[2] || x[3] != y[3];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (v0 + k < a.V && y[k] != y0[k]) a.id[v0 + k] = y[k];
     }
 }
 
@@ -279,7 +298,8 @@ struct CcPlan {
     int32_t *chunk_v = nullptr;
     CcState *st = nullptr;
     int32_t *host = nullptr;  // pinned copy of the state
-    std::map<std::tuple<int32_t, int32_t *, bool>, hipGraphExec_t> execs;  // (rounds, labels, sampled)
+    // (rounds, labels, sampled, with the run's first launches)
+    std::map<std::tuple<int32_t, int32_t *, bool, bool>, hipGraphExec_t> execs;
     int32_t last_rounds = 0;
 };
 
@@ -295,6 +315,11 @@ static void free_cc(CcPlan *p) {
 
 void release_cc(const gi_graph *g) {
     if (CcPlan *p = g_cc.take(g)) free_cc(p);
+}
+
+const int32_t *cc_giant_label(gi_graph *g) {
+    CcPlan *p = g_cc.find(g);
+    return p ? &p->st->giant : nullptr;
 }
 
 static int ensure_cc(gi_graph *g, CcPlan **out) {
@@ -339,10 +364,17 @@ static void launch_round(const CcArgs &a, bool sampled, Launcher &L) {
     L(cc_control_kernel, 1, 1, a.st);
 }
 
-int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, bool sampled, hipStream_t s) {
+// The run is one recorded graph: the label init, for the sampled schedule
+// its two sampling passes and the giant label, then a block of rounds sized
+// by the last run (rounds after the end return at once); the state copy and
+// the epilogue (the caller's, e.g. the labels' mapping back to original ids)
+// are queued behind it, and the host waits once. A run longer than the block
+// replays further blocks of rounds, each followed by the epilogue again.
+int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, bool sampled, hipStream_t s,
+                 const std::function<int(hipStream_t)> *epilogue) {
     if (g->V == 0) {
         if (rounds_out) *rounds_out = 0;
-        return GI_OK;
+        return epilogue ? (*epilogue)(s) : GI_OK;
     }
     if (g->directed) return fail(GI_ERR_UNSUPPORTED, "fused CC needs a symmetric graph");
     CcPlan *p = nullptr;
@@ -356,20 +388,12 @@ int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, bool sampl
     a.V = g->V;
     a.E = g->E;
     a.chunks = p->chunks;
-    int ig = (int)ceil_div(g->V, 256);
-    cc_init_kernel<<<ig < 8192 ? ig : 8192, 256, 0, s>>>(d_labels, g->V, p->st);
-    if (sampled) {  // two sampling passes, each followed by full shortcutting
-        for (int32_t r = 0; r < 2; ++r) {
-            cc_sample_kernel<<<vertex_grid(g->V), kCcBlock, 0, s>>>(a, r);
-            cc_compress_kernel<<<ig < 8192 ? ig : 8192, 256, 0, s>>>(a);
-        }
-        cc_giant_kernel<<<1, kCcBlock, 0, s>>>(a);
-    }
-    GI_LAUNCH_CHECK();
+    const int ig = (int)ceil_div(g->V, 256);
     int32_t block = p->last_rounds > 0 ? p->last_rounds : 4;
+    bool first = true;
     for (int32_t done_rounds = 0; done_rounds < (1 << 20);) {
         if (block > 32) block = 32;
-        auto key = std::make_tuple(block, d_labels, sampled);
+        auto key = std::make_tuple(block, d_labels, sampled, first);
         auto it = p->execs.find(key);
         if (it == p->execs.end()) {
             if (p->execs.size() >= 8) {
@@ -379,6 +403,16 @@ int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, bool sampl
             hipGraphExec_t exec = nullptr;
             GI_TRY(build_graph(
                 [&](Launcher &L) {
+                    if (first) {
+                        L(cc_init_kernel, ig < 8192 ? ig : 8192, 256, d_labels, g->V, p->st);
+                        if (sampled) {  // two sampling passes, each followed by full shortcutting
+                            for (int32_t r = 0; r < 2; ++r) {
+                                L(cc_sample_kernel, vertex_grid(g->V), kCcBlock, a, r);
+                                L(cc_compress_kernel, ig < 8192 ? ig : 8192, 256, a);
+                            }
+                            L(cc_giant_kernel, 1, kCcGiantBlock, a);
+                        }
+                    }
                     for (int32_t r = 0; r < block; ++r) launch_round(a, sampled, L);
                 },
                 &exec));
@@ -386,7 +420,9 @@ int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds_out, bool sampl
         }
         GI_HIP(hipGraphLaunch(it->second, s));
         GI_HIP(hipMemcpyAsync(p->host, p->st, sizeof(CcState), hipMemcpyDeviceToHost, s));
+        if (epilogue) GI_TRY((*epilogue)(s));
         GI_HIP(hipStreamSynchronize(s));
+        first = false;
         done_rounds += block;
         if (p->host[1]) {  // done
             p->last_rounds = p->host[2];

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -411,7 +412,10 @@ int pr_fused_run(gi_graph *g, int32_t iters, float damping, float *d_rank, hipSt
 void release_pr(const gi_graph *g);
 // sampled: the VERTEX_BASED schedule (neighbour sampling, then rounds over
 // the vertices outside the most frequent label); else full EDGE_ONLY sweeps.
-int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds, bool sampled, hipStream_t s);
+// epilogue (optional): queued after each replay of rounds, before the host
+// reads whether the run has ended.
+int cc_fused_run(gi_graph *g, int32_t *d_labels, int32_t *rounds, bool sampled, hipStream_t s,
+                 const std::function<int(hipStream_t)> *epilogue = nullptr);
 void release_cc(const gi_graph *g);
 // counts (optional): relax steps, pull steps, windows entered, recorded slots.
 int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,
@@ -433,5 +437,10 @@ int layout_map_back(const gi_graph *g, const int32_t *in_new, int32_t *out_old, 
 // graph the edgeless tail of the view is filled without reading depth_new.
 int layout_depth_back(const gi_graph *g, const int32_t *depth_new, int32_t source_old, int32_t *out_old,
                       hipStream_t s);
-int layout_labels_back(const gi_graph *g, const int32_t *labels_new, int32_t *out_old, hipStream_t s);
+// hint (optional, device): the label most vertices carry (e.g. the giant
+// component's), folded with one atomic per workgroup.
+int layout_labels_back(const gi_graph *g, const int32_t *labels_new, int32_t *out_old, hipStream_t s,
+                       const int32_t *hint = nullptr);
+// The device address of the giant label the sampled CC run on g found (-1 there: none).
+const int32_t *cc_giant_label(gi_graph *g);
 }  // namespace gi

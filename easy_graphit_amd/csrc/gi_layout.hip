@@ -123,33 +123,95 @@ __global__ void layout_fill_kernel(int32_t *a, int32_t V, int32_t x) {
 }
 
 // Component minimum of the original ids: labels_new are vertex ids of the
-// view; lanes of a wave that share a label (the common case: one giant
-// component) combine before their atomic.
-__global__ void layout_comp_min_kernel(const int32_t *labels_new, const int32_t *old_of_new, int32_t V,
-                                       int32_t *comp_min) {
-    const int32_t lane = lane_id();
-    for (int32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63; base < V;
-         base += gridDim.x * blockDim.x) {
-        const int32_t i = base + lane;
-        const bool ok = i < V;
-        const int32_t r = ok ? labels_new[i] : -1;
-        const int32_t o = ok ? old_of_new[i] : INT_MAX;
-        const int32_t r0 = __shfl(r, 0);
-        const bool same = ok && r == r0;
-        int32_t m = same ? o : INT_MAX;
-        for (int k = 32; k > 0; k >>= 1) {
-            const int32_t y = __shfl_xor(m, k);
-            m = y < m ? y : m;
-        }
-        if (lane == 0 && m < INT_MAX && m < comp_min[r0]) atomicMin(&comp_min[r0], m);
-        if (ok && !same && o < comp_min[r]) atomicMin(&comp_min[r], o);
+// view. The vertices of the label `hint` (the giant component's, when the
+// caller knows it; -1: none) fold into a per-thread minimum, reduced over
+// the workgroup, one atomic per workgroup; other labels take one atomic per
+// vertex, unless the current minimum is already lower.
+__global__ void __launch_bounds__(256) layout_comp_min_kernel(const int32_t *labels_new, const int32_t *old_of_new,
+                                                              int32_t V, const int32_t *hint, int32_t *comp_min) {
+    __shared__ int32_t red[256 / 64];
+    const int32_t h = hint ? *hint : -1;
+    int32_t hm = INT_MAX;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        const int32_t r = labels_new[i], o = old_of_new[i];
+        if (r == h) hm = o < hm ? o : hm;
+        else if (o < comp_min[r]) atomicMin(&comp_min[r], o);
+    }
+    for (int k = 32; k > 0; k >>= 1) {
+        const int32_t y = __shfl_xor(hm, k);
+        hm = y < hm ? y : hm;
+    }
+    if (lane_id() == 0) red[threadIdx.x >> 6] = hm;
+    __syncthreads();
+    if (threadIdx.x == 0 && h >= 0) {
+        for (int w = 1; w < 256 / 64; ++w) hm = red[w] < hm ? red[w] : hm;
+        if (hm < INT_MAX) atomicMin(&comp_min[h], hm);
     }
 }
 
+// The most frequent label among 1024 spread-out vertices of [0, n) (one
+// workgroup, one sample per thread counted against the others from LDS):
+// the hint of layout_comp_min_kernel when the caller has none.
+__global__ void __launch_bounds__(1024) layout_mode_kernel(const int32_t *labels, int32_t n, int32_t *out) {
+    __shared__ int32_t lab[1024];
+    __shared__ int32_t bn_s[1024 / 64], bl_s[1024 / 64];
+    const int32_t mine = labels[(int32_t)(((uint64_t)threadIdx.x * 2654435761ull + 12345ull) % (uint64_t)n)];
+    lab[threadIdx.x] = mine;
+    __syncthreads();
+    int32_t bn = 0;
+#pragma unroll 16
+    for (int32_t j = 0; j < 1024; ++j) bn += lab[j] == mine;
+    int32_t bl = mine;
+    for (int o = 32; o > 0; o >>= 1) {
+        const int32_t on = __shfl_xor(bn, o), ol = __shfl_xor(bl, o);
+        if (on > bn || (on == bn && ol < bl)) {
+            bn = on;
+            bl = ol;
+        }
+    }
+    if (lane_id() == 0) {
+        bn_s[threadIdx.x >> 6] = bn;
+        bl_s[threadIdx.x >> 6] = bl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 1024 / 64; ++w)
+            if (bn_s[w] > bn || (bn_s[w] == bn && bl_s[w] < bl)) {
+                bn = bn_s[w];
+                bl = bl_s[w];
+            }
+        *out = bl;
+    }
+}
+
+// Labels in original ids: the component minimum of the vertex's label. A
+// vertex past iso_begin (of a symmetric graph: no edges) is a component of
+// its own, labelled with its own id, with no gather (iso_begin = V: none).
+// Four original ids per thread (16-byte index load and store, the gathers
+// in flight together) when out_old is 16-byte aligned.
 __global__ void layout_labels_back_kernel(const int32_t *labels_new, const int32_t *comp_min,
-                                          const int32_t *new_of_old, int32_t V, int32_t *out_old) {
-    for (int32_t o = blockIdx.x * blockDim.x + threadIdx.x; o < V; o += gridDim.x * blockDim.x)
-        out_old[o] = comp_min[labels_new[new_of_old[o]]];
+                                          const int32_t *new_of_old, int32_t V, int32_t iso_begin,
+                                          int32_t *out_old) {
+    const int32_t stride = gridDim.x * blockDim.x;
+    const int32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool vec = (reinterpret_cast<uintptr_t>(out_old) & 15) == 0;  // (new_of_old: 256-byte aligned)
+    const int32_t n4 = vec ? V / 4 : 0;
+    for (int32_t q = tid; q < n4; q += stride) {
+        const int4 nw = reinterpret_cast<const int4 *>(new_of_old)[q];
+        const int32_t o = 4 * q;
+        const int32_t l0 = nw.x < iso_begin ? labels_new[nw.x] : -1, l1 = nw.y < iso_begin ? labels_new[nw.y] : -1;
+        const int32_t l2 = nw.z < iso_begin ? labels_new[nw.z] : -1, l3 = nw.w < iso_begin ? labels_new[nw.w] : -1;
+        int4 r;
+        r.x = l0 >= 0 ? comp_min[l0] : o;
+        r.y = l1 >= 0 ? comp_min[l1] : o + 1;
+        r.z = l2 >= 0 ? comp_min[l2] : o + 2;
+        r.w = l3 >= 0 ? comp_min[l3] : o + 3;
+        reinterpret_cast<int4 *>(out_old)[q] = r;
+    }
+    for (int32_t o = 4 * n4 + tid; o < V; o += stride) {
+        const int32_t n = new_of_old[o];
+        out_old[o] = n < iso_begin ? comp_min[labels_new[n]] : o;
+    }
 }
 
 static int grid_of(int64_t n) {
@@ -329,12 +391,26 @@ int layout_depth_back(const gi_graph *g, const int32_t *depth_new, int32_t sourc
     return GI_OK;
 }
 
-int layout_labels_back(const gi_graph *g, const int32_t *labels_new, int32_t *out_old, hipStream_t s) {
+int layout_labels_back(const gi_graph *g, const int32_t *labels_new, int32_t *out_old, hipStream_t s,
+                       const int32_t *hint) {
     const GraphLayout *L = g->layout;
     int32_t *comp_min = L->tmp[1];
-    layout_fill_kernel<<<grid_of(g->V), 256, 0, s>>>(comp_min, g->V, INT_MAX);
-    layout_comp_min_kernel<<<grid_of(g->V), 256, 0, s>>>(labels_new, L->old_of_new, g->V, comp_min);
-    layout_labels_back_kernel<<<grid_of(g->V), 256, 0, s>>>(labels_new, comp_min, L->new_of_old, g->V, out_old);
+    // on a symmetric graph the edgeless tail of the view is singletons: their
+    // labels need no minimum (one atomic each would be millions of
+    // device-scope atomics on RMAT); the components of the other vertices
+    // lie below iso_begin, and so do their labels
+    const int32_t n = g->directed ? g->V : L->iso_begin;
+    if (n > 0) {
+        if (!hint) {  // the most frequent label, from samples (in comp_min's spare last slot)
+            layout_mode_kernel<<<1, 1024, 0, s>>>(labels_new, n, comp_min + g->V);
+            hint = comp_min + g->V;
+        }
+        layout_fill_kernel<<<grid_of(n), 256, 0, s>>>(comp_min, n, INT_MAX);
+        const int cg = grid_of(n) < 2048 ? grid_of(n) : 2048;  // (one atomic per workgroup on the hint's label)
+        layout_comp_min_kernel<<<cg, 256, 0, s>>>(labels_new, L->old_of_new, n, hint, comp_min);
+    }
+    layout_labels_back_kernel<<<grid_of(ceil_div(g->V, 4)), 256, 0, s>>>(labels_new, comp_min, L->new_of_old, g->V, n,
+                                                                         out_old);
     GI_LAUNCH_CHECK();
     return GI_OK;
 }
