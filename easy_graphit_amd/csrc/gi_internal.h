@@ -421,6 +421,15 @@ void release_cc(const gi_graph *g);
 int sssp_fused_run(gi_graph *g, int32_t source, int32_t delta, int32_t *d_dist, int32_t *counts,
                    hipStream_t s, int32_t max_cta = 0);
 void release_sssp(const gi_graph *g);
+// The light/heavy split of g's out-lists for delta, shared with the fused
+// SSSP engine (built on first use of a delta): each list reordered light
+// edges (w < delta) first, lend[v] the end of v's light part. All null when
+// no edge is heavy. The arrays stay valid until the split is rebuilt for
+// another delta or the graph is released.
+struct SsSplitView {
+    const int32_t *nbr = nullptr, *wt = nullptr, *lend = nullptr;
+};
+int sssp_light_split(gi_graph *g, int32_t delta, SsSplitView *out, hipStream_t s);
 
 // Degree-ordered layout (gi_layout.hip): the view the fused engines run on,
 // source mapping, two V-sized scratch arrays in view order, and the

@@ -1336,6 +1336,19 @@ static int ensure_split(gi_graph *g, SsWork *w, int32_t delta, hipStream_t s) {
     return GI_OK;
 }
 
+int sssp_light_split(gi_graph *g, int32_t delta, SsSplitView *out, hipStream_t s) {
+    SsWork *w = nullptr;
+    GI_TRY(ensure_ss(g, &w));
+    GI_TRY(ensure_split(g, w, delta, s));
+    *out = SsSplitView{};
+    if (w->split) {
+        out->nbr = w->snbr;
+        out->wt = w->swt;
+        out->lend = w->lend;
+    }
+    return GI_OK;
+}
+
 static int ss_reset(SsWork *w, const SsArgs &a, int32_t source, int32_t delta, int32_t pull_min, hipStream_t s) {
     w->host_flag[0] = 0;  // no run of this graph's engine is in flight: set again by the last step
     const int64_t n = std::max<int64_t>({(int64_t)a.V, (int64_t)kSsFlagSets * a.flag_bytes / 8,
