@@ -12,24 +12,28 @@
 //   W[v]      the window's members (bit b: v was in source b's frontier in
 //             this window), two sets, one per window parity
 //   P[v]      pending mask (bit b: v improved to a distance beyond the window)
-//   Q, QW     the out-lists of F's and W's vertices as items of at most 256
-//             edges: the light part (w < delta) for F, the heavy part for W
 //
 // All sources share one window [lo, lo + delta). Light steps relax F's light
-// edges until no source's F' gains a member; then one heavy step relaxes the
-// heavy edges of W once, from the settled distances (a heavy edge from the
-// window lands at or beyond hi, so it can only add pending members). The
-// next window starts at the least pending distance over all sources (the
-// reference's queue starts a window at the least remaining priority too,
-// gpu_priority_queue.h:126-146) and the pending members inside it move to F.
-// A source with no member in a window idles through it. A step pushes (a
-// wave per item: writeMin of D[v][b] for the sources in the item's mask) or,
-// when its lists hold more than an eighth of the edges of their kind, pulls
-// (a wave per 64 vertices: each vertex takes the minimum over its
-// in-neighbours u of D[u][b] + w for the b in u's mask and stores it; no
-// atomics on D). Shortest-path distances are unique, so neither the order of
-// relaxations nor the window schedule changes them. A step is four launches
-// recorded in hipGraph blocks; the host reads the done flag once per block.
+// edges (w < delta) until no source's F' gains a member; then one heavy step
+// relaxes the heavy edges of W once, from the settled distances (a heavy edge
+// from the window lands at or beyond hi, so it can only add pending
+// members). The next window starts at the least pending distance over all
+// sources (the reference's queue starts a window at the least remaining
+// priority too, gpu_priority_queue.h:126-146) and the pending members inside
+// it move to F. A source with no member in a window idles through it.
+//
+// A step pushes or, when its vertices hold more than an eighth of the edges
+// of their kind, pulls. A push sweeps the mask array 64 vertices per wave
+// and relaxes the edges of the vertices found (writeMin of D[v][b] for the
+// sources in the mask); edge lists longer than 256 are listed as segments
+// that any wave takes. A pull gives a wave 16 consecutive vertices, each
+// taking the minimum over its in-neighbours u of D[u][b] + w for the b in
+// u's mask, and stores it (no atomics on D). No vertex lists are kept:
+// counts of the joining vertices and their edges go to per-workgroup sums
+// that the one-workgroup decision reads. Shortest-path distances are unique,
+// so neither the order of relaxations nor the window schedule changes them.
+// A step is five launches recorded in hipGraph blocks; the host reads the
+// done flag once per block.
 #include <climits>
 #include <vector>
 
@@ -39,14 +43,27 @@ namespace gi {
 
 constexpr int kVbLanes = 64;         // sources per pass
 constexpr int kVbBlock = 256;
-constexpr int kVbSeg = 256;          // out-edges per list item
+constexpr int kVbWaves = kVbBlock / 64;
+constexpr int kVbSeg = 256;          // edge lists longer than this are listed as segments of this many edges
 constexpr int kVbPullWide = 1024;    // in-lists this long are pulled in chunks
 constexpr int kVbPullChunk = 1024;   // in-edges per chunk item
 constexpr int kVbBlockSteps = 16;
-constexpr int kVbPullDiv = 8;        // a step pulls when its lists hold more than 1 / kVbPullDiv of the edges
+constexpr int kVbPullDiv = 8;        // a step pulls when its vertices hold more than 1 / kVbPullDiv of the edges
 constexpr int kVbBatch = 4;          // rows in flight per wave
+constexpr int kVbGroup = 16;         // vertices per wave in a pull
 
 enum VbMode : int32_t { VB_LIGHT = 0, VB_HEAVY = 1 };
+
+// What a relax or move launch's workgroup adds up for the next decision:
+// vertices that joined F (or F') and their light out-edges, vertices that
+// joined W and their heavy out-edges.
+struct VbPart {
+    long long fn, fe, wn, we;
+};
+
+struct alignas(128) VbCount {  // on a line of its own
+    int32_t n;
+};
 
 struct VbState {
     int32_t done;
@@ -58,18 +75,10 @@ struct VbState {
     int32_t steps, pulls, windows;
     int32_t wi;       // window index: W = w[wi & 1]
     int32_t wclr;     // the coming relax step clears the other W set
-    int32_t qn[3];    // F items per set
-    int32_t wn[2];    // W items per set
     int32_t pad;
-    long long qe[3];  // edges of the F items per set
-    long long we[2];  // edges of the W items per set
-};
-
-// An item list and its counters.
-struct VbList {
-    int4 *q;  // (vertex, first edge, end edge)
-    int32_t *n;
-    long long *e;
+    long long fe;     // light out-edges of the F the coming relax step reads
+    long long wn, we; // this window's W: vertices, heavy out-edges
+    VbCount seg[5];   // segments listed for the F sets (0-2) and the W sets (3-4)
 };
 
 struct VbArgs {
@@ -79,14 +88,15 @@ struct VbArgs {
     const int32_t *poff, *pnbr, *pwt;  // in-lists
     const int32_t *plend;              // end of each light in-part (null: select by weight)
     int32_t V;
-    int32_t qcap;
-    long long pull_light, pull_heavy;  // lists with more edges than these pull
+    int32_t scap;                      // segment list capacity
+    long long pull_light, pull_heavy;  // vertices with more edges than these pull
     int32_t *D;  // [V][64]
     uint64_t *f[3];
     uint64_t *w[2];
     uint64_t *p;
-    int4 *q[3];
-    int4 *qw[2];
+    int4 *seg[5];   // (vertex, first edge, end edge) of the long lists of the F sets and the W sets
+    VbPart *part;   // [grid]
+    int32_t grid;
     const int4 *wide;  // chunks of the long in-lists (vertex, first edge, end edge)
     int32_t nwide;
 };
@@ -99,8 +109,6 @@ __device__ __forceinline__ uint64_t vb_readlane64(uint64_t x, int32_t j) {
     const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int32_t)(uint32_t)(x >> 32), j);
     return ((uint64_t)hi << 32) | lo;
 }
-__device__ __forceinline__ VbList vb_flist(const VbArgs &a, int32_t k) { return VbList{a.q[k], &a.st->qn[k], &a.st->qe[k]}; }
-__device__ __forceinline__ VbList vb_wlist(const VbArgs &a, int32_t k) { return VbList{a.qw[k], &a.st->wn[k], &a.st->we[k]}; }
 
 // The light or heavy part of v's out-list.
 __device__ __forceinline__ void vb_part(const VbArgs &a, int32_t v, bool heavy, int32_t &e0, int32_t &e1) {
@@ -110,50 +118,41 @@ __device__ __forceinline__ void vb_part(const VbArgs &a, int32_t v, bool heavy, 
     e1 = heavy ? o1 : l;
 }
 
-// v joined the mask set of list L: its part of the out-list joins L as items
-// of at most kVbSeg edges (one empty item for an empty part, so that the
-// clearing pass finds every vertex with a mask). One lane calls.
-__device__ void vb_enqueue(const VbArgs &a, const VbList &L, bool heavy, int32_t v) {
+// A lane's running counts of the vertices that joined F' / W and their edges.
+struct VbAcc {
+    long long fn = 0, fe = 0, wn = 0, we = 0;
+};
+
+// The part [e0, e1) of v's list joins segment list k (0-2: F sets, 3-4: W
+// sets) when it is longer than kVbSeg.
+__device__ void vb_list_segments(const VbArgs &a, int32_t k, int32_t v, int32_t e0, int32_t e1) {
+    const int32_t n = (e1 - e0 + kVbSeg - 1) / kVbSeg;
+    const int32_t p = atomicAdd(&a.st->seg[k].n, n);
+    for (int32_t j = 0; j < n && p + j < a.scap; ++j) {
+        const int32_t s0 = e0 + j * kVbSeg;
+        a.seg[k][p + j] = make_int4(v, s0, s0 + kVbSeg < e1 ? s0 + kVbSeg : e1, 0);
+    }
+}
+
+// Lane call: v joined F set k (heavy false) or W set k (heavy true).
+__device__ __forceinline__ void vb_note(const VbArgs &a, int32_t k, bool heavy, int32_t v, VbAcc &acc) {
     int32_t e0, e1;
     vb_part(a, v, heavy, e0, e1);
-    int32_t n = (e1 - e0 + kVbSeg - 1) / kVbSeg;
-    if (n == 0) n = 1;
-    const int32_t p = atomicAdd(L.n, n);
-    for (int32_t j = 0; j < n && p + j < a.qcap; ++j) {
-        const int32_t s0 = e0 + j * kVbSeg;
-        L.q[p + j] = make_int4(v, s0, s0 + kVbSeg < e1 ? s0 + kVbSeg : e1, 0);
+    if (heavy) {
+        acc.wn += 1;
+        acc.we += e1 - e0;
+    } else {
+        acc.fn += 1;
+        acc.fe += e1 - e0;
     }
-    if (e1 > e0) atomicAdd((unsigned long long *)L.e, (unsigned long long)(e1 - e0));
+    if (e1 - e0 > kVbSeg) vb_list_segments(a, heavy ? 3 + k : k, v, e0, e1);
 }
 
 // The same for the vertices held by the lanes in `mask` (lane j holds vj),
-// with one pair of counter atomics for the wave. The whole wave calls.
-__device__ void vb_enqueue_wave(const VbArgs &a, const VbList &L, bool heavy, uint64_t mask, int32_t vj) {
-    if (!mask) return;  // uniform
-    const int32_t lane = lane_id();
-    const bool on = (mask >> lane) & 1ull;
-    int32_t e0 = 0, e1 = 0;
-    if (on) vb_part(a, vj, heavy, e0, e1);
-    int32_t n = on ? (e1 - e0 + kVbSeg - 1) / kVbSeg : 0;
-    if (on && n == 0) n = 1;
-    int32_t incl = n;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int32_t y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-    }
-    long long deg = e1 - e0;
-    for (int o = 32; o > 0; o >>= 1) deg += __shfl_xor(deg, o);
-    const int32_t total = __builtin_amdgcn_readlane(incl, 63);  // read with the whole wave active
-    int32_t base = 0;
-    if (lane == 0) {
-        base = atomicAdd(L.n, total);
-        if (deg) atomicAdd((unsigned long long *)L.e, (unsigned long long)deg);
-    }
-    base = __shfl(base, 0);
-    for (int32_t j = 0, p = base + incl - n; j < n && p + j < a.qcap; ++j) {
-        const int32_t s0 = e0 + j * kVbSeg;
-        L.q[p + j] = make_int4(vj, s0, s0 + kVbSeg < e1 ? s0 + kVbSeg : e1, 0);
-    }
+// their lists read in parallel. The whole wave calls.
+__device__ __forceinline__ void vb_note_wave(const VbArgs &a, int32_t k, bool heavy, uint64_t mask, int32_t vj,
+                                             VbAcc &acc) {
+    if ((mask >> lane_id()) & 1ull) vb_note(a, k, heavy, vj, acc);
 }
 
 // Where the improvements of a relax step go.
@@ -163,85 +162,114 @@ struct VbOut {
     int32_t hi;
 };
 
-// Improvements of v found by a writeMin (one lane calls): below hi they join
-// F' and W, beyond it P.
-__device__ void vb_mark_atomic(const VbArgs &a, const VbOut &o, int32_t v, uint64_t near, uint64_t far) {
+// Improvements of v found by a writeMin (lane 0 calls): below hi they join
+// F' and W, beyond it P. Bit 0 of the result: v joined F' here; bit 1: W.
+__device__ int32_t vb_mark_atomic(const VbArgs &a, const VbOut &o, int32_t v, uint64_t near, uint64_t far) {
+    int32_t r = 0;
     if (near) {
-        if (!atomicOr((unsigned long long *)&a.f[o.nxt][v], near)) vb_enqueue(a, vb_flist(a, o.nxt), false, v);
-        if (a.lend && (a.w[o.cw][v] & near) != near &&
-            !atomicOr((unsigned long long *)&a.w[o.cw][v], near))
-            vb_enqueue(a, vb_wlist(a, o.cw), true, v);
+        if (!atomicOr((unsigned long long *)&a.f[o.nxt][v], near)) r |= 1;
+        if (a.lend && (a.w[o.cw][v] & near) != near && !atomicOr((unsigned long long *)&a.w[o.cw][v], near)) r |= 2;
     }
     if (far && (a.p[v] & far) != far) atomicOr((unsigned long long *)&a.p[v], far);
+    return r;
 }
 
-// Push over the items of list L with masks mk: a wave per item, lane b
-// relaxes for source b when b is in mk[u]; kVbBatch target rows are read and
-// written at a time. The vertices that join F' or W are listed 64 at a time.
-__device__ void vb_push(const VbArgs &a, const VbList &L, const uint64_t *mk, const VbOut &o) {
-    const int32_t n = *L.n < a.qcap ? *L.n : a.qcap;
+// Relaxes the edges held by lanes [0, cnt) (lane j: edge uj -> vj of weight
+// wj, sources mj), kVbBatch at a time: lane b reads D[u][b] and D[v][b] for
+// the sources b in the mask and writeMins D[v][b].
+__device__ void vb_relax_lanes(const VbArgs &a, const VbOut &o, int32_t uj, int32_t vj, int32_t wj, uint64_t mj,
+                               int32_t cnt, VbAcc &acc) {
     const int32_t lane = lane_id();
-    int32_t fj = 0, nf = 0, wj = 0, nw = 0;  // pending listings (lane j holds the j-th)
-    const VbList FL = vb_flist(a, o.nxt), WL = vb_wlist(a, o.cw);
-    for (int32_t i = vb_wave(); i < n; i += vb_waves()) {
-        const int4 it = L.q[i];
-        const int32_t u = it.x;
-        const uint64_t m = mk[u];
-        const bool mine = (m >> lane) & 1ull;
-        const int32_t du = mine ? vb_row(a, u)[lane] : 0;
-        for (int32_t e0 = it.y; e0 < it.z; e0 += 64) {
-            const int32_t ne = it.z - e0 < 64 ? it.z - e0 : 64;
-            const int32_t vl = lane < ne ? a.nbr[e0 + lane] : 0;
-            const int32_t wl = lane < ne ? a.wt[e0 + lane] : 0;
-            for (int32_t j = 0; j < ne; j += kVbBatch) {
-                int32_t v[kVbBatch], nd[kVbBatch], dv[kVbBatch], old[kVbBatch];
+    int32_t fv = 0, wv = 0, nf = 0, nw = 0;  // vertices that joined F' / W (lane i holds the i-th): at most cnt
+    for (int32_t j = 0; j < cnt; j += kVbBatch) {
+        int32_t v[kVbBatch], nd[kVbBatch], dv[kVbBatch], old[kVbBatch];
+        bool on[kVbBatch];
 #pragma unroll
-                for (int k = 0; k < kVbBatch; ++k) {
-                    const int32_t jj = j + k < ne ? j + k : ne - 1;
-                    v[k] = j + k < ne ? __builtin_amdgcn_readlane(vl, jj) : -1;
-                    nd[k] = du + __builtin_amdgcn_readlane(wl, jj);
-                }
+        for (int k = 0; k < kVbBatch; ++k) {
+            const int32_t jj = j + k < cnt ? j + k : cnt - 1;
+            const int32_t u = __builtin_amdgcn_readlane(uj, jj);
+            const uint64_t m = vb_readlane64(mj, jj);
+            v[k] = j + k < cnt ? __builtin_amdgcn_readlane(vj, jj) : -1;
+            on[k] = v[k] >= 0 && ((m >> lane) & 1ull);
+            nd[k] = on[k] ? vb_row(a, u)[lane] : 0;
+            dv[k] = on[k] ? vb_row(a, v[k])[lane] : INT_MIN;
+        }
 #pragma unroll
-                for (int k = 0; k < kVbBatch; ++k) dv[k] = mine && v[k] >= 0 ? vb_row(a, v[k])[lane] : INT_MIN;
+        for (int k = 0; k < kVbBatch; ++k) {
+            nd[k] += __builtin_amdgcn_readlane(wj, j + k < cnt ? j + k : cnt - 1);
+            old[k] = INT_MIN;
+            if (on[k] && nd[k] < dv[k]) old[k] = atomicMin(&vb_row(a, v[k])[lane], nd[k]);
+        }
 #pragma unroll
-                for (int k = 0; k < kVbBatch; ++k) {
-                    old[k] = INT_MIN;
-                    if (nd[k] < dv[k]) old[k] = atomicMin(&vb_row(a, v[k])[lane], nd[k]);
-                }
-#pragma unroll
-                for (int k = 0; k < kVbBatch; ++k) {
-                    const bool imp = nd[k] < old[k];
-                    const uint64_t near = __ballot(imp && nd[k] < o.hi), far = __ballot(imp && nd[k] >= o.hi);
-                    if (!(near | far)) continue;  // uniform
-                    unsigned long long fo = 1, wo = 1;
-                    if (lane == 0) {
-                        if (near) {
-                            fo = atomicOr((unsigned long long *)&a.f[o.nxt][v[k]], near);
-                            if (a.lend && (a.w[o.cw][v[k]] & near) != near)
-                                wo = atomicOr((unsigned long long *)&a.w[o.cw][v[k]], near);
-                        }
-                        if (far && (a.p[v[k]] & far) != far) atomicOr((unsigned long long *)&a.p[v[k]], far);
-                    }
-                    if (!vb_readlane64(fo, 0)) {  // uniform: v joined F' here
-                        if (lane == nf) fj = v[k];
-                        if (++nf == 64) {
-                            vb_enqueue_wave(a, FL, false, ~0ull, fj);
-                            nf = 0;
-                        }
-                    }
-                    if (!vb_readlane64(wo, 0)) {  // uniform: v joined W here
-                        if (lane == nw) wj = v[k];
-                        if (++nw == 64) {
-                            vb_enqueue_wave(a, WL, true, ~0ull, wj);
-                            nw = 0;
-                        }
-                    }
-                }
+        for (int k = 0; k < kVbBatch; ++k) {
+            const bool imp = nd[k] < old[k];
+            const uint64_t near = __ballot(imp && nd[k] < o.hi), far = __ballot(imp && nd[k] >= o.hi);
+            if (!(near | far)) continue;  // uniform
+            int32_t r = 0;
+            if (lane == 0) r = vb_mark_atomic(a, o, v[k], near, far);
+            r = __builtin_amdgcn_readfirstlane(r);
+            if (r & 1) {
+                if (lane == nf) fv = v[k];
+                ++nf;
+            }
+            if (r & 2) {
+                if (lane == nw) wv = v[k];
+                ++nw;
             }
         }
     }
-    vb_enqueue_wave(a, FL, false, nf == 64 ? ~0ull : (1ull << nf) - 1, fj);
-    vb_enqueue_wave(a, WL, true, nw == 64 ? ~0ull : (1ull << nw) - 1, wj);
+    vb_note_wave(a, o.nxt, false, nf == 64 ? ~0ull : (1ull << nf) - 1, fv, acc);
+    vb_note_wave(a, o.cw, true, nw == 64 ? ~0ull : (1ull << nw) - 1, wv, acc);
+}
+
+// Push: sweep the masks mk 64 vertices per wave; the lists of the vertices
+// found (their light or heavy part, up to kVbSeg edges) are concatenated and
+// relaxed 64 edges at a time. Longer lists are segment list k's items.
+__device__ void vb_push(const VbArgs &a, const uint64_t *mk, bool heavy, int32_t k, const VbOut &o, VbAcc &acc) {
+    const int32_t lane = lane_id();
+    const int32_t groups = (a.V + 63) / 64;
+    for (int32_t gi = vb_wave(); gi < groups; gi += vb_waves()) {
+        const int32_t v = gi * 64 + lane;
+        const uint64_t mv = v < a.V ? mk[v] : 0ull;
+        if (!__ballot(mv != 0)) continue;  // uniform
+        int32_t e0 = 0, e1 = 0;
+        if (mv) vb_part(a, v, heavy, e0, e1);
+        const int32_t len = e1 - e0 <= kVbSeg ? e1 - e0 : 0;
+        int32_t incl = len;
+        for (int s = 1; s < 64; s <<= 1) {
+            const int32_t y = __shfl_up(incl, s);
+            if (lane >= s) incl += y;
+        }
+        const int32_t total = __builtin_amdgcn_readlane(incl, 63);
+        const int32_t pre = incl - len;
+        for (int32_t base = 0; base < total; base += 64) {
+            const int32_t g = base + lane;
+            int32_t ow = 0;  // owner: the last lane whose exclusive prefix is <= g
+            for (int32_t step = 32; step > 0; step >>= 1) {
+                const int32_t t = ow + step;
+                const int32_t pt = __shfl(pre, t & 63);  // every lane takes part
+                if (t < 64 && pt <= g) ow = t;
+            }
+            const int32_t e = __shfl(e0, ow) + (g - __shfl(pre, ow));
+            const bool has = g < total;
+            const int32_t vj = has ? a.nbr[e] : 0;
+            const int32_t wj = has ? a.wt[e] : 0;
+            const uint64_t mj = __shfl(mv, ow);
+            const int32_t cnt = total - base < 64 ? total - base : 64;
+            vb_relax_lanes(a, o, gi * 64 + ow, vj, wj, mj, cnt, acc);
+        }
+    }
+    const int32_t ns = a.st->seg[k].n < a.scap ? a.st->seg[k].n : a.scap;
+    for (int32_t i = vb_wave(); i < ns; i += vb_waves()) {
+        const int4 it = a.seg[k][i];
+        const uint64_t m = mk[it.x];
+        for (int32_t e = it.y; e < it.z; e += 64) {
+            const int32_t cnt = it.z - e < 64 ? it.z - e : 64;
+            const int32_t vj = lane < cnt ? a.nbr[e + lane] : 0;
+            const int32_t wj = lane < cnt ? a.wt[e + lane] : 0;
+            vb_relax_lanes(a, o, it.x, vj, wj, m, cnt, acc);
+        }
+    }
 }
 
 // The in-edges a step reads: the light or heavy part through plend, or the
@@ -291,43 +319,26 @@ __device__ __forceinline__ int32_t vb_pull_range(const VbArgs &a, const uint64_t
     return cand;
 }
 
-// The pull step's result for vertex v (owned by this wave; no other wave
-// writes its row or masks in a pull step): store the improved distances and
-// set its F', W and pending bits. Bit 0 of the result: v joined F'; bit 1: W.
-__device__ __forceinline__ int32_t vb_pull_store(const VbArgs &a, const VbOut &o, int32_t v, int32_t cand) {
-    const int32_t lane = lane_id();
-    int32_t *row = vb_row(a, v);
-    const bool imp = cand < row[lane];
-    if (imp) row[lane] = cand;
-    const uint64_t near = __ballot(imp && cand < o.hi), far = __ballot(imp && cand >= o.hi);
-    uint64_t w0 = 1;
-    if (lane == 0) {
-        if (near) {
-            a.f[o.nxt][v] = near;  // F' was clear
-            if (a.lend) {
-                w0 = a.w[o.cw][v];
-                a.w[o.cw][v] = w0 | near;
-            }
-        }
-        if (far) a.p[v] |= far;
-    }
-    return (near ? 1 : 0) | (near && !vb_readlane64(w0, 0) ? 2 : 0);
-}
-
-// Pull over every vertex: a wave takes 64 consecutive vertices, reads their
-// in-lists 64 edges at a time (lane j takes edge j of the concatenation),
-// and for each edge whose source u has a nonzero mask reads D[u] once for
-// all sources. In-lists of kVbPullWide edges or more are chunk items below.
-__device__ void vb_pull(const VbArgs &a, const uint64_t *mk, bool heavy, const VbOut &o) {
+// Pull over every vertex: a wave takes kVbGroup consecutive vertices, reads
+// their in-lists 64 edges at a time (lane j takes edge j of the
+// concatenation), and for each edge whose source u has a nonzero mask reads
+// D[u] once for all sources, folding D[u][b] + w into the wave's LDS table
+// of minima (row: vertex of the group, column: source; a lane only touches
+// its own column). Then the rows of the vertices that were reached are read
+// kVbBatch at a time and the improvements stored; each vertex is owned by
+// this wave, so its row, F' and W masks and pending bits take plain stores.
+// In-lists of kVbPullWide edges or more are chunk items below.
+__device__ void vb_pull(const VbArgs &a, const uint64_t *mk, bool heavy, const VbOut &o, int32_t (*cand)[64],
+                        VbAcc &acc) {
     const int32_t lane = lane_id();
     const int32_t delta = a.st->delta;
     const int32_t sel = a.plend || !a.lend ? 0 : (heavy ? 2 : 1);
-    const VbList FL = vb_flist(a, o.nxt), WL = vb_wlist(a, o.cw);
-    const int32_t groups = (a.V + 63) / 64;
+    const int32_t groups = (a.V + kVbGroup - 1) / kVbGroup;
     for (int32_t gi = vb_wave(); gi < groups; gi += vb_waves()) {
-        const int32_t v = gi * 64 + lane;
+        const int32_t v0 = gi * kVbGroup;
+        const int32_t v = v0 + lane;
         int32_t p0 = 0, p1 = 0;
-        if (v < a.V && a.poff[v + 1] - a.poff[v] < kVbPullWide) vb_in_part(a, v, heavy, p0, p1);
+        if (lane < kVbGroup && v < a.V && a.poff[v + 1] - a.poff[v] < kVbPullWide) vb_in_part(a, v, heavy, p0, p1);
         const int32_t len = p1 - p0;
         int32_t incl = len;
         for (int s = 1; s < 64; s <<= 1) {
@@ -336,8 +347,8 @@ __device__ void vb_pull(const VbArgs &a, const uint64_t *mk, bool heavy, const V
         }
         const int32_t total = __builtin_amdgcn_readlane(incl, 63);
         const int32_t pre = incl - len;
-        int32_t cv = -1, cand = INT_MAX;  // the vertex being gathered (lane index) and its minimum
-        uint64_t jf = 0, jw = 0;          // the group's vertices that joined F' and W
+        for (int32_t r = 0; r < kVbGroup; ++r) cand[r][lane] = INT_MAX;
+        uint64_t reached = 0;  // the group's vertices with an in-neighbour in the mask
         for (int32_t base = 0; base < total; base += 64) {
             const int32_t g = base + lane;
             int32_t ow = 0;  // owner: the last lane whose exclusive prefix is <= g
@@ -367,33 +378,52 @@ __device__ void vb_pull(const VbArgs &a, const uint64_t *mk, bool heavy, const V
                     const int32_t u = __builtin_amdgcn_readlane(uj, jj);
                     const uint64_t m = vb_readlane64(mj, jj);
                     w[k] = __builtin_amdgcn_readlane(wj, jj);
-                    own[k] = j[k] >= 0 ? __builtin_amdgcn_readlane(ow, jj) : -1;
+                    own[k] = __builtin_amdgcn_readlane(ow, jj);
                     on[k] = j[k] >= 0 && ((m >> lane) & 1ull);
                     du[k] = on[k] ? vb_row(a, u)[lane] : 0;
                 }
 #pragma unroll
                 for (int k = 0; k < kVbBatch; ++k) {
-                    if (own[k] < 0) continue;  // uniform
-                    if (own[k] != cv) {        // uniform: edges come in vertex order
-                        if (cv >= 0) {
-                            const int32_t r = vb_pull_store(a, o, gi * 64 + cv, cand);
-                            jf |= (uint64_t)(r & 1) << cv;
-                            jw |= (uint64_t)(r >> 1) << cv;
-                        }
-                        cv = own[k];
-                        cand = INT_MAX;
-                    }
-                    if (on[k] && du[k] + w[k] < cand) cand = du[k] + w[k];
+                    if (j[k] < 0) continue;  // uniform
+                    reached |= 1ull << own[k];
+                    if (on[k] && du[k] + w[k] < cand[own[k]][lane]) cand[own[k]][lane] = du[k] + w[k];
                 }
             }
         }
-        if (cv >= 0) {
-            const int32_t r = vb_pull_store(a, o, gi * 64 + cv, cand);
-            jf |= (uint64_t)(r & 1) << cv;
-            jw |= (uint64_t)(r >> 1) << cv;
+        uint64_t jf = 0, jw = 0;  // the group's vertices that joined F' and W
+        while (reached) {
+            int32_t r[kVbBatch], dv[kVbBatch];
+#pragma unroll
+            for (int k = 0; k < kVbBatch; ++k) {
+                r[k] = reached ? __ffsll((unsigned long long)reached) - 1 : -1;
+                reached &= reached - 1;
+                dv[k] = r[k] >= 0 ? vb_row(a, v0 + r[k])[lane] : INT_MIN;
+            }
+#pragma unroll
+            for (int k = 0; k < kVbBatch; ++k) {
+                if (r[k] < 0) continue;  // uniform
+                const int32_t c = cand[r[k]][lane];
+                const bool imp = c < dv[k];
+                const int32_t vv = v0 + r[k];
+                if (imp) vb_row(a, vv)[lane] = c;
+                const uint64_t near = __ballot(imp && c < o.hi), far = __ballot(imp && c >= o.hi);
+                uint64_t w0 = 1;
+                if (lane == 0) {
+                    if (near) {
+                        a.f[o.nxt][vv] = near;  // F' was clear
+                        if (a.lend) {
+                            w0 = a.w[o.cw][vv];
+                            a.w[o.cw][vv] = w0 | near;
+                        }
+                    }
+                    if (far) a.p[vv] |= far;
+                }
+                if (near) jf |= 1ull << r[k];
+                if (near && !vb_readlane64(w0, 0)) jw |= 1ull << r[k];
+            }
         }
-        vb_enqueue_wave(a, FL, false, jf, v);
-        vb_enqueue_wave(a, WL, true, jw, v);
+        vb_note_wave(a, o.nxt, false, jf, v, acc);
+        vb_note_wave(a, o.cw, true, jw, v, acc);
     }
     // chunks of the long in-lists: several waves share a vertex, so the
     // result joins D by writeMin and the masks by atomicOr
@@ -407,53 +437,119 @@ __device__ void vb_pull(const VbArgs &a, const uint64_t *mk, bool heavy, const V
         bool imp = false;
         if (cand != INT_MAX) imp = cand < atomicMin(&vb_row(a, c.x)[lane], cand);
         const uint64_t near = __ballot(imp && cand < o.hi), far = __ballot(imp && cand >= o.hi);
-        if (lane == 0 && (near | far)) vb_mark_atomic(a, o, c.x, near, far);
+        if (lane == 0 && (near | far)) {
+            const int32_t r = vb_mark_atomic(a, o, c.x, near, far);
+            if (r & 1) vb_note(a, o.nxt, false, c.x, acc);
+            if (r & 2) vb_note(a, o.cw, true, c.x, acc);
+        }
     }
 }
 
+// The workgroup's sums of its lanes' counts, stored as its row of a.part.
+// The whole workgroup calls.
+__device__ void vb_store_part(const VbArgs &a, VbAcc acc) {
+    __shared__ long long sum_s[kVbWaves][4];
+    for (int o = 32; o > 0; o >>= 1) {
+        acc.fn += __shfl_xor(acc.fn, o);
+        acc.fe += __shfl_xor(acc.fe, o);
+        acc.wn += __shfl_xor(acc.wn, o);
+        acc.we += __shfl_xor(acc.we, o);
+    }
+    const int32_t wid = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        sum_s[wid][0] = acc.fn;
+        sum_s[wid][1] = acc.fe;
+        sum_s[wid][2] = acc.wn;
+        sum_s[wid][3] = acc.we;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        long long t = 0;
+        for (int k = 0; k < kVbWaves; ++k) t += sum_s[k][threadIdx.x];
+        (&a.part[blockIdx.x].fn)[threadIdx.x] = t;
+    }
+}
+
+// The sums over the grid's rows of a.part (whole workgroup calls; thread 0
+// gets them).
+__device__ VbPart vb_sum_parts(const VbArgs &a) {
+    __shared__ long long sum_s[kVbBlock / 64][4];
+    VbPart t{0, 0, 0, 0};
+    for (int32_t i = threadIdx.x; i < a.grid; i += blockDim.x) {
+        const VbPart q = a.part[i];
+        t.fn += q.fn;
+        t.fe += q.fe;
+        t.wn += q.wn;
+        t.we += q.we;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        t.fn += __shfl_xor(t.fn, o);
+        t.fe += __shfl_xor(t.fe, o);
+        t.wn += __shfl_xor(t.wn, o);
+        t.we += __shfl_xor(t.we, o);
+    }
+    if (lane_id() == 0) {
+        sum_s[threadIdx.x >> 6][0] = t.fn;
+        sum_s[threadIdx.x >> 6][1] = t.fe;
+        sum_s[threadIdx.x >> 6][2] = t.wn;
+        sum_s[threadIdx.x >> 6][3] = t.we;
+    }
+    __syncthreads();
+    VbPart r{0, 0, 0, 0};
+    for (int k = 0; k < (int)(blockDim.x / 64); ++k) {
+        r.fn += sum_s[k][0];
+        r.fe += sum_s[k][1];
+        r.wn += sum_s[k][2];
+        r.we += sum_s[k][3];
+    }
+    return r;
+}
+
 // One relax step: clear the F set two steps back (and, at a window's first
-// step, the last window's W set; their vertices are their items), then push
-// or pull the light edges of F or the heavy edges of W. Every workgroup
-// reads the same counts, so all take the same branch.
+// step, the last window's W set), then push or pull the light edges of F or
+// the heavy edges of W. Every workgroup reads the same counts, so all take
+// the same branch.
 __global__ void __launch_bounds__(kVbBlock) vb_relax_kernel(VbArgs a) {
     VbState *st = a.st;
     if (st->done) return;  // uniform
     const int32_t s = st->step, cur = s % 3, nxt = (s + 1) % 3, prv = (s + 2) % 3;
     const int32_t cw = st->wi & 1;
-    const int32_t t0 = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x), nt = (int32_t)(gridDim.x * blockDim.x);
-    const int32_t nprev = st->qn[prv] < a.qcap ? st->qn[prv] : a.qcap;
-    for (int32_t i = t0; i < nprev; i += nt) a.f[prv][a.q[prv][i].x] = 0;
-    if (st->wclr) {
-        const int32_t ow = cw ^ 1, nw = st->wn[ow] < a.qcap ? st->wn[ow] : a.qcap;
-        for (int32_t i = t0; i < nw; i += nt) a.w[ow][a.qw[ow][i].x] = 0;
+    const bool wclr = st->wclr != 0;
+    for (int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); i < a.V; i += (int32_t)(gridDim.x * blockDim.x)) {
+        a.f[prv][i] = 0;
+        if (wclr) a.w[cw ^ 1][i] = 0;
     }
     const VbOut o{nxt, cw, st->hi};
+    __shared__ int32_t cand[kVbWaves][kVbGroup][64];  // a pull's minima, one table per wave
+    int32_t(*mine)[64] = cand[threadIdx.x >> 6];
+    VbAcc acc;
     if (st->mode == VB_HEAVY) {
-        if (st->we[cw] > a.pull_heavy) vb_pull(a, a.w[cw], true, o);
-        else vb_push(a, vb_wlist(a, cw), a.w[cw], o);
+        if (st->we > a.pull_heavy) vb_pull(a, a.w[cw], true, o, mine, acc);
+        else vb_push(a, a.w[cw], true, 3 + cw, o, acc);
     } else {
-        if (st->qe[cur] > a.pull_light) vb_pull(a, a.f[cur], false, o);
-        else vb_push(a, vb_flist(a, cur), a.f[cur], o);
+        if (st->fe > a.pull_light) vb_pull(a, a.f[cur], false, o, mine, acc);
+        else vb_push(a, a.f[cur], false, cur, o, acc);
     }
+    vb_store_part(a, acc);
 }
 
 // After the relax step: while F' gains members the window's light steps go
 // on; then (with a split) one heavy step; then the window is settled.
-__global__ void vb_decide_kernel(VbArgs a) {
+__global__ void __launch_bounds__(kVbBlock) vb_decide_kernel(VbArgs a) {
     VbState *st = a.st;
-    if (threadIdx.x != 0 || st->done) return;
-    const int32_t s = st->step, cur = s % 3, nxt = (s + 1) % 3, prv = (s + 2) % 3;
-    const int32_t cw = st->wi & 1;
+    if (st->done) return;  // uniform
+    const VbPart t = vb_sum_parts(a);
+    if (threadIdx.x != 0) return;
+    const int32_t s = st->step, prv = (s + 2) % 3;
     st->steps += 1;
-    st->pulls += st->mode == VB_HEAVY ? st->we[cw] > a.pull_heavy : st->qe[cur] > a.pull_light;
-    if (st->wclr) {  // the relax step cleared the other W set
-        st->wn[cw ^ 1] = 0;
-        st->we[cw ^ 1] = 0;
-        st->wclr = 0;
-    }
+    st->pulls += st->mode == VB_HEAVY ? st->we > a.pull_heavy : st->fe > a.pull_light;
+    st->wn += t.wn;
+    st->we += t.we;
+    st->wclr = 0;  // the relax step cleared the other W set
     st->adv = 0;
-    if (st->qn[nxt] > 0) {
+    if (t.fn > 0) {
         st->mode = VB_LIGHT;
+        st->fe = t.fe;
     } else if (st->mode == VB_LIGHT && a.lend) {
         st->mode = VB_HEAVY;
     } else {
@@ -463,9 +559,11 @@ __global__ void vb_decide_kernel(VbArgs a) {
         st->windows += 1;
         st->wi += 1;
         st->wclr = a.lend != nullptr;
+        st->wn = 0;
+        st->we = 0;
+        st->seg[3 + (st->wi & 1)].n = 0;
     }
-    st->qn[prv] = 0;
-    st->qe[prv] = 0;
+    st->seg[prv].n = 0;  // the set the next relax step builds
     st->step = s + 1;
 }
 
@@ -525,8 +623,8 @@ __global__ void __launch_bounds__(kVbBlock) vb_move_kernel(VbArgs a) {
         st->hi = hi;
     }
     const int32_t cur = st->step % 3, cw = st->wi & 1, lane = lane_id();
-    const VbList FL = vb_flist(a, cur), WL = vb_wlist(a, cw);
     const int32_t groups = (a.V + 63) / 64;
+    VbAcc acc;
     for (int32_t gi = vb_wave(); gi < groups; gi += vb_waves()) {
         const int32_t v0 = gi * 64;
         const uint64_t pj = v0 + lane < a.V ? a.p[v0 + lane] : 0ull;
@@ -555,9 +653,21 @@ __global__ void __launch_bounds__(kVbBlock) vb_move_kernel(VbArgs a) {
                 }
             }
         }
-        vb_enqueue_wave(a, FL, false, joined, v0 + lane);
-        if (a.lend) vb_enqueue_wave(a, WL, true, joined, v0 + lane);
+        vb_note_wave(a, cur, false, joined, v0 + lane, acc);
+        if (a.lend) vb_note_wave(a, cw, true, joined, v0 + lane, acc);
     }
+    vb_store_part(a, acc);
+}
+
+// After an advance: the counts of the F and W the move pass filled.
+__global__ void __launch_bounds__(kVbBlock) vb_count_kernel(VbArgs a) {
+    VbState *st = a.st;
+    if (st->done || !st->adv) return;  // uniform
+    const VbPart t = vb_sum_parts(a);
+    if (threadIdx.x != 0) return;
+    st->fe = t.fe;
+    st->wn += t.wn;
+    st->we += t.we;
 }
 
 __global__ void vb_reset_kernel(VbArgs a) {
@@ -591,21 +701,24 @@ __global__ void vb_seed_kernel(VbArgs a, const int32_t *src, int32_t B, int32_t 
         st->steps = st->pulls = st->windows = 0;
         st->wi = 0;
         st->wclr = 0;
-        for (int k = 0; k < 3; ++k) {
-            st->qn[k] = 0;
-            st->qe[k] = 0;
-        }
-        for (int k = 0; k < 2; ++k) {
-            st->wn[k] = 0;
-            st->we[k] = 0;
-        }
+        st->fe = 0;
+        st->wn = 0;
+        st->we = 0;
+        for (int k = 0; k < 5; ++k) st->seg[k].n = 0;
     }
     __syncthreads();
+    VbAcc acc;
     if (b < B) {
         const int32_t s = src[b];
         vb_row(a, s)[b] = 0;
-        if (!atomicOr((unsigned long long *)&a.f[0][s], 1ull << b)) vb_enqueue(a, vb_flist(a, 0), false, s);
-        if (a.lend && !atomicOr((unsigned long long *)&a.w[0][s], 1ull << b)) vb_enqueue(a, vb_wlist(a, 0), true, s);
+        if (!atomicOr((unsigned long long *)&a.f[0][s], 1ull << b)) vb_note(a, 0, false, s, acc);
+        if (a.lend && !atomicOr((unsigned long long *)&a.w[0][s], 1ull << b)) vb_note(a, 0, true, s, acc);
+    }
+    __syncthreads();
+    if (b < B) {  // a few sources: atomics on the totals are fine here
+        atomicAdd((unsigned long long *)&st->fe, (unsigned long long)acc.fe);
+        atomicAdd((unsigned long long *)&st->wn, (unsigned long long)acc.wn);
+        atomicAdd((unsigned long long *)&st->we, (unsigned long long)acc.we);
     }
 }
 
@@ -648,11 +761,11 @@ __global__ void vb_light_count_kernel(const int32_t *off, const int32_t *lend, i
 }
 
 static void vb_launch_step(const VbArgs &a, Launcher &L) {
-    const int grid = persistent_grid(kVbBlock, 8);
-    L(vb_relax_kernel, grid, kVbBlock, a);
-    L(vb_decide_kernel, 1, 64, a);
-    L(vb_pmin_kernel, grid, kVbBlock, a);
-    L(vb_move_kernel, grid, kVbBlock, a);
+    L(vb_relax_kernel, a.grid, kVbBlock, a);
+    L(vb_decide_kernel, 1, kVbBlock, a);
+    L(vb_pmin_kernel, a.grid, kVbBlock, a);
+    L(vb_move_kernel, a.grid, kVbBlock, a);
+    L(vb_count_kernel, 1, kVbBlock, a);
 }
 
 struct VbRun {
@@ -660,8 +773,8 @@ struct VbRun {
     uint64_t *f[3] = {nullptr, nullptr, nullptr};
     uint64_t *w[2] = {nullptr, nullptr};
     uint64_t *p = nullptr;
-    int4 *q[3] = {nullptr, nullptr, nullptr};
-    int4 *qw[2] = {nullptr, nullptr};
+    int4 *seg[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    VbPart *part = nullptr;
     int4 *wide = nullptr;
     unsigned long long *cnt = nullptr;  // wide chunks, light edges
     VbState *st = nullptr;
@@ -673,9 +786,9 @@ struct VbRun {
         dfree(D);
         for (auto x : f) dfree(x);
         for (auto x : w) dfree(x);
-        for (auto x : q) dfree(x);
-        for (auto x : qw) dfree(x);
+        for (auto x : seg) dfree(x);
         dfree(p);
+        dfree(part);
         dfree(wide);
         dfree(cnt);
         dfree(st);
@@ -697,19 +810,16 @@ static int sssp_batched(gi_graph *g, const int32_t *h_src, int32_t n, int32_t de
     GI_TRY(sssp_light_split(g, delta, &sv, s));
     VbRun r;
     const int64_t V = g->V;
-    // a set lists each vertex once: max(1, ceil(part / kVbSeg)) items
-    const int64_t qcap = 2 * V + g->E / kVbSeg + 1;
-    if (qcap > INT32_MAX) return fail(GI_ERR_RANGE, "batched SSSP: graph too large");
+    // a set lists at most E / kVbSeg + (lists longer than kVbSeg) <= 2 E / kVbSeg segments
+    const int64_t scap64 = 2 * g->E / kVbSeg + 1;
+    if (scap64 > INT32_MAX) return fail(GI_ERR_RANGE, "batched SSSP: graph too large");
+    const int32_t grid = persistent_grid(kVbBlock, 8);
     GI_TRY(dalloc(&r.D, (size_t)V * kVbLanes + 1));
-    for (int k = 0; k < 3; ++k) {
-        GI_TRY(dalloc(&r.f[k], (size_t)V + 1));
-        GI_TRY(dalloc(&r.q[k], (size_t)qcap));
-    }
-    for (int k = 0; k < 2; ++k) {
-        GI_TRY(dalloc(&r.w[k], (size_t)V + 1));
-        GI_TRY(dalloc(&r.qw[k], (size_t)qcap));
-    }
+    for (int k = 0; k < 3; ++k) GI_TRY(dalloc(&r.f[k], (size_t)V + 1));
+    for (int k = 0; k < 2; ++k) GI_TRY(dalloc(&r.w[k], (size_t)V + 1));
+    for (int k = 0; k < 5; ++k) GI_TRY(dalloc(&r.seg[k], (size_t)scap64));
     GI_TRY(dalloc(&r.p, (size_t)V + 1));
+    GI_TRY(dalloc(&r.part, (size_t)grid));
     GI_TRY(dalloc(&r.st, 1));
     GI_TRY(dalloc(&r.d_src, (size_t)kVbLanes));
     GI_TRY(dalloc(&r.cnt, 2));
@@ -744,20 +854,17 @@ static int sssp_batched(gi_graph *g, const int32_t *h_src, int32_t n, int32_t de
         a.plend = sv.lend;
     }
     a.V = g->V;
-    a.qcap = (int32_t)qcap;
+    a.scap = (int32_t)scap64;
     const bool whole = a.plend == nullptr;  // a pull reads every in-edge
     a.pull_light = (whole ? g->E : light) / kVbPullDiv;
     a.pull_heavy = (whole ? g->E : g->E - light) / kVbPullDiv;
     a.D = r.D;
-    for (int k = 0; k < 3; ++k) {
-        a.f[k] = r.f[k];
-        a.q[k] = r.q[k];
-    }
-    for (int k = 0; k < 2; ++k) {
-        a.w[k] = r.w[k];
-        a.qw[k] = r.qw[k];
-    }
+    for (int k = 0; k < 3; ++k) a.f[k] = r.f[k];
+    for (int k = 0; k < 2; ++k) a.w[k] = r.w[k];
+    for (int k = 0; k < 5; ++k) a.seg[k] = r.seg[k];
     a.p = r.p;
+    a.part = r.part;
+    a.grid = grid;
     a.wide = r.wide;
     a.nwide = (int32_t)(nwide < wcap ? nwide : wcap);
     GI_TRY(build_graph(

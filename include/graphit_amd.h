@@ -360,10 +360,13 @@ int gi_bfs_batched(gi_graph *g, const int32_t *sources, int32_t num_sources, int
 /* Delta-stepping SSSP from many sources at once: d_dist is
  * [num_sources][V] (source-major; INT32_MAX where unreached). The SSSP app
  * (apps/sssp.cpp:34-57, gpu_priority_queue.h:115-185) runs one source per
- * call; here batches of up to `batch` sources (<= 256; 0 = 256) share one
- * set of launches per step, each source with its own window of width
- * delta and pending set. Distances equal gi_sssp's. stats: iterations = relax
- * steps summed over the sources, ms = device time. Synchronises. */
+ * call; here passes of up to `batch` sources (<= 64; 0 = 64) run together,
+ * one wave lane per source over vertex-major distance rows, sharing one
+ * window of width delta: light steps, one heavy step per window, push or
+ * pull per step. A pass shares its window among its sources, so it favours
+ * wider windows than one source does; the distances do not depend on delta.
+ * Distances equal gi_sssp's. stats: iterations = relax steps summed over the
+ * passes (pull_iterations of them pulled), ms = device time. Synchronises. */
 int gi_sssp_batched(gi_graph *g, const int32_t *sources, int32_t num_sources, int32_t delta, int32_t batch,
                     int32_t *d_dist, gi_run_stats *stats, void *stream);
 /* BFS that allocates nothing. No counterpart in the reference: it is for

@@ -93,11 +93,14 @@ def test_sssp_batched_rmat(gi, batch):
     host = host_csr(g, True)
     srcs = O.pick_sources(host, 5, 70)
     srcs[9] = srcs[2]
-    for delta in (16, 200):
+    pulls = 0
+    for delta in (16, 200, 4096):  # 4096: no edge is heavy (no light/heavy split)
         dist, st = gi.sssp_batched(g, srcs, delta, batch)
         for i, s in enumerate(srcs):
             assert np.array_equal(dist[i], O.sssp(host, int(s))), (batch, delta, i)
         assert st["iterations"] >= 1 and st["iterations"] == st["push_iterations"] + st["pull_iterations"]
+        pulls += st["pull_iterations"]
+    assert pulls > 0  # the pull steps ran too
 
 
 @pytest.mark.parametrize("graph", ["grid", "directed", "forest"])
