@@ -592,6 +592,7 @@ static int bc_batch_run(gi_graph *g, BcRun &r, const int32_t *h_src, int32_t B, 
 // added into d_scores. *levels: the levels summed over the batches.
 int bc_batched(gi_graph *g, const int32_t *h_src, int32_t num_sources, int32_t batch, float *d_scores,
                hipStream_t s, int64_t *levels) {
+    if (batch > 1) return bc_vertex_major(g, h_src, num_sources, batch, d_scores, s, levels);
     if (batch < 1) batch = 1;
     if (batch > kBcMaxBatch) batch = kBcMaxBatch;
     if (batch > num_sources) batch = num_sources;

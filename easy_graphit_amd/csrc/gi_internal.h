@@ -387,6 +387,10 @@ int ensure_max_degree(gi_graph *g, hipStream_t s);
 int ensure_scratch(gi_graph *g, int64_t len);
 int bc_batched(gi_graph *g, const int32_t *h_src, int32_t num_sources, int32_t batch, float *d_scores,
                hipStream_t s, int64_t *levels);
+// Batches of more than one source (gi_bc_vmajor.hip): passes of up to 64
+// sources, one lane each over vertex-major rows.
+int bc_vertex_major(gi_graph *g, const int32_t *h_src, int32_t num_sources, int32_t batch, float *d_scores,
+                    hipStream_t s, int64_t *levels);
 // Scratch layout per graph: load-balance region first (TWC bins 3V+3, or the
 // STRICT prefix V + chunk sums), then the pull enumerator (count, queue).
 inline int64_t scratch_lb_len(int64_t V) {
