@@ -336,6 +336,25 @@ def bench_apps(gi, g, sources, edges_of, log, only: str | None = None, cpu_on: b
                      "config": f"RMAT-22 directed, 20 iterations, d 0.85, {'degree' if gp.layout else 'native'} layout"}
         log(f"pr: {out['pr']}")
         gp.destroy()
+    if only == "bc_batched":
+        # the batched BC line alone (the profiled command: the rest of the BC
+        # section runs the fused single-source mode as well)
+        gb = gi.Graph.rmat(20, 16, SEED, True)
+        bsrc = gb.pick_sources(SEED, 256)
+        gi.bc(gb, bsrc[:8], 256)
+        _, st_b = gi.bc(gb, bsrc, 256)
+        dtmp = gi.DeviceArray(gb.num_vertices, np.int32)
+        be, bv = 0, 0
+        for s in bsrc:
+            gi.bfs_device(gb, int(s), None, dtmp)
+            be += gb.reached_edges(dtmp)
+            bv += int((dtmp.numpy() >= 0).sum())
+        tb = st_b["ms"] * 1e-3
+        out["bc_batched"] = {"sources": len(bsrc), "batched_ms": round(tb * 1e3, 2),
+                             "gteps": round(be / 2 / tb / 1e9, 3),
+                             "roofline_frac": alg_frac(20 * bv + 20 * be + 16 * gb.num_vertices * len(bsrc), tb),
+                             "config": "RMAT-20 sym, 256 sources, batch 256"}
+        log(f"bc_batched: {out['bc_batched']}")
     if only in (None, "bc"):
         # Batched Brandes BC: RMAT-20, 256 sources, batched vs one source at a time
         gb = gi.Graph.rmat(20, 16, SEED, True)
@@ -534,7 +553,8 @@ def main():
     ap.add_argument("--no-selector", action="store_true",
                     help="skip the fixed-rule comparison (profiles of the timed BFS runs alone)")
     ap.add_argument("--sources", type=int, default=64, help="distinct BFS sources cycled over")
-    ap.add_argument("--app", choices=["sssp", "sssp_grid", "sssp_trace", "cc", "pr", "bc", "many"], default=None,
+    ap.add_argument("--app", choices=["sssp", "sssp_grid", "sssp_trace", "cc", "pr", "bc", "bc_batched", "many"],
+                    default=None,
                     help="only this app's section (for profiling; prints its results, no bench line)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run three device-to-device copies of MIB MiB (HBM counter scale)")

@@ -516,10 +516,11 @@ int gi_cc(gi_graph *g, const gi_schedule *s, int32_t *d_labels, gi_run_stats *st
 int gi_pagerank(gi_graph *g, int32_t iters, float damping, const gi_schedule *s,
                 float *d_rank, gi_run_stats *stats, void *stream);
 /* Brandes BC from num_sources sources (apps/bc.cpp:64-125), summed into
- * d_scores (float, zeroed first). batch > 1: up to 256 sources advance
- * level-synchronously, the level loops on the device (recorded); batch <= 1:
- * one source at a time as the app runs unfused (push TWCE, a host round trip
- * per level, apps/bc.cpp:149-151). */
+ * d_scores (float, zeroed first). batch > 1: passes of up to min(batch, 64)
+ * sources advance level-synchronously, one lane per source over vertex-major
+ * rows, the level loops on the device (recorded); batch <= 1: one source at
+ * a time as the app runs unfused (push TWCE, a host round trip per level,
+ * apps/bc.cpp:149-151). */
 int gi_bc(gi_graph *g, const int32_t *h_sources, int32_t num_sources, int32_t batch,
           float *d_scores, gi_run_stats *stats, void *stream);
 /* BC one source at a time under a schedule (apps/bc.cpp:64-120 with its s1
