@@ -336,14 +336,16 @@ def bench_apps(gi, g, sources, edges_of, log, only: str | None = None, cpu_on: b
                      "config": f"RMAT-22 directed, 20 iterations, d 0.85, {'degree' if gp.layout else 'native'} layout"}
         log(f"pr: {out['pr']}")
         gp.destroy()
-    if only == "bc_batched":
-        # the batched BC line alone (the profiled command: the rest of the BC
-        # section runs the fused single-source mode as well)
+    if only == "batched":
+        # the batched engines alone (the profiled command: the BC section also
+        # runs the single-source engines it compares against): BC from 256
+        # sources, per-source BFS from 256 and SSSP from 64 of them
         gb = gi.Graph.rmat(20, 16, SEED, True)
         bsrc = gb.pick_sources(SEED, 256)
+        Vb = gb.num_vertices
         gi.bc(gb, bsrc[:8], 256)
         _, st_b = gi.bc(gb, bsrc, 256)
-        dtmp = gi.DeviceArray(gb.num_vertices, np.int32)
+        dtmp = gi.DeviceArray(Vb, np.int32)
         be, bv = 0, 0
         for s in bsrc:
             gi.bfs_device(gb, int(s), None, dtmp)
@@ -352,9 +354,22 @@ def bench_apps(gi, g, sources, edges_of, log, only: str | None = None, cpu_on: b
         tb = st_b["ms"] * 1e-3
         out["bc_batched"] = {"sources": len(bsrc), "batched_ms": round(tb * 1e3, 2),
                              "gteps": round(be / 2 / tb / 1e9, 3),
-                             "roofline_frac": alg_frac(20 * bv + 20 * be + 16 * gb.num_vertices * len(bsrc), tb),
+                             "roofline_frac": alg_frac(20 * bv + 20 * be + 16 * Vb * len(bsrc), tb),
                              "config": "RMAT-20 sym, 256 sources, batch 256"}
         log(f"bc_batched: {out['bc_batched']}")
+        dall = gi.DeviceArray(len(bsrc) * Vb, np.int32)
+        gi.bfs_batched(gb, bsrc[:64], dall)
+        _, st_f = gi.bfs_batched(gb, bsrc, dall)
+        out["bfs_batched"] = {"sources": len(bsrc), "ms": round(st_f["ms"], 2), "config": "RMAT-20 sym, 256 sources"}
+        log(f"bfs_batched: {out['bfs_batched']}")
+        gb.set_weights(SEED, 1, 255)
+        ss = [int(x) for x in bsrc[:64]]
+        gi.sssp_batched(gb, ss[:8], 1024, 0, dall)
+        _, st_s = gi.sssp_batched(gb, ss, 1024, 0, dall)
+        out["sssp_batched"] = {"sources": len(ss), "ms": round(st_s["ms"], 2), "steps": st_s["iterations"],
+                               "pull_steps": st_s["pull_iterations"],
+                               "config": "RMAT-20 sym, w U[1,255], 64 sources, delta 1024"}
+        log(f"sssp_batched: {out['sssp_batched']}")
     if only in (None, "bc"):
         # Batched Brandes BC: RMAT-20, 256 sources, batched vs one source at a time
         gb = gi.Graph.rmat(20, 16, SEED, True)
@@ -553,7 +568,7 @@ def main():
     ap.add_argument("--no-selector", action="store_true",
                     help="skip the fixed-rule comparison (profiles of the timed BFS runs alone)")
     ap.add_argument("--sources", type=int, default=64, help="distinct BFS sources cycled over")
-    ap.add_argument("--app", choices=["sssp", "sssp_grid", "sssp_trace", "cc", "pr", "bc", "bc_batched", "many"],
+    ap.add_argument("--app", choices=["sssp", "sssp_grid", "sssp_trace", "cc", "pr", "bc", "batched", "many"],
                     default=None,
                     help="only this app's section (for profiling; prints its results, no bench line)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
