@@ -10,6 +10,9 @@
 // Level l (F = fr[l % 3], F' = fr[(l + 1) % 3]):
 //   push  every u with a nonzero mask ORs mask(u) & ~seen(v) into F'(v) of
 //         its out-neighbours (atomicOr, skipped when the bits are present);
+//         out-lists of kMsChunk edges or more are chunk items that any wave
+//         takes (a hub's list walked by one wave had made RMAT-20's second
+//         level take 1.1 ms of a 64-source pass's 2.1);
 //   pull  every v missing some of the group's sources ORs F(u) over its
 //         in-neighbours u, restricted to the missing bits, and stops once
 //         every missing bit is found (a plain store: a vertex has one owner);
@@ -32,8 +35,12 @@ constexpr int kMsBlock = 256;
 constexpr int kMsDecBlock = 1024;
 constexpr int32_t kMsWide = 64;       // degree from which a whole wave walks the list
 constexpr int32_t kMsMaxGrid = 8192;  // partial-sum slots (one per workgroup)
-constexpr long long kMsPullDiv = 16;  // pull when F' has >= E / kMsPullDiv out-edges
+// pull when F' has >= E / kMsPullDiv out-edges: a vertex's pull stops only
+// once every source it misses is found, so an early level pulls nearly every
+// in-edge (RMAT-20, 256 sources: E/16 8.1 ms, E/4 4.9, E/2 5.1, never 8.5)
+constexpr long long kMsPullDiv = 4;
 constexpr int32_t kMsBlockLevels = 8;
+constexpr int32_t kMsChunk = 1024;    // out-lists this long are pushed as chunk items of this many edges
 
 struct MsState {
     int32_t done;
@@ -56,6 +63,8 @@ struct MsArgs {
     uint64_t *seen;
     uint64_t *fr[3];
     long long *slots;  // [2][kMsMaxGrid]: front, fdeg per workgroup
+    const int4 *wide;  // chunk items of the long out-lists (vertex, first edge, end edge)
+    int32_t nwide;
 };
 
 __device__ __forceinline__ uint64_t *ms_pick3(uint64_t *a, uint64_t *b, uint64_t *c, int32_t i) {
@@ -126,6 +135,7 @@ __global__ void __launch_bounds__(kMsBlock) ms_step_kernel(MsArgs a) {
             if (fu) {
                 e0 = a.off[u];
                 e1 = a.off[u + 1];
+                if (e1 - e0 >= kMsChunk) e1 = e0;  // a chunk item below
             }
             const bool wide = e1 - e0 >= kMsWide;
             if (!wide)
@@ -138,6 +148,26 @@ __global__ void __launch_bounds__(kMsBlock) ms_step_kernel(MsArgs a) {
                 const uint64_t wfu = __shfl(fu, j);
                 for (int32_t e = we0 + lane; e < we1; e += 64) ms_push_edge(a, nxt, wfu, a.nbr[e]);
             }
+        }
+        for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < a.nwide; i += nwaves) {
+            const int4 c = a.wide[i];
+            const uint64_t fu = cur[c.x];
+            if (!fu) continue;  // uniform
+            for (int32_t e = c.y + lane; e < c.z; e += 64) ms_push_edge(a, nxt, fu, a.nbr[e]);
+        }
+    }
+}
+
+// Chunk items of the out-lists of kMsChunk edges or more.
+__global__ void ms_wide_kernel(const int32_t *off, int32_t V, int4 *out, int32_t cap, int32_t *n) {
+    for (int32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        const int32_t e0 = off[v], e1 = off[v + 1];
+        if (e1 - e0 < kMsChunk) continue;
+        const int32_t k = (e1 - e0 + kMsChunk - 1) / kMsChunk;
+        const int32_t p = atomicAdd(n, k);
+        for (int32_t j = 0; j < k && p + j < cap; ++j) {
+            const int32_t s0 = e0 + j * kMsChunk;
+            out[p + j] = make_int4(v, s0, s0 + kMsChunk < e1 ? s0 + kMsChunk : e1, 0);
         }
     }
 }
@@ -304,9 +334,13 @@ struct MsRun {
     MsState *st = nullptr;
     MsState *host = nullptr;  // pinned
     int32_t *d_src = nullptr;
+    int4 *wide = nullptr;
+    int32_t *nwide = nullptr;
     hipGraphExec_t exec = nullptr;
     ~MsRun() {
         if (exec) (void)hipGraphExecDestroy(exec);
+        dfree(wide);
+        dfree(nwide);
         dfree(seen);
         for (auto p : fr) dfree(p);
         dfree(slots);
@@ -345,6 +379,19 @@ static int bfs_batched(gi_graph *g, const int32_t *h_src, int32_t n, int32_t *d_
     a.slots = r.slots;
     ms_zero_slots_kernel<<<16, 256, 0, s>>>(r.slots, 2 * kMsMaxGrid);
     GI_LAUNCH_CHECK();
+    // a list of d >= kMsChunk edges makes ceil(d / kMsChunk) <= 2 d / kMsChunk items
+    const int32_t wcap = (int32_t)(2 * g->E / kMsChunk + 1);
+    GI_TRY(dalloc(&r.wide, (size_t)wcap));
+    GI_TRY(dalloc(&r.nwide, 1));
+    GI_HIP(hipMemsetAsync(r.nwide, 0, sizeof(int32_t), s));
+    if (V) ms_wide_kernel<<<(int)(ceil_div((int64_t)V, 256) < 4096 ? ceil_div((int64_t)V, 256) : 4096), 256, 0, s>>>(
+        g->d_off, g->V, r.wide, wcap, r.nwide);
+    GI_LAUNCH_CHECK();
+    int32_t nwide = 0;
+    GI_HIP(hipMemcpyAsync(&nwide, r.nwide, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    GI_HIP(hipStreamSynchronize(s));
+    a.wide = r.wide;
+    a.nwide = nwide < wcap ? nwide : wcap;
     GI_TRY(build_graph(
         [&](Launcher &L) {
             for (int32_t i = 0; i < kMsBlockLevels; ++i) ms_launch_level(a, L);
