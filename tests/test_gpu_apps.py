@@ -321,6 +321,17 @@ def test_bc_batched_matches_oracle(gi, batch):
     assert rel_l1(sc, O.bc(host, srcs)) <= 1e-4
 
 
+def test_bc_batched_uneven_passes(gi):
+    # 70 sources in passes of 64 and 6: the second pass records its own slots
+    g = gi.Graph.rmat(12, 8, SEED, True)
+    host = host_csr(g)
+    srcs = O.pick_sources(host, 17, 70)
+    sc, _ = gi.bc(g, srcs, batch=64)
+    assert rel_l1(sc, O.bc(host, srcs)) <= 1e-4
+    sc2, _ = gi.bc(g, srcs[:6], batch=64)  # a fresh call with the short pass alone
+    assert rel_l1(sc2, O.bc(host, srcs[:6])) <= 1e-4
+
+
 @pytest.mark.parametrize("graph", ["grid", "directed"])
 def test_bc_batched_long_and_directed(gi, graph):
     # a grid: many levels, pushed and pulled in both passes; a directed graph:

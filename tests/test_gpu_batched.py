@@ -94,7 +94,7 @@ def test_sssp_batched_rmat(gi, batch):
     srcs = O.pick_sources(host, 5, 70)
     srcs[9] = srcs[2]
     pulls = 0
-    for delta in (16, 200, 4096):  # 4096: no edge is heavy (no light/heavy split)
+    for delta in (1, 16, 200, 4096):  # 1: every edge heavy; 4096: none (no light/heavy split)
         dist, st = gi.sssp_batched(g, srcs, delta, batch)
         for i, s in enumerate(srcs):
             assert np.array_equal(dist[i], O.sssp(host, int(s))), (batch, delta, i)
