@@ -650,7 +650,7 @@ def main():
         f"({'learned selector' if use_learned else 'heuristic'})")
     selector_cmp = None
     if learned is not None and not args.no_selector:
-        # GraphIt's fixed rule |F| <= t*V, judged on both layouts: the
+        # GraphIt's fixed rule |F| <= t*V, timed on both layouts: the
         # strongest rule (the push/pull mode pair and t tuned on the
         # calibration graphs, stored with the model) and the BFS app's own rule
         # (PUSH TWCE / PULL VERTEX_BASED); the headline ratio is against the

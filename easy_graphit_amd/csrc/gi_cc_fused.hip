@@ -181,11 +181,13 @@ __global__ void __launch_bounds__(kCcBlock) cc_sample_kernel(CcArgs a, int32_t r
     }
 }
 
-// The most frequent label among 1024 spread-out vertices (one workgroup of
+// The most frequent label among 256 spread-out vertices (one workgroup of
 // kCcGiantBlock threads, one sample each, counted against all the others
 // from LDS; the label only steers which vertices the filtered rounds skip,
-// any label gives the same result).
-constexpr int kCcGiantBlock = 1024;
+// any label gives the same result). A giant component holding half the
+// vertices gets about 128 of the samples; 1024 samples cost 22 us a run on
+// RMAT-24, almost all of it the 1024 x 1024 comparisons.
+constexpr int kCcGiantBlock = 256;
 __global__ void __launch_bounds__(kCcGiantBlock) cc_giant_kernel(CcArgs a) {
     constexpr int kSamples = kCcGiantBlock;
     __shared__ int32_t lab[kSamples];
